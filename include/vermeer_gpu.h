@@ -1,0 +1,198 @@
+/*
+ * vermeer_gpu.h — C ABI of the MI355X (gfx950) ray-traversal library.
+ *
+ * This is the drop-in boundary for the reference's traversal hot path.
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference checkout).  The ABI uses plain pointers and
+ * sizes only, so it can be bound from cgo, ctypes or C/C++.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative VGPU_ERR_* code on
+ *     failure; vgpu_last_error() then describes the failure (per host thread).
+ *     Nothing aborts across the boundary (the reference panics instead).
+ *   - The caller owns every host buffer.  The library copies what it needs
+ *     and keeps no caller pointer after a call returns (cgo pointer rules).
+ *   - Device memory is owned by the scene object.
+ *   - A scene is not internally synchronised: use one scene per host thread,
+ *     or serialise calls on it.
+ */
+#ifndef VERMEER_GPU_H
+#define VERMEER_GPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VGPU_OK 0
+#define VGPU_ERR_ARG (-1)      /* invalid argument or size */
+#define VGPU_ERR_HIP (-2)      /* HIP runtime error */
+#define VGPU_ERR_BUILD (-3)    /* BVH builder failure (degenerate input the reference recurses on forever) */
+#define VGPU_ERR_STACK (-4)    /* a ray needed more traversal stack than VGPU_MAX_STACK */
+#define VGPU_ERR_STATE (-5)    /* call out of order (e.g. trace before commit) */
+#define VGPU_ERR_NOMEM (-6)    /* host or device allocation failed */
+
+/* Traversal stack bound per ray.  The reference holds 90 entries
+ * (core/ray.go:158) shared by the scene and mesh levels. */
+#define VGPU_MAX_STACK 128
+
+/* Ray record, 32 bytes.  P/D as core.Ray.P/D (core/ray.go:28), tmax is the
+ * initial core.Ray.Tclosest (Ray.Init maxdist, core/ray.go:59), time is
+ * core.Ray.Time in [0,1) for motion blur. */
+typedef struct vgpu_ray {
+    float P[3];
+    float tmax;
+    float D[3];
+    float time;
+} vgpu_ray;
+
+/* Hit record, 16 bytes.  prim = ShaderContext.ElemID, the triangle index
+ * after the BVH leaf reorder (builtin/geom/polymesh/trace.go:513), or -1 on
+ * a miss; t = Ray.Tclosest after the hit; u,v = ShaderContext.Bu/Bv, the
+ * barycentric weights of the triangle's first and second vertex
+ * (trace.go:343-344).  On a miss t holds the ray's tmax. */
+typedef struct vgpu_hit {
+    int32_t prim;
+    float t;
+    float u;
+    float v;
+} vgpu_hit;
+
+/* qbvh.Node, 128 bytes, the reference layout (qbvh/qbvh.go:31-36):
+ * boxes[child + i*12 + axis*4] (i=0 min, 1 max); children >= 0 are nodes,
+ * -1 is an empty leaf, otherwise 1<<31 | first<<4 | (count-1). */
+typedef struct vgpu_qbvh_node {
+    float boxes[24];
+    uint32_t axis0, axis1, axis2;
+    int32_t children[4];
+    int32_t parent;
+} vgpu_qbvh_node;
+
+/* qbvh.MotionNode topology (qbvh/mqbvh.go:24-29) reduced to what traversal
+ * reads: axis0..2, children[4], one pad word. */
+typedef struct vgpu_mqbvh_node {
+    int32_t axis0, axis1, axis2;
+    int32_t children[4];
+    int32_t pad;
+} vgpu_mqbvh_node;
+
+/* vgpu_trace flags */
+#define VGPU_TRACE_SHADOW 1u        /* RayTypeShadow: any-hit early out (qbvh/intersect.go:232-236) */
+#define VGPU_TRACE_DEVICE_PTRS 2u   /* rays/hits/geoms/counters are device pointers; the call only
+                                       enqueues work on `stream` and returns without synchronising */
+
+/* vgpu_scene_add_mesh flags */
+#define VGPU_MESH_MOTION_REFERENCE_INDEXING 1u
+    /* motion meshes only: test the triangle idxp[leaf slot] as the reference does
+       (builtin/geom/polymesh/trace.go:532-537, idxp is never reordered for motion
+       meshes, buildqbvh.go:19-57).  Default: the face the BVH leaf bounds, accel.idx[slot]. */
+
+typedef struct vgpu_scene vgpu_scene;
+
+/* Description of the last failure on the calling host thread (never NULL). */
+const char* vgpu_last_error(void);
+
+/* Number of visible GPUs. */
+int vgpu_device_count(int* count);
+
+/* Host builder.  Replaces qbvh.BuildAccel (qbvh/build.go:293-307) with an
+ * output identical node for node: boxes (n*6: min xyz, max xyz), centroids
+ * (n*3) and idx (n) are permuted in place exactly as the reference permutes
+ * them; nodes_out receives up to max_nodes nodes; *nnodes_out the count;
+ * bounds_out (6 floats, nullable) the root box.  nthreads <= 0 uses all
+ * hardware threads; the output does not depend on it. */
+int vgpu_qbvh_build(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
+                    vgpu_qbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out,
+                    float* bounds_out, int32_t nthreads);
+
+/* Replaces qbvh.BuildAccelMotion (qbvh/motionbuild.go:108-122): topology only. */
+int vgpu_qbvh_build_motion(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
+                           vgpu_mqbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out,
+                           int32_t nthreads);
+
+/* Scene lifetime.  Replaces core.Init(scene.New()) (core/core.go:26,
+ * builtin/scene/scene.go:24) with a scene whose geometry lives on `device`. */
+int vgpu_scene_create(vgpu_scene** out, int device);
+int vgpu_scene_destroy(vgpu_scene* scene);
+
+/* Adds a triangle mesh (a PolyMesh with FaceIdx and no PolyCount,
+ * builtin/geom/polymesh/init.go:71-94).  verts: nkeys*nverts*3 floats,
+ * key-major (param.PointArray, core/param/array.go:19-28); idx: ntris*3
+ * vertex indices.  The per-mesh accel is built at commit exactly as
+ * PolyMesh.initAccel (buildqbvh.go:14-144).  *geom_out receives the
+ * insertion index (the post-commit slot is reported by vgpu_scene_info). */
+int vgpu_scene_add_mesh(vgpu_scene* scene, const float* verts, uint32_t nverts, uint32_t nkeys,
+                        const uint32_t* idx, uint32_t ntris, float ray_bias, uint32_t flags,
+                        int32_t* geom_out);
+
+/* Adds a static mesh whose accel the caller already built with the
+ * reference's own builder (the cgo path: Go's PolyMesh.initAccel output).
+ * idxp_sorted: ntris*3 post-sort indices; nodes: mesh.accel.qbvh; bounds:
+ * mesh.bounds (6 floats). */
+int vgpu_scene_add_mesh_prebuilt(vgpu_scene* scene, const float* verts, uint32_t nverts,
+                                 const uint32_t* idxp_sorted, uint32_t ntris,
+                                 const vgpu_qbvh_node* nodes, uint32_t nnodes,
+                                 const float* bounds, float ray_bias, int32_t* geom_out);
+
+/* Builds the per-mesh accels not yet built, the scene top level
+ * (Scene.initAccel, scene.go:135-203, leafMax 1) and uploads everything to
+ * the device.  Replaces Scene.PreRender (scene.go:131-133). */
+int vgpu_scene_commit(vgpu_scene* scene);
+
+/* After commit: geom_order[slot] = insertion index of the geom in post-sort
+ * slot `slot` (the permutation of scene.go:164-171); geom_nodes / geom_tris
+ * per slot (nullable arrays of ngeoms); *top_nodes the top-level node count. */
+int vgpu_scene_info(const vgpu_scene* scene, int32_t* ngeoms, int32_t* top_nodes,
+                    int32_t* geom_order, int32_t* geom_nodes, int32_t* geom_tris);
+
+/* After commit: copies a static geom's host-side accel (for checking the
+ * builder): nodes (geom_nodes[slot] nodes), post-sort idxp (ntris*3) and
+ * accel.idx (ntris).  Any output may be NULL. */
+int vgpu_scene_mesh_accel(const vgpu_scene* scene, int32_t slot, vgpu_qbvh_node* nodes,
+                          uint32_t* idxp_sorted, int32_t* perm);
+
+/* Traces n rays against the committed scene.  Replaces core.TraceProbe ->
+ * Scene.Trace -> qbvh.Trace/TraceMotion -> PolyMesh.Trace ->
+ * qbvh.Trace/TraceMotion -> PolyMesh.TraceElems/TraceMotionElems
+ * (core/trace.go:26-35, builtin/scene/scene.go:30-97,
+ * qbvh/intersect.go:91-246, qbvh/motionintersect.go:22-127,
+ * builtin/geom/polymesh/trace.go:15-194, :520-669) for a batch of rays.
+ * hits: n records.  geoms (nullable): n post-sort geom slots
+ * (ShaderContext.Geom, scene.go:67) or -1.  counters (nullable): 4 int32
+ * per ray — node visits, leaf visits, triangle tests, maximum stack depth.
+ * Without VGPU_TRACE_DEVICE_PTRS all arrays are host memory and the call is
+ * synchronous; with it they are device memory and the work is enqueued on
+ * `stream` (a hipStream_t, NULL = the null stream). */
+int vgpu_trace(vgpu_scene* scene, uint32_t flags, const vgpu_ray* rays, uint64_t n,
+               vgpu_hit* hits, int32_t* geoms, int32_t* counters, void* stream);
+
+/* Synchronises `stream` and reports (then clears) an error latched by
+ * traversal work enqueued with VGPU_TRACE_DEVICE_PTRS: VGPU_ERR_STACK if a
+ * ray overflowed VGPU_MAX_STACK (its hit was written as a miss).  No
+ * reference counterpart: the reference panics on stack overflow. */
+int vgpu_scene_status(vgpu_scene* scene, void* stream);
+
+/* Traversal launch configuration (the autotuner's knobs). */
+typedef struct vgpu_trace_config {
+    int32_t block;        /* threads per workgroup: 64, 128 or 256 */
+    int32_t lds_stack;    /* stack entries per ray held in LDS: 8, 16, 24 or 32; the rest spill to HBM */
+    int32_t waves_per_cu; /* resident-grid sizing: workgroups = CUs * waves_per_cu * 64 / block */
+    int32_t sort;         /* 1: reorder rays by a Morton key of origin+direction before traversal */
+} vgpu_trace_config;
+
+int vgpu_scene_set_trace_config(vgpu_scene* scene, const vgpu_trace_config* cfg);
+int vgpu_scene_get_trace_config(const vgpu_scene* scene, vgpu_trace_config* cfg);
+
+/* Runs the autotuner: times each candidate configuration on the given rays
+ * (device pointers, n rays) and keeps the fastest.  best_out (nullable)
+ * receives the chosen configuration; ms_out (nullable) its time. */
+int vgpu_scene_autotune(vgpu_scene* scene, uint32_t flags, const vgpu_ray* d_rays, uint64_t n,
+                        vgpu_hit* d_hits, void* stream, vgpu_trace_config* best_out, float* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VERMEER_GPU_H */

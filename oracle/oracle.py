@@ -1,0 +1,171 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of liboracle.so, the CPU restatement of the reference's
+traversal path (oracle_accel.cpp).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module, and only as the checker
+or the timed CPU baseline — never as part of the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+NODE_DTYPE = np.dtype([("boxes", "<f4", 24), ("axis", "<u4", 3), ("children", "<i4", 4), ("parent", "<i4")])
+RAY_DTYPE = np.dtype([("P", "<f4", 3), ("tmax", "<f4"), ("D", "<f4", 3), ("time", "<f4")])
+OHIT_DTYPE = np.dtype([("geom", "<i4"), ("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+
+_lib = None
+
+
+def build():
+    """Compiles liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    P = C.c_void_p
+    sig = {
+        "orc_build_qbvh": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, C.c_int32, P]),
+        "orc_intersect_boxes": (None, [P, P, P, P, P]),
+        "orc_ray_setup": (None, [P, P, P, P]),
+        "orc_scene_new": (P, []),
+        "orc_scene_free": (None, [P]),
+        "orc_scene_add_mesh": (C.c_int, [P, P, C.c_int32, C.c_int32, P, C.c_int32, C.c_float, C.c_int32]),
+        "orc_scene_commit": (C.c_int, [P]),
+        "orc_scene_info": (C.c_int, [P, P, P, P, P, P]),
+        "orc_scene_mesh_accel": (C.c_int, [P, C.c_int32, P, P, P]),
+        "orc_scene_mesh_motion_accel": (C.c_int, [P, C.c_int32, P, P, P]),
+        "orc_scene_top_accel": (C.c_int, [P, P]),
+        "orc_scene_top_motion_accel": (C.c_int, [P, P, P]),
+        "orc_trace": (C.c_int, [P, P, C.c_int64, C.c_int32, P, P, C.c_int32]),
+        "orc_brute": (C.c_int, [P, P, C.c_int64, P, C.c_int32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def build_qbvh(boxes, centroids, idx, leaf_max=16):
+    """Restated qbvh.BuildAccel; permutes boxes/centroids/idx in place. Returns (nodes, bounds)."""
+    lib = load()
+    n = len(idx)
+    nodes = np.zeros(max(n, 1), NODE_DTYPE)
+    bounds = np.zeros(6, np.float32)
+    rc = lib.orc_build_qbvh(_p(boxes), _p(centroids), _p(idx), n, leaf_max, _p(nodes), len(nodes), _p(bounds))
+    if rc < 0:
+        raise RuntimeError(f"oracle build failed ({rc})")
+    return nodes[:rc].copy(), bounds
+
+
+def intersect_boxes(P, D, boxes24):
+    lib = load()
+    P = np.ascontiguousarray(P, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    b = np.ascontiguousarray(boxes24, np.float32)
+    hits = np.zeros(4, np.int32)
+    t = np.zeros(4, np.float32)
+    lib.orc_intersect_boxes(_p(P), _p(D), _p(b), _p(hits), _p(t))
+    return hits, t
+
+
+def ray_setup(D):
+    lib = load()
+    D = np.ascontiguousarray(D, np.float32)
+    S = np.zeros(3, np.float32)
+    Dinv = np.zeros(3, np.float32)
+    K = np.zeros(3, np.int32)
+    lib.orc_ray_setup(_p(D), _p(S), _p(Dinv), _p(K))
+    return S, Dinv, K
+
+
+class OracleScene:
+    """Scene of PolyMeshes, built and traced exactly as the reference does (on the CPU)."""
+
+    def __init__(self):
+        self.lib = load()
+        self.h = self.lib.orc_scene_new()
+        self.meshes = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_scene_free(self.h)
+            self.h = None
+
+    def add_mesh(self, verts, idx, ray_bias=0.0, motion_reference_indexing=True):
+        v = np.ascontiguousarray(verts, np.float32)
+        if v.ndim == 2:
+            v = v[None]
+        ix = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        g = self.lib.orc_scene_add_mesh(self.h, _p(v), v.shape[1], v.shape[0], _p(ix), len(ix), float(ray_bias),
+                                        1 if motion_reference_indexing else 0)
+        self.meshes.append((v, ix))
+        return g
+
+    def commit(self):
+        if self.lib.orc_scene_commit(self.h) != 0:
+            raise RuntimeError("oracle scene build failed")
+
+    def info(self):
+        n = len(self.meshes)
+        ntop = np.zeros(1, np.int32)
+        order, nodes, tris, keys = (np.zeros(n, np.int32) for _ in range(4))
+        self.lib.orc_scene_info(self.h, _p(ntop), _p(order), _p(nodes), _p(tris), _p(keys))
+        return {"top_nodes": int(ntop[0]), "geom_order": order, "geom_nodes": nodes, "geom_tris": tris, "geom_keys": keys}
+
+    def mesh_accel(self, slot):
+        inf = self.info()
+        nodes = np.zeros(inf["geom_nodes"][slot], NODE_DTYPE)
+        ntris = inf["geom_tris"][slot]
+        idxp = np.zeros((ntris, 3), np.uint32)
+        perm = np.zeros(ntris, np.int32)
+        if self.lib.orc_scene_mesh_accel(self.h, slot, _p(nodes), _p(idxp), _p(perm)) != 0:
+            raise RuntimeError("not a static mesh")
+        return nodes, idxp, perm
+
+    def mesh_motion_accel(self, slot):
+        inf = self.info()
+        nn, ntris, nk = inf["geom_nodes"][slot], inf["geom_tris"][slot], inf["geom_keys"][slot]
+        topo = np.zeros((nn, 8), np.int32)
+        boxes = np.zeros((nk, nn, 24), np.float32)
+        perm = np.zeros(ntris, np.int32)
+        if self.lib.orc_scene_mesh_motion_accel(self.h, slot, _p(topo), _p(boxes), _p(perm)) != 0:
+            raise RuntimeError("not a motion mesh")
+        return topo, boxes, perm
+
+    def top_accel(self):
+        nodes = np.zeros(self.info()["top_nodes"], NODE_DTYPE)
+        if self.lib.orc_scene_top_accel(self.h, _p(nodes)) != 0:
+            raise RuntimeError("top level is a motion QBVH")
+        return nodes
+
+    def trace(self, rays, shadow=False, counters=False, nthreads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        n = len(rays)
+        hits = np.zeros(n, OHIT_DTYPE)
+        cnt = np.zeros((n, 4), np.int32) if counters else None
+        self.lib.orc_trace(self.h, _p(rays), n, 1 if shadow else 0, _p(hits), _p(cnt), nthreads)
+        return hits, cnt
+
+    def brute(self, rays, nthreads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), OHIT_DTYPE)
+        self.lib.orc_brute(self.h, _p(rays), len(rays), _p(hits), nthreads)
+        return hits

@@ -1,0 +1,99 @@
+"""CPU tests of the product library: the C ABI exports and the host QBVH builder.
+
+The builder (vermeer_amd/csrc/qbvh_build.cpp) is parallel; its output must
+equal the oracle's restatement of qbvh.BuildAccel node for node, and the
+permutation must be the same, because hit IDs are post-sort indices.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import vermeer_amd as va
+from vermeer_amd import _lib, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    header = open(os.path.join(ROOT, "include", "vermeer_gpu.h")).read()
+    declared = set(re.findall(r"^\w[\w\s\*]*?\b(vgpu_\w+)\s*\(", header, re.M))
+    assert len(declared) >= 16
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    # the ctypes table covers the same set
+    assert declared == {n for n, _, _ in _lib.SIGNATURES}
+
+
+def _tri_boxes(v, f):
+    V = v[f.astype(np.int64)]
+    lo = np.minimum(np.minimum(V[:, 0], V[:, 1]), V[:, 2])
+    hi = np.maximum(np.maximum(V[:, 0], V[:, 1]), V[:, 2])
+    cent = ((V[:, 0] + V[:, 1]) + V[:, 2]) / np.float32(3)
+    return (np.ascontiguousarray(np.concatenate([lo, hi], 1), np.float32), np.ascontiguousarray(cent, np.float32),
+            np.arange(len(f), dtype=np.int32))
+
+
+def _compare(boxes, cent, idx, leaf_max, nthreads=0):
+    b1, c1, i1 = boxes.copy(), cent.copy(), idx.copy()
+    b2, c2, i2 = boxes.copy(), cent.copy(), idx.copy()
+    n1, bd1 = oracle.build_qbvh(b1, c1, i1, leaf_max)
+    n2, bd2 = va.build_accel(b2, c2, i2, leaf_max, nthreads)
+    assert n1.tobytes() == n2.tobytes()
+    assert np.array_equal(i1, i2) and b1.tobytes() == b2.tobytes() and c1.tobytes() == c2.tobytes()
+    assert bd1.tobytes() == bd2.tobytes()
+
+
+@pytest.mark.parametrize("leaf_max", [1, 2, 4, 16])
+def test_builder_matches_oracle_random_boxes(leaf_max):
+    g = scenes.rng(100 + leaf_max)
+    n = 3000
+    c = g.uniform(-10, 10, (n, 3))
+    e = g.uniform(0.0, 0.5, (n, 3))
+    boxes = np.ascontiguousarray(np.concatenate([c - e, c + e], 1), np.float32)
+    cent = np.ascontiguousarray(c, np.float32)
+    _compare(boxes, cent, np.arange(n, dtype=np.int32), leaf_max)
+
+
+def test_builder_matches_oracle_sphere_and_is_thread_count_independent():
+    v, f = scenes.uv_sphere()
+    boxes, cent, idx = _tri_boxes(v, f)
+    for nt in (1, 3, 0):
+        _compare(boxes, cent, idx, 16, nt)
+
+
+def test_builder_matches_oracle_icosphere_field_parallel_path():
+    v, f = scenes.icosphere_field(count=256)[:2]        # 327,680 triangles: exercises threaded subtrees
+    boxes, cent, idx = _tri_boxes(v, f)
+    _compare(boxes, cent, idx, 16)
+
+
+def test_builder_reports_degenerate_input():
+    boxes = np.tile(np.array([0, 0, 0, 1, 1, 1], np.float32), (2, 1))
+    cent = np.full((2, 3), 0.5, np.float32)
+    with pytest.raises(va.VgpuError) as ei:
+        va.build_accel(boxes, cent, np.arange(2, dtype=np.int32), 1)
+    assert ei.value.code == _lib.VGPU_ERR_BUILD
+
+
+def test_builder_rejects_non_inplace_inputs():
+    with pytest.raises(TypeError):
+        va.build_accel(np.zeros((2, 6)), np.zeros((2, 3)), np.arange(2), 16)
+
+
+def test_motion_builder_topology_matches_oracle_motion_mesh():
+    verts, idx = scenes.moving_spheres(count=40)
+    o = oracle.OracleScene()
+    o.add_mesh(verts, idx, motion_reference_indexing=False)
+    o.commit()
+    topo, _, perm = o.mesh_motion_accel(0)
+    # same inputs as PolyMesh.initAccel for a 2-key mesh: vertices lerped at t = 0.5
+    V = (np.float32(0.5) * verts[0] + np.float32(0.5) * verts[1]).astype(np.float32)
+    boxes, cent, ids = _tri_boxes(V, idx)
+    mn = va.build_accel_motion(boxes, cent, ids, 16)
+    assert np.array_equal(ids, perm)
+    assert np.array_equal(mn["children"], topo[:, 3:7]) and np.array_equal(mn["axis"], topo[:, 0:3])

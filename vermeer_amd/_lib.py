@@ -1,0 +1,103 @@
+"""ctypes binding of libvermeer_gpu.so (the C ABI in include/vermeer_gpu.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in this package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvermeer_gpu.so")
+
+VGPU_OK = 0
+VGPU_ERR_ARG = -1
+VGPU_ERR_HIP = -2
+VGPU_ERR_BUILD = -3
+VGPU_ERR_STACK = -4
+VGPU_ERR_STATE = -5
+VGPU_ERR_NOMEM = -6
+VGPU_MAX_STACK = 128
+
+VGPU_TRACE_SHADOW = 1
+VGPU_TRACE_DEVICE_PTRS = 2
+VGPU_MESH_MOTION_REFERENCE_INDEXING = 1
+
+# vgpu_qbvh_node (128 bytes, qbvh.Node layout)
+NODE_DTYPE = np.dtype([("boxes", "<f4", 24), ("axis", "<u4", 3), ("children", "<i4", 4), ("parent", "<i4")])
+MNODE_DTYPE = np.dtype([("axis", "<i4", 3), ("children", "<i4", 4), ("pad", "<i4")])
+RAY_DTYPE = np.dtype([("P", "<f4", 3), ("tmax", "<f4"), ("D", "<f4", 3), ("time", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+assert NODE_DTYPE.itemsize == 128 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and MNODE_DTYPE.itemsize == 32
+
+
+class TraceConfig(C.Structure):
+    _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("waves_per_cu", C.c_int32), ("sort", C.c_int32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+# (name, restype, argtypes) for every entry point declared in include/vermeer_gpu.h
+_P = C.c_void_p
+_I32 = C.c_int32
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_F = C.c_float
+SIGNATURES = [
+    ("vgpu_last_error", C.c_char_p, []),
+    ("vgpu_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("vgpu_qbvh_build", C.c_int, [_P, _P, _P, _I32, _I32, _P, _I32, C.POINTER(_I32), _P, _I32]),
+    ("vgpu_qbvh_build_motion", C.c_int, [_P, _P, _P, _I32, _I32, _P, _I32, C.POINTER(_I32), _I32]),
+    ("vgpu_scene_create", C.c_int, [C.POINTER(_P), C.c_int]),
+    ("vgpu_scene_destroy", C.c_int, [_P]),
+    ("vgpu_scene_add_mesh", C.c_int, [_P, _P, _U32, _U32, _P, _U32, _F, _U32, C.POINTER(_I32)]),
+    ("vgpu_scene_add_mesh_prebuilt", C.c_int, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _F, C.POINTER(_I32)]),
+    ("vgpu_scene_commit", C.c_int, [_P]),
+    ("vgpu_scene_info", C.c_int, [_P, C.POINTER(_I32), C.POINTER(_I32), _P, _P, _P]),
+    ("vgpu_scene_mesh_accel", C.c_int, [_P, _I32, _P, _P, _P]),
+    ("vgpu_trace", C.c_int, [_P, _U32, _P, _U64, _P, _P, _P, _P]),
+    ("vgpu_scene_status", C.c_int, [_P, _P]),
+    ("vgpu_scene_set_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
+    ("vgpu_scene_get_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
+    ("vgpu_scene_autotune", C.c_int, [_P, _U32, _P, _U64, _P, _P, C.POINTER(TraceConfig), C.POINTER(_F)]),
+]
+
+_lib = None
+
+
+class VgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vgpu error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Returns the loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                           " (there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != VGPU_OK:
+        raise VgpuError(rc, load().vgpu_last_error().decode())
+    return rc
+
+
+def ptr(a):
+    """Raw data pointer of a numpy array (or None)."""
+    if a is None:
+        return None
+    return C.c_void_p(a.ctypes.data)

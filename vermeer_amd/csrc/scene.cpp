@@ -1,0 +1,661 @@
+// Scene object and C-ABI entry points (include/vermeer_gpu.h).
+//
+// Host side of the drop-in boundary: it mirrors what the reference does at
+// PreRender time — PolyMesh.initAccel per mesh (builtin/geom/polymesh/
+// buildqbvh.go:14-212) and Scene.initAccel for the top level
+// (builtin/scene/scene.go:135-268) — then flattens the result into the
+// device layout of trace_kernels.h and uploads it once.  Traversal calls are
+// batched: one kernel launch per call, never one call per ray.
+//
+// Compiled with -ffp-contract=off: the box/centroid arithmetic feeding the
+// builder must round exactly as the reference's.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/vermeer_gpu.h"
+#include "qbvh_build.h"
+#include "trace_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(VGPU_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+const float kInf = std::numeric_limits<float>::infinity();
+inline float mins(float x, float y) { return (x < y) ? x : y; }
+inline float maxs(float x, float y) { return (x > y) ? x : y; }
+
+struct Box6 {
+    float v[6];   // lo xyz, hi xyz
+    void reset() { for (int k = 0; k < 3; k++) { v[k] = kInf; v[3 + k] = -kInf; } }
+    void grow_point(const float* p) { for (int k = 0; k < 3; k++) { v[k] = mins(p[k], v[k]); v[3 + k] = maxs(p[k], v[3 + k]); } }
+    void grow_box(const float* b) { for (int k = 0; k < 3; k++) { v[k] = mins(v[k], b[k]); v[3 + k] = maxs(v[3 + k], b[3 + k]); } }
+};
+
+// math/vec3.go:52-58
+inline void lerp3(const float* a, const float* b, float t, float* o) {
+    for (int k = 0; k < 3; k++) o[k] = (1.0f - t) * a[k] + t * b[k];
+}
+
+struct MeshHost {
+    uint32_t nverts = 0, nkeys = 1, ntris = 0;
+    std::vector<float> verts;          // nkeys * nverts * 3
+    std::vector<uint32_t> idx;         // as given (static prebuilt: already post-sort)
+    float ray_bias = 0.0f;
+    uint32_t flags = 0;
+    bool built = false;
+    std::vector<vgpu_qbvh_node> nodes;
+    std::vector<uint32_t> idxp_sorted;
+    std::vector<int32_t> perm;
+    std::vector<vgpu_mqbvh_node> mnodes;
+    std::vector<float> mboxes;         // nkeys * nmnodes * 24
+    float bounds[6];
+    std::vector<float> motion_bounds;  // nkeys * 6
+
+    const float* vert(uint32_t key, uint32_t i) const { return &verts[((size_t)key * nverts + i) * 3]; }
+
+    // PolyMesh.Bounds (builtin/geom/polymesh/bounds.go:159-186)
+    void bounds_at(float time, float* out) const {
+        if (nkeys == 1) { std::memcpy(out, bounds, sizeof(bounds)); return; }
+        const float k = time * (float)(nkeys - 1);
+        const float t = k - std::floor(k);
+        const int key = (int)std::floor(k), key2 = (int)std::ceil(k);
+        for (int i = 0; i < 6; i++) out[i] = (1 - t) * motion_bounds[key * 6 + i] + t * motion_bounds[key2 * 6 + i];
+    }
+};
+
+}  // namespace
+
+struct vgpu_scene {
+    int device = 0;
+    int num_cus = 256;
+    std::vector<MeshHost> meshes;      // insertion order
+    std::vector<int32_t> order;        // post-sort slot -> insertion index
+    bool committed = false;
+    bool top_motion = false;
+    int top_keys = 1;
+    std::vector<vgpu_qbvh_node> top_nodes;
+    std::vector<vgpu_mqbvh_node> top_mnodes;
+    std::vector<float> top_mboxes;
+    bool any_motion = false;
+
+    vgpu::DevNode* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    float4* d_mnodes = nullptr;
+    float4* d_mtris = nullptr;
+    vgpu::DevGeom* d_geoms = nullptr;
+    int32_t* d_err = nullptr;
+    float* d_spT = nullptr;
+    int32_t* d_spN = nullptr;
+    uint64_t spill_threads = 0;
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    vgpu::SceneView view{};
+    vgpu_trace_config cfg{256, 16, 16, 0};
+
+    ~vgpu_scene() {
+        int prev = 0;
+        if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(device);
+        void* ptrs[] = {d_nodes, d_tris, d_mnodes, d_mtris, d_geoms, d_err, d_spT, d_spN, d_stage};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        (void)hipSetDevice(prev);
+    }
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) { if (hipGetDevice(&prev) != hipSuccess) prev = -1; (void)hipSetDevice(dev); }
+    ~DeviceGuard() { if (prev >= 0) (void)hipSetDevice(prev); }
+};
+
+// Static mesh accel, as PolyMesh.initAccel (buildqbvh.go:60-129).
+int build_static_mesh(MeshHost& m) {
+    const uint32_t n = m.ntris;
+    std::vector<float> boxes((size_t)n * 6), cent((size_t)n * 3);
+    std::vector<int32_t> ids(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const float* V0 = m.vert(0, m.idx[i * 3 + 0]);
+        const float* V1 = m.vert(0, m.idx[i * 3 + 1]);
+        const float* V2 = m.vert(0, m.idx[i * 3 + 2]);
+        Box6 b; b.reset(); b.grow_point(V0); b.grow_point(V1); b.grow_point(V2);
+        std::memcpy(&boxes[(size_t)i * 6], b.v, sizeof(b.v));
+        for (int k = 0; k < 3; k++) cent[(size_t)i * 3 + k] = (V0[k] + V1[k] + V2[k]) / 3.0f;
+        ids[i] = (int32_t)i;
+    }
+    int32_t nn = 0;
+    std::vector<vgpu_qbvh_node> nodes(std::max<uint32_t>(n, 1));
+    int rc = vgpu::build_qbvh(boxes.data(), cent.data(), ids.data(), (int32_t)n, 16, nodes.data(), (int32_t)nodes.size(),
+                              &nn, m.bounds, 0);
+    if (rc == -3) return fail(VGPU_ERR_BUILD, "BVH build failed: degenerate mesh (coincident centroids)");
+    if (rc != 0) return fail(VGPU_ERR_BUILD, "BVH build failed (%d)", rc);
+    nodes.resize(nn);
+    m.nodes.swap(nodes);
+    m.perm = ids;
+    m.idxp_sorted.resize((size_t)n * 3);
+    for (uint32_t i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) m.idxp_sorted[(size_t)i * 3 + c] = m.idx[(size_t)ids[i] * 3 + c];
+    m.built = true;
+    return VGPU_OK;
+}
+
+// Per-key node boxes of a motion topology (buildqbvh.go:171-212, scene.go:228-268).
+void motion_boxes_rec(const std::vector<vgpu_mqbvh_node>& topo, int32_t node, float* boxes_key,
+                      const std::function<void(int, int, Box6&)>& leaf, Box6& nodebox) {
+    nodebox.reset();
+    for (int c = 0; c < 4; c++) {
+        const int32_t ch = topo[node].children[c];
+        if (ch < 0) {
+            if (ch == -1) continue;
+            Box6 b; b.reset();
+            leaf((int)((ch & 0x7ffffff) >> 4), (int)((ch & 0xf) + 1), b);
+            for (int k = 0; k < 3; k++) { boxes_key[node * 24 + c + k * 4] = b.v[k]; boxes_key[node * 24 + c + 12 + k * 4] = b.v[3 + k]; }
+            nodebox.grow_box(b.v);
+        } else {
+            Box6 b;
+            motion_boxes_rec(topo, ch, boxes_key, leaf, b);
+            for (int k = 0; k < 3; k++) { boxes_key[node * 24 + c + k * 4] = b.v[k]; boxes_key[node * 24 + c + 12 + k * 4] = b.v[3 + k]; }
+            nodebox.grow_box(b.v);
+        }
+    }
+}
+
+// Motion mesh accel (buildqbvh.go:19-57 + initMotionBoxes :148-169).
+int build_motion_mesh(MeshHost& m) {
+    const uint32_t n = m.ntris;
+    std::vector<float> boxes((size_t)n * 6), cent((size_t)n * 3);
+    std::vector<int32_t> ids(n);
+    const float k = 0.5f * (float)(m.nkeys - 1);
+    const float time = k - std::floor(k);
+    const int key = (int)std::floor(k), key2 = (int)std::ceil(k);
+    for (uint32_t i = 0; i < n; i++) {
+        float V[3][3];
+        for (int j = 0; j < 3; j++) lerp3(m.vert(key, m.idx[i * 3 + j]), m.vert(key2, m.idx[i * 3 + j]), time, V[j]);
+        Box6 b; b.reset(); b.grow_point(V[0]); b.grow_point(V[1]); b.grow_point(V[2]);
+        std::memcpy(&boxes[(size_t)i * 6], b.v, sizeof(b.v));
+        for (int c = 0; c < 3; c++) cent[(size_t)i * 3 + c] = (V[0][c] + V[1][c] + V[2][c]) / 3.0f;
+        ids[i] = (int32_t)i;
+    }
+    int32_t nn = 0;
+    std::vector<vgpu_mqbvh_node> nodes(std::max<uint32_t>(n, 1));
+    int rc = vgpu::build_qbvh_motion(boxes.data(), cent.data(), ids.data(), (int32_t)n, 16, nodes.data(),
+                                     (int32_t)nodes.size(), &nn, 0);
+    if (rc == -3) return fail(VGPU_ERR_BUILD, "motion BVH build failed: degenerate mesh");
+    if (rc != 0) return fail(VGPU_ERR_BUILD, "motion BVH build failed (%d)", rc);
+    nodes.resize(nn);
+    m.mnodes.swap(nodes);
+    m.perm = ids;
+    m.mboxes.assign((size_t)m.nkeys * nn * 24, 0.0f);
+    m.motion_bounds.assign((size_t)m.nkeys * 6, 0.0f);
+    Box6 full; full.reset();
+    for (uint32_t kk = 0; kk < m.nkeys; kk++) {
+        auto leaf = [&](int lb, int lc, Box6& b) {
+            for (int i = lb; i < lb + lc; i++) {
+                const int f = m.perm[i];
+                for (int j = 0; j < 3; j++) b.grow_point(m.vert(kk, m.idx[(size_t)f * 3 + j]));
+            }
+        };
+        Box6 rb;
+        motion_boxes_rec(m.mnodes, 0, &m.mboxes[(size_t)kk * nn * 24], leaf, rb);
+        std::memcpy(&m.motion_bounds[kk * 6], rb.v, sizeof(rb.v));
+        full.grow_box(rb.v);
+    }
+    std::memcpy(m.bounds, full.v, sizeof(full.v));
+    m.built = true;
+    return VGPU_OK;
+}
+
+void to_dev(const vgpu_qbvh_node& s, vgpu::DevNode& d) {
+    std::memcpy(d.lox, s.boxes, 24 * sizeof(float));   // rows lox, loy, loz, hix, hiy, hiz
+    for (int c = 0; c < 4; c++) d.children[c] = s.children[c];
+    d.axes[0] = (int32_t)s.axis0; d.axes[1] = (int32_t)s.axis1; d.axes[2] = (int32_t)s.axis2; d.axes[3] = 0;
+}
+
+void append_motion_nodes(std::vector<float4>& out, const std::vector<vgpu_mqbvh_node>& topo, const std::vector<float>& boxes,
+                         int nkeys) {
+    const size_t nn = topo.size();
+    for (size_t i = 0; i < nn; i++) {
+        float4 t0, t1;
+        std::memcpy(&t0, topo[i].children, 16);
+        int32_t ax[4] = {topo[i].axis0, topo[i].axis1, topo[i].axis2, 0};
+        std::memcpy(&t1, ax, 16);
+        out.push_back(t0); out.push_back(t1);
+        for (int k = 0; k < nkeys; k++) {
+            const float* b = &boxes[((size_t)k * nn + i) * 24];
+            for (int r = 0; r < 6; r++) out.push_back(make_float4(b[r * 4 + 0], b[r * 4 + 1], b[r * 4 + 2], b[r * 4 + 3]));
+        }
+    }
+}
+
+template <class T>
+int upload(T** dptr, const std::vector<T>& v) {
+    if (v.empty()) return VGPU_OK;
+    HIP_TRY(hipMalloc((void**)dptr, v.size() * sizeof(T)));
+    HIP_TRY(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return VGPU_OK;
+}
+
+int ensure_spill(vgpu_scene* s, uint64_t threads) {
+    if (s->d_spT && s->spill_threads >= threads) return VGPU_OK;
+    if (s->d_spT) { HIP_TRY(hipFree(s->d_spT)); s->d_spT = nullptr; }
+    if (s->d_spN) { HIP_TRY(hipFree(s->d_spN)); s->d_spN = nullptr; }
+    // size for the smallest LDS depth so later configuration changes never need more
+    const uint64_t per = (uint64_t)(VGPU_MAX_STACK - 8);
+    HIP_TRY(hipMalloc((void**)&s->d_spT, per * threads * sizeof(float)));
+    HIP_TRY(hipMalloc((void**)&s->d_spN, per * threads * sizeof(int32_t)));
+    s->spill_threads = threads;
+    return VGPU_OK;
+}
+
+int check_config(const vgpu_trace_config& c) {
+    if (c.block != 64 && c.block != 128 && c.block != 256) return fail(VGPU_ERR_ARG, "block must be 64, 128 or 256");
+    if (c.lds_stack != 8 && c.lds_stack != 16 && c.lds_stack != 24 && c.lds_stack != 32)
+        return fail(VGPU_ERR_ARG, "lds_stack must be 8, 16, 24 or 32");
+    if (c.block * c.lds_stack * 8 > 65536) return fail(VGPU_ERR_ARG, "block*lds_stack exceeds 64 KiB of LDS");
+    if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return fail(VGPU_ERR_ARG, "waves_per_cu must be in [1,32]");
+    if (c.sort != 0) return fail(VGPU_ERR_ARG, "ray sorting is not available in this build");
+    return VGPU_OK;
+}
+
+int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,
+                  int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg) {
+    vgpu::TraceLaunch L{cfg.block, cfg.lds_stack, cfg.waves_per_cu};
+    const uint64_t threads = vgpu::trace_grid_threads(L, n, s->num_cus);
+    int rc = ensure_spill(s, threads);
+    if (rc) return rc;
+    hipError_t e = vgpu::launch_trace(s->view, (flags & VGPU_TRACE_SHADOW) != 0, s->any_motion, L, d_rays, n, d_hits,
+                                      d_geoms, d_counters, s->d_spT, s->d_spN, s->spill_threads, s->d_err, s->num_cus, st);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
+    return VGPU_OK;
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+
+extern "C" {
+
+const char* vgpu_last_error(void) { return g_err.c_str(); }
+
+int vgpu_device_count(int* count) {
+    if (!count) return fail(VGPU_ERR_ARG, "count is NULL");
+    HIP_TRY(hipGetDeviceCount(count));
+    return VGPU_OK;
+}
+
+int vgpu_qbvh_build(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max, vgpu_qbvh_node* nodes_out,
+                    int32_t max_nodes, int32_t* nnodes_out, float* bounds_out, int32_t nthreads) {
+    int rc = vgpu::build_qbvh(boxes, centroids, idx, n, leaf_max, nodes_out, max_nodes, nnodes_out, bounds_out, nthreads);
+    if (rc == -3) return fail(VGPU_ERR_BUILD, "degenerate input: the reference recursion does not terminate");
+    if (rc != 0) return fail(VGPU_ERR_ARG, "bad arguments or max_nodes too small (need %d)", nnodes_out ? *nnodes_out : -1);
+    return VGPU_OK;
+}
+
+int vgpu_qbvh_build_motion(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
+                           vgpu_mqbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out, int32_t nthreads) {
+    int rc = vgpu::build_qbvh_motion(boxes, centroids, idx, n, leaf_max, nodes_out, max_nodes, nnodes_out, nthreads);
+    if (rc == -3) return fail(VGPU_ERR_BUILD, "degenerate input: the reference recursion does not terminate");
+    if (rc != 0) return fail(VGPU_ERR_ARG, "bad arguments or max_nodes too small (need %d)", nnodes_out ? *nnodes_out : -1);
+    return VGPU_OK;
+}
+
+int vgpu_scene_create(vgpu_scene** out, int device) {
+    if (!out) return fail(VGPU_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(VGPU_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    vgpu_scene* s = new (std::nothrow) vgpu_scene();
+    if (!s) return fail(VGPU_ERR_NOMEM, "out of host memory");
+    s->device = device;
+    DeviceGuard dg(device);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) s->num_cus = cus;
+    if (hipMalloc((void**)&s->d_err, sizeof(int32_t)) != hipSuccess || hipMemset(s->d_err, 0, sizeof(int32_t)) != hipSuccess) {
+        delete s;
+        return fail(VGPU_ERR_HIP, "cannot allocate the error flag");
+    }
+    *out = s;
+    return VGPU_OK;
+}
+
+int vgpu_scene_destroy(vgpu_scene* scene) {
+    delete scene;
+    return VGPU_OK;
+}
+
+int vgpu_scene_add_mesh(vgpu_scene* s, const float* verts, uint32_t nverts, uint32_t nkeys, const uint32_t* idx,
+                        uint32_t ntris, float ray_bias, uint32_t flags, int32_t* geom_out) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (!verts || !idx || nverts == 0 || ntris == 0 || nkeys == 0) return fail(VGPU_ERR_ARG, "empty mesh");
+    if (ntris >= (1u << 23)) return fail(VGPU_ERR_ARG, "mesh has %u triangles; the leaf encoding holds < 2^23 (qbvh/qbvh.go:64)", ntris);
+    for (uint64_t i = 0; i < (uint64_t)ntris * 3; i++)
+        if (idx[i] >= nverts) return fail(VGPU_ERR_ARG, "index %u out of range at %llu", idx[i], (unsigned long long)i);
+    MeshHost m;
+    m.nverts = nverts; m.nkeys = nkeys; m.ntris = ntris; m.ray_bias = ray_bias; m.flags = flags;
+    m.verts.assign(verts, verts + (size_t)nverts * nkeys * 3);
+    m.idx.assign(idx, idx + (size_t)ntris * 3);
+    s->meshes.push_back(std::move(m));
+    if (geom_out) *geom_out = (int32_t)s->meshes.size() - 1;
+    return VGPU_OK;
+}
+
+int vgpu_scene_add_mesh_prebuilt(vgpu_scene* s, const float* verts, uint32_t nverts, const uint32_t* idxp_sorted,
+                                 uint32_t ntris, const vgpu_qbvh_node* nodes, uint32_t nnodes, const float* bounds,
+                                 float ray_bias, int32_t* geom_out) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (!verts || !idxp_sorted || !nodes || !bounds || nverts == 0 || ntris == 0 || nnodes == 0)
+        return fail(VGPU_ERR_ARG, "empty mesh");
+    for (uint64_t i = 0; i < (uint64_t)ntris * 3; i++)
+        if (idxp_sorted[i] >= nverts) return fail(VGPU_ERR_ARG, "index out of range");
+    for (uint32_t i = 0; i < nnodes; i++)
+        for (int c = 0; c < 4; c++) {
+            const int32_t ch = nodes[i].children[c];
+            if (ch >= (int32_t)nnodes) return fail(VGPU_ERR_ARG, "node %u child %d out of range", i, ch);
+            // BuildAccel numbers nodes in depth-first pre-order, so a child always follows its parent;
+            // requiring it rules out cycles, which would never terminate on the device
+            if (ch >= 0 && ch <= (int32_t)i) return fail(VGPU_ERR_ARG, "node %u child %d does not follow its parent", i, ch);
+            if (ch < -1) {
+                const uint32_t b = (uint32_t)((ch & 0x7ffffff) >> 4), cnt = (uint32_t)((ch & 0xf) + 1);
+                if (b + cnt > ntris) return fail(VGPU_ERR_ARG, "node %u leaf [%u,+%u) beyond %u triangles", i, b, cnt, ntris);
+            }
+        }
+    MeshHost m;
+    m.nverts = nverts; m.nkeys = 1; m.ntris = ntris; m.ray_bias = ray_bias;
+    m.verts.assign(verts, verts + (size_t)nverts * 3);
+    m.idxp_sorted.assign(idxp_sorted, idxp_sorted + (size_t)ntris * 3);
+    m.nodes.assign(nodes, nodes + nnodes);
+    std::memcpy(m.bounds, bounds, sizeof(m.bounds));
+    m.built = true;
+    s->meshes.push_back(std::move(m));
+    if (geom_out) *geom_out = (int32_t)s->meshes.size() - 1;
+    return VGPU_OK;
+}
+
+int vgpu_scene_commit(vgpu_scene* s) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (s->meshes.empty()) return fail(VGPU_ERR_STATE, "scene has no geometry");
+    for (auto& m : s->meshes) {
+        if (m.built) continue;
+        int rc = m.nkeys == 1 ? build_static_mesh(m) : build_motion_mesh(m);
+        if (rc) return rc;
+    }
+    // Scene.initAccel (scene.go:135-203)
+    const int ng = (int)s->meshes.size();
+    int maxKeys = 0;
+    for (auto& m : s->meshes) maxKeys = std::max<int>(maxKeys, (int)m.nkeys);
+    std::vector<float> boxes((size_t)ng * 6), cent((size_t)ng * 3);
+    std::vector<int32_t> ids(ng);
+    const float t_build = maxKeys == 1 ? 0.0f : 0.5f;
+    for (int i = 0; i < ng; i++) {
+        float b[6];
+        s->meshes[i].bounds_at(t_build, b);
+        std::memcpy(&boxes[(size_t)i * 6], b, sizeof(b));
+        for (int k = 0; k < 3; k++) cent[(size_t)i * 3 + k] = (b[3 + k] + b[k]) * 0.5f;   // BoundingBox.Centroid
+        ids[i] = i;
+    }
+    int32_t nn = 0;
+    if (maxKeys == 1) {
+        s->top_nodes.resize(std::max(ng, 1));
+        int rc = vgpu::build_qbvh(boxes.data(), cent.data(), ids.data(), ng, 1, s->top_nodes.data(),
+                                  (int32_t)s->top_nodes.size(), &nn, nullptr, 1);
+        if (rc) return fail(VGPU_ERR_BUILD, "scene top-level build failed (coincident geom centroids?)");
+        s->top_nodes.resize(nn);
+        s->top_motion = false;
+        s->top_keys = 1;
+    } else {
+        s->top_mnodes.resize(std::max(ng, 1));
+        int rc = vgpu::build_qbvh_motion(boxes.data(), cent.data(), ids.data(), ng, 1, s->top_mnodes.data(),
+                                         (int32_t)s->top_mnodes.size(), &nn, 1);
+        if (rc) return fail(VGPU_ERR_BUILD, "scene top-level motion build failed");
+        s->top_mnodes.resize(nn);
+        s->top_motion = true;
+        s->top_keys = maxKeys;
+    }
+    s->order = ids;
+    if (s->top_motion) {
+        // scene.go:207-268
+        s->top_mboxes.assign((size_t)maxKeys * nn * 24, 0.0f);
+        for (int kk = 0; kk < maxKeys; kk++) {
+            auto leaf = [&](int lb, int lc, Box6& b) {
+                for (int i = lb; i < lb + lc; i++) {
+                    const float time = (float)kk / (float)(maxKeys - 1);
+                    float gb[6];
+                    s->meshes[s->order[i]].bounds_at(time, gb);
+                    b.grow_box(gb);
+                }
+            };
+            Box6 rb;
+            motion_boxes_rec(s->top_mnodes, 0, &s->top_mboxes[(size_t)kk * nn * 24], leaf, rb);
+        }
+    }
+    s->any_motion = maxKeys > 1;
+
+    // Flatten into the device layout.
+    std::vector<vgpu::DevNode> dnodes;
+    std::vector<float4> dtris, dmnodes, dmtris;
+    std::vector<vgpu::DevGeom> dgeoms(ng);
+    if (!s->top_motion) {
+        for (auto& nd : s->top_nodes) { vgpu::DevNode d; to_dev(nd, d); dnodes.push_back(d); }
+    } else {
+        s->view.top_mbase = 0;
+        append_motion_nodes(dmnodes, s->top_mnodes, s->top_mboxes, maxKeys);
+    }
+    for (int slot = 0; slot < ng; slot++) {
+        MeshHost& m = s->meshes[s->order[slot]];
+        vgpu::DevGeom& g = dgeoms[slot];
+        g.nkeys = (int32_t)m.nkeys;
+        g.ray_bias = m.ray_bias;
+        if (m.nkeys == 1) {
+            g.node_base = (int32_t)dnodes.size();
+            for (auto& nd : m.nodes) { vgpu::DevNode d; to_dev(nd, d); dnodes.push_back(d); }
+            g.tri_base = (int32_t)(dtris.size() / 3);
+            for (uint32_t i = 0; i < m.ntris; i++)
+                for (int j = 0; j < 3; j++) {
+                    const float* v = m.vert(0, m.idxp_sorted[(size_t)i * 3 + j]);
+                    dtris.push_back(make_float4(v[0], v[1], v[2], 0.0f));
+                }
+        } else {
+            g.node_base = (int32_t)dmnodes.size();
+            append_motion_nodes(dmnodes, m.mnodes, m.mboxes, (int)m.nkeys);
+            g.tri_base = (int32_t)dmtris.size();
+            const bool ref_idx = (m.flags & VGPU_MESH_MOTION_REFERENCE_INDEXING) != 0;
+            for (uint32_t i = 0; i < m.ntris; i++) {
+                const uint32_t f = ref_idx ? i : (uint32_t)m.perm[i];
+                for (uint32_t kk = 0; kk < m.nkeys; kk++)
+                    for (int j = 0; j < 3; j++) {
+                        const float* v = m.vert(kk, m.idx[(size_t)f * 3 + j]);
+                        dmtris.push_back(make_float4(v[0], v[1], v[2], 0.0f));
+                    }
+            }
+        }
+        if (dtris.size() / 3 > 0x7fffffffull || dmtris.size() > 0x7fffffffull || dmnodes.size() > 0x7fffffffull)
+            return fail(VGPU_ERR_ARG, "scene too large for 32-bit offsets");
+    }
+    DeviceGuard dg(s->device);
+    int rc;
+    if ((rc = upload(&s->d_nodes, dnodes))) return rc;
+    if ((rc = upload(&s->d_tris, dtris))) return rc;
+    if ((rc = upload(&s->d_mnodes, dmnodes))) return rc;
+    if ((rc = upload(&s->d_mtris, dmtris))) return rc;
+    if ((rc = upload(&s->d_geoms, dgeoms))) return rc;
+    s->view.nodes = s->d_nodes;
+    s->view.tris = s->d_tris;
+    s->view.mnodes = s->d_mnodes;
+    s->view.mtris = s->d_mtris;
+    s->view.geoms = s->d_geoms;
+    s->view.top_motion = s->top_motion ? 1 : 0;
+    s->view.top_keys = s->top_keys;
+    s->committed = true;
+    return VGPU_OK;
+}
+
+int vgpu_scene_info(const vgpu_scene* s, int32_t* ngeoms, int32_t* top_nodes, int32_t* geom_order, int32_t* geom_nodes,
+                    int32_t* geom_tris) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (!s->committed) return fail(VGPU_ERR_STATE, "scene not committed");
+    const int ng = (int)s->meshes.size();
+    if (ngeoms) *ngeoms = ng;
+    if (top_nodes) *top_nodes = s->top_motion ? (int32_t)s->top_mnodes.size() : (int32_t)s->top_nodes.size();
+    for (int i = 0; i < ng; i++) {
+        const MeshHost& m = s->meshes[s->order[i]];
+        if (geom_order) geom_order[i] = s->order[i];
+        if (geom_nodes) geom_nodes[i] = m.nkeys == 1 ? (int32_t)m.nodes.size() : (int32_t)m.mnodes.size();
+        if (geom_tris) geom_tris[i] = (int32_t)m.ntris;
+    }
+    return VGPU_OK;
+}
+
+int vgpu_scene_mesh_accel(const vgpu_scene* s, int32_t slot, vgpu_qbvh_node* nodes, uint32_t* idxp_sorted, int32_t* perm) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (!s->committed) return fail(VGPU_ERR_STATE, "scene not committed");
+    if (slot < 0 || slot >= (int32_t)s->meshes.size()) return fail(VGPU_ERR_ARG, "slot out of range");
+    const MeshHost& m = s->meshes[s->order[slot]];
+    if (m.nkeys != 1) return fail(VGPU_ERR_ARG, "slot %d is a motion mesh", slot);
+    if (nodes) std::memcpy(nodes, m.nodes.data(), m.nodes.size() * sizeof(vgpu_qbvh_node));
+    if (idxp_sorted) std::memcpy(idxp_sorted, m.idxp_sorted.data(), m.idxp_sorted.size() * sizeof(uint32_t));
+    if (perm) {
+        if (m.perm.empty()) return fail(VGPU_ERR_STATE, "prebuilt mesh carries no permutation");
+        std::memcpy(perm, m.perm.data(), m.perm.size() * sizeof(int32_t));
+    }
+    return VGPU_OK;
+}
+
+int vgpu_trace(vgpu_scene* s, uint32_t flags, const vgpu_ray* rays, uint64_t n, vgpu_hit* hits, int32_t* geoms,
+               int32_t* counters, void* stream) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (!s->committed) return fail(VGPU_ERR_STATE, "trace before commit");
+    if (n == 0) return VGPU_OK;
+    if (!rays || !hits) return fail(VGPU_ERR_ARG, "rays/hits are NULL");
+    DeviceGuard dg(s->device);
+    hipStream_t st = (hipStream_t)stream;
+    if (flags & VGPU_TRACE_DEVICE_PTRS)
+        return enqueue_trace(s, flags, rays, n, hits, geoms, counters, st, s->cfg);
+
+    // host buffers: stage through device memory owned by the scene
+    const size_t rb = n * sizeof(vgpu_ray), hb = n * sizeof(vgpu_hit), gb = geoms ? n * 4 : 0, cb = counters ? n * 16 : 0;
+    const size_t need = rb + hb + gb + cb;
+    if (need > s->stage_bytes) {
+        if (s->d_stage) { HIP_TRY(hipFree(s->d_stage)); s->d_stage = nullptr; s->stage_bytes = 0; }
+        HIP_TRY(hipMalloc(&s->d_stage, need));
+        s->stage_bytes = need;
+    }
+    char* base = (char*)s->d_stage;
+    void* d_rays = base;
+    void* d_hits = base + rb;
+    int32_t* d_geoms = geoms ? (int32_t*)(base + rb + hb) : nullptr;
+    int32_t* d_cnt = counters ? (int32_t*)(base + rb + hb + gb) : nullptr;
+    HIP_TRY(hipMemsetAsync(s->d_err, 0, sizeof(int32_t), st));
+    HIP_TRY(hipMemcpyAsync(d_rays, rays, rb, hipMemcpyHostToDevice, st));
+    int rc = enqueue_trace(s, flags, d_rays, n, d_hits, d_geoms, d_cnt, st, s->cfg);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(hits, d_hits, hb, hipMemcpyDeviceToHost, st));
+    if (geoms) HIP_TRY(hipMemcpyAsync(geoms, d_geoms, gb, hipMemcpyDeviceToHost, st));
+    if (counters) HIP_TRY(hipMemcpyAsync(counters, d_cnt, cb, hipMemcpyDeviceToHost, st));
+    int32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
+    return VGPU_OK;
+}
+
+int vgpu_scene_status(vgpu_scene* s, void* stream) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    DeviceGuard dg(s->device);
+    hipStream_t st = (hipStream_t)stream;
+    int32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemsetAsync(s->d_err, 0, sizeof(int32_t), st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
+    return VGPU_OK;
+}
+
+int vgpu_scene_set_trace_config(vgpu_scene* s, const vgpu_trace_config* cfg) {
+    if (!s || !cfg) return fail(VGPU_ERR_ARG, "NULL argument");
+    int rc = check_config(*cfg);
+    if (rc) return rc;
+    s->cfg = *cfg;
+    return VGPU_OK;
+}
+
+int vgpu_scene_get_trace_config(const vgpu_scene* s, vgpu_trace_config* cfg) {
+    if (!s || !cfg) return fail(VGPU_ERR_ARG, "NULL argument");
+    *cfg = s->cfg;
+    return VGPU_OK;
+}
+
+int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, uint64_t n, vgpu_hit* d_hits, void* stream,
+                        vgpu_trace_config* best_out, float* ms_out) {
+    if (!s || !d_rays || !d_hits) return fail(VGPU_ERR_ARG, "NULL argument");
+    if (!s->committed) return fail(VGPU_ERR_STATE, "autotune before commit");
+    DeviceGuard dg(s->device);
+    hipStream_t st = (hipStream_t)stream;
+    const int blocks[] = {64, 128, 256};
+    const int ldss[] = {8, 16, 24, 32};
+    const int waves[] = {8, 16, 32};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    vgpu_trace_config bc = s->cfg;
+    for (int b : blocks)
+        for (int l : ldss)
+            for (int w : waves) {
+                vgpu_trace_config c{b, l, w, 0};
+                if (check_config(c)) continue;
+                int rc = enqueue_trace(s, flags | VGPU_TRACE_DEVICE_PTRS, d_rays, n, d_hits, nullptr, nullptr, st, c);
+                if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+                HIP_TRY(hipEventRecord(e0, st));
+                for (int rep = 0; rep < 2; rep++) {
+                    rc = enqueue_trace(s, flags | VGPU_TRACE_DEVICE_PTRS, d_rays, n, d_hits, nullptr, nullptr, st, c);
+                    if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+                }
+                HIP_TRY(hipEventRecord(e1, st));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                ms *= 0.5f;
+                if (ms < best) { best = ms; bc = c; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_err.clear();
+    s->cfg = bc;
+    if (best_out) *best_out = bc;
+    if (ms_out) *ms_out = best;
+    return vgpu_scene_status(s, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Internal interface between the scene object (scene.cpp) and the
+// traversal kernels (trace.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace vgpu {
+
+// Device node, 128 bytes = one cache line.  Same content as qbvh.Node
+// (qbvh/qbvh.go:31-36) re-laid for aligned 16-byte loads: the six box rows
+// (min x/y/z, max x/y/z; lane c = child c) are the reference's Boxes[24]
+// verbatim, followed by the children and the three split axes.
+struct alignas(16) DevNode {
+    float lox[4], loy[4], loz[4];
+    float hix[4], hiy[4], hiz[4];
+    int32_t children[4];   // reference encoding, node indices relative to the geom's node base
+    int32_t axes[4];       // axis0, axis1, axis2, 0
+};
+static_assert(sizeof(DevNode) == 128, "DevNode is one 128-byte line");
+
+// Motion node (qbvh.MotionNode + its per-key MotionNodeBoxes): a 32-byte
+// topology record followed by nkeys box blocks of 96 bytes (6 float4 rows as
+// above).  Stride in float4 units: 2 + 6*nkeys.
+struct alignas(16) DevMTopo {
+    int32_t children[4];
+    int32_t axes[4];
+};
+
+// Geom table entry.  Static geom: node_base indexes DevNode, tri_base indexes
+// triangles (3 float4 each: v0, v1, v2 with w unused).  Motion geom:
+// node_base/tri_base are float4 offsets into the motion arrays; a motion
+// triangle is nkeys consecutive groups of 3 float4.
+struct alignas(16) DevGeom {
+    int32_t node_base;
+    int32_t tri_base;
+    int32_t nkeys;      // 1 = static
+    float ray_bias;     // PolyMesh.RayBias (builtin/geom/polymesh/polymesh.go:19)
+};
+
+struct SceneView {
+    const DevNode* nodes;     // static nodes: top level at [0, ntop) when the top is static
+    const float4* tris;       // static triangles
+    const float4* mnodes;     // motion nodes (topology + boxes)
+    const float4* mtris;      // motion triangles
+    const DevGeom* geoms;     // post-sort geom slots
+    int32_t top_motion;       // 1: the top level is a motion QBVH at mnodes[top_mbase]
+    int32_t top_keys;         // motion keys of the top level
+    int32_t top_mbase;
+    int32_t pad;
+};
+
+struct TraceLaunch {
+    int block;          // 64, 128, 256
+    int lds_depth;      // 8, 16, 24, 32
+    int waves_per_cu;   // resident waves per CU used to size the grid
+};
+
+// Enqueues one traversal over n rays (device pointers).  spill_T / spill_N
+// hold (VGPU_MAX_STACK - lds_depth) entries per launched thread; the launch
+// uses at most spill_threads threads.  Returns hipSuccess or the launch error.
+hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const TraceLaunch& cfg,
+                        const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
+                        float* spill_T, int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag,
+                        int num_cus, hipStream_t stream);
+
+// Threads the launch above will use (so the caller can size the spill area).
+uint64_t trace_grid_threads(const TraceLaunch& cfg, uint64_t n, int num_cus);
+
+}  // namespace vgpu
