@@ -81,6 +81,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
                            " (there is no CPU fallback)")
+    # One HIP runtime per process.  torch ships its own libamdhip64 (SONAME libamdhip64.so.7) and
+    # loads it by a different file name; loading this library first would pull in /opt/rocm's copy
+    # and leave two runtimes fighting over the device.  Importing torch first makes the dynamic
+    # linker bind this library to the runtime torch already holds.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)
