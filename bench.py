@@ -202,6 +202,16 @@ def main():
                          f"C++ restatement of the Go traversal, OpenMP {threads} threads, {dt:.1f}s"}
         log(f"[bench] CPU baseline {cpu['value']} Mrays/s on {threads} threads")
 
+    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a
+    # plain copy), scaled to this run's mean rays per launch; null when no measurement matches this kernel
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_trace_traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        if tj.get("workload") == "icosphere_field_bounce_rays" and tj["kernel"].startswith(
+                f"k_trace<{cfg['block']},{cfg['lds_stack']},"):
+            traffic = round(tj["hbm_bytes_per_ray"] * n / len(segs))
+
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -234,7 +244,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_ray": round(bytes_per_ray, 1),
             "kernel_ms_per_step": round(kernel_ms_per_step, 4),
             "kernel_avg_launch_ms": round(float(np.mean(launch_ms)), 4),

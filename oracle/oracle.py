@@ -169,3 +169,148 @@ class OracleScene:
         hits = np.zeros(len(rays), OHIT_DTYPE)
         self.lib.orc_brute(self.h, _p(rays), len(rays), _p(hits), nthreads)
         return hits
+
+
+# ------------------------------------------------------------------ renderer
+
+MATERIAL_DTYPE = np.dtype([
+    ("emission", "<f4", 3), ("emission_strength", "<f4"), ("has_emission_colour", "<i4"), ("has_emission_strength", "<i4"),
+    ("diffuse", "<f4", 3), ("has_diffuse_colour", "<i4"),
+    ("diffuse_strength", "<f4"), ("has_diffuse_strength", "<i4"),
+    ("diffuse_roughness", "<f4"), ("has_diffuse_roughness", "<i4"),
+    ("spec1", "<f4", 3), ("has_spec1_colour", "<i4"),
+    ("spec1_strength", "<f4"), ("has_spec1_strength", "<i4"),
+    ("spec1_roughness", "<f4"), ("has_spec1_roughness", "<i4"),
+    ("ior", "<f4"), ("has_ior", "<i4"),
+    ("fresnel_model", "<i4"),
+])
+
+
+def _render_sigs(lib):
+    P = C.c_void_p
+    sig = {
+        "orc_camera_lookat": (None, [P, P, P, C.c_float, C.c_float, P, P]),
+        "orc_vdc": (C.c_double, [C.c_uint64, C.c_uint64]),
+        "orc_sobol": (C.c_double, [C.c_uint64, C.c_uint64]),
+        "orc_raster_xy": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P]),
+        "orc_raster_tables": (None, [C.c_int32, P, P]),
+        "orc_smits": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
+        "orc_spectrum_to_rgb": (None, [P, C.c_float, P]),
+        "orc_cie_tables": (None, [P, P, P]),
+        "orc_scene_add_shaded_mesh": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, P, P, C.c_int32]),
+        "orc_render_new": (P, [P, C.c_int32, C.c_int32, P, C.c_float, C.c_float, C.c_float, P, C.c_int32]),
+        "orc_render_free": (None, [P]),
+        "orc_render_add_trilight": (None, [P, P, P, P, C.c_int32, C.c_int32, C.c_int32]),
+        "orc_render_pass": (C.c_int64, [P, P, C.c_uint32, P, P, C.c_int32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def camera_lookat(frm, to, up, fov_deg, focal=12.0):
+    """Camera matrix (column-major 4x4) and TanThetaFocal as the reference computes them."""
+    lib = load()
+    _render_sigs(lib)
+    M = np.zeros(16, np.float32)
+    ttf = np.zeros(1, np.float32)
+    f = np.asarray(frm, np.float32)
+    t = np.asarray(to, np.float32)
+    u = np.asarray(up, np.float32)
+    lib.orc_camera_lookat(_p(f), _p(t), _p(u), float(fov_deg), float(focal), _p(M), _p(ttf))
+    return M, float(ttf[0])
+
+
+def vdc(i, s=0):
+    lib = load(); _render_sigs(lib)
+    return lib.orc_vdc(i, s)
+
+
+def sobol(i, s=0):
+    lib = load(); _render_sigs(lib)
+    return lib.orc_sobol(i, s)
+
+
+def raster_xy(m, frame, px, py):
+    lib = load(); _render_sigs(lib)
+    idx = np.zeros(1, np.uint64)
+    rx = np.zeros(1, np.float64)
+    ry = np.zeros(1, np.float64)
+    lib.orc_raster_xy(m, frame, px, py, _p(idx), _p(rx), _p(ry))
+    return int(idx[0]), float(rx[0]), float(ry[0])
+
+
+def raster_tables(m):
+    lib = load(); _render_sigs(lib)
+    fwd = np.zeros(52, np.uint64)
+    inv = np.zeros(52, np.uint64)
+    lib.orc_raster_tables(m, _p(fwd), _p(inv))
+    return fwd, inv
+
+
+def smits(r, g, b, lam):
+    lib = load(); _render_sigs(lib)
+    return lib.orc_smits(r, g, b, lam)
+
+
+def spectrum_to_rgb(C4, lam):
+    lib = load(); _render_sigs(lib)
+    c = np.ascontiguousarray(C4, np.float32)
+    out = np.zeros(3, np.float32)
+    lib.orc_spectrum_to_rgb(_p(c), float(lam), _p(out))
+    return out
+
+
+def cie_tables():
+    lib = load(); _render_sigs(lib)
+    x, y, z = (np.zeros(95, np.float32) for _ in range(3))
+    lib.orc_cie_tables(_p(x), _p(y), _p(z))
+    return x, y, z
+
+
+class OracleRenderer:
+    """The reference's render loop (reference semantics) on the CPU.
+
+    scene_desc: dict from vermeer_amd.scenes (meshes with shader tables, materials, tri lights, camera).
+    """
+
+    def __init__(self, desc, width, height, M, tan_theta_focal, focal=12.0):
+        self.lib = load()
+        _render_sigs(self.lib)
+        self.scene = OracleScene()
+        self.w, self.h = width, height
+        self._keep = []
+        for mesh in desc["meshes"]:
+            v = np.ascontiguousarray(mesh["verts"], np.float32)
+            ix = np.ascontiguousarray(mesh["idx"], np.uint32)
+            nrm = mesh.get("normals")
+            nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
+            sidx = mesh.get("shaderidx")
+            sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
+            shaders = np.ascontiguousarray(mesh["shaders"], np.int32)
+            self._keep += [v, ix, nrm, sidx, shaders]
+            self.lib.orc_scene_add_shaded_mesh(self.scene.h, _p(v), len(v), _p(ix), len(ix), _p(nrm), _p(sidx),
+                                               _p(shaders), len(shaders))
+            self.scene.meshes.append((v, ix))
+        self.scene.commit()
+        mats = np.ascontiguousarray(desc["materials"], MATERIAL_DTYPE)
+        Mx = np.ascontiguousarray(M, np.float32)
+        self.h_r = self.lib.orc_render_new(self.scene.h, width, height, _p(Mx), float(tan_theta_focal),
+                                           float(width) / float(height), float(focal), _p(mats), len(mats))
+        for L in desc["lights"]:
+            p = [np.ascontiguousarray(L[k], np.float32) for k in ("P0", "P1", "P2")]
+            self.lib.orc_render_add_trilight(self.h_r, _p(p[0]), _p(p[1]), _p(p[2]), int(L["material"]),
+                                             int(L["samples"]), int(L["geom"]))
+        self.film = np.zeros((height, width, 3), np.float32)
+
+    def __del__(self):
+        if getattr(self, "h_r", None):
+            self.lib.orc_render_free(self.h_r)
+            self.h_r = None
+
+    def render_pass(self, scrambles, iteration, nthreads=0, want_sample=False):
+        scr = np.ascontiguousarray(scrambles, np.uint64)
+        samp = np.zeros((self.h, self.w, 3), np.float32) if want_sample else None
+        n = self.lib.orc_render_pass(self.h_r, _p(scr), iteration, _p(self.film), _p(samp), nthreads)
+        return n, samp

@@ -34,800 +34,8 @@
 // Mesh transforms (trace.go:22-55), instancing and procedurals are not part
 // of this restatement: the workloads it checks have none.
 
-#include <cstdint>
-#include <cstring>
-#include <cmath>
-#include <cstdio>
-#include <vector>
-#include <limits>
-#include <xmmintrin.h>
+#include "oracle_core.h"
 
-#ifdef _OPENMP
-#include <omp.h>
-#endif
-
-namespace orc {
-
-// ---------------------------------------------------------------- scalar math
-
-static inline float bits2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
-static inline uint32_t f2bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
-
-// MAXSS X1,X0 : X0 = (x > y) ? x : y   (math/dim_amd64.s:8-13)
-static inline float Max(float x, float y) { return (x > y) ? x : y; }
-// MINSS X1,X0 : X0 = (x < y) ? x : y   (math/dim_amd64.s:16-21)
-static inline float Min(float x, float y) { return (x < y) ? x : y; }
-// math/abs.go:13-15
-static inline float Abs(float x) { return bits2f(f2bits(x) & 0x7fffffffu); }
-// math/unsafe.go:15-37
-static inline uint32_t SignMask(float v) { return f2bits(v) & 0x80000000u; }
-static inline float Xorf(float v, uint32_t i) { return bits2f(f2bits(v) ^ i); }
-// ROUNDSS $1 / $2 (math/floor_amd64.s:12-21)
-static inline float Floor(float x) { return std::floor(x); }
-static inline float Ceil(float x) { return std::ceil(x); }
-
-static const float kInf = std::numeric_limits<float>::infinity();
-// math/ferror.go:16,19 (decimal constants rounded to float32: 2^-23 and 2^-24)
-static const float EpsilonFloat32 = 1.19209290E-07f;
-static const float MachineEpsilon32 = 5.9604645e-08f;
-// math/ferror.go:22-24
-static inline float Gamma(int32_t n) {
-    float ne = (float)n * MachineEpsilon32;
-    return ne / (1.0f - ne);
-}
-
-struct Vec3 { float v[3]; float& operator[](int i) { return v[i]; } float operator[](int i) const { return v[i]; } };
-
-// math/vec3.go:52-58
-static inline Vec3 Lerp(const Vec3& a, const Vec3& b, float t) {
-    Vec3 r;
-    for (int k = 0; k < 3; k++) r[k] = (1.0f - t) * a[k] + t * b[k];
-    return r;
-}
-
-// ------------------------------------------------------------ bounding boxes
-
-struct Box {
-    float b[2][3];
-    void Reset() { for (int i = 0; i < 3; i++) { b[0][i] = kInf; b[1][i] = -kInf; } }
-    // math/boundingbox.go:107-116 (note the operand order differs from GrowBox)
-    void Grow(float X, float Y, float Z) {
-        b[0][0] = Min(X, b[0][0]); b[1][0] = Max(X, b[1][0]);
-        b[0][1] = Min(Y, b[0][1]); b[1][1] = Max(Y, b[1][1]);
-        b[0][2] = Min(Z, b[0][2]); b[1][2] = Max(Z, b[1][2]);
-    }
-    void GrowVec3(const Vec3& p) { Grow(p[0], p[1], p[2]); }
-    // math/boundingbox.go:99-104
-    void GrowBox(const Box& p) {
-        for (int k = 0; k < 3; k++) { b[0][k] = Min(b[0][k], p.b[0][k]); b[1][k] = Max(b[1][k], p.b[1][k]); }
-    }
-    float Dim(int a) const { return b[1][a] - b[0][a]; }
-    // math/boundingbox.go:17-19
-    float SurfaceArea() const { return Dim(0) * Dim(1) * 2.0f + Dim(1) * Dim(2) * 2.0f + Dim(0) * Dim(2) * 2.0f; }
-    // math/boundingbox.go:54-70
-    int LongestAxis() const {
-        if (Dim(0) < Dim(1)) return (Dim(1) < Dim(2)) ? 2 : 1;
-        return (Dim(0) < Dim(2)) ? 2 : 0;
-    }
-    // math/boundingbox.go:27-32
-    Vec3 Centroid() const {
-        Vec3 o;
-        for (int k = 0; k < 3; k++) o[k] = (b[1][k] + b[0][k]) * 0.5f;
-        return o;
-    }
-};
-
-static Box InfBox() { Box b; for (int i = 0; i < 2; i++) for (int k = 0; k < 3; k++) b.b[i][k] = kInf; return b; }
-
-// math/boundingbox.go:119-127
-static Box BoxLerp(const Box& a, const Box& c, float t) {
-    Box o;
-    for (int i = 0; i < 2; i++) for (int k = 0; k < 3; k++) o.b[i][k] = (1 - t) * a.b[i][k] + t * c.b[i][k];
-    return o;
-}
-
-// --------------------------------------------------------------- QBVH nodes
-
-// qbvh/qbvh.go:31-36 — 128-byte node, boxes SoA at idx + i*12 + k*4.
-struct Node {
-    float boxes[24];
-    uint32_t axis0, axis1, axis2;
-    int32_t children[4];
-    int32_t parent;
-};
-static_assert(sizeof(Node) == 128, "node layout");
-
-// qbvh/mqbvh.go:24-29 (topology only; boxes per key are separate)
-struct MNode {
-    int32_t axis0, axis1, axis2;
-    int32_t children[4];
-    int32_t parent;
-    uint32_t pad[2];
-};
-typedef float MBoxes[24];
-
-static inline void SetBounds(float* boxes, int idx, const Box& bb) {
-    for (int i = 0; i < 2; i++) for (int k = 0; k < 3; k++) boxes[idx + i * 12 + k * 4] = bb.b[i][k];
-}
-static inline Box GetBounds(const float* boxes, int idx) {
-    Box bb;
-    for (int i = 0; i < 2; i++) for (int k = 0; k < 3; k++) bb.b[i][k] = boxes[idx + i * 12 + k * 4];
-    return bb;
-}
-// qbvh/qbvh.go:61-64
-static inline int LeafCount(int32_t l) { return (int)((l & 0xf) + 1); }
-static inline int LeafBase(int32_t l) { return (int)((l & 0x7ffffff) >> 4); }
-static inline int32_t LeafCode(uint32_t first, uint32_t count) {
-    uint32_t v = (1u << 31) | ((first << 4) & 0xfffffff0u) | ((count - 1) & 0xfu);
-    return (int32_t)v;
-}
-// qbvh/qbvh.go:67-93
-static void NodeSetLeaf(Node& n, int idx, uint32_t first, uint32_t count) {
-    if (count == 0) { n.children[idx] = -1; SetBounds(n.boxes, idx, InfBox()); return; }
-    n.children[idx] = LeafCode(first, count);
-}
-// qbvh/mqbvh.go:62-78
-static void MNodeSetLeaf(MNode& n, int idx, uint32_t first, uint32_t count) {
-    if (count == 0) { n.children[idx] = -1; return; }
-    n.children[idx] = LeafCode(first, count);
-}
-
-// ----------------------------------------------------------------- builder
-
-struct BuildError {};
-
-static const int nbins = 8;
-
-// qbvh/build.go:28-136
-static void calcMinCost(const Box& bounds, Vec3* centroids, Box* boxes, int32_t* indxs, int n, int& axis_out, int& pivot_out) {
-    Box binBounds[nbins];
-    int32_t binN[nbins] = {0};
-    int axis = bounds.LongestAxis();
-    if (bounds.b[1][axis] == bounds.b[0][axis]) { axis_out = axis; pivot_out = n / 2 + 1; return; }
-    const float k1 = 7.99952f / (bounds.b[1][axis] - bounds.b[0][axis]);   // float32(8)*(1-0.00006) folded exactly, then rounded
-    const float k0 = bounds.b[0][axis];
-    for (int i = 0; i < nbins; i++) binBounds[i].Reset();
-    for (int i = 0; i < n; i++) {
-        float c = centroids[i][axis];
-        int bin = (int)(k1 * (c - k0));
-        if (bin < 0 || bin > nbins - 1) throw BuildError();
-        binN[bin]++;
-        binBounds[bin].GrowBox(boxes[i]);
-    }
-    Box lbox[nbins], rbox[nbins];
-    int32_t lN[nbins], rN[nbins];
-    Box box; box.Reset();
-    int32_t cnt = 0;
-    for (int i = 0; i < nbins; i++) { box.GrowBox(binBounds[i]); cnt += binN[i]; lbox[i] = box; lN[i] = cnt; }
-    box.Reset(); cnt = 0;
-    for (int i = 0; i < nbins; i++) {
-        box.GrowBox(binBounds[nbins - 1 - i]); cnt += binN[nbins - 1 - i];
-        rbox[nbins - 1 - i] = box; rN[nbins - 1 - i] = cnt;
-    }
-    int binMinCost = -1;
-    float minCost = kInf;
-    for (int i = 1; i < nbins; i++) {
-        float cost = lbox[i - 1].SurfaceArea() * (float)lN[i - 1] + rbox[i].SurfaceArea() * (float)rN[i];
-        if (cost < minCost) { binMinCost = i; minCost = cost; }
-    }
-    int left = 0, right = n - 1;
-    while (left <= right) {
-        float c = centroids[left][axis];
-        int bin = (int)(k1 * (c - k0));
-        if (bin < binMinCost) {
-            left++;
-        } else {
-            std::swap(indxs[left], indxs[right]);
-            std::swap(centroids[left], centroids[right]);
-            std::swap(boxes[left], boxes[right]);
-            right--;
-        }
-    }
-    axis_out = axis; pivot_out = left;
-}
-
-// qbvh/build.go:138-149
-static Box calcBox(const Box* boxes, int n) {
-    Box b; b.Reset();
-    for (int i = 0; i < n; i++) b.GrowBox(boxes[i]);
-    return b;
-}
-
-// qbvh/build.go:151-178
-static void binarySplit(Box* boxes, Vec3* centroids, int leafMax, int32_t* indxs, int n, int& axis, int& pivot) {
-    Box bounds; bounds.Reset();
-    for (int i = 0; i < n; i++) bounds.GrowVec3(centroids[i]);
-    if (n <= leafMax) { axis = 0; pivot = n; return; }
-    calcMinCost(bounds, centroids, boxes, indxs, n, axis, pivot);
-}
-
-// qbvh/build.go:180-287
-static int32_t buildAccelRec(std::vector<Node>& nodes, Box* boxes, Vec3* centroids, int32_t* indxs, int n,
-                             int leafMax, int baseidx, Box& nodebox_out, int depth) {
-    if (depth > 4096) throw BuildError();   // the reference recurses forever on degenerate input
-    int axis0, pivot0, axis1, pivot1, axis2, pivot2;
-    binarySplit(boxes, centroids, leafMax, indxs, n, axis0, pivot0);
-    binarySplit(boxes, centroids, leafMax, indxs, pivot0, axis1, pivot1);
-    binarySplit(boxes + pivot0, centroids + pivot0, leafMax, indxs + pivot0, n - pivot0, axis2, pivot2);
-    int32_t nodei = (int32_t)nodes.size();
-    nodes.push_back(Node());
-    std::memset(&nodes[nodei], 0, sizeof(Node));
-    nodes[nodei].axis0 = (uint32_t)axis0; nodes[nodei].axis1 = (uint32_t)axis1; nodes[nodei].axis2 = (uint32_t)axis2;
-    const int lo[4] = {0, pivot1, pivot0, pivot0 + pivot2};
-    const int hi[4] = {pivot1, pivot0, pivot0 + pivot2, n};
-    for (int c = 0; c < 4; c++) {
-        int cn = hi[c] - lo[c];
-        if (cn <= leafMax) {
-            SetBounds(nodes[nodei].boxes, c, calcBox(boxes + lo[c], cn));
-            NodeSetLeaf(nodes[nodei], c, (uint32_t)(baseidx + lo[c]), (uint32_t)cn);
-        } else {
-            Box cb;
-            int32_t child = buildAccelRec(nodes, boxes + lo[c], centroids + lo[c], indxs + lo[c], cn, leafMax, baseidx + lo[c], cb, depth + 1);
-            SetBounds(nodes[nodei].boxes, c, cb);
-            nodes[nodei].children[c] = child;
-        }
-    }
-    Box nb; nb.Reset();
-    for (int i = 0; i < 4; i++)
-        if (nodes[nodei].children[i] != -1) nb.GrowBox(GetBounds(nodes[nodei].boxes, i));
-    nodebox_out = nb;
-    return nodei;
-}
-
-// qbvh/build.go:293-307
-static Box BuildAccel(std::vector<Node>& nodes, Box* boxes, Vec3* centroids, int32_t* indxs, int n, int leafMax) {
-    if (leafMax > 16) leafMax = 16;
-    if (leafMax < 1) leafMax = 1;
-    Box bounds;
-    buildAccelRec(nodes, boxes, centroids, indxs, n, leafMax, 0, bounds, 0);
-    return bounds;
-}
-
-// qbvh/motionbuild.go:12-100
-static int32_t buildAccelMotionRec(std::vector<MNode>& nodes, Box* boxes, Vec3* centroids, int32_t* indxs, int n,
-                                   int leafMax, int baseidx, int depth) {
-    if (depth > 4096) throw BuildError();
-    int axis0, pivot0, axis1, pivot1, axis2, pivot2;
-    binarySplit(boxes, centroids, leafMax, indxs, n, axis0, pivot0);
-    binarySplit(boxes, centroids, leafMax, indxs, pivot0, axis1, pivot1);
-    binarySplit(boxes + pivot0, centroids + pivot0, leafMax, indxs + pivot0, n - pivot0, axis2, pivot2);
-    int32_t nodei = (int32_t)nodes.size();
-    nodes.push_back(MNode());
-    std::memset(&nodes[nodei], 0, sizeof(MNode));
-    nodes[nodei].axis0 = axis0; nodes[nodei].axis1 = axis1; nodes[nodei].axis2 = axis2;
-    const int lo[4] = {0, pivot1, pivot0, pivot0 + pivot2};
-    const int hi[4] = {pivot1, pivot0, pivot0 + pivot2, n};
-    for (int c = 0; c < 4; c++) {
-        int cn = hi[c] - lo[c];
-        if (cn <= leafMax) {
-            MNodeSetLeaf(nodes[nodei], c, (uint32_t)(baseidx + lo[c]), (uint32_t)cn);
-        } else {
-            int32_t child = buildAccelMotionRec(nodes, boxes + lo[c], centroids + lo[c], indxs + lo[c], cn, leafMax, baseidx + lo[c], depth + 1);
-            nodes[nodei].children[c] = child;
-        }
-    }
-    return nodei;
-}
-
-// qbvh/motionbuild.go:108-122
-static void BuildAccelMotion(std::vector<MNode>& nodes, Box* boxes, Vec3* centroids, int32_t* indxs, int n, int leafMax) {
-    if (leafMax > 16) leafMax = 16;
-    if (leafMax < 1) leafMax = 1;
-    buildAccelMotionRec(nodes, boxes, centroids, indxs, n, leafMax, 0, 0);
-}
-
-// ---------------------------------------------------------------------- rays
-
-enum { RayTypeCamera = 1, RayTypeShadow = 2, RayTypeReflected = 4 };
-
-struct StackEntry { float T; int32_t Node; };
-
-struct Task {
-    float T[4];
-    int32_t Hits[4];
-    float Boxes[24];
-    int32_t StackTop;
-    StackEntry Stack[256];     // the reference holds 90 entries (core/ray.go:158); deeper use is reported
-    int32_t maxDepth;
-};
-
-struct Ray {
-    Vec3 P, D, Dinv;
-    float Tclosest;
-    float S[3];
-    int32_t Kx, Ky, Kz;
-    float Time;
-    uint32_t Type;
-    int64_t NodesT, LeafsT, TrisT;
-    Task* task;
-};
-
-// core/ray.go:103-148
-static void RaySetup(Ray& r) {
-    r.Kz = 0;
-    if (Abs(r.D[1]) > Abs(r.D[2])) {
-        if (Abs(r.D[1]) > Abs(r.D[0])) r.Kz = 1;
-    } else {
-        if (Abs(r.D[2]) > Abs(r.D[0])) r.Kz = 2;
-    }
-    r.Kx = r.Kz + 1; if (r.Kx == 3) r.Kx = 0;
-    r.Ky = r.Kx + 1; if (r.Ky == 3) r.Ky = 0;
-    if (r.D[r.Kz] < 0.0f) { int32_t t = r.Kx; r.Kx = r.Ky; r.Ky = t; }
-    double z = (double)r.D[r.Kz];
-    r.S[2] = (float)(1.0 / z);
-    r.S[0] = (float)((double)r.D[r.Kx] / z);
-    r.S[1] = (float)((double)r.D[r.Ky] / z);
-    for (int k = 0; k < 3; k++) r.Dinv[k] = (float)(1.0 / (double)r.D[k]);
-}
-
-// qbvh/intersect_amd64.s:13-100 — lane semantics of the SSE sequence.
-static void intersectBoxes(const Ray& ray, const float* boxes, int32_t* hits, float* t) {
-    for (int c = 0; c < 4; c++) {
-        float o, di, t1, t2, tmin, tmax, m, M;
-        o = ray.P[0]; di = ray.Dinv[0];
-        t1 = (boxes[c + 0] - o) * di;          // boxmin.x
-        t2 = (boxes[c + 12] - o) * di;         // boxmax.x
-        tmin = (t2 < t1) ? t2 : t1;            // MINPS X2,X6
-        tmax = (t2 > t1) ? t2 : t1;            // MAXPS X2,X7
-        for (int k = 1; k < 3; k++) {
-            o = ray.P[k]; di = ray.Dinv[k];
-            t1 = (boxes[c + 4 * k] - o) * di;
-            t2 = (boxes[c + 12 + 4 * k] - o) * di;
-            m = (t2 < t1) ? t2 : t1;           // MINPS X2,X1
-            M = (t2 > t1) ? t2 : t1;           // MAXPS X2,X0
-            tmin = (tmin > m) ? tmin : m;      // MAXPS X1,X6
-            tmax = (tmax < M) ? tmax : M;      // MINPS X0,X7
-        }
-        float tNear = (0.0f > tmin) ? 0.0f : tmin;   // XORPS; MAXPS X6,X0
-        t[c] = tNear;
-        hits[c] = (tNear <= tmax) ? -1 : 0;          // CMPPS $2 (ordered LE)
-    }
-}
-
-// ------------------------------------------------------------------- meshes
-
-struct HitRecord {
-    int32_t elem;          // ElemID (post-sort triangle index, trace.go:513)
-    float U, V, W;         // barycentric weights of v0, v1, v2
-    Vec3 Po;               // motion: point accumulated during the test loop (trace.go:625-629)
-    Vec3 Ng;               // motion path sets these inside the loop (trace.go:650-665)
-    Vec3 Poffset;
-    Vec3 DdPdu, DdPdv;
-    float Bu, Bv;
-};
-
-struct Mesh {
-    int nkeys = 1;
-    int elemsPerKey = 0;
-    std::vector<Vec3> verts;          // nkeys * elemsPerKey
-    std::vector<uint32_t> idxp;       // post-sort for static meshes; original order for motion meshes
-    std::vector<int32_t> accelIdx;    // accel.idx: leaf slot -> original face
-    int facecount = 0;
-    float RayBias = 0.0f;
-    bool motionBug = true;            // reproduce trace.go:532-537 (idxp indexed by leaf slot)
-    std::vector<Node> qbvh;
-    std::vector<MNode> mnodes;
-    std::vector<std::vector<float>> mboxes;   // per key: 24 floats per node
-    Box bounds;
-    std::vector<Box> motionBounds;
-
-    // builtin/geom/polymesh/buildqbvh.go:14-144
-    void initAccel() {
-        int n = facecount;
-        std::vector<Box> boxes(n);
-        std::vector<Vec3> centroids(n);
-        std::vector<int32_t> idxs(n);
-        if (nkeys > 1) {
-            float k = 0.5f * (float)(nkeys - 1);
-            float time = k - Floor(k);
-            int key = (int)Floor(k), key2 = (int)Ceil(k);
-            for (int i = 0; i < n; i++) {
-                Vec3 V0 = Lerp(verts[idxp[i * 3 + 0] + elemsPerKey * key], verts[idxp[i * 3 + 0] + elemsPerKey * key2], time);
-                Vec3 V1 = Lerp(verts[idxp[i * 3 + 1] + elemsPerKey * key], verts[idxp[i * 3 + 1] + elemsPerKey * key2], time);
-                Vec3 V2 = Lerp(verts[idxp[i * 3 + 2] + elemsPerKey * key], verts[idxp[i * 3 + 2] + elemsPerKey * key2], time);
-                boxes[i].Reset(); boxes[i].GrowVec3(V0); boxes[i].GrowVec3(V1); boxes[i].GrowVec3(V2);
-                for (int c = 0; c < 3; c++) centroids[i][c] = (V0[c] + V1[c] + V2[c]) / 3.0f;
-                idxs[i] = i;
-            }
-            BuildAccelMotion(mnodes, boxes.data(), centroids.data(), idxs.data(), n, 16);
-            accelIdx = idxs;
-            initMotionBoxes();
-            return;
-        }
-        for (int i = 0; i < n; i++) {
-            const Vec3& V0 = verts[idxp[i * 3 + 0]];
-            const Vec3& V1 = verts[idxp[i * 3 + 1]];
-            const Vec3& V2 = verts[idxp[i * 3 + 2]];
-            boxes[i].Reset(); boxes[i].GrowVec3(V0); boxes[i].GrowVec3(V1); boxes[i].GrowVec3(V2);
-            for (int c = 0; c < 3; c++) centroids[i][c] = (V0[c] + V1[c] + V2[c]) / 3.0f;
-            idxs[i] = i;
-        }
-        bounds = BuildAccel(qbvh, boxes.data(), centroids.data(), idxs.data(), n, 16);
-        accelIdx = idxs;
-        std::vector<uint32_t> nidx(idxp.size());
-        for (int i = 0; i < n; i++)
-            for (int c = 0; c < 3; c++) nidx[i * 3 + c] = idxp[idxs[i] * 3 + c];
-        idxp.swap(nidx);
-    }
-
-    // buildqbvh.go:148-212
-    void initMotionBoxes() {
-        mboxes.assign(nkeys, std::vector<float>(mnodes.size() * 24, 0.0f));
-        Box full; full.Reset();
-        for (int k = 0; k < nkeys; k++) {
-            Box b = initMotionBoxesRec(k, 0);
-            motionBounds.push_back(b);
-            full.GrowBox(b);
-        }
-        bounds = full;
-    }
-    Box initMotionBoxesRec(int key, int32_t node) {
-        Box nodebox; nodebox.Reset();
-        for (int c = 0; c < 4; c++) {
-            int32_t ch = mnodes[node].children[c];
-            if (ch < 0) {
-                if (ch == -1) continue;
-                Box box; box.Reset();
-                int lb = LeafBase(ch), lc = LeafCount(ch);
-                for (int i = lb; i < lb + lc; i++) {
-                    int f = accelIdx[i];
-                    box.GrowVec3(verts[idxp[f * 3 + 0] + elemsPerKey * key]);
-                    box.GrowVec3(verts[idxp[f * 3 + 1] + elemsPerKey * key]);
-                    box.GrowVec3(verts[idxp[f * 3 + 2] + elemsPerKey * key]);
-                }
-                SetBounds(&mboxes[key][node * 24], c, box);
-                nodebox.GrowBox(box);
-            } else {
-                Box box = initMotionBoxesRec(key, ch);
-                SetBounds(&mboxes[key][node * 24], c, box);
-                nodebox.GrowBox(box);
-            }
-        }
-        return nodebox;
-    }
-
-    // polymesh/bounds.go:159-186
-    Box Bounds(float time) const {
-        if (nkeys == 1) return bounds;
-        float k = time * (float)(motionBounds.size() - 1);
-        float t = k - Floor(k);
-        int key = (int)Floor(k), key2 = (int)Ceil(k);
-        return BoxLerp(motionBounds[key], motionBounds[key2], t);
-    }
-    int MotionKeys() const { return nkeys == 1 ? 1 : (int)mboxes.size(); }
-
-    // builtin/geom/polymesh/trace.go:108-194 (idxp branch) — returns true on an improving hit.
-    bool TraceElems(Ray& ray, HitRecord& hr, int base, int count) const {
-        int32_t idx = -1;
-        float U = 0, V = 0, W = 0;
-        const int Kx = ray.Kx, Ky = ray.Ky, Kz = ray.Kz;
-        for (int i = base; i < base + count; i++) {
-            ray.TrisT++;
-            const Vec3& v0 = verts[idxp[i * 3 + 0]];
-            const Vec3& v1 = verts[idxp[i * 3 + 1]];
-            const Vec3& v2 = verts[idxp[i * 3 + 2]];
-            float AKz = v0[Kz] - ray.P[Kz];
-            float BKz = v1[Kz] - ray.P[Kz];
-            float CKz = v2[Kz] - ray.P[Kz];
-            float Cx = (v2[Kx] - ray.P[Kx]) - ray.S[0] * CKz;
-            float By = (v1[Ky] - ray.P[Ky]) - ray.S[1] * BKz;
-            float Cy = (v2[Ky] - ray.P[Ky]) - ray.S[1] * CKz;
-            float Bx = (v1[Kx] - ray.P[Kx]) - ray.S[0] * BKz;
-            float Ax = (v0[Kx] - ray.P[Kx]) - ray.S[0] * AKz;
-            float Ay = (v0[Ky] - ray.P[Ky]) - ray.S[1] * AKz;
-            float fU = Cx * By - Cy * Bx;
-            float fV = Ax * Cy - Ay * Cx;
-            float fW = Bx * Ay - By * Ax;
-            if (fU == 0.0f || fV == 0.0f || fW == 0.0f) {
-                fU = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-                fV = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-                fW = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
-            }
-            if ((fU < 0.0f || fV < 0.0f || fW < 0.0f) && (fU > 0.0f || fV > 0.0f || fW > 0.0f)) continue;
-            float det = fU + fV + fW;
-            if (det == 0.0f) continue;
-            float T = ray.S[2] * (fU * AKz + fV * BKz + fW * CKz);
-            uint32_t detSign = SignMask(det);
-            if (Xorf(T, detSign) < (EpsilonFloat32 + RayBias) * Xorf(det, detSign) ||
-                Xorf(T, detSign) > ray.Tclosest * Xorf(det, detSign)) continue;
-            float rcpDet = 1.0f / det;
-            U = fU * rcpDet; V = fV * rcpDet; W = fW * rcpDet;
-            ray.Tclosest = T * rcpDet;
-            idx = i;
-        }
-        if (idx == -1) return false;
-        hr.elem = idx; hr.U = U; hr.V = V; hr.W = W; hr.Bu = U; hr.Bv = V;
-        // hit attributes, trace.go:292-342
-        const Vec3& v0 = verts[idxp[idx * 3 + 0]];
-        const Vec3& v1 = verts[idxp[idx * 3 + 1]];
-        const Vec3& v2 = verts[idxp[idx * 3 + 2]];
-        float xs = Abs(U * v0[0]) + Abs(V * v1[0]) + Abs(W * v2[0]);
-        float ys = Abs(U * v0[1]) + Abs(V * v1[1]) + Abs(W * v2[1]);
-        float zs = Abs(U * v0[2]) + Abs(V * v1[2]) + Abs(W * v2[2]);
-        float e00 = v1[0] - v0[0], e01 = v1[1] - v0[1], e02 = v1[2] - v0[2];
-        float e10 = v2[0] - v0[0], e11 = v2[1] - v0[1], e12 = v2[2] - v0[2];
-        Vec3 Ng; Ng[0] = e01 * e12 - e02 * e11; Ng[1] = e02 * e10 - e00 * e12; Ng[2] = e00 * e11 - e01 * e10;
-        Ng = Normalize(Ng);
-        hr.Ng = Ng;
-        float d = Gamma(7) * xs * Abs(Ng[0]) + Gamma(7) * ys * Abs(Ng[1]) + Gamma(7) * zs * Abs(Ng[2]);
-        for (int k = 0; k < 3; k++) hr.Poffset[k] = Ng[k] * d;
-        for (int k = 0; k < 3; k++) hr.Po[k] = U * v0[k] + V * v1[k] + W * v2[k];
-        return true;
-    }
-
-    // trace.go:520-669 — the motion variant.
-    bool TraceMotionElems(float time, int key, int key2, Ray& ray, HitRecord& hr, int base, int count) const {
-        int32_t idx = -1;
-        float U = 0, V = 0, W = 0;
-        const int Kx = ray.Kx, Ky = ray.Ky, Kz = ray.Kz;
-        for (int i = base; i < base + count; i++) {
-            ray.TrisT++;
-            int32_t faceidx = i;
-            int f = motionBug ? i : accelIdx[i];
-            int i0 = (int)idxp[f * 3 + 0], i1 = (int)idxp[f * 3 + 1], i2 = (int)idxp[f * 3 + 2];
-            Vec3 V0 = Lerp(verts[i0 + elemsPerKey * key], verts[i0 + elemsPerKey * key2], time);
-            Vec3 V1 = Lerp(verts[i1 + elemsPerKey * key], verts[i1 + elemsPerKey * key2], time);
-            Vec3 V2 = Lerp(verts[i2 + elemsPerKey * key], verts[i2 + elemsPerKey * key2], time);
-            float AKz = V0[Kz] - ray.P[Kz];
-            float BKz = V1[Kz] - ray.P[Kz];
-            float CKz = V2[Kz] - ray.P[Kz];
-            float Cx = (V2[Kx] - ray.P[Kx]) - ray.S[0] * CKz;
-            float By = (V1[Ky] - ray.P[Ky]) - ray.S[1] * BKz;
-            float Cy = (V2[Ky] - ray.P[Ky]) - ray.S[1] * CKz;
-            float Bx = (V1[Kx] - ray.P[Kx]) - ray.S[0] * BKz;
-            float Ax = (V0[Kx] - ray.P[Kx]) - ray.S[0] * AKz;
-            float Ay = (V0[Ky] - ray.P[Ky]) - ray.S[1] * AKz;
-            float fU = Cx * By - Cy * Bx;
-            float fV = Ax * Cy - Ay * Cx;
-            float fW = Bx * Ay - By * Ax;
-            if (fU == 0.0f || fV == 0.0f || fW == 0.0f) {
-                fU = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-                fV = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-                fW = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
-            }
-            if ((fU < 0.0f || fV < 0.0f || fW < 0.0f) && (fU > 0.0f || fV > 0.0f || fW > 0.0f)) continue;
-            float det = fU + fV + fW;
-            if (det == 0.0f) continue;
-            float T = ray.S[2] * (fU * AKz + fV * BKz + fW * CKz);
-            uint32_t detSign = SignMask(det);
-            if (Xorf(T, detSign) <= RayBias * Xorf(det, detSign) ||
-                Xorf(T, detSign) > ray.Tclosest * Xorf(det, detSign)) continue;
-            float rcpDet = 1.0f / det;
-            U = fU * rcpDet; V = fV * rcpDet; W = fW * rcpDet;
-            ray.Tclosest = T * rcpDet;
-            idx = faceidx;
-            for (int k = 0; k < 3; k++) hr.Po[k] = U * V0[k] + V * V1[k] + W * V2[k];
-            float e00 = V1[0] - V0[0], e01 = V1[1] - V0[1], e02 = V1[2] - V0[2];
-            float e10 = V2[0] - V0[0], e11 = V2[1] - V0[1], e12 = V2[2] - V0[2];
-            Vec3 Ng; Ng[0] = e01 * e12 - e02 * e11; Ng[1] = e02 * e10 - e00 * e12; Ng[2] = e00 * e11 - e01 * e10;
-            hr.Ng = Normalize(Ng);
-            hr.Bu = U; hr.Bv = V;
-        }
-        if (idx == -1) return false;
-        hr.elem = idx; hr.U = U; hr.V = V; hr.W = W;
-        return true;
-    }
-
-    // math/vec3_amd64.s:11-42 — RSQRTSS estimate + one Newton-Raphson step.
-    static Vec3 Normalize(const Vec3& a) {
-        float x2 = a[0] * a[0], y2 = a[1] * a[1], z2 = a[2] * a[2];
-        float l2 = (y2 + x2) + z2;
-        float g = _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(l2)));
-        float h = l2 * 0.5f;
-        float gg = g * g;
-        gg = gg * h;
-        float r = (1.5f - gg) * g;
-        Vec3 o; o[0] = a[0] * r; o[1] = a[1] * r; o[2] = a[2] * r;
-        return o;
-    }
-};
-
-// ------------------------------------------------------------------ traversal
-
-// qbvh/intersect.go:91-246 with the Primitive call expressed as a callback.
-template <class Leaf>
-static bool TraceQBVH(const Node* qbvh, Ray& ray, Leaf&& traceElems) {
-    Task& tk = *ray.task;
-    int32_t stackTop = tk.StackTop;
-    tk.Stack[stackTop].Node = 0;
-    tk.Stack[stackTop].T = ray.Tclosest;
-    bool hit = false;
-    stackTop++;
-    while (stackTop > tk.StackTop) {
-        stackTop--;
-        int32_t node = tk.Stack[stackTop].Node;
-        if (ray.Tclosest < tk.Stack[stackTop].T || node == -1) continue;
-        if (node >= 0) {
-            ray.NodesT++;
-            const Node& nd = qbvh[node];
-            intersectBoxes(ray, nd.boxes, tk.Hits, tk.T);
-            int order[4];
-            if (ray.D[nd.axis0] < 0) {
-                if (ray.D[nd.axis1] < 0) { order[0] = 0; order[1] = 1; } else { order[0] = 1; order[1] = 0; }
-                if (ray.D[nd.axis2] < 0) { order[2] = 2; order[3] = 3; } else { order[2] = 3; order[3] = 2; }
-            } else {
-                if (ray.D[nd.axis2] < 0) { order[0] = 2; order[1] = 3; } else { order[0] = 3; order[1] = 2; }
-                if (ray.D[nd.axis1] < 0) { order[2] = 0; order[3] = 1; } else { order[2] = 1; order[3] = 0; }
-            }
-            for (int j = 0; j < 4; j++) {
-                int k = order[j];
-                tk.Stack[stackTop].Node = nd.children[k];
-                tk.Stack[stackTop].T = tk.T[k];
-                stackTop -= tk.Hits[k];
-            }
-            if (stackTop > tk.maxDepth) tk.maxDepth = stackTop;
-        } else {
-            ray.LeafsT++;
-            int32_t tmp = tk.StackTop;
-            tk.StackTop = stackTop + 1;
-            if (traceElems(LeafBase(node), LeafCount(node))) {
-                hit = true;
-                if (ray.Type & RayTypeShadow) { tk.StackTop = tmp; return true; }
-            }
-            tk.StackTop = tmp;
-        }
-    }
-    return hit;
-}
-
-// qbvh/motionintersect.go:22-127
-template <class Leaf>
-static bool TraceMotionQBVH(const MNode* nodes, const std::vector<std::vector<float>>& boxes, float time, int key, int key2,
-                            Ray& ray, Leaf&& traceElems) {
-    Task& tk = *ray.task;
-    int32_t stackTop = tk.StackTop;
-    tk.Stack[stackTop].Node = 0;
-    tk.Stack[stackTop].T = ray.Tclosest;
-    bool hit = false;
-    while (stackTop >= tk.StackTop) {
-        int32_t node = tk.Stack[stackTop].Node;
-        float T = tk.Stack[stackTop].T;
-        stackTop--;
-        if (ray.Tclosest < T) node = -1;
-        if (node >= 0) {
-            ray.NodesT++;
-            const MNode& pn = nodes[node];
-            const float* b0 = &boxes[key][node * 24];
-            const float* b1 = &boxes[key2][node * 24];
-            for (int i = 0; i < 24; i++) tk.Boxes[i] = (1.0f - time) * b0[i] + time * b1[i];
-            intersectBoxes(ray, tk.Boxes, tk.Hits, tk.T);
-            int order[4] = {0, 1, 2, 3};
-            if (ray.D[pn.axis0] < 0) {
-                if (ray.D[pn.axis2] < 0) { order[3] = 3; order[2] = 2; } else { order[3] = 2; order[2] = 3; }
-                if (ray.D[pn.axis1] < 0) { order[1] = 1; order[0] = 0; } else { order[1] = 0; order[0] = 1; }
-            } else {
-                if (ray.D[pn.axis2] < 0) { order[1] = 3; order[0] = 2; } else { order[1] = 2; order[0] = 3; }
-                if (ray.D[pn.axis1] < 0) { order[3] = 1; order[2] = 0; } else { order[3] = 0; order[2] = 1; }
-            }
-            for (int j = 0; j < 4; j++) {
-                int k = order[j];
-                if (tk.Hits[k] != 0) {
-                    stackTop++;
-                    tk.Stack[stackTop].Node = pn.children[k];
-                    tk.Stack[stackTop].T = tk.T[k];
-                }
-            }
-            if (stackTop + 1 > tk.maxDepth) tk.maxDepth = stackTop + 1;
-        } else if (node < -1) {
-            ray.LeafsT++;
-            int32_t tmp = tk.StackTop;
-            tk.StackTop = stackTop + 1;
-            if (traceElems(LeafBase(node), LeafCount(node))) {
-                hit = true;
-                if (ray.Type & RayTypeShadow) { tk.StackTop = tmp; return true; }
-            }
-            tk.StackTop = tmp;
-        }
-    }
-    return hit;
-}
-
-// -------------------------------------------------------------------- scene
-
-struct SceneHit {
-    int32_t geom;
-    HitRecord hr;
-};
-
-struct Scene {
-    std::vector<Mesh*> geoms;        // post-sort order after commit
-    std::vector<int32_t> geomOrder;  // post-sort slot -> insertion index
-    std::vector<Node> qbvh;
-    std::vector<MNode> mnodes;
-    std::vector<std::vector<float>> mboxes;
-    bool motion = false;
-    Box bounds;
-
-    ~Scene() { for (auto* g : geoms) delete g; }
-
-    // builtin/scene/scene.go:135-203
-    void initAccel() {
-        int n = (int)geoms.size();
-        std::vector<Box> boxes; std::vector<int32_t> indices; std::vector<Vec3> centroids;
-        int maxKeys = 0;
-        for (auto* g : geoms) if (g->MotionKeys() > maxKeys) maxKeys = g->MotionKeys();
-        if (maxKeys == 1) {
-            for (int i = 0; i < n; i++) { Box b = geoms[i]->Bounds(0); boxes.push_back(b); indices.push_back(i); centroids.push_back(b.Centroid()); }
-            bounds = BuildAccel(qbvh, boxes.data(), centroids.data(), indices.data(), n, 1);
-        } else {
-            motion = true;
-            for (int i = 0; i < n; i++) { Box b = geoms[i]->Bounds(0.5f); boxes.push_back(b); indices.push_back(i); centroids.push_back(b.Centroid()); }
-            BuildAccelMotion(mnodes, boxes.data(), centroids.data(), indices.data(), n, 1);
-        }
-        std::vector<Mesh*> ng(n);
-        for (int i = 0; i < n; i++) ng[i] = geoms[indices[i]];
-        geoms.swap(ng);
-        geomOrder = indices;
-        if (motion) {
-            // scene.go:207-268
-            mboxes.assign(maxKeys, std::vector<float>(mnodes.size() * 24, 0.0f));
-            Box full; full.Reset();
-            for (int k = 0; k < maxKeys; k++) full.GrowBox(initMotionBoxesRec(k, 0, maxKeys));
-            bounds = full;
-        }
-    }
-    Box initMotionBoxesRec(int key, int32_t node, int keys) {
-        Box nodebox; nodebox.Reset();
-        for (int c = 0; c < 4; c++) {
-            int32_t ch = mnodes[node].children[c];
-            if (ch < 0) {
-                if (ch == -1) continue;
-                Box box; box.Reset();
-                int lb = LeafBase(ch), lc = LeafCount(ch);
-                for (int i = lb; i < lb + lc; i++) {
-                    float time = (float)key / (float)(keys - 1);
-                    box.GrowBox(geoms[i]->Bounds(time));
-                }
-                SetBounds(&mboxes[key][node * 24], c, box);
-                nodebox.GrowBox(box);
-            } else {
-                Box box = initMotionBoxesRec(key, ch, keys);
-                SetBounds(&mboxes[key][node * 24], c, box);
-                nodebox.GrowBox(box);
-            }
-        }
-        return nodebox;
-    }
-
-    // builtin/geom/polymesh/trace.go:15-102 without transforms
-    bool GeomTrace(const Mesh& mesh, Ray& ray, SceneHit& sh) const {
-        HitRecord& hr = sh.hr;
-        if (mesh.nkeys == 1) {
-            return TraceQBVH(mesh.qbvh.data(), ray, [&](int base, int count) { return mesh.TraceElems(ray, hr, base, count); });
-        }
-        float k = ray.Time * (float)(mesh.nkeys - 1);
-        float time = k - Floor(k);
-        int key = (int)Floor(k), key2 = (int)Ceil(k);
-        return TraceMotionQBVH(mesh.mnodes.data(), mesh.mboxes, time, key, key2, ray,
-                               [&](int base, int count) { return mesh.TraceMotionElems(time, key, key2, ray, hr, base, count); });
-    }
-
-    // scene.go:61-97
-    bool TraceElems(Ray& ray, SceneHit& sh, int base, int count) const {
-        bool hit = false;
-        for (int i = base; i < base + count; i++) {
-            if (GeomTrace(*geoms[i], ray, sh)) {
-                sh.geom = i;
-                hit = true;
-                if (ray.Type & RayTypeShadow) return true;
-            }
-        }
-        return hit;
-    }
-
-    // scene.go:30-58
-    bool Trace(Ray& ray, SceneHit& sh) const {
-        if (!motion)
-            return TraceQBVH(qbvh.data(), ray, [&](int base, int count) { return TraceElems(ray, sh, base, count); });
-        float k = ray.Time * (float)(mboxes.size() - 1);
-        float time = k - Floor(k);
-        int key = (int)Floor(k), key2 = (int)Ceil(k);
-        return TraceMotionQBVH(mnodes.data(), mboxes, time, key, key2, ray,
-                               [&](int base, int count) { return TraceElems(ray, sh, base, count); });
-    }
-};
-
-}  // namespace orc
 
 // =================================================================== C ABI
 

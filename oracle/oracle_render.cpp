@@ -1,0 +1,1036 @@
+// ORACLE — test infrastructure only (same scope rules as oracle_accel.cpp).
+//
+// CPU restatement of the reference's per-pixel render/shade loop in its
+// reference semantics (direct lighting with MIS, no indirect bounces):
+//   core/render.go:66-137          render: RasterXY / VdC / Sobol sample generation, film update
+//   core/trace.go:40-83            Trace: ShaderContext setup, ApplyTransform, Shader.Eval
+//   core/shader.go:139-402         OffsetP, LightsPrepare, NextLight, EvaluateLightSamples
+//   core/ray.go:56-88              Ray.Init
+//   builtin/camera/camera.go:80-323  PreRender (TanThetaFocal), calcLookatMatrices, ComputeRay (pinhole)
+//   builtin/shader/std.go:77-316     ShaderStd.Eval / EvalEmission (rough specular only)
+//   builtin/shader/bsdf/orennayar.go:24-73, microfacetggx.go:26-146
+//   builtin/shader/fresnel/dielectric.go:34-47
+//   builtin/light/triangle.go:64-535 Tri light: ValidSample, SampleArea, spherical triangle sampling
+//   builtin/light/disk.go:38-51      rayPlaneIntersect
+//   builtin/geom/polymesh/trace.go:292-513  hit attributes (N, Poffset, P, DdPdu, DdPdv)
+//   colour/spectrum.go:49-112, spectrum_smits9.go:16-84, cie1931_2deg.go:62-95, space_srgb.go:8-14
+//   math/ldseq/ldseq.go:50-96, raster.go:10-57   VanDerCorput, Sobol, RasterXY
+//   math/sample/sample.go:18-27     CosineHemisphere
+//   math/ferror.go:27-67            NextFloatUp / NextFloatDown
+//   math/matrix4.go, quat.go, animdecomp.go   camera matrix decomposition/recomposition
+//
+// Float64 transcendentals go through the C library (the reference calls Go's
+// math package, both are within 1 ulp of the true value; the float32 results
+// they feed agree except on rare rounding ties).
+//
+// Not restated (absent from the workloads): perfect-mirror recursion
+// (Spec1Roughness == 0, std.go:209-261), conductor Fresnel, pixel filters,
+// thin-lens cameras, textures, disk/sphere/quad lights.
+
+#include "oracle_core.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace orc {
+
+// --------------------------------------------------------------- colour
+
+struct RGB {
+    float c[3] = {0, 0, 0};
+    void Add(const RGB& o) { for (int k = 0; k < 3; k++) c[k] += o.c[k]; }
+    void Mul(const RGB& o) { for (int k = 0; k < 3; k++) c[k] *= o.c[k]; }
+    void Scale(float s) { for (int k = 0; k < 3; k++) c[k] *= s; }
+};
+
+// CIE 1931 2-degree standard observer, 5 nm from 360 nm (95 samples); public CIE data.
+static const float cieX[95] = {
+    0.0001299f, 0.0002321f, 0.0004149f, 0.0007416f, 0.001368f, 0.002236f, 0.004243f, 0.00765f, 0.01431f, 0.02319f,
+    0.04351f, 0.07763f, 0.13438f, 0.21477f, 0.2839f, 0.3285f, 0.34828f, 0.34806f, 0.3362f, 0.3187f,
+    0.2908f, 0.2511f, 0.19536f, 0.1421f, 0.09564f, 0.05795001f, 0.03201f, 0.0147f, 0.0049f, 0.0024f,
+    0.0093f, 0.0291f, 0.06327f, 0.1096f, 0.1655f, 0.2257499f, 0.2904f, 0.3597f, 0.4334499f, 0.5120501f,
+    0.5945f, 0.6784f, 0.7621f, 0.8425f, 0.9163f, 0.9786f, 1.0263f, 1.0567f, 1.0622f, 1.0456f,
+    1.0026f, 0.9384f, 0.8544499f, 0.7514f, 0.6424f, 0.5419f, 0.4479f, 0.3608f, 0.2835f, 0.2187f,
+    0.1649f, 0.1212f, 0.0874f, 0.0636f, 0.04677f, 0.0329f, 0.0227f, 0.01584f, 0.01135916f, 0.008110916f,
+    0.005790346f, 0.004109457f, 0.002899327f, 0.00204919f, 0.001439971f, 0.0009999493f, 0.0006900786f, 0.0004760213f,
+    0.0003323011f, 0.0002348261f, 0.0001661505f, 0.000117413f, 0.00008307527f, 0.00005870652f, 0.00004150994f,
+    0.00002935326f, 0.00002067383f, 0.00001455977f, 0.00001025398f, 0.000007221456f, 0.000005085868f,
+    0.000003581652f, 0.000002522525f, 0.000001776509f, 0.000001251141f};
+static const float cieY[95] = {
+    0.000003917f, 0.000006965f, 0.00001239f, 0.00002202f, 0.000039f, 0.000064f, 0.00012f, 0.000217f, 0.000396f, 0.00064f,
+    0.00121f, 0.00218f, 0.004f, 0.0073f, 0.0116f, 0.01684f, 0.023f, 0.0298f, 0.038f, 0.048f,
+    0.06f, 0.0739f, 0.09098f, 0.1126f, 0.13902f, 0.1693f, 0.20802f, 0.2586f, 0.323f, 0.4073f,
+    0.503f, 0.6082f, 0.71f, 0.7932f, 0.862f, 0.9148501f, 0.954f, 0.9803f, 0.9949501f, 1.0f,
+    0.995f, 0.9786f, 0.952f, 0.9154f, 0.87f, 0.8163f, 0.757f, 0.6949f, 0.631f, 0.5668f,
+    0.503f, 0.4412f, 0.381f, 0.321f, 0.265f, 0.217f, 0.175f, 0.1382f, 0.107f, 0.0816f,
+    0.061f, 0.04458f, 0.032f, 0.0232f, 0.017f, 0.01192f, 0.00821f, 0.005723f, 0.004102f, 0.002929f,
+    0.002091f, 0.001484f, 0.001047f, 0.00074f, 0.00052f, 0.0003611f, 0.0002492f, 0.0001719f, 0.00012f, 0.0000848f,
+    0.00006f, 0.0000424f, 0.00003f, 0.0000212f, 0.00001499f, 0.0000106f, 0.000007465f, 0.000005257f, 0.000003702f,
+    0.000002607f, 0.000001836f, 0.000001293f, 0.000000911f, 0.000000642f, 0.000000451f};
+static const float cieZ[95] = {
+    0.0006061f, 0.001086f, 0.001946f, 0.003486f, 0.006450001f, 0.01054999f, 0.02005001f, 0.03621f, 0.06785001f, 0.1102f,
+    0.2074f, 0.3713f, 0.6456f, 1.0390501f, 1.3856f, 1.62296f, 1.74706f, 1.7826f, 1.77211f, 1.7441f,
+    1.6692f, 1.5281f, 1.28764f, 1.0419f, 0.8129501f, 0.6162f, 0.46518f, 0.3533f, 0.272f, 0.2123f,
+    0.1582f, 0.1117f, 0.07824999f, 0.05725001f, 0.04216f, 0.02984f, 0.0203f, 0.0134f, 0.008749999f, 0.005749999f,
+    0.0039f, 0.002749999f, 0.0021f, 0.0018f, 0.001650001f, 0.0014f, 0.0011f, 0.001f, 0.0008f, 0.0006f,
+    0.00034f, 0.00024f, 0.00019f, 0.0001f, 0.00005f, 0.00003f, 0.00002f, 0.00001f, 0, 0,
+    0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+// colour/cie1931_2deg.go:62-95
+static float cieLookup(const float* tab, float lambda) {
+    if (lambda < 360.0f || lambda >= 830.0f) return -1;
+    int bin = (int)(((lambda - 360.0f) / (830.0f - 360.0f)) * 95.0f);
+    return tab[bin];
+}
+
+// Smits 1999 RGB-to-spectrum basis spectra, 10 bins on [380,720) (colour/spectrum_smits9.go:34-42).
+static const float smWhite[10] = {1, 1, 0.999f, 0.9993f, 0.9992f, 0.9998f, 1, 1, 1, 1};
+static const float smCyan[10] = {0.9710f, 0.9426f, 1.0007f, 1.0007f, 1.0007f, 0.1564f, 0, 0, 0, 0};
+static const float smMagenta[10] = {1, 1, 0.9685f, 0.2229f, 0, 0.0458f, 0.8369f, 1, 1, 0.9959f};
+static const float smYellow[10] = {0.0001f, 0, 0.1088f, 0.6651f, 1, 1, 0.9996f, 0.9586f, 0.9685f, 0.9840f};
+static const float smRed[10] = {0.1012f, 0.0515f, 0, 0, 0, 0, 0.8325f, 1.0149f, 1.0149f, 1.0149f};
+static const float smGreen[10] = {0, 0, 0.0273f, 0.7937f, 1, 0.9418f, 0.1719f, 0, 0, 0.0025f};
+static const float smBlue[10] = {1, 1, 0.8916f, 0.3323f, 0, 0, 0.0003f, 0.0369f, 0.0483f, 0.0496f};
+
+static float smEval(const float* s, float lambda) {
+    if (lambda < 380.0f || lambda >= 720.0f) return 0;
+    int bin = (int)(((lambda - 380.0f) / (720.0f - 380.0f)) * 10.0f);
+    return s[bin];
+}
+
+// colour/spectrum_smits9.go:48-84
+static float Smits(float r, float g, float b, float lambda) {
+    float c = 0;
+    if (r <= g && r <= b) {
+        c += r * smEval(smWhite, lambda);
+        if (g <= b) { c += (g - r) * smEval(smCyan, lambda); c += (b - g) * smEval(smBlue, lambda); }
+        else { c += (b - r) * smEval(smCyan, lambda); c += (g - b) * smEval(smGreen, lambda); }
+    } else if (g <= r && g <= b) {
+        c += g * smEval(smWhite, lambda);
+        if (r <= b) { c += (r - g) * smEval(smMagenta, lambda); c += (b - r) * smEval(smBlue, lambda); }
+        else { c += (b - g) * smEval(smMagenta, lambda); c += (r - b) * smEval(smRed, lambda); }
+    } else {
+        c += b * smEval(smWhite, lambda);
+        if (r <= g) { c += (r - b) * smEval(smYellow, lambda); c += (g - r) * smEval(smGreen, lambda); }
+        else { c += (g - b) * smEval(smYellow, lambda); c += (r - g) * smEval(smRed, lambda); }
+    }
+    return c;
+}
+
+// colour/spectrum.go:27-112 (hero wavelength, 4 samples over [450,750))
+struct Spectrum {
+    float C[4] = {0, 0, 0, 0};
+    float Lambda = 0;
+    float Wavelength(int j) const {
+        float v = (Lambda - 450.0f + ((float)j / 4.0f) * 300.0f);
+        if (v >= 300.0f) v -= 300.0f;
+        v += 450.0f;
+        return v;
+    }
+    void FromRGB(const RGB& rgb) { for (int k = 0; k < 4; k++) C[k] = Smits(rgb.c[0], rgb.c[1], rgb.c[2], Wavelength(k)); }
+    RGB ToRGB() const {
+        float x = 0, y = 0, z = 0;
+        for (int i = 0; i < 4; i++) {
+            x += C[i] * cieLookup(cieX, Wavelength(i));
+            y += C[i] * cieLookup(cieY, Wavelength(i));
+            z += C[i] * cieLookup(cieZ, Wavelength(i));
+        }
+        // colour/space_srgb.go:8-14, colourspace.go XYZToRGB
+        RGB o;
+        o.c[0] = x * 3.2404542f + y * -1.5371385f + z * -0.4985314f;
+        o.c[1] = x * -0.9692660f + y * 1.8760108f + z * 0.0415560f;
+        o.c[2] = x * 0.0556434f + y * -0.2040259f + z * 1.0572252f;
+        return o;
+    }
+    void Mul(const Spectrum& o) { for (int k = 0; k < 4; k++) C[k] *= o.C[k]; }
+    void Scale(float s) { for (int k = 0; k < 4; k++) C[k] *= s; }
+};
+
+// ------------------------------------------------------------- QMC (ldseq)
+
+// math/ldseq/ldseq.go:55-75
+static uint64_t vdcBits(uint64_t i, uint64_t scramble) {
+    uint64_t bits = (i << 32) | (i >> 32);
+    bits = ((bits & 0x0000ffff0000ffffull) << 16) | ((bits & 0xffff0000ffff0000ull) >> 16);
+    bits = ((bits & 0x00ff00ff00ff00ffull) << 8) | ((bits & 0xff00ff00ff00ff00ull) >> 8);
+    bits = ((bits & 0x0f0f0f0f0f0f0f0full) << 4) | ((bits & 0xf0f0f0f0f0f0f0f0ull) >> 4);
+    bits = ((bits & 0x3333333333333333ull) << 2) | ((bits & 0xccccccccccccccccull) >> 2);
+    bits = ((bits & 0x5555555555555555ull) << 1) | ((bits & 0xaaaaaaaaaaaaaaaaull) >> 1);
+    return (scramble ^ bits) >> (64 - 52);
+}
+static double VanDerCorput(uint64_t i, uint64_t s) { return (double)vdcBits(i, s) / (double)(1ull << 52); }
+// ldseq.go:83-96
+static uint64_t sobolBits(uint64_t i, uint64_t scramble) {
+    uint64_t r = scramble >> (64 - 52);
+    for (uint64_t v = 1ull << 51; i != 0; i >>= 1) {
+        if (i & 1) r ^= v;
+        v ^= v >> 1;
+    }
+    return r;
+}
+static double Sobol(uint64_t i, uint64_t s) { return (double)sobolBits(i, s) / (double)(1ull << 52); }
+
+// Enumeration matrices for RasterXY (Gruenschloss, Raab, Keller 2012), derived from the two
+// generator matrices above: image of index bit k on the top m bits of (x, y), and the inverse of
+// the 2m x 2m block of the low index bits.  (math/ldseq/vdc_sobol_matrices52.go holds the same
+// tables as literals.)
+struct RasterTables {
+    uint64_t fwd[52];   // fwd[c]: image of index bit 2m+c, packed (px << m) | py
+    uint64_t inv[52];   // inv[c]: low-index bits produced by bit c of the packed (px,py)
+};
+static uint64_t indexBitImage(int m, int k) {
+    uint64_t x = vdcBits(1ull << k, 0) >> (52 - m);
+    uint64_t y = sobolBits(1ull << k, 0) >> (52 - m);
+    return (x << m) | y;
+}
+static RasterTables makeRasterTables(int m) {
+    RasterTables t;
+    std::memset(&t, 0, sizeof(t));
+    const int n = 2 * m;
+    for (int c = 0; c + n < 52; c++) t.fwd[c] = indexBitImage(m, n + c);
+    // Gauss-Jordan over GF(2): columns A[k] = image of low index bit k (n x n)
+    uint64_t A[52], I[52];
+    for (int k = 0; k < n; k++) { A[k] = indexBitImage(m, k); I[k] = 1ull << k; }
+    // solve for each unit vector e_c: find combination of columns giving e_c
+    // Work on rows: build row-major matrix R[r] = bits over k of A[k] bit r.
+    uint64_t R[52], Rinv[52];
+    for (int r = 0; r < n; r++) {
+        R[r] = 0;
+        for (int k = 0; k < n; k++) if ((A[k] >> r) & 1) R[r] |= 1ull << k;
+        Rinv[r] = 1ull << r;
+    }
+    for (int col = 0; col < n; col++) {
+        int piv = -1;
+        for (int r = col; r < n; r++) if ((R[r] >> col) & 1) { piv = r; break; }
+        if (piv < 0) continue;
+        std::swap(R[piv], R[col]); std::swap(Rinv[piv], Rinv[col]);
+        for (int r = 0; r < n; r++)
+            if (r != col && ((R[r] >> col) & 1)) { R[r] ^= R[col]; Rinv[r] ^= Rinv[col]; }
+    }
+    // R is now identity; Rinv = inverse (row r of inverse: which image bits give index bit r)
+    for (int c = 0; c < n; c++) {
+        uint64_t v = 0;
+        for (int r = 0; r < n; r++) if ((Rinv[r] >> c) & 1) v |= 1ull << r;
+        t.inv[c] = v;
+    }
+    (void)I;
+    return t;
+}
+static const RasterTables& rasterTables(int m) {
+    static RasterTables tabs[26];
+    static bool init = false;
+    if (!init) { for (int k = 1; k <= 25; k++) tabs[k] = makeRasterTables(k); init = true; }
+    return tabs[m];
+}
+// math/ldseq/raster.go:10-40
+static uint64_t rasterLookup(uint32_t m, uint32_t frame, uint32_t px, uint32_t py, uint64_t sx, uint64_t sy) {
+    const RasterTables& T = rasterTables((int)m);
+    uint32_t m2 = m << 1;
+    uint64_t index = (uint64_t)frame << m2;
+    uint64_t delta = 0;
+    for (uint32_t c = 0; frame != 0; frame >>= 1) {
+        if (frame & 1) delta ^= T.fwd[c];
+        c++;
+    }
+    px ^= (uint32_t)(sx >> (64 - m));
+    py ^= (uint32_t)(sy >> (64 - m));
+    uint64_t b = (((uint64_t)px << m) | (uint64_t)py) ^ delta;
+    for (uint32_t c = 0; b != 0; b >>= 1) {
+        if (b & 1) index ^= T.inv[c];
+        c++;
+    }
+    return index;
+}
+// raster.go:50-57
+static void RasterXY(uint32_t m, uint32_t frame, uint32_t px, uint32_t py, uint64_t sx, uint64_t sy, uint64_t& index,
+                     double& rx, double& ry) {
+    index = rasterLookup(m, frame, px, py, sx, sy);
+    rx = (double)vdcBits(index, sx) / (double)(1ull << (52 - m));
+    ry = (double)sobolBits(index, sy) / (double)(1ull << (52 - m));
+}
+
+// ----------------------------------------------------------------- vectors
+
+static inline Vec3 V3(float x, float y, float z) { Vec3 v; v[0] = x; v[1] = y; v[2] = z; return v; }
+static inline Vec3 Sub(const Vec3& a, const Vec3& b) { return V3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+static inline Vec3 Add(const Vec3& a, const Vec3& b) { return V3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+static inline Vec3 Scale(float s, const Vec3& a) { return V3(a[0] * s, a[1] * s, a[2] * s); }
+static inline Vec3 Neg(const Vec3& a) { return V3(-a[0], -a[1], -a[2]); }
+static inline float Dot(const Vec3& a, const Vec3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline float DotAbs(const Vec3& a, const Vec3& b) { return Abs(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]); }
+static inline float Length2(const Vec3& a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
+static inline Vec3 Cross(const Vec3& a, const Vec3& b) {
+    return V3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+static inline Vec3 Mad(const Vec3& a, const Vec3& b, float s) { return V3(a[0] + (b[0] * s), a[1] + (b[1] * s), a[2] + (b[2] * s)); }
+static inline Vec3 Normalize(const Vec3& a) { return Mesh::Normalize(a); }
+// math/vec3_amd64.s:45-56
+static inline float Length(const Vec3& a) {
+    float x2 = a[0] * a[0], y2 = a[1] * a[1], z2 = a[2] * a[2];
+    return std::sqrt((y2 + x2) + z2);
+}
+static inline Vec3 BasisProject(const Vec3& U, const Vec3& V, const Vec3& W, const Vec3& S) { return V3(Dot(U, S), Dot(V, S), Dot(W, S)); }
+static inline Vec3 BasisExpand(const Vec3& U, const Vec3& V, const Vec3& W, const Vec3& S) {
+    return V3(U[0] * S[0] + V[0] * S[1] + W[0] * S[2], U[1] * S[0] + V[1] * S[1] + W[1] * S[2], U[2] * S[0] + V[2] * S[1] + W[2] * S[2]);
+}
+static const float Pi32 = 3.14159265358f;
+static inline float sqr32(float x) { return x * x; }
+static inline float fSin(float x) { return (float)std::sin((double)x); }
+static inline float fCos(float x) { return (float)std::cos((double)x); }
+static inline float fTan(float x) { return (float)std::tan((double)x); }
+static inline float fAcos(float x) { return (float)std::acos((double)x); }
+static inline float fAtan(float x) { return (float)std::atan((double)x); }
+static inline float fAtan2(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+
+// math/ferror.go:27-67
+static float NextFloatUp(float v) {
+    if (std::isinf(v) && v > 0) return v;
+    if (v == 0.0f) v = 0.0f;
+    uint32_t ui = f2bits(v);
+    if (v >= 0.0f) ui++; else ui--;
+    return bits2f(ui);
+}
+static float NextFloatDown(float v) {
+    if (std::isinf(v) && v < 0) return v;
+    if (v == 0.0f) v = 0.0f;
+    uint32_t ui = f2bits(v);
+    if (v >= 0.0f) ui--; else ui++;
+    return bits2f(ui);
+}
+
+// --------------------------------------------------------------- matrices
+
+typedef float M4[16];   // column-major, Elt(i,j) = m[j*4+i] (math/matrix4.go:50-57)
+
+static void M4Identity(M4 m) { std::memset(m, 0, sizeof(M4)); m[0] = m[5] = m[10] = m[15] = 1.0f; }
+static void M4Mul(const M4 a, const M4 b, M4 c) {
+    float t[16] = {0};
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) for (int k = 0; k < 4; k++) t[j * 4 + i] += a[k * 4 + i] * b[j * 4 + k];
+    std::memcpy(c, t, sizeof(t));
+}
+static void M4Transpose(const M4 a, M4 c) { float t[16]; for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) t[i * 4 + j] = a[j * 4 + i]; std::memcpy(c, t, sizeof(t)); }
+static void M4Cof(const float* m, float* inv) {
+    inv[0] = m[5]*m[10]*m[15] - m[5]*m[11]*m[14] - m[9]*m[6]*m[15] + m[9]*m[7]*m[14] + m[13]*m[6]*m[11] - m[13]*m[7]*m[10];
+    inv[4] = -m[4]*m[10]*m[15] + m[4]*m[11]*m[14] + m[8]*m[6]*m[15] - m[8]*m[7]*m[14] - m[12]*m[6]*m[11] + m[12]*m[7]*m[10];
+    inv[8] = m[4]*m[9]*m[15] - m[4]*m[11]*m[13] - m[8]*m[5]*m[15] + m[8]*m[7]*m[13] + m[12]*m[5]*m[11] - m[12]*m[7]*m[9];
+    inv[12] = -m[4]*m[9]*m[14] + m[4]*m[10]*m[13] + m[8]*m[5]*m[14] - m[8]*m[6]*m[13] - m[12]*m[5]*m[10] + m[12]*m[6]*m[9];
+    inv[1] = -m[1]*m[10]*m[15] + m[1]*m[11]*m[14] + m[9]*m[2]*m[15] - m[9]*m[3]*m[14] - m[13]*m[2]*m[11] + m[13]*m[3]*m[10];
+    inv[5] = m[0]*m[10]*m[15] - m[0]*m[11]*m[14] - m[8]*m[2]*m[15] + m[8]*m[3]*m[14] + m[12]*m[2]*m[11] - m[12]*m[3]*m[10];
+    inv[9] = -m[0]*m[9]*m[15] + m[0]*m[11]*m[13] + m[8]*m[1]*m[15] - m[8]*m[3]*m[13] - m[12]*m[1]*m[11] + m[12]*m[3]*m[9];
+    inv[13] = m[0]*m[9]*m[14] - m[0]*m[10]*m[13] - m[8]*m[1]*m[14] + m[8]*m[2]*m[13] + m[12]*m[1]*m[10] - m[12]*m[2]*m[9];
+    inv[2] = m[1]*m[6]*m[15] - m[1]*m[7]*m[14] - m[5]*m[2]*m[15] + m[5]*m[3]*m[14] + m[13]*m[2]*m[7] - m[13]*m[3]*m[6];
+    inv[6] = -m[0]*m[6]*m[15] + m[0]*m[7]*m[14] + m[4]*m[2]*m[15] - m[4]*m[3]*m[14] - m[12]*m[2]*m[7] + m[12]*m[3]*m[6];
+    inv[10] = m[0]*m[5]*m[15] - m[0]*m[7]*m[13] - m[4]*m[1]*m[15] + m[4]*m[3]*m[13] + m[12]*m[1]*m[7] - m[12]*m[3]*m[5];
+    inv[14] = -m[0]*m[5]*m[14] + m[0]*m[6]*m[13] + m[4]*m[1]*m[14] - m[4]*m[2]*m[13] - m[12]*m[1]*m[6] + m[12]*m[2]*m[5];
+    inv[3] = -m[1]*m[6]*m[11] + m[1]*m[7]*m[10] + m[5]*m[2]*m[11] - m[5]*m[3]*m[10] - m[9]*m[2]*m[7] + m[9]*m[3]*m[6];
+    inv[7] = m[0]*m[6]*m[11] - m[0]*m[7]*m[10] - m[4]*m[2]*m[11] + m[4]*m[3]*m[10] + m[8]*m[2]*m[7] - m[8]*m[3]*m[6];
+    inv[11] = -m[0]*m[5]*m[11] + m[0]*m[7]*m[9] + m[4]*m[1]*m[11] - m[4]*m[3]*m[9] - m[8]*m[1]*m[7] + m[8]*m[3]*m[5];
+    inv[15] = m[0]*m[5]*m[10] - m[0]*m[6]*m[9] - m[4]*m[1]*m[10] + m[4]*m[2]*m[9] + m[8]*m[1]*m[6] - m[8]*m[2]*m[5];
+}
+// math/matrix4.go:116-149
+static bool M4Inverse(const M4 m, M4 c) {
+    float inv[16];
+    M4Cof(m, inv);
+    float det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    if (det == 0.0f) return false;
+    det = 1.0f / det;
+    for (int i = 0; i < 16; i++) c[i] = inv[i] * det;
+    return true;
+}
+static float M4Det(const M4 m) { float inv[16]; M4Cof(m, inv); return m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12]; }
+// math/matrix4.go:343-363
+static bool M4PolarFactor(const M4 m, M4 Q) {
+    std::memcpy(Q, m, sizeof(M4));
+    for (int it = 0; it < 10; it++) {
+        M4 Qinv, QinvT, Qnew;
+        if (!M4Inverse(Q, Qinv)) return false;
+        M4Transpose(Qinv, QinvT);
+        for (int i = 0; i < 16; i++) Qnew[i] = 0.5f * (Q[i] + QinvT[i]);
+        bool eq = true;
+        for (int i = 0; i < 16; i++) if (Qnew[i] - Q[i] > 0.000001f) { eq = false; break; }
+        if (eq) { std::memcpy(Q, Qnew, sizeof(M4)); return true; }
+        std::memcpy(Q, Qnew, sizeof(M4));
+    }
+    return false;
+}
+struct Quat { float X, Y, Z, W; };
+// math/quat.go:133-163
+static Quat M4ToQuat(const M4 m) {
+    auto E = [&](int i, int j) { return m[j * 4 + i]; };
+    Quat q;
+    float tr = E(0, 0) + E(1, 1) + E(2, 2);
+    if (tr > 0.0f) {
+        float S = std::sqrt(tr + 1.0f) * 2;
+        q.W = 0.25f * S; q.X = (E(2, 1) - E(1, 2)) / S; q.Y = (E(0, 2) - E(2, 0)) / S; q.Z = (E(1, 0) - E(0, 1)) / S;
+    } else if ((E(0, 0) > E(1, 1)) && (E(0, 0) > E(2, 2))) {
+        float S = std::sqrt(1.0f + E(0, 0) - E(1, 1) - E(2, 2)) * 2;
+        q.W = (E(2, 1) - E(1, 2)) / S; q.X = 0.25f * S; q.Y = (E(0, 1) + E(1, 0)) / S; q.Z = (E(0, 2) + E(2, 0)) / S;
+    } else if (E(1, 1) > E(2, 2)) {
+        float S = std::sqrt(1.0f + E(1, 1) - E(0, 0) - E(2, 2)) * 2;
+        q.W = (E(0, 2) - E(2, 0)) / S; q.X = (E(0, 1) + E(1, 0)) / S; q.Y = 0.25f * S; q.Z = (E(1, 2) + E(2, 1)) / S;
+    } else {
+        float S = std::sqrt(1.0f + E(2, 2) - E(0, 0) - E(1, 1)) * 2;
+        q.W = (E(1, 0) - E(0, 1)) / S; q.X = (E(0, 2) + E(2, 0)) / S; q.Y = (E(1, 2) + E(2, 1)) / S; q.Z = 0.25f * S;
+    }
+    return q;
+}
+// math/quat.go:80-104
+static void QuatToM4(const Quat& q, M4 m) {
+    std::memset(m, 0, sizeof(M4));
+    float n = 1.0f / std::sqrt(q.X * q.X + q.Y * q.Y + q.Z * q.Z + q.W * q.W);
+    float X = q.X * n, Y = q.Y * n, Z = q.Z * n, W = q.W * n;
+    auto S = [&](int i, int j, float v) { m[j * 4 + i] = v; };
+    S(0, 0, 1 - 2 * Y * Y - 2 * Z * Z); S(0, 1, 2 * X * Y - 2 * W * Z); S(0, 2, 2 * X * Z + 2 * W * Y);
+    S(1, 0, 2 * X * Y + 2 * W * Z); S(1, 1, 1 - 2 * X * X - 2 * Z * Z); S(1, 2, 2 * Y * Z - 2 * W * X);
+    S(2, 0, 2 * X * Z - 2 * W * Y); S(2, 1, 2 * Y * Z + 2 * W * X); S(2, 2, 1 - 2 * X * X - 2 * Y * Y);
+    S(3, 3, 1.0f);
+}
+// math/quat.go:30-67
+static Quat QuatSlerp(const Quat& a, const Quat& b, float t) {
+    float c = a.W * b.W + a.X * b.X + a.Y * b.Y + a.Z * b.Z;
+    if (Abs(c) >= 1.0f) return a;
+    float half = fAcos(c);
+    float s = std::sqrt(1.0f - c * c);
+    if (Abs(s) < 0.001f) return Quat{a.X * 0.5f + b.X * 0.5f, a.Y * 0.5f + b.Y * 0.5f, a.Z * 0.5f + b.Z * 0.5f, a.W * 0.5f + b.W * 0.5f};
+    float ra = fSin((1 - t) * half) / s, rb = fSin(t * half) / s;
+    return Quat{a.X * ra + b.X * rb, a.Y * ra + b.Y * rb, a.Z * ra + b.Z * rb, a.W * ra + b.W * rb};
+}
+struct Decomp { Vec3 T; Quat R; M4 S; };
+// math/animdecomp.go:20-63 (octanormal.go:16 sign)
+static Decomp DecompM4(const M4 min) {
+    Decomp d;
+    M4 m; std::memcpy(m, min, sizeof(M4));
+    float sgn = M4Det(m) >= 0.0f ? 1.0f : -1.0f;
+    for (int i = 0; i < 3; i++) d.T[i] = m[3 * 4 + i];
+    m[12] = 0; m[13] = 0; m[14] = 0;
+    if (sgn < 0.0f) for (int i = 0; i < 16; i++) m[i] = -1.0f * m[i];   // Matrix4Mul(Scale(-1, I), m)
+    M4 Q;
+    if (!M4PolarFactor(m, Q)) { d.R = Quat{0, 0, 0, 1}; M4Identity(d.S); return d; }
+    M4 QT; M4Transpose(Q, QT);
+    M4 S; M4Mul(QT, m, S);
+    d.R = M4ToQuat(Q);
+    if (sgn < 0.0f) { for (int i = 0; i < 16; i++) d.S[i] = -1.0f * S[i]; d.S[15] = 1; }
+    else std::memcpy(d.S, S, sizeof(M4));
+    return d;
+}
+// animdecomp.go:66-80 with a == b (a static camera: time t reduces to 0)
+static void DecompToM4(const Decomp& a, float t, M4 out) {
+    Decomp l;
+    l.T = Lerp(a.T, a.T, t);
+    l.R = QuatSlerp(a.R, a.R, t);
+    for (int i = 0; i < 16; i++) l.S[i] = (1.0f - t) * a.S[i] + t * a.S[i];
+    M4 Tm; M4Identity(Tm); Tm[12] = l.T[0]; Tm[13] = l.T[1]; Tm[14] = l.T[2];
+    M4 R; QuatToM4(l.R, R);
+    M4 RS; M4Mul(R, l.S, RS);
+    M4Mul(Tm, RS, out);
+}
+static Vec3 MulVec(const M4 a, const Vec3& b) {
+    return V3(a[0] * b[0] + a[4] * b[1] + a[8] * b[2], a[1] * b[0] + a[5] * b[1] + a[9] * b[2], a[2] * b[0] + a[6] * b[1] + a[10] * b[2]);
+}
+static Vec3 MulPoint(const M4 a, const Vec3& b) {
+    return V3(a[0] * b[0] + a[4] * b[1] + a[8] * b[2] + a[12], a[1] * b[0] + a[5] * b[1] + a[9] * b[2] + a[13],
+              a[2] * b[0] + a[6] * b[1] + a[10] * b[2] + a[14]);
+}
+
+// --------------------------------------------------------------- materials
+
+struct Material {              // ShaderStd parameters (builtin/shader/std.go:25-47); has_* = parameter set
+    float emission[3]; float emission_strength; int32_t has_emission_colour, has_emission_strength;
+    float diffuse[3]; int32_t has_diffuse_colour;
+    float diffuse_strength; int32_t has_diffuse_strength;
+    float diffuse_roughness; int32_t has_diffuse_roughness;
+    float spec1[3]; int32_t has_spec1_colour;
+    float spec1_strength; int32_t has_spec1_strength;
+    float spec1_roughness; int32_t has_spec1_roughness;
+    float ior; int32_t has_ior;
+    int32_t fresnel_model;     // 1 = Dielectric (the only model restated)
+};
+
+struct TriLight {
+    Vec3 P0, P1, P2;
+    int32_t material;
+    int32_t samples;
+    int32_t geom;              // post-sort geom slot of the light's mesh
+};
+
+// ----------------------------------------------------------------- shading
+
+struct SC {                    // the subset of core.ShaderContext the shading path reads
+    int32_t geom;
+    Vec3 P, Poffset, N, Ng, DdPdu, DdPdv, Rd;
+    float Lambda;
+    uint64_t I;
+    uint64_t Scramble[2];
+    int32_t shaderIdx;
+};
+
+struct OrenNayar {             // builtin/shader/bsdf/orennayar.go
+    float Lambda; Vec3 OmegaI; float Roughness; Vec3 U, V, N;
+    Vec3 Sample(double r0, double r1) const {
+        double r = std::sqrt(1 - r0), theta = 2 * M_PI * r1;   // sample.go:18-27
+        Vec3 s = V3((float)(r * std::cos(theta)), (float)(r * std::sin(theta)), (float)std::sqrt(r0));
+        return BasisExpand(U, V, N, s);
+    }
+    double PDF(const Vec3& o_) const {
+        Vec3 o = BasisProject(U, V, N, o_);
+        double odn = (double)Max(0, o[2]);
+        return odn / M_PI;
+    }
+    Spectrum Eval(const Vec3& o_) const {
+        Vec3 o = BasisProject(U, V, N, o_);
+        float sigma = Roughness;
+        float A = 1 - (0.5f * (sigma * sigma) / ((sigma * sigma) + 0.57f));
+        float B = 0.45f * (sigma * sigma) / ((sigma * sigma) + 0.09f);
+        float phiI = fAtan2(OmegaI[1], OmegaI[0]);
+        float phiO = fAtan2(o[1], o[0]);
+        float thetaI = fAcos(OmegaI[2]);
+        float thetaO = fAcos(o[2]);
+        float alpha = Max(thetaI, thetaO);
+        float beta = Min(thetaI, thetaO);
+        float C = fSin(alpha) * fTan(beta);
+        float gamma = fCos(phiO - phiI);
+        float scale = o[2] * (A + (B * Max(0, gamma) * C));
+        Spectrum rho; rho.Lambda = Lambda;
+        RGB one; one.c[0] = one.c[1] = one.c[2] = 1;
+        rho.FromRGB(one);
+        rho.Scale(scale / Pi32);
+        return rho;
+    }
+};
+
+static float chi(float x) { return x > 0.0f ? 1 : 0; }
+static float ggxG1(const Vec3& w, const Vec3& m, float alpha) {
+    float odn = Dot(w, m);
+    float denom = 1 + std::sqrt(1 + (alpha * alpha) * ((1.0f / (w[2] * w[2])) - 1));
+    return chi(odn / w[2]) * 2 / denom;
+}
+static float ggxD(const Vec3& m, float alpha) {
+    float numer = alpha * alpha * chi(m[2]);
+    if (m[2] == 1.0f) return 1.0f / (Pi32 * alpha * alpha);
+    float denom = Pi32 * sqr32(m[2] * m[2]) * sqr32(alpha * alpha + ((1.0f / (m[2] * m[2])) - 1));
+    return numer / denom;
+}
+static float signf(float v) { return v < 0 ? -1 : 1; }
+// builtin/shader/fresnel/dielectric.go:34-47
+static RGB DielectricKr(float eta, float c) {
+    RGB o;
+    float g = (eta * eta) - 1 + (c * c);
+    if (g < 0.0f) { o.c[0] = o.c[1] = o.c[2] = 1; return o; }
+    g = std::sqrt(g);
+    float fr = 0.5f * sqr32((g - c) / (g + c)) * (1 + sqr32((c * (g + c) - 1) / (c * (g - c) + 1)));
+    o.c[0] = o.c[1] = o.c[2] = fr;
+    return o;
+}
+struct GGX {                   // builtin/shader/bsdf/microfacetggx.go
+    float Lambda; Vec3 OmegaR; float Roughness; float eta; Vec3 U, V, N;
+    Vec3 Sample(double r0, double r1) const {
+        float alpha = sqr32(Roughness);
+        double thetaM = std::atan2((double)alpha * std::sqrt(r0), std::sqrt(1 - r0));
+        double phiM = 2.0 * M_PI * r1;
+        Vec3 m = V3(fSin((float)thetaM) * fCos((float)phiM), fSin((float)thetaM) * fSin((float)phiM), fCos((float)thetaM));
+        Vec3 o = Sub(Scale(2.0f * DotAbs(m, OmegaR), m), OmegaR);
+        return BasisExpand(U, V, N, Normalize(o));
+    }
+    double PDF(const Vec3& o_) const {
+        Vec3 o = BasisProject(U, V, N, o_);
+        float alpha = sqr32(Roughness);
+        Vec3 m = Scale(signf(OmegaR[2]), Normalize(Add(OmegaR, o)));
+        double pdf = (double)(ggxD(m, alpha) * m[2]);
+        if (std::isnan(pdf)) return 0;
+        return pdf;
+    }
+    Spectrum Eval(const Vec3& o_) const {
+        Vec3 wi = BasisProject(U, V, N, o_);
+        float alpha = sqr32(Roughness);
+        Vec3 h = Scale(signf(OmegaR[2]), Normalize(Add(OmegaR, wi)));
+        RGB F = DielectricKr(eta, DotAbs(OmegaR, h));
+        float numer = ggxG1(OmegaR, h, alpha) * ggxG1(wi, h, alpha) * ggxD(h, alpha);
+        float denom = 4 * Abs(OmegaR[2]) * Abs(wi[2]);
+        Spectrum rho; rho.Lambda = Lambda;
+        rho.FromRGB(F);
+        rho.Scale(Abs(wi[2]) * numer / denom);
+        for (int k = 0; k < 4; k++) if (rho.C[k] < 0 || std::isnan(rho.C[k])) rho.C[k] = 0;
+        return rho;
+    }
+};
+
+struct LightSample { float Pdf; Spectrum Liu; Vec3 Ld; float Ldist; };
+struct BSDFSample { Vec3 D; double Pdf; float PdfLight; Spectrum Liu; Vec3 Ld; float Ldist; };
+
+// builtin/light/triangle.go helpers
+static bool rayTriangleIntersect(const Vec3& Ro, const Vec3& Rd, const Vec3& P0, const Vec3& P1, const Vec3& P2, Vec3& pout) {
+    Vec3 e1 = Sub(P1, P0), e2 = Sub(P2, P0);
+    Vec3 P = Cross(Rd, e2);
+    float det = Dot(e1, P);
+    if (det > -1e-6f && det < 1e-6f) return false;
+    float inv_det = 1 / det;
+    Vec3 T = Sub(Ro, P0);
+    float u = Dot(T, P) * inv_det;
+    if (u < 0 || u > 1) return false;
+    Vec3 Q = Cross(T, e1);
+    float v = Dot(Rd, Q) * inv_det;
+    if (v < 0 || u + v > 1) return false;
+    float t = Dot(e2, Q) * inv_det;
+    if (t > 1e-6f) {
+        // Vec3Add3(Scale(1-u-v, P0), Scale(u, P1), Scale(v, P2))
+        Vec3 a = Scale(1 - u - v, P0), b = Scale(u, P1), c = Scale(v, P2);
+        pout = V3(a[0] + b[0] + c[0], a[1] + b[1] + c[1], a[2] + b[2] + c[2]);
+        return true;
+    }
+    return false;
+}
+static float triangleArea(const Vec3& P0, const Vec3& P1, const Vec3& P2) { return 0.5f * Length(Cross(Sub(P1, P0), Sub(P2, P0))); }
+static void sidesToAngles(float as, float bs, float cs, float& a, float& b, float& c) {
+    float ssu = (as + bs + cs) / 2;
+    float sa = fSin(ssu - as), sb = fSin(ssu - bs), scc = fSin(ssu - cs), s = fSin(ssu);
+    float tA = std::sqrt(sb * scc / (s * sa));
+    float tB = std::sqrt(sa * scc / (s * sb));
+    float tC = std::sqrt(sa * sb / (s * scc));
+    a = 2 * fAtan(tA); b = 2 * fAtan(tB); c = 2 * fAtan(tC);
+}
+static void verticesToSides(const Vec3& pa, const Vec3& pb, const Vec3& pc, float& a, float& b, float& c) {
+    a = fAcos(Dot(pb, pc)); b = fAcos(Dot(pc, pa)); c = fAcos(Dot(pa, pb));
+}
+static Vec3 sampleSphericalTriangle(const Vec3& p0, const Vec3& p1, const Vec3& p2, const Vec3& p, double r0, double r1, double& pdf) {
+    Vec3 pa = Normalize(Sub(p0, p)), pb = Normalize(Sub(p1, p)), pc = Normalize(Sub(p2, p));
+    float a, b, c; verticesToSides(pa, pb, pc, a, b, c);
+    float alpha, beta, gamma; sidesToAngles(a, b, c, alpha, beta, gamma);
+    float area = alpha + beta + gamma - Pi32;
+    float areaHat = (float)r0 * area;
+    float s = fSin(areaHat - alpha), t = fCos(areaHat - alpha);
+    float sinAlpha = fSin(alpha), cosAlpha = fCos(alpha);
+    float u = t - cosAlpha;
+    float v = s + sinAlpha * fCos(c);
+    float q = ((v * t - u * s) * cosAlpha - v) / ((v * s + u * t) * sinAlpha);
+    q = Max(-1, Min(q, 1));
+    float w = Dot(pc, pa);
+    Vec3 v31; for (int k = 0; k < 3; k++) v31[k] = pc[k] - w * pa[k];
+    v31 = Normalize(v31);
+    Vec3 v4; for (int k = 0; k < 3; k++) v4[k] = q * pa[k] + std::sqrt(1 - q * q) * v31[k];
+    float z = 1 - (float)r1 * (1 - Dot(v4, pb));
+    w = Dot(v4, pb);
+    Vec3 v42; for (int k = 0; k < 3; k++) v42[k] = v4[k] - w * pb[k];
+    v42 = Normalize(v42);
+    Vec3 x = Add(Scale(z, pb), Scale(std::sqrt(1 - z * z), v42));
+    pdf = 1 / (double)area;
+    return x;
+}
+static bool rayPlaneIntersect(const Vec3& Ro, const Vec3& Rd, const Vec3& P, const Vec3& N, float& t) {
+    float denom = Dot(N, Rd);
+    if (Abs(denom) > 1e-6f) { Vec3 p0l0 = Sub(P, Ro); t = Dot(p0l0, N) / denom; return t >= 0; }
+    t = 0;
+    return false;
+}
+
+struct Renderer {
+    Scene* scene;
+    int width, height;
+    M4 M;
+    float tanThetaFocal, aspect, focal;
+    std::vector<Material> mats;
+    std::vector<TriLight> lights;
+    std::vector<std::vector<int32_t>> geomShaders;   // per post-sort geom slot: material id per shader index
+    int shadowRays = 0;
+
+    // EvalEmission (std.go:299-316)
+    RGB Emission(const Material& m) const {
+        RGB e;
+        if (m.has_emission_colour) for (int k = 0; k < 3; k++) e.c[k] = m.emission[k];
+        if (!m.has_emission_strength) return RGB();
+        e.Scale(m.emission_strength);
+        return e;
+    }
+
+    bool TraceShadow(Task* tk, const Vec3& P, const Vec3& D, float tmax) const {
+        Ray r{};
+        r.P = P; r.D = D; r.Tclosest = tmax; r.Type = RayTypeShadow; r.Time = 0;
+        RaySetup(r);
+        tk->StackTop = 0; r.task = tk;
+        SceneHit sh{}; sh.geom = -1;
+        return scene->Trace(r, sh);
+    }
+
+    // core/shader.go:139-163
+    static Vec3 OffsetP(const SC& sc, int dir) {
+        Vec3 pofs = dir < 0 ? Neg(sc.Poffset) : sc.Poffset;
+        Vec3 po = Add(sc.P, pofs);
+        for (int i = 0; i < 3; i++) {
+            if (pofs[i] > 0) po[i] = NextFloatUp(po[i]);
+            else if (pofs[i] < 0) po[i] = NextFloatDown(po[i]);
+        }
+        return po;
+    }
+
+    // Tri.ValidSample (triangle.go:136-230)
+    bool ValidSample(const TriLight& L, const SC& sg, BSDFSample& s) const {
+        Vec3 P;
+        if (!rayTriangleIntersect(sg.P, s.D, L.P0, L.P1, L.P2, P)) return false;
+        Vec3 N = Normalize(Cross(Sub(L.P1, L.P0), Sub(L.P2, L.P0)));
+        const Material& lm = mats[L.material];
+        if (Dot(sg.Ng, Sub(L.P0, P)) < 0 || Dot(sg.Ng, Sub(L.P1, P)) < 0 || Dot(sg.Ng, Sub(L.P2, P)) < 0) {
+            double pdf = (double)(1 / triangleArea(L.P0, L.P1, L.P2));
+            Vec3 D = Sub(P, sg.P);
+            s.Ldist = Length(D); s.Ld = Normalize(D);
+            if (Dot(s.Ld, N) > 0 || Dot(s.Ld, sg.Ng) < 0) return false;
+            s.Liu.Lambda = sg.Lambda; s.Liu.FromRGB(Emission(lm));
+            s.PdfLight = (float)pdf * sqr32(s.Ldist) / DotAbs(s.Ld, N);
+        } else {
+            Vec3 pa = Normalize(Sub(L.P0, sg.P)), pb = Normalize(Sub(L.P1, sg.P)), pc = Normalize(Sub(L.P2, sg.P));
+            float a, b, c; verticesToSides(pa, pb, pc, a, b, c);
+            float al, be, ga; sidesToAngles(a, b, c, al, be, ga);
+            float area = al + be + ga - Pi32;
+            double pdf = (double)(1 / area);
+            Vec3 D = Sub(P, sg.P);
+            s.Ldist = Length(D); s.Ld = Normalize(D);
+            if (Dot(s.Ld, N) > 0 || Dot(s.Ld, sg.Ng) < 0) return false;
+            s.Liu.Lambda = sg.Lambda; s.Liu.FromRGB(Emission(lm));
+            s.PdfLight = (float)pdf;
+        }
+        return true;
+    }
+
+    // Tri.SampleArea (triangle.go:232-343)
+    void SampleArea(const TriLight& L, const SC& sg, int n, std::vector<LightSample>& out) const {
+        const Material& lm = mats[L.material];
+        Vec3 N = Normalize(Cross(Sub(L.P1, L.P0), Sub(L.P2, L.P0)));
+        if (Dot(sg.Ng, Sub(L.P0, sg.P)) < 0 || Dot(sg.Ng, Sub(L.P1, sg.P)) < 0 || Dot(sg.Ng, Sub(L.P2, sg.P)) < 0) {
+            double pdf = (double)(1 / triangleArea(L.P0, L.P1, L.P2));
+            for (int i = 0; i < n; i++) {
+                uint64_t idx = sg.I * (uint64_t)n + (uint64_t)i;
+                double r0 = VanDerCorput(idx, sg.Scramble[0]), r1 = Sobol(idx, sg.Scramble[1]);
+                Vec3 a = Scale((float)(r1 * std::sqrt(1 - r0)), Sub(L.P1, L.P0));
+                Vec3 b = Scale((float)(1 - std::sqrt(1 - r0)), Sub(L.P2, L.P0));
+                Vec3 P = V3(L.P0[0] + a[0] + b[0], L.P0[1] + a[1] + b[1], L.P0[2] + a[2] + b[2]);
+                Vec3 D = Sub(P, sg.P);
+                LightSample ls;
+                ls.Ldist = Length(D); ls.Ld = Normalize(D);
+                if (Dot(ls.Ld, N) > 0 || Dot(ls.Ld, sg.Ng) < 0) continue;
+                ls.Liu.Lambda = sg.Lambda; ls.Liu.FromRGB(Emission(lm));
+                ls.Pdf = (float)pdf * sqr32(ls.Ldist) / DotAbs(ls.Ld, N);
+                out.push_back(ls);
+            }
+            return;
+        }
+        for (int i = 0; i < n; i++) {
+            uint64_t idx = sg.I * (uint64_t)n + (uint64_t)i;
+            double r0 = VanDerCorput(idx, sg.Scramble[0]), r1 = Sobol(idx, sg.Scramble[1]);
+            double pdf;
+            Vec3 x = sampleSphericalTriangle(L.P0, L.P1, L.P2, sg.P, r0, r1, pdf);
+            float t; rayPlaneIntersect(sg.P, x, L.P0, N, t);
+            Vec3 P = Mad(sg.P, x, t);
+            Vec3 D = Sub(P, sg.P);
+            LightSample ls;
+            ls.Ldist = Length(D); ls.Ld = Normalize(D);
+            if (Dot(ls.Ld, N) > 0 || Dot(ls.Ld, sg.Ng) < 0) continue;
+            ls.Liu.Lambda = sg.Lambda; ls.Liu.FromRGB(Emission(lm));
+            ls.Pdf = (float)pdf;
+            out.push_back(ls);
+        }
+    }
+
+    // core/shader.go:203-402
+    template <class B>
+    RGB EvaluateLightSamples(Task* tk, const SC& sc, const TriLight& L, int NSamples, const B& bsdf) {
+        RGB col;
+        const float k = 1.0f - 0.0001f;   // 1 - ShadowRayEpsilon (core/ray.go:22)
+        if (NSamples > 1) {
+            int half = NSamples / 2;
+            std::vector<BSDFSample> bs;
+            for (int i = 0; i < half; i++) {
+                uint64_t idx = sc.I * (uint64_t)half + (uint64_t)i;
+                double r0 = VanDerCorput(idx, sc.Scramble[0]), r1 = Sobol(idx, sc.Scramble[1]);
+                Vec3 o = Normalize(bsdf.Sample(r0, r1));
+                BSDFSample s{}; s.D = o; s.Pdf = bsdf.PDF(o);
+                if (s.Pdf <= 0) continue;
+                if (ValidSample(L, sc, s)) bs.push_back(s);
+            }
+            std::vector<LightSample> ls;
+            SampleArea(L, sc, half, ls);
+            int nB = half, nL = half;
+            if (ls.empty()) nL = 0;
+            if (bs.empty()) nB = 0;
+            int total = nB + nL;
+            if (total == 0) return RGB();
+            for (const auto& l : ls) {
+                if (Dot(l.Ld, sc.N) <= 0) continue;
+                Vec3 o = Dot(l.Ld, sc.Ng) < 0 ? OffsetP(sc, -1) : OffsetP(sc, 1);
+                shadowRays++;
+                if (!TraceShadow(tk, o, Scale(l.Ldist * k, l.Ld), 1.0f)) {
+                    Spectrum rho = bsdf.Eval(l.Ld);
+                    rho.Mul(l.Liu);
+                    float p_hat = (float)nB * (float)bsdf.PDF(l.Ld) / (float)total;
+                    p_hat += (float)nL * l.Pdf / (float)total;
+                    rho.Scale(1.0f / p_hat);
+                    RGB rgb = rho.ToRGB();
+                    for (int c = 0; c < 3; c++) if (rgb.c[c] < 0) rgb.c[c] = 0;
+                    col.Add(rgb);
+                }
+            }
+            for (const auto& b : bs) {
+                if (Dot(b.Ld, sc.N) <= 0) continue;
+                Vec3 o = Dot(b.Ld, sc.Ng) < 0 ? OffsetP(sc, -1) : OffsetP(sc, 1);
+                shadowRays++;
+                if (!TraceShadow(tk, o, Scale(b.Ldist * k, b.Ld), 1.0f)) {
+                    Spectrum rho = bsdf.Eval(b.Ld);
+                    rho.Mul(b.Liu);
+                    float p_hat = (float)nB * (float)b.Pdf / (float)total;
+                    p_hat += (float)nL * b.PdfLight / (float)total;
+                    rho.Scale(1.0f / p_hat);
+                    RGB rgb = rho.ToRGB();
+                    for (int c = 0; c < 3; c++) if (rgb.c[c] < 0) rgb.c[c] = 0;
+                    col.Add(rgb);
+                }
+            }
+            col.Scale(1.0f / (float)total);
+        } else {
+            std::vector<LightSample> ls;
+            SampleArea(L, sc, NSamples, ls);
+            for (const auto& l : ls) {
+                if (Dot(l.Ld, sc.N) <= 0) continue;
+                Vec3 o = Dot(l.Ld, sc.Ng) < 0 ? OffsetP(sc, -1) : OffsetP(sc, 1);
+                shadowRays++;
+                if (!TraceShadow(tk, o, Scale(l.Ldist * k, l.Ld), 1.0f)) {
+                    Spectrum rho = bsdf.Eval(l.Ld);
+                    rho.Mul(l.Liu);
+                    rho.Scale(1.0f / l.Pdf);
+                    col.Add(rho.ToRGB());
+                }
+            }
+        }
+        return col;
+    }
+
+    // ShaderStd.Eval (std.go:77-296), Level 0, roughness > 0
+    RGB Shade(Task* tk, const SC& sg, const Material& sh) {
+        Vec3 V = Cross(sg.N, sg.DdPdu);
+        if (Length2(V) < 0.1f) V = Cross(sg.N, sg.DdPdv);
+        V = Normalize(V);
+        Vec3 U = Normalize(Cross(sg.N, V));
+        float diffRoughness = sh.has_diffuse_roughness ? sh.diffuse_roughness : 0.5f;
+        OrenNayar diff{sg.Lambda, BasisProject(U, V, sg.N, Neg(sg.Rd)), diffRoughness * diffRoughness, U, V, sg.N};
+        RGB diffContrib, diffColour;
+        if (sh.has_diffuse_colour) for (int c = 0; c < 3; c++) diffColour.c[c] = sh.diffuse[c];
+        float dw = sh.has_diffuse_strength ? sh.diffuse_strength : 0.0f;
+        float sw = sh.has_spec1_strength ? sh.spec1_strength : 0.0f;
+        float total = dw + sw;
+        dw /= total; sw /= total;
+        // the lights a shading point sees: all but the one owning the hit geom (scene.go:100-116)
+        if (dw > 0.0f) {
+            for (const auto& L : lights) {
+                if (L.geom == sg.geom) continue;
+                int NS = 1 << L.samples;
+                RGB c = EvaluateLightSamples(tk, sg, L, NS, diff);
+                c.Mul(diffColour);
+                diffContrib.Add(c);
+            }
+            diffContrib.Scale(dw);
+        }
+        float ior = sh.has_ior ? sh.ior : 1.7f;
+        RGB specContrib;
+        if (sw > 0.0f) {
+            float rough = sh.has_spec1_roughness ? sh.spec1_roughness : 0.5f;
+            RGB specColour;
+            if (sh.has_spec1_colour) for (int c = 0; c < 3; c++) specColour.c[c] = sh.spec1[c];
+            GGX spec{sg.Lambda, BasisProject(U, V, sg.N, Neg(sg.Rd)), rough * rough, ior, U, V, sg.N};
+            if (rough > 0.0f) {
+                for (const auto& L : lights) {
+                    if (L.geom == sg.geom) continue;
+                    int NS = 1 << L.samples;
+                    RGB c = EvaluateLightSamples(tk, sg, L, NS, spec);
+                    c.Mul(specColour);
+                    specContrib.Add(c);
+                }
+            }
+        }
+        RGB contrib;
+        contrib.Add(Emission(sh));
+        contrib.Add(diffContrib);
+        contrib.Add(specContrib);
+        return contrib;
+    }
+
+    // core/render.go:75-130 for one pixel; returns the sample colour (and whether it hit)
+    RGB Sample(Task* tk, uint32_t iter, uint32_t x, uint32_t y, const uint64_t* scr, bool& hit, int32_t& hitGeom) {
+        uint64_t index; double rx, ry;
+        RasterXY(12, iter, x, y, 0, 0, index, rx, ry);
+        double lambda = (720 - 450) * VanDerCorput(iter, scr[3]) + 450;
+        float Sx = (float)(-1.0 + 2.0 * (rx / (double)width));
+        float Sy = -(float)(-1.0 + 2.0 * (ry / (double)height));
+        // camera.ComputeRay (pinhole), camera.go:221-319
+        float camu = Sx * tanThetaFocal;
+        float camv = Sy * (tanThetaFocal / aspect);
+        Vec3 s = Sub(Add(Scale(camu, V3(1, 0, 0)), Scale(camv, V3(0, 1, 0))), Scale(focal, V3(0, 0, 1)));
+        Vec3 d = MulVec(M, s);
+        Ray r{};
+        r.D = Normalize(d);
+        r.P = MulPoint(M, V3(0, 0, 0));
+        r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = 0;
+        RaySetup(r);
+        tk->StackTop = 0; r.task = tk;
+        SceneHit sh{}; sh.geom = -1; sh.hr.elem = -1;
+        RGB out;
+        hit = scene->Trace(r, sh);
+        hitGeom = hit ? sh.geom : -1;
+        if (!hit) return out;
+        const HitRecord& h = sh.hr;
+        SC sc;
+        sc.geom = sh.geom; sc.Lambda = (float)lambda; sc.I = iter; sc.Scramble[0] = scr[4]; sc.Scramble[1] = scr[5];
+        sc.Rd = r.D;
+        // ApplyTransform with identity matrices (core/shader.go:129-135)
+        sc.P = h.Po;
+        sc.N = Normalize(h.N); sc.Ng = Normalize(h.Ng);
+        sc.DdPdu = Normalize(h.DdPdu); sc.DdPdv = Normalize(h.DdPdv);
+        sc.Poffset = h.Poffset;
+        const auto& shaders = geomShaders[sh.geom];
+        int mat = shaders.empty() ? -1 : shaders[h.shaderIdx];
+        if (mat < 0) { hit = false; return out; }
+        return Shade(tk, sc, mats[mat]);
+    }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// Camera matrix as the reference computes it for a static LookAt camera: calcLookatMatrices
+// (camera.go:149-192), TransformDecompMatrix4 and TransformDecompToMatrix4 of the lerp at t=0
+// (camera.go:225-236).  Also returns TanThetaFocal (camera.go:89).
+void orc_camera_lookat(const float* from, const float* to, const float* up, float fov_deg, float focal, float* M_out,
+                       float* tan_theta_focal) {
+    Vec3 F = V3(from[0], from[1], from[2]), T = V3(to[0], to[1], to[2]), Up = V3(up[0], up[1], up[2]);
+    Vec3 W = Normalize(Sub(F, T));
+    Vec3 u = Normalize(Cross(Up, W));
+    Vec3 v = Normalize(Cross(u, W));
+    float roll = 0;
+    Vec3 U = Add(Scale(fCos(roll), u), Scale(fSin(roll), v));
+    Vec3 V = Add(Scale(-fSin(roll), u), Scale(fCos(roll), v));
+    M4 Tr; M4Identity(Tr); Tr[12] = F[0]; Tr[13] = F[1]; Tr[14] = F[2];
+    M4 B; std::memset(B, 0, sizeof(M4));
+    B[0] = U[0]; B[1] = U[1]; B[2] = U[2]; B[4] = V[0]; B[5] = V[1]; B[6] = V[2]; B[8] = W[0]; B[9] = W[1]; B[10] = W[2]; B[15] = 1;
+    M4 mtx; M4Mul(Tr, B, mtx);
+    Decomp d = DecompM4(mtx);
+    DecompToM4(d, 0.0f, M_out);
+    *tan_theta_focal = fTan((fov_deg / 2) * Pi32 / 180.0f) * focal;
+}
+
+// VdC / Sobol / RasterXY for known-answer tests
+double orc_vdc(uint64_t i, uint64_t s) { return VanDerCorput(i, s); }
+double orc_sobol(uint64_t i, uint64_t s) { return Sobol(i, s); }
+void orc_raster_xy(uint32_t m, uint32_t frame, uint32_t px, uint32_t py, uint64_t* index, double* rx, double* ry) {
+    RasterXY(m, frame, px, py, 0, 0, *index, *rx, *ry);
+}
+// enumeration matrices for m in [1,25]: fwd (52) and inv (52)
+void orc_raster_tables(int32_t m, uint64_t* fwd, uint64_t* inv) {
+    const RasterTables& t = rasterTables(m);
+    std::memcpy(fwd, t.fwd, sizeof(t.fwd));
+    std::memcpy(inv, t.inv, sizeof(t.inv));
+}
+// spectrum helpers for known-answer tests
+float orc_smits(float r, float g, float b, float lambda) { return Smits(r, g, b, lambda); }
+void orc_spectrum_to_rgb(const float* C4, float lambda, float* rgb) {
+    Spectrum s; for (int k = 0; k < 4; k++) s.C[k] = C4[k]; s.Lambda = lambda;
+    RGB o = s.ToRGB(); for (int k = 0; k < 3; k++) rgb[k] = o.c[k];
+}
+void orc_cie_tables(float* x, float* y, float* z) {
+    std::memcpy(x, cieX, sizeof(cieX)); std::memcpy(y, cieY, sizeof(cieY)); std::memcpy(z, cieZ, sizeof(cieZ));
+}
+
+struct orc_material {
+    float emission[3]; float emission_strength; int32_t has_emission_colour, has_emission_strength;
+    float diffuse[3]; int32_t has_diffuse_colour;
+    float diffuse_strength; int32_t has_diffuse_strength;
+    float diffuse_roughness; int32_t has_diffuse_roughness;
+    float spec1[3]; int32_t has_spec1_colour;
+    float spec1_strength; int32_t has_spec1_strength;
+    float spec1_roughness; int32_t has_spec1_roughness;
+    float ior; int32_t has_ior;
+    int32_t fresnel_model;
+};
+static_assert(sizeof(orc_material) == sizeof(Material), "material layout");
+
+// Adds a mesh with shading data: normals (nverts*3, nullable), shader index per triangle (nullable),
+// shader table (material ids, nshaders).  Returns the insertion index.
+int orc_scene_add_shaded_mesh(void* sp, const float* verts, int32_t nverts, const uint32_t* idx, int32_t ntris,
+                              const float* normals, const uint8_t* shaderidx, const int32_t* shaders, int32_t nshaders) {
+    Scene* s = (Scene*)sp;
+    Mesh* m = new Mesh();
+    m->nkeys = 1; m->elemsPerKey = nverts;
+    m->verts.resize(nverts);
+    std::memcpy(m->verts.data(), verts, sizeof(float) * 3 * (size_t)nverts);
+    m->idxp.assign(idx, idx + (size_t)ntris * 3);
+    m->facecount = ntris;
+    if (normals) {
+        m->normals.resize(nverts);
+        std::memcpy(m->normals.data(), normals, sizeof(float) * 3 * (size_t)nverts);
+        m->normalidx.assign(idx, idx + (size_t)ntris * 3);
+    }
+    if (shaderidx) m->shaderidx.assign(shaderidx, shaderidx + ntris);
+    m->shaders.assign(shaders, shaders + nshaders);
+    s->geoms.push_back(m);
+    return (int)s->geoms.size() - 1;
+}
+
+void* orc_render_new(void* sp, int32_t w, int32_t h, const float* M, float tan_theta_focal, float aspect, float focal,
+                     const orc_material* mats, int32_t nmats) {
+    Renderer* r = new Renderer();
+    r->scene = (Scene*)sp;
+    r->width = w; r->height = h;
+    std::memcpy(r->M, M, sizeof(M4));
+    r->tanThetaFocal = tan_theta_focal; r->aspect = aspect; r->focal = focal;
+    r->mats.resize(nmats);
+    std::memcpy(r->mats.data(), mats, sizeof(Material) * nmats);
+    for (auto* g : r->scene->geoms) r->geomShaders.push_back(g->shaders);
+    return r;
+}
+void orc_render_free(void* r) { delete (Renderer*)r; }
+
+// light_geom_insertion: the insertion index of the light's mesh (mapped to its post-sort slot here)
+void orc_render_add_trilight(void* rp, const float* P0, const float* P1, const float* P2, int32_t material, int32_t samples,
+                             int32_t light_geom_insertion) {
+    Renderer* r = (Renderer*)rp;
+    TriLight L;
+    L.P0 = V3(P0[0], P0[1], P0[2]); L.P1 = V3(P1[0], P1[1], P1[2]); L.P2 = V3(P2[0], P2[1], P2[2]);
+    L.material = material; L.samples = samples;
+    L.geom = -1;
+    for (size_t i = 0; i < r->scene->geomOrder.size(); i++) if (r->scene->geomOrder[i] == light_geom_insertion) L.geom = (int32_t)i;
+    r->lights.push_back(L);
+}
+
+// One pass (core/render.go render() for every pixel).  scrambles: 6 uint64 per pixel (lensU, lensV,
+// time, lambda, scramble0, scramble1 as core/render.go:166-176).  iter: the 1-based pass number the
+// reference passes to render().  film: w*h*3 floats updated as fb = (fb*iter + c)/(iter+1).
+// sample_out (nullable): w*h*3 floats receiving this pass's sample colours.  Returns shadow rays traced.
+int64_t orc_render_pass(void* rp, const uint64_t* scrambles, uint32_t iter, float* film, float* sample_out, int32_t nthreads) {
+    Renderer* r = (Renderer*)rp;
+    const int w = r->width, h = r->height;
+    int64_t shadow = 0;
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt) reduction(+ : shadow)
+#endif
+    {
+        Task* tk = new Task();
+        Renderer local = *r;
+        local.shadowRays = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int p = 0; p < w * h; p++) {
+            int x = p % w, y = p / w;
+            bool hit; int32_t g;
+            RGB c = local.Sample(tk, iter, (uint32_t)x, (uint32_t)y, scrambles + (size_t)p * 6, hit, g);
+            for (int k = 0; k < 3; k++) film[p * 3 + k] = (film[p * 3 + k] * (float)iter + c.c[k]) / (float)(iter + 1);
+            if (sample_out) for (int k = 0; k < 3; k++) sample_out[p * 3 + k] = c.c[k];
+        }
+        shadow += local.shadowRays;
+        delete tk;
+    }
+    (void)nthreads;
+    return shadow;
+}
+
+}  // extern "C"

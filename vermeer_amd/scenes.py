@@ -243,3 +243,144 @@ def moving_spheres(count=2000, nseg=25, nring=6, extent=20.0, rmin=0.3, rmax=1.0
     k1 = (sv[None] * radii[:, None, None] + (centres + move)[:, None, :]).reshape(-1, 3)
     idx = (sf[None].astype(np.int64) + (np.arange(count) * len(sv))[:, None, None]).reshape(-1, 3).astype(np.uint32)
     return np.stack([k0, k1]).astype(np.float32), idx
+
+
+# ------------------------------------------------------------ shaded scenes
+
+# ShaderStd parameters (builtin/shader/std.go:25-47); has_* flags mark parameters that are set.
+MATERIAL_DTYPE = np.dtype([
+    ("emission", "<f4", 3), ("emission_strength", "<f4"), ("has_emission_colour", "<i4"), ("has_emission_strength", "<i4"),
+    ("diffuse", "<f4", 3), ("has_diffuse_colour", "<i4"),
+    ("diffuse_strength", "<f4"), ("has_diffuse_strength", "<i4"),
+    ("diffuse_roughness", "<f4"), ("has_diffuse_roughness", "<i4"),
+    ("spec1", "<f4", 3), ("has_spec1_colour", "<i4"),
+    ("spec1_strength", "<f4"), ("has_spec1_strength", "<i4"),
+    ("spec1_roughness", "<f4"), ("has_spec1_roughness", "<i4"),
+    ("ior", "<f4"), ("has_ior", "<i4"),
+    ("fresnel_model", "<i4"),
+])
+FRESNEL_DIELECTRIC = 1
+
+
+def material(diffuse=None, diffuse_strength=None, diffuse_roughness=None, spec=None, spec_strength=None,
+             spec_roughness=None, ior=None, emission=None, emission_strength=None):
+    m = np.zeros((), MATERIAL_DTYPE)
+    for key, val, flag in (("diffuse", diffuse, "has_diffuse_colour"), ("spec1", spec, "has_spec1_colour"),
+                           ("emission", emission, "has_emission_colour")):
+        if val is not None:
+            m[key] = val
+            m[flag] = 1
+    for key, val in (("diffuse_strength", diffuse_strength), ("diffuse_roughness", diffuse_roughness),
+                     ("spec1_strength", spec_strength), ("spec1_roughness", spec_roughness), ("ior", ior),
+                     ("emission_strength", emission_strength)):
+        if val is not None:
+            m[key] = val
+            m["has_" + key] = 1
+    m["fresnel_model"] = FRESNEL_DIELECTRIC
+    return m
+
+
+def value_noise(p, seed, freq=4.0):
+    """Seeded 3-D value noise in [0,1): random values on an integer lattice (hashed from the lattice
+    coordinates and the seed), trilinear interpolation with a smoothstep fade."""
+    q = np.asarray(p, np.float64) * freq
+    i0 = np.floor(q).astype(np.int64)
+    f = q - i0
+    f = f * f * (3.0 - 2.0 * f)
+
+    def lattice(ix, iy, iz):
+        h = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+        h = (h ^ (h >> 13)) * 1274126177
+        h = h ^ (h >> 16)
+        return (h & 0xFFFFFF).astype(np.float64) / float(1 << 24)
+
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * (f[:, 2] if dz else 1 - f[:, 2])
+                out = out + w * lattice(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz)
+    return out
+
+
+def quad(p0, p1, p2, p3):
+    return np.array([p0, p1, p2, p3], np.float64), np.array([[0, 1, 2], [0, 2, 3]], np.int64)
+
+
+def box_spheres_scene(seed=3, nseg=64, nring=32, nspheres=8, amplitude=0.05):
+    """Closed box (floor, ceiling, back, left, right walls; the ceiling as four quads around a 0.5 x 0.5
+    opening) with `nspheres` UV spheres
+    (nseg x nring segments) displaced radially by seeded value noise of the given amplitude, and a
+    0.5 x 0.5 light in the ceiling opening made of two triangle lights (radiance 10).  Sphere materials: Lambertian
+    albedo U[0.2,0.8] per channel plus a GGX lobe of roughness U[0.05,0.5] (dielectric, IOR 1.5).
+
+    Returns a dict: meshes (verts, idx, shaderidx, shaders, normals), materials, lights, camera.
+    Geom insertion order: the flattened box+spheres mesh, then one mesh per light triangle."""
+    g = rng(seed)
+    mats = [material(diffuse=(0.7, 0.7, 0.7), diffuse_strength=1.0, diffuse_roughness=0.0)]
+    verts, faces, sidx = [], [], []
+    nv = 0
+    h = 0.25   # half size of the light and of the ceiling opening it sits in
+    walls = [quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)),        # floor
+             # ceiling with a 0.5 x 0.5 opening for the light (four quads around it)
+             quad((-1, 2, -1), (-h, 2, -1), (-h, 2, 1), (-1, 2, 1)),
+             quad((h, 2, -1), (1, 2, -1), (1, 2, 1), (h, 2, 1)),
+             quad((-h, 2, -1), (h, 2, -1), (h, 2, -h), (-h, 2, -h)),
+             quad((-h, 2, h), (h, 2, h), (h, 2, 1), (-h, 2, 1)),
+             quad((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)),      # back wall
+             quad((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)),      # left wall
+             quad((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1))]          # right wall
+    for v, f in walls:
+        # the reference shades one side only (core/shader.go:265 skips samples below N): wind inwards
+        n = np.cross(v[f[0, 1]] - v[f[0, 0]], v[f[0, 2]] - v[f[0, 0]])
+        if np.dot(n, np.array([0.0, 1.0, 0.0]) - v[0]) < 0:
+            f = f[:, ::-1]
+        verts.append(v)
+        faces.append(f + nv)
+        sidx.append(np.zeros(len(f), np.int64))
+        nv += len(v)
+    sv, sf = uv_sphere(1.0, nseg, nring)
+    sv = sv.astype(np.float64)
+    for s in range(nspheres):
+        r = 0.22
+        c = np.array([-0.6 + 0.4 * (s % 4), r + 0.05 + 0.9 * (s // 4), -0.3 + 0.3 * (s % 2)])
+        disp = r + amplitude * (2.0 * value_noise(sv, seed * 100 + s) - 1.0)
+        v = sv * disp[:, None] + c
+        mats.append(material(diffuse=tuple(g.uniform(0.2, 0.8, 3)), diffuse_strength=0.6, diffuse_roughness=0.0,
+                             spec=(1.0, 1.0, 1.0), spec_strength=0.4, spec_roughness=float(g.uniform(0.05, 0.5)),
+                             ior=1.5))
+        verts.append(v)
+        faces.append(sf.astype(np.int64) + nv)
+        sidx.append(np.full(len(sf), 1 + s, np.int64))
+        nv += len(v)
+    light_mat = len(mats)
+    mats.append(material(emission=(1.0, 1.0, 1.0), emission_strength=10.0, diffuse=(0.0, 0.0, 0.0),
+                         diffuse_strength=1.0))
+    main = {"verts": np.concatenate(verts).astype(np.float32), "idx": np.concatenate(faces).astype(np.uint32),
+            "shaderidx": np.concatenate(sidx).astype(np.uint8), "shaders": np.arange(1 + nspheres, dtype=np.int32),
+            "normals": None}
+    # Light quad recessed above the ceiling opening, facing down (cross(P1-P0, P2-P0) = -y).  Every
+    # shading point then lies at least 1 mm below the light's plane: the reference's spherical-triangle
+    # sampler takes sqrt of a negative value (NaN pdf) for points within ~1e-4 of that plane
+    # (builtin/light/triangle.go:376-411).  The second triangle sits 1 mm higher because two halves of
+    # one square share a bounding box, and the scene's leafMax-1 BuildAccel never terminates on
+    # coincident centroids (qbvh/build.go:35-43).
+    y, y2 = 2.001, 2.002
+    L = [((-h, y, -h), (h, y, -h), (h, y, h)), ((-h, y2, -h), (h, y2, h), (-h, y2, h))]
+    meshes = [main]
+    lights = []
+    for k, (p0, p1, p2) in enumerate(L):
+        P = np.array([p0, p1, p2], np.float32)
+        n = np.cross(P[1] - P[0], P[2] - P[0])
+        n = (n / np.linalg.norm(n)).astype(np.float32)
+        meshes.append({"verts": P, "idx": np.array([[0, 1, 2]], np.uint32), "shaderidx": None,
+                       "shaders": np.array([light_mat], np.int32), "normals": np.tile(n, (3, 1))})
+        lights.append({"P0": P[0], "P1": P[1], "P2": P[2], "material": light_mat, "samples": 1, "geom": 1 + k})
+    # up = -y: the reference's LookAt basis takes v = u x W (camera.go:164), which points down for up = +y
+    camera = {"from": (0.0, 1.0, 3.4), "to": (0.0, 1.0, 0.0), "up": (0.0, -1.0, 0.0), "fov": 45.0, "focal": 12.0}
+    return {"meshes": meshes, "materials": np.array(mats, MATERIAL_DTYPE), "lights": lights, "camera": camera}
+
+
+def pixel_scrambles(width, height, seed=99):
+    """6 uint64 per pixel: lensU, lensV, time, lambda, scramble[0], scramble[1] (core/render.go:166-176)."""
+    return rng(seed).integers(0, np.iinfo(np.uint64).max, (height, width, 6), dtype=np.uint64, endpoint=True)
