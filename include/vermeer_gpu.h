@@ -89,7 +89,41 @@ typedef struct vgpu_mqbvh_node {
        (builtin/geom/polymesh/trace.go:532-537, idxp is never reordered for motion
        meshes, buildqbvh.go:19-57).  Default: the face the BVH leaf bounds, accel.idx[slot]. */
 
+/* ShaderStd parameters (builtin/shader/std.go:25-47).  has_* = 1 marks a
+ * parameter that is set (the reference's nil-vs-set distinction, which
+ * selects its defaults: diffuse roughness 0.5, spec roughness 0.5, IOR 1.7). */
+typedef struct vgpu_material {
+    float emission[3]; float emission_strength; int32_t has_emission_colour, has_emission_strength;
+    float diffuse[3]; int32_t has_diffuse_colour;
+    float diffuse_strength; int32_t has_diffuse_strength;
+    float diffuse_roughness; int32_t has_diffuse_roughness;
+    float spec1[3]; int32_t has_spec1_colour;
+    float spec1_strength; int32_t has_spec1_strength;
+    float spec1_roughness; int32_t has_spec1_roughness;
+    float ior; int32_t has_ior;
+    int32_t fresnel_model;   /* 1 = Dielectric (fresnel/dielectric.go) */
+} vgpu_material;
+
+/* TriLight node (builtin/light/triangle.go:18-28): an emitting triangle with
+ * its own one-triangle mesh.  geom = insertion index of that mesh. */
+typedef struct vgpu_tri_light {
+    float P0[3], P1[3], P2[3];
+    int32_t material;
+    int32_t samples;         /* NumSamples = 1 << samples (default 1) */
+    int32_t geom;
+} vgpu_tri_light;
+
+/* Camera for one frame: the 4x4 column-major camera-to-world matrix M
+ * (camera.go:225-236), TanThetaFocal = tan(Fov/2)*Focal, Aspect, Focal. */
+typedef struct vgpu_camera {
+    float M[16];
+    float tan_theta_focal;
+    float aspect;
+    float focal;
+} vgpu_camera;
+
 typedef struct vgpu_scene vgpu_scene;
+typedef struct vgpu_render vgpu_render;
 
 /* Description of the last failure on the calling host thread (never NULL). */
 const char* vgpu_last_error(void);
@@ -173,6 +207,35 @@ int vgpu_trace(vgpu_scene* scene, uint32_t flags, const vgpu_ray* rays, uint64_t
  * ray overflowed VGPU_MAX_STACK (its hit was written as a miss).  No
  * reference counterpart: the reference panics on stack overflow. */
 int vgpu_scene_status(vgpu_scene* scene, void* stream);
+
+/* Adds a static triangle mesh with shading data: normals (nverts*3 floats,
+ * per vertex, nullable: N = Ng), shaderidx (one byte per triangle, nullable:
+ * all 0) and the mesh's shader table (material ids; PolyMesh.Shader,
+ * polymesh.go:293-306).  As vgpu_scene_add_mesh otherwise. */
+int vgpu_scene_add_shaded_mesh(vgpu_scene* scene, const float* verts, uint32_t nverts, const uint32_t* idx,
+                               uint32_t ntris, const float* normals, const uint8_t* shaderidx,
+                               const int32_t* shaders, uint32_t nshaders, int32_t* geom_out);
+
+/* Renderer over a committed scene: materials, triangle lights, camera and
+ * image size.  Replaces the body of core.Render's pass loop
+ * (core/render.go:66-137) in the reference semantics: per pixel one camera
+ * ray, ShaderStd.Eval direct lighting with MIS (std.go:77-296,
+ * core/shader.go:203-402), spectral hero-wavelength colour, film
+ * fb = (fb*iter + c)/(iter+1). */
+int vgpu_render_create(vgpu_render** out, vgpu_scene* scene, const vgpu_material* materials, uint32_t nmaterials,
+                       const vgpu_tri_light* lights, uint32_t nlights, const vgpu_camera* camera,
+                       uint32_t width, uint32_t height);
+int vgpu_render_destroy(vgpu_render* render);
+
+/* One pass over every pixel.  scrambles: 6 uint64 per pixel (lensU, lensV,
+ * time, lambda, scramble0, scramble1; core/render.go:166-176); film: width*
+ * height*3 floats, updated in place; sample_out (nullable): this pass's
+ * sample colour per pixel.  iter: the 1-based pass number.  With
+ * VGPU_TRACE_DEVICE_PTRS the arrays are device memory and the pass is
+ * enqueued on `stream`; otherwise host memory and synchronous.
+ * *shadow_rays_out (nullable, host mode only): shadow rays traced. */
+int vgpu_render_pass(vgpu_render* render, uint32_t flags, const uint64_t* scrambles, uint32_t iter, float* film,
+                     float* sample_out, void* stream, uint64_t* shadow_rays_out);
 
 /* Traversal launch configuration (the autotuner's knobs). */
 typedef struct vgpu_trace_config {

@@ -202,6 +202,9 @@ def _render_sigs(lib):
         "orc_render_free": (None, [P]),
         "orc_render_add_trilight": (None, [P, P, P, P, C.c_int32, C.c_int32, C.c_int32]),
         "orc_render_pass": (C.c_int64, [P, P, C.c_uint32, P, P, C.c_int32]),
+        "orc_fresnel_dielectric": (C.c_float, [C.c_float, C.c_float]),
+        "orc_ggx_d": (C.c_float, [P, C.c_float]),
+        "orc_oren_nayar_eval": (None, [C.c_float, P, P, C.c_float, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -314,3 +317,23 @@ class OracleRenderer:
         samp = np.zeros((self.h, self.w, 3), np.float32) if want_sample else None
         n = self.lib.orc_render_pass(self.h_r, _p(scr), iteration, _p(self.film), _p(samp), nthreads)
         return n, samp
+
+
+def fresnel_dielectric(eta, c):
+    lib = load(); _render_sigs(lib)
+    return lib.orc_fresnel_dielectric(eta, c)
+
+
+def ggx_d(m, alpha):
+    lib = load(); _render_sigs(lib)
+    mm = np.ascontiguousarray(m, np.float32)
+    return lib.orc_ggx_d(_p(mm), alpha)
+
+
+def oren_nayar_eval(lam, wi, wo, roughness):
+    lib = load(); _render_sigs(lib)
+    a = np.ascontiguousarray(wi, np.float32)
+    b = np.ascontiguousarray(wo, np.float32)
+    out = np.zeros(4, np.float32)
+    lib.orc_oren_nayar_eval(lam, _p(a), _p(b), roughness, _p(out))
+    return out

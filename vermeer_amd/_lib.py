@@ -62,7 +62,27 @@ SIGNATURES = [
     ("vgpu_scene_set_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
     ("vgpu_scene_get_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
     ("vgpu_scene_autotune", C.c_int, [_P, _U32, _P, _U64, _P, _P, C.POINTER(TraceConfig), C.POINTER(_F)]),
+    ("vgpu_scene_add_shaded_mesh", C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _P, _U32, C.POINTER(_I32)]),
+    ("vgpu_render_create", C.c_int, [C.POINTER(_P), _P, _P, _U32, _P, _U32, _P, _U32, _U32]),
+    ("vgpu_render_destroy", C.c_int, [_P]),
+    ("vgpu_render_pass", C.c_int, [_P, _U32, _P, _U32, _P, _P, _P, C.POINTER(_U64)]),
 ]
+
+# vgpu_material / vgpu_tri_light / vgpu_camera layouts
+MATERIAL_DTYPE = np.dtype([
+    ("emission", "<f4", 3), ("emission_strength", "<f4"), ("has_emission_colour", "<i4"), ("has_emission_strength", "<i4"),
+    ("diffuse", "<f4", 3), ("has_diffuse_colour", "<i4"),
+    ("diffuse_strength", "<f4"), ("has_diffuse_strength", "<i4"),
+    ("diffuse_roughness", "<f4"), ("has_diffuse_roughness", "<i4"),
+    ("spec1", "<f4", 3), ("has_spec1_colour", "<i4"),
+    ("spec1_strength", "<f4"), ("has_spec1_strength", "<i4"),
+    ("spec1_roughness", "<f4"), ("has_spec1_roughness", "<i4"),
+    ("ior", "<f4"), ("has_ior", "<i4"),
+    ("fresnel_model", "<i4"),
+])
+TRI_LIGHT_DTYPE = np.dtype([("P0", "<f4", 3), ("P1", "<f4", 3), ("P2", "<f4", 3), ("material", "<i4"),
+                            ("samples", "<i4"), ("geom", "<i4")])
+CAMERA_DTYPE = np.dtype([("M", "<f4", 16), ("tan_theta_focal", "<f4"), ("aspect", "<f4"), ("focal", "<f4")])
 
 _lib = None
 

@@ -1,0 +1,248 @@
+// Render object and its C-ABI entry points (vgpu_render_*).
+//
+// One pass = five launches on one stream: camera rays, closest hit, shade
+// (shadow-ray slots), shadow any-hit over every slot, resolve into the film.
+// All buffers are allocated once at creation; a pass allocates nothing
+// except, in host-pointer mode, a staging area the first time it is needed.
+
+#include <new>
+
+#include "scene_internal.h"
+
+using vgpu::fail;
+using namespace vgpu_detail;
+
+namespace {
+
+// RasterXY enumeration tables (math/ldseq/raster.go:10-40) for 2^m x 2^m elementary intervals.
+// Index bit k of the (VdC, Sobol) sequence moves the top m bits of x and y by image(k); the
+// lookup needs the images of bits 2m.. (fwd) and the GF(2) inverse of the images of bits 0..2m-1.
+uint64_t vdc52(uint64_t i) {
+    uint64_t r = 0;
+    for (int b = 0; b < 64; b++) if ((i >> b) & 1) r |= 1ull << (63 - b);
+    return r >> 12;
+}
+uint64_t sobol52(uint64_t i) {
+    uint64_t r = 0;
+    for (uint64_t v = 1ull << 51; i != 0; i >>= 1) {
+        if (i & 1) r ^= v;
+        v ^= v >> 1;
+    }
+    return r;
+}
+uint64_t bit_image(int m, int k) {
+    const uint64_t x = vdc52(1ull << k) >> (52 - m), y = sobol52(1ull << k) >> (52 - m);
+    return (x << m) | y;
+}
+bool raster_tables(int m, uint64_t* fwd, int nfwd, uint64_t* inv) {
+    const int n = 2 * m;
+    for (int c = 0; c < nfwd; c++) fwd[c] = (n + c < 64) ? bit_image(m, n + c) : 0;
+    // columns col[k] = image of low bit k; invert the n x n GF(2) matrix by elimination on [A | I]
+    std::vector<uint64_t> rowsA(n), rowsI(n);
+    for (int r = 0; r < n; r++) {
+        uint64_t a = 0;
+        for (int k = 0; k < n; k++) if ((bit_image(m, k) >> r) & 1) a |= 1ull << k;
+        rowsA[r] = a;
+        rowsI[r] = 1ull << r;
+    }
+    for (int c = 0; c < n; c++) {
+        int piv = -1;
+        for (int r = c; r < n; r++) if ((rowsA[r] >> c) & 1) { piv = r; break; }
+        if (piv < 0) return false;
+        std::swap(rowsA[piv], rowsA[c]);
+        std::swap(rowsI[piv], rowsI[c]);
+        for (int r = 0; r < n; r++)
+            if (r != c && ((rowsA[r] >> c) & 1)) { rowsA[r] ^= rowsA[c]; rowsI[r] ^= rowsI[c]; }
+    }
+    for (int c = 0; c < n; c++) {
+        uint64_t v = 0;
+        for (int r = 0; r < n; r++) if ((rowsI[r] >> c) & 1) v |= 1ull << r;
+        inv[c] = v;
+    }
+    return true;
+}
+
+}  // namespace
+
+struct vgpu_render {
+    vgpu_scene* scene = nullptr;
+    vgpu::RenderView rv{};
+    uint32_t width = 0, height = 0;
+    vgpu_material* d_materials = nullptr;
+    vgpu::DevLight* d_lights = nullptr;
+    float4* d_rays = nullptr;      // camera rays (2 float4 per pixel)
+    float4* d_pix = nullptr;       // per pixel: D, lambda
+    int4* d_hits32 = nullptr;      // extended hits (2 int4 per pixel)
+    float4* d_slot_rays = nullptr;
+    float4* d_slot_rgb = nullptr;
+    int4* d_slot_hits = nullptr;
+    float4* d_groups = nullptr;
+    float4* d_pixshade = nullptr;
+    void* d_stage = nullptr;       // host-pointer mode: scrambles, film, samples
+    size_t stage_bytes = 0;
+
+    ~vgpu_render() {
+        int prev = 0;
+        if (hipGetDevice(&prev) == hipSuccess && scene) (void)hipSetDevice(scene->device);
+        void* ptrs[] = {d_materials, d_lights, d_rays, d_pix, d_hits32, d_slot_rays, d_slot_rgb, d_slot_hits,
+                        d_groups, d_pixshade, d_stage};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        (void)hipSetDevice(prev);
+    }
+};
+
+extern "C" {
+
+int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* materials, uint32_t nmaterials,
+                       const vgpu_tri_light* lights, uint32_t nlights, const vgpu_camera* cam, uint32_t width,
+                       uint32_t height) {
+    if (!out || !s || !materials || !cam) return fail(VGPU_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (!s->committed) return fail(VGPU_ERR_STATE, "render_create before scene commit");
+    if (s->any_motion) return fail(VGPU_ERR_ARG, "the renderer supports static scenes only");
+    if (width == 0 || height == 0 || width > 4096 || height > 4096)
+        return fail(VGPU_ERR_ARG, "image size %ux%u outside [1,4096] (RasterXY uses 2^12 intervals)", width, height);
+    if (s->max_material >= (int32_t)nmaterials)
+        return fail(VGPU_ERR_ARG, "a mesh refers to material %d of %u", s->max_material, nmaterials);
+    for (uint32_t i = 0; i < nmaterials; i++) {
+        const vgpu_material& m = materials[i];
+        const float rough = m.has_spec1_roughness ? m.spec1_roughness : 0.5f;
+        const float sw = m.has_spec1_strength ? m.spec1_strength : 0.0f;
+        if (sw > 0.0f && rough == 0.0f)
+            return fail(VGPU_ERR_ARG, "material %u: perfect mirrors (Spec1Roughness 0) are not supported", i);
+        if (sw > 0.0f && m.fresnel_model != 1) return fail(VGPU_ERR_ARG, "material %u: only the dielectric Fresnel model", i);
+        const float dw = m.has_diffuse_strength ? m.diffuse_strength : 0.0f;
+        if (dw + sw == 0.0f) return fail(VGPU_ERR_ARG, "material %u has no weight (std.go:141-143 panics)", i);
+    }
+    int ngeoms = (int)s->meshes.size();
+    std::vector<vgpu::DevLight> dl(nlights);
+    int max_ns = 1;
+    for (uint32_t l = 0; l < nlights; l++) {
+        const vgpu_tri_light& L = lights[l];
+        if (L.material < 0 || L.material >= (int32_t)nmaterials) return fail(VGPU_ERR_ARG, "light %u material", l);
+        if (L.samples < 0 || L.samples > 8) return fail(VGPU_ERR_ARG, "light %u samples must be in [0,8]", l);
+        if (L.geom < 0 || L.geom >= ngeoms) return fail(VGPU_ERR_ARG, "light %u geom %d out of range", l, L.geom);
+        dl[l].P0 = make_float4(L.P0[0], L.P0[1], L.P0[2], 0.0f);
+        dl[l].P1 = make_float4(L.P1[0], L.P1[1], L.P1[2], 0.0f);
+        dl[l].P2 = make_float4(L.P2[0], L.P2[1], L.P2[2], 0.0f);
+        dl[l].material = L.material;
+        dl[l].samples = L.samples;
+        dl[l].geom = -1;
+        for (int slot = 0; slot < ngeoms; slot++) if (s->order[slot] == L.geom) dl[l].geom = slot;
+        max_ns = std::max(max_ns, 1 << L.samples);
+    }
+    vgpu_render* r = new (std::nothrow) vgpu_render();
+    if (!r) return fail(VGPU_ERR_NOMEM, "out of host memory");
+    r->scene = s;
+    r->width = width;
+    r->height = height;
+    vgpu::RenderView& rv = r->rv;
+    rv.sv = s->view;
+    rv.width = (int32_t)width;
+    rv.height = (int32_t)height;
+    rv.nlights = (int32_t)nlights;
+    rv.groups = 2 * (int32_t)nlights;
+    rv.max_half = std::max(1, max_ns / 2);
+    rv.slots = std::max(1, rv.groups * 2 * rv.max_half);
+    std::memcpy(rv.M, cam->M, sizeof(rv.M));
+    rv.tan_theta_focal = cam->tan_theta_focal;
+    rv.aspect = cam->aspect;
+    rv.focal = cam->focal;
+    if (!raster_tables(12, rv.raster_fwd, 52, rv.raster_inv)) { delete r; return fail(VGPU_ERR_ARG, "RasterXY tables"); }
+    DeviceGuard dg(s->device);
+    const size_t np = (size_t)width * height, ns = np * (size_t)rv.slots;
+    auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)); };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = alloc((void**)&r->d_materials, sizeof(vgpu_material) * nmaterials);
+    if (e == hipSuccess) e = alloc((void**)&r->d_lights, sizeof(vgpu::DevLight) * nlights);
+    if (e == hipSuccess) e = alloc((void**)&r->d_rays, np * 32);
+    if (e == hipSuccess) e = alloc((void**)&r->d_pix, np * 16);
+    if (e == hipSuccess) e = alloc((void**)&r->d_hits32, np * 32);
+    if (e == hipSuccess) e = alloc((void**)&r->d_slot_rays, ns * 32);
+    if (e == hipSuccess) e = alloc((void**)&r->d_slot_rgb, ns * 16);
+    if (e == hipSuccess) e = alloc((void**)&r->d_slot_hits, ns * 16);
+    if (e == hipSuccess) e = alloc((void**)&r->d_groups, np * (size_t)rv.groups * 16);
+    if (e == hipSuccess) e = alloc((void**)&r->d_pixshade, np * 64);
+    if (e == hipSuccess) e = hipMemcpy(r->d_materials, materials, sizeof(vgpu_material) * nmaterials, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nlights) e = hipMemcpy(r->d_lights, dl.data(), sizeof(vgpu::DevLight) * nlights, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { delete r; return fail(VGPU_ERR_HIP, "render buffers: %s", hipGetErrorString(e)); }
+    rv.normals = s->d_normals;
+    rv.shaderidx = s->d_shaderidx;
+    rv.shaders = s->d_shaders;
+    rv.gshade = s->d_gshade;
+    rv.materials = r->d_materials;
+    rv.lights = r->d_lights;
+    // size the traversal spill area for the larger (shadow) launch
+    vgpu::TraceLaunch L{s->cfg.block, s->cfg.lds_stack, s->cfg.waves_per_cu};
+    int rc = ensure_spill(s, std::max(vgpu::trace_grid_threads(L, ns, s->num_cus),
+                                      vgpu::trace_grid_threads(vgpu::TraceLaunch{256, 16, 16}, np, s->num_cus)));
+    if (rc) { delete r; return rc; }
+    *out = r;
+    return VGPU_OK;
+}
+
+int vgpu_render_destroy(vgpu_render* r) {
+    delete r;
+    return VGPU_OK;
+}
+
+int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, uint32_t iter, float* film,
+                     float* sample_out, void* stream, uint64_t* shadow_rays_out) {
+    if (!r || !scrambles || !film) return fail(VGPU_ERR_ARG, "NULL argument");
+    if (iter == 0) return fail(VGPU_ERR_ARG, "iter is 1-based");
+    vgpu_scene* s = r->scene;
+    DeviceGuard dg(s->device);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t np = (size_t)r->width * r->height, ns = np * (size_t)r->rv.slots;
+    const uint64_t* d_scr = scrambles;
+    float* d_film = film;
+    float* d_samp = sample_out;
+    const bool host = (flags & VGPU_TRACE_DEVICE_PTRS) == 0;
+    if (host) {
+        const size_t need = np * 48 + np * 12 + (sample_out ? np * 12 : 0);
+        if (need > r->stage_bytes) {
+            if (r->d_stage) { HIP_TRY(hipFree(r->d_stage)); r->d_stage = nullptr; r->stage_bytes = 0; }
+            HIP_TRY(hipMalloc(&r->d_stage, need));
+            r->stage_bytes = need;
+        }
+        char* b = (char*)r->d_stage;
+        d_scr = (const uint64_t*)b;
+        d_film = (float*)(b + np * 48);
+        d_samp = sample_out ? (float*)(b + np * 60) : nullptr;
+        HIP_TRY(hipMemcpyAsync((void*)d_scr, scrambles, np * 48, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_film, film, np * 12, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemsetAsync(s->d_err, 0, sizeof(int32_t), st));
+    }
+    hipError_t e = vgpu::launch_camera(r->rv, d_scr, iter, r->d_rays, r->d_pix, st);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "camera: %s", hipGetErrorString(e));
+    e = vgpu::launch_trace_ext(s->view, false, r->d_rays, np, r->d_hits32, s->d_spT, s->d_spN, s->spill_threads, s->d_err,
+                               s->num_cus, st);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "closest hit: %s", hipGetErrorString(e));
+    e = vgpu::launch_shade(r->rv, d_scr, iter, r->d_hits32, r->d_pix, r->d_slot_rays, r->d_slot_rgb, r->d_groups,
+                           r->d_pixshade, st);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "shade: %s", hipGetErrorString(e));
+    int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns, r->d_slot_hits, nullptr,
+                           nullptr, st, s->cfg);
+    if (rc) return rc;
+    e = vgpu::launch_resolve(r->rv, iter, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, d_film, d_samp, st);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "resolve: %s", hipGetErrorString(e));
+    if (host) {
+        HIP_TRY(hipMemcpyAsync(film, d_film, np * 12, hipMemcpyDeviceToHost, st));
+        if (sample_out) HIP_TRY(hipMemcpyAsync(sample_out, d_samp, np * 12, hipMemcpyDeviceToHost, st));
+        int32_t err = 0;
+        HIP_TRY(hipMemcpyAsync(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
+        if (shadow_rays_out) {
+            // count active slots (diagnostic, host mode only)
+            std::vector<float4> rgb(ns);
+            HIP_TRY(hipMemcpy(rgb.data(), r->d_slot_rgb, ns * 16, hipMemcpyDeviceToHost));
+            uint64_t c = 0;
+            for (const float4& v : rgb) c += v.w != 0.0f;
+            *shadow_rays_out = c;
+        }
+    }
+    return VGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// Internal interface between the render host code (render.cpp) and the
+// render kernels (render.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/vermeer_gpu.h"
+#include "trace_kernels.h"
+
+namespace vgpu {
+
+// Per post-sort geom slot: where its shading data lives.
+struct alignas(16) DevGeomShade {
+    int32_t nrm_base;      // first float4 of its per-triangle vertex normals (3 per triangle), -1: none
+    int32_t sidx_base;     // first byte of its per-triangle shader index, -1: none (index 0)
+    int32_t shader_base;   // first entry of its shader table (material ids)
+    int32_t nshaders;
+};
+
+struct alignas(16) DevLight {  // builtin/light/triangle.go:18-28
+    float4 P0, P1, P2;         // xyz
+    int32_t material;
+    int32_t samples;           // NumSamples = 1 << samples
+    int32_t geom;              // post-sort geom slot of the light's own mesh
+    int32_t pad;
+};
+
+struct RenderView {
+    SceneView sv;
+    const float4* normals;         // per-triangle vertex normals
+    const uint8_t* shaderidx;      // per-triangle shader index
+    const int32_t* shaders;        // shader tables (material ids)
+    const DevGeomShade* gshade;
+    const vgpu_material* materials;
+    const DevLight* lights;
+    int32_t nlights;
+    int32_t width, height;
+    int32_t slots;                 // shadow-ray slots per pixel
+    int32_t groups;                // (light, lobe) groups per pixel = 2 * nlights
+    int32_t max_half;              // max over lights of NSamples/2 (slots per group = 2 * max_half)
+    float M[16];                   // camera matrix (column-major)
+    float tan_theta_focal, aspect, focal;
+    uint64_t raster_fwd[52];       // RasterXY enumeration tables for m = 12
+    uint64_t raster_inv[24];
+};
+
+// Launch wrappers (all enqueue on `stream`).
+hipError_t launch_camera(const RenderView& rv, const uint64_t* scrambles, uint32_t iter, float4* rays, float4* pix,
+                         hipStream_t stream);
+hipError_t launch_shade(const RenderView& rv, const uint64_t* scrambles, uint32_t iter, const int4* hits32,
+                        const float4* pix, float4* slot_rays, float4* slot_rgb, float4* groups, float4* pixshade,
+                        hipStream_t stream);
+hipError_t launch_resolve(const RenderView& rv, uint32_t iter, const float4* slot_rgb, const int4* slot_hits,
+                          const float4* groups, const float4* pixshade, float* film, float* sample_out,
+                          hipStream_t stream);
+
+}  // namespace vgpu

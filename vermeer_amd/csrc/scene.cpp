@@ -10,26 +10,15 @@
 // Compiled with -ffp-contract=off: the box/centroid arithmetic feeding the
 // builder must round exactly as the reference's.
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <functional>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <limits>
 #include <new>
-#include <string>
-#include <vector>
 
-#include "../../include/vermeer_gpu.h"
-#include "qbvh_build.h"
-#include "trace_kernels.h"
+#include "scene_internal.h"
+
+namespace vgpu {
 
 namespace {
-
 thread_local std::string g_err;
+}
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -41,100 +30,14 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                         \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return fail(VGPU_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+void clear_error() { g_err.clear(); }
 
-const float kInf = std::numeric_limits<float>::infinity();
-inline float mins(float x, float y) { return (x < y) ? x : y; }
-inline float maxs(float x, float y) { return (x > y) ? x : y; }
+}  // namespace vgpu
 
-struct Box6 {
-    float v[6];   // lo xyz, hi xyz
-    void reset() { for (int k = 0; k < 3; k++) { v[k] = kInf; v[3 + k] = -kInf; } }
-    void grow_point(const float* p) { for (int k = 0; k < 3; k++) { v[k] = mins(p[k], v[k]); v[3 + k] = maxs(p[k], v[3 + k]); } }
-    void grow_box(const float* b) { for (int k = 0; k < 3; k++) { v[k] = mins(v[k], b[k]); v[3 + k] = maxs(v[3 + k], b[3 + k]); } }
-};
+using vgpu::fail;
+using namespace vgpu_detail;
 
-// math/vec3.go:52-58
-inline void lerp3(const float* a, const float* b, float t, float* o) {
-    for (int k = 0; k < 3; k++) o[k] = (1.0f - t) * a[k] + t * b[k];
-}
-
-struct MeshHost {
-    uint32_t nverts = 0, nkeys = 1, ntris = 0;
-    std::vector<float> verts;          // nkeys * nverts * 3
-    std::vector<uint32_t> idx;         // as given (static prebuilt: already post-sort)
-    float ray_bias = 0.0f;
-    uint32_t flags = 0;
-    bool built = false;
-    std::vector<vgpu_qbvh_node> nodes;
-    std::vector<uint32_t> idxp_sorted;
-    std::vector<int32_t> perm;
-    std::vector<vgpu_mqbvh_node> mnodes;
-    std::vector<float> mboxes;         // nkeys * nmnodes * 24
-    float bounds[6];
-    std::vector<float> motion_bounds;  // nkeys * 6
-
-    const float* vert(uint32_t key, uint32_t i) const { return &verts[((size_t)key * nverts + i) * 3]; }
-
-    // PolyMesh.Bounds (builtin/geom/polymesh/bounds.go:159-186)
-    void bounds_at(float time, float* out) const {
-        if (nkeys == 1) { std::memcpy(out, bounds, sizeof(bounds)); return; }
-        const float k = time * (float)(nkeys - 1);
-        const float t = k - std::floor(k);
-        const int key = (int)std::floor(k), key2 = (int)std::ceil(k);
-        for (int i = 0; i < 6; i++) out[i] = (1 - t) * motion_bounds[key * 6 + i] + t * motion_bounds[key2 * 6 + i];
-    }
-};
-
-}  // namespace
-
-struct vgpu_scene {
-    int device = 0;
-    int num_cus = 256;
-    std::vector<MeshHost> meshes;      // insertion order
-    std::vector<int32_t> order;        // post-sort slot -> insertion index
-    bool committed = false;
-    bool top_motion = false;
-    int top_keys = 1;
-    std::vector<vgpu_qbvh_node> top_nodes;
-    std::vector<vgpu_mqbvh_node> top_mnodes;
-    std::vector<float> top_mboxes;
-    bool any_motion = false;
-
-    vgpu::DevNode* d_nodes = nullptr;
-    float4* d_tris = nullptr;
-    float4* d_mnodes = nullptr;
-    float4* d_mtris = nullptr;
-    vgpu::DevGeom* d_geoms = nullptr;
-    int32_t* d_err = nullptr;
-    float* d_spT = nullptr;
-    int32_t* d_spN = nullptr;
-    uint64_t spill_threads = 0;
-    void* d_stage = nullptr;
-    size_t stage_bytes = 0;
-    vgpu::SceneView view{};
-    vgpu_trace_config cfg{256, 16, 16, 0};
-
-    ~vgpu_scene() {
-        int prev = 0;
-        if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(device);
-        void* ptrs[] = {d_nodes, d_tris, d_mnodes, d_mtris, d_geoms, d_err, d_spT, d_spN, d_stage};
-        for (void* p : ptrs) if (p) (void)hipFree(p);
-        (void)hipSetDevice(prev);
-    }
-};
-
-namespace {
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) { if (hipGetDevice(&prev) != hipSuccess) prev = -1; (void)hipSetDevice(dev); }
-    ~DeviceGuard() { if (prev >= 0) (void)hipSetDevice(prev); }
-};
+namespace vgpu_detail {
 
 // Static mesh accel, as PolyMesh.initAccel (buildqbvh.go:60-129).
 int build_static_mesh(MeshHost& m) {
@@ -162,6 +65,10 @@ int build_static_mesh(MeshHost& m) {
     m.idxp_sorted.resize((size_t)n * 3);
     for (uint32_t i = 0; i < n; i++)
         for (int c = 0; c < 3; c++) m.idxp_sorted[(size_t)i * 3 + c] = m.idx[(size_t)ids[i] * 3 + c];
+    if (!m.shaderidx.empty()) {   // shaderidx follows the leaf order (buildqbvh.go:98-104)
+        m.shaderidx_sorted.resize(n);
+        for (uint32_t i = 0; i < n; i++) m.shaderidx_sorted[i] = m.shaderidx[ids[i]];
+    }
     m.built = true;
     return VGPU_OK;
 }
@@ -302,7 +209,7 @@ int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n,
 
 extern "C" {
 
-const char* vgpu_last_error(void) { return g_err.c_str(); }
+const char* vgpu_last_error(void) { return vgpu::g_err.c_str(); }
 
 int vgpu_device_count(int* count) {
     if (!count) return fail(VGPU_ERR_ARG, "count is NULL");
@@ -365,6 +272,24 @@ int vgpu_scene_add_mesh(vgpu_scene* s, const float* verts, uint32_t nverts, uint
     m.idx.assign(idx, idx + (size_t)ntris * 3);
     s->meshes.push_back(std::move(m));
     if (geom_out) *geom_out = (int32_t)s->meshes.size() - 1;
+    return VGPU_OK;
+}
+
+int vgpu_scene_add_shaded_mesh(vgpu_scene* s, const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntris,
+                               const float* normals, const uint8_t* shaderidx, const int32_t* shaders, uint32_t nshaders,
+                               int32_t* geom_out) {
+    if (!shaders || nshaders == 0) return fail(VGPU_ERR_ARG, "a shaded mesh needs a shader table");
+    for (uint32_t i = 0; i < nshaders; i++)
+        if (shaders[i] < 0) return fail(VGPU_ERR_ARG, "shader %u has negative material id", i);
+    if (shaderidx)
+        for (uint32_t i = 0; i < ntris; i++)
+            if (shaderidx[i] >= nshaders) return fail(VGPU_ERR_ARG, "triangle %u uses shader %u of %u", i, shaderidx[i], nshaders);
+    int rc = vgpu_scene_add_mesh(s, verts, nverts, 1, idx, ntris, 0.0f, 0, geom_out);
+    if (rc) return rc;
+    MeshHost& m = s->meshes.back();
+    if (normals) m.normals.assign(normals, normals + (size_t)nverts * 3);
+    if (shaderidx) m.shaderidx.assign(shaderidx, shaderidx + ntris);
+    m.shaders.assign(shaders, shaders + nshaders);
     return VGPU_OK;
 }
 
@@ -509,6 +434,41 @@ int vgpu_scene_commit(vgpu_scene* s) {
     if ((rc = upload(&s->d_mnodes, dmnodes))) return rc;
     if ((rc = upload(&s->d_mtris, dmtris))) return rc;
     if ((rc = upload(&s->d_geoms, dgeoms))) return rc;
+    // Shading data per slot: per-triangle vertex normals (the mesh's normalidx equals its idxp,
+    // init.go:85-91), per-triangle shader index, shader tables.
+    {
+        std::vector<float4> dn;
+        std::vector<uint8_t> dsi;
+        std::vector<int32_t> dsh;
+        std::vector<vgpu::DevGeomShade> dgs(ng);
+        for (int slot = 0; slot < ng; slot++) {
+            MeshHost& m = s->meshes[s->order[slot]];
+            vgpu::DevGeomShade& gsd = dgs[slot];
+            gsd.nrm_base = -1;
+            gsd.sidx_base = -1;
+            if (!m.normals.empty() && m.nkeys == 1) {
+                gsd.nrm_base = (int32_t)dn.size();
+                for (uint32_t i = 0; i < m.ntris; i++)
+                    for (int j = 0; j < 3; j++) {
+                        const float* nv = &m.normals[(size_t)m.idxp_sorted[(size_t)i * 3 + j] * 3];
+                        dn.push_back(make_float4(nv[0], nv[1], nv[2], 0.0f));
+                    }
+            }
+            if (!m.shaderidx_sorted.empty()) {
+                gsd.sidx_base = (int32_t)dsi.size();
+                dsi.insert(dsi.end(), m.shaderidx_sorted.begin(), m.shaderidx_sorted.end());
+            }
+            gsd.shader_base = (int32_t)dsh.size();
+            gsd.nshaders = (int32_t)m.shaders.size();
+            dsh.insert(dsh.end(), m.shaders.begin(), m.shaders.end());
+            for (int32_t mid : m.shaders) s->max_material = std::max(s->max_material, mid);
+        }
+        if (dsh.empty()) dsh.push_back(-1);
+        if ((rc = upload(&s->d_normals, dn))) return rc;
+        if ((rc = upload(&s->d_shaderidx, dsi))) return rc;
+        if ((rc = upload(&s->d_shaders, dsh))) return rc;
+        if ((rc = upload(&s->d_gshade, dgs))) return rc;
+    }
     s->view.nodes = s->d_nodes;
     s->view.tris = s->d_tris;
     s->view.mnodes = s->d_mnodes;
@@ -651,7 +611,7 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
             }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    g_err.clear();
+    vgpu::clear_error();
     s->cfg = bc;
     if (best_out) *best_out = bc;
     if (ms_out) *ms_out = best;

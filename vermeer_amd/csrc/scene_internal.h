@@ -1,0 +1,146 @@
+// Scene internals shared by the C-ABI translation units (scene.cpp, render.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/vermeer_gpu.h"
+#include "qbvh_build.h"
+#include "render_kernels.h"
+#include "trace_kernels.h"
+
+#define HIP_TRY(expr)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) return vgpu::fail(VGPU_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+namespace vgpu {
+
+// Records the message for vgpu_last_error() and returns code.
+int fail(int code, const char* fmt, ...);
+void clear_error();
+
+}  // namespace vgpu
+
+namespace vgpu_detail {
+
+using vgpu::fail;
+
+const float kInf = std::numeric_limits<float>::infinity();
+inline float mins(float x, float y) { return (x < y) ? x : y; }
+inline float maxs(float x, float y) { return (x > y) ? x : y; }
+
+struct Box6 {
+    float v[6];   // lo xyz, hi xyz
+    void reset() { for (int k = 0; k < 3; k++) { v[k] = kInf; v[3 + k] = -kInf; } }
+    void grow_point(const float* p) { for (int k = 0; k < 3; k++) { v[k] = mins(p[k], v[k]); v[3 + k] = maxs(p[k], v[3 + k]); } }
+    void grow_box(const float* b) { for (int k = 0; k < 3; k++) { v[k] = mins(v[k], b[k]); v[3 + k] = maxs(v[3 + k], b[3 + k]); } }
+};
+
+// math/vec3.go:52-58
+inline void lerp3(const float* a, const float* b, float t, float* o) {
+    for (int k = 0; k < 3; k++) o[k] = (1.0f - t) * a[k] + t * b[k];
+}
+
+struct MeshHost {
+    uint32_t nverts = 0, nkeys = 1, ntris = 0;
+    std::vector<float> verts;          // nkeys * nverts * 3
+    std::vector<uint32_t> idx;         // as given (static prebuilt: already post-sort)
+    float ray_bias = 0.0f;
+    uint32_t flags = 0;
+    bool built = false;
+    std::vector<vgpu_qbvh_node> nodes;
+    std::vector<uint32_t> idxp_sorted;
+    std::vector<int32_t> perm;
+    std::vector<vgpu_mqbvh_node> mnodes;
+    std::vector<float> mboxes;         // nkeys * nmnodes * 24
+    float bounds[6];
+    std::vector<float> motion_bounds;  // nkeys * 6
+    std::vector<float> normals;        // nverts * 3 (shaded meshes), empty: N = Ng
+    std::vector<uint8_t> shaderidx;    // per triangle (insertion order; reordered at build)
+    std::vector<uint8_t> shaderidx_sorted;
+    std::vector<int32_t> shaders;      // shader table: material ids
+
+    const float* vert(uint32_t key, uint32_t i) const { return &verts[((size_t)key * nverts + i) * 3]; }
+
+    // PolyMesh.Bounds (builtin/geom/polymesh/bounds.go:159-186)
+    void bounds_at(float time, float* out) const {
+        if (nkeys == 1) { std::memcpy(out, bounds, sizeof(bounds)); return; }
+        const float k = time * (float)(nkeys - 1);
+        const float t = k - std::floor(k);
+        const int key = (int)std::floor(k), key2 = (int)std::ceil(k);
+        for (int i = 0; i < 6; i++) out[i] = (1 - t) * motion_bounds[key * 6 + i] + t * motion_bounds[key2 * 6 + i];
+    }
+};
+
+}  // namespace vgpu_detail
+
+struct vgpu_scene {
+    int device = 0;
+    int num_cus = 256;
+    std::vector<vgpu_detail::MeshHost> meshes;      // insertion order
+    std::vector<int32_t> order;        // post-sort slot -> insertion index
+    bool committed = false;
+    bool top_motion = false;
+    int top_keys = 1;
+    std::vector<vgpu_qbvh_node> top_nodes;
+    std::vector<vgpu_mqbvh_node> top_mnodes;
+    std::vector<float> top_mboxes;
+    bool any_motion = false;
+
+    vgpu::DevNode* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    float4* d_mnodes = nullptr;
+    float4* d_mtris = nullptr;
+    vgpu::DevGeom* d_geoms = nullptr;
+    int32_t* d_err = nullptr;
+    float* d_spT = nullptr;
+    int32_t* d_spN = nullptr;
+    uint64_t spill_threads = 0;
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    vgpu::SceneView view{};
+    // shading data (vgpu_scene_add_shaded_mesh)
+    float4* d_normals = nullptr;
+    uint8_t* d_shaderidx = nullptr;
+    int32_t* d_shaders = nullptr;
+    vgpu::DevGeomShade* d_gshade = nullptr;
+    int32_t max_material = -1;
+    vgpu_trace_config cfg{256, 16, 16, 0};
+
+    ~vgpu_scene() {
+        int prev = 0;
+        if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(device);
+        void* ptrs[] = {d_nodes, d_tris, d_mnodes, d_mtris, d_geoms, d_err, d_spT, d_spN, d_stage, d_normals, d_shaderidx,
+                        d_shaders, d_gshade};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        (void)hipSetDevice(prev);
+    }
+};
+
+
+namespace vgpu_detail {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) { if (hipGetDevice(&prev) != hipSuccess) prev = -1; (void)hipSetDevice(dev); }
+    ~DeviceGuard() { if (prev >= 0) (void)hipSetDevice(prev); }
+};
+
+// Ensures the spill area covers `threads` traversal threads.
+int ensure_spill(vgpu_scene* s, uint64_t threads);
+// Enqueues one traversal (device pointers) with configuration cfg.
+int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,
+                  int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg);
+
+}  // namespace vgpu_detail

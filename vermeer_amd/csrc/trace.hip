@@ -122,7 +122,7 @@ __device__ __forceinline__ int leaf_count(int32_t l) { return (int)((l & 0xf) + 
 // Returns true when the triangle is accepted; updates tclosest, U, V.
 template <bool MOTION_BOUND>
 __device__ __forceinline__ bool tri_test(RayS& r, const float4& v0, const float4& v1, const float4& v2, float bias,
-                                         float& U, float& V) {
+                                         float& U, float& V, float& W) {
     const float AKz = sel3(v0.x, v0.y, v0.z, r.Kz) - r.Pkz;
     const float BKz = sel3(v1.x, v1.y, v1.z, r.Kz) - r.Pkz;
     const float CKz = sel3(v2.x, v2.y, v2.z, r.Kz) - r.Pkz;
@@ -155,6 +155,7 @@ __device__ __forceinline__ bool tri_test(RayS& r, const float4& v0, const float4
     const float rcp = 1.0f / det;
     U = fU * rcp;
     V = fV * rcp;
+    W = fW * rcp;
     r.tclosest = T * rcp;
     return true;
 }
@@ -202,7 +203,8 @@ __device__ __forceinline__ void motion_keys(float rtime, int nkeys, int& key, in
     key2 = (int)ceilf(k);
 }
 
-template <int BLOCK, int LDS, bool SHADOW, bool MOTION, bool COUNTERS>
+// EXT: hits are 32-byte records {prim, geom, t, U, V, W, 0, 0} for the shading kernels.
+template <int BLOCK, int LDS, bool SHADOW, bool MOTION, bool COUNTERS, bool EXT = false>
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __restrict__ rays, uint64_t n,
                                                  int4* __restrict__ hits, int32_t* __restrict__ geoms_out,
                                                  int4* __restrict__ counters, float* __restrict__ spT,
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
 
         int nodesT = 0, leafsT = 0, trisT = 0, maxDepth = 0;
         int best_geom = -1, best_prim = -1;
-        float bu = 0.0f, bv = 0.0f;
+        float bu = 0.0f, bv = 0.0f, bw = 0.0f;
 
         // top level
         const bool top_motion = MOTION && sv.top_motion;
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
                 if (in_mesh) {
                     if (COUNTERS) trisT += count;
                     int idx = -1;
-                    float U = 0.0f, V = 0.0f;
+                    float U = 0.0f, V = 0.0f, W = 0.0f;
                     if (MOTION && g.nkeys > 1) {
                         const float4* tp = sv.mtris + g.tri_base + (size_t)base * 3 * g.nkeys;
                         for (int i = 0; i < count; i++) {
@@ -309,17 +311,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
                             const float4 v0 = lerp4(q0[0], q1[0], mtime);
                             const float4 v1 = lerp4(q0[1], q1[1], mtime);
                             const float4 v2 = lerp4(q0[2], q1[2], mtime);
-                            if (tri_test<true>(r, v0, v1, v2, g.ray_bias, U, V)) idx = base + i;
+                            if (tri_test<true>(r, v0, v1, v2, g.ray_bias, U, V, W)) idx = base + i;
                         }
                     } else {
                         const float4* tp = sv.tris + ((size_t)g.tri_base + base) * 3;
                         for (int i = 0; i < count; i++) {
                             const float4 v0 = tp[3 * i], v1 = tp[3 * i + 1], v2 = tp[3 * i + 2];
-                            if (tri_test<false>(r, v0, v1, v2, g.ray_bias, U, V)) idx = base + i;
+                            if (tri_test<false>(r, v0, v1, v2, g.ray_bias, U, V, W)) idx = base + i;
                         }
                     }
                     if (idx >= 0) {
-                        best_geom = gcur; best_prim = idx; bu = U; bv = V;
+                        best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
                         if (SHADOW) break;   // qbvh/intersect.go:232-236 at both levels
                     }
                 } else {
@@ -339,8 +341,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
             atomicOr(err, 1);
             best_geom = -1; best_prim = -1;
         }
-        hits[ri] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
-        if (geoms_out) geoms_out[ri] = best_geom;
+        if (EXT) {
+            hits[2 * ri] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
+            hits[2 * ri + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
+        } else {
+            hits[ri] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
+            if (geoms_out) geoms_out[ri] = best_geom;
+        }
         if (COUNTERS) counters[ri] = make_int4(nodesT, leafsT, trisT, maxDepth);
     }
 }
@@ -370,6 +377,22 @@ hipError_t launch_mode(const SceneView& sv, bool shadow, bool motion, const void
 }
 
 }  // namespace
+
+hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, uint64_t n, void* hits32, float* spT,
+                            int32_t* spN, uint64_t spill_threads, int32_t* err, int num_cus, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const TraceLaunch cfg{256, 16, 16};
+    const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
+    if (threads > spill_threads) return hipErrorInvalidValue;
+    const unsigned grid = (unsigned)(threads / cfg.block);
+    if (motion)
+        hipLaunchKernelGGL((k_trace<256, 16, false, true, false, true>), dim3(grid), dim3(256), 0, stream, sv,
+                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err);
+    else
+        hipLaunchKernelGGL((k_trace<256, 16, false, false, false, true>), dim3(grid), dim3(256), 0, stream, sv,
+                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err);
+    return hipGetLastError();
+}
 
 uint64_t trace_grid_threads(const TraceLaunch& cfg, uint64_t n, int num_cus) {
     const uint64_t resident = (uint64_t)num_cus * (uint64_t)cfg.waves_per_cu * 64ull;

@@ -64,6 +64,11 @@ hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const Tra
                         float* spill_T, int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag,
                         int num_cus, hipStream_t stream);
 
+// Closest hit with 32-byte records {prim, geom, t, U, V, W, 0, 0} (U, V, W: barycentric weights of the
+// triangle's three vertices), for the shading kernels.  Fixed configuration (block 256, LDS stack 16).
+hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, uint64_t n, void* hits32, float* spill_T,
+                            int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag, int num_cus, hipStream_t stream);
+
 // Threads the launch above will use (so the caller can size the spill area).
 uint64_t trace_grid_threads(const TraceLaunch& cfg, uint64_t n, int num_cus);
 

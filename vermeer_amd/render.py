@@ -1,0 +1,97 @@
+"""Renderer: the reference's render pass loop on the GPU.
+
+Mirrors core.Render (core/render.go:140-218) for a scene of shaded meshes
+and triangle lights: create once per frame setup, then one pass() per
+iteration with the 1-based pass number the reference hands to render().
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import CAMERA_DTYPE, MATERIAL_DTYPE, TRI_LIGHT_DTYPE, check, ptr
+from .accel import Scene
+
+
+def lookat_camera(frm, to, up, fov_deg, focal, width, height):
+    """Camera record for a LookAt camera (camera.go:109-193), computed directly from the basis.
+
+    The reference recovers the same matrix through a polar decomposition and quaternion round trip,
+    which can differ in the last bit; callers that need the reference's exact matrix pass it in."""
+    f = np.asarray(frm, np.float32)
+    t = np.asarray(to, np.float32)
+    u = np.asarray(up, np.float32)
+    W = (f - t) / np.linalg.norm(f - t)
+    U = np.cross(u, W)
+    U = U / np.linalg.norm(U)
+    V = np.cross(U, W)
+    V = V / np.linalg.norm(V)
+    cam = np.zeros((), CAMERA_DTYPE)
+    M = np.eye(4, dtype=np.float32)
+    M[0, :3], M[1, :3], M[2, :3], M[3, :3] = U, V, W, f      # column-major storage: columns U, V, W, T
+    cam["M"] = M.reshape(16)
+    cam["tan_theta_focal"] = np.float32(math.tan(math.radians(fov_deg / 2.0)) * focal)
+    cam["aspect"] = np.float32(width / height)
+    cam["focal"] = np.float32(focal)
+    return cam
+
+
+class Renderer:
+    """GPU render passes over a scene description (see vermeer_amd.scenes.box_spheres_scene)."""
+
+    def __init__(self, desc, width, height, camera, device=0):
+        self._lib = _lib.load()
+        self.scene = Scene(device)
+        self.width, self.height = width, height
+        self._keep = []
+        for mesh in desc["meshes"]:
+            v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(-1, 3)
+            ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
+            nrm = mesh.get("normals")
+            nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+            sidx = mesh.get("shaderidx")
+            sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
+            sh = np.ascontiguousarray(mesh["shaders"], np.int32)
+            g = C.c_int32(-1)
+            check(self._lib.vgpu_scene_add_shaded_mesh(self.scene._h, ptr(v), len(v), ptr(ix), len(ix), ptr(nrm),
+                                                       ptr(sidx), ptr(sh), len(sh), C.byref(g)))
+            self.scene.geoms.append(None)
+        self.scene.pre_render()
+        mats = np.ascontiguousarray(desc["materials"], MATERIAL_DTYPE)
+        lights = np.zeros(len(desc["lights"]), TRI_LIGHT_DTYPE)
+        for i, L in enumerate(desc["lights"]):
+            for k in ("P0", "P1", "P2", "material", "samples", "geom"):
+                lights[i][k] = L[k]
+        cam = np.ascontiguousarray(camera, CAMERA_DTYPE)
+        h = C.c_void_p()
+        check(self._lib.vgpu_render_create(C.byref(h), self.scene._h, ptr(mats), len(mats), ptr(lights), len(lights),
+                                           ptr(cam), width, height))
+        self._h = h
+        self.film = np.zeros((height, width, 3), np.float32)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vgpu_render_destroy(self._h)
+            self._h = None
+        self.scene.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render_pass(self, scrambles, iteration, want_sample=False):
+        """One pass with host arrays (synchronous); updates self.film.  Returns (shadow rays, sample)."""
+        scr = np.ascontiguousarray(scrambles, np.uint64)
+        samp = np.zeros((self.height, self.width, 3), np.float32) if want_sample else None
+        n = C.c_uint64(0)
+        check(self._lib.vgpu_render_pass(self._h, 0, ptr(scr), iteration, ptr(self.film), ptr(samp), None, C.byref(n)))
+        return n.value, samp
+
+    def render_pass_device(self, d_scrambles, iteration, d_film, d_sample=0, stream=0):
+        """One pass on device buffers (ints: data pointers), enqueued on `stream`."""
+        check(self._lib.vgpu_render_pass(self._h, _lib.VGPU_TRACE_DEVICE_PTRS, C.c_void_p(d_scrambles), iteration,
+                                         C.c_void_p(d_film), C.c_void_p(d_sample or None), C.c_void_p(stream or None),
+                                         None))

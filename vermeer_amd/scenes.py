@@ -248,17 +248,8 @@ def moving_spheres(count=2000, nseg=25, nring=6, extent=20.0, rmin=0.3, rmax=1.0
 # ------------------------------------------------------------ shaded scenes
 
 # ShaderStd parameters (builtin/shader/std.go:25-47); has_* flags mark parameters that are set.
-MATERIAL_DTYPE = np.dtype([
-    ("emission", "<f4", 3), ("emission_strength", "<f4"), ("has_emission_colour", "<i4"), ("has_emission_strength", "<i4"),
-    ("diffuse", "<f4", 3), ("has_diffuse_colour", "<i4"),
-    ("diffuse_strength", "<f4"), ("has_diffuse_strength", "<i4"),
-    ("diffuse_roughness", "<f4"), ("has_diffuse_roughness", "<i4"),
-    ("spec1", "<f4", 3), ("has_spec1_colour", "<i4"),
-    ("spec1_strength", "<f4"), ("has_spec1_strength", "<i4"),
-    ("spec1_roughness", "<f4"), ("has_spec1_roughness", "<i4"),
-    ("ior", "<f4"), ("has_ior", "<i4"),
-    ("fresnel_model", "<i4"),
-])
+from ._lib import MATERIAL_DTYPE  # noqa: E402
+
 FRESNEL_DIELECTRIC = 1
 
 
