@@ -50,6 +50,7 @@ def load():
         "orc_scene_top_motion_accel": (C.c_int, [P, P, P]),
         "orc_trace": (C.c_int, [P, P, C.c_int64, C.c_int32, P, P, C.c_int32]),
         "orc_brute": (C.c_int, [P, P, C.c_int64, P, C.c_int32]),
+        "orc_set_normalize_ieee": (None, [C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -57,6 +58,11 @@ def load():
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def set_normalize_ieee(on):
+    """Vec3Normalize estimate: False = RSQRTSS (the reference on this CPU), True = IEEE 1/sqrt."""
+    load().orc_set_normalize_ieee(1 if on else 0)
 
 
 def _p(a):

@@ -83,6 +83,9 @@ void orc_ray_setup(const float* D, float* S_out, float* Dinv_out, int32_t* K_out
     K_out[0] = r.Kx; K_out[1] = r.Ky; K_out[2] = r.Kz;
 }
 
+// 0: RSQRTSS estimate in Vec3Normalize (the reference on this CPU); 1: IEEE 1/sqrt estimate.
+void orc_set_normalize_ieee(int32_t on) { normalize_ieee() = on != 0; }
+
 void* orc_scene_new() { return new Scene(); }
 void orc_scene_free(void* s) { delete (Scene*)s; }
 

@@ -45,6 +45,9 @@ static inline float Gamma(int32_t n) {
     return ne / (1.0f - ne);
 }
 
+// Normalize estimate mode (see Mesh::Normalize); process-wide.
+inline bool& normalize_ieee() { static bool v = false; return v; }
+
 struct Vec3 { float v[3]; float& operator[](int i) { return v[i]; } float operator[](int i) const { return v[i]; } };
 
 // math/vec3.go:52-58
@@ -619,11 +622,14 @@ struct Mesh {
         return true;
     }
 
-    // math/vec3_amd64.s:11-42 — RSQRTSS estimate + one Newton-Raphson step.
+    // math/vec3_amd64.s:11-42 — reciprocal-square-root estimate + one Newton-Raphson step.
+    // The estimate is RSQRTSS (12-bit, its bits differ between CPU vendors) in the default mode, or
+    // the IEEE value 1/sqrt(x) when normalize_ieee() is set: the same Newton step on an estimate
+    // every machine computes identically.
     static Vec3 Normalize(const Vec3& a) {
         float x2 = a[0] * a[0], y2 = a[1] * a[1], z2 = a[2] * a[2];
         float l2 = (y2 + x2) + z2;
-        float g = _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(l2)));
+        float g = normalize_ieee() ? 1.0f / std::sqrt(l2) : _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(l2)));
         float h = l2 * 0.5f;
         float gg = g * g;
         gg = gg * h;

@@ -19,9 +19,8 @@
 //
 // Compiled with -ffp-contract=off.  Float64 transcendentals (the reference
 // calls Go's float64 math package and rounds to float32) use the device's
-// double-precision library.  Vec3Normalize follows the reference's
-// reciprocal-square-root estimate plus one Newton-Raphson step
-// (math/vec3_amd64.s:11-42) on the device's rsqrt estimate.
+// double-precision library.  Vec3Normalize keeps the reference's Newton-Raphson
+// step (math/vec3_amd64.s:11-42) on the IEEE estimate 1/sqrt(x).
 
 #include "render_kernels.h"
 
@@ -98,11 +97,13 @@ __device__ __forceinline__ V3 mad(V3 a, V3 b, float s) { return v3(a.x + (b.x * 
 __device__ __forceinline__ float maxs(float x, float y) { return (x > y) ? x : y; }
 __device__ __forceinline__ float mins(float x, float y) { return (x < y) ? x : y; }
 __device__ __forceinline__ V3 from4(float4 a) { return v3(a.x, a.y, a.z); }
-// math/vec3_amd64.s:11-42: squared length (y2 + x2) + z2, rsqrt estimate, one Newton-Raphson step
+// math/vec3_amd64.s:11-42: squared length (y2 + x2) + z2, reciprocal-square-root estimate, one
+// Newton-Raphson step.  The reference's estimate is RSQRTSS, whose bits differ between CPU vendors;
+// here it is the IEEE value 1/sqrt(x), so the result is reproducible on any machine.
 __device__ __forceinline__ V3 normalize(V3 a) {
     const float x2 = a.x * a.x, y2 = a.y * a.y, z2 = a.z * a.z;
     const float l2 = (y2 + x2) + z2;
-    const float g = __builtin_amdgcn_rsqf(l2);
+    const float g = 1.0f / sqrtf(l2);
     float gg = g * g;
     gg = gg * (l2 * 0.5f);
     const float r = (1.5f - gg) * g;
