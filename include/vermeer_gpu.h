@@ -254,6 +254,47 @@ int vgpu_scene_get_trace_config(const vgpu_scene* scene, vgpu_trace_config* cfg)
 int vgpu_scene_autotune(vgpu_scene* scene, uint32_t flags, const vgpu_ray* d_rays, uint64_t n,
                         vgpu_hit* d_hits, void* stream, vgpu_trace_config* best_out, float* ms_out);
 
+/* ------------------------------------------------------------------------
+ * Path-guiding network.  No reference counterpart: the reference samples
+ * only its BSDFs (builtin/shader/bsdf/*.go).  A small MLP maps a hit
+ * position (normalised by the scene bounds) and shading normal, with a
+ * 12-frequency sin/cos encoding (144 features), through 4 hidden layers of
+ * 64 (ReLU) to 8 von Mises-Fisher lobes (mu, kappa, weight).  Weights are
+ * multiplied in bf16 on the matrix cores with fp32 accumulation; it is
+ * trained online with Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the loss
+ * (1/B) sum_i -(c_i/cbar) log q(d_i), cbar = mean c.  Parameters: 24,360
+ * floats in the order W1 (64x144, row = output), b1, W2, b2, W3, b3, W4, b4,
+ * W5 (40x64), b5; output k*5+0..2 = lobe direction (unnormalised),
+ * k*5+3 = log kappa (clamped to [-5,7]), k*5+4 = mixture logit.
+ * ------------------------------------------------------------------------ */
+typedef struct vgpu_guide vgpu_guide;
+
+#define VGPU_GUIDE_NPARAMS 24360
+#define VGPU_GUIDE_RAW 4        /* vgpu_guide_infer: write the 40 raw outputs */
+#define VGPU_GUIDE_NO_UPDATE 8  /* vgpu_guide_train_step: compute the gradient only */
+
+/* bounds: scene box (lo xyz, hi xyz) used to normalise positions; seed:
+ * weight initialisation; max_batch: largest training batch (a multiple of
+ * 1024, at most 2^22). */
+int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_t seed, uint32_t max_batch);
+int vgpu_guide_destroy(vgpu_guide* guide);
+/* Replaces the parameters (24,360 floats) and resets the Adam state. */
+int vgpu_guide_set_params(vgpu_guide* guide, const float* params);
+int vgpu_guide_get_params(vgpu_guide* guide, float* params);
+/* The mean gradient of the last training step (24,360 floats). */
+int vgpu_guide_get_grad(vgpu_guide* guide, float* grad);
+/* Evaluates n queries (8 floats each: position xyz, pad, normal xyz, pad).
+ * out: n x 40 floats, either 8 decoded lobes of (mu xyz, kappa, weight) or,
+ * with VGPU_GUIDE_RAW, the raw outputs in parameter order.  Host pointers
+ * and synchronous unless VGPU_TRACE_DEVICE_PTRS. */
+int vgpu_guide_infer(vgpu_guide* guide, uint32_t flags, const float* queries, uint64_t n, float* out, void* stream);
+/* One training step on n samples (12 floats each: position xyz, c, normal
+ * xyz, pad, unit direction xyz, pad); n a multiple of 1024 up to max_batch.
+ * Samples with c = 0 contribute no gradient; a batch whose c sum is 0 makes
+ * no update.  *loss_out (nullable, host mode only): the batch loss. */
+int vgpu_guide_train_step(vgpu_guide* guide, uint32_t flags, const float* samples, uint32_t n, float lr,
+                          void* stream, float* loss_out);
+
 #ifdef __cplusplus
 }
 #endif

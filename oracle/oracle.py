@@ -343,3 +343,93 @@ def oren_nayar_eval(lam, wi, wo, roughness):
     out = np.zeros(4, np.float32)
     lib.orc_oren_nayar_eval(lam, _p(a), _p(b), roughness, _p(out))
     return out
+
+
+# ------------------------------------------------------------ guiding network (oracle_guide.cpp)
+
+def _guide_sigs(lib):
+    if getattr(lib, "_guide_ready", False):
+        return lib
+    P = C.c_void_p
+    sig = {
+        "ogd_nparams": (C.c_int, []),
+        "ogd_forward": (None, [P, C.c_int64, P, P, P, C.c_int, P, P]),
+        "ogd_mix_pdf": (None, [P, C.c_int64, P, P]),
+        "ogd_nll_grad_raw": (None, [P, C.c_int64, P, P]),
+        "ogd_nll": (None, [P, C.c_int64, P, P]),
+        "ogd_loss_grad": (C.c_int, [P, C.c_int64, P, P, P, P, P, C.c_int, P, P]),
+        "ogd_adam": (None, [P, P, P, P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "ogd_bf16": (C.c_float, [C.c_float]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._guide_ready = True
+    return lib
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def guide_forward(params, pos, nrm, bounds, emulate=False):
+    """FP64 forward pass: (raw (n,40), lobes (n,8,5))."""
+    lib = _guide_sigs(load())
+    pos, nrm = _f32(pos, (-1, 3)), _f32(nrm, (-1, 3))
+    n = len(pos)
+    raw = np.empty((n, 40), np.float32)
+    lobes = np.empty((n, 8, 5), np.float32)
+    lib.ogd_forward(_p(_f32(params)), n, _p(pos), _p(nrm), _p(_f32(bounds)), int(emulate), _p(raw), _p(lobes))
+    return raw, lobes
+
+
+def guide_mix_pdf(raw, dirs):
+    lib = _guide_sigs(load())
+    raw, dirs = _f32(raw, (-1, 40)), _f32(dirs, (-1, 3))
+    out = np.empty(len(raw), np.float64)
+    lib.ogd_mix_pdf(_p(raw), len(raw), _p(dirs), _p(out))
+    return out
+
+
+def guide_nll(raw, dirs):
+    lib = _guide_sigs(load())
+    raw = np.ascontiguousarray(raw, np.float64).reshape(-1, 40)
+    dirs = _f32(dirs, (-1, 3))
+    out = np.empty(len(raw), np.float64)
+    lib.ogd_nll(_p(raw), len(raw), _p(dirs), _p(out))
+    return out
+
+
+def guide_nll_grad_raw(raw, dirs):
+    lib = _guide_sigs(load())
+    raw = np.ascontiguousarray(raw, np.float64).reshape(-1, 40)
+    dirs = _f32(dirs, (-1, 3))
+    out = np.empty_like(raw)
+    lib.ogd_nll_grad_raw(_p(raw), len(raw), _p(dirs), _p(out))
+    return out
+
+
+def guide_loss_grad(params, pos, nrm, dirs, c, bounds, emulate=False):
+    """Batch loss and mean gradient (float64, NPARAM); grad is None when sum c == 0."""
+    lib = _guide_sigs(load())
+    pos, nrm, dirs = _f32(pos, (-1, 3)), _f32(nrm, (-1, 3)), _f32(dirs, (-1, 3))
+    c = _f32(c)
+    loss = np.zeros(1, np.float64)
+    grad = np.empty(lib.ogd_nparams(), np.float64)
+    rc = lib.ogd_loss_grad(_p(_f32(params)), len(c), _p(pos), _p(nrm), _p(dirs), _p(c), _p(_f32(bounds)),
+                           int(emulate), _p(loss), _p(grad))
+    return float(loss[0]), (None if rc else grad)
+
+
+def guide_adam(params, m, v, grad, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in place on float32 arrays params, m, v."""
+    lib = _guide_sigs(load())
+    g = np.ascontiguousarray(grad, np.float64)
+    lib.ogd_adam(_p(params), _p(m), _p(v), _p(g), len(params), int(t), lr, b1, b2, eps)
+
+
+def bf16(x):
+    lib = _guide_sigs(load())
+    return np.array([lib.ogd_bf16(float(v)) for v in np.ravel(x)], np.float32).reshape(np.shape(x))

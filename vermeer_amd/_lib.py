@@ -23,6 +23,9 @@ VGPU_MAX_STACK = 128
 VGPU_TRACE_SHADOW = 1
 VGPU_TRACE_DEVICE_PTRS = 2
 VGPU_MESH_MOTION_REFERENCE_INDEXING = 1
+VGPU_GUIDE_RAW = 4
+VGPU_GUIDE_NO_UPDATE = 8
+VGPU_GUIDE_NPARAMS = 24360
 
 # vgpu_qbvh_node (128 bytes, qbvh.Node layout)
 NODE_DTYPE = np.dtype([("boxes", "<f4", 24), ("axis", "<u4", 3), ("children", "<i4", 4), ("parent", "<i4")])
@@ -66,6 +69,13 @@ SIGNATURES = [
     ("vgpu_render_create", C.c_int, [C.POINTER(_P), _P, _P, _U32, _P, _U32, _P, _U32, _U32]),
     ("vgpu_render_destroy", C.c_int, [_P]),
     ("vgpu_render_pass", C.c_int, [_P, _U32, _P, _U32, _P, _P, _P, C.POINTER(_U64)]),
+    ("vgpu_guide_create", C.c_int, [C.POINTER(_P), C.c_int, _P, _U64, _U32]),
+    ("vgpu_guide_destroy", C.c_int, [_P]),
+    ("vgpu_guide_set_params", C.c_int, [_P, _P]),
+    ("vgpu_guide_get_params", C.c_int, [_P, _P]),
+    ("vgpu_guide_get_grad", C.c_int, [_P, _P]),
+    ("vgpu_guide_infer", C.c_int, [_P, _U32, _P, _U64, _P, _P]),
+    ("vgpu_guide_train_step", C.c_int, [_P, _U32, _P, _U32, _F, _P, C.POINTER(_F)]),
 ]
 
 # vgpu_material / vgpu_tri_light / vgpu_camera layouts

@@ -12,8 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvermeer_gpu.so")
 
-SOURCES = ["trace.hip", "render.hip", "scene.cpp", "render.cpp", "qbvh_build.cpp"]
-HEADERS = ["trace_kernels.h", "render_kernels.h", "scene_internal.h", "qbvh_build.h"]
+SOURCES = ["trace.hip", "render.hip", "guide.hip", "scene.cpp", "render.cpp", "guide.cpp", "qbvh_build.cpp"]
+HEADERS = ["trace_kernels.h", "render_kernels.h", "guide_kernels.h", "guide_internal.h", "scene_internal.h", "qbvh_build.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

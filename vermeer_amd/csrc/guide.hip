@@ -1,0 +1,606 @@
+// Path-guiding network kernels for gfx950: bf16 matrix-core (MFMA 16x16x32)
+// inference and training with fp32 accumulation.
+//
+// Layout.  The network is evaluated "transposed": for each layer Y = W X,
+// the weights are the A operand (rows = outputs) and the activations of 16
+// samples are the B operand (columns = samples).  A 16x16 fp32 result tile
+// leaves lane l holding rows 4(l/16)..4(l/16)+3 of sample column l%16; two
+// such tiles (features 32j..32j+31) are exactly the 8 values per lane that the
+// next layer's B operand needs for its k-chunk j, provided the k-order of the
+// next layer's weight fragment is permuted to match (hid_feature below).  So
+// activations stay in registers from layer to layer, with no transpose.
+// Weight fragments are packed once per parameter update (k_pack) and staged
+// into LDS by every workgroup (50 KB forward, +32 KB transposed for backward).
+//
+// Training (one step over B samples): k_csum sums the sample weights c;
+// k_train runs forward, the loss gradient and the backward pass in registers,
+// writing every layer input X and output gradient dY in feature-major bf16;
+// k_dw forms the weight gradients dW = dY X^T as split-K MFMA GEMMs into fp32
+// partials; k_adam reduces the partials in a fixed order and applies Adam.
+
+#include "guide_kernels.h"
+
+namespace vgpu {
+namespace guide {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int F_L1 = 0, F_L2 = 20, F_L3 = 28, F_L4 = 36, F_L5 = 44;
+constexpr int FB_L5 = 0, FB_L4 = 8, FB_L3 = 16, FB_L2 = 24;
+static_assert(F_L5 + 6 == NF_FWD, "forward fragment count");
+static_assert(FB_L2 + 8 == NF_BWD, "backward fragment count");
+
+// Hidden feature held in k-slot i of lane group g for k-chunk j (the result-tile order).
+__device__ __forceinline__ int hid_feature(int j, int g, int i) {
+    return i < 4 ? 32 * j + 4 * g + i : 32 * j + 16 + 4 * g + (i - 4);
+}
+// Output-layer rows are permuted so that lane group g holds lobes 2g and 2g+1 in its
+// slots s = 4T + r (T = result tile, r = row in the lane's quad); slots 10, 11 are empty.
+__device__ __forceinline__ int out5_index(int g, int s) { return s < 10 ? 5 * (2 * g + s / 5) + s % 5 : -1; }
+
+__device__ __forceinline__ uint16_t bf16_bits(__bf16 b) { return __builtin_bit_cast(uint16_t, b); }
+__device__ __forceinline__ float bf16_val(__bf16 b) { return (float)b; }
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 frag(const u32x4* s, int f, int lane) {
+    return __builtin_bit_cast(bf16x8, s[f * 64 + lane]);
+}
+
+// ---------------------------------------------------------------- packing
+
+__device__ float pack_fwd(const float* P, int f, int lane, int i) {
+    const int r = lane & 15, g = lane >> 4;
+    if (f < F_L2) {
+        const int o = f / 5, j = f % 5, q = 32 * j + 8 * g + i;
+        return q < NENC ? P[OW1 + (16 * o + r) * NENC + q] : 0.0f;
+    }
+    if (f < F_L5) {
+        const int l = (f - F_L2) / 8, ff = (f - F_L2) % 8, o = ff / 2, j = ff % 2;
+        const int ow = l == 0 ? OW2 : (l == 1 ? OW3 : OW4);
+        return P[ow + (16 * o + r) * NH + hid_feature(j, g, i)];
+    }
+    const int ff = f - F_L5, T = ff / 2, j = ff % 2;
+    const int out = out5_index(r >> 2, 4 * T + (r & 3));
+    return out >= 0 ? P[OW5 + out * NH + hid_feature(j, g, i)] : 0.0f;
+}
+
+__device__ float pack_bwd(const float* P, int f, int lane, int i) {
+    const int r = lane & 15, g = lane >> 4;
+    const int o = (f % 8) / 2, j = f % 2, row = 16 * o + r;
+    if (f < FB_L4) {
+        const int out = out5_index(g, 8 * j + i);
+        return out >= 0 ? P[OW5 + out * NH + row] : 0.0f;
+    }
+    const int l = (f - FB_L4) / 8;
+    const int ow = l == 0 ? OW4 : (l == 1 ? OW3 : OW2);
+    return P[ow + hid_feature(j, g, i) * NH + row];
+}
+
+__global__ void k_pack(const float* __restrict__ P, u32x4* __restrict__ ffwd, u32x4* __restrict__ fbwd) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (NF_FWD + NF_BWD) * 64) return;
+    const bool fwd = t < NF_FWD * 64;
+    const int idx = fwd ? t : t - NF_FWD * 64;
+    const int f = idx >> 6, lane = idx & 63;
+    uint32_t h[8];
+    for (int i = 0; i < 8; i++) {
+        const float v = fwd ? pack_fwd(P, f, lane, i) : pack_bwd(P, f, lane, i);
+        h[i] = bf16_bits((__bf16)v);
+    }
+    u32x4 w;
+    w.x = h[0] | (h[1] << 16); w.y = h[2] | (h[3] << 16); w.z = h[4] | (h[5] << 16); w.w = h[6] | (h[7] << 16);
+    (fwd ? ffwd : fbwd)[idx] = w;
+}
+
+// ---------------------------------------------------------------- forward pieces
+
+__device__ __forceinline__ float sel6(const float x[6], int d) {
+    return d == 0 ? x[0] : d == 1 ? x[1] : d == 2 ? x[2] : d == 3 ? x[3] : d == 4 ? x[4] : x[5];
+}
+
+// Encoded features 32j + 8g .. +7 (canonical order: 24 d + 2 f (+1 for cos)).
+// sin(pi 2^f x) is evaluated on the exactly reduced argument r = 2^f x - 2 round(2^(f-1) x),
+// in revolutions (r / 2) for the hardware sine.
+__device__ __forceinline__ bf16x8 enc_chunk(const float x[6], int j, int g) {
+    bf16x8 b;
+    const int q0 = 32 * j + 8 * g;
+    if (q0 >= NENC) {
+        for (int i = 0; i < 8; i++) b[i] = (__bf16)0.0f;
+        return b;
+    }
+    const int d = q0 / 24, f0 = (q0 % 24) / 2;
+    const float xd = sel6(x, d);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float t = ldexpf(xd, f0 + k);
+        const float r = t - 2.0f * rintf(0.5f * t);
+        const float rev = 0.5f * r;
+        b[2 * k] = (__bf16)__builtin_amdgcn_sinf(rev);
+        b[2 * k + 1] = (__bf16)__builtin_amdgcn_cosf(rev);
+    }
+    return b;
+}
+
+__device__ __forceinline__ void normalise(const float4& p, const float4& n, const float* bounds, float x[6]) {
+    const float P[3] = {p.x, p.y, p.z};
+    for (int k = 0; k < 3; k++) {
+        const float lo = bounds[k], hi = bounds[3 + k];
+        const float c = 0.5f * (lo + hi), h = 0.5f * (hi - lo);
+        x[k] = (P[k] - c) / h;
+    }
+    x[3] = n.x; x[4] = n.y; x[5] = n.z;
+}
+
+template <int NT>
+__device__ __forceinline__ void bias_hidden(f32x4 (&acc)[4][NT], const float* b, int g) {
+#pragma unroll
+    for (int o = 0; o < 4; o++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float v = b[16 * o + 4 * g + r];
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[o][t][r] = v;
+        }
+}
+
+template <int NT>
+__device__ __forceinline__ void relu_pack(const f32x4 (&acc)[4][NT], bf16x8 (&X)[2][NT]) {
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const float v = acc[2 * j + (i >> 2)][t][i & 3];
+                X[j][t][i] = (__bf16)(v > 0.0f ? v : 0.0f);
+            }
+}
+
+template <int NT>
+__device__ __forceinline__ void layer_hidden(const u32x4* sF, int fbase, const float* bias, int g, int lane,
+                                             const bf16x8 (&Xin)[2][NT], f32x4 (&acc)[4][NT]) {
+    bias_hidden<NT>(acc, bias, g);
+#pragma unroll
+    for (int o = 0; o < 4; o++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const bf16x8 a = frag(sF, fbase + 2 * o + j, lane);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[o][t] = mfma(a, Xin[j][t], acc[o][t]);
+        }
+}
+
+template <int NT>
+__device__ __forceinline__ void layer_first(const u32x4* sF, const float* bias, int g, int lane, const float (&x)[NT][6],
+                                            f32x4 (&acc)[4][NT], uint16_t* X1, uint32_t ldb, const uint64_t (&s)[NT]) {
+    bias_hidden<NT>(acc, bias, g);
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        bf16x8 b[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            b[t] = enc_chunk(x[t], j, g);
+            if (X1) {
+                const int q0 = 32 * j + 8 * g;
+                if (q0 < NENC)
+                    for (int i = 0; i < 8; i++) X1[(uint64_t)(q0 + i) * ldb + s[t]] = bf16_bits(b[t][i]);
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < 4; o++) {
+            const bf16x8 a = frag(sF, F_L1 + 5 * o + j, lane);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[o][t] = mfma(a, b[t], acc[o][t]);
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void layer_out(const u32x4* sF, const float* b5, int g, int lane, const bf16x8 (&Xin)[2][NT],
+                                          f32x4 (&acc)[3][NT]) {
+#pragma unroll
+    for (int T = 0; T < 3; T++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int out = out5_index(g, 4 * T + r);
+            const float v = out >= 0 ? b5[out] : 0.0f;
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[T][t][r] = v;
+        }
+#pragma unroll
+    for (int T = 0; T < 3; T++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const bf16x8 a = frag(sF, F_L5 + 2 * T + j, lane);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[T][t] = mfma(a, Xin[j][t], acc[T][t]);
+        }
+}
+
+template <int NT>
+__device__ __forceinline__ void store_hidden(uint16_t* X, uint32_t ldb, const bf16x8 (&Xf)[2][NT], int g,
+                                             const uint64_t (&s)[NT]) {
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) X[(uint64_t)hid_feature(j, g, i) * ldb + s[t]] = bf16_bits(Xf[j][t][i]);
+}
+
+// ---------------------------------------------------------------- lobes
+
+struct LobeD {
+    float mu[3], kappa, logit, ulen;
+    bool kin;
+};
+
+__device__ __forceinline__ void decode_lobe(const float* v, LobeD& L) {
+    const float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    L.ulen = len;
+    if (len > 0.0f) {
+        const float inv = 1.0f / len;
+        L.mu[0] = v[0] * inv; L.mu[1] = v[1] * inv; L.mu[2] = v[2] * inv;
+    } else {
+        L.mu[0] = 0.0f; L.mu[1] = 0.0f; L.mu[2] = 1.0f;
+    }
+    L.kin = v[3] > KAPPA_RAW_MIN && v[3] < KAPPA_RAW_MAX;
+    L.kappa = expf(fminf(fmaxf(v[3], KAPPA_RAW_MIN), KAPPA_RAW_MAX));
+    L.logit = v[4];
+}
+
+__device__ __forceinline__ float quad_max(float v) {
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+__device__ __forceinline__ float quad_sum(float v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+
+// log of the vMF normalisation kappa / (2 pi (1 - exp(-2 kappa)))
+__device__ __forceinline__ float log_vmf_norm(float k) {
+    return logf(k) - 1.8378770664093453f - logf(-expm1f(-2.0f * k));
+}
+
+// 1/kappa - 2/(exp(2 kappa) - 1), with its series below 0.05
+__device__ __forceinline__ float dlog_norm(float k) {
+    if (k < 0.05f) return 1.0f - k * (1.0f / 3.0f) + k * k * k * (1.0f / 45.0f);
+    return 1.0f / k - 2.0f / expm1f(2.0f * k);
+}
+
+// ---------------------------------------------------------------- inference
+
+template <bool RAW>
+__global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict__ q, uint64_t n,
+                                               const uint32_t* __restrict__ n_dev, float* __restrict__ out) {
+    constexpr int NT = 4;
+    __shared__ u32x4 sF[NF_FWD * 64];
+    for (int k = threadIdx.x; k < NF_FWD * 64; k += 256) sF[k] = reinterpret_cast<const u32x4*>(net.frag_fwd)[k];
+    __syncthreads();
+    uint64_t nn = n;
+    if (n_dev) nn = min(nn, (uint64_t)*n_dev);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+    const uint64_t nw = (uint64_t)gridDim.x * 4;
+    const float* P = net.params;
+    for (uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * 64; base < nn; base += nw * 64) {
+        float x[NT][6];
+        uint64_t s[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            s[t] = base + 16 * t + c;
+            if (s[t] < nn) {
+                normalise(q[2 * s[t]], q[2 * s[t] + 1], net.bounds, x[t]);
+            } else {
+                for (int k = 0; k < 6; k++) x[t][k] = 0.0f;
+            }
+        }
+        f32x4 acc[4][NT];
+        bf16x8 X[2][NT];
+        layer_first<NT>(sF, P + OB1, g, lane, x, acc, nullptr, 0, s);
+        relu_pack<NT>(acc, X);
+        layer_hidden<NT>(sF, F_L2, P + OB2, g, lane, X, acc);
+        relu_pack<NT>(acc, X);
+        layer_hidden<NT>(sF, F_L3, P + OB3, g, lane, X, acc);
+        relu_pack<NT>(acc, X);
+        layer_hidden<NT>(sF, F_L4, P + OB4, g, lane, X, acc);
+        relu_pack<NT>(acc, X);
+        f32x4 o5[3][NT];
+        layer_out<NT>(sF, P + OB5, g, lane, X, o5);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            float v[12];
+#pragma unroll
+            for (int T = 0; T < 3; T++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[4 * T + r] = o5[T][t][r];
+            if (RAW) {
+                if (s[t] < nn)
+                    for (int sl = 0; sl < 10; sl++) out[s[t] * NOUT + out5_index(g, sl)] = v[sl];
+            } else {
+                LobeD A, B;
+                decode_lobe(v, A);
+                decode_lobe(v + 5, B);
+                const float m = quad_max(fmaxf(A.logit, B.logit));
+                const float ea = expf(A.logit - m), eb = expf(B.logit - m);
+                const float z = quad_sum(ea + eb);
+                if (s[t] < nn) {
+                    float* o = out + s[t] * NOUT + 10 * g;
+                    o[0] = A.mu[0]; o[1] = A.mu[1]; o[2] = A.mu[2]; o[3] = A.kappa; o[4] = ea / z;
+                    o[5] = B.mu[0]; o[6] = B.mu[1]; o[7] = B.mu[2]; o[8] = B.kappa; o[9] = eb / z;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- training
+
+__global__ __launch_bounds__(1024) void k_csum(const TrainSample* __restrict__ S, uint32_t B, float* __restrict__ csum) {
+    __shared__ float red[1024];
+    float a = 0.0f;
+    for (uint32_t i = threadIdx.x; i < B; i += 1024) a += S[i].c;
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) csum[0] = red[0];
+}
+
+// Masks the back-propagated tile with the layer input (ReLU derivative), stores it and repacks
+// it as the next B operand.
+template <int NT>
+__device__ __forceinline__ void mask_pack_store(const f32x4 (&acc)[4][NT], const bf16x8 (&Xf)[2][NT],
+                                                bf16x8 (&D)[2][NT], uint16_t* dY, uint32_t ldb, int g,
+                                                const uint64_t (&s)[NT]) {
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const float v = acc[2 * j + (i >> 2)][t][i & 3];
+                const __bf16 b = (__bf16)(bf16_val(Xf[j][t][i]) > 0.0f ? v : 0.0f);
+                D[j][t][i] = b;
+                dY[(uint64_t)hid_feature(j, g, i) * ldb + s[t]] = bf16_bits(b);
+            }
+}
+
+template <int NT>
+__device__ __forceinline__ void layer_back(const u32x4* sB, int fbase, int lane, const bf16x8 (&Din)[2][NT],
+                                           f32x4 (&acc)[4][NT]) {
+#pragma unroll
+    for (int o = 0; o < 4; o++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[o][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int o = 0; o < 4; o++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const bf16x8 a = frag(sB, fbase + 2 * o + j, lane);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[o][t] = mfma(a, Din[j][t], acc[o][t]);
+        }
+}
+
+// 8 waves x 32 samples per workgroup.
+__global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __restrict__ S, uint32_t B, TrainBuffers tb) {
+    constexpr int NT = 2;
+    extern __shared__ u32x4 lds[];
+    u32x4* sF = lds;
+    u32x4* sB = lds + NF_FWD * 64;
+    for (int k = threadIdx.x; k < NF_FWD * 64; k += 512) sF[k] = reinterpret_cast<const u32x4*>(net.frag_fwd)[k];
+    for (int k = threadIdx.x; k < NF_BWD * 64; k += 512) sB[k] = reinterpret_cast<const u32x4*>(net.frag_bwd)[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+    const uint64_t base = (uint64_t)blockIdx.x * 256 + wave * 32;
+    const float* P = net.params;
+    const uint32_t ldb = tb.ldb;
+    const float csum = tb.csum[0];
+    const float cscale = csum > 0.0f ? (float)B / csum : 0.0f;
+
+    float x[NT][6], dir[NT][3], cw[NT];
+    uint64_t s[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        s[t] = base + 16 * t + c;
+        const TrainSample& ts = S[s[t]];
+        normalise(make_float4(ts.px, ts.py, ts.pz, 0.0f), make_float4(ts.nx, ts.ny, ts.nz, 0.0f), net.bounds, x[t]);
+        dir[t][0] = ts.dx; dir[t][1] = ts.dy; dir[t][2] = ts.dz;
+        cw[t] = ts.c;
+    }
+    f32x4 acc[4][NT];
+    bf16x8 X2[2][NT], X3[2][NT], X4[2][NT], X5[2][NT];
+    layer_first<NT>(sF, P + OB1, g, lane, x, acc, tb.X[0], ldb, s);
+    relu_pack<NT>(acc, X2);
+    store_hidden<NT>(tb.X[1], ldb, X2, g, s);
+    layer_hidden<NT>(sF, F_L2, P + OB2, g, lane, X2, acc);
+    relu_pack<NT>(acc, X3);
+    store_hidden<NT>(tb.X[2], ldb, X3, g, s);
+    layer_hidden<NT>(sF, F_L3, P + OB3, g, lane, X3, acc);
+    relu_pack<NT>(acc, X4);
+    store_hidden<NT>(tb.X[3], ldb, X4, g, s);
+    layer_hidden<NT>(sF, F_L4, P + OB4, g, lane, X4, acc);
+    relu_pack<NT>(acc, X5);
+    store_hidden<NT>(tb.X[4], ldb, X5, g, s);
+    f32x4 o5[3][NT];
+    layer_out<NT>(sF, P + OB5, g, lane, X5, o5);
+
+    // loss gradient d(-(c/cbar) log q)/d raw for this lane's two lobes
+    bf16x8 G5[2][NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        float v[12];
+#pragma unroll
+        for (int T = 0; T < 3; T++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) v[4 * T + r] = o5[T][t][r];
+        LobeD L[2];
+        decode_lobe(v, L[0]);
+        decode_lobe(v + 5, L[1]);
+        const float mlog = quad_max(fmaxf(L[0].logit, L[1].logit));
+        const float e0 = expf(L[0].logit - mlog), e1 = expf(L[1].logit - mlog);
+        const float z = quad_sum(e0 + e1);
+        const float w[2] = {e0 / z, e1 / z};
+        float a[2], cosv[2];
+        for (int k = 0; k < 2; k++) {
+            cosv[k] = L[k].mu[0] * dir[t][0] + L[k].mu[1] * dir[t][1] + L[k].mu[2] * dir[t][2];
+            a[k] = logf(w[k]) + log_vmf_norm(L[k].kappa) + L[k].kappa * (cosv[k] - 1.0f);
+        }
+        const float am = quad_max(fmaxf(a[0], a[1]));
+        const float ex0 = expf(a[0] - am), ex1 = expf(a[1] - am);
+        const float se = quad_sum(ex0 + ex1);
+        const float logq = am + logf(se);
+        const float rr[2] = {ex0 / se, ex1 / se};
+        const float sc = cw[t] * cscale;
+        float gr[12];
+        for (int k = 0; k < 2; k++) {
+            const float kap = L[k].kappa;
+            gr[5 * k + 4] = sc * (w[k] - rr[k]);
+            const float dl = dlog_norm(kap) + (cosv[k] - 1.0f);
+            gr[5 * k + 3] = L[k].kin ? -sc * rr[k] * dl * kap : 0.0f;
+            for (int jj = 0; jj < 3; jj++)
+                gr[5 * k + jj] =
+                    L[k].ulen > 0.0f ? -sc * rr[k] * kap * (dir[t][jj] - cosv[k] * L[k].mu[jj]) / L[k].ulen : 0.0f;
+        }
+        gr[10] = 0.0f; gr[11] = 0.0f;
+        if (g == 0) tb.loss[s[t]] = -sc * logq;
+#pragma unroll
+        for (int i = 0; i < 8; i++) G5[0][t][i] = (__bf16)gr[i];
+#pragma unroll
+        for (int i = 0; i < 8; i++) G5[1][t][i] = (__bf16)(i < 4 ? gr[8 + i] : 0.0f);
+        for (int sl = 0; sl < 10; sl++)
+            tb.dY[4][(uint64_t)out5_index(g, sl) * ldb + s[t]] = bf16_bits(sl < 8 ? G5[0][t][sl] : G5[1][t][sl - 8]);
+    }
+
+    // backward: dX_l = W_l^T dY_l, dY_{l-1} = dX_l * (X_l > 0)
+    bf16x8 D[2][NT];
+    layer_back<NT>(sB, FB_L5, lane, G5, acc);
+    mask_pack_store<NT>(acc, X5, D, tb.dY[3], ldb, g, s);
+    layer_back<NT>(sB, FB_L4, lane, D, acc);
+    mask_pack_store<NT>(acc, X4, D, tb.dY[2], ldb, g, s);
+    layer_back<NT>(sB, FB_L3, lane, D, acc);
+    mask_pack_store<NT>(acc, X3, D, tb.dY[1], ldb, g, s);
+    layer_back<NT>(sB, FB_L2, lane, D, acc);
+    mask_pack_store<NT>(acc, X2, D, tb.dY[0], ldb, g, s);
+}
+
+// Weight-gradient partials: one wave per (16x16 tile of some dW_l, split of KSPLIT samples).
+constexpr int NJOB = 4 * 9 + 3 * 16 + 3 * 4;
+
+__global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t job = w % NJOB, split = w / NJOB;
+    if (split >= nsplit) return;
+    int l, ot, it, nin, nout;
+    if (job < 36) { l = 0; ot = job / 9; it = job % 9; nin = NENC; nout = NH; }
+    else if (job < 84) { l = 1 + (job - 36) / 16; ot = ((job - 36) % 16) / 4; it = (job - 36) % 4; nin = NH; nout = NH; }
+    else { l = 4; ot = (job - 84) / 4; it = (job - 84) % 4; nin = NH; nout = NOUT; }
+    const uint32_t ldb = tb.ldb;
+    const uint16_t* A = tb.dY[l] + (uint64_t)(16 * ot + c) * ldb;
+    const uint16_t* Bp = tb.X[l] + (uint64_t)(16 * it + c) * ldb;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bs = 0.0f;
+    const uint64_t s_end = (uint64_t)(split + 1) * KSPLIT;
+    for (uint64_t s0 = (uint64_t)split * KSPLIT; s0 < s_end; s0 += 32) {
+        const u32x4 av = *reinterpret_cast<const u32x4*>(A + s0 + 8 * g);
+        const u32x4 bv = *reinterpret_cast<const u32x4*>(Bp + s0 + 8 * g);
+        const bf16x8 a = __builtin_bit_cast(bf16x8, av);
+        acc = mfma(a, __builtin_bit_cast(bf16x8, bv), acc);
+        if (it == 0)
+            for (int i = 0; i < 8; i++) bs += bf16_val(a[i]);
+    }
+    const int ow[5] = {OW1, OW2, OW3, OW4, OW5}, ob[5] = {OB1, OB2, OB3, OB4, OB5};
+    float* Pp = tb.partial + (uint64_t)split * NPARAM;
+    for (int r = 0; r < 4; r++) {
+        const int out = 16 * ot + 4 * g + r;
+        if (out < nout) Pp[ow[l] + out * nin + 16 * it + c] = acc[r];
+    }
+    if (it == 0) {
+        bs = quad_sum(bs);
+        if (g == 0 && 16 * ot + c < nout) Pp[ob[l] + 16 * ot + c] = bs;
+    }
+}
+
+__global__ void k_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
+                       const float* __restrict__ partial, uint32_t nsplit, float invB, float lr,
+                       const float* __restrict__ csum, const int32_t* __restrict__ step, int update) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NPARAM) return;
+    float gs = 0.0f;
+    for (uint32_t s = 0; s < nsplit; s++) gs += partial[(uint64_t)s * NPARAM + p];
+    const float gr = gs * invB;
+    grad[p] = gr;
+    if (!update || !(csum[0] > 0.0f)) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const int t = step[0] + 1;
+    const float mm = b1 * m[p] + (1.0f - b1) * gr;
+    const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
+    m[p] = mm;
+    v[p] = vv;
+    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    P[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+__global__ void k_step(int32_t* step, const float* csum) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && csum[0] > 0.0f) step[0] += 1;
+}
+
+}  // namespace
+
+hipError_t launch_pack(const float* params, uint4* ffwd, uint4* fbwd, hipStream_t st) {
+    const int total = (NF_FWD + NF_BWD) * 64;
+    hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), dim3(256), 0, st, params, (u32x4*)ffwd, (u32x4*)fbwd);
+    return hipGetLastError();
+}
+
+hipError_t launch_infer(const Net& net, const float4* queries, uint64_t n, const uint32_t* n_dev, float* out, bool raw,
+                        int num_cus, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    const uint64_t cap = (uint64_t)num_cus * 3;
+    if (blocks > cap) blocks = cap;
+    if (raw)
+        hipLaunchKernelGGL(k_infer<true>, dim3((unsigned)blocks), dim3(256), 0, st, net, queries, n, n_dev, out);
+    else
+        hipLaunchKernelGGL(k_infer<false>, dim3((unsigned)blocks), dim3(256), 0, st, net, queries, n, n_dev, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_train(const Net& net, float* params, float* m, float* v, const TrainSample* samples, uint32_t B,
+                        float lr, bool update, const TrainBuffers& tb, hipStream_t st) {
+    if (B == 0 || B % KSPLIT != 0 || B > tb.ldb) return hipErrorInvalidValue;
+    static bool attr_set = false;
+    const size_t lds = (size_t)(NF_FWD + NF_BWD) * 64 * 16;
+    if (!attr_set) {
+        const hipError_t e = hipFuncSetAttribute((const void*)k_train, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_csum, dim3(1), dim3(1024), 0, st, samples, B, tb.csum);
+    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb);
+    const uint32_t nsplit = B / KSPLIT;
+    const uint32_t waves = NJOB * nsplit;
+    hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsplit);
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, params, m, v, tb.grad, tb.partial, nsplit,
+                       1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0);
+    if (update) {
+        hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, st, tb.step, tb.csum);
+        hipLaunchKernelGGL(k_pack, dim3(((NF_FWD + NF_BWD) * 64 + 255) / 256), dim3(256), 0, st, (const float*)params,
+                           (u32x4*)net.frag_fwd, (u32x4*)net.frag_bwd);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace guide
+}  // namespace vgpu

@@ -1,0 +1,69 @@
+// Internal interface of the path-guiding network kernels (guide.hip).
+//
+// Network: 144 encoded inputs -> 4 x 64 hidden (ReLU) -> 40 outputs = 8 von
+// Mises-Fisher lobes (u[3], kappa_raw, logit).  Canonical parameter order:
+// W1 (64x144, row = output), b1, W2, b2, W3, b3, W4, b4, W5 (40x64), b5.
+// Weights are multiplied in bf16 on the matrix cores with fp32 accumulation;
+// the fp32 master copy, the Adam moments and the biases stay in fp32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace vgpu {
+namespace guide {
+
+constexpr int NENC = 144, NH = 64, NOUT = 40, NLOBE = 8;
+constexpr int OW1 = 0, OB1 = OW1 + NH * NENC, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
+              OB3 = OW3 + NH * NH, OW4 = OB3 + NH, OB4 = OW4 + NH * NH, OW5 = OB4 + NH, OB5 = OW5 + NOUT * NH,
+              NPARAM = OB5 + NOUT;
+static_assert(NPARAM == 24360, "parameter count");
+
+// bf16 operand fragments of the 16x16x32 MFMA, 64 lanes x 16 bytes each.
+constexpr int NF_FWD = 4 * 5 + 3 * 4 * 2 + 3 * 2;   // W1 (4 out tiles x 5 k-chunks), W2..W4, W5 (3 x 2)
+constexpr int NF_BWD = 4 * 2 * 4;                   // W5^T, W4^T, W3^T, W2^T (4 in tiles x 2 k-chunks)
+constexpr int KSPLIT = 1024;                        // samples per weight-gradient partial
+constexpr float KAPPA_RAW_MIN = -5.0f, KAPPA_RAW_MAX = 7.0f;
+
+// One training sample: hit position + c, shading normal, sampled direction (unit).
+struct alignas(16) TrainSample {
+    float px, py, pz, c;
+    float nx, ny, nz, pad0;
+    float dx, dy, dz, pad1;
+};
+static_assert(sizeof(TrainSample) == 48, "TrainSample is 48 bytes");
+
+struct Net {
+    const float* params;       // fp32 master (NPARAM)
+    const uint4* frag_fwd;     // NF_FWD x 64
+    const uint4* frag_bwd;     // NF_BWD x 64
+    float bounds[6];           // scene box used to normalise positions
+};
+
+struct TrainBuffers {
+    uint16_t* X[5];     // layer inputs, feature-major [features][ldb] bf16: 144, 64, 64, 64, 64
+    uint16_t* dY[5];    // output gradients, feature-major bf16: 64, 64, 64, 64, 48 (rows 40..47 zero)
+    float* partial;     // [nsplit][NPARAM]
+    float* loss;        // per sample
+    float* csum;        // [0]: sum of c over the batch
+    int32_t* step;      // Adam step count (device)
+    float* grad;        // last mean gradient (NPARAM)
+    uint32_t ldb;       // row stride (samples) of X / dY
+};
+
+// Rebuilds the bf16 fragments from the fp32 master copy.
+hipError_t launch_pack(const float* params, uint4* frag_fwd, uint4* frag_bwd, hipStream_t st);
+
+// Inference on n queries (2 float4 each: position, normal).  When n_dev is not null the
+// count is min(n, *n_dev).  raw = true writes the 40 raw outputs in canonical order,
+// otherwise 8 decoded lobes of (mu[3], kappa, weight).
+hipError_t launch_infer(const Net& net, const float4* queries, uint64_t n, const uint32_t* n_dev, float* out, bool raw,
+                        int num_cus, hipStream_t st);
+
+// One training step on a batch of B samples (B a multiple of KSPLIT).  update = false computes
+// the mean gradient only.
+hipError_t launch_train(const Net& net, float* params, float* adam_m, float* adam_v, const TrainSample* samples,
+                        uint32_t B, float lr, bool update, const TrainBuffers& tb, hipStream_t st);
+
+}  // namespace guide
+}  // namespace vgpu
