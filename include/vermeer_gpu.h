@@ -295,6 +295,49 @@ int vgpu_guide_infer(vgpu_guide* guide, uint32_t flags, const float* queries, ui
 int vgpu_guide_train_step(vgpu_guide* guide, uint32_t flags, const float* samples, uint32_t n, float lr,
                           void* stream, float* loss_out);
 
+/* ------------------------------------------------------------------------
+ * Path mode.  No reference counterpart: core.Render shades one camera hit
+ * with direct light (core/render.go:66-137, builtin/shader/std.go:77-296).
+ * Path mode traces one path per pixel per pass, up to max_depth segments,
+ * over Lambertian surfaces with emission: next-event estimation to every
+ * emissive triangle (chosen by area x luminance) and one-sample MIS between
+ * cosine-weighted BSDF sampling and, when a guide is attached, the guide's
+ * vMF mixture, p = (1-alpha) cos/pi + alpha q; NEE and the mixture are
+ * combined by the balance heuristic.  Each pass adds its radiance to an
+ * FP64 accumulation buffer (the image is accum / passes).
+ * ------------------------------------------------------------------------ */
+typedef struct vgpu_pt vgpu_pt;
+
+typedef struct vgpu_pt_material {
+    float albedo[3];     /* Lambertian reflectance, each in [0,1] */
+    float emission[3];   /* emitted radiance */
+    int32_t one_sided;   /* 1: emits from the front face only (v0, v1, v2 counter-clockwise) */
+    int32_t pad;
+} vgpu_pt_material;
+
+#define VGPU_PT_NO_TRAIN 16   /* vgpu_pt_pass: use the guide without training it */
+
+/* Scene meshes refer to materials through their shader tables
+ * (vgpu_scene_add_shaded_mesh).  camera: as for vgpu_render_create. */
+int vgpu_pt_create(vgpu_pt** out, vgpu_scene* scene, const vgpu_pt_material* materials, uint32_t nmaterials,
+                   const vgpu_camera* camera, uint32_t width, uint32_t height, uint32_t max_depth, uint64_t seed);
+int vgpu_pt_destroy(vgpu_pt* pt);
+/* Attaches a guide (NULL detaches: plain BSDF sampling).  alpha in (0,1):
+ * probability of sampling the guide.  With train_steps > 0, each pass
+ * records the paths of one pixel in record_stride, turns their vertices
+ * into training samples (c = luminance of the incident radiance estimate x
+ * cos / sampling pdf) and runs train_steps Adam steps on batches of `batch`
+ * samples drawn from them. */
+int vgpu_pt_set_guide(vgpu_pt* pt, vgpu_guide* guide, float alpha, uint32_t record_stride, uint32_t train_steps,
+                      uint32_t batch, float lr);
+/* One pass (pass index selects the random numbers).  accum: width*height*3
+ * doubles, updated in place; device memory with VGPU_TRACE_DEVICE_PTRS
+ * (enqueued on stream), else host memory (synchronous). */
+int vgpu_pt_pass(vgpu_pt* pt, uint32_t flags, uint32_t pass, double* accum, void* stream);
+/* Rays traced since creation (closest + shadow) and the training samples of
+ * the last pass.  Synchronises the device. */
+int vgpu_pt_stats(vgpu_pt* pt, uint64_t* rays, uint32_t* train_samples);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@ VGPU_MESH_MOTION_REFERENCE_INDEXING = 1
 VGPU_GUIDE_RAW = 4
 VGPU_GUIDE_NO_UPDATE = 8
 VGPU_GUIDE_NPARAMS = 24360
+VGPU_PT_NO_TRAIN = 16
 
 # vgpu_qbvh_node (128 bytes, qbvh.Node layout)
 NODE_DTYPE = np.dtype([("boxes", "<f4", 24), ("axis", "<u4", 3), ("children", "<i4", 4), ("parent", "<i4")])
@@ -76,6 +77,11 @@ SIGNATURES = [
     ("vgpu_guide_get_grad", C.c_int, [_P, _P]),
     ("vgpu_guide_infer", C.c_int, [_P, _U32, _P, _U64, _P, _P]),
     ("vgpu_guide_train_step", C.c_int, [_P, _U32, _P, _U32, _F, _P, C.POINTER(_F)]),
+    ("vgpu_pt_create", C.c_int, [C.POINTER(_P), _P, _P, _U32, _P, _U32, _U32, _U32, _U64]),
+    ("vgpu_pt_destroy", C.c_int, [_P]),
+    ("vgpu_pt_set_guide", C.c_int, [_P, _P, _F, _U32, _U32, _U32, _F]),
+    ("vgpu_pt_pass", C.c_int, [_P, _U32, _U32, _P, _P]),
+    ("vgpu_pt_stats", C.c_int, [_P, C.POINTER(_U64), C.POINTER(_U32)]),
 ]
 
 # vgpu_material / vgpu_tri_light / vgpu_camera layouts

@@ -4,6 +4,7 @@ Every source is compiled with -ffp-contract=off: the traversal must round
 each float operation exactly as the reference's unfused SSE code, and the
 builder's box arithmetic likewise.  Packed-FP32 instructions are disabled.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -12,8 +13,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvermeer_gpu.so")
 
-SOURCES = ["trace.hip", "render.hip", "guide.hip", "scene.cpp", "render.cpp", "guide.cpp", "qbvh_build.cpp"]
-HEADERS = ["trace_kernels.h", "render_kernels.h", "guide_kernels.h", "guide_internal.h", "scene_internal.h", "qbvh_build.h"]
+SOURCES = ["trace.hip", "render.hip", "guide.hip", "pathtrace.hip", "scene.cpp", "render.cpp", "guide.cpp",
+           "pathtrace.cpp", "qbvh_build.cpp"]
+HEADERS = ["trace_kernels.h", "render_kernels.h", "guide_kernels.h", "guide_internal.h", "pathtrace_kernels.h",
+           "scene_internal.h", "qbvh_build.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
@@ -38,15 +41,18 @@ def build(verbose=False):
     """Compiles all sources and links LIB. Always rebuilds."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, src + ".o")
-        cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+        cmds.append([HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", path, "-o", obj])
         objs.append(obj)
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd), flush=True)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"]
     if verbose:

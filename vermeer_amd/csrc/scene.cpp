@@ -192,13 +192,14 @@ int check_config(const vgpu_trace_config& c) {
 }
 
 int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,
-                  int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg) {
+                  int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg, const uint32_t* n_dev) {
     vgpu::TraceLaunch L{cfg.block, cfg.lds_stack, cfg.waves_per_cu};
     const uint64_t threads = vgpu::trace_grid_threads(L, n, s->num_cus);
     int rc = ensure_spill(s, threads);
     if (rc) return rc;
     hipError_t e = vgpu::launch_trace(s->view, (flags & VGPU_TRACE_SHADOW) != 0, s->any_motion, L, d_rays, n, d_hits,
-                                      d_geoms, d_counters, s->d_spT, s->d_spN, s->spill_threads, s->d_err, s->num_cus, st);
+                                      d_geoms, d_counters, s->d_spT, s->d_spN, s->spill_threads, s->d_err, s->num_cus, st,
+                                      n_dev);
     if (e != hipSuccess) return fail(VGPU_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
     return VGPU_OK;
 }

@@ -139,8 +139,9 @@ struct DeviceGuard {
 
 // Ensures the spill area covers `threads` traversal threads.
 int ensure_spill(vgpu_scene* s, uint64_t threads);
-// Enqueues one traversal (device pointers) with configuration cfg.
+// Enqueues one traversal (device pointers) with configuration cfg.  n_dev (nullable): device-resident
+// count of rays to trace, at most n.
 int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,
-                  int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg);
+                  int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg, const uint32_t* n_dev = nullptr);
 
 }  // namespace vgpu_detail

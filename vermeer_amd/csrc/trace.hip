@@ -208,7 +208,8 @@ template <int BLOCK, int LDS, bool SHADOW, bool MOTION, bool COUNTERS, bool EXT 
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __restrict__ rays, uint64_t n,
                                                  int4* __restrict__ hits, int32_t* __restrict__ geoms_out,
                                                  int4* __restrict__ counters, float* __restrict__ spT,
-                                                 int32_t* __restrict__ spN, int32_t* __restrict__ err) {
+                                                 int32_t* __restrict__ spN, int32_t* __restrict__ err,
+                                                 const uint32_t* __restrict__ n_dev) {
     __shared__ float sT[LDS * BLOCK];
     __shared__ int32_t sN[LDS * BLOCK];
     const int tid = threadIdx.x;
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
     st.sT = sT + tid; st.sN = sN + tid;
     st.gT = spT + gtid; st.gN = spN + gtid; st.gstride = nthr;
 
+    if (n_dev) n = min(n, (uint64_t)*n_dev);   // device-resident count (compacted queues)
     for (uint64_t ri = gtid; ri < n; ri += nthr) {
         const float4 ra = rays[2 * ri], rb = rays[2 * ri + 1];
         RayS r;
@@ -354,32 +356,35 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
 
 template <int BLOCK, int LDS, bool SHADOW, bool MOTION>
 hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
-                      float* spT, int32_t* spN, uint64_t grid, int32_t* err, hipStream_t stream) {
+                      float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev, hipStream_t stream) {
     if (counters)
         hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, MOTION, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err);
+                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err, n_dev);
     else
         hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, MOTION, false>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err);
+                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err, n_dev);
     return hipGetLastError();
 }
 
 template <int BLOCK, int LDS>
 hipError_t launch_mode(const SceneView& sv, bool shadow, bool motion, const void* rays, uint64_t n, void* hits,
                        int32_t* geoms, int32_t* counters, float* spT, int32_t* spN, uint64_t grid, int32_t* err,
-                       hipStream_t stream) {
+                       const uint32_t* n_dev, hipStream_t stream) {
     if (shadow) {
-        if (motion) return launch_cfg<BLOCK, LDS, true, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, stream);
-        return launch_cfg<BLOCK, LDS, true, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, stream);
+        if (motion)
+            return launch_cfg<BLOCK, LDS, true, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
+        return launch_cfg<BLOCK, LDS, true, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
     }
-    if (motion) return launch_cfg<BLOCK, LDS, false, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, stream);
-    return launch_cfg<BLOCK, LDS, false, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, stream);
+    if (motion)
+        return launch_cfg<BLOCK, LDS, false, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
+    return launch_cfg<BLOCK, LDS, false, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
 }
 
 }  // namespace
 
 hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, uint64_t n, void* hits32, float* spT,
-                            int32_t* spN, uint64_t spill_threads, int32_t* err, int num_cus, hipStream_t stream) {
+                            int32_t* spN, uint64_t spill_threads, int32_t* err, int num_cus, hipStream_t stream,
+                            const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
     const TraceLaunch cfg{256, 16, 16};
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
@@ -387,10 +392,10 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
     const unsigned grid = (unsigned)(threads / cfg.block);
     if (motion)
         hipLaunchKernelGGL((k_trace<256, 16, false, true, false, true>), dim3(grid), dim3(256), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err);
+                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev);
     else
         hipLaunchKernelGGL((k_trace<256, 16, false, false, false, true>), dim3(grid), dim3(256), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err);
+                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev);
     return hipGetLastError();
 }
 
@@ -405,14 +410,14 @@ uint64_t trace_grid_threads(const TraceLaunch& cfg, uint64_t n, int num_cus) {
 
 hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const TraceLaunch& cfg, const void* rays, uint64_t n,
                         void* hits, int32_t* geoms, int32_t* counters, float* spT, int32_t* spN, uint64_t spill_threads,
-                        int32_t* err, int num_cus, hipStream_t stream) {
+                        int32_t* err, int num_cus, hipStream_t stream, const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
     const uint64_t grid = threads / cfg.block;
 #define VGPU_CASE(B, L)                                                                                         \
     if (cfg.block == B && cfg.lds_depth == L)                                                                   \
-        return launch_mode<B, L>(sv, shadow, motion, rays, n, hits, geoms, counters, spT, spN, grid, err, stream);
+        return launch_mode<B, L>(sv, shadow, motion, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
     VGPU_CASE(64, 8) VGPU_CASE(64, 16) VGPU_CASE(64, 24) VGPU_CASE(64, 32)
     VGPU_CASE(128, 8) VGPU_CASE(128, 16) VGPU_CASE(128, 24) VGPU_CASE(128, 32)
     VGPU_CASE(256, 8) VGPU_CASE(256, 16) VGPU_CASE(256, 24) VGPU_CASE(256, 32)

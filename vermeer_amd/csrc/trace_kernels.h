@@ -58,16 +58,19 @@ struct TraceLaunch {
 
 // Enqueues one traversal over n rays (device pointers).  spill_T / spill_N
 // hold (VGPU_MAX_STACK - lds_depth) entries per launched thread; the launch
-// uses at most spill_threads threads.  Returns hipSuccess or the launch error.
+// uses at most spill_threads threads.  n_dev (nullable): a device-resident
+// ray count; the kernel traces min(n, *n_dev) rays (n sizes the grid).
+// Returns hipSuccess or the launch error.
 hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const TraceLaunch& cfg,
                         const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
                         float* spill_T, int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag,
-                        int num_cus, hipStream_t stream);
+                        int num_cus, hipStream_t stream, const uint32_t* n_dev = nullptr);
 
 // Closest hit with 32-byte records {prim, geom, t, U, V, W, 0, 0} (U, V, W: barycentric weights of the
 // triangle's three vertices), for the shading kernels.  Fixed configuration (block 256, LDS stack 16).
 hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, uint64_t n, void* hits32, float* spill_T,
-                            int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag, int num_cus, hipStream_t stream);
+                            int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag, int num_cus, hipStream_t stream,
+                            const uint32_t* n_dev = nullptr);
 
 // Threads the launch above will use (so the caller can size the spill area).
 uint64_t trace_grid_threads(const TraceLaunch& cfg, uint64_t n, int num_cus);

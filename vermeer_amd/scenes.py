@@ -375,3 +375,192 @@ def box_spheres_scene(seed=3, nseg=64, nring=32, nspheres=8, amplitude=0.05):
 def pixel_scrambles(width, height, seed=99):
     """6 uint64 per pixel: lensU, lensV, time, lambda, scramble[0], scramble[1] (core/render.go:166-176)."""
     return rng(seed).integers(0, np.iinfo(np.uint64).max, (height, width, 6), dtype=np.uint64, endpoint=True)
+
+
+# --------------------------------------------------------------------- path mode: the occluded-light room
+
+PT_MATERIAL_DTYPE = np.dtype([("albedo", "<f4", 3), ("emission", "<f4", 3), ("one_sided", "<i4"), ("pad", "<i4")])
+
+
+def pt_material(albedo=(0.0, 0.0, 0.0), emission=(0.0, 0.0, 0.0), one_sided=False):
+    m = np.zeros((), PT_MATERIAL_DTYPE)
+    m["albedo"] = albedo
+    m["emission"] = emission
+    m["one_sided"] = int(one_sided)
+    return m
+
+
+def _grid_wall(origin, eu, ev, nu, nv, inward, amp, seed, window, hole=None):
+    """nu x nv quads spanning origin + [0,1] eu + [0,1] ev, displaced along `inward` by value noise of
+    amplitude amp times window(point) (0 at the wall border); quads whose centre lies in `hole`
+    (a predicate on the centre) are left out.  Triangles are wound to face `inward`."""
+    o, eu, ev, inward = (np.asarray(a, np.float64) for a in (origin, eu, ev, inward))
+    i, j = np.meshgrid(np.arange(nu + 1), np.arange(nv + 1), indexing="ij")
+    P = o + (i[..., None] / nu) * eu + (j[..., None] / nv) * ev
+    P = P.reshape(-1, 3)
+    d = amp * (2.0 * value_noise(P, seed, freq=2.0) - 1.0) * window(P)
+    P = P + d[:, None] * inward
+    ci, cj = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    ci, cj = ci.reshape(-1), cj.reshape(-1)
+    if hole is not None:
+        ctr = o + ((ci[:, None] + 0.5) / nu) * eu + ((cj[:, None] + 0.5) / nv) * ev
+        keep = ~hole(ctr)
+        ci, cj = ci[keep], cj[keep]
+    a = ci * (nv + 1) + cj
+    b = (ci + 1) * (nv + 1) + cj
+    c = (ci + 1) * (nv + 1) + cj + 1
+    dd = ci * (nv + 1) + cj + 1
+    f = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, dd], 1)])
+    if np.dot(np.cross(eu, ev), inward) < 0:
+        f = f[:, ::-1]
+    return P, f
+
+
+def _unit_box_tris():
+    corners = np.array([[x, y, z] for z in (-1, 1) for y in (-1, 1) for x in (-1, 1)], np.float64)
+    quads = [(0, 2, 6, 4), (1, 3, 7, 5), (0, 1, 5, 4), (2, 3, 7, 6), (0, 1, 3, 2), (4, 5, 7, 6)]
+    tris = []
+    for q in quads:
+        for t in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
+            p = corners[list(t)]
+            n = np.cross(p[1] - p[0], p[2] - p[0])
+            if np.dot(n, p.mean(0)) < 0:
+                t = (t[0], t[2], t[1])
+            tris.append(t)
+    return corners, np.array(tris, np.int64)
+
+
+def _random_rotations(g, n):
+    q = g.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+                     np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+                     np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+ROOM_SLIT = (-1.2, 1.2, 2.8, 3.2)          # z0, z1, y0, y1 of the slit in the x = +4 wall
+ROOM_ALCOVE = (4.0, 4.8, 2.4, 3.6, -1.6, 1.6)
+
+
+def guided_room_scene(seed=11, res=20, nboxes=66667, emission=2000.0, amplitude=0.03):
+    """The occluded-light room: a sealed 8 x 4 x 8 room whose six walls are res-per-metre grids of
+    noise-displaced quads (res 20: 204,000 triangles), a slit of 2.4 x 0.4 in the x = +4 wall opening
+    into an alcove behind it, and `nboxes` randomly rotated boxes of 12 triangles (66,667: 800,004
+    triangles) cluttering the lower 1.5 m.  The only light is a 2.0 x 0.05 rectangle inside the alcove
+    facing away from the slit (one-sided), so light reaches the room only after bouncing off the
+    alcove's walls.  Materials are Lambertian.
+
+    Returns a dict: meshes (main mesh, light mesh), materials (PT_MATERIAL_DTYPE), camera."""
+    g = rng(seed)
+    z0, z1, y0, y1 = ROOM_SLIT
+    ax0, ax1, ay0, ay1, az0, az1 = ROOM_ALCOVE
+    margin = 0.3
+
+    def border_window(lo, hi):
+        lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+
+        def w(P):
+            d = np.full(len(P), np.inf)
+            for k in range(3):
+                if hi[k] > lo[k]:
+                    d = np.minimum(d, np.minimum(P[:, k] - lo[k], hi[k] - P[:, k]))
+            t = np.clip(d / margin, 0.0, 1.0)
+            return t * t * (3 - 2 * t)
+        return w
+
+    def slit_wall_window(P):
+        w = border_window((4, 0, -4), (4, 4, 4))(P)
+        dy = np.maximum(np.maximum(ay0 - P[:, 1], P[:, 1] - ay1), 0.0)
+        dz = np.maximum(np.maximum(az0 - P[:, 2], P[:, 2] - az1), 0.0)
+        t = np.clip(np.hypot(dy, dz) / margin, 0.0, 1.0)
+        return w * t * t * (3 - 2 * t)
+
+    def slit_hole(C):
+        return (C[:, 2] > z0) & (C[:, 2] < z1) & (C[:, 1] > y0) & (C[:, 1] < y1)
+
+    n8, n4 = 8 * res, 4 * res
+    walls = [  # origin, eu, ev, nu, nv, inward, shader, window, hole
+        ((-4, 0, -4), (8, 0, 0), (0, 0, 8), n8, n8, (0, 1, 0), 1, border_window((-4, 0, -4), (4, 0, 4)), None),
+        ((-4, 4, -4), (8, 0, 0), (0, 0, 8), n8, n8, (0, -1, 0), 2, border_window((-4, 4, -4), (4, 4, 4)), None),
+        ((-4, 0, -4), (0, 0, 8), (0, 4, 0), n8, n4, (1, 0, 0), 0, border_window((-4, 0, -4), (-4, 4, 4)), None),
+        ((4, 0, -4), (0, 0, 8), (0, 4, 0), n8, n4, (-1, 0, 0), 0, slit_wall_window, slit_hole),
+        ((-4, 0, -4), (8, 0, 0), (0, 4, 0), n8, n4, (0, 0, 1), 0, border_window((-4, 0, -4), (4, 4, -4)), None),
+        ((-4, 0, 4), (8, 0, 0), (0, 4, 0), n8, n4, (0, 0, -1), 0, border_window((-4, 0, 4), (4, 4, 4)), None),
+    ]
+    verts, faces, sidx = [], [], []
+    nv = 0
+    for k, (o, eu, ev, nu, nvv, inward, sh, win, hole) in enumerate(walls):
+        P, f = _grid_wall(o, eu, ev, nu, nvv, inward, amplitude, seed * 10 + k, win, hole)
+        verts.append(P)
+        faces.append(f + nv)
+        sidx.append(np.full(len(f), sh, np.int64))
+        nv += len(P)
+    alcove = [quad((ax1, ay0, az0), (ax1, ay0, az1), (ax1, ay1, az1), (ax1, ay1, az0)),
+              quad((ax0, ay1, az0), (ax1, ay1, az0), (ax1, ay1, az1), (ax0, ay1, az1)),
+              quad((ax0, ay0, az0), (ax1, ay0, az0), (ax1, ay0, az1), (ax0, ay0, az1)),
+              quad((ax0, ay0, az0), (ax1, ay0, az0), (ax1, ay1, az0), (ax0, ay1, az0)),
+              quad((ax0, ay0, az1), (ax1, ay0, az1), (ax1, ay1, az1), (ax0, ay1, az1))]
+    centre = np.array([0.5 * (ax0 + ax1), 0.5 * (ay0 + ay1), 0.0])
+    for P, f in alcove:
+        n = np.cross(P[f[0, 1]] - P[f[0, 0]], P[f[0, 2]] - P[f[0, 0]])
+        if np.dot(n, centre - P[0]) < 0:
+            f = f[:, ::-1]
+        verts.append(P)
+        faces.append(f + nv)
+        sidx.append(np.full(len(f), 3, np.int64))
+        nv += len(P)
+    cam_from = np.array([-3.5, 1.9, -0.4])
+    if nboxes:
+        corners, btris = _unit_box_tris()
+        C = np.empty((0, 3))
+        while len(C) < nboxes:
+            c = np.stack([g.uniform(-3.7, 3.7, nboxes), g.uniform(0.05, 1.5, nboxes), g.uniform(-3.7, 3.7, nboxes)], 1)
+            c = c[np.linalg.norm(c - cam_from, axis=1) > 0.8]
+            C = np.concatenate([C, c])[:nboxes]
+        H = 0.5 * np.exp(g.uniform(math.log(0.03), math.log(0.2), (nboxes, 3)))
+        R = _random_rotations(g, nboxes)
+        bv = np.einsum("nij,nkj->nki", R, corners[None] * H[:, None, :]) + C[:, None, :]
+        verts.append(bv.reshape(-1, 3))
+        faces.append((btris[None] + (nv + 8 * np.arange(nboxes))[:, None, None]).reshape(-1, 3))
+        pal = g.integers(0, 32, nboxes)
+        sidx.append(np.repeat(4 + pal, 12))
+        nv += 8 * nboxes
+    mats = [pt_material((0.7, 0.7, 0.7)), pt_material((0.5, 0.5, 0.5)), pt_material((0.8, 0.8, 0.8)),
+            pt_material((0.8, 0.8, 0.8))]
+    mats += [pt_material(tuple(g.uniform(0.2, 0.8, 3))) for _ in range(32)]
+    light_mat = len(mats)
+    mats.append(pt_material((0.0, 0.0, 0.0), (emission, emission, emission), one_sided=True))
+    main = {"verts": np.concatenate(verts).astype(np.float32), "idx": np.concatenate(faces).astype(np.uint32),
+            "shaderidx": np.concatenate(sidx).astype(np.uint8), "shaders": np.arange(light_mat, dtype=np.int32),
+            "normals": None}
+    lx, ly0, ly1 = 4.35, 2.975, 3.025
+    LP = np.array([(lx, ly0, 1.0), (lx, ly0, -1.0), (lx, ly1, -1.0), (lx, ly1, 1.0)], np.float32)
+    light = {"verts": LP, "idx": np.array([[0, 1, 2], [0, 2, 3]], np.uint32), "shaderidx": None,
+             "shaders": np.array([light_mat], np.int32), "normals": None}
+    camera = {"from": tuple(cam_from), "to": (4.0, 2.2, 0.3), "up": (0.0, -1.0, 0.0), "fov": 75.0, "focal": 1.0}
+    return {"meshes": [main, light], "materials": np.array(mats, PT_MATERIAL_DTYPE), "camera": camera}
+
+
+def furnace_box_scene(albedo=0.5, emission=1.0, subdiv=2):
+    """Closed cube [-1,1]^3 whose inward-facing walls (subdiv x subdiv quads each) all have Lambertian
+    albedo `albedo` and emit `emission` (one-sided, inwards).  Radiance seen from inside with paths of
+    at most D segments: emission * (1 - albedo^D) / (1 - albedo)."""
+    verts, faces = [], []
+    nv = 0
+    for axis in range(3):
+        for s in (-1.0, 1.0):
+            u, v = [k for k in range(3) if k != axis]
+            o = np.zeros(3); o[axis] = s; o[u] = -1; o[v] = -1
+            eu = np.zeros(3); eu[u] = 2
+            ev = np.zeros(3); ev[v] = 2
+            inward = np.zeros(3); inward[axis] = -s
+            P, f = _grid_wall(o, eu, ev, subdiv, subdiv, inward, 0.0, 0, lambda p: np.zeros(len(p)))
+            verts.append(P)
+            faces.append(f + nv)
+            nv += len(P)
+    m = pt_material((albedo,) * 3, (emission,) * 3, one_sided=True)
+    main = {"verts": np.concatenate(verts).astype(np.float32), "idx": np.concatenate(faces).astype(np.uint32),
+            "shaderidx": None, "shaders": np.array([0], np.int32), "normals": None}
+    camera = {"from": (0.1, -0.2, 0.3), "to": (0.5, 0.3, -1.0), "up": (0.0, -1.0, 0.0), "fov": 90.0, "focal": 1.0}
+    return {"meshes": [main], "materials": np.array([m], PT_MATERIAL_DTYPE), "camera": camera}
