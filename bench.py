@@ -9,10 +9,15 @@ tagged with a path depth drawn from U{1..8}.  Rays are compacted by depth
 (a stable partition) and one traversal launch per bounce processes that
 bounce's rays; a step is the 8 launches over all 16,777,216 rays.
 
+Each launch may first reorder its rays by (origin cell, direction octant)
+inside the library (the autotuner decides, sort time included); results land
+in the callers' order either way.
+
 Inputs are resident in HBM before the timed region.  The timed region is K
-steps bracketed by torch.cuda.synchronize(); HIP events around every launch
-give the traversal kernel's average duration, from which the roofline's
-achieved bandwidth is computed with the algorithmic byte model
+steps bracketed by torch.cuda.synchronize(); HIP events recorded by the
+library around the sort and the traversal kernel of every launch give the
+traversal kernel's average duration, from which the roofline's achieved
+bandwidth is computed with the algorithmic byte model
     bytes/ray = 32 (ray) + 16 (hit) + 128 * node visits + 48 * triangle tests
 using per-ray visit counts taken from one counting pass of the same kernel
 (they equal the oracle's counts, which the GPU tests check).
@@ -44,11 +49,121 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--spheres", type=int, default=4096)
-    ap.add_argument("--trace-config", default="", help="block,lds_stack,waves_per_cu (default: autotune)")
+    ap.add_argument("--trace-config", default="", help="block,lds_stack,waves_per_cu[,sort_bits] (default: autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work for the baseline sample")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay measurement")
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided path-tracing measurement")
+    ap.add_argument("--pt-size", type=int, default=1024, help="guided measurement: image width = height")
+    ap.add_argument("--pt-ref-spp", type=int, default=4096, help="passes of the FP64 plain reference render")
+    ap.add_argument("--pt-plain-spp", type=int, default=64)
+    ap.add_argument("--pt-alpha", type=float, default=0.5)
+    ap.add_argument("--pt-stride", type=int, default=16, help="record the paths of one pixel in this many")
+    ap.add_argument("--pt-steps", type=int, default=4, help="Adam steps per pass")
+    ap.add_argument("--pt-batch", type=int, default=65536)
+    ap.add_argument("--pt-lr", type=float, default=1e-3)
+    ap.add_argument("--pt-warm", type=int, default=0,
+                    help="guided passes left out of the image (their time still counts)")
     return ap.parse_args()
+
+
+def guided_measurement(args, torch, np):
+    """Occluded-light room (about 1M triangles), max depth 8: plain BSDF sampling at pt_plain_spp passes
+    against guided sampling (one-sample MIS with the online-trained guide); the time the guided renderer
+    needs to reach the plain render's RMSE, both measured against an FP64-accumulated plain reference.
+    Times are GPU times of the passes (HIP events), training and inference included; the RMSE checks
+    between guided passes are outside them."""
+    from vermeer_amd import guide, pathtrace, scenes
+    W = args.pt_size
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    st = stream.cuda_stream
+    t0 = time.time()
+    desc = scenes.guided_room_scene()
+    ntri = int(sum(len(m["idx"]) for m in desc["meshes"]))
+    scene = pathtrace.build_scene(desc)
+    log(f"[bench] guided: room with {ntri} triangles built in {time.time() - t0:.1f}s")
+    npx = W * W
+
+    def passes(pt, acc, first, count, train=True):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for p in range(first, first + count):
+            pt.render_pass_device(acc.data_ptr(), p, stream=st, train=train)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    def rmse(acc, n, ref):
+        return float(torch.sqrt(torch.mean((acc / n - ref) ** 2)))
+
+    # FP64 plain reference
+    ref_pt = pathtrace.PathTracer(desc, W, W, max_depth=8, seed=1000, scene=scene)
+    acc_ref = torch.zeros(npx * 3, dtype=torch.float64, device=dev)
+    r0, _ = ref_pt.stats()
+    ref_ms = passes(ref_pt, acc_ref, 0, args.pt_ref_spp)
+    r1, _ = ref_pt.stats()
+    ref_pt.close()
+    ref = acc_ref / args.pt_ref_spp
+    plain_mrays = (r1 - r0) / (ref_ms * 1e-3) / 1e6
+    log(f"[bench] guided: reference {args.pt_ref_spp} spp in {ref_ms / 1e3:.1f}s ({ref_ms / args.pt_ref_spp:.2f} ms/pass, "
+        f"{plain_mrays:.0f} Mrays/s), mean radiance {float(ref.mean()):.4f}")
+
+    # plain
+    pl = pathtrace.PathTracer(desc, W, W, max_depth=8, seed=1, scene=scene)
+    acc_p = torch.zeros_like(acc_ref)
+    t_plain = passes(pl, acc_p, 0, args.pt_plain_spp)
+    e_plain = rmse(acc_p, args.pt_plain_spp, ref)
+    pl.close()
+    log(f"[bench] guided: plain {args.pt_plain_spp} spp rmse {e_plain:.5f} in {t_plain:.1f} ms")
+
+    # guided, online training, until its RMSE reaches the plain one (or 4x the plain time)
+    gd = guide.Guide(pathtrace.scene_bounds(desc), seed=1, max_batch=args.pt_batch)
+    gp = pathtrace.PathTracer(desc, W, W, max_depth=8, seed=2, scene=scene)
+    gp.set_guide(gd, alpha=args.pt_alpha, record_stride=args.pt_stride, train_steps=args.pt_steps,
+                 batch=args.pt_batch, lr=args.pt_lr)
+    acc_g = torch.zeros_like(acc_ref)
+    t_g, n_img, p = 0.0, 0, 0
+    t_equal = None
+    e_half = None
+    trace = []
+    g0, _ = gp.stats()
+    while t_g < 4 * t_plain:
+        if p == args.pt_warm and p > 0:
+            acc_g.zero_()
+            n_img = 0
+        t_g += passes(gp, acc_g, p, 1)
+        p += 1
+        n_img += 1
+        e = rmse(acc_g, n_img, ref)
+        trace.append((round(t_g, 2), round(e, 5)))
+        if t_g <= 0.5 * t_plain and p > args.pt_warm:
+            e_half = round(e, 6)
+        if t_equal is None and e <= e_plain and p > args.pt_warm:
+            t_equal = t_g
+            break
+    g1, nsamp = gp.stats()
+    guided_mrays = (g1 - g0) / (t_g * 1e-3) / 1e6
+    gp.close()
+    gd.close()
+    scene.close()
+    ratio = None if t_equal is None else t_equal / t_plain
+    log(f"[bench] guided: reached plain rmse after {p} passes, {t_g:.1f} ms (ratio {ratio}); "
+        f"{guided_mrays:.0f} Mrays/s")
+    return {
+        "scene": "occluded-light room", "triangles": ntri, "image": [W, W], "max_depth": 8,
+        "reference_spp": args.pt_ref_spp, "reference_mean": round(float(ref.mean()), 6),
+        "plain_spp": args.pt_plain_spp, "plain_rmse": round(e_plain, 6), "plain_ms": round(t_plain, 2),
+        "plain_samples_per_s": round(npx * args.pt_plain_spp / (t_plain * 1e-3), 1),
+        "plain_mrays_per_s": round(plain_mrays, 1),
+        "guided_passes": p, "guided_ms_to_equal_rmse": None if t_equal is None else round(t_equal, 2),
+        "time_ratio": None if ratio is None else round(ratio, 4),
+        "guided_rmse_at_half_plain_time": e_half,
+        "guided_mrays_per_s": round(guided_mrays, 1),
+        "guide": {"alpha": args.pt_alpha, "record_stride": args.pt_stride, "adam_steps_per_pass": args.pt_steps,
+                  "batch": args.pt_batch, "lr": args.pt_lr, "warm_passes": args.pt_warm},
+        "rmse_trace_ms": trace[:: max(1, len(trace) // 16)],
+    }
 
 
 def main():
@@ -97,8 +212,8 @@ def main():
     hp = d_hits.data_ptr()
 
     if args.trace_config:
-        b, l, w = (int(x) for x in args.trace_config.split(","))
-        sc.set_trace_config(b, l, w)
+        b, l, w, so = (int(x) for x in (args.trace_config.split(",") + ["0"])[:4])
+        sc.set_trace_config(b, l, w, so)
         tuned_ms = None
     else:
         cfg, tuned_ms = sc.autotune(rp, n, hp, stream=st)
@@ -134,20 +249,24 @@ def main():
     K = args.steps
     evs = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)] for _ in segs]
            for _ in range(K)]
+    sc.set_timing(True)          # library events around the sort and the traversal kernel of every launch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(K):
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    trace_ms, sort_ms, nlaunch = sc.timing()
+    sc.set_timing(False)
     sc.status(st)
-    kern_ms = [sum(e[0].elapsed_time(e[1]) for e in evs[k]) for k in range(K)]
-    launch_ms = [e[0].elapsed_time(e[1]) for k in range(K) for e in evs[k]]
+    call_ms = [sum(e[0].elapsed_time(e[1]) for e in evs[k]) for k in range(K)]
     ms_per_step = wall * 1000.0 / K
-    kernel_ms_per_step = float(np.mean(kern_ms))
+    kernel_ms_per_step = trace_ms / K
+    sort_ms_per_step = sort_ms / K
     mrays = n * K / wall / 1e6
     achieved = bytes_per_ray * n / (kernel_ms_per_step * 1e-3) / 1e9
-    log(f"[bench] {ms_per_step:.3f} ms/step wall, {kernel_ms_per_step:.3f} ms kernel, {mrays:.1f} Mrays/s, "
+    log(f"[bench] {ms_per_step:.3f} ms/step wall ({float(np.mean(call_ms)):.3f} ms in trace calls: "
+        f"{sort_ms_per_step:.3f} ms sort + {kernel_ms_per_step:.3f} ms traversal kernel), {mrays:.1f} Mrays/s, "
         f"{achieved:.0f} GB/s algorithmic")
 
     graph = None
@@ -209,7 +328,7 @@ def main():
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         if tj.get("workload") == "icosphere_field_bounce_rays" and tj["kernel"].startswith(
-                f"k_trace<{cfg['block']},{cfg['lds_stack']},"):
+                f"k_trace<{cfg['block']},{cfg['lds_stack']},") and tj.get("sort", 0) == cfg["sort"]:
             traffic = round(tj["hbm_bytes_per_ray"] * n / len(segs))
 
     out = {
@@ -246,13 +365,21 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "algorithmic_bytes_per_ray": round(bytes_per_ray, 1),
+            "kernel": "k_trace (traversal)",
             "kernel_ms_per_step": round(kernel_ms_per_step, 4),
-            "kernel_avg_launch_ms": round(float(np.mean(launch_ms)), 4),
+            "kernel_avg_launch_ms": round(trace_ms / max(nlaunch, 1), 4),
+            "sort_ms_per_step": round(sort_ms_per_step, 4),
         },
         "cpu_baseline": cpu,
         "hipgraph": graph,
         "host_build_s": round(build_s, 2),
+        "guided": None,
     }
+    if not args.no_guided:
+        del d_hits, d_cnt, d_rays
+        sc.close()
+        torch.cuda.empty_cache()
+        out["guided"] = guided_measurement(args, torch, np)
     print(json.dumps(out), flush=True)
 
 

@@ -237,18 +237,31 @@ int vgpu_render_destroy(vgpu_render* render);
 int vgpu_render_pass(vgpu_render* render, uint32_t flags, const uint64_t* scrambles, uint32_t iter, float* film,
                      float* sample_out, void* stream, uint64_t* shadow_rays_out);
 
+/* Kernel timing, a diagnostic with no reference counterpart (the reference
+ * reports only ray counts, core/stats.go:14-45).  While enabled, every
+ * traversal records HIP events on its stream around the ray sort and the
+ * traversal kernel (do not enable it while capturing a graph).
+ * vgpu_scene_timing synchronises on them, returns the summed traversal and
+ * sort milliseconds and the number of traversals since the last call, and
+ * starts a new interval. */
+int vgpu_scene_set_timing(vgpu_scene* scene, int32_t enable);
+int vgpu_scene_timing(vgpu_scene* scene, float* trace_ms, float* sort_ms, int32_t* launches);
+
 /* Traversal launch configuration (the autotuner's knobs). */
 typedef struct vgpu_trace_config {
     int32_t block;        /* threads per workgroup: 64, 128 or 256 */
     int32_t lds_stack;    /* stack entries per ray held in LDS: 8, 16, 24 or 32; the rest spill to HBM */
     int32_t waves_per_cu; /* resident-grid sizing: workgroups = CUs * waves_per_cu * 64 / block */
-    int32_t sort;         /* 1: reorder rays by a Morton key of origin+direction before traversal */
+    int32_t sort;         /* 0: trace in the given order; k in 1..6: first reorder the rays by bucket =
+                           * (Morton cell of the origin in a 2^k-per-axis grid over the scene box) x
+                           * (direction octant); results still land in the callers' order */
 } vgpu_trace_config;
 
 int vgpu_scene_set_trace_config(vgpu_scene* scene, const vgpu_trace_config* cfg);
 int vgpu_scene_get_trace_config(const vgpu_scene* scene, vgpu_trace_config* cfg);
 
-/* Runs the autotuner: times each candidate configuration on the given rays
+/* Runs the autotuner: times each candidate configuration (block x LDS depth x
+ * waves x sort in {0, 4, 5, 6}, the sort included in the time) on the given rays
  * (device pointers, n rays) and keeps the fastest.  best_out (nullable)
  * receives the chosen configuration; ms_out (nullable) its time. */
 int vgpu_scene_autotune(vgpu_scene* scene, uint32_t flags, const vgpu_ray* d_rays, uint64_t n,

@@ -194,6 +194,27 @@ def test_icosphere_field_bounce_rays_bitexact(icosphere_field):
     _same(hits_s, geoms_s, ref_s, "icosphere field shadow")
 
 
+@pytest.mark.parametrize("sort", [1, 4, 6])
+def test_sorted_traversal_is_bitexact(icosphere_field, sort):
+    """Reordering rays by (origin cell, direction octant) changes only the order of work: every
+    ray's hit, geom slot and traversal counters land in its own slot, equal to the oracle's."""
+    g, o, v, f = icosphere_field
+    rays, _ = scenes.bounce_rays(v, f, 200_000, seed=23)
+    rays[::97]["P"] = np.nan          # NaN origins fall into cell 0 and still miss
+    print(f"[case] icosphere field 200k bounce rays, sort bits {sort}", flush=True)
+    g.set_trace_config(256, 16, 16, sort)
+    try:
+        hits, geoms, cnt = g.trace(rays, want_counters=True)
+        hits_s, geoms_s, _ = g.trace(rays, shadow=True)
+    finally:
+        g.set_trace_config(256, 16, 16, 0)
+    ref, rc = o.trace(rays, counters=True)
+    _same(hits, geoms, ref, f"sorted {sort}")
+    assert np.array_equal(cnt, rc)
+    ref_s, _ = o.trace(rays, shadow=True)
+    _same(hits_s, geoms_s, ref_s, f"sorted {sort} shadow")
+
+
 def test_icosphere_field_full_16m_properties(icosphere_field):
     """Full size (16,777,216 rays): sizes the oracle cannot check in seconds are checked by
     properties: a sampled slice equals the oracle, two runs are bit-identical, every hit lies

@@ -189,6 +189,15 @@ class Scene:
         cfg = _lib.TraceConfig(block, lds_stack, waves_per_cu, sort)
         check(self._lib.vgpu_scene_set_trace_config(self._h, C.byref(cfg)))
 
+    def set_timing(self, enable=True):
+        check(self._lib.vgpu_scene_set_timing(self._h, int(bool(enable))))
+
+    def timing(self):
+        """(traversal ms, sort ms, traversals) summed since the last call (timing must be enabled)."""
+        t, so, k = C.c_float(), C.c_float(), C.c_int32()
+        check(self._lib.vgpu_scene_timing(self._h, C.byref(t), C.byref(so), C.byref(k)))
+        return float(t.value), float(so.value), int(k.value)
+
     def trace_config(self):
         cfg = _lib.TraceConfig()
         check(self._lib.vgpu_scene_get_trace_config(self._h, C.byref(cfg)))

@@ -187,8 +187,37 @@ int check_config(const vgpu_trace_config& c) {
         return fail(VGPU_ERR_ARG, "lds_stack must be 8, 16, 24 or 32");
     if (c.block * c.lds_stack * 8 > 65536) return fail(VGPU_ERR_ARG, "block*lds_stack exceeds 64 KiB of LDS");
     if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return fail(VGPU_ERR_ARG, "waves_per_cu must be in [1,32]");
-    if (c.sort != 0) return fail(VGPU_ERR_ARG, "ray sorting is not available in this build");
+    if (c.sort < 0 || c.sort > 6) return fail(VGPU_ERR_ARG, "sort must be 0 (off) or 1..6 grid bits per axis");
     return VGPU_OK;
+}
+
+int ensure_sort(vgpu_scene* s, uint64_t n) {
+    vgpu::RaySortBuffers& sb = s->sort;
+    if (!sb.hist) {
+        HIP_TRY(hipMalloc((void**)&sb.hist, sizeof(uint32_t) * vgpu::ray_sort_buckets(6)));
+        HIP_TRY(hipMalloc((void**)&sb.bsum, sizeof(uint32_t) * vgpu::ray_sort_scan_blocks(6)));
+    }
+    if (sb.capacity >= n) return VGPU_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    if (sb.keys) { HIP_TRY(hipFree(sb.keys)); sb.keys = nullptr; }
+    if (sb.perm) { HIP_TRY(hipFree(sb.perm)); sb.perm = nullptr; }
+    if (sb.rays) { HIP_TRY(hipFree(sb.rays)); sb.rays = nullptr; }
+    sb.capacity = 0;
+    HIP_TRY(hipMalloc((void**)&sb.keys, sizeof(uint32_t) * n));
+    HIP_TRY(hipMalloc((void**)&sb.perm, sizeof(uint32_t) * n));
+    HIP_TRY(hipMalloc((void**)&sb.rays, 32 * n));
+    sb.capacity = n;
+    return VGPU_OK;
+}
+
+// Next free timing event (the pool grows; vgpu_scene_timing recycles it).
+hipEvent_t timing_event(vgpu_scene* s) {
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        s->ev_pool.push_back(e);
+    }
+    return s->ev_pool[s->ev_used++];
 }
 
 int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,
@@ -197,10 +226,27 @@ int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n,
     const uint64_t threads = vgpu::trace_grid_threads(L, n, s->num_cus);
     int rc = ensure_spill(s, threads);
     if (rc) return rc;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (s->timing && n > 0) {
+        for (auto& e : ev) e = timing_event(s);
+        if (!ev[0] || !ev[1] || !ev[2]) return fail(VGPU_ERR_HIP, "timing events");
+        HIP_TRY(hipEventRecord(ev[0], st));
+    }
+    const uint32_t* perm = nullptr;
+    if (cfg.sort > 0) {
+        if (n > 0xffffffffull) return fail(VGPU_ERR_ARG, "ray sorting handles at most 2^32-1 rays per call");
+        if ((rc = ensure_sort(s, n))) return rc;
+        hipError_t e = vgpu::launch_ray_sort(s->box, cfg.sort, d_rays, n, n_dev, s->sort, s->num_cus, st);
+        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "ray sort launch: %s", hipGetErrorString(e));
+        d_rays = s->sort.rays;
+        perm = s->sort.perm;
+    }
+    if (ev[1]) HIP_TRY(hipEventRecord(ev[1], st));
     hipError_t e = vgpu::launch_trace(s->view, (flags & VGPU_TRACE_SHADOW) != 0, s->any_motion, L, d_rays, n, d_hits,
                                       d_geoms, d_counters, s->d_spT, s->d_spN, s->spill_threads, s->d_err, s->num_cus, st,
-                                      n_dev);
+                                      n_dev, perm);
     if (e != hipSuccess) return fail(VGPU_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
+    if (ev[2]) HIP_TRY(hipEventRecord(ev[2], st));
     return VGPU_OK;
 }
 
@@ -349,6 +395,15 @@ int vgpu_scene_commit(vgpu_scene* s) {
         std::memcpy(&boxes[(size_t)i * 6], b, sizeof(b));
         for (int k = 0; k < 3; k++) cent[(size_t)i * 3 + k] = (b[3 + k] + b[k]) * 0.5f;   // BoundingBox.Centroid
         ids[i] = i;
+    }
+    {
+        Box6 all;
+        all.reset();
+        for (auto& m : s->meshes) {
+            if (m.nkeys == 1) all.grow_box(m.bounds);
+            else for (uint32_t k = 0; k < m.nkeys; k++) all.grow_box(&m.motion_bounds[(size_t)k * 6]);
+        }
+        std::memcpy(s->box, all.v, sizeof(s->box));
     }
     int32_t nn = 0;
     if (maxKeys == 1) {
@@ -550,6 +605,36 @@ int vgpu_trace(vgpu_scene* s, uint32_t flags, const vgpu_ray* rays, uint64_t n, 
     return VGPU_OK;
 }
 
+int vgpu_scene_set_timing(vgpu_scene* s, int32_t enable) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    DeviceGuard dg(s->device);
+    HIP_TRY(hipDeviceSynchronize());
+    s->timing = enable != 0;
+    s->ev_used = 0;
+    return VGPU_OK;
+}
+
+int vgpu_scene_timing(vgpu_scene* s, float* trace_ms, float* sort_ms, int32_t* launches) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    DeviceGuard dg(s->device);
+    double tt = 0.0, ts = 0.0;
+    const size_t nl = s->ev_used / 3;
+    for (size_t i = 0; i < nl; i++) {
+        hipEvent_t* e = &s->ev_pool[3 * i];
+        HIP_TRY(hipEventSynchronize(e[2]));
+        float a = 0.0f, b = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
+        HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
+        ts += a;
+        tt += b;
+    }
+    s->ev_used = 0;
+    if (trace_ms) *trace_ms = (float)tt;
+    if (sort_ms) *sort_ms = (float)ts;
+    if (launches) *launches = (int32_t)nl;
+    return VGPU_OK;
+}
+
 int vgpu_scene_status(vgpu_scene* s, void* stream) {
     if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
     DeviceGuard dg(s->device);
@@ -591,10 +676,12 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
     vgpu_trace_config bc = s->cfg;
+    const int sorts[] = {0, 4, 5, 6};
+    for (int so : sorts)
     for (int b : blocks)
         for (int l : ldss)
             for (int w : waves) {
-                vgpu_trace_config c{b, l, w, 0};
+                vgpu_trace_config c{b, l, w, so};
                 if (check_config(c)) continue;
                 int rc = enqueue_trace(s, flags | VGPU_TRACE_DEVICE_PTRS, d_rays, n, d_hits, nullptr, nullptr, st, c);
                 if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }

@@ -117,13 +117,20 @@ struct vgpu_scene {
     vgpu::DevGeomShade* d_gshade = nullptr;
     int32_t max_material = -1;
     vgpu_trace_config cfg{256, 16, 16, 0};
+    float box[6] = {0, 0, 0, 0, 0, 0};   // scene bounds over all keys (ray-sort grid)
+    vgpu::RaySortBuffers sort{};
+    // kernel timing (vgpu_scene_set_timing): events [sort start, trace start, trace end] per launch
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
 
     ~vgpu_scene() {
         int prev = 0;
         if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(device);
         void* ptrs[] = {d_nodes, d_tris, d_mnodes, d_mtris, d_geoms, d_err, d_spT, d_spN, d_stage, d_normals, d_shaderidx,
-                        d_shaders, d_gshade};
+                        d_shaders, d_gshade, sort.keys, sort.perm, sort.rays, sort.hist, sort.bsum};
         for (void* p : ptrs) if (p) (void)hipFree(p);
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         (void)hipSetDevice(prev);
     }
 };

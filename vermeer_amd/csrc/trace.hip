@@ -209,7 +209,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
                                                  int4* __restrict__ hits, int32_t* __restrict__ geoms_out,
                                                  int4* __restrict__ counters, float* __restrict__ spT,
                                                  int32_t* __restrict__ spN, int32_t* __restrict__ err,
-                                                 const uint32_t* __restrict__ n_dev) {
+                                                 const uint32_t* __restrict__ n_dev,
+                                                 const uint32_t* __restrict__ perm) {
     __shared__ float sT[LDS * BLOCK];
     __shared__ int32_t sN[LDS * BLOCK];
     const int tid = threadIdx.x;
@@ -343,41 +344,43 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
             atomicOr(err, 1);
             best_geom = -1; best_prim = -1;
         }
+        const uint64_t oi = perm ? (uint64_t)perm[ri] : ri;
         if (EXT) {
-            hits[2 * ri] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
-            hits[2 * ri + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
+            hits[2 * oi] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
+            hits[2 * oi + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
         } else {
-            hits[ri] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
-            if (geoms_out) geoms_out[ri] = best_geom;
+            hits[oi] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
+            if (geoms_out) geoms_out[oi] = best_geom;
         }
-        if (COUNTERS) counters[ri] = make_int4(nodesT, leafsT, trisT, maxDepth);
+        if (COUNTERS) counters[oi] = make_int4(nodesT, leafsT, trisT, maxDepth);
     }
 }
 
 template <int BLOCK, int LDS, bool SHADOW, bool MOTION>
 hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
-                      float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev, hipStream_t stream) {
+                      float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev,
+                      const uint32_t* perm, hipStream_t stream) {
     if (counters)
         hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, MOTION, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err, n_dev);
+                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err, n_dev, perm);
     else
         hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, MOTION, false>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err, n_dev);
+                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err, n_dev, perm);
     return hipGetLastError();
 }
 
 template <int BLOCK, int LDS>
 hipError_t launch_mode(const SceneView& sv, bool shadow, bool motion, const void* rays, uint64_t n, void* hits,
                        int32_t* geoms, int32_t* counters, float* spT, int32_t* spN, uint64_t grid, int32_t* err,
-                       const uint32_t* n_dev, hipStream_t stream) {
+                       const uint32_t* n_dev, const uint32_t* perm, hipStream_t stream) {
     if (shadow) {
         if (motion)
-            return launch_cfg<BLOCK, LDS, true, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
-        return launch_cfg<BLOCK, LDS, true, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
+            return launch_cfg<BLOCK, LDS, true, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
+        return launch_cfg<BLOCK, LDS, true, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
     }
     if (motion)
-        return launch_cfg<BLOCK, LDS, false, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
-    return launch_cfg<BLOCK, LDS, false, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
+        return launch_cfg<BLOCK, LDS, false, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
+    return launch_cfg<BLOCK, LDS, false, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
 }
 
 }  // namespace
@@ -392,10 +395,12 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
     const unsigned grid = (unsigned)(threads / cfg.block);
     if (motion)
         hipLaunchKernelGGL((k_trace<256, 16, false, true, false, true>), dim3(grid), dim3(256), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev);
+                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
+                           (const uint32_t*)nullptr);
     else
         hipLaunchKernelGGL((k_trace<256, 16, false, false, false, true>), dim3(grid), dim3(256), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev);
+                           (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
+                           (const uint32_t*)nullptr);
     return hipGetLastError();
 }
 
@@ -410,14 +415,15 @@ uint64_t trace_grid_threads(const TraceLaunch& cfg, uint64_t n, int num_cus) {
 
 hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const TraceLaunch& cfg, const void* rays, uint64_t n,
                         void* hits, int32_t* geoms, int32_t* counters, float* spT, int32_t* spN, uint64_t spill_threads,
-                        int32_t* err, int num_cus, hipStream_t stream, const uint32_t* n_dev) {
+                        int32_t* err, int num_cus, hipStream_t stream, const uint32_t* n_dev, const uint32_t* perm) {
     if (n == 0) return hipSuccess;
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
     const uint64_t grid = threads / cfg.block;
 #define VGPU_CASE(B, L)                                                                                         \
     if (cfg.block == B && cfg.lds_depth == L)                                                                   \
-        return launch_mode<B, L>(sv, shadow, motion, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, stream);
+        return launch_mode<B, L>(sv, shadow, motion, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, \
+                                 stream);
     VGPU_CASE(64, 8) VGPU_CASE(64, 16) VGPU_CASE(64, 24) VGPU_CASE(64, 32)
     VGPU_CASE(128, 8) VGPU_CASE(128, 16) VGPU_CASE(128, 24) VGPU_CASE(128, 32)
     VGPU_CASE(256, 8) VGPU_CASE(256, 16) VGPU_CASE(256, 24) VGPU_CASE(256, 32)

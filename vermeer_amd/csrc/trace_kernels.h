@@ -56,15 +56,33 @@ struct TraceLaunch {
     int waves_per_cu;   // resident waves per CU used to size the grid
 };
 
+// Scratch of the ray sort (raysort.hip), sized for `capacity` rays.
+struct RaySortBuffers {
+    uint32_t* keys;      // capacity
+    uint32_t* perm;      // capacity: sorted position -> original ray index
+    float4* rays;        // 2 * capacity: sorted rays
+    uint32_t* hist;      // ray_sort_buckets(bits) counters
+    uint32_t* bsum;      // ray_sort_scan_blocks(bits)
+    uint64_t capacity;
+};
+uint32_t ray_sort_buckets(int bits);
+uint32_t ray_sort_scan_blocks(int bits);
+// Sorts min(n, *n_dev) rays into sb.rays / sb.perm by (Morton cell of the origin in a 2^bits grid over
+// box = lo xyz, hi xyz) x (direction octant); bits in [1,6].
+hipError_t launch_ray_sort(const float* box, int bits, const void* rays, uint64_t n, const uint32_t* n_dev,
+                           const RaySortBuffers& sb, int num_cus, hipStream_t stream);
+
 // Enqueues one traversal over n rays (device pointers).  spill_T / spill_N
 // hold (VGPU_MAX_STACK - lds_depth) entries per launched thread; the launch
 // uses at most spill_threads threads.  n_dev (nullable): a device-resident
-// ray count; the kernel traces min(n, *n_dev) rays (n sizes the grid).
+// ray count; the kernel traces min(n, *n_dev) rays (n sizes the grid).  perm (nullable):
+// ray i's results go to slot perm[i] (rays reordered by launch_ray_sort).
 // Returns hipSuccess or the launch error.
 hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const TraceLaunch& cfg,
                         const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
                         float* spill_T, int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag,
-                        int num_cus, hipStream_t stream, const uint32_t* n_dev = nullptr);
+                        int num_cus, hipStream_t stream, const uint32_t* n_dev = nullptr,
+                        const uint32_t* perm = nullptr);
 
 // Closest hit with 32-byte records {prim, geom, t, U, V, W, 0, 0} (U, V, W: barycentric weights of the
 // triangle's three vertices), for the shading kernels.  Fixed configuration (block 256, LDS stack 16).

@@ -19,22 +19,36 @@ def camera_for(desc, width, height):
     return lookat_camera(c["from"], c["to"], c["up"], c["fov"], c["focal"], width, height)
 
 
+def build_scene(desc, device=0):
+    """Committed GPU scene of the description's shaded meshes (shareable by several PathTracers)."""
+    lib = _lib.load()
+    scene = Scene(device)
+    for mesh in desc["meshes"]:
+        v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(-1, 3)
+        ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
+        sidx = mesh.get("shaderidx")
+        sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
+        sh = np.ascontiguousarray(mesh["shaders"], np.int32)
+        g = C.c_int32(-1)
+        check(lib.vgpu_scene_add_shaded_mesh(scene._h, ptr(v), len(v), ptr(ix), len(ix), None, ptr(sidx), ptr(sh),
+                                             len(sh), C.byref(g)))
+        scene.geoms.append(None)
+    scene.pre_render()
+    return scene
+
+
+def scene_bounds(desc, margin=0.05):
+    lo = np.min([np.asarray(m["verts"]).reshape(-1, 3).min(0) for m in desc["meshes"]], axis=0) - margin
+    hi = np.max([np.asarray(m["verts"]).reshape(-1, 3).max(0) for m in desc["meshes"]], axis=0) + margin
+    return np.concatenate([lo, hi]).astype(np.float32)
+
+
 class PathTracer:
-    def __init__(self, desc, width, height, max_depth=8, seed=1, cam=None, device=0):
+    def __init__(self, desc, width, height, max_depth=8, seed=1, cam=None, device=0, scene=None):
         self._lib = _lib.load()
-        self.scene = Scene(device)
+        self._own_scene = scene is None
+        self.scene = build_scene(desc, device) if scene is None else scene
         self.width, self.height = width, height
-        for mesh in desc["meshes"]:
-            v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(-1, 3)
-            ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
-            sidx = mesh.get("shaderidx")
-            sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
-            sh = np.ascontiguousarray(mesh["shaders"], np.int32)
-            g = C.c_int32(-1)
-            check(self._lib.vgpu_scene_add_shaded_mesh(self.scene._h, ptr(v), len(v), ptr(ix), len(ix), None,
-                                                       ptr(sidx), ptr(sh), len(sh), C.byref(g)))
-            self.scene.geoms.append(None)
-        self.scene.pre_render()
         mats = np.ascontiguousarray(desc["materials"], PT_MATERIAL_DTYPE)
         self.cam = np.ascontiguousarray(cam if cam is not None else camera_for(desc, width, height), CAMERA_DTYPE)
         h = C.c_void_p()
@@ -49,7 +63,8 @@ class PathTracer:
         if getattr(self, "_h", None):
             self._lib.vgpu_pt_destroy(self._h)
             self._h = None
-        self.scene.close()
+        if self._own_scene:
+            self.scene.close()
 
     def __del__(self):
         try:
