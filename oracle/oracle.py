@@ -449,6 +449,8 @@ def _pt_sigs(lib):
         "orc_pt_add_mesh_lights": (None, [P, P, P, C.c_int32, P, P, C.c_int32]),
         "orc_pt_pass": (C.c_int64, [P, C.c_uint32, P, P, C.c_int32]),
         "orc_pt_sincos2pi": (None, [C.c_float, P, P]),
+        "orc_pt_set_fixed_guide": (None, [P, C.c_float, P]),
+        "orc_pt_set_no_nee": (None, [P, C.c_int32]),
         "orc_pt_rnd": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
@@ -495,6 +497,15 @@ class OraclePathTracer:
         if getattr(self, "h_pt", None):
             self.lib.orc_pt_free(self.h_pt)
             self.h_pt = None
+
+    def set_fixed_guide(self, alpha, lobes):
+        """Guided sampling with the same 8 lobes (mu xyz, kappa, weight) at every vertex."""
+        self._lobes = np.ascontiguousarray(lobes, np.float32).reshape(40)
+        self.lib.orc_pt_set_fixed_guide(self.h_pt, float(alpha), _p(self._lobes))
+
+    def set_no_nee(self, on=True):
+        """Diagnostic: sample directions only (no next-event estimation)."""
+        self.lib.orc_pt_set_no_nee(self.h_pt, int(bool(on)))
 
     def render_pass(self, pass_index, nthreads=0, want_sample=False):
         samp = np.zeros((self.h, self.w, 3), np.float32) if want_sample else None

@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only.
 //
-// CPU restatement of path mode in plain sampling (alpha = 0): the GPU path
+// CPU restatement of path mode (plain sampling, and guided sampling with a fixed
+// proposal given as lobes): the GPU path
 // tracer's estimator, written independently and in the same order of float
 // operations, so that plain-mode paths can be compared bit for bit.  Only
 // tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may call it.
@@ -17,7 +18,8 @@
 //                        normal = geometric normal turned towards the ray
 //   emission             found by a sampled direction: weight p / (p + pdf_light)
 //   NEE                  one emissive triangle by area x lum(Le), uniform point,
-//                        weight pdf_light / (pdf_light + p), shadow ray to 0.9999 dist
+//                        weight pdf_light / (pdf_light + p), shadow ray from the offset
+//                        origin to 0.9999 of its distance to the point
 //   continuation         cosine-weighted direction, throughput rho cos / (pi p)
 //   depth                up to max_depth segments from the camera
 // Closest-hit and shadow traversal go through the traversal oracle (oracle_core.h).
@@ -98,6 +100,22 @@ static float off1(float p, float n) {      // Waechter & Binder 2019
 
 static inline orc::Vec3 vec(F3 a) { orc::Vec3 v; v[0] = a.x; v[1] = a.y; v[2] = a.z; return v; }
 
+// Guided sampling with a fixed proposal: the same 8 vMF lobes (mu xyz, kappa, weight) at every
+// vertex (the GPU evaluates them per vertex from the network).  Definitions as in the GPU kernels:
+// q(w) = sum_k w_k kappa_k / (2 pi (1 - exp(-2 kappa_k))) exp(kappa_k (mu_k . w - 1)); a lobe is
+// picked by its weight (dim 6), then cos theta = 1 + log(u + (1 - u) exp(-2 kappa)) / kappa
+// (Jakob 2012) around mu with the azimuth of dim 5.
+static float vmf_c(float k) { return k / (6.28318548f * -std::expm1(-2.0f * k)); }
+static float guide_pdf(const float* lob, F3 w) {
+    float q = 0.0f;
+    for (int k = 0; k < 8; k++) {
+        const float* L = lob + 5 * k;
+        const float c = (L[0] * w.x + L[1] * w.y) + L[2] * w.z;
+        q += L[4] * (vmf_c(L[3]) * std::exp(L[3] * (c - 1.0f)));
+    }
+    return q;
+}
+
 struct Mat { F3 a, e; bool one_sided; float pdf_area; };
 struct Light { F3 v0, e1, e2, n, Le; float cdf, pdf_area; bool one_sided; };
 
@@ -111,6 +129,9 @@ struct PT {
     float M[16];
     float ttf = 0, aspect = 1, focal = 1;
     uint32_t s_lo = 0, s_hi = 0;
+    float alpha = 0.0f;
+    float lobes[40];
+    bool no_nee = false;   // diagnostic: BSDF/guide sampling only, emission weight 1
 
     void finalize() {
         if (finalized) return;
@@ -188,7 +209,7 @@ struct PT {
             const Mat& m = mats[mesh.shaders[si]];
             if ((m.e.x > 0.0f || m.e.y > 0.0f || m.e.z > 0.0f) && (!m.one_sided || front)) {
                 float wB = 1.0f;
-                if (depth > 0) {
+                if (depth > 0 && !no_nee) {
                     const float pdfL = m.pdf_area * (t * t) / std::fabs(cosi);
                     wB = pdf_prev / (pdf_prev + pdfL);
                 }
@@ -197,7 +218,7 @@ struct PT {
             const F3 rho = m.a;
             if (!(depth + 1 < D && (rho.x > 0.0f || rho.y > 0.0f || rho.z > 0.0f))) break;
             const uint32_t d0 = 2 + 8 * depth;
-            if (!lights.empty()) {
+            if (!lights.empty() && !no_nee) {
                 const float ul = rnd(key, d0);
                 int j = 0;
                 while (j + 1 < (int)lights.size() && !(lights[j].cdf > ul)) j++;
@@ -216,32 +237,62 @@ struct PT {
                 const float cosx = dot(N, wl);
                 if (cosl > 0.0f && cosx > 0.0f) {
                     const float pdfL = Lt.pdf_area * d2 / cosl;
-                    const float pmL = cosx * INV_PI;
+                    float pmL = cosx * INV_PI;
+                    if (alpha > 0.0f) pmL = (1.0f - alpha) * pmL + alpha * guide_pdf(lobes, wl);
                     const float w = pdfL / (pdfL + pmL);
                     const float k = cosx * w / pdfL;
                     const F3 E = f3(rho.x * INV_PI * k * Lt.Le.x, rho.y * INV_PI * k * Lt.Le.y, rho.z * INV_PI * k * Lt.Le.z);
                     if (E.x > 0.0f || E.y > 0.0f || E.z > 0.0f) {
                         rays++;
+                        // the shadow ray runs from the offset origin to y, stopping 1e-4 short of it
                         const F3 o = f3(off1(P.x, N.x), off1(P.y, N.y), off1(P.z, N.z));
-                        if (!occluded(tk, o, wl, dist * 0.9999f)) {
+                        const F3 sv = sub(y, o);
+                        const float sd = std::sqrt(dot(sv, sv));
+                        const F3 sdir = f3(sv.x / sd, sv.y / sd, sv.z / sd);
+                        if (!occluded(tk, o, sdir, sd * 0.9999f)) {
                             const F3 cc = mul(beta, E);
                             L = f3(L.x + cc.x, L.y + cc.y, L.z + cc.z);
                         }
                     }
                 }
             }
-            const float u4 = rnd(key, d0 + 4), u5 = rnd(key, d0 + 5);
-            F3 t1, t2;
-            onb(N, t1, t2);
-            const float r = std::sqrt(u4);
-            float sp, cp;
-            sincos2pi(u5, sp, cp);
-            const float lx = r * cp, ly = r * sp, lz = std::sqrt(std::fmax(0.0f, 1.0f - u4));
-            const F3 wd = f3((t1.x * lx + t2.x * ly) + N.x * lz, (t1.y * lx + t2.y * ly) + N.y * lz,
-                             (t1.z * lx + t2.z * ly) + N.z * lz);
+            const float u3 = rnd(key, d0 + 3), u4 = rnd(key, d0 + 4), u5 = rnd(key, d0 + 5);
+            F3 wd;
+            if (alpha > 0.0f && u3 < alpha) {
+                const float ul = rnd(key, d0 + 6);
+                int kk = 7;
+                float acc = 0.0f;
+                for (int k = 0; k < 8; k++) {
+                    acc += lobes[5 * k + 4];
+                    if (ul < acc) { kk = k; break; }
+                }
+                const float* Lb = lobes + 5 * kk;
+                const F3 mu = f3(Lb[0], Lb[1], Lb[2]);
+                const float kap = Lb[3];
+                float wc = 1.0f + std::log(u4 + (1.0f - u4) * std::exp(-2.0f * kap)) / kap;
+                wc = std::fmin(std::fmax(wc, -1.0f), 1.0f);
+                const float st = std::sqrt(std::fmax(0.0f, 1.0f - wc * wc));
+                float sp, cp;
+                sincos2pi(u5, sp, cp);
+                F3 t1, t2;
+                onb(mu, t1, t2);
+                const float lx = st * cp, ly = st * sp;
+                wd = f3((t1.x * lx + t2.x * ly) + mu.x * wc, (t1.y * lx + t2.y * ly) + mu.y * wc,
+                        (t1.z * lx + t2.z * ly) + mu.z * wc);
+            } else {
+                F3 t1, t2;
+                onb(N, t1, t2);
+                const float r = std::sqrt(u4);
+                float sp, cp;
+                sincos2pi(u5, sp, cp);
+                const float lx = r * cp, ly = r * sp, lz = std::sqrt(std::fmax(0.0f, 1.0f - u4));
+                wd = f3((t1.x * lx + t2.x * ly) + N.x * lz, (t1.y * lx + t2.y * ly) + N.y * lz,
+                        (t1.z * lx + t2.z * ly) + N.z * lz);
+            }
             const float cosw = dot(N, wd);
             if (!(cosw > 0.0f)) break;
-            const float pm = cosw * INV_PI;
+            float pm = cosw * INV_PI;
+            if (alpha > 0.0f) pm = (1.0f - alpha) * pm + alpha * guide_pdf(lobes, wd);
             const float k = cosw * INV_PI / pm;
             beta = mul(beta, f3(rho.x * k, rho.y * k, rho.z * k));
             pdf_prev = pm;
@@ -332,6 +383,16 @@ int64_t orc_pt_pass(void* pp, uint32_t pass, double* accum, float* sample_out, i
     }
     return rays;
 }
+
+// Fixed-proposal guided sampling (alpha > 0): lobes = 8 x (mu xyz, kappa, weight).
+void orc_pt_set_fixed_guide(void* pp, float alpha, const float* lobes) {
+    PT* p = (PT*)pp;
+    p->alpha = alpha;
+    if (lobes) std::memcpy(p->lobes, lobes, sizeof(p->lobes));
+}
+
+// Diagnostic: 1 = no next-event estimation (emission found by sampled directions counts fully).
+void orc_pt_set_no_nee(void* pp, int32_t on) { ((PT*)pp)->no_nee = on != 0; }
 
 void orc_pt_sincos2pi(float u, float* s, float* c) { opt::sincos2pi(u, *s, *c); }
 

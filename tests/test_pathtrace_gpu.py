@@ -99,7 +99,7 @@ def test_guided_and_plain_agree_on_the_room(room):
     bb = b.mean(2).reshape(8, 8, 8, 8).mean((1, 3)).ravel()
     ma, mb = a.mean(), b.mean()
     print(f"[case] plain {ma:.4f} guided {mb:.4f}", flush=True)
-    assert abs(ma - mb) < 0.1 * ma
+    assert abs(ma - mb) < 0.05 * ma
     assert np.corrcoef(ba, bb)[0, 1] > 0.9
 
 
@@ -115,3 +115,65 @@ def test_guiding_reduces_error_at_equal_samples(room):
     e_g = math.sqrt(np.mean((guided - ref) ** 2))
     print(f"[case] RMSE plain {e_p:.4f} guided {e_g:.4f}", flush=True)
     assert e_g < 0.8 * e_p
+
+
+def _fixed_sharp_guide(bounds, dirs, kappa_raw=6.0):
+    """A guide whose output ignores its input: 8 lobes along `dirs`, log kappa = kappa_raw, equal weights."""
+    gd = guide.Guide(bounds, seed=3, max_batch=1024)
+    p = np.zeros(guide.NPARAMS, np.float32)
+    b5 = np.zeros(40, np.float32)
+    for k in range(8):
+        b5[5 * k:5 * k + 3] = dirs[k]
+        b5[5 * k + 3] = kappa_raw
+    p[24320:24360] = b5
+    gd.params = p
+    return gd
+
+
+def test_furnace_with_fixed_sharp_guide():
+    """A sharp fixed proposal (8 lobes, kappa ~ 400) must not bias the estimate: furnace closed form."""
+    rho, depth = 0.5, 4
+    d = scenes.furnace_box_scene(albedo=rho, emission=1.0)
+    rng = np.random.default_rng(4)
+    dirs = rng.normal(size=(8, 3)).astype(np.float32)
+    gd = _fixed_sharp_guide(np.array([-1, -1, -1, 1, 1, 1], np.float32), dirs)
+    pt = pathtrace.PathTracer(d, 32, 32, max_depth=depth, seed=9)
+    pt.set_guide(gd, alpha=0.5, record_stride=0, train_steps=0, batch=1024)
+    per = []
+    for p in range(96):
+        before = pt.accum.copy()
+        pt.render_pass(p)
+        per.append((pt.accum - before).mean())
+    pt.close()
+    per = np.array(per)
+    want = (1 - rho ** depth) / (1 - rho)
+    m, se = per.mean(), per.std() / math.sqrt(len(per))
+    print(f"[case] furnace fixed sharp guide: {m:.5f} +- {se:.5f} (want {want:.5f})", flush=True)
+    assert abs(m - want) < 4 * se + 1e-3
+
+
+def test_guided_fixed_proposal_matches_oracle():
+    """Guided sampling with a fixed proposal against the oracle's restatement of it: the GPU guide's
+    lobes (read back) drive the oracle.  The vMF density goes through the devices' own exp/log, so
+    a path can change after a last-bit difference; most pixels still agree to 1e-4 and the image
+    means agree within their errors."""
+    d = scenes.guided_room_scene(res=5, nboxes=500)
+    d = dict(d, camera=dict(d["camera"], **{"from": (4.1, 3.3, 0.5), "to": (4.8, 3.0, 0.0)}))
+    W = 48
+    rng = np.random.default_rng(5)
+    dirs = rng.normal(size=(8, 3)).astype(np.float32)
+    gd = _fixed_sharp_guide(pathtrace.scene_bounds(d), dirs)
+    lobes = gd.infer(np.zeros((1, 3), np.float32), np.array([[0, 1, 0]], np.float32))[0]
+    g = pathtrace.PathTracer(d, W, W, max_depth=4, seed=6)
+    g.set_guide(gd, alpha=0.5, record_stride=0, train_steps=0, batch=1024)
+    o = oracle.OraclePathTracer(d, W, W, g.cam, max_depth=4, seed=6)
+    o.set_fixed_guide(0.5, lobes)
+    for p in range(8):
+        g.render_pass(p)
+        o.render_pass(p)
+    a, b = g.accum, o.accum
+    close = np.abs(a - b) <= 1e-4 * np.maximum(np.abs(b), 1e-3)
+    print(f"[case] guided fixed proposal: {close.mean():.4f} of values within 1e-4", flush=True)
+    assert close.mean() > 0.9
+    assert abs(a.mean() - b.mean()) < 0.02 * b.mean()
+    g.close()

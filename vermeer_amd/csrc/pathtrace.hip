@@ -294,10 +294,14 @@ __global__ __launch_bounds__(256) void k_pt_scatter(PTView v, uint32_t pass, int
                     const float k = cosx * w / pdfL;
                     E = f3(rho.x * INV_PI * k * L.Le.x, rho.y * INV_PI * k * L.Le.y, rho.z * INV_PI * k * L.Le.z);
                     if (E.x > 0.0f || E.y > 0.0f || E.z > 0.0f) {
+                        // aimed from the offset origin: with the shading point's distance the light's own
+                        // plane can come before tmax when the offset is large next to the distance
                         shadow = true;
                         sorg = offset(P, N);
-                        sdir = wl;
-                        stmax = dist * 0.9999f;
+                        const F3 sv = sub(y, sorg);
+                        const float sd = sqrtf(dot(sv, sv));
+                        sdir = f3(sv.x / sd, sv.y / sd, sv.z / sd);
+                        stmax = sd * 0.9999f;
                     }
                 }
             }
