@@ -58,8 +58,8 @@ def parse():
     ap.add_argument("--pt-ref-spp", type=int, default=4096, help="passes of the FP64 plain reference render")
     ap.add_argument("--pt-plain-spp", type=int, default=64)
     ap.add_argument("--pt-alpha", type=float, default=0.5)
-    ap.add_argument("--pt-stride", type=int, default=16, help="record the paths of one pixel in this many")
-    ap.add_argument("--pt-steps", type=int, default=4, help="Adam steps per pass")
+    ap.add_argument("--pt-stride", type=int, default=1, help="record the paths of one pixel in this many")
+    ap.add_argument("--pt-steps", type=int, default=16, help="Adam steps per pass")
     ap.add_argument("--pt-batch", type=int, default=65536)
     ap.add_argument("--pt-lr", type=float, default=1e-3)
     ap.add_argument("--pt-warm", type=int, default=0,

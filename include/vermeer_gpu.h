@@ -237,6 +237,12 @@ int vgpu_render_destroy(vgpu_render* render);
 int vgpu_render_pass(vgpu_render* render, uint32_t flags, const uint64_t* scrambles, uint32_t iter, float* film,
                      float* sample_out, void* stream, uint64_t* shadow_rays_out);
 
+/* Before commit: the leafMax argument BuildAccel gets for static meshes (1..16).
+ * The reference uses 16 (builtin/geom/polymesh/buildqbvh.go:32), the default;
+ * hit IDs match the reference only with it.  Path mode, which has no
+ * reference counterpart, may use smaller leaves for fewer triangle tests. */
+int vgpu_scene_set_leaf_size(vgpu_scene* scene, int32_t leaf_max);
+
 /* Kernel timing, a diagnostic with no reference counterpart (the reference
  * reports only ray counts, core/stats.go:14-45).  While enabled, every
  * traversal records HIP events on its stream around the ray sort and the

@@ -43,6 +43,7 @@ def load():
         "orc_scene_free": (None, [P]),
         "orc_scene_add_mesh": (C.c_int, [P, P, C.c_int32, C.c_int32, P, C.c_int32, C.c_float, C.c_int32]),
         "orc_scene_commit": (C.c_int, [P]),
+        "orc_scene_set_leaf_size": (None, [P, C.c_int32]),
         "orc_scene_info": (C.c_int, [P, P, P, P, P, P]),
         "orc_scene_mesh_accel": (C.c_int, [P, C.c_int32, P, P, P]),
         "orc_scene_mesh_motion_accel": (C.c_int, [P, C.c_int32, P, P, P]),
@@ -125,7 +126,8 @@ class OracleScene:
         self.meshes.append((v, ix))
         return g
 
-    def commit(self):
+    def commit(self, leaf_size=16):
+        self.lib.orc_scene_set_leaf_size(self.h, int(leaf_size))
         if self.lib.orc_scene_commit(self.h) != 0:
             raise RuntimeError("oracle scene build failed")
 
@@ -465,7 +467,7 @@ class OraclePathTracer:
     """Plain path mode (alpha = 0) on the CPU.  desc: vermeer_amd.scenes.guided_room_scene() layout;
     cam: a vgpu_camera record (M, tan_theta_focal, aspect, focal)."""
 
-    def __init__(self, desc, width, height, cam, max_depth=8, seed=1):
+    def __init__(self, desc, width, height, cam, max_depth=8, seed=1, leaf_size=16):
         self.lib = _pt_sigs(load())
         self.scene = OracleScene()
         self.w, self.h = width, height
@@ -482,7 +484,7 @@ class OraclePathTracer:
                                                _p(shaders), len(shaders))
             self.scene.meshes.append((v, ix))
             meshes.append((v, ix, sidx, shaders))
-        self.scene.commit()
+        self.scene.commit(leaf_size)
         mats = np.ascontiguousarray(desc["materials"]).view(np.float32).reshape(-1, 8)
         self._keep.append(mats)
         Mx = np.ascontiguousarray(cam["M"], np.float32)

@@ -107,6 +107,11 @@ int orc_scene_add_mesh(void* sp, const float* verts, int32_t nverts, int32_t nke
 }
 
 // PreRender: per-mesh accel, then the scene's top level.
+// leafMax of the static meshes' build (16: the reference's); call before orc_scene_commit.
+void orc_scene_set_leaf_size(void* sp, int32_t leaf_max) {
+    for (auto* g : ((Scene*)sp)->geoms) g->leafMax = leaf_max;
+}
+
 int orc_scene_commit(void* sp) {
     Scene* s = (Scene*)sp;
     try {

@@ -390,6 +390,7 @@ struct Mesh {
     std::vector<Box> motionBounds;
 
     // builtin/geom/polymesh/buildqbvh.go:14-144
+    int leafMax = 16;                 // BuildAccel leafMax (buildqbvh.go:32)
     void initAccel() {
         int n = facecount;
         std::vector<Box> boxes(n);
@@ -420,7 +421,7 @@ struct Mesh {
             for (int c = 0; c < 3; c++) centroids[i][c] = (V0[c] + V1[c] + V2[c]) / 3.0f;
             idxs[i] = i;
         }
-        bounds = BuildAccel(qbvh, boxes.data(), centroids.data(), idxs.data(), n, 16);
+        bounds = BuildAccel(qbvh, boxes.data(), centroids.data(), idxs.data(), n, leafMax);
         accelIdx = idxs;
         std::vector<uint32_t> nidx(idxp.size());
         for (int i = 0; i < n; i++)

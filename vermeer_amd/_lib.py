@@ -66,6 +66,7 @@ SIGNATURES = [
     ("vgpu_scene_set_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
     ("vgpu_scene_get_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
     ("vgpu_scene_autotune", C.c_int, [_P, _U32, _P, _U64, _P, _P, C.POINTER(TraceConfig), C.POINTER(_F)]),
+    ("vgpu_scene_set_leaf_size", C.c_int, [_P, _I32]),
     ("vgpu_scene_set_timing", C.c_int, [_P, _I32]),
     ("vgpu_scene_timing", C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_I32)]),
     ("vgpu_scene_add_shaded_mesh", C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _P, _U32, C.POINTER(_I32)]),

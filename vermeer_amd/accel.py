@@ -189,6 +189,10 @@ class Scene:
         cfg = _lib.TraceConfig(block, lds_stack, waves_per_cu, sort)
         check(self._lib.vgpu_scene_set_trace_config(self._h, C.byref(cfg)))
 
+    def set_leaf_size(self, leaf_max):
+        """leafMax of the static meshes' BVH build (before pre_render; 16 = the reference's)."""
+        check(self._lib.vgpu_scene_set_leaf_size(self._h, int(leaf_max)))
+
     def set_timing(self, enable=True):
         check(self._lib.vgpu_scene_set_timing(self._h, int(bool(enable))))
 

@@ -108,7 +108,7 @@ int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_
     }
     if (e == hipSuccess) e = hipMalloc(&g->tb.partial, (size_t)nsplit * P4);
     if (e == hipSuccess) e = hipMalloc(&g->tb.loss, sizeof(float) * max_batch);
-    if (e == hipSuccess) e = hipMalloc(&g->tb.csum, 256);
+    if (e == hipSuccess) e = hipMalloc(&g->tb.csum, sizeof(float) * (1 + nsplit) + 256);
     if (e == hipSuccess) e = hipMalloc(&g->tb.step, 256);
     if (e == hipSuccess) e = hipMalloc(&g->tb.grad, P4);
     if (e != hipSuccess) { delete g; return fail(VGPU_ERR_HIP, "guide buffers: %s", hipGetErrorString(e)); }

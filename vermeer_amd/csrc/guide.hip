@@ -12,7 +12,7 @@
 // Weight fragments are packed once per parameter update (k_pack) and staged
 // into LDS by every workgroup (50 KB forward, +32 KB transposed for backward).
 //
-// Training (one step over B samples): k_csum sums the sample weights c;
+// Training (one step over B samples): k_csum_part / k_csum_final sum the sample weights c;
 // k_train runs forward, the loss gradient and the backward pass in registers,
 // writing every layer input X and output gradient dY in feature-major bf16;
 // k_dw forms the weight gradients dW = dY X^T as split-K MFMA GEMMs into fp32
@@ -343,17 +343,27 @@ __global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict
 
 // ---------------------------------------------------------------- training
 
-__global__ __launch_bounds__(1024) void k_csum(const TrainSample* __restrict__ S, uint32_t B, float* __restrict__ csum) {
-    __shared__ float red[1024];
+// Sum of the sample weights in two deterministic stages: one partial per KSPLIT samples, then the
+// partials in order (the partial area is the tail of the weight-gradient partial buffer's first row).
+__global__ __launch_bounds__(256) void k_csum_part(const TrainSample* __restrict__ S, float* __restrict__ part) {
+    __shared__ float red[256];
+    const uint32_t base = blockIdx.x * KSPLIT;
     float a = 0.0f;
-    for (uint32_t i = threadIdx.x; i < B; i += 1024) a += S[i].c;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)KSPLIT; i += 256) a += S[base + i].c;
     red[threadIdx.x] = a;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) csum[0] = red[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(64) void k_csum_final(const float* __restrict__ part, uint32_t nsplit, float* __restrict__ csum) {
+    if (threadIdx.x != 0) return;
+    float a = 0.0f;
+    for (uint32_t i = 0; i < nsplit; i++) a += part[i];
+    csum[0] = a;
 }
 
 // Masks the back-propagated tile with the layer input (ReLU derivative), stores it and repacks
@@ -587,9 +597,10 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_csum, dim3(1), dim3(1024), 0, st, samples, B, tb.csum);
-    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb);
     const uint32_t nsplit = B / KSPLIT;
+    hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, samples, tb.csum + 1);
+    hipLaunchKernelGGL(k_csum_final, dim3(1), dim3(64), 0, st, (const float*)(tb.csum + 1), nsplit, tb.csum);
+    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb);
     const uint32_t waves = NJOB * nsplit;
     hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsplit);
     hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, params, m, v, tb.grad, tb.partial, nsplit,

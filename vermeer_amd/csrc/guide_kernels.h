@@ -45,7 +45,7 @@ struct TrainBuffers {
     uint16_t* dY[5];    // output gradients, feature-major bf16: 64, 64, 64, 64, 48 (rows 40..47 zero)
     float* partial;     // [nsplit][NPARAM]
     float* loss;        // per sample
-    float* csum;        // [0]: sum of c over the batch
+    float* csum;        // [0]: sum of c over the batch; [1 .. 1 + max_batch/KSPLIT): per-split partial sums
     int32_t* step;      // Adam step count (device)
     float* grad;        // last mean gradient (NPARAM)
     uint32_t ldb;       // row stride (samples) of X / dY

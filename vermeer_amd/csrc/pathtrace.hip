@@ -381,38 +381,41 @@ __global__ __launch_bounds__(256) void k_pt_accumulate(PTView v, PTBuffers b, do
 //   R_j = E_j + w_j (wB_{j+1} Le_{j+1} + R_{j+1})        (radiance leaving j, emission excluded)
 //   L_in(j) = Le_{j+1} + R_{j+1}                         (radiance arriving at j along its sample)
 // Vertex j yields the sample (P_j, N_j, dir_j) with c = lum(L_in(j)) cos_j / p_j when c > 0.
+// The wave walks depth max_depth-1 .. 0 in lockstep (lanes past their path's end idle) so that each
+// depth's samples are appended with one atomic per wave.
 __global__ __launch_bounds__(256) void k_pt_gather(PTView v, uint32_t pass, PTBuffers b) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= v.width * v.height) return;
-    const int n = b.plen[p];
-    if (n <= 0) return;
-    const float4* R = b.rec + (size_t)p * v.max_depth * PT_REC_F4;
+    const int n = p < v.width * v.height ? b.plen[p] : 0;
+    const float4* R = b.rec + (size_t)(p < v.width * v.height ? p : 0) * v.max_depth * PT_REC_F4;
     F3 Rn = f3(0, 0, 0), Len = f3(0, 0, 0);
     float wBn = 0.0f;
-    for (int j = n - 1; j >= 0; j--) {
-        const float4* r = R + (size_t)j * PT_REC_F4;
-        const bool next = j + 1 < n;
-        const float4 r5 = r[5];
-        if (next && r5.w != 0.0f) {
-            const F3 Lin = add(Len, Rn);
-            const float4 r0 = r[0], r1 = r[1], r2 = r[2];
-            const float c = lum(Lin) * r1.w / r0.w;
-            if (c > 0.0f && c < 3.0e38f) {
-                const uint32_t k = atomicAdd(b.count + 3, 1u);
-                if (k < b.tcap) {
-                    guide::TrainSample s;
+    for (int j = v.max_depth - 1; j >= 0; j--) {
+        const bool live = j < n;
+        bool emit = false;
+        guide::TrainSample s{};
+        if (live) {
+            const float4* r = R + (size_t)j * PT_REC_F4;
+            const bool next = j + 1 < n;
+            const float4 r5 = r[5];
+            if (next && r5.w != 0.0f) {
+                const F3 Lin = add(Len, Rn);
+                const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+                const float c = lum(Lin) * r1.w / r0.w;
+                if (c > 0.0f && c < 3.0e38f) {
+                    emit = true;
                     s.px = r0.x; s.py = r0.y; s.pz = r0.z; s.c = c;
-                    s.nx = r1.x; s.ny = r1.y; s.nz = r1.z; s.pad0 = 0.0f;
-                    s.dx = r2.x; s.dy = r2.y; s.dz = r2.z; s.pad1 = 0.0f;
-                    b.tsamples[k] = s;
+                    s.nx = r1.x; s.ny = r1.y; s.nz = r1.z;
+                    s.dx = r2.x; s.dy = r2.y; s.dz = r2.z;
                 }
             }
+            const F3 inner = next ? add(mul(Len, wBn), Rn) : f3(0, 0, 0);
+            const float4 r3 = r[3], r4 = r[4];
+            Rn = add(f3(r4), mul(f3(r5), inner));
+            Len = f3(r3);
+            wBn = r3.w;
         }
-        const F3 inner = next ? add(mul(Len, wBn), Rn) : f3(0, 0, 0);
-        const float4 r3 = r[3], r4 = r[4];
-        Rn = add(f3(r4), mul(f3(r5), inner));
-        Len = f3(r3);
-        wBn = r3.w;
+        const uint32_t k = wave_append(b.count + 3, emit);
+        if (emit && k < b.tcap) b.tsamples[k] = s;
     }
 }
 

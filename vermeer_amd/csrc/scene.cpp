@@ -40,7 +40,7 @@ using namespace vgpu_detail;
 namespace vgpu_detail {
 
 // Static mesh accel, as PolyMesh.initAccel (buildqbvh.go:60-129).
-int build_static_mesh(MeshHost& m) {
+int build_static_mesh(MeshHost& m, int leaf_max) {
     const uint32_t n = m.ntris;
     std::vector<float> boxes((size_t)n * 6), cent((size_t)n * 3);
     std::vector<int32_t> ids(n);
@@ -55,8 +55,8 @@ int build_static_mesh(MeshHost& m) {
     }
     int32_t nn = 0;
     std::vector<vgpu_qbvh_node> nodes(std::max<uint32_t>(n, 1));
-    int rc = vgpu::build_qbvh(boxes.data(), cent.data(), ids.data(), (int32_t)n, 16, nodes.data(), (int32_t)nodes.size(),
-                              &nn, m.bounds, 0);
+    int rc = vgpu::build_qbvh(boxes.data(), cent.data(), ids.data(), (int32_t)n, leaf_max, nodes.data(),
+                              (int32_t)nodes.size(), &nn, m.bounds, 0);
     if (rc == -3) return fail(VGPU_ERR_BUILD, "BVH build failed: degenerate mesh (coincident centroids)");
     if (rc != 0) return fail(VGPU_ERR_BUILD, "BVH build failed (%d)", rc);
     nodes.resize(nn);
@@ -379,7 +379,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
     if (s->meshes.empty()) return fail(VGPU_ERR_STATE, "scene has no geometry");
     for (auto& m : s->meshes) {
         if (m.built) continue;
-        int rc = m.nkeys == 1 ? build_static_mesh(m) : build_motion_mesh(m);
+        int rc = m.nkeys == 1 ? build_static_mesh(m, s->leaf_max) : build_motion_mesh(m);
         if (rc) return rc;
     }
     // Scene.initAccel (scene.go:135-203)
@@ -602,6 +602,14 @@ int vgpu_trace(vgpu_scene* s, uint32_t flags, const vgpu_ray* rays, uint64_t n, 
     HIP_TRY(hipMemcpyAsync(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
+    return VGPU_OK;
+}
+
+int vgpu_scene_set_leaf_size(vgpu_scene* s, int32_t leaf_max) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "leaf size must be set before commit");
+    if (leaf_max < 1 || leaf_max > 16) return fail(VGPU_ERR_ARG, "leaf size %d outside [1,16]", leaf_max);
+    s->leaf_max = leaf_max;
     return VGPU_OK;
 }
 

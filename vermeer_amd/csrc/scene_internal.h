@@ -117,6 +117,7 @@ struct vgpu_scene {
     vgpu::DevGeomShade* d_gshade = nullptr;
     int32_t max_material = -1;
     vgpu_trace_config cfg{256, 16, 16, 0};
+    int32_t leaf_max = 16;              // leafMax of the static meshes' BuildAccel (buildqbvh.go:32: 16)
     float box[6] = {0, 0, 0, 0, 0, 0};   // scene bounds over all keys (ray-sort grid)
     vgpu::RaySortBuffers sort{};
     // kernel timing (vgpu_scene_set_timing): events [sort start, trace start, trace end] per launch

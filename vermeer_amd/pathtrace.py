@@ -19,10 +19,12 @@ def camera_for(desc, width, height):
     return lookat_camera(c["from"], c["to"], c["up"], c["fov"], c["focal"], width, height)
 
 
-def build_scene(desc, device=0):
-    """Committed GPU scene of the description's shaded meshes (shareable by several PathTracers)."""
+def build_scene(desc, device=0, leaf_size=16):
+    """Committed GPU scene of the description's shaded meshes (shareable by several PathTracers).
+    leaf_size: BVH leafMax of the meshes (16, the reference's, keeps the reference's hit IDs)."""
     lib = _lib.load()
     scene = Scene(device)
+    scene.set_leaf_size(leaf_size)
     for mesh in desc["meshes"]:
         v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(-1, 3)
         ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
@@ -44,10 +46,10 @@ def scene_bounds(desc, margin=0.05):
 
 
 class PathTracer:
-    def __init__(self, desc, width, height, max_depth=8, seed=1, cam=None, device=0, scene=None):
+    def __init__(self, desc, width, height, max_depth=8, seed=1, cam=None, device=0, scene=None, leaf_size=16):
         self._lib = _lib.load()
         self._own_scene = scene is None
-        self.scene = build_scene(desc, device) if scene is None else scene
+        self.scene = build_scene(desc, device, leaf_size) if scene is None else scene
         self.width, self.height = width, height
         mats = np.ascontiguousarray(desc["materials"], PT_MATERIAL_DTYPE)
         self.cam = np.ascontiguousarray(cam if cam is not None else camera_for(desc, width, height), CAMERA_DTYPE)

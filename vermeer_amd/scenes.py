@@ -510,7 +510,7 @@ def guided_room_scene(seed=11, res=20, nboxes=66667, emission=2000.0, amplitude=
         faces.append(f + nv)
         sidx.append(np.full(len(f), 3, np.int64))
         nv += len(P)
-    cam_from = np.array([-3.5, 1.9, -0.4])
+    cam_from = np.array([2.6, 1.7, -2.6])
     if nboxes:
         corners, btris = _unit_box_tris()
         C = np.empty((0, 3))
@@ -538,7 +538,9 @@ def guided_room_scene(seed=11, res=20, nboxes=66667, emission=2000.0, amplitude=
     LP = np.array([(lx, ly0, 1.0), (lx, ly0, -1.0), (lx, ly1, -1.0), (lx, ly1, 1.0)], np.float32)
     light = {"verts": LP, "idx": np.array([[0, 1, 2], [0, 2, 3]], np.uint32), "shaderidx": None,
              "shaders": np.array([light_mat], np.int32), "normals": None}
-    camera = {"from": tuple(cam_from), "to": (4.0, 2.2, 0.3), "up": (0.0, -1.0, 0.0), "fov": 75.0, "focal": 1.0}
+    # looking away from the slit (it is behind the camera): every visible surface is lit only by light
+    # that left the alcove through the slit
+    camera = {"from": tuple(cam_from), "to": (-4.0, 1.3, 1.2), "up": (0.0, -1.0, 0.0), "fov": 75.0, "focal": 1.0}
     return {"meshes": [main, light], "materials": np.array(mats, PT_MATERIAL_DTYPE), "camera": camera}
 
 
