@@ -59,9 +59,10 @@ def parse():
     ap.add_argument("--pt-plain-spp", type=int, default=64)
     ap.add_argument("--pt-alpha", type=float, default=0.5)
     ap.add_argument("--pt-stride", type=int, default=1, help="record the paths of one pixel in this many")
-    ap.add_argument("--pt-steps", type=int, default=16, help="Adam steps per pass")
+    ap.add_argument("--pt-steps", type=int, default=32, help="Adam steps per pass")
     ap.add_argument("--pt-batch", type=int, default=65536)
     ap.add_argument("--pt-lr", type=float, default=1e-3)
+    ap.add_argument("--pt-leaf", type=int, default=2, help="BVH leaf size of the room (16 = the reference's)")
     ap.add_argument("--pt-warm", type=int, default=0,
                     help="guided passes left out of the image (their time still counts)")
     return ap.parse_args()
@@ -81,7 +82,7 @@ def guided_measurement(args, torch, np):
     t0 = time.time()
     desc = scenes.guided_room_scene()
     ntri = int(sum(len(m["idx"]) for m in desc["meshes"]))
-    scene = pathtrace.build_scene(desc)
+    scene = pathtrace.build_scene(desc, leaf_size=args.pt_leaf)
     log(f"[bench] guided: room with {ntri} triangles built in {time.time() - t0:.1f}s")
     npx = W * W
 
@@ -151,7 +152,7 @@ def guided_measurement(args, torch, np):
     log(f"[bench] guided: reached plain rmse after {p} passes, {t_g:.1f} ms (ratio {ratio}); "
         f"{guided_mrays:.0f} Mrays/s")
     return {
-        "scene": "occluded-light room", "triangles": ntri, "image": [W, W], "max_depth": 8,
+        "scene": "occluded-light room", "triangles": ntri, "image": [W, W], "max_depth": 8, "bvh_leaf_size": args.pt_leaf,
         "reference_spp": args.pt_ref_spp, "reference_mean": round(float(ref.mean()), 6),
         "plain_spp": args.pt_plain_spp, "plain_rmse": round(e_plain, 6), "plain_ms": round(t_plain, 2),
         "plain_samples_per_s": round(npx * args.pt_plain_spp / (t_plain * 1e-3), 1),
@@ -216,8 +217,10 @@ def main():
         sc.set_trace_config(b, l, w, so)
         tuned_ms = None
     else:
-        cfg, tuned_ms = sc.autotune(rp, n, hp, stream=st)
-        log(f"[bench] autotuned trace config {cfg} ({tuned_ms:.2f} ms per full pass)")
+        # tuned on one bounce's launch (the size every step issues), sort time included
+        s0, c0 = int(seg_start[0]), int(seg_counts[0])
+        cfg, tuned_ms = sc.autotune(rp + s0 * 32, c0, hp + s0 * 16, stream=st)
+        log(f"[bench] autotuned trace config {cfg} ({tuned_ms:.3f} ms per {c0}-ray launch)")
     cfg = sc.trace_config()
 
     # counting pass (untimed): per-ray node visits and triangle tests for the byte model

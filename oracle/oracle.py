@@ -488,8 +488,9 @@ class OraclePathTracer:
         mats = np.ascontiguousarray(desc["materials"]).view(np.float32).reshape(-1, 8)
         self._keep.append(mats)
         Mx = np.ascontiguousarray(cam["M"], np.float32)
-        self.h_pt = self.lib.orc_pt_new(self.scene.h, _p(mats), len(mats), _p(Mx), float(cam["tan_theta_focal"]),
-                                        float(cam["aspect"]), float(cam["focal"]), width, height, max_depth, seed)
+        ttf, asp, foc = (float(np.asarray(cam[k]).reshape(-1)[0]) for k in ("tan_theta_focal", "aspect", "focal"))
+        self.h_pt = self.lib.orc_pt_new(self.scene.h, _p(mats), len(mats), _p(Mx), ttf, asp, foc, width, height,
+                                        max_depth, seed)
         for v, ix, sidx, shaders in meshes:
             self.lib.orc_pt_add_mesh_lights(self.h_pt, _p(v), _p(ix), len(ix), _p(sidx), _p(shaders), len(shaders))
         self.accum = np.zeros((height, width, 3), np.float64)

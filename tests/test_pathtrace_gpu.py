@@ -6,8 +6,8 @@ Bars:
 - furnace: plain and guided (alpha = 0.5, with online training) estimates converge to
   Le (1 - rho^D) / (1 - rho) within 4 standard errors: the one-sample MIS mixture stays unbiased;
 - guided and plain agree on the occluded-light room's mean radiance within their errors;
-- after online training, the guided estimate of the room at equal samples has a lower error
-  against a high-sample plain render than the plain estimate.
+- after online training, the guided estimate of the room at equal samples has an RMSE below 0.8x
+  the plain estimate's, against a high-sample plain render.
 """
 import math
 
@@ -89,32 +89,64 @@ def _render(desc, W, passes, guided, seed, first_pass=0, train_passes=None):
     return img
 
 
+def _per_pass_means(desc, W, passes, guided, seed):
+    pt = pathtrace.PathTracer(desc, W, W, max_depth=8, seed=seed, leaf_size=2)
+    if guided:
+        gd = guide.Guide(pathtrace.scene_bounds(desc), seed=1, max_batch=16384)
+        pt.set_guide(gd, alpha=0.5, record_stride=1, train_steps=4, batch=16384, lr=2e-3)
+        pt._gd = gd
+    per = []
+    for p in range(passes):
+        before = pt.accum.copy()
+        pt.render_pass(p)
+        per.append((pt.accum - before).mean())
+    pt.close()
+    return np.array(per)
+
+
 def test_guided_and_plain_agree_on_the_room(room):
+    """The guided estimator (trained online from the first pass) and plain sampling estimate the same
+    image mean: within 4 combined standard errors of their per-pass means."""
     print("[case] room mean guided vs plain", flush=True)
-    W, passes = 64, 64
-    a = _render(room, W, passes, False, seed=21)
-    b = _render(room, W, passes, True, seed=22)
-    # compare means over 8x8 blocks of pixels: independent estimates of the same quantity
-    ba = a.mean(2).reshape(8, 8, 8, 8).mean((1, 3)).ravel()
-    bb = b.mean(2).reshape(8, 8, 8, 8).mean((1, 3)).ravel()
-    ma, mb = a.mean(), b.mean()
-    print(f"[case] plain {ma:.4f} guided {mb:.4f}", flush=True)
-    assert abs(ma - mb) < 0.05 * ma
-    assert np.corrcoef(ba, bb)[0, 1] > 0.9
+    a = _per_pass_means(room, 64, 128, False, seed=21)
+    b = _per_pass_means(room, 64, 128, True, seed=22)
+    se = math.hypot(a.std() / math.sqrt(len(a)), b.std() / math.sqrt(len(b)))
+    print(f"[case] plain {a.mean():.4f} guided {b.mean():.4f} (combined se {se:.4f})", flush=True)
+    assert abs(a.mean() - b.mean()) < 4 * se
 
 
-@pytest.mark.skip(reason="with the guide trained during the same 32 passes it does not yet beat plain sampling "
-                         "(RMSE 0.45 vs 0.34); the training schedule is being tuned")
 def test_guiding_reduces_error_at_equal_samples(room):
+    """After online training, the guided estimate at equal samples has a markedly lower RMSE against a
+    high-sample plain render than plain sampling (the room's light reaches the camera's view only
+    through the slit, which the guide learns to aim at)."""
     print("[case] guided vs plain RMSE", flush=True)
-    W = 64
-    ref = _render(room, W, 1024, False, seed=100)
-    plain = _render(room, W, 32, False, seed=7)
-    guided = _render(room, W, 32, True, seed=8, first_pass=0)
+    W = 128
+    scene = pathtrace.build_scene(room, leaf_size=2)
+
+    def render(passes, guided, seed, train_passes=0):
+        pt = pathtrace.PathTracer(room, W, W, max_depth=8, seed=seed, scene=scene)
+        if guided:
+            gd = guide.Guide(pathtrace.scene_bounds(room), seed=1, max_batch=16384)
+            pt.set_guide(gd, alpha=0.5, record_stride=1, train_steps=16, batch=16384, lr=3e-3)
+            pt._gd = gd
+            for p in range(train_passes):
+                pt.render_pass(10_000 + p)
+            pt.accum[:] = 0
+            pt.passes = 0
+        for p in range(passes):
+            pt.render_pass(p)
+        img = pt.image
+        pt.close()
+        return img
+
+    ref = render(512, False, 100)
+    plain = render(32, False, 7)
+    guided = render(32, True, 8, train_passes=48)
     e_p = math.sqrt(np.mean((plain - ref) ** 2))
     e_g = math.sqrt(np.mean((guided - ref) ** 2))
-    print(f"[case] RMSE plain {e_p:.4f} guided {e_g:.4f}", flush=True)
+    print(f"[case] RMSE plain {e_p:.4f} guided {e_g:.4f} ratio {e_g / e_p:.3f}", flush=True)
     assert e_g < 0.8 * e_p
+    scene.close()
 
 
 def _fixed_sharp_guide(bounds, dirs, kappa_raw=6.0):

@@ -291,6 +291,8 @@ __global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict
     const uint64_t nw = (uint64_t)gridDim.x * 4;
     const float* P = net.params;
     for (uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * 64; base < nn; base += nw * 64) {
+        // keeps the weight fragments' LDS reads inside the loop: hoisted, they would pin 200 registers
+        __asm__ volatile("" ::: "memory");
         float x[NT][6];
         uint64_t s[NT];
 #pragma unroll
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __res
 // Weight-gradient partials: one wave per (16x16 tile of some dW_l, split of KSPLIT samples).
 constexpr int NJOB = 4 * 9 + 3 * 16 + 3 * 4;
 
-__global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit) {
+__global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, uint32_t ks) {
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t job = w % NJOB, split = w / NJOB;
@@ -521,8 +523,8 @@ __global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit) {
     const uint16_t* Bp = tb.X[l] + (uint64_t)(16 * it + c) * ldb;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     float bs = 0.0f;
-    const uint64_t s_end = (uint64_t)(split + 1) * KSPLIT;
-    for (uint64_t s0 = (uint64_t)split * KSPLIT; s0 < s_end; s0 += 32) {
+    const uint64_t s_end = (uint64_t)(split + 1) * ks;
+    for (uint64_t s0 = (uint64_t)split * ks; s0 < s_end; s0 += 32) {
         const u32x4 av = *reinterpret_cast<const u32x4*>(A + s0 + 8 * g);
         const u32x4 bv = *reinterpret_cast<const u32x4*>(Bp + s0 + 8 * g);
         const bf16x8 a = __builtin_bit_cast(bf16x8, av);
@@ -601,9 +603,12 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
     hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, samples, tb.csum + 1);
     hipLaunchKernelGGL(k_csum_final, dim3(1), dim3(64), 0, st, (const float*)(tb.csum + 1), nsplit, tb.csum);
     hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb);
-    const uint32_t waves = NJOB * nsplit;
-    hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsplit);
-    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, params, m, v, tb.grad, tb.partial, nsplit,
+    // weight gradients in splits of 4096 samples when the batch allows (fewer partials to reduce)
+    const uint32_t ks = B % (4 * KSPLIT) == 0 ? 4 * KSPLIT : KSPLIT;
+    const uint32_t nsw = B / ks;
+    const uint32_t waves = NJOB * nsw;
+    hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsw, ks);
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, params, m, v, tb.grad, tb.partial, nsw,
                        1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0);
     if (update) {
         hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, st, tb.step, tb.csum);
