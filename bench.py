@@ -330,9 +330,10 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "k_trace_traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
-        if tj.get("workload") == "icosphere_field_bounce_rays" and tj["kernel"].startswith(
-                f"k_trace<{cfg['block']},{cfg['lds_stack']},") and tj.get("sort", 0) == cfg["sort"]:
-            traffic = round(tj["hbm_bytes_per_ray"] * n / len(segs))
+        if tj.get("workload") == "icosphere_field_bounce_rays":
+            for ent in tj.get("entries", []):
+                if (ent["block"], ent["lds_stack"], ent["sort"]) == (cfg["block"], cfg["lds_stack"], cfg["sort"]):
+                    traffic = round(ent["hbm_bytes_per_ray"] * n / len(segs))
 
     out = {
         "metric": METRIC,
