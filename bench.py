@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work for the baseline sample")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay measurement")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided path-tracing measurement")
+    ap.add_argument("--no-others", action="store_true", help="skip the sphere/renderer/motion measurements")
     ap.add_argument("--pt-size", type=int, default=1024, help="guided measurement: image width = height")
     ap.add_argument("--pt-ref-spp", type=int, default=4096, help="passes of the FP64 plain reference render")
     ap.add_argument("--pt-plain-spp", type=int, default=64)
@@ -66,6 +67,113 @@ def parse():
     ap.add_argument("--pt-warm", type=int, default=0,
                     help="guided passes left out of the image (their time still counts)")
     return ap.parse_args()
+
+
+def _time_calls(torch, fn, reps):
+    """GPU milliseconds per call of fn() (HIP events on the current stream), after one warm call."""
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def other_workloads(torch, np):
+    """The remaining configurations, measured briefly on the same GPU (rays resident on the device):
+    sphere + ground primary and uniform rays, the reference-semantics renderer on the box of spheres,
+    and motion-blurred traversal of 2000 moving spheres."""
+    import vermeer_amd as va
+    from vermeer_amd import render, scenes
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+    out = {}
+
+    def dev_rays(r):
+        return torch.from_numpy(np.ascontiguousarray(r).view(np.uint8).copy()).to(dev)
+
+    # sphere (256 x 128) + ground: 1024^2 pinhole rays, 1M uniform rays; closest and any hit
+    sc = va.Scene(0)
+    sc.add_geom(va.PolyMesh(*scenes.uv_sphere()))
+    sc.add_geom(va.PolyMesh(*scenes.ground_quad()))
+    sc.pre_render()
+    res = {}
+    for name, rays in (("pinhole_1024x1024", scenes.pinhole_rays()), ("uniform_1M", scenes.uniform_rays())):
+        d = dev_rays(rays)
+        n = len(rays)
+        h = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+        for shadow in (False, True):
+            ms = _time_calls(torch, lambda: sc.trace_device(d.data_ptr(), n, h.data_ptr(), shadow=shadow, stream=st), 20)
+            res[f"{name}_{'anyhit' if shadow else 'closest'}_mrays_per_s"] = round(n / ms / 1e3, 1)
+    sc.status(st)
+    sc.close()
+    out["sphere_ground"] = dict(res, triangles=65026)
+    log(f"[bench] sphere+ground: {res}")
+
+    # reference-semantics renderer: box + 8 displaced spheres, 512^2, 16 passes of RasterXY QMC
+    desc = scenes.box_spheres_scene()
+    c = desc["camera"]
+    W = 512
+    cam = render.lookat_camera(c["from"], c["to"], c["up"], c["fov"], c["focal"], W, W)
+    R = render.Renderer(desc, W, W, cam)
+    scr = torch.from_numpy(scenes.pixel_scrambles(W, W).view(np.int64).reshape(-1).copy()).to(dev)
+    film = torch.zeros(W * W * 3, dtype=torch.float32, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    R.render_pass_device(scr.data_ptr(), 1, film.data_ptr(), stream=st)
+    e0.record()
+    for it in range(1, 17):
+        R.render_pass_device(scr.data_ptr(), it, film.data_ptr(), stream=st)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    R.scene.status(st)
+    out["box_spheres_render"] = {"image": [W, W], "passes": 16, "ms": round(ms, 2),
+                                 "samples_per_s": round(W * W * 16 / (ms * 1e-3), 1),
+                                 "triangles": int(sum(len(m["idx"]) for m in desc["meshes"]))}
+    R.close()
+    log(f"[bench] box+spheres renderer: {out['box_spheres_render']}")
+
+    # motion blur: 2000 spheres x 250 triangles, 2 keys; 1024^2 x 4 camera rays, time from the van der Corput
+    # sequence; then one any-hit shadow ray per hit towards a point light (D = L - P, tmax 1 - 1e-4)
+    verts, idx = scenes.moving_spheres()
+    sc = va.Scene(0)
+    sc.add_geom(va.PolyMesh(verts, idx))
+    sc.pre_render()
+    base = scenes.pinhole_rays(1024, 1024, fov_deg=60.0, eye=(0.0, 0.0, 45.0), target=(0.0, 0.0, 0.0))
+    rays = np.concatenate([base] * 4)
+    k = np.arange(len(rays), dtype=np.uint64)
+    rev = np.zeros(len(rays), np.float64)
+    for b in range(32):
+        rev += ((k >> np.uint64(b)) & np.uint64(1)).astype(np.float64) * 2.0 ** -(b + 1)
+    rays["time"] = rev.astype(np.float32)
+    n = len(rays)
+    d = dev_rays(rays)
+    h = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+    ms_cam = _time_calls(torch, lambda: sc.trace_device(d.data_ptr(), n, h.data_ptr(), stream=st), 5)
+    hits = h.view(torch.int32).view(n, 4)
+    dr = d.view(torch.float32).view(n, 8)
+    t = hits[:, 1].view(torch.float32)
+    hit = hits[:, 0] >= 0
+    P = dr[:, 0:3] + dr[:, 4:7] * torch.where(hit, t, torch.zeros_like(t))[:, None]
+    L = torch.tensor([10.0, 30.0, 20.0], device=dev)
+    srays = torch.zeros_like(dr)
+    srays[:, 0:3] = P + (L - P) * 1e-4
+    srays[:, 4:7] = L - P
+    srays[:, 3] = torch.where(hit, torch.full_like(t, 1.0 - 1e-4), torch.full_like(t, -1.0))
+    srays[:, 7] = dr[:, 7]
+    sh = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+    ms_sh = _time_calls(torch, lambda: sc.trace_device(srays.data_ptr(), n, sh.data_ptr(), shadow=True, stream=st), 5)
+    sc.status(st)
+    sc.close()
+    out["moving_spheres"] = {"triangles": int(len(idx)), "keys": 2, "camera_rays": n,
+                             "camera_closest_mrays_per_s": round(n / ms_cam / 1e3, 1),
+                             "shadow_anyhit_mrays_per_s": round(n / ms_sh / 1e3, 1),
+                             "combined_mrays_per_s": round(2 * n / (ms_cam + ms_sh) / 1e3, 1),
+                             "hit_fraction": round(float(hit.float().mean()), 4)}
+    log(f"[bench] moving spheres: {out['moving_spheres']}")
+    return out
 
 
 def guided_measurement(args, torch, np):
@@ -379,10 +487,13 @@ def main():
         "host_build_s": round(build_s, 2),
         "guided": None,
     }
-    if not args.no_guided:
+    if not args.no_guided or not args.no_others:
         del d_hits, d_cnt, d_rays
         sc.close()
         torch.cuda.empty_cache()
+    if not args.no_others:
+        out["other_workloads"] = other_workloads(torch, np)
+    if not args.no_guided:
         out["guided"] = guided_measurement(args, torch, np)
     print(json.dumps(out), flush=True)
 
