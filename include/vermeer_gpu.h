@@ -256,7 +256,7 @@ int vgpu_scene_timing(vgpu_scene* scene, float* trace_ms, float* sort_ms, int32_
 /* Traversal launch configuration (the autotuner's knobs). */
 typedef struct vgpu_trace_config {
     int32_t block;        /* threads per workgroup: 64, 128 or 256 */
-    int32_t lds_stack;    /* stack entries per ray held in LDS: 8, 16, 24 or 32; the rest spill to HBM */
+    int32_t lds_stack;    /* stack entries per ray held in LDS: 8, 12, 16, 24 or 32; the rest spill to HBM */
     int32_t waves_per_cu; /* resident-grid sizing: workgroups = CUs * waves_per_cu * 64 / block */
     int32_t sort;         /* 0: trace in the given order; k in 1..6: first reorder the rays by bucket =
                            * (Morton cell of the origin in a 2^k-per-axis grid over the scene box) x

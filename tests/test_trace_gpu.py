@@ -145,7 +145,7 @@ def test_empty_batch(sphere_ground):
     assert len(hits) == 0
 
 
-@pytest.mark.parametrize("cfg", [(64, 8, 8), (128, 24, 16), (256, 32, 32), (256, 8, 4)])
+@pytest.mark.parametrize("cfg", [(64, 8, 8), (64, 12, 32), (128, 24, 16), (256, 32, 32), (256, 8, 4)])
 def test_every_launch_config_gives_identical_hits(sphere_ground, cfg):
     g, o = sphere_ground
     rays = scenes.uniform_rays(300_000, seed=12)

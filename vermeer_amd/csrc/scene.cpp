@@ -183,8 +183,8 @@ int ensure_spill(vgpu_scene* s, uint64_t threads) {
 
 int check_config(const vgpu_trace_config& c) {
     if (c.block != 64 && c.block != 128 && c.block != 256) return fail(VGPU_ERR_ARG, "block must be 64, 128 or 256");
-    if (c.lds_stack != 8 && c.lds_stack != 16 && c.lds_stack != 24 && c.lds_stack != 32)
-        return fail(VGPU_ERR_ARG, "lds_stack must be 8, 16, 24 or 32");
+    if (c.lds_stack != 8 && c.lds_stack != 12 && c.lds_stack != 16 && c.lds_stack != 24 && c.lds_stack != 32)
+        return fail(VGPU_ERR_ARG, "lds_stack must be 8, 12, 16, 24 or 32");
     if (c.block * c.lds_stack * 8 > 65536) return fail(VGPU_ERR_ARG, "block*lds_stack exceeds 64 KiB of LDS");
     if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return fail(VGPU_ERR_ARG, "waves_per_cu must be in [1,32]");
     if (c.sort < 0 || c.sort > 6) return fail(VGPU_ERR_ARG, "sort must be 0 (off) or 1..6 grid bits per axis");
@@ -677,7 +677,7 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
     DeviceGuard dg(s->device);
     hipStream_t st = (hipStream_t)stream;
     const int blocks[] = {64, 128, 256};
-    const int ldss[] = {8, 16, 24, 32};
+    const int ldss[] = {8, 12, 16, 24, 32};
     const int waves[] = {8, 16, 32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));

@@ -424,9 +424,9 @@ hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const Tra
     if (cfg.block == B && cfg.lds_depth == L)                                                                   \
         return launch_mode<B, L>(sv, shadow, motion, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, \
                                  stream);
-    VGPU_CASE(64, 8) VGPU_CASE(64, 16) VGPU_CASE(64, 24) VGPU_CASE(64, 32)
-    VGPU_CASE(128, 8) VGPU_CASE(128, 16) VGPU_CASE(128, 24) VGPU_CASE(128, 32)
-    VGPU_CASE(256, 8) VGPU_CASE(256, 16) VGPU_CASE(256, 24) VGPU_CASE(256, 32)
+    VGPU_CASE(64, 8) VGPU_CASE(64, 12) VGPU_CASE(64, 16) VGPU_CASE(64, 24) VGPU_CASE(64, 32)
+    VGPU_CASE(128, 8) VGPU_CASE(128, 12) VGPU_CASE(128, 16) VGPU_CASE(128, 24) VGPU_CASE(128, 32)
+    VGPU_CASE(256, 8) VGPU_CASE(256, 12) VGPU_CASE(256, 16) VGPU_CASE(256, 24) VGPU_CASE(256, 32)
 #undef VGPU_CASE
     return hipErrorInvalidValue;
 }
