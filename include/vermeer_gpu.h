@@ -275,7 +275,7 @@ int vgpu_scene_autotune(vgpu_scene* scene, uint32_t flags, const vgpu_ray* d_ray
 
 /* ------------------------------------------------------------------------
  * Path-guiding network.  No reference counterpart: the reference samples
- * only its BSDFs (builtin/shader/bsdf/*.go).  A small MLP maps a hit
+ * only its BSDFs (the Go files of builtin/shader/bsdf).  A small MLP maps a hit
  * position (normalised by the scene bounds) and shading normal, with a
  * 12-frequency sin/cos encoding (144 features), through 4 hidden layers of
  * 64 (ReLU) to 8 von Mises-Fisher lobes (mu, kappa, weight).  Weights are

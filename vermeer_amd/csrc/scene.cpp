@@ -532,6 +532,13 @@ int vgpu_scene_commit(vgpu_scene* s) {
     s->view.geoms = s->d_geoms;
     s->view.top_motion = s->top_motion ? 1 : 0;
     s->view.top_keys = s->top_keys;
+    s->view.nan_free_boxes = 1;
+    for (const vgpu::DevNode& d : dnodes) {
+        float bx[24];
+        std::memcpy(bx, d.lox, sizeof(bx));
+        for (int k = 0; k < 24; k++)
+            if (std::isnan(bx[k])) s->view.nan_free_boxes = 0;
+    }
     s->committed = true;
     return VGPU_OK;
 }

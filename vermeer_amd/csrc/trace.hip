@@ -82,12 +82,27 @@ struct Box4 {
     bool h[4];
 };
 
+// FAST: every slab value is known not to be NaN (finite origin, finite non-zero 1/D, no NaN in
+// the node boxes).  MINPS/MAXPS then agree with IEEE min/max except for which zero of +0/-0
+// they return, and the entry distances are only ever compared, where +0 == -0, so the
+// hardware min/max (v_min3/v_max3) give the same hit flags and the same traversal.
+template <bool FAST>
 __device__ __forceinline__ void box4(const RayS& r, const float4& lx, const float4& ly, const float4& lz,
                                      const float4& hx, const float4& hy, const float4& hz, Box4& b) {
     const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
     const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
+        if (FAST) {
+            const float x1 = (lxa[c] - r.P[0]) * r.Dinv[0], x2 = (hxa[c] - r.P[0]) * r.Dinv[0];
+            const float y1 = (lya[c] - r.P[1]) * r.Dinv[1], y2 = (hya[c] - r.P[1]) * r.Dinv[1];
+            const float z1 = (lza[c] - r.P[2]) * r.Dinv[2], z2 = (hza[c] - r.P[2]) * r.Dinv[2];
+            const float tn = fmaxf(fmaxf(fmaxf(fminf(x1, x2), fminf(y1, y2)), fminf(z1, z2)), 0.0f);
+            const float tx = fminf(fminf(fmaxf(x1, x2), fmaxf(y1, y2)), fmaxf(z1, z2));
+            b.t[c] = tn;
+            b.h[c] = tn <= tx;
+            continue;
+        }
         float tmin, tmax;
         slab(lxa[c], hxa[c], r.P[0], r.Dinv[0], tmin, tmax, true);
         slab(lya[c], hya[c], r.P[1], r.Dinv[1], tmin, tmax, false);
@@ -118,20 +133,39 @@ struct Stack {
 __device__ __forceinline__ int leaf_base(int32_t l) { return (int)((l & 0x7ffffff) >> 4); }
 __device__ __forceinline__ int leaf_count(int32_t l) { return (int)((l & 0xf) + 1); }
 
-// Watertight ray/triangle test (builtin/geom/polymesh/trace.go:123-194 static, :556-624 motion).
-// Returns true when the triangle is accepted; updates tclosest, U, V.
+// Ray quantities the watertight test reads: the shear constants and the permuted origin.
+struct TriRay {
+    float S0, S1, S2, Pkx, Pky, Pkz;
+    int Kx, Ky, Kz;
+};
+
+__device__ __forceinline__ TriRay tri_ray(const RayS& r) {
+    TriRay q;
+    q.S0 = r.S[0]; q.S1 = r.S[1]; q.S2 = r.S[2];
+    q.Pkx = r.Pkx; q.Pky = r.Pky; q.Pkz = r.Pkz;
+    q.Kx = r.Kx; q.Ky = r.Ky; q.Kz = r.Kz;
+    return q;
+}
+
+struct TriPre {
+    float T, det, fU, fV, fW;
+};
+
+// Watertight ray/triangle test (builtin/geom/polymesh/trace.go:123-194 static, :556-624 motion),
+// the part that does not read the closest distance so far: edge functions (retried in double
+// precision when one is zero), the sign test, det != 0 and the near bound.
 template <bool MOTION_BOUND>
-__device__ __forceinline__ bool tri_test(RayS& r, const float4& v0, const float4& v1, const float4& v2, float bias,
-                                         float& U, float& V, float& W) {
-    const float AKz = sel3(v0.x, v0.y, v0.z, r.Kz) - r.Pkz;
-    const float BKz = sel3(v1.x, v1.y, v1.z, r.Kz) - r.Pkz;
-    const float CKz = sel3(v2.x, v2.y, v2.z, r.Kz) - r.Pkz;
-    const float Cx = (sel3(v2.x, v2.y, v2.z, r.Kx) - r.Pkx) - r.S[0] * CKz;
-    const float By = (sel3(v1.x, v1.y, v1.z, r.Ky) - r.Pky) - r.S[1] * BKz;
-    const float Cy = (sel3(v2.x, v2.y, v2.z, r.Ky) - r.Pky) - r.S[1] * CKz;
-    const float Bx = (sel3(v1.x, v1.y, v1.z, r.Kx) - r.Pkx) - r.S[0] * BKz;
-    const float Ax = (sel3(v0.x, v0.y, v0.z, r.Kx) - r.Pkx) - r.S[0] * AKz;
-    const float Ay = (sel3(v0.x, v0.y, v0.z, r.Ky) - r.Pky) - r.S[1] * AKz;
+__device__ __forceinline__ bool tri_pre(const TriRay& q, const float4& v0, const float4& v1, const float4& v2, float bias,
+                                        TriPre& p) {
+    const float AKz = sel3(v0.x, v0.y, v0.z, q.Kz) - q.Pkz;
+    const float BKz = sel3(v1.x, v1.y, v1.z, q.Kz) - q.Pkz;
+    const float CKz = sel3(v2.x, v2.y, v2.z, q.Kz) - q.Pkz;
+    const float Cx = (sel3(v2.x, v2.y, v2.z, q.Kx) - q.Pkx) - q.S0 * CKz;
+    const float By = (sel3(v1.x, v1.y, v1.z, q.Ky) - q.Pky) - q.S1 * BKz;
+    const float Cy = (sel3(v2.x, v2.y, v2.z, q.Ky) - q.Pky) - q.S1 * CKz;
+    const float Bx = (sel3(v1.x, v1.y, v1.z, q.Kx) - q.Pkx) - q.S0 * BKz;
+    const float Ax = (sel3(v0.x, v0.y, v0.z, q.Kx) - q.Pkx) - q.S0 * AKz;
+    const float Ay = (sel3(v0.x, v0.y, v0.z, q.Ky) - q.Pky) - q.S1 * AKz;
     float fU = Cx * By - Cy * Bx;
     float fV = Ax * Cy - Ay * Cx;
     float fW = Bx * Ay - By * Ax;
@@ -143,21 +177,39 @@ __device__ __forceinline__ bool tri_test(RayS& r, const float4& v0, const float4
     if ((fU < 0.0f || fV < 0.0f || fW < 0.0f) && (fU > 0.0f || fV > 0.0f || fW > 0.0f)) return false;
     const float det = fU + fV + fW;
     if (det == 0.0f) return false;
-    const float T = r.S[2] * (fU * AKz + fV * BKz + fW * CKz);
+    const float T = q.S2 * (fU * AKz + fV * BKz + fW * CKz);
     const int sgn = __float_as_int(det) & (int)0x80000000;
     const float Tx = __int_as_float(__float_as_int(T) ^ sgn);
     const float Dx = __int_as_float(__float_as_int(det) ^ sgn);
     if (MOTION_BOUND) {
-        if (Tx <= bias * Dx || Tx > r.tclosest * Dx) return false;
+        if (Tx <= bias * Dx) return false;
     } else {
-        if (Tx < (1.19209290E-07f + bias) * Dx || Tx > r.tclosest * Dx) return false;
+        if (Tx < (1.19209290E-07f + bias) * Dx) return false;
     }
-    const float rcp = 1.0f / det;
-    U = fU * rcp;
-    V = fV * rcp;
-    W = fW * rcp;
-    r.tclosest = T * rcp;
+    p.T = T; p.det = det; p.fU = fU; p.fV = fV; p.fW = fW;
     return true;
+}
+
+// The rest of the test: far bound against the closest distance so far, then the hit update.
+__device__ __forceinline__ bool tri_accept(RayS& r, const TriPre& p, float& U, float& V, float& W) {
+    const int sgn = __float_as_int(p.det) & (int)0x80000000;
+    const float Tx = __int_as_float(__float_as_int(p.T) ^ sgn);
+    const float Dx = __int_as_float(__float_as_int(p.det) ^ sgn);
+    if (Tx > r.tclosest * Dx) return false;
+    const float rcp = 1.0f / p.det;
+    U = p.fU * rcp;
+    V = p.fV * rcp;
+    W = p.fW * rcp;
+    r.tclosest = p.T * rcp;
+    return true;
+}
+
+// The whole test.  Returns true when the triangle is accepted; updates tclosest, U, V, W.
+template <bool MOTION_BOUND>
+__device__ __forceinline__ bool tri_test(RayS& r, const float4& v0, const float4& v1, const float4& v2, float bias,
+                                         float& U, float& V, float& W) {
+    TriPre p;
+    return tri_pre<MOTION_BOUND>(tri_ray(r), v0, v1, v2, bias, p) && tri_accept(r, p, U, V, W);
 }
 
 __device__ __forceinline__ float4 lerp4(const float4& a, const float4& b, float t) {
@@ -228,6 +280,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
         const float D[3] = {rb.x, rb.y, rb.z};
         r.time = rb.w;
         ray_setup(r, D);
+        // Wave-uniform choice of the box test (see box4): the fast form when no lane can make a
+        // NaN slab value.  Motion boxes are interpolated, and 0 * inf there gives NaN, so motion
+        // traversal always takes the exact form.
+        bool fast = false;
+        if (!MOTION && sv.nan_free_boxes) {
+            bool ok = true;
+            for (int k = 0; k < 3; k++) ok = ok && absf(r.P[k]) < INFINITY && absf(r.Dinv[k]) < INFINITY && r.Dinv[k] != 0.0f;
+            fast = __all(ok ? 1 : 0) != 0;
+        }
 
         int nodesT = 0, leafsT = 0, trisT = 0, maxDepth = 0;
         int best_geom = -1, best_prim = -1;
@@ -276,7 +337,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
                     else fetch_static(sv.nodes, node, nd);
                 }
                 Box4 b;
-                box4(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
                 // Push order of qbvh/intersect.go:137-216 (same as motionintersect.go:53-97): the
                 // children are pushed as two pairs; rank(c) is child c's position in that sequence.
                 // Hit children are stored compacted in rank order, i.e. exactly the entries the
@@ -356,16 +418,238 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
     }
 }
 
+// Static scenes.  The same traversal as k_trace, one step per loop iteration, except that
+// the triangles of the leaves popped in a step are tested by the whole wavefront: lane l
+// owns a leaf of c_l triangles, the sum of the c_l triangle tests is dealt out over the 64
+// lanes (consecutive lanes take consecutive triangles of one leaf, so the loads coalesce),
+// and each lane reads the owning ray's shear constants with a cross-lane shuffle.  That part
+// of the watertight test does not read the closest distance; the owner then runs the full
+// test, in leaf order, on the triangles that passed it, which accepts the same triangles in
+// the same order as testing every triangle of the leaf in turn.  Lanes whose ray is finished
+// stay in the loop as workers until the whole wavefront is done.
+template <int BLOCK, int LDS, bool SHADOW, bool COUNTERS, bool EXT>
+__global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4* __restrict__ rays, uint64_t n,
+                                                      int4* __restrict__ hits, int32_t* __restrict__ geoms_out,
+                                                      int4* __restrict__ counters, float* __restrict__ spT,
+                                                      int32_t* __restrict__ spN, int32_t* __restrict__ err,
+                                                      const uint32_t* __restrict__ n_dev,
+                                                      const uint32_t* __restrict__ perm) {
+    static_assert(BLOCK % 64 == 0, "whole wavefronts");
+    __shared__ float sT[LDS * BLOCK];
+    __shared__ int32_t sN[LDS * BLOCK];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
+    const uint64_t nthr = (uint64_t)gridDim.x * BLOCK;
+    Stack<BLOCK, LDS> st;
+    st.sT = sT + tid; st.sN = sN + tid;
+    st.gT = spT + gtid; st.gN = spN + gtid; st.gstride = nthr;
+
+    if (n_dev) n = min(n, (uint64_t)*n_dev);
+    // The loop bound is the wavefront's first ray, so all 64 lanes run every iteration together.
+    for (uint64_t rb = gtid - (uint64_t)lane; rb < n; rb += nthr) {
+        const uint64_t ri = rb + (uint64_t)lane;
+        const bool live = ri < n;
+        RayS r;
+        float D[3] = {1.0f, 1.0f, 1.0f};
+        r.P[0] = 0.0f; r.P[1] = 0.0f; r.P[2] = 0.0f; r.tclosest = 0.0f; r.time = 0.0f;
+        if (live) {
+            const float4 ra = rays[2 * ri], rb4 = rays[2 * ri + 1];
+            r.P[0] = ra.x; r.P[1] = ra.y; r.P[2] = ra.z; r.tclosest = ra.w;
+            D[0] = rb4.x; D[1] = rb4.y; D[2] = rb4.z;
+            r.time = rb4.w;
+        }
+        ray_setup(r, D);
+        bool fast = false;
+        if (sv.nan_free_boxes) {
+            bool ok = true;
+            for (int k = 0; k < 3; k++) ok = ok && absf(r.P[k]) < INFINITY && absf(r.Dinv[k]) < INFINITY && r.Dinv[k] != 0.0f;
+            fast = __all(ok ? 1 : 0) != 0;
+        }
+
+        int nodesT = 0, leafsT = 0, trisT = 0, maxDepth = 0;
+        int best_geom = -1, best_prim = -1;
+        float bu = 0.0f, bv = 0.0f, bw = 0.0f;
+        bool in_mesh = false;
+        int mesh_base = 0, gcur = 0, gend = 0;
+        DevGeom g = {0, 0, 1, 0.0f};
+        bool overflow = false;
+        bool done = !live;
+        if (live) st.store(0, r.tclosest, 0);
+        int sp = 1;
+        while (true) {
+            bool pend = false;   // this lane popped a triangle leaf: [lbase, lbase + lcount) of geom gcur
+            int lbase = 0, lcount = 0;
+            if (!done) {
+                do {
+                    if (in_mesh && sp == mesh_base) {
+                        // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
+                        gcur++;
+                        if (gcur < gend) {
+                            g = sv.geoms[gcur];
+                            st.store(mesh_base, r.tclosest, 0);
+                            sp = mesh_base + 1;
+                            break;
+                        }
+                        in_mesh = false;
+                        sp = mesh_base - 1;
+                    }
+                    if (sp == 0) { done = true; break; }
+                    sp--;
+                    float T; int32_t node;
+                    st.load(sp, T, node);
+                    if (r.tclosest < T || node == -1) break;
+                    if (node >= 0) {
+                        if (COUNTERS) nodesT++;
+                        NodeData nd;
+                        fetch_static(sv.nodes, in_mesh ? g.node_base + node : node, nd);
+                        Box4 b;
+                        if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                        else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                        // push order of qbvh/intersect.go:137-216, as in k_trace
+                        const int n0 = (r.neg >> nd.ax[0]) & 1;
+                        const int n1 = (r.neg >> nd.ax[1]) & 1;
+                        const int n2 = (r.neg >> nd.ax[2]) & 1;
+                        const int gA = n0 ? 0 : 2, gB = 2 - gA;
+                        const int rk0 = gA + (n1 ^ 1), rk1 = gA + n1, rk2 = gB + (n2 ^ 1), rk3 = gB + n2;
+                        const int m = (b.h[0] ? 1 << rk0 : 0) | (b.h[1] ? 1 << rk1 : 0) | (b.h[2] ? 1 << rk2 : 0) |
+                                      (b.h[3] ? 1 << rk3 : 0);
+                        if (sp + 4 > VGPU_MAX_STACK) {
+                            overflow = true;
+                            done = true;
+                            break;
+                        }
+                        if (b.h[0]) st.store(sp + __popc(m & ((1 << rk0) - 1)), b.t[0], nd.ch[0]);
+                        if (b.h[1]) st.store(sp + __popc(m & ((1 << rk1) - 1)), b.t[1], nd.ch[1]);
+                        if (b.h[2]) st.store(sp + __popc(m & ((1 << rk2) - 1)), b.t[2], nd.ch[2]);
+                        if (b.h[3]) st.store(sp + __popc(m & ((1 << rk3) - 1)), b.t[3], nd.ch[3]);
+                        sp += __popc(m);
+                        if (COUNTERS) maxDepth = max(maxDepth, sp);
+                    } else {
+                        const int base = leaf_base(node), count = leaf_count(node);
+                        if (COUNTERS) leafsT++;
+                        if (in_mesh) {
+                            if (COUNTERS) trisT += count;
+                            pend = true;
+                            lbase = base;
+                            lcount = count;
+                        } else {
+                            // Scene.TraceElems: geoms [base, base+count) (scene.go:61-78)
+                            mesh_base = sp + 1;
+                            if (mesh_base >= VGPU_MAX_STACK) { overflow = true; done = true; break; }
+                            gcur = base; gend = base + count;
+                            g = sv.geoms[gcur];
+                            st.store(mesh_base, r.tclosest, 0);
+                            sp = mesh_base + 1;
+                            in_mesh = true;
+                        }
+                    }
+                } while (false);
+            }
+
+            if (__ballot(pend) != 0) {
+                // Deal the leaves' triangle tests out over the wavefront: inclusive prefix sum of the
+                // counts; work item w belongs to the first lane whose prefix exceeds w.
+                const int c = pend ? lcount : 0;
+                int pin = c;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int y = __shfl_up(pin, d);
+                    if (lane >= d) pin += y;
+                }
+                const int total = __shfl(pin, 63);
+                const int pex = pin - c;
+                const int tb = g.tri_base + lbase;
+                const TriRay q = tri_ray(r);
+                const int kpack = q.Kx | (q.Ky << 2) | (q.Kz << 4);
+                const float bias = g.ray_bias;
+                uint32_t cand = 0;   // owner: leaf triangles that passed the distance-free part of the test
+                for (int k0 = 0; k0 < total; k0 += 64) {
+                    const int w = k0 + lane;
+                    int o = 0;
+#pragma unroll
+                    for (int s = 32; s >= 1; s >>= 1)
+                        if (__shfl(pin, o + s - 1) <= w) o += s;
+                    const int i = w - __shfl(pex, o);
+                    TriRay oq;
+                    oq.S0 = __shfl(q.S0, o); oq.S1 = __shfl(q.S1, o); oq.S2 = __shfl(q.S2, o);
+                    oq.Pkx = __shfl(q.Pkx, o); oq.Pky = __shfl(q.Pky, o); oq.Pkz = __shfl(q.Pkz, o);
+                    const int ok = __shfl(kpack, o);
+                    oq.Kx = ok & 3; oq.Ky = (ok >> 2) & 3; oq.Kz = (ok >> 4) & 3;
+                    const float obias = __shfl(bias, o);
+                    const int otb = __shfl(tb, o);
+                    bool pass = false;
+                    if (w < total) {
+                        const float4* tp = sv.tris + (size_t)(otb + i) * 3;
+                        TriPre pr;
+                        pass = tri_pre<false>(oq, tp[0], tp[1], tp[2], obias, pr);
+                    }
+                    const uint64_t cm = __ballot(pass);
+                    if (pend && cm != 0) {
+                        const int s0 = max(pex, k0), e0 = min(pin, k0 + 64);
+                        if (e0 > s0)
+                            cand |= (uint32_t)((cm >> (s0 - k0)) & ((1ull << (e0 - s0)) - 1ull)) << (s0 - pex);
+                    }
+                }
+                if (cand != 0) {
+                    // the owner's sequential pass over the candidates, in leaf order (trace.go:108-122)
+                    int idx = -1;
+                    float U = 0.0f, V = 0.0f, W = 0.0f;
+                    const float4* tp = sv.tris + (size_t)tb * 3;
+                    while (cand != 0) {
+                        const int i = __ffs(cand) - 1;
+                        cand &= cand - 1;
+                        if (tri_test<false>(r, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
+                            idx = lbase + i;
+                    }
+                    if (idx >= 0) {
+                        best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
+                        if (SHADOW) done = true;   // qbvh/intersect.go:232-236 at both levels
+                    }
+                }
+            }
+            if (__ballot(!done) == 0) break;
+        }
+        if (live) {
+            if (overflow) {
+                atomicOr(err, 1);
+                best_geom = -1; best_prim = -1;
+            }
+            const uint64_t oi = perm ? (uint64_t)perm[ri] : ri;
+            if (EXT) {
+                hits[2 * oi] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
+                hits[2 * oi + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
+            } else {
+                hits[oi] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
+                if (geoms_out) geoms_out[oi] = best_geom;
+            }
+            if (COUNTERS) counters[oi] = make_int4(nodesT, leafsT, trisT, maxDepth);
+        }
+    }
+}
+
 template <int BLOCK, int LDS, bool SHADOW, bool MOTION>
 hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
                       float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev,
                       const uint32_t* perm, hipStream_t stream) {
-    if (counters)
-        hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, MOTION, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err, n_dev, perm);
-    else
-        hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, MOTION, false>), dim3((unsigned)grid), dim3(BLOCK), 0, stream, sv,
-                           (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err, n_dev, perm);
+    if constexpr (!MOTION) {
+        if (counters)
+            hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, true, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
+                               stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err,
+                               n_dev, perm);
+        else
+            hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, false, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
+                               stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err,
+                               n_dev, perm);
+    } else {
+        if (counters)
+            hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, true, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream,
+                               sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err, n_dev,
+                               perm);
+        else
+            hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, true, false>), dim3((unsigned)grid), dim3(BLOCK), 0, stream,
+                               sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err, n_dev,
+                               perm);
+    }
     return hipGetLastError();
 }
 
@@ -398,7 +682,7 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr);
     else
-        hipLaunchKernelGGL((k_trace<256, 16, false, false, false, true>), dim3(grid), dim3(256), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace_wide<256, 16, false, false, true>), dim3(grid), dim3(256), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr);
     return hipGetLastError();

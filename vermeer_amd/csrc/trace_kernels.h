@@ -47,7 +47,7 @@ struct SceneView {
     int32_t top_motion;       // 1: the top level is a motion QBVH at mnodes[top_mbase]
     int32_t top_keys;         // motion keys of the top level
     int32_t top_mbase;
-    int32_t pad;
+    int32_t nan_free_boxes;   // 1: no static node box holds a NaN bound
 };
 
 struct TraceLaunch {
