@@ -55,11 +55,14 @@ __device__ __forceinline__ void ray_setup(RayS& r, const float D[3]) {
     int Ky = Kx + 1; if (Ky == 3) Ky = 0;
     const float dkz = sel3(D[0], D[1], D[2], Kz);
     if (dkz < 0.0f) { int t = Kx; Kx = Ky; Ky = t; }
-    const double z = (double)dkz;
-    r.S[2] = (float)(1.0 / z);
-    r.S[0] = (float)((double)sel3(D[0], D[1], D[2], Kx) / z);
-    r.S[1] = (float)((double)sel3(D[0], D[1], D[2], Ky) / z);
-    for (int k = 0; k < 3; k++) r.Dinv[k] = (float)(1.0 / (double)D[k]);
+    // The reference divides in float64 and rounds the quotient to float32.  For float32
+    // operands that double rounding is innocuous (53 >= 2 * 24 + 2 bits: the float64 quotient
+    // never lands on a float32 rounding midpoint unless the exact quotient does), so a
+    // correctly rounded float32 division gives the same bits.
+    r.S[2] = 1.0f / dkz;
+    r.S[0] = sel3(D[0], D[1], D[2], Kx) / dkz;
+    r.S[1] = sel3(D[0], D[1], D[2], Ky) / dkz;
+    for (int k = 0; k < 3; k++) r.Dinv[k] = 1.0f / D[k];
     r.Kx = Kx; r.Ky = Ky; r.Kz = Kz;
     r.neg = (D[0] < 0.0f ? 1 : 0) | (D[1] < 0.0f ? 2 : 0) | (D[2] < 0.0f ? 4 : 0);
     r.Pkx = sel3(r.P[0], r.P[1], r.P[2], Kx);
@@ -433,7 +436,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                                                       int4* __restrict__ counters, float* __restrict__ spT,
                                                       int32_t* __restrict__ spN, int32_t* __restrict__ err,
                                                       const uint32_t* __restrict__ n_dev,
-                                                      const uint32_t* __restrict__ perm) {
+                                                      const uint32_t* __restrict__ perm, int refill_min) {
     static_assert(BLOCK % 64 == 0, "whole wavefronts");
     __shared__ float sT[LDS * BLOCK];
     __shared__ int32_t sN[LDS * BLOCK];
@@ -446,200 +449,231 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
     st.gT = spT + gtid; st.gN = spN + gtid; st.gstride = nthr;
 
     if (n_dev) n = min(n, (uint64_t)*n_dev);
-    // The loop bound is the wavefront's first ray, so all 64 lanes run every iteration together.
-    for (uint64_t rb = gtid - (uint64_t)lane; rb < n; rb += nthr) {
-        const uint64_t ri = rb + (uint64_t)lane;
-        const bool live = ri < n;
-        RayS r;
-        float D[3] = {1.0f, 1.0f, 1.0f};
-        r.P[0] = 0.0f; r.P[1] = 0.0f; r.P[2] = 0.0f; r.tclosest = 0.0f; r.time = 0.0f;
-        if (live) {
-            const float4 ra = rays[2 * ri], rb4 = rays[2 * ri + 1];
-            r.P[0] = ra.x; r.P[1] = ra.y; r.P[2] = ra.z; r.tclosest = ra.w;
-            D[0] = rb4.x; D[1] = rb4.y; D[2] = rb4.z;
-            r.time = rb4.w;
-        }
-        ray_setup(r, D);
-        bool fast = false;
-        if (sv.nan_free_boxes) {
-            bool ok = true;
-            for (int k = 0; k < 3; k++) ok = ok && absf(r.P[k]) < INFINITY && absf(r.Dinv[k]) < INFINITY && r.Dinv[k] != 0.0f;
-            fast = __all(ok ? 1 : 0) != 0;
-        }
-
-        int nodesT = 0, leafsT = 0, trisT = 0, maxDepth = 0;
-        int best_geom = -1, best_prim = -1;
-        float bu = 0.0f, bv = 0.0f, bw = 0.0f;
-        bool in_mesh = false;
-        int mesh_base = 0, gcur = 0, gend = 0;
-        DevGeom g = {0, 0, 1, 0.0f};
-        bool overflow = false;
-        bool done = !live;
-        if (live) st.store(0, r.tclosest, 0);
-        int sp = 1;
-        while (true) {
-            bool pend = false;   // this lane popped a triangle leaf: [lbase, lbase + lcount) of geom gcur
-            int lbase = 0, lcount = 0;
-            if (!done) {
-                do {
-                    if (in_mesh && sp == mesh_base) {
-                        // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
-                        gcur++;
-                        if (gcur < gend) {
-                            g = sv.geoms[gcur];
-                            st.store(mesh_base, r.tclosest, 0);
-                            sp = mesh_base + 1;
-                            break;
-                        }
-                        in_mesh = false;
-                        sp = mesh_base - 1;
+    // Rays of this wavefront: v = 0, 1, 2, ... -> ray wave0 + (v / 64) * nthr + v % 64 (increasing
+    // in v).  A lane whose ray is finished idles (and serves as a worker in the leaf tests) until
+    // refill_min lanes are idle; then the idle lanes write their results and take the next rays
+    // of the wavefront together, so the ray set-up runs once for many lanes.
+    const uint64_t wave0 = gtid - (uint64_t)lane;
+    uint32_t cursor = 0;
+    bool done = true, live = false, ok = true, fast = false;
+    uint64_t ri = 0;
+    RayS r;
+    r.P[0] = 0.0f; r.P[1] = 0.0f; r.P[2] = 0.0f; r.tclosest = 0.0f; r.time = 0.0f;
+    {
+        const float D1[3] = {1.0f, 1.0f, 1.0f};
+        ray_setup(r, D1);
+    }
+    int nodesT = 0, leafsT = 0, trisT = 0, maxDepth = 0;
+    int best_geom = -1, best_prim = -1;
+    float bu = 0.0f, bv = 0.0f, bw = 0.0f;
+    bool in_mesh = false;
+    int mesh_base = 0, gcur = 0, gend = 0;
+    DevGeom g = {0, 0, 1, 0.0f};
+    bool overflow = false;
+    int sp = 0;
+    while (true) {
+        const uint64_t dm = __ballot(done);
+        const uint64_t next = wave0 + (uint64_t)(cursor >> 6) * nthr + (cursor & 63u);
+        const bool more = next < n;
+        if (more ? __popcll(dm) >= refill_min : dm == ~0ull) {
+            if (done) {
+                if (live) {
+                    if (overflow) {
+                        atomicOr(err, 1);
+                        best_geom = -1; best_prim = -1;
                     }
-                    if (sp == 0) { done = true; break; }
-                    sp--;
-                    float T; int32_t node;
-                    st.load(sp, T, node);
-                    if (r.tclosest < T || node == -1) break;
-                    if (node >= 0) {
-                        if (COUNTERS) nodesT++;
-                        NodeData nd;
-                        fetch_static(sv.nodes, in_mesh ? g.node_base + node : node, nd);
-                        Box4 b;
-                        if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
-                        else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
-                        // push order of qbvh/intersect.go:137-216, as in k_trace
-                        const int n0 = (r.neg >> nd.ax[0]) & 1;
-                        const int n1 = (r.neg >> nd.ax[1]) & 1;
-                        const int n2 = (r.neg >> nd.ax[2]) & 1;
-                        const int gA = n0 ? 0 : 2, gB = 2 - gA;
-                        const int rk0 = gA + (n1 ^ 1), rk1 = gA + n1, rk2 = gB + (n2 ^ 1), rk3 = gB + n2;
-                        const int m = (b.h[0] ? 1 << rk0 : 0) | (b.h[1] ? 1 << rk1 : 0) | (b.h[2] ? 1 << rk2 : 0) |
-                                      (b.h[3] ? 1 << rk3 : 0);
-                        if (sp + 4 > VGPU_MAX_STACK) {
-                            overflow = true;
-                            done = true;
-                            break;
-                        }
-                        if (b.h[0]) st.store(sp + __popc(m & ((1 << rk0) - 1)), b.t[0], nd.ch[0]);
-                        if (b.h[1]) st.store(sp + __popc(m & ((1 << rk1) - 1)), b.t[1], nd.ch[1]);
-                        if (b.h[2]) st.store(sp + __popc(m & ((1 << rk2) - 1)), b.t[2], nd.ch[2]);
-                        if (b.h[3]) st.store(sp + __popc(m & ((1 << rk3) - 1)), b.t[3], nd.ch[3]);
-                        sp += __popc(m);
-                        if (COUNTERS) maxDepth = max(maxDepth, sp);
+                    const uint64_t oi = perm ? (uint64_t)perm[ri] : ri;
+                    if (EXT) {
+                        hits[2 * oi] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
+                        hits[2 * oi + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
                     } else {
-                        const int base = leaf_base(node), count = leaf_count(node);
-                        if (COUNTERS) leafsT++;
-                        if (in_mesh) {
-                            if (COUNTERS) trisT += count;
-                            pend = true;
-                            lbase = base;
-                            lcount = count;
-                        } else {
-                            // Scene.TraceElems: geoms [base, base+count) (scene.go:61-78)
-                            mesh_base = sp + 1;
-                            if (mesh_base >= VGPU_MAX_STACK) { overflow = true; done = true; break; }
-                            gcur = base; gend = base + count;
-                            g = sv.geoms[gcur];
-                            st.store(mesh_base, r.tclosest, 0);
-                            sp = mesh_base + 1;
-                            in_mesh = true;
-                        }
+                        hits[oi] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
+                        if (geoms_out) geoms_out[oi] = best_geom;
                     }
-                } while (false);
-            }
-
-            if (__ballot(pend) != 0) {
-                // Deal the leaves' triangle tests out over the wavefront: inclusive prefix sum of the
-                // counts; work item w belongs to the first lane whose prefix exceeds w.
-                const int c = pend ? lcount : 0;
-                int pin = c;
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int y = __shfl_up(pin, d);
-                    if (lane >= d) pin += y;
+                    if (COUNTERS) counters[oi] = make_int4(nodesT, leafsT, trisT, maxDepth);
+                    live = false;
                 }
-                const int total = __shfl(pin, 63);
-                const int pex = pin - c;
-                const int tb = g.tri_base + lbase;
-                const TriRay q = tri_ray(r);
-                const int kpack = q.Kx | (q.Ky << 2) | (q.Kz << 4);
-                const float bias = g.ray_bias;
-                uint32_t cand = 0;   // owner: leaf triangles that passed the distance-free part of the test
-                for (int k0 = 0; k0 < total; k0 += 64) {
-                    const int w = k0 + lane;
-                    int o = 0;
-#pragma unroll
-                    for (int s = 32; s >= 1; s >>= 1)
-                        if (__shfl(pin, o + s - 1) <= w) o += s;
-                    const int i = w - __shfl(pex, o);
-                    TriRay oq;
-                    oq.S0 = __shfl(q.S0, o); oq.S1 = __shfl(q.S1, o); oq.S2 = __shfl(q.S2, o);
-                    oq.Pkx = __shfl(q.Pkx, o); oq.Pky = __shfl(q.Pky, o); oq.Pkz = __shfl(q.Pkz, o);
-                    const int ok = __shfl(kpack, o);
-                    oq.Kx = ok & 3; oq.Ky = (ok >> 2) & 3; oq.Kz = (ok >> 4) & 3;
-                    const float obias = __shfl(bias, o);
-                    const int otb = __shfl(tb, o);
-                    bool pass = false;
-                    if (w < total) {
-                        const float4* tp = sv.tris + (size_t)(otb + i) * 3;
-                        TriPre pr;
-                        pass = tri_pre<false>(oq, tp[0], tp[1], tp[2], obias, pr);
-                    }
-                    const uint64_t cm = __ballot(pass);
-                    if (pend && cm != 0) {
-                        const int s0 = max(pex, k0), e0 = min(pin, k0 + 64);
-                        if (e0 > s0)
-                            cand |= (uint32_t)((cm >> (s0 - k0)) & ((1ull << (e0 - s0)) - 1ull)) << (s0 - pex);
-                    }
-                }
-                if (cand != 0) {
-                    // the owner's sequential pass over the candidates, in leaf order (trace.go:108-122)
-                    int idx = -1;
-                    float U = 0.0f, V = 0.0f, W = 0.0f;
-                    const float4* tp = sv.tris + (size_t)tb * 3;
-                    while (cand != 0) {
-                        const int i = __ffs(cand) - 1;
-                        cand &= cand - 1;
-                        if (tri_test<false>(r, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
-                            idx = lbase + i;
-                    }
-                    if (idx >= 0) {
-                        best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
-                        if (SHADOW) done = true;   // qbvh/intersect.go:232-236 at both levels
+                if (more) {
+                    const uint32_t v = cursor + (uint32_t)__popcll(dm & ((1ull << lane) - 1ull));
+                    ri = wave0 + (uint64_t)(v >> 6) * nthr + (v & 63u);
+                    live = ri < n;
+                    if (live) {
+                        const float4 ra = rays[2 * ri], rb4 = rays[2 * ri + 1];
+                        r.P[0] = ra.x; r.P[1] = ra.y; r.P[2] = ra.z; r.tclosest = ra.w;
+                        const float D[3] = {rb4.x, rb4.y, rb4.z};
+                        r.time = rb4.w;
+                        ray_setup(r, D);
+                        ok = true;
+                        for (int k = 0; k < 3; k++)
+                            ok = ok && absf(r.P[k]) < INFINITY && absf(r.Dinv[k]) < INFINITY && r.Dinv[k] != 0.0f;
+                        nodesT = 0; leafsT = 0; trisT = 0; maxDepth = 0;
+                        best_geom = -1; best_prim = -1;
+                        bu = 0.0f; bv = 0.0f; bw = 0.0f;
+                        in_mesh = false;
+                        overflow = false;
+                        st.store(0, r.tclosest, 0);
+                        sp = 1;
+                        done = false;
+                    } else {
+                        ok = true;
                     }
                 }
             }
-            if (__ballot(!done) == 0) break;
+            if (!more) break;
+            cursor += (uint32_t)__popcll(dm);
+            fast = sv.nan_free_boxes && __all(ok ? 1 : 0) != 0;
         }
-        if (live) {
-            if (overflow) {
-                atomicOr(err, 1);
-                best_geom = -1; best_prim = -1;
+        bool pend = false;   // this lane popped a triangle leaf: [lbase, lbase + lcount) of geom gcur
+        int lbase = 0, lcount = 0;
+        if (!done) {
+            do {
+                if (in_mesh && sp == mesh_base) {
+                    // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
+                    gcur++;
+                    if (gcur < gend) {
+                        g = sv.geoms[gcur];
+                        st.store(mesh_base, r.tclosest, 0);
+                        sp = mesh_base + 1;
+                        break;
+                    }
+                    in_mesh = false;
+                    sp = mesh_base - 1;
+                }
+                if (sp == 0) { done = true; break; }
+                sp--;
+                float T; int32_t node;
+                st.load(sp, T, node);
+                if (r.tclosest < T || node == -1) break;
+                if (node >= 0) {
+                    if (COUNTERS) nodesT++;
+                    NodeData nd;
+                    fetch_static(sv.nodes, in_mesh ? g.node_base + node : node, nd);
+                    Box4 b;
+                    if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                    else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                    // push order of qbvh/intersect.go:137-216, as in k_trace
+                    const int n0 = (r.neg >> nd.ax[0]) & 1;
+                    const int n1 = (r.neg >> nd.ax[1]) & 1;
+                    const int n2 = (r.neg >> nd.ax[2]) & 1;
+                    const int gA = n0 ? 0 : 2, gB = 2 - gA;
+                    const int rk0 = gA + (n1 ^ 1), rk1 = gA + n1, rk2 = gB + (n2 ^ 1), rk3 = gB + n2;
+                    const int m = (b.h[0] ? 1 << rk0 : 0) | (b.h[1] ? 1 << rk1 : 0) | (b.h[2] ? 1 << rk2 : 0) |
+                                  (b.h[3] ? 1 << rk3 : 0);
+                    if (sp + 4 > VGPU_MAX_STACK) {
+                        overflow = true;
+                        done = true;
+                        break;
+                    }
+                    if (b.h[0]) st.store(sp + __popc(m & ((1 << rk0) - 1)), b.t[0], nd.ch[0]);
+                    if (b.h[1]) st.store(sp + __popc(m & ((1 << rk1) - 1)), b.t[1], nd.ch[1]);
+                    if (b.h[2]) st.store(sp + __popc(m & ((1 << rk2) - 1)), b.t[2], nd.ch[2]);
+                    if (b.h[3]) st.store(sp + __popc(m & ((1 << rk3) - 1)), b.t[3], nd.ch[3]);
+                    sp += __popc(m);
+                    if (COUNTERS) maxDepth = max(maxDepth, sp);
+                } else {
+                    const int base = leaf_base(node), count = leaf_count(node);
+                    if (COUNTERS) leafsT++;
+                    if (in_mesh) {
+                        if (COUNTERS) trisT += count;
+                        pend = true;
+                        lbase = base;
+                        lcount = count;
+                    } else {
+                        // Scene.TraceElems: geoms [base, base+count) (scene.go:61-78)
+                        mesh_base = sp + 1;
+                        if (mesh_base >= VGPU_MAX_STACK) { overflow = true; done = true; break; }
+                        gcur = base; gend = base + count;
+                        g = sv.geoms[gcur];
+                        st.store(mesh_base, r.tclosest, 0);
+                        sp = mesh_base + 1;
+                        in_mesh = true;
+                    }
+                }
+            } while (false);
+        }
+
+        if (__ballot(pend) != 0) {
+            // Deal the leaves' triangle tests out over the wavefront: inclusive prefix sum of the
+            // counts; work item w belongs to the first lane whose prefix exceeds w.
+            const int c = pend ? lcount : 0;
+            int pin = c;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y = __shfl_up(pin, d);
+                if (lane >= d) pin += y;
             }
-            const uint64_t oi = perm ? (uint64_t)perm[ri] : ri;
-            if (EXT) {
-                hits[2 * oi] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
-                hits[2 * oi + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
-            } else {
-                hits[oi] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
-                if (geoms_out) geoms_out[oi] = best_geom;
+            const int total = __shfl(pin, 63);
+            const int pex = pin - c;
+            const int tb = g.tri_base + lbase;
+            const TriRay q = tri_ray(r);
+            const int kpack = q.Kx | (q.Ky << 2) | (q.Kz << 4);
+            const float bias = g.ray_bias;
+            uint32_t cand = 0;   // owner: leaf triangles that passed the distance-free part of the test
+            for (int k0 = 0; k0 < total; k0 += 64) {
+                const int w = k0 + lane;
+                int o = 0;
+#pragma unroll
+                for (int s = 32; s >= 1; s >>= 1)
+                    if (__shfl(pin, o + s - 1) <= w) o += s;
+                const int i = w - __shfl(pex, o);
+                TriRay oq;
+                oq.S0 = __shfl(q.S0, o); oq.S1 = __shfl(q.S1, o); oq.S2 = __shfl(q.S2, o);
+                oq.Pkx = __shfl(q.Pkx, o); oq.Pky = __shfl(q.Pky, o); oq.Pkz = __shfl(q.Pkz, o);
+                const int ok = __shfl(kpack, o);
+                oq.Kx = ok & 3; oq.Ky = (ok >> 2) & 3; oq.Kz = (ok >> 4) & 3;
+                const float obias = __shfl(bias, o);
+                const int otb = __shfl(tb, o);
+                bool pass = false;
+                if (w < total) {
+                    const float4* tp = sv.tris + (size_t)(otb + i) * 3;
+                    TriPre pr;
+                    pass = tri_pre<false>(oq, tp[0], tp[1], tp[2], obias, pr);
+                }
+                const uint64_t cm = __ballot(pass);
+                if (pend && cm != 0) {
+                    const int s0 = max(pex, k0), e0 = min(pin, k0 + 64);
+                    if (e0 > s0)
+                        cand |= (uint32_t)((cm >> (s0 - k0)) & ((1ull << (e0 - s0)) - 1ull)) << (s0 - pex);
+                }
             }
-            if (COUNTERS) counters[oi] = make_int4(nodesT, leafsT, trisT, maxDepth);
+            if (cand != 0) {
+                // the owner's sequential pass over the candidates, in leaf order (trace.go:108-122)
+                int idx = -1;
+                float U = 0.0f, V = 0.0f, W = 0.0f;
+                const float4* tp = sv.tris + (size_t)tb * 3;
+                while (cand != 0) {
+                    const int i = __ffs(cand) - 1;
+                    cand &= cand - 1;
+                    if (tri_test<false>(r, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
+                        idx = lbase + i;
+                }
+                if (idx >= 0) {
+                    best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
+                    if (SHADOW) done = true;   // qbvh/intersect.go:232-236 at both levels
+                }
+            }
         }
     }
 }
+
+// Idle lanes that trigger a refill of the wavefront (k_trace_wide); 48 of 64 measured fastest on
+// incoherent bounce rays (64: refill only when the whole wavefront is idle).
+constexpr int kRefillMin = 48;
+int refill_min_lanes() { return kRefillMin; }
 
 template <int BLOCK, int LDS, bool SHADOW, bool MOTION>
 hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
                       float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev,
                       const uint32_t* perm, hipStream_t stream) {
+    const int refill_min = refill_min_lanes();
     if constexpr (!MOTION) {
         if (counters)
             hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, true, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
                                stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err,
-                               n_dev, perm);
+                               n_dev, perm, refill_min);
         else
             hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, false, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
                                stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err,
-                               n_dev, perm);
+                               n_dev, perm, refill_min);
     } else {
         if (counters)
             hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, true, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream,
@@ -684,7 +718,7 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
     else
         hipLaunchKernelGGL((k_trace_wide<256, 16, false, false, true>), dim3(grid), dim3(256), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
-                           (const uint32_t*)nullptr);
+                           (const uint32_t*)nullptr, refill_min_lanes());
     return hipGetLastError();
 }
 
