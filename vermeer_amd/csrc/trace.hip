@@ -592,7 +592,23 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
             } while (false);
         }
 
-        if (__ballot(pend) != 0) {
+        const uint64_t pm = __ballot(pend);
+        if (pm != 0 && __ballot(pend && lcount > 2) == 0) {
+            // Leaves of one or two triangles only: each owner tests its own (dealing them out
+            // costs more than two tests).
+            if (pend) {
+                int idx = -1;
+                float U = 0.0f, V = 0.0f, W = 0.0f;
+                const float4* tp = sv.tris + ((size_t)g.tri_base + lbase) * 3;
+                for (int i = 0; i < lcount; i++)
+                    if (tri_test<false>(r, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
+                        idx = lbase + i;
+                if (idx >= 0) {
+                    best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
+                    if (SHADOW) done = true;
+                }
+            }
+        } else if (pm != 0) {
             // Deal the leaves' triangle tests out over the wavefront: inclusive prefix sum of the
             // counts; work item w belongs to the first lane whose prefix exceeds w.
             const int c = pend ? lcount : 0;
