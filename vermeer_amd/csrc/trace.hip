@@ -723,16 +723,17 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
                             int32_t* spN, uint64_t spill_threads, int32_t* err, int num_cus, hipStream_t stream,
                             const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
-    const TraceLaunch cfg{256, 16, 16};
+    constexpr int kBlock = 256, kLds = 16, kWaves = 16;   // the block size is the kernels' launch bound
+    const TraceLaunch cfg{kBlock, kLds, kWaves};
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
-    const unsigned grid = (unsigned)(threads / cfg.block);
+    const unsigned grid = (unsigned)(threads / kBlock);
     if (motion)
-        hipLaunchKernelGGL((k_trace<256, 16, false, true, false, true>), dim3(grid), dim3(256), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace<kBlock, kLds, false, true, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr);
     else
-        hipLaunchKernelGGL((k_trace_wide<256, 16, false, false, true>), dim3(grid), dim3(256), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace_wide<kBlock, kLds, false, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr, refill_min_lanes());
     return hipGetLastError();
