@@ -101,17 +101,37 @@ typedef struct vgpu_material {
     float spec1_strength; int32_t has_spec1_strength;
     float spec1_roughness; int32_t has_spec1_roughness;
     float ior; int32_t has_ior;
-    int32_t fresnel_model;   /* 1 = Dielectric (fresnel/dielectric.go) */
+    int32_t fresnel_model;   /* Spec1FresnelModel: 0 (unset, the reference's default) or
+                                VGPU_FRESNEL_DIELECTRIC = fresnel/dielectric.go with ior;
+                                VGPU_FRESNEL_METAL = fresnel/conductor.go (Gulbrandsen) */
+    float spec1_fresnel_refl[3]; int32_t has_spec1_fresnel_refl;   /* Spec1FresnelRefl (default 0.5 grey) */
+    float spec1_fresnel_edge[3]; int32_t has_spec1_fresnel_edge;   /* Spec1FresnelEdge: as in std.go:185-187
+                                it replaces the reflectivity, and the edge tint stays 0.5 grey */
 } vgpu_material;
+#define VGPU_FRESNEL_DIELECTRIC 1
+#define VGPU_FRESNEL_METAL 2
 
-/* TriLight node (builtin/light/triangle.go:18-28): an emitting triangle with
- * its own one-triangle mesh.  geom = insertion index of that mesh. */
-typedef struct vgpu_tri_light {
+/* A light node and the geometry it inserts into the scene (geom = insertion
+ * index of that geometry; the renderer drops a light while shading its own
+ * geometry, scene.go:100-116):
+ *   VGPU_LIGHT_TRI    TriLight (builtin/light/triangle.go:18-28): P0, P1, P2 the
+ *                     vertices; geometry: a one-triangle mesh.
+ *   VGPU_LIGHT_DISK   DiskLight (disk.go:17-31): P0 = P, P1 = Up, P2 = LookAt,
+ *                     radius; geometry: the Segments-triangle fan of
+ *                     Disk.createMesh (disk.go:224-262), added by the caller.
+ *   VGPU_LIGHT_SPHERE SphereLight (sphere.go:16-28): P0 = centre, radius;
+ *                     geometry: vgpu_scene_add_sphere. */
+typedef struct vgpu_light {
     float P0[3], P1[3], P2[3];
     int32_t material;
     int32_t samples;         /* NumSamples = 1 << samples (default 1) */
     int32_t geom;
-} vgpu_tri_light;
+    int32_t type;            /* VGPU_LIGHT_* */
+    float radius;            /* disk and sphere */
+} vgpu_light;
+#define VGPU_LIGHT_TRI 0
+#define VGPU_LIGHT_DISK 1
+#define VGPU_LIGHT_SPHERE 2
 
 /* Camera for one frame: the 4x4 column-major camera-to-world matrix M
  * (camera.go:225-236), TanThetaFocal = tan(Fov/2)*Focal, Aspect, Focal. */
@@ -216,15 +236,34 @@ int vgpu_scene_add_shaded_mesh(vgpu_scene* scene, const float* verts, uint32_t n
                                uint32_t ntris, const float* normals, const uint8_t* shaderidx,
                                const int32_t* shaders, uint32_t nshaders, int32_t* geom_out);
 
-/* Renderer over a committed scene: materials, triangle lights, camera and
- * image size.  Replaces the body of core.Render's pass loop
- * (core/render.go:66-137) in the reference semantics: per pixel one camera
- * ray, ShaderStd.Eval direct lighting with MIS (std.go:77-296,
- * core/shader.go:203-402), spectral hero-wavelength colour, film
+/* Adds the analytic sphere a SphereLight inserts (builtin/geom/sphere:
+ * sphere.go:15-95, trace.go:13-52) with its shader (a material id).  It is a
+ * scene geom like a mesh: bounds P -/+ radius in the top-level BVH, hit by
+ * the closed-form quadratic with no ray bias; a hit reports prim 0, u = v = 0. */
+int vgpu_scene_add_sphere(vgpu_scene* scene, const float* centre, float radius, int32_t material, int32_t* geom_out);
+
+/* Renderer over a committed scene: materials, lights, camera and image
+ * size.  Replaces the body of core.Render's pass loop (core/render.go:66-137)
+ * in the reference semantics: per pixel one camera ray, ShaderStd.Eval
+ * direct lighting with MIS (std.go:77-296, core/shader.go:203-402), perfect
+ * mirrors (Spec1Roughness 0) by recursion to level 3 with one light sample
+ * per light below level 0, spectral hero-wavelength colour, film
  * fb = (fb*iter + c)/(iter+1). */
 int vgpu_render_create(vgpu_render** out, vgpu_scene* scene, const vgpu_material* materials, uint32_t nmaterials,
-                       const vgpu_tri_light* lights, uint32_t nlights, const vgpu_camera* camera,
+                       const vgpu_light* lights, uint32_t nlights, const vgpu_camera* camera,
                        uint32_t width, uint32_t height);
+
+/* Pixel filter (core.PixelFilter, builtin/filter): VGPU_FILTER_BOX (no
+ * filter node: samples stay where RasterXY puts them), VGPU_FILTER_GAUSSIAN
+ * (gauss.go; defaults Width 2, Res 17) or VGPU_FILTER_AIRY (airy.go; defaults
+ * Width 6, Res 49, Peak 4).  width/res/peak <= 0 take the defaults.  The
+ * filter's Res x Res table is built on the host in float64 as
+ * filter.CreateSampler does (filter.go:87-173) and samples are warped as
+ * Sampler.WarpSample (filter.go:40-85), quirks included. */
+int vgpu_render_set_filter(vgpu_render* render, int32_t type, float width, int32_t res, float peak);
+#define VGPU_FILTER_BOX 0
+#define VGPU_FILTER_GAUSSIAN 1
+#define VGPU_FILTER_AIRY 2
 int vgpu_render_destroy(vgpu_render* render);
 
 /* One pass over every pixel.  scrambles: 6 uint64 per pixel (lensU, lensV,

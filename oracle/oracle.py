@@ -42,6 +42,7 @@ def load():
         "orc_scene_new": (P, []),
         "orc_scene_free": (None, [P]),
         "orc_scene_add_mesh": (C.c_int, [P, P, C.c_int32, C.c_int32, P, C.c_int32, C.c_float, C.c_int32]),
+        "orc_scene_add_sphere": (C.c_int, [P, P, C.c_float, C.c_int32]),
         "orc_scene_commit": (C.c_int, [P]),
         "orc_scene_set_leaf_size": (None, [P, C.c_int32]),
         "orc_scene_info": (C.c_int, [P, P, P, P, P, P]),
@@ -126,6 +127,13 @@ class OracleScene:
         self.meshes.append((v, ix))
         return g
 
+    def add_sphere(self, centre, radius, material=0):
+        """A SphereLight's analytic sphere (builtin/geom/sphere); hits report prim 0, u = v = 0."""
+        c = np.ascontiguousarray(centre, np.float32)
+        g = self.lib.orc_scene_add_sphere(self.h, _p(c), float(radius), int(material))
+        self.meshes.append(None)
+        return g
+
     def commit(self, leaf_size=16):
         self.lib.orc_scene_set_leaf_size(self.h, int(leaf_size))
         if self.lib.orc_scene_commit(self.h) != 0:
@@ -191,6 +199,8 @@ MATERIAL_DTYPE = np.dtype([
     ("spec1_roughness", "<f4"), ("has_spec1_roughness", "<i4"),
     ("ior", "<f4"), ("has_ior", "<i4"),
     ("fresnel_model", "<i4"),
+    ("spec1_fresnel_refl", "<f4", 3), ("has_spec1_fresnel_refl", "<i4"),
+    ("spec1_fresnel_edge", "<f4", 3), ("has_spec1_fresnel_edge", "<i4"),
 ])
 
 
@@ -208,7 +218,11 @@ def _render_sigs(lib):
         "orc_scene_add_shaded_mesh": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, P, P, C.c_int32]),
         "orc_render_new": (P, [P, C.c_int32, C.c_int32, P, C.c_float, C.c_float, C.c_float, P, C.c_int32]),
         "orc_render_free": (None, [P]),
-        "orc_render_add_trilight": (None, [P, P, P, P, C.c_int32, C.c_int32, C.c_int32]),
+        "orc_render_add_light": (None, [P, C.c_int32, P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32]),
+        "orc_render_set_filter": (None, [P, C.c_int32, C.c_float, C.c_int32, C.c_float]),
+        "orc_render_warp_sample": (None, [P, C.c_double, C.c_double, P]),
+        "orc_render_filter_tables": (C.c_int32, [P, P]),
+        "orc_fresnel_conductor": (C.c_float, [C.c_float, C.c_float, C.c_float]),
         "orc_render_pass": (C.c_int64, [P, P, C.c_uint32, P, P, C.c_int32]),
         "orc_fresnel_dielectric": (C.c_float, [C.c_float, C.c_float]),
         "orc_ggx_d": (C.c_float, [P, C.c_float]),
@@ -293,6 +307,12 @@ class OracleRenderer:
         self.w, self.h = width, height
         self._keep = []
         for mesh in desc["meshes"]:
+            if mesh.get("sphere"):
+                c = np.ascontiguousarray(mesh["centre"], np.float32)
+                self._keep.append(c)
+                self.lib.orc_scene_add_sphere(self.scene.h, _p(c), float(mesh["radius"]), int(mesh["material"]))
+                self.scene.meshes.append(None)
+                continue
             v = np.ascontiguousarray(mesh["verts"], np.float32)
             ix = np.ascontiguousarray(mesh["idx"], np.uint32)
             nrm = mesh.get("normals")
@@ -309,16 +329,38 @@ class OracleRenderer:
         Mx = np.ascontiguousarray(M, np.float32)
         self.h_r = self.lib.orc_render_new(self.scene.h, width, height, _p(Mx), float(tan_theta_focal),
                                            float(width) / float(height), float(focal), _p(mats), len(mats))
+        types = {"tri": 0, "disk": 1, "sphere": 2}
         for L in desc["lights"]:
-            p = [np.ascontiguousarray(L[k], np.float32) for k in ("P0", "P1", "P2")]
-            self.lib.orc_render_add_trilight(self.h_r, _p(p[0]), _p(p[1]), _p(p[2]), int(L["material"]),
-                                             int(L["samples"]), int(L["geom"]))
+            kind = L.get("type", "tri")
+            keys = {"tri": ("P0", "P1", "P2"), "disk": ("P", "Up", "LookAt"), "sphere": ("P", "P", "P")}[kind]
+            p = [np.ascontiguousarray(L[k], np.float32) for k in keys]
+            self.lib.orc_render_add_light(self.h_r, types[kind], _p(p[0]), _p(p[1]), _p(p[2]),
+                                          float(L.get("radius", 0.0)), int(L["material"]), int(L["samples"]),
+                                          int(L["geom"]))
         self.film = np.zeros((height, width, 3), np.float32)
 
     def __del__(self):
         if getattr(self, "h_r", None):
             self.lib.orc_render_free(self.h_r)
             self.h_r = None
+
+    def set_filter(self, kind="box", width=0.0, res=0, peak=0.0):
+        t = {"box": 0, "gaussian": 1, "airy": 2}[kind]
+        self.lib.orc_render_set_filter(self.h_r, t, float(width), int(res), float(peak))
+
+    def warp_sample(self, r0, r1):
+        uv = np.zeros(2, np.float64)
+        self.lib.orc_render_warp_sample(self.h_r, float(r0), float(r1), _p(uv))
+        return float(uv[0]), float(uv[1])
+
+    def filter_tables(self):
+        """(cdfV, cdfVU) of the current filter, or None for the box filter."""
+        n = self.lib.orc_render_filter_tables(self.h_r, None)
+        if n == 0:
+            return None
+        out = np.zeros(n + n * n, np.float64)
+        self.lib.orc_render_filter_tables(self.h_r, _p(out))
+        return out[:n], out[n:].reshape(n, n)
 
     def render_pass(self, scrambles, iteration, nthreads=0, want_sample=False):
         scr = np.ascontiguousarray(scrambles, np.uint64)
@@ -330,6 +372,11 @@ class OracleRenderer:
 def fresnel_dielectric(eta, c):
     lib = load(); _render_sigs(lib)
     return lib.orc_fresnel_dielectric(eta, c)
+
+
+def fresnel_conductor(r, g, c):
+    lib = load(); _render_sigs(lib)
+    return lib.orc_fresnel_conductor(r, g, c)
 
 
 def ggx_d(m, alpha):
@@ -474,6 +521,8 @@ class OraclePathTracer:
         self._keep = []
         meshes = []
         for mesh in desc["meshes"]:
+            if mesh.get("sphere"):
+                raise ValueError("path mode takes triangle meshes only")
             v = np.ascontiguousarray(mesh["verts"], np.float32)
             ix = np.ascontiguousarray(mesh["idx"], np.uint32)
             sidx = mesh.get("shaderidx")

@@ -106,6 +106,21 @@ int orc_scene_add_mesh(void* sp, const float* verts, int32_t nverts, int32_t nke
     return (int)s->geoms.size() - 1;
 }
 
+// Adds the analytic sphere of a SphereLight (builtin/geom/sphere) with shader table {material}.
+int orc_scene_add_sphere(void* sp, const float* centre, float radius, int32_t material) {
+    Scene* s = (Scene*)sp;
+    Mesh* m = new Mesh();
+    m->sphere = true;
+    m->nkeys = 1;
+    for (int k = 0; k < 3; k++) m->sphereC[k] = centre[k];
+    m->sphereR = radius;
+    // Sphere.Bounds (sphere.go:68-77)
+    for (int k = 0; k < 3; k++) { m->bounds.b[0][k] = centre[k] - radius; m->bounds.b[1][k] = centre[k] + radius; }
+    m->shaders.assign(1, material);
+    s->geoms.push_back(m);
+    return (int)s->geoms.size() - 1;
+}
+
 // PreRender: per-mesh accel, then the scene's top level.
 // leafMax of the static meshes' build (16: the reference's); call before orc_scene_commit.
 void orc_scene_set_leaf_size(void* sp, int32_t leaf_max) {

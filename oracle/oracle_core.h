@@ -388,10 +388,13 @@ struct Mesh {
     std::vector<std::vector<float>> mboxes;   // per key: 24 floats per node
     Box bounds;
     std::vector<Box> motionBounds;
+    bool sphere = false;              // analytic sphere geom (builtin/geom/sphere), no triangles
+    Vec3 sphereC; float sphereR = 0;
 
     // builtin/geom/polymesh/buildqbvh.go:14-144
     int leafMax = 16;                 // BuildAccel leafMax (buildqbvh.go:32)
     void initAccel() {
+        if (sphere) return;
         int n = facecount;
         std::vector<Box> boxes(n);
         std::vector<Vec3> centroids(n);
@@ -807,9 +810,48 @@ struct Scene {
         return nodebox;
     }
 
+    // builtin/geom/sphere/trace.go:54-103 (solveQuadratic, raySphereIntersect)
+    static bool RaySphereIntersect(const Vec3& Ro, const Vec3& Rd, const Vec3& C, float radius, float& t) {
+        Vec3 L; for (int k = 0; k < 3; k++) L[k] = Ro[k] - C[k];
+        float a = Rd[0] * Rd[0] + Rd[1] * Rd[1] + Rd[2] * Rd[2];
+        float b = 2 * (Rd[0] * L[0] + Rd[1] * L[1] + Rd[2] * L[2]);
+        float c = (L[0] * L[0] + L[1] * L[1] + L[2] * L[2]) - radius * radius;
+        float x0, x1;
+        float discr = b * b - 4 * a * c;
+        if (discr < 0) return false;
+        if (discr == 0) { x1 = -0.5f * b / a; x0 = x1; }
+        else {
+            float q = b > 0 ? -0.5f * (b + std::sqrt(discr)) : -0.5f * (b - std::sqrt(discr));
+            x0 = q / a; x1 = c / q;
+        }
+        if (x0 > x1) std::swap(x0, x1);
+        if (x0 > x1) std::swap(x0, x1);
+        if (x0 < 0) { x0 = x1; if (x0 < 0) return false; }
+        t = x0;
+        return true;
+    }
+    // Sphere.Trace (builtin/geom/sphere/trace.go:13-52): accepted when t < Tclosest, no ray bias.
+    // The trace API reports prim 0 and u = v = 0 (the reference's Bu/Bv here are the sphere's UV
+    // parameters, which the direct-lighting path never reads).
+    static bool SphereTrace(const Mesh& m, Ray& ray, HitRecord& hr) {
+        float t;
+        if (!RaySphereIntersect(ray.P, ray.D, m.sphereC, m.sphereR, t)) return false;
+        if (!(t < ray.Tclosest)) return false;
+        ray.Tclosest = t;
+        Vec3 P; for (int k = 0; k < 3; k++) P[k] = ray.P[k] + (ray.D[k] * t);
+        Vec3 d; for (int k = 0; k < 3; k++) d[k] = P[k] - m.sphereC[k];
+        Vec3 N = Mesh::Normalize(d);
+        for (int k = 0; k < 3; k++) { hr.Poffset[k] = 0.001f * N[k]; hr.Ng[k] = N[k]; hr.N[k] = N[k]; hr.Po[k] = P[k]; }
+        hr.DdPdu[0] = 1; hr.DdPdu[1] = 0; hr.DdPdu[2] = 0;
+        hr.DdPdv[0] = 0; hr.DdPdv[1] = 0; hr.DdPdv[2] = 1;
+        hr.elem = 0; hr.U = hr.V = hr.W = 0; hr.Bu = hr.Bv = 0; hr.shaderIdx = 0;
+        return true;
+    }
+
     // builtin/geom/polymesh/trace.go:15-102 without transforms
     bool GeomTrace(const Mesh& mesh, Ray& ray, SceneHit& sh) const {
         HitRecord& hr = sh.hr;
+        if (mesh.sphere) return SphereTrace(mesh, ray, hr);
         if (mesh.nkeys == 1) {
             return TraceQBVH(mesh.qbvh.data(), ray, [&](int base, int count) { return mesh.TraceElems(ray, hr, base, count); });
         }

@@ -1,17 +1,19 @@
 // ORACLE — test infrastructure only (same scope rules as oracle_accel.cpp).
 //
 // CPU restatement of the reference's per-pixel render/shade loop in its
-// reference semantics (direct lighting with MIS, no indirect bounces):
+// reference semantics (direct lighting with MIS; indirect light only through perfect mirrors):
 //   core/render.go:66-137          render: RasterXY / VdC / Sobol sample generation, film update
 //   core/trace.go:40-83            Trace: ShaderContext setup, ApplyTransform, Shader.Eval
 //   core/shader.go:139-402         OffsetP, LightsPrepare, NextLight, EvaluateLightSamples
 //   core/ray.go:56-88              Ray.Init
 //   builtin/camera/camera.go:80-323  PreRender (TanThetaFocal), calcLookatMatrices, ComputeRay (pinhole)
-//   builtin/shader/std.go:77-316     ShaderStd.Eval / EvalEmission (rough specular only)
-//   builtin/shader/bsdf/orennayar.go:24-73, microfacetggx.go:26-146
-//   builtin/shader/fresnel/dielectric.go:34-47
+//   builtin/shader/std.go:77-316     ShaderStd.Eval / EvalEmission, perfect mirrors by recursion to level 3
+//   builtin/shader/bsdf/orennayar.go:24-73, microfacetggx.go:26-146, specular.go:13-101
+//   builtin/shader/fresnel/dielectric.go:34-47, conductor.go:31-89
 //   builtin/light/triangle.go:64-535 Tri light: ValidSample, SampleArea, spherical triangle sampling
-//   builtin/light/disk.go:38-51      rayPlaneIntersect
+//   builtin/light/disk.go:38-215     Disk light; sphere.go:66-256 Sphere light
+//   builtin/geom/sphere/trace.go:13-103  analytic sphere geometry (oracle_core.h)
+//   builtin/filter/filter.go:40-173, gauss.go:34-58, airy.go:35-106  pixel filters
 //   builtin/geom/polymesh/trace.go:292-513  hit attributes (N, Poffset, P, DdPdu, DdPdv)
 //   colour/spectrum.go:49-112, spectrum_smits9.go:16-84, cie1931_2deg.go:62-95, space_srgb.go:8-14
 //   math/ldseq/ldseq.go:50-96, raster.go:10-57   VanDerCorput, Sobol, RasterXY
@@ -23,9 +25,7 @@
 // math package, both are within 1 ulp of the true value; the float32 results
 // they feed agree except on rare rounding ties).
 //
-// Not restated (absent from the workloads): perfect-mirror recursion
-// (Spec1Roughness == 0, std.go:209-261), conductor Fresnel, pixel filters,
-// thin-lens cameras, textures, disk/sphere/quad lights.
+// Not restated (absent from the workloads): thin-lens cameras, textures, quad lights.
 
 #include "oracle_core.h"
 
@@ -449,14 +449,20 @@ struct Material {              // ShaderStd parameters (builtin/shader/std.go:25
     float spec1_strength; int32_t has_spec1_strength;
     float spec1_roughness; int32_t has_spec1_roughness;
     float ior; int32_t has_ior;
-    int32_t fresnel_model;     // 1 = Dielectric (the only model restated)
+    int32_t fresnel_model;     // 0 / 1 = Dielectric, 2 = Metal (conductor)
+    float spec1_fresnel_refl[3]; int32_t has_spec1_fresnel_refl;
+    float spec1_fresnel_edge[3]; int32_t has_spec1_fresnel_edge;
 };
 
-struct TriLight {
+// A light node: Tri (builtin/light/triangle.go:18-28), Disk (disk.go:17-31: P0 = P, P1 = Up,
+// P2 = LookAt) or Sphere (sphere.go:16-28: P0 = centre).
+struct Light {
+    int32_t type;              // 0 tri, 1 disk, 2 sphere
     Vec3 P0, P1, P2;
+    float radius;
     int32_t material;
     int32_t samples;
-    int32_t geom;              // post-sort geom slot of the light's mesh
+    int32_t geom;              // post-sort geom slot of the light's geometry
 };
 
 // ----------------------------------------------------------------- shading
@@ -527,8 +533,43 @@ static RGB DielectricKr(float eta, float c) {
     o.c[0] = o.c[1] = o.c[2] = fr;
     return o;
 }
+// builtin/shader/fresnel/conductor.go:31-81 ("Artist Friendly Metallic Fresnel", Gulbrandsen 2014):
+// refractive index n and extinction k^2 from reflectivity r and edge tint g, then the unpolarised
+// conductor Fresnel term.  Min/Max are MINSS/MAXSS (math/dim_amd64.s).
+static float mins32(float x, float y) { return x < y ? x : y; }
+static float maxs32(float x, float y) { return x > y ? x : y; }
+static float condNMin(float r) { return (1 - r) / (1 + r); }
+static float condNMax(float r) { return (1 + std::sqrt(r)) / (1 - std::sqrt(r)); }
+static float ConductorFresnel(float r, float g, float cosTheta) {
+    float nr = maxs32(0, mins32(r, 0.99f));
+    float n = condNMin(nr) * g + (1 - g) * condNMax(nr);
+    float k2 = ((n + 1) * (n + 1) * nr - (n - 1) * (n - 1)) / (1 - nr);
+    float rs = (n * n + k2 - 2 * n * cosTheta + cosTheta * cosTheta) / (n * n + k2 + 2 * n * cosTheta + cosTheta * cosTheta);
+    float rp = ((n * n + k2) * cosTheta * cosTheta - 2 * n * cosTheta + 1) / ((n * n + k2) * cosTheta * cosTheta + 2 * n * cosTheta + 1);
+    return 0.5f * (rs + rp);
+}
+// The core.Fresnel a ShaderStd builds (std.go:171-190): Dielectric(ior), or Conductor(refl, edge).
+struct FresnelModel {
+    bool metal = false; float eta = 1.7f; float refl[3] = {0.5f, 0.5f, 0.5f}, edge[3] = {0.5f, 0.5f, 0.5f};
+    RGB Kr(float c) const {
+        if (!metal) return DielectricKr(eta, c);
+        RGB o;
+        for (int k = 0; k < 3; k++) o.c[k] = ConductorFresnel(refl[k], edge[k], c);
+        return o;
+    }
+};
+static FresnelModel MakeFresnel(const Material& sh, float ior) {
+    FresnelModel f;
+    f.eta = ior;
+    if (sh.fresnel_model == 2) {
+        f.metal = true;
+        if (sh.has_spec1_fresnel_refl) for (int k = 0; k < 3; k++) f.refl[k] = sh.spec1_fresnel_refl[k];
+        if (sh.has_spec1_fresnel_edge) for (int k = 0; k < 3; k++) f.refl[k] = sh.spec1_fresnel_edge[k];   // std.go:185-187
+    }
+    return f;
+}
 struct GGX {                   // builtin/shader/bsdf/microfacetggx.go
-    float Lambda; Vec3 OmegaR; float Roughness; float eta; Vec3 U, V, N;
+    float Lambda; Vec3 OmegaR; float Roughness; FresnelModel fresnel; Vec3 U, V, N;
     Vec3 Sample(double r0, double r1) const {
         float alpha = sqr32(Roughness);
         double thetaM = std::atan2((double)alpha * std::sqrt(r0), std::sqrt(1 - r0));
@@ -549,13 +590,36 @@ struct GGX {                   // builtin/shader/bsdf/microfacetggx.go
         Vec3 wi = BasisProject(U, V, N, o_);
         float alpha = sqr32(Roughness);
         Vec3 h = Scale(signf(OmegaR[2]), Normalize(Add(OmegaR, wi)));
-        RGB F = DielectricKr(eta, DotAbs(OmegaR, h));
+        RGB F = fresnel.Kr(DotAbs(OmegaR, h));
         float numer = ggxG1(OmegaR, h, alpha) * ggxG1(wi, h, alpha) * ggxD(h, alpha);
         float denom = 4 * Abs(OmegaR[2]) * Abs(wi[2]);
         Spectrum rho; rho.Lambda = Lambda;
         rho.FromRGB(F);
         rho.Scale(Abs(wi[2]) * numer / denom);
         for (int k = 0; k < 4; k++) if (rho.C[k] < 0 || std::isnan(rho.C[k])) rho.C[k] = 0;
+        return rho;
+    }
+};
+
+// builtin/shader/bsdf/specular.go: perfect mirror; reflect(omegaR, N) = 2 (N . omegaR) N - omegaR
+static Vec3 ReflectZ(const Vec3& wr) {
+    Vec3 N = V3(0, 0, 1);
+    return Sub(Scale(2.0f * Dot(N, wr), N), wr);
+}
+struct Specular {
+    float Lambda; Vec3 OmegaR; FresnelModel fresnel; Vec3 U, V, N;
+    Vec3 Sample() const { return BasisExpand(U, V, N, Normalize(ReflectZ(OmegaR))); }
+    double PDF(const Vec3& o_) const {
+        Vec3 o = BasisProject(U, V, N, o_);
+        return Dot(o, ReflectZ(OmegaR)) < 0.9999f ? 0 : 1;
+    }
+    Spectrum Eval(const Vec3& o_) const {
+        Vec3 o = BasisProject(U, V, N, o_);
+        Spectrum rho;
+        if (Dot(o, ReflectZ(OmegaR)) < 0.9999f) return rho;
+        rho.Lambda = Lambda;
+        rho.FromRGB(fresnel.Kr(OmegaR[2]));
+        rho.Scale(DotAbs(o, V3(0, 0, 1)));
         return rho;
     }
 };
@@ -628,13 +692,64 @@ static bool rayPlaneIntersect(const Vec3& Ro, const Vec3& Rd, const Vec3& P, con
     return false;
 }
 
+// builtin/filter/airy.go:35-79 (Numerical Recipes bessj1, as the reference copies it from gipsy bessel.c)
+static double BesselJ1(double x) {
+    double ax = std::fabs(x);
+    if (ax < 8.0) {
+        double y = x * x;
+        double ans1 = x * (72362614232.0 + y * (-7895059235.0 + y * (242396853.1 + y * (-2972611.439 +
+                      y * (15704.48260 + y * (-30.16036606))))));
+        double ans2 = 144725228442.0 + y * (2300535178.0 + y * (18583304.74 + y * (99447.43394 +
+                      y * (376.9991397 + y * 1.0))));
+        return ans1 / ans2;
+    }
+    double z = 8.0 / ax, y = z * z, xx = ax - 2.356194491;
+    double ans1 = 1.0 + y * (0.183105e-2 + y * (-0.3516396496e-4 + y * (0.2457520174e-5 + y * (-0.240337019e-6))));
+    double ans2 = 0.04687499995 + y * (-0.2002690873e-3 + y * (0.8449199096e-5 + y * (-0.88228987e-6 + y * 0.105787412e-6)));
+    double ans = std::sqrt(0.636619772 / ax) * (std::cos(xx) * ans1 - z * std::sin(xx) * ans2);
+    if (x < 0.0) ans = -ans;
+    return ans;
+}
+// filter.CreateSampler (filter.go:87-173); out = cdfV then cdfVU
+template <class F>
+static void CreateSampler(int n, double w, F f, std::vector<double>& out) {
+    std::vector<double> filter((size_t)n * n);
+    double du = w / (double)(n - 1), u = -w / 2, Fsum = 0;
+    for (int j = 0; j < n; j++) {
+        double dv = w / (double)(n - 1), v = -w / 2;
+        for (int i = 0; i < n; i++) {
+            double fuv = f(u, v);
+            filter[(size_t)j + (size_t)i * n] = fuv;
+            Fsum += fuv;
+            v += dv;
+        }
+        u += du;
+    }
+    std::vector<std::vector<double>> pdf(n, std::vector<double>(n)), pVU(n, std::vector<double>(n));
+    std::vector<double> pV(n, 0.0);
+    for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) pdf[j][i] = filter[(size_t)j + (size_t)i * n] / Fsum;
+    for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) pV[j] += pdf[j][i];
+    for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) pVU[j][i] = pdf[j][i] / pV[j];
+    out.assign((size_t)n + (size_t)n * n, 0.0);
+    double p = 0;
+    for (int i = 0; i < n; i++) { p += pV[i]; out[i] = p; }
+    for (int j = 0; j < n; j++) {
+        double q = 0;
+        for (int i = 0; i < n; i++) { q += pVU[j][i]; out[(size_t)n + (size_t)j * n + i] = q; }
+    }
+}
+
 struct Renderer {
     Scene* scene;
     int width, height;
     M4 M;
     float tanThetaFocal, aspect, focal;
     std::vector<Material> mats;
-    std::vector<TriLight> lights;
+    std::vector<Light> lights;
+    std::vector<double> filt;     // pixel filter sampler: cdfV (filtN) then cdfVU (filtN x filtN); empty: box
+    int filtN = 0;
+    double filtW = 0;
+    int maxLevel = 0;             // 3 when a material is a perfect mirror
     std::vector<std::vector<int32_t>> geomShaders;   // per post-sort geom slot: material id per shader index
     int shadowRays = 0;
 
@@ -668,7 +783,7 @@ struct Renderer {
     }
 
     // Tri.ValidSample (triangle.go:136-230)
-    bool ValidSample(const TriLight& L, const SC& sg, BSDFSample& s) const {
+    bool TriValidSample(const Light& L, const SC& sg, BSDFSample& s) const {
         Vec3 P;
         if (!rayTriangleIntersect(sg.P, s.D, L.P0, L.P1, L.P2, P)) return false;
         Vec3 N = Normalize(Cross(Sub(L.P1, L.P0), Sub(L.P2, L.P0)));
@@ -696,7 +811,7 @@ struct Renderer {
     }
 
     // Tri.SampleArea (triangle.go:232-343)
-    void SampleArea(const TriLight& L, const SC& sg, int n, std::vector<LightSample>& out) const {
+    void TriSampleArea(const Light& L, const SC& sg, int n, std::vector<LightSample>& out) const {
         const Material& lm = mats[L.material];
         Vec3 N = Normalize(Cross(Sub(L.P1, L.P0), Sub(L.P2, L.P0)));
         if (Dot(sg.Ng, Sub(L.P0, sg.P)) < 0 || Dot(sg.Ng, Sub(L.P1, sg.P)) < 0 || Dot(sg.Ng, Sub(L.P2, sg.P)) < 0) {
@@ -734,9 +849,109 @@ struct Renderer {
         }
     }
 
+    // Disk.PreRender's frame (disk.go:84-86)
+    static void DiskFrame(const Light& L, Vec3& N, Vec3& T, Vec3& B) {
+        N = Normalize(Sub(L.P2, L.P0));
+        T = Normalize(Cross(N, L.P1));
+        B = Cross(N, T);
+    }
+    // Disk.ValidSample (disk.go:120-168) with rayDiskIntersect (disk.go:54-68)
+    bool DiskValidSample(const Light& L, const SC& sg, BSDFSample& s) const {
+        Vec3 N, T, B; DiskFrame(L, N, T, B);
+        const float R = L.radius;
+        double pdf = (double)(1.0f / (Pi32 * R * R));
+        float t;
+        if (!rayPlaneIntersect(sg.P, s.D, L.P0, N, t) || !(t >= 0)) return false;
+        {
+            Vec3 v = Sub(Mad(sg.P, s.D, t), L.P0);
+            if (!(std::sqrt(Dot(v, v)) <= R)) return false;
+        }
+        Vec3 p = Mad(sg.P, s.D, t);
+        Vec3 V = Sub(p, sg.P);
+        if (Dot(V, sg.Ng) <= 0.0f || Dot(V, N) >= 0.0f) return false;
+        s.Ldist = Length(V);
+        s.Ld = Normalize(V);
+        s.Liu.Lambda = sg.Lambda; s.Liu.FromRGB(Emission(mats[L.material]));
+        s.PdfLight = (float)pdf * (s.Ldist * s.Ldist) / Abs(Dot(s.Ld, N));
+        return true;
+    }
+    // Disk.SampleArea (disk.go:171-215)
+    void DiskSampleArea(const Light& L, const SC& sg, int n, std::vector<LightSample>& out) const {
+        Vec3 N, T, B; DiskFrame(L, N, T, B);
+        const float R = L.radius;
+        for (int i = 0; i < n; i++) {
+            uint64_t idx = sg.I * (uint64_t)n + (uint64_t)i;
+            double r0 = VanDerCorput(idx, sg.Scramble[0]), r1 = Sobol(idx, sg.Scramble[1]);
+            float u = R * std::sqrt((float)r0) * fCos(2 * Pi32 * (float)r1);
+            float v = R * std::sqrt((float)r0) * fSin(2 * Pi32 * (float)r1);
+            double pdf = (double)(1.0f / (Pi32 * R * R));
+            Vec3 su = Scale(u, B), sv = Scale(v, T);
+            Vec3 P = V3(L.P0[0] + su[0] + sv[0], L.P0[1] + su[1] + sv[1], L.P0[2] + su[2] + sv[2]);
+            Vec3 V = Sub(P, sg.P);
+            if (!(Dot(V, sg.Ng) > 0.0f && Dot(V, N) < 0.0f)) continue;
+            LightSample ls;
+            ls.Ldist = Length(V);
+            ls.Ld = Normalize(V);
+            ls.Liu.Lambda = sg.Lambda; ls.Liu.FromRGB(Emission(mats[L.material]));
+            ls.Pdf = (float)pdf * (ls.Ldist * ls.Ldist) / Abs(Dot(ls.Ld, N));
+            out.push_back(ls);
+        }
+    }
+    static float ConePdf(float R, float l) { return 1 / (2 * Pi32 * (1 - std::sqrt(1 - sqr32(R / l)))); }
+    // Sphere.ValidSample (sphere.go:129-177)
+    bool SphereValidSample(const Light& L, const SC& sg, BSDFSample& s) const {
+        float t;
+        if (!Scene::RaySphereIntersect(sg.P, s.D, L.P0, L.radius, t)) return false;
+        Vec3 x = Mad(sg.P, s.D, t);
+        Vec3 D = Sub(x, sg.P);
+        s.Ldist = Length(D);
+        s.Ld = Normalize(D);
+        s.Liu.Lambda = sg.Lambda; s.Liu.FromRGB(Emission(mats[L.material]));
+        float l = Length(Sub(L.P0, sg.P));
+        s.PdfLight = ConePdf(L.radius, l);
+        return true;
+    }
+    // Sphere.SampleArea (sphere.go:180-256), sg.Sample = 0
+    void SphereSampleArea(const Light& L, const SC& sg, int n, std::vector<LightSample>& out) const {
+        Vec3 V = Sub(L.P0, sg.P);
+        float l = Length(V);
+        Vec3 w = Normalize(V);
+        Vec3 v = Normalize(Cross(w, sg.Ng));
+        Vec3 u = Cross(w, v);
+        for (int i = 0; i < n; i++) {
+            uint64_t idx = sg.I * (uint64_t)n + (uint64_t)i;
+            double r0 = VanDerCorput(idx, sg.Scramble[0]), r1 = Sobol(idx, sg.Scramble[1]);
+            float theta = fAcos(1 - (float)r0 + (float)r0 * std::sqrt(1 - sqr32(L.radius / l)));
+            float phi = 2 * Pi32 * (float)r1;
+            Vec3 a = V3(fCos(phi) * fSin(theta), fSin(phi) * fSin(theta), fCos(theta));
+            Vec3 omega = BasisExpand(u, v, w, a);
+            if (Dot(omega, sg.Ng) < 0) continue;
+            float t;
+            if (!Scene::RaySphereIntersect(sg.P, omega, L.P0, L.radius, t)) continue;
+            Vec3 x = Mad(sg.P, omega, t);
+            Vec3 D = Sub(x, sg.P);
+            LightSample ls;
+            ls.Ldist = Length(D);
+            ls.Ld = Normalize(D);
+            ls.Liu.Lambda = sg.Lambda; ls.Liu.FromRGB(Emission(mats[L.material]));
+            ls.Pdf = ConePdf(L.radius, l);
+            out.push_back(ls);
+        }
+    }
+    bool ValidSample(const Light& L, const SC& sg, BSDFSample& s) const {
+        if (L.type == 1) return DiskValidSample(L, sg, s);
+        if (L.type == 2) return SphereValidSample(L, sg, s);
+        return TriValidSample(L, sg, s);
+    }
+    void SampleArea(const Light& L, const SC& sg, int n, std::vector<LightSample>& out) const {
+        if (L.type == 1) DiskSampleArea(L, sg, n, out);
+        else if (L.type == 2) SphereSampleArea(L, sg, n, out);
+        else TriSampleArea(L, sg, n, out);
+    }
+
     // core/shader.go:203-402
     template <class B>
-    RGB EvaluateLightSamples(Task* tk, const SC& sc, const TriLight& L, int NSamples, const B& bsdf) {
+    RGB EvaluateLightSamples(Task* tk, const SC& sc, const Light& L, int NSamples, const B& bsdf) {
         RGB col;
         const float k = 1.0f - 0.0001f;   // 1 - ShadowRayEpsilon (core/ray.go:22)
         if (NSamples > 1) {
@@ -806,8 +1021,9 @@ struct Renderer {
         return col;
     }
 
-    // ShaderStd.Eval (std.go:77-296), Level 0, roughness > 0
-    RGB Shade(Task* tk, const SC& sg, const Material& sh) {
+    // ShaderStd.Eval (std.go:77-296) at recursion level `level`
+    RGB Shade(Task* tk, const SC& sg, const Material& sh, int level) {
+        if (level > 3) return RGB();
         Vec3 V = Cross(sg.N, sg.DdPdu);
         if (Length2(V) < 0.1f) V = Cross(sg.N, sg.DdPdv);
         V = Normalize(V);
@@ -820,11 +1036,12 @@ struct Renderer {
         float sw = sh.has_spec1_strength ? sh.spec1_strength : 0.0f;
         float total = dw + sw;
         dw /= total; sw /= total;
-        // the lights a shading point sees: all but the one owning the hit geom (scene.go:100-116)
+        // NextLight: 1 << Samples samples per light at level 0, one below (shader.go:179-195); the lights
+        // a shading point sees are all but the one owning the hit geom (scene.go:100-116)
         if (dw > 0.0f) {
             for (const auto& L : lights) {
                 if (L.geom == sg.geom) continue;
-                int NS = 1 << L.samples;
+                int NS = level == 0 ? 1 << L.samples : 1;
                 RGB c = EvaluateLightSamples(tk, sg, L, NS, diff);
                 c.Mul(diffColour);
                 diffContrib.Add(c);
@@ -832,17 +1049,40 @@ struct Renderer {
             diffContrib.Scale(dw);
         }
         float ior = sh.has_ior ? sh.ior : 1.7f;
+        FresnelModel fres = MakeFresnel(sh, ior);
         RGB specContrib;
         if (sw > 0.0f) {
             float rough = sh.has_spec1_roughness ? sh.spec1_roughness : 0.5f;
             RGB specColour;
             if (sh.has_spec1_colour) for (int c = 0; c < 3; c++) specColour.c[c] = sh.spec1[c];
-            GGX spec{sg.Lambda, BasisProject(U, V, sg.N, Neg(sg.Rd)), rough * rough, ior, U, V, sg.N};
-            if (rough > 0.0f) {
+            if (rough == 0.0f) {
+                // perfect mirror (std.go:209-261): one reflected ray, recursively shaded at level + 1
+                Specular spec{sg.Lambda, BasisProject(U, V, sg.N, Neg(sg.Rd)), fres, U, V, sg.N};
+                Vec3 o = spec.Sample();
+                double pdf = spec.PDF(o);
+                if (Dot(o, sg.Ng) > 0.0f) {
+                    Ray r{};
+                    r.P = OffsetP(sg, 1); r.D = o; r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = 0;
+                    RaySetup(r);
+                    bool hit;
+                    RGB sc = TraceShade(tk, r, level + 1, sg.Lambda, sg.I, sg.Scramble, hit);
+                    if (hit) {
+                        Spectrum rho = spec.Eval(o);
+                        rho.Scale(1.0f / (float)pdf);
+                        RGB col = rho.ToRGB();
+                        col.Mul(specColour);
+                        col.Mul(sc);
+                        for (int k = 0; k < 3; k++) if (col.c[k] < 0 || std::isnan(col.c[k])) col.c[k] = 0;
+                        specContrib.Add(col);
+                    }
+                }
+                specContrib.Scale(sw / 1.0f);
+            } else if (rough > 0.0f) {   // no direct lighting for other roughness values (std.go:263)
+                GGX ggx{sg.Lambda, BasisProject(U, V, sg.N, Neg(sg.Rd)), rough * rough, fres, U, V, sg.N};
                 for (const auto& L : lights) {
                     if (L.geom == sg.geom) continue;
-                    int NS = 1 << L.samples;
-                    RGB c = EvaluateLightSamples(tk, sg, L, NS, spec);
+                    int NS = level == 0 ? 1 << L.samples : 1;
+                    RGB c = EvaluateLightSamples(tk, sg, L, NS, ggx);
                     c.Mul(specColour);
                     specContrib.Add(c);
                 }
@@ -855,11 +1095,67 @@ struct Renderer {
         return contrib;
     }
 
+    // core.Trace (trace.go:40-83): closest hit, then the hit's shader; hit = false on a miss or a geom
+    // without a shader.
+    RGB TraceShade(Task* tk, Ray& r, int level, float lambda, uint64_t I, const uint64_t* scramble, bool& hit) {
+        tk->StackTop = 0; r.task = tk;
+        SceneHit sh{}; sh.geom = -1; sh.hr.elem = -1;
+        RGB out;
+        hit = scene->Trace(r, sh);
+        if (!hit) return out;
+        const HitRecord& h = sh.hr;
+        SC sc;
+        sc.geom = sh.geom; sc.Lambda = lambda; sc.I = I; sc.Scramble[0] = scramble[0]; sc.Scramble[1] = scramble[1];
+        sc.Rd = r.D;
+        // ApplyTransform with identity matrices (core/shader.go:129-135)
+        sc.P = h.Po;
+        sc.N = Normalize(h.N); sc.Ng = Normalize(h.Ng);
+        sc.DdPdu = Normalize(h.DdPdu); sc.DdPdv = Normalize(h.DdPdv);
+        sc.Poffset = h.Poffset;
+        const auto& shaders = geomShaders[sh.geom];
+        int mat = shaders.empty() ? -1 : shaders[h.shaderIdx];
+        if (mat < 0) { hit = false; return out; }
+        return Shade(tk, sc, mats[mat], level);
+    }
+
+    // Sampler.WarpSample (builtin/filter/filter.go:40-85)
+    void WarpSample(double r0, double r1, double& u, double& v) const {
+        const int n = filtN;
+        const double w = filtW;
+        const double* cdfV = filt.data();
+        u = 0; v = 0;
+        int uI = -1;
+        for (int i = 0; i < n; i++) {
+            uI = i;
+            if (r0 < cdfV[i]) {
+                if (i == 0) { double du = r0 / cdfV[i]; u = (-w / 2) + (du / w); }
+                else { double du = (r0 - cdfV[i - 1]) / (cdfV[i] - cdfV[i - 1]); u = (-w / 2) + w * ((double)i + du) / (double)(n - 1); }
+                break;
+            }
+        }
+        const double* row = filt.data() + n + (size_t)uI * n;
+        for (int i = 0; i < n; i++) {
+            if (r1 < row[i]) {
+                if (i == 0) { double dv = r1 / row[i]; v = (-w / 2) + (dv / w); }
+                else { double dv = (r1 - row[i - 1]) / (row[i] - row[i - 1]); v = (-w / 2) + w * ((double)i + dv) / (double)(n - 1); }
+                return;
+            }
+        }
+        u = 0; v = 0;
+    }
+
     // core/render.go:75-130 for one pixel; returns the sample colour (and whether it hit)
     RGB Sample(Task* tk, uint32_t iter, uint32_t x, uint32_t y, const uint64_t* scr, bool& hit, int32_t& hitGeom) {
         uint64_t index; double rx, ry;
         RasterXY(12, iter, x, y, 0, 0, index, rx, ry);
         double lambda = (720 - 450) * VanDerCorput(iter, scr[3]) + 450;
+        if (filtN > 0) {
+            double pixu = rx - std::floor(rx), pixv = ry - std::floor(ry);
+            double u, v;
+            WarpSample(pixu, pixv, u, v);
+            rx = std::floor(rx) + 0.5 + u;
+            ry = std::floor(ry) + 0.5 + v;
+        }
         float Sx = (float)(-1.0 + 2.0 * (rx / (double)width));
         float Sy = -(float)(-1.0 + 2.0 * (ry / (double)height));
         // camera.ComputeRay (pinhole), camera.go:221-319
@@ -872,25 +1168,10 @@ struct Renderer {
         r.P = MulPoint(M, V3(0, 0, 0));
         r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = 0;
         RaySetup(r);
-        tk->StackTop = 0; r.task = tk;
-        SceneHit sh{}; sh.geom = -1; sh.hr.elem = -1;
-        RGB out;
-        hit = scene->Trace(r, sh);
-        hitGeom = hit ? sh.geom : -1;
-        if (!hit) return out;
-        const HitRecord& h = sh.hr;
-        SC sc;
-        sc.geom = sh.geom; sc.Lambda = (float)lambda; sc.I = iter; sc.Scramble[0] = scr[4]; sc.Scramble[1] = scr[5];
-        sc.Rd = r.D;
-        // ApplyTransform with identity matrices (core/shader.go:129-135)
-        sc.P = h.Po;
-        sc.N = Normalize(h.N); sc.Ng = Normalize(h.Ng);
-        sc.DdPdu = Normalize(h.DdPdu); sc.DdPdv = Normalize(h.DdPdv);
-        sc.Poffset = h.Poffset;
-        const auto& shaders = geomShaders[sh.geom];
-        int mat = shaders.empty() ? -1 : shaders[h.shaderIdx];
-        if (mat < 0) { hit = false; return out; }
-        return Shade(tk, sc, mats[mat]);
+        RGB out = TraceShade(tk, r, 0, (float)lambda, iter, scr + 4, hit);
+        hitGeom = -1;
+        if (!hit) return RGB();
+        return out;
     }
 };
 
@@ -953,6 +1234,8 @@ struct orc_material {
     float spec1_roughness; int32_t has_spec1_roughness;
     float ior; int32_t has_ior;
     int32_t fresnel_model;
+    float spec1_fresnel_refl[3]; int32_t has_spec1_fresnel_refl;
+    float spec1_fresnel_edge[3]; int32_t has_spec1_fresnel_edge;
 };
 static_assert(sizeof(orc_material) == sizeof(Material), "material layout");
 
@@ -993,15 +1276,67 @@ void* orc_render_new(void* sp, int32_t w, int32_t h, const float* M, float tan_t
 void orc_render_free(void* r) { delete (Renderer*)r; }
 
 // light_geom_insertion: the insertion index of the light's mesh (mapped to its post-sort slot here)
-void orc_render_add_trilight(void* rp, const float* P0, const float* P1, const float* P2, int32_t material, int32_t samples,
-                             int32_t light_geom_insertion) {
+// type: 0 TriLight (P0, P1, P2 vertices), 1 DiskLight (P0 = P, P1 = Up, P2 = LookAt, radius),
+// 2 SphereLight (P0 = centre, radius).  light_geom_insertion: the insertion index of the light's
+// geometry (mapped to its post-sort slot here).
+void orc_render_add_light(void* rp, int32_t type, const float* P0, const float* P1, const float* P2, float radius,
+                          int32_t material, int32_t samples, int32_t light_geom_insertion) {
     Renderer* r = (Renderer*)rp;
-    TriLight L;
+    Light L;
+    L.type = type;
     L.P0 = V3(P0[0], P0[1], P0[2]); L.P1 = V3(P1[0], P1[1], P1[2]); L.P2 = V3(P2[0], P2[1], P2[2]);
+    L.radius = radius;
     L.material = material; L.samples = samples;
     L.geom = -1;
     for (size_t i = 0; i < r->scene->geomOrder.size(); i++) if (r->scene->geomOrder[i] == light_geom_insertion) L.geom = (int32_t)i;
     r->lights.push_back(L);
+}
+
+// type 0 box (no filter), 1 Gaussian (gauss.go:34-58; defaults Width 2, Res 17), 2 Airy (airy.go:82-106;
+// Width 6, Res 49, Peak 4).  Values <= 0 take the defaults.
+void orc_render_set_filter(void* rp, int32_t type, float width, int32_t res, float peak) {
+    Renderer* r = (Renderer*)rp;
+    r->filt.clear();
+    r->filtN = 0;
+    if (type == 0) return;
+    const bool gauss = type == 1;
+    const float W = width > 0 ? width : (gauss ? 2.0f : 6.0f);
+    const int n = res > 0 ? res : (gauss ? 17 : 49);
+    const float pk = peak > 0 ? peak : 4.0f;
+    if (gauss) {
+        CreateSampler(n, (double)W, [W](double x, double y) {
+            double q = std::sqrt(x * x + y * y);
+            if (q > (double)(W / 2)) return 0.0;
+            double sigma = (double)(1.0 / std::sqrt((double)W));
+            double G = (1 / (2 * M_PI * (sigma * sigma))) * std::exp(-(x * x + y * y) / 2 * (sigma * sigma));
+            return G;
+        }, r->filt);
+    } else {
+        CreateSampler(n, (double)W, [W, pk](double x, double y) {
+            double q = std::sqrt(x * x + y * y);
+            if (q > (double)(W / 2)) return 0.0;
+            double lambda = 550, N = 5.6;
+            double v = (20000 / (double)W) * (M_PI * q) / (lambda * N);
+            double t = 2 * BesselJ1(v) / v;
+            return (double)pk * (t * t);
+        }, r->filt);
+    }
+    r->filtN = n;
+    r->filtW = (double)W;
+}
+
+// Sampler.WarpSample on (r0, r1) for known-answer tests (box filter: (0, 0))
+void orc_render_warp_sample(void* rp, double r0, double r1, double* uv) {
+    Renderer* r = (Renderer*)rp;
+    double u = 0, v = 0;
+    if (r->filtN > 0) r->WarpSample(r0, r1, u, v);
+    uv[0] = u; uv[1] = v;
+}
+// the sampler's tables: cdfV (n) then cdfVU (n x n); returns n (0: box)
+int32_t orc_render_filter_tables(void* rp, double* out) {
+    Renderer* r = (Renderer*)rp;
+    if (out) std::memcpy(out, r->filt.data(), r->filt.size() * sizeof(double));
+    return r->filtN;
 }
 
 // One pass (core/render.go render() for every pixel).  scrambles: 6 uint64 per pixel (lensU, lensV,
@@ -1042,6 +1377,7 @@ int64_t orc_render_pass(void* rp, const uint64_t* scrambles, uint32_t iter, floa
 extern "C" {
 // BSDF pieces for known-answer tests
 float orc_fresnel_dielectric(float eta, float c) { return DielectricKr(eta, c).c[0]; }
+float orc_fresnel_conductor(float r, float g, float c) { return ConductorFresnel(r, g, c); }
 float orc_ggx_d(const float* m, float alpha) { return ggxD(V3(m[0], m[1], m[2]), alpha); }
 // Oren-Nayar Eval (as a spectrum, returned as its 4 values) for omegaI/omegaO given in the tangent frame
 void orc_oren_nayar_eval(float lambda, const float* wi, const float* wo, float roughness, float* out4) {

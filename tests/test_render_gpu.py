@@ -1,8 +1,9 @@
 """GPU parity of the renderer (reference semantics) against the oracle, through the C ABI.
 
-Workload: the box-and-spheres scene (5-wall box, 8 noise-displaced 64x32 UV spheres, 0.5 x 0.5 light
+Workloads: the box-and-spheres scene (5-wall box, 8 noise-displaced 64x32 UV spheres, 0.5 x 0.5 light
 as two triangle lights), 512 x 512, 16 passes of deterministic QMC (RasterXY / VdC / Sobol with a
-seeded scramble array).
+seeded scramble array); and the feature scene (conductor and perfect-mirror spheres, a Disk light and a
+Sphere light with its analytic sphere) under the box, Gaussian and Airy pixel filters.
 
 Bar: with both sides using the IEEE estimate in Vec3Normalize, every per-pass sample and the film are
 bit-identical (stricter than the 1e-4 relative bar).  With the oracle on the reference's RSQRTSS
@@ -103,11 +104,71 @@ def test_render_device_pointer_path(box_scene):
         oracle.set_normalize_ieee(False)
 
 
-def test_render_rejects_unsupported_materials(box_scene):
+def test_render_rejects_invalid_materials_and_lights(box_scene):
     mats = box_scene["materials"].copy()
-    mats[1]["spec1_roughness"] = 0.0           # perfect mirror: recursion not supported
+    mats[1]["fresnel_model"] = 3               # neither Dielectric nor Metal
     d = dict(box_scene, materials=mats)
     cam, _, _ = _camera(d, 64, 64)
     with pytest.raises(va.VgpuError) as ei:
         render.Renderer(d, 64, 64, cam)
     assert ei.value.code == va._lib.VGPU_ERR_ARG
+    lights = [dict(box_scene["lights"][0], type="sphere", P=(0.0, 1.0, 0.0), radius=0.1)]   # geom is a mesh
+    d = dict(box_scene, lights=lights)
+    with pytest.raises(va.VgpuError) as ei:
+        render.Renderer(d, 64, 64, cam)
+    assert ei.value.code == va._lib.VGPU_ERR_ARG
+
+
+@pytest.fixture(scope="module")
+def feature_scene():
+    print("\n[case] feature scene (metal / mirror spheres, disk and sphere lights)", flush=True)
+    return scenes.feature_scene()
+
+
+@pytest.mark.parametrize("filt", ["box", "gaussian", "airy"])
+def test_render_feature_scene_bitexact(feature_scene, filt):
+    """Conductor Fresnel, perfect mirrors to level 3, Disk and Sphere lights, an analytic sphere geom and
+    the three pixel filters, against the oracle: every per-pass sample bit-identical."""
+    oracle.set_normalize_ieee(True)
+    try:
+        w = h = 160
+        cam, M, ttf = _camera(feature_scene, w, h)
+        g = render.Renderer(feature_scene, w, h, cam)
+        o = oracle.OracleRenderer(feature_scene, w, h, M, ttf, feature_scene["camera"]["focal"])
+        g.set_filter(filt)
+        o.set_filter(filt)
+        scr = scenes.pixel_scrambles(w, h, seed=21)
+        for it in range(1, 5):
+            print(f"[case] feature scene filter {filt} pass {it}", flush=True)
+            ng, gs = g.render_pass(scr, it, want_sample=True)
+            no, os_ = o.render_pass(scr, it, want_sample=True)
+            assert ng == no, f"pass {it}: shadow rays {ng} vs {no}"
+            bad = ~((gs.view(np.uint32) == os_.view(np.uint32)) | (np.isnan(gs) & np.isnan(os_)))
+            assert not bad.any(), f"pass {it}: {int(bad.any(axis=2).sum())} pixels differ, first {np.argwhere(bad)[:3]}"
+        assert _nan_equal(g.film, o.film)
+        assert g.film.mean() > 0.02
+    finally:
+        oracle.set_normalize_ieee(False)
+
+
+def test_render_feature_scene_device_pointer_path(feature_scene):
+    import torch
+    oracle.set_normalize_ieee(True)
+    try:
+        w = h = 96
+        cam, M, ttf = _camera(feature_scene, w, h)
+        g = render.Renderer(feature_scene, w, h, cam)
+        o = oracle.OracleRenderer(feature_scene, w, h, M, ttf, feature_scene["camera"]["focal"])
+        g.set_filter("gaussian")
+        o.set_filter("gaussian")
+        scr = scenes.pixel_scrambles(w, h, seed=8)
+        d_scr = torch.from_numpy(scr.view(np.int64).reshape(-1).copy()).cuda()
+        d_film = torch.zeros(w * h * 3, dtype=torch.float32, device="cuda")
+        st = torch.cuda.current_stream().cuda_stream
+        for it in (1, 2, 3):
+            g.render_pass_device(d_scr.data_ptr(), it, d_film.data_ptr(), stream=st)
+            o.render_pass(scr, it)
+        g.scene.status(st)
+        assert _nan_equal(d_film.cpu().numpy().reshape(h, w, 3), o.film)
+    finally:
+        oracle.set_normalize_ieee(False)

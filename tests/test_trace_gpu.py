@@ -139,6 +139,66 @@ def test_edge_case_rays(sphere_ground):
         _same(hits, geoms, ref, "edge cases")
 
 
+def test_edge_rays_mixed_into_a_large_batch(sphere_ground):
+    """Zero, tiny and infinite direction components and non-finite origins scattered through a batch
+    of ordinary rays: wavefronts holding one of them take the exact box test, the rest the
+    hardware min/max form; ray refill mixes both kinds in one wavefront over time."""
+    g, o = sphere_ground
+    rng = np.random.default_rng(23)
+    n = 1 << 16
+    P = rng.uniform(-3, 3, (n, 3)).astype(np.float32)
+    D = rng.normal(size=(n, 3)).astype(np.float32)
+    special = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1], [1e-30, -1, 0], [-1, -1e-20, 0], [0, 0, 0],
+                        [np.inf, -1, 0], [-0.0, -1, 0.0], [1e-38, 1e-38, -1]], np.float32)
+    idx = np.arange(0, n, 53)
+    D[idx] = special[np.arange(len(idx)) % len(special)]
+    P[idx[::7]] = np.array([np.nan, 0, 0], np.float32)
+    P[idx[3::7]] = np.array([0, np.inf, 0], np.float32)
+    rays = scenes.make_rays(P, D)
+    for shadow in (False, True):
+        hits, geoms, _ = g.trace(rays, shadow=shadow)
+        ref, _ = o.trace(rays, shadow=shadow)
+        _same(hits, geoms, ref, f"mixed edge rays shadow={shadow}")
+
+
+@pytest.mark.parametrize("motion", [False, True])
+def test_scene_with_analytic_spheres_bitexact(motion):
+    """Analytic sphere geoms (a SphereLight's, builtin/geom/sphere/trace.go) beside meshes, static and
+    motion scenes: closest hit and shadow rays from outside, inside and grazing, against the oracle."""
+    print(f"\n[case] analytic spheres motion={motion}", flush=True)
+    gv, gf = scenes.ground_quad()
+    sv, sf = scenes.uv_sphere(0.5, 32, 16)
+    sv = sv + np.array([0.8, 0.6, 0.0], np.float32)
+    g = va.Scene(0)
+    o = oracle.OracleScene()
+    g.add_geom(va.PolyMesh(gv, gf))
+    o.add_mesh(gv, gf)
+    if motion:
+        keys = np.stack([sv, sv + np.array([0.0, 0.3, 0.0], np.float32)])
+        g.add_geom(va.PolyMesh(keys, sf))
+        o.add_mesh(keys, sf, motion_reference_indexing=False)
+    else:
+        g.add_geom(va.PolyMesh(sv, sf))
+        o.add_mesh(sv, sf)
+    for c, r in (((-0.7, 0.5, 0.2), 0.45), ((0.0, 1.6, -0.4), 0.25), ((-0.2, 0.35, 0.9), 0.3)):
+        g.add_sphere(c, r, 0)
+        o.add_sphere(c, r, 0)
+    g.pre_render()
+    o.commit()
+    rng = np.random.default_rng(31)
+    n = 1 << 15
+    P = rng.uniform(-1.5, 1.5, (n, 3)).astype(np.float32)
+    P[:, 1] = rng.uniform(0.05, 2.0, n)
+    D = rng.normal(size=(n, 3)).astype(np.float32)
+    P[:200] = np.array([-0.7, 0.5, 0.2], np.float32)          # inside a sphere
+    rays = scenes.make_rays(P, D, time=rng.uniform(0, 1, n).astype(np.float32) if motion else 0.0)
+    for shadow in (False, True):
+        hits, geoms, _ = g.trace(rays, shadow=shadow)
+        ref, _ = o.trace(rays, shadow=shadow)
+        _same(hits, geoms, ref, f"spheres motion={motion} shadow={shadow}")
+    assert (ref["prim"] == 0).sum() > 1000
+
+
 def test_empty_batch(sphere_ground):
     g, _ = sphere_ground
     hits, geoms, _ = g.trace(np.zeros(0, va.RAY_DTYPE))

@@ -27,6 +27,12 @@ VGPU_GUIDE_RAW = 4
 VGPU_GUIDE_NO_UPDATE = 8
 VGPU_GUIDE_NPARAMS = 24360
 VGPU_PT_NO_TRAIN = 16
+VGPU_LIGHT_TRI = 0
+VGPU_LIGHT_DISK = 1
+VGPU_LIGHT_SPHERE = 2
+VGPU_FILTER_BOX = 0
+VGPU_FILTER_GAUSSIAN = 1
+VGPU_FILTER_AIRY = 2
 
 # vgpu_qbvh_node (128 bytes, qbvh.Node layout)
 NODE_DTYPE = np.dtype([("boxes", "<f4", 24), ("axis", "<u4", 3), ("children", "<i4", 4), ("parent", "<i4")])
@@ -70,7 +76,9 @@ SIGNATURES = [
     ("vgpu_scene_set_timing", C.c_int, [_P, _I32]),
     ("vgpu_scene_timing", C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_I32)]),
     ("vgpu_scene_add_shaded_mesh", C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _P, _U32, C.POINTER(_I32)]),
+    ("vgpu_scene_add_sphere", C.c_int, [_P, _P, _F, _I32, C.POINTER(_I32)]),
     ("vgpu_render_create", C.c_int, [C.POINTER(_P), _P, _P, _U32, _P, _U32, _P, _U32, _U32]),
+    ("vgpu_render_set_filter", C.c_int, [_P, _I32, _F, _I32, _F]),
     ("vgpu_render_destroy", C.c_int, [_P]),
     ("vgpu_render_pass", C.c_int, [_P, _U32, _P, _U32, _P, _P, _P, C.POINTER(_U64)]),
     ("vgpu_guide_create", C.c_int, [C.POINTER(_P), C.c_int, _P, _U64, _U32]),
@@ -87,7 +95,7 @@ SIGNATURES = [
     ("vgpu_pt_stats", C.c_int, [_P, C.POINTER(_U64), C.POINTER(_U32)]),
 ]
 
-# vgpu_material / vgpu_tri_light / vgpu_camera layouts
+# vgpu_material / vgpu_light / vgpu_camera layouts
 MATERIAL_DTYPE = np.dtype([
     ("emission", "<f4", 3), ("emission_strength", "<f4"), ("has_emission_colour", "<i4"), ("has_emission_strength", "<i4"),
     ("diffuse", "<f4", 3), ("has_diffuse_colour", "<i4"),
@@ -98,9 +106,11 @@ MATERIAL_DTYPE = np.dtype([
     ("spec1_roughness", "<f4"), ("has_spec1_roughness", "<i4"),
     ("ior", "<f4"), ("has_ior", "<i4"),
     ("fresnel_model", "<i4"),
+    ("spec1_fresnel_refl", "<f4", 3), ("has_spec1_fresnel_refl", "<i4"),
+    ("spec1_fresnel_edge", "<f4", 3), ("has_spec1_fresnel_edge", "<i4"),
 ])
-TRI_LIGHT_DTYPE = np.dtype([("P0", "<f4", 3), ("P1", "<f4", 3), ("P2", "<f4", 3), ("material", "<i4"),
-                            ("samples", "<i4"), ("geom", "<i4")])
+LIGHT_DTYPE = np.dtype([("P0", "<f4", 3), ("P1", "<f4", 3), ("P2", "<f4", 3), ("material", "<i4"),
+                        ("samples", "<i4"), ("geom", "<i4"), ("type", "<i4"), ("radius", "<f4")])
 CAMERA_DTYPE = np.dtype([("M", "<f4", 16), ("tan_theta_focal", "<f4"), ("aspect", "<f4"), ("focal", "<f4")])
 
 _lib = None

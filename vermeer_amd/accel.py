@@ -136,6 +136,14 @@ class Scene:
         self.geoms.append(None)
         return g.value
 
+    def add_sphere(self, centre, radius, material):
+        """The analytic sphere a SphereLight inserts (builtin/geom/sphere) with its shader (a material id)."""
+        c = np.ascontiguousarray(centre, dtype=np.float32).reshape(3)
+        g = C.c_int32(-1)
+        check(self._lib.vgpu_scene_add_sphere(self._h, ptr(c), float(radius), int(material), C.byref(g)))
+        self.geoms.append(None)
+        return g.value
+
     def pre_render(self):
         check(self._lib.vgpu_scene_commit(self._h))
         self.committed = True

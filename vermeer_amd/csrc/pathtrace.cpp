@@ -66,6 +66,8 @@ int vgpu_pt_create(vgpu_pt** out, vgpu_scene* s, const vgpu_pt_material* materia
             if (!(a >= 0.0f && a <= 1.0f)) return fail(VGPU_ERR_ARG, "material %u: albedo must be in [0,1]", i);
             if (!(e >= 0.0f) || !std::isfinite(e)) return fail(VGPU_ERR_ARG, "material %u: emission must be >= 0", i);
         }
+    for (const MeshHost& m : s->meshes)
+        if (m.sphere) return fail(VGPU_ERR_ARG, "path mode takes triangle meshes only (the scene holds a sphere)");
     // emissive triangles become the NEE light list (selection ~ area x luminance)
     std::vector<vgpu::PTLight> L;
     std::vector<double> w;

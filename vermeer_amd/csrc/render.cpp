@@ -1,9 +1,11 @@
 // Render object and its C-ABI entry points (vgpu_render_*).
 //
-// One pass = five launches on one stream: camera rays, closest hit, shade
-// (shadow-ray slots), shadow any-hit over every slot, resolve into the film.
-// All buffers are allocated once at creation; a pass allocates nothing
-// except, in host-pointer mode, a staging area the first time it is needed.
+// One pass on one stream: camera rays; then per recursion level (one level
+// without perfect mirrors, four with) closest hit, shade (shadow-ray slots
+// and the mirror rays of the next level), shadow any-hit over every slot and
+// resolve; finally the levels are folded bottom-up into the film.  All
+// buffers are allocated once at creation; a pass allocates nothing except,
+// in host-pointer mode, a staging area the first time it is needed.
 
 #include <new>
 
@@ -62,6 +64,68 @@ bool raster_tables(int m, uint64_t* fwd, int nfwd, uint64_t* inv) {
     return true;
 }
 
+// builtin/filter/airy.go:35-79: J1 by the rational / asymptotic approximation the reference
+// copies (Numerical Recipes bessj1, via gipsy bessel.c).
+double bessel_j1(double x) {
+    const double ax = std::fabs(x);
+    if (ax < 8.0) {
+        const double y = x * x;
+        const double ans1 = x * (72362614232.0 + y * (-7895059235.0 + y * (242396853.1 + y * (-2972611.439 +
+                                 y * (15704.48260 + y * (-30.16036606))))));
+        const double ans2 = 144725228442.0 + y * (2300535178.0 + y * (18583304.74 + y * (99447.43394 +
+                                 y * (376.9991397 + y * 1.0))));
+        return ans1 / ans2;
+    }
+    const double z = 8.0 / ax, y = z * z, xx = ax - 2.356194491;
+    const double ans1 = 1.0 + y * (0.183105e-2 + y * (-0.3516396496e-4 + y * (0.2457520174e-5 + y * (-0.240337019e-6))));
+    const double ans2 = 0.04687499995 + y * (-0.2002690873e-3 + y * (0.8449199096e-5 + y * (-0.88228987e-6 +
+                        y * 0.105787412e-6)));
+    double ans = std::sqrt(0.636619772 / ax) * (std::cos(xx) * ans1 - z * std::sin(xx) * ans2);
+    if (x < 0.0) ans = -ans;
+    return ans;
+}
+
+// filter.CreateSampler (builtin/filter/filter.go:87-173): the filter on an n x n grid over
+// [-w/2, w/2]^2 (u and v advanced by repeated addition), normalised to a pdf, its marginal over v
+// and the conditionals, as running-sum CDFs.  out = cdfV (n) followed by cdfVU (n x n).
+template <class F>
+void create_sampler(int n, double w, F f, std::vector<double>& out) {
+    std::vector<double> filter((size_t)n * n), pV(n, 0.0), cdfV(n), pdf((size_t)n * n), pVU((size_t)n * n);
+    const double du = w / (double)(n - 1);
+    double u = -w / 2;
+    double F_ = 0;
+    for (int j = 0; j < n; j++) {
+        const double dv = w / (double)(n - 1);
+        double v = -w / 2;
+        for (int i = 0; i < n; i++) {
+            const double fuv = f(u, v);
+            filter[(size_t)j + (size_t)i * n] = fuv;
+            F_ += fuv;
+            v += dv;
+        }
+        u += du;
+    }
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) pdf[(size_t)j * n + i] = filter[(size_t)j + (size_t)i * n] / F_;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) pV[j] += pdf[(size_t)j * n + i];
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) pVU[(size_t)j * n + i] = pdf[(size_t)j * n + i] / pV[j];
+    out.assign((size_t)n + (size_t)n * n, 0.0);
+    double pacc = 0;
+    for (int i = 0; i < n; i++) {
+        pacc += pV[i];
+        out[i] = pacc;
+    }
+    for (int j = 0; j < n; j++) {
+        double q = 0;
+        for (int i = 0; i < n; i++) {
+            q += pVU[(size_t)j * n + i];
+            out[(size_t)n + (size_t)j * n + i] = q;
+        }
+    }
+}
+
 }  // namespace
 
 struct vgpu_render {
@@ -70,7 +134,10 @@ struct vgpu_render {
     uint32_t width = 0, height = 0;
     vgpu_material* d_materials = nullptr;
     vgpu::DevLight* d_lights = nullptr;
-    float4* d_rays = nullptr;      // camera rays (2 float4 per pixel)
+    float4* d_rays = nullptr;      // camera rays and even levels' mirror rays (2 float4 per pixel)
+    float4* d_rays2 = nullptr;     // odd levels' mirror rays
+    vgpu::LevelRec* d_recs = nullptr;   // per pixel, per level
+    double* d_filter = nullptr;    // pixel filter sampler tables
     float4* d_pix = nullptr;       // per pixel: D, lambda
     int4* d_hits32 = nullptr;      // extended hits (2 int4 per pixel)
     float4* d_slot_rays = nullptr;
@@ -84,8 +151,8 @@ struct vgpu_render {
     ~vgpu_render() {
         int prev = 0;
         if (hipGetDevice(&prev) == hipSuccess && scene) (void)hipSetDevice(scene->device);
-        void* ptrs[] = {d_materials, d_lights, d_rays, d_pix, d_hits32, d_slot_rays, d_slot_rgb, d_slot_hits,
-                        d_groups, d_pixshade, d_stage};
+        void* ptrs[] = {d_materials, d_lights, d_rays, d_rays2, d_recs, d_filter, d_pix, d_hits32, d_slot_rays,
+                        d_slot_rgb, d_slot_hits, d_groups, d_pixshade, d_stage};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         (void)hipSetDevice(prev);
     }
@@ -94,7 +161,7 @@ struct vgpu_render {
 extern "C" {
 
 int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* materials, uint32_t nmaterials,
-                       const vgpu_tri_light* lights, uint32_t nlights, const vgpu_camera* cam, uint32_t width,
+                       const vgpu_light* lights, uint32_t nlights, const vgpu_camera* cam, uint32_t width,
                        uint32_t height) {
     if (!out || !s || !materials || !cam) return fail(VGPU_ERR_ARG, "NULL argument");
     *out = nullptr;
@@ -104,13 +171,15 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
         return fail(VGPU_ERR_ARG, "image size %ux%u outside [1,4096] (RasterXY uses 2^12 intervals)", width, height);
     if (s->max_material >= (int32_t)nmaterials)
         return fail(VGPU_ERR_ARG, "a mesh refers to material %d of %u", s->max_material, nmaterials);
+    bool any_mirror = false;
     for (uint32_t i = 0; i < nmaterials; i++) {
         const vgpu_material& m = materials[i];
         const float rough = m.has_spec1_roughness ? m.spec1_roughness : 0.5f;
         const float sw = m.has_spec1_strength ? m.spec1_strength : 0.0f;
-        if (sw > 0.0f && rough == 0.0f)
-            return fail(VGPU_ERR_ARG, "material %u: perfect mirrors (Spec1Roughness 0) are not supported", i);
-        if (sw > 0.0f && m.fresnel_model != 1) return fail(VGPU_ERR_ARG, "material %u: only the dielectric Fresnel model", i);
+        if (sw > 0.0f && rough == 0.0f) any_mirror = true;
+        if (m.fresnel_model < 0 || m.fresnel_model > VGPU_FRESNEL_METAL)
+            return fail(VGPU_ERR_ARG, "material %u: fresnel_model %d is not 0, 1 (Dielectric) or 2 (Metal)", i,
+                        m.fresnel_model);
         const float dw = m.has_diffuse_strength ? m.diffuse_strength : 0.0f;
         if (dw + sw == 0.0f) return fail(VGPU_ERR_ARG, "material %u has no weight (std.go:141-143 panics)", i);
     }
@@ -118,15 +187,22 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     std::vector<vgpu::DevLight> dl(nlights);
     int max_ns = 1;
     for (uint32_t l = 0; l < nlights; l++) {
-        const vgpu_tri_light& L = lights[l];
+        const vgpu_light& L = lights[l];
         if (L.material < 0 || L.material >= (int32_t)nmaterials) return fail(VGPU_ERR_ARG, "light %u material", l);
         if (L.samples < 0 || L.samples > 8) return fail(VGPU_ERR_ARG, "light %u samples must be in [0,8]", l);
         if (L.geom < 0 || L.geom >= ngeoms) return fail(VGPU_ERR_ARG, "light %u geom %d out of range", l, L.geom);
-        dl[l].P0 = make_float4(L.P0[0], L.P0[1], L.P0[2], 0.0f);
+        if (L.type != VGPU_LIGHT_TRI && L.type != VGPU_LIGHT_DISK && L.type != VGPU_LIGHT_SPHERE)
+            return fail(VGPU_ERR_ARG, "light %u type %d", l, L.type);
+        if (L.type != VGPU_LIGHT_TRI && !(L.radius > 0.0f)) return fail(VGPU_ERR_ARG, "light %u radius must be > 0", l);
+        if ((L.type == VGPU_LIGHT_SPHERE) != s->meshes[L.geom].sphere)
+            return fail(VGPU_ERR_ARG, "light %u: a sphere light's geom must be its vgpu_scene_add_sphere geom", l);
+        const float rad = L.type == VGPU_LIGHT_TRI ? 0.0f : L.radius;
+        dl[l].P0 = make_float4(L.P0[0], L.P0[1], L.P0[2], rad);
         dl[l].P1 = make_float4(L.P1[0], L.P1[1], L.P1[2], 0.0f);
         dl[l].P2 = make_float4(L.P2[0], L.P2[1], L.P2[2], 0.0f);
         dl[l].material = L.material;
         dl[l].samples = L.samples;
+        dl[l].type = L.type;
         dl[l].geom = -1;
         for (int slot = 0; slot < ngeoms; slot++) if (s->order[slot] == L.geom) dl[l].geom = slot;
         max_ns = std::max(max_ns, 1 << L.samples);
@@ -148,6 +224,10 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     rv.tan_theta_focal = cam->tan_theta_focal;
     rv.aspect = cam->aspect;
     rv.focal = cam->focal;
+    rv.filt = nullptr;
+    rv.filt_n = 0;
+    rv.filt_w = 0.0;
+    rv.max_level = any_mirror ? vgpu::kMaxLevel : 0;
     if (!raster_tables(12, rv.raster_fwd, 52, rv.raster_inv)) { delete r; return fail(VGPU_ERR_ARG, "RasterXY tables"); }
     DeviceGuard dg(s->device);
     const size_t np = (size_t)width * height, ns = np * (size_t)rv.slots;
@@ -156,6 +236,8 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     if (e == hipSuccess) e = alloc((void**)&r->d_materials, sizeof(vgpu_material) * nmaterials);
     if (e == hipSuccess) e = alloc((void**)&r->d_lights, sizeof(vgpu::DevLight) * nlights);
     if (e == hipSuccess) e = alloc((void**)&r->d_rays, np * 32);
+    if (e == hipSuccess && any_mirror) e = alloc((void**)&r->d_rays2, np * 32);
+    if (e == hipSuccess) e = alloc((void**)&r->d_recs, np * sizeof(vgpu::LevelRec) * (vgpu::kMaxLevel + 1));
     if (e == hipSuccess) e = alloc((void**)&r->d_pix, np * 16);
     if (e == hipSuccess) e = alloc((void**)&r->d_hits32, np * 32);
     if (e == hipSuccess) e = alloc((void**)&r->d_slot_rays, ns * 32);
@@ -215,17 +297,31 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
     }
     hipError_t e = vgpu::launch_camera(r->rv, d_scr, iter, r->d_rays, r->d_pix, st);
     if (e != hipSuccess) return fail(VGPU_ERR_HIP, "camera: %s", hipGetErrorString(e));
-    e = vgpu::launch_trace_ext(s->view, false, r->d_rays, np, r->d_hits32, s->d_spT, s->d_spN, s->spill_threads, s->d_err,
-                               s->num_cus, st);
-    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "closest hit: %s", hipGetErrorString(e));
-    e = vgpu::launch_shade(r->rv, d_scr, iter, r->d_hits32, r->d_pix, r->d_slot_rays, r->d_slot_rgb, r->d_groups,
-                           r->d_pixshade, st);
-    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "shade: %s", hipGetErrorString(e));
-    int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns, r->d_slot_hits, nullptr,
-                           nullptr, st, s->cfg);
-    if (rc) return rc;
-    e = vgpu::launch_resolve(r->rv, iter, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, d_film, d_samp, st);
-    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "resolve: %s", hipGetErrorString(e));
+    uint64_t shadow_count = 0;
+    for (int level = 0; level <= r->rv.max_level; level++) {
+        float4* rin = (level & 1) ? r->d_rays2 : r->d_rays;
+        float4* rout = level < r->rv.max_level ? ((level & 1) ? r->d_rays : r->d_rays2) : nullptr;
+        e = vgpu::launch_trace_ext(s->view, false, rin, np, r->d_hits32, s->d_spT, s->d_spN, s->spill_threads, s->d_err,
+                                   s->num_cus, st);
+        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "closest hit: %s", hipGetErrorString(e));
+        e = vgpu::launch_shade(r->rv, d_scr, iter, level, rin, r->d_hits32, r->d_pix, r->d_slot_rays, r->d_slot_rgb,
+                               r->d_groups, r->d_pixshade, rout, r->d_recs, st);
+        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "shade: %s", hipGetErrorString(e));
+        int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns, r->d_slot_hits, nullptr,
+                               nullptr, st, s->cfg);
+        if (rc) return rc;
+        e = vgpu::launch_resolve(r->rv, level, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, r->d_recs, st);
+        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "resolve: %s", hipGetErrorString(e));
+        if (host && shadow_rays_out) {
+            // count active slots (diagnostic, host mode only)
+            std::vector<float4> rgb(ns);
+            HIP_TRY(hipMemcpyAsync(rgb.data(), r->d_slot_rgb, ns * 16, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (const float4& v : rgb) shadow_count += v.w != 0.0f;
+        }
+    }
+    e = vgpu::launch_combine(r->rv, iter, r->d_recs, d_film, d_samp, st);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "combine: %s", hipGetErrorString(e));
     if (host) {
         HIP_TRY(hipMemcpyAsync(film, d_film, np * 12, hipMemcpyDeviceToHost, st));
         if (sample_out) HIP_TRY(hipMemcpyAsync(sample_out, d_samp, np * 12, hipMemcpyDeviceToHost, st));
@@ -233,15 +329,53 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
         HIP_TRY(hipMemcpyAsync(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
-        if (shadow_rays_out) {
-            // count active slots (diagnostic, host mode only)
-            std::vector<float4> rgb(ns);
-            HIP_TRY(hipMemcpy(rgb.data(), r->d_slot_rgb, ns * 16, hipMemcpyDeviceToHost));
-            uint64_t c = 0;
-            for (const float4& v : rgb) c += v.w != 0.0f;
-            *shadow_rays_out = c;
-        }
+        if (shadow_rays_out) *shadow_rays_out = shadow_count;
     }
+    return VGPU_OK;
+}
+
+int vgpu_render_set_filter(vgpu_render* r, int32_t type, float width, int32_t res, float peak) {
+    if (!r) return fail(VGPU_ERR_ARG, "render is NULL");
+    if (type != VGPU_FILTER_BOX && type != VGPU_FILTER_GAUSSIAN && type != VGPU_FILTER_AIRY)
+        return fail(VGPU_ERR_ARG, "filter type %d", type);
+    DeviceGuard dg(r->scene->device);
+    if (type == VGPU_FILTER_BOX) {
+        r->rv.filt = nullptr;
+        r->rv.filt_n = 0;
+        return VGPU_OK;
+    }
+    const bool gauss = type == VGPU_FILTER_GAUSSIAN;
+    // node defaults (filter.go:14-25)
+    const float W = width > 0.0f ? width : (gauss ? 2.0f : 6.0f);
+    const int n = res > 0 ? res : (gauss ? 17 : 49);
+    const float pk = peak > 0.0f ? peak : 4.0f;
+    if (n < 2 || n > 1024) return fail(VGPU_ERR_ARG, "filter resolution %d outside [2,1024]", n);
+    std::vector<double> tab;
+    if (gauss) {
+        // GaussianFilter.PreRender (gauss.go:34-58); the exponent reads -(r^2)/2 * sigma^2 as written
+        create_sampler(n, (double)W, [W](double x, double y) {
+            const double q = std::sqrt(x * x + y * y);
+            if (q > (double)(W / 2)) return 0.0;
+            const double sigma = (double)(1.0 / std::sqrt((double)W));
+            return (1 / (2 * M_PI * (sigma * sigma))) * std::exp(-(x * x + y * y) / 2 * (sigma * sigma));
+        }, tab);
+    } else {
+        // AiryFilter.PreRender (airy.go:82-106); at q = 0 the value is 0/0 = NaN, as in the reference
+        create_sampler(n, (double)W, [W, pk](double x, double y) {
+            const double q = std::sqrt(x * x + y * y);
+            if (q > (double)(W / 2)) return 0.0;
+            const double lambda = 550.0, N = 5.6;
+            const double v = (20000 / (double)W) * (M_PI * q) / (lambda * N);
+            const double t = 2 * bessel_j1(v) / v;
+            return (double)pk * (t * t);
+        }, tab);
+    }
+    if (r->d_filter) { HIP_TRY(hipFree(r->d_filter)); r->d_filter = nullptr; }
+    HIP_TRY(hipMalloc((void**)&r->d_filter, tab.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(r->d_filter, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    r->rv.filt = r->d_filter;
+    r->rv.filt_n = n;
+    r->rv.filt_w = (double)W;
     return VGPU_OK;
 }
 

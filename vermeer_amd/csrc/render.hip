@@ -7,7 +7,7 @@
 //   (trace)    closest hit with barycentric weights (trace.hip, EXT records)
 //   k_shade    hit attributes (builtin/geom/polymesh/trace.go:280-513),
 //              ShaderStd.Eval (builtin/shader/std.go:77-296) with Oren-Nayar and
-//              GGX lobes, TriLight ValidSample / SampleArea
+//              GGX lobes (Dielectric or Conductor Fresnel), TriLight ValidSample / SampleArea
 //              (builtin/light/triangle.go:136-343) and the MIS weights of
 //              EvaluateLightSamples (core/shader.go:203-402).  Every sample that
 //              would trace a shadow ray gets a slot: the shadow ray and the
@@ -290,9 +290,43 @@ __device__ float dielectric_kr(float eta, float c) {
     g = sqrtf(g);
     return 0.5f * sqr32((g - c) / (g + c)) * (1 + sqr32((c * (g + c) - 1) / (c * (g - c) + 1)));
 }
+// builtin/shader/fresnel/conductor.go:31-81 ("Artist Friendly Metallic Fresnel", Gulbrandsen 2014):
+// index n and extinction k^2 from reflectivity r and edge tint g, then the unpolarised conductor term.
+__device__ __forceinline__ float cond_nmin(float r) { return (1 - r) / (1 + r); }
+__device__ __forceinline__ float cond_nmax(float r) { return (1 + sqrtf(r)) / (1 - sqrtf(r)); }
+__device__ float conductor_fresnel(float r, float g, float c) {
+    const float nr = maxs(0.0f, mins(r, 0.99f));   // math.Clamp = Max(min, Min(x, max))
+    const float n = cond_nmin(nr) * g + (1 - g) * cond_nmax(nr);
+    const float k2 = ((n + 1) * (n + 1) * nr - (n - 1) * (n - 1)) / (1 - nr);
+    const float rs = (n * n + k2 - 2 * n * c + c * c) / (n * n + k2 + 2 * n * c + c * c);
+    const float rp = ((n * n + k2) * c * c - 2 * n * c + 1) / ((n * n + k2) * c * c + 2 * n * c + 1);
+    return 0.5f * (rs + rp);
+}
+// The core.Fresnel ShaderStd builds (std.go:171-190): Dielectric(ior) or Conductor(refl, edge).
+struct Fresnel {
+    bool metal; float eta; RGB refl, edge;
+    __device__ RGB kr(float c) const {
+        if (!metal) { const float f = dielectric_kr(eta, c); return RGB{f, f, f}; }
+        return RGB{conductor_fresnel(refl.r, edge.r, c), conductor_fresnel(refl.g, edge.g, c),
+                   conductor_fresnel(refl.b, edge.b, c)};
+    }
+};
+__device__ Fresnel make_fresnel(const vgpu_material& m, float ior) {
+    Fresnel f;
+    f.metal = m.fresnel_model == VGPU_FRESNEL_METAL;
+    f.eta = ior;
+    f.refl = RGB{0.5f, 0.5f, 0.5f};
+    f.edge = RGB{0.5f, 0.5f, 0.5f};
+    if (f.metal) {
+        if (m.has_spec1_fresnel_refl) f.refl = RGB{m.spec1_fresnel_refl[0], m.spec1_fresnel_refl[1], m.spec1_fresnel_refl[2]};
+        // std.go:185-187: a set Spec1FresnelEdge overwrites the reflectivity; the edge tint stays grey
+        if (m.has_spec1_fresnel_edge) f.refl = RGB{m.spec1_fresnel_edge[0], m.spec1_fresnel_edge[1], m.spec1_fresnel_edge[2]};
+    }
+    return f;
+}
 // builtin/shader/bsdf/microfacetggx.go
 struct GGX {
-    float lambda; V3 wr; float rough; float eta; Frame f;
+    float lambda; V3 wr; float rough; Fresnel fres; Frame f;
     __device__ V3 sample(double r0, double r1) const {
         const float alpha = sqr32(rough);
         const double thM = atan2((double)alpha * sqrt(r0), sqrt(1 - r0));
@@ -312,10 +346,10 @@ struct GGX {
         const V3 wi = basis_project(f.U, f.V, f.N, o_);
         const float alpha = sqr32(rough);
         const V3 h = scale(signf(wr.z), normalize(add(wr, wi)));
-        const float F = dielectric_kr(eta, dotabs(wr, h));
+        const RGB F = fres.kr(dotabs(wr, h));
         const float numer = ggx_g1(wr, h, alpha) * ggx_g1(wi, h, alpha) * ggx_d(h, alpha);
         const float denom = 4 * fabs32(wr.z) * fabs32(wi.z);
-        Spec rho = spec_from_rgb(lambda, RGB{F, F, F});
+        Spec rho = spec_from_rgb(lambda, F);
         spec_scale(rho, fabs32(wi.z) * numer / denom);
         for (int k = 0; k < 4; k++) if (rho.c[k] < 0 || __builtin_isnan(rho.c[k])) rho.c[k] = 0;
         return rho;
@@ -400,7 +434,7 @@ struct LSample { float pdf; Spec liu; V3 ld; float ldist; bool ok; };
 struct BSample { V3 d; double pdf; float pdf_light; Spec liu; V3 ld; float ldist; bool ok; };
 
 // Tri.ValidSample (triangle.go:136-230)
-__device__ bool valid_sample(const DevLight& L, const vgpu_material& lm, const SC& sg, BSample& s) {
+__device__ bool tri_valid_sample(const DevLight& L, const vgpu_material& lm, const SC& sg, BSample& s) {
     const V3 P0 = from4(L.P0), P1 = from4(L.P1), P2 = from4(L.P2);
     V3 P;
     if (!ray_triangle(sg.P, s.d, P0, P1, P2, P)) return false;
@@ -428,7 +462,7 @@ __device__ bool valid_sample(const DevLight& L, const vgpu_material& lm, const S
 }
 
 // Tri.SampleArea for sample i of n (triangle.go:232-343); ok=false when the sample is skipped.
-__device__ LSample sample_area(const DevLight& L, const vgpu_material& lm, const SC& sg, int n, int i) {
+__device__ LSample tri_sample_area(const DevLight& L, const vgpu_material& lm, const SC& sg, int n, int i) {
     LSample ls; ls.ok = false;
     const V3 P0 = from4(L.P0), P1 = from4(L.P1), P2 = from4(L.P2);
     const V3 N = normalize(cross(sub(P1, P0), sub(P2, P0)));
@@ -460,6 +494,156 @@ __device__ LSample sample_area(const DevLight& L, const vgpu_material& lm, const
     ls.pdf = area_branch ? pdfv * sqr32(ls.ldist) / dotabs(ls.ld, N) : pdfv;
     ls.ok = true;
     return ls;
+}
+
+
+// ---- Disk (builtin/light/disk.go)
+
+// rayPlaneIntersect (disk.go:38-51)
+__device__ __forceinline__ bool ray_plane(V3 Ro, V3 Rd, V3 P, V3 N, float& t) {
+    const float denom = dot(N, Rd);
+    t = 0.0f;
+    if (fabs32(denom) > 1e-6f) {
+        t = dot(sub(P, Ro), N) / denom;
+        return t >= 0;
+    }
+    return false;
+}
+// Disk.PreRender frame (disk.go:84-86): N towards LookAt, T = N x Up, B = N x T
+__device__ __forceinline__ void disk_frame(const DevLight& L, V3& N, V3& T, V3& B) {
+    N = normalize(sub(from4(L.P2), from4(L.P0)));
+    T = normalize(cross(N, from4(L.P1)));
+    B = cross(N, T);
+}
+// Disk.ValidSample (disk.go:120-168)
+__device__ bool disk_valid_sample(const DevLight& L, const vgpu_material& lm, const SC& sg, BSample& s) {
+    const V3 P = from4(L.P0);
+    const float R = L.P0.w;
+    V3 N, T, B;
+    disk_frame(L, N, T, B);
+    const double pdf = (double)(1.0f / (PI32 * R * R));
+    float t;
+    if (!ray_plane(sg.P, s.d, P, N, t)) return false;    // rayDiskIntersect (disk.go:54-68)
+    {
+        const V3 v = sub(mad(sg.P, s.d, t), P);
+        if (!(sqrtf(dot(v, v)) <= R)) return false;
+    }
+    const V3 p = mad(sg.P, s.d, t);
+    const V3 V = sub(p, sg.P);
+    if (dot(V, sg.Ng) <= 0.0f || dot(V, N) >= 0.0f) return false;
+    s.ldist = length(V);
+    s.ld = normalize(V);
+    s.liu = spec_from_rgb(sg.lambda, emission(lm));
+    s.pdf_light = (float)pdf * (s.ldist * s.ldist) / fabs32(dot(s.ld, N));
+    return true;
+}
+// Disk.SampleArea for sample i of n (disk.go:171-215)
+__device__ LSample disk_sample_area(const DevLight& L, const vgpu_material& lm, const SC& sg, int n, int i) {
+    LSample ls; ls.ok = false;
+    const V3 P0 = from4(L.P0);
+    const float R = L.P0.w;
+    V3 N, T, B;
+    disk_frame(L, N, T, B);
+    const uint64_t idx = sg.I * (uint64_t)n + (uint64_t)i;
+    const double r0 = vdc(idx, sg.scr0), r1 = sobol(idx, sg.scr1);
+    const float u = R * sqrtf((float)r0) * fcos(2.0f * PI32 * (float)r1);
+    const float v = R * sqrtf((float)r0) * fsin(2.0f * PI32 * (float)r1);
+    const double pdf = (double)(1.0f / (PI32 * R * R));
+    const V3 su = scale(u, B), sv = scale(v, T);
+    const V3 P = v3(P0.x + su.x + sv.x, P0.y + su.y + sv.y, P0.z + su.z + sv.z);   // Vec3Add3
+    const V3 V = sub(P, sg.P);
+    if (!(dot(V, sg.Ng) > 0.0f && dot(V, N) < 0.0f)) return ls;
+    ls.ldist = length(V);
+    ls.ld = normalize(V);
+    ls.liu = spec_from_rgb(sg.lambda, emission(lm));
+    ls.pdf = (float)pdf * (ls.ldist * ls.ldist) / fabs32(dot(ls.ld, N));
+    ls.ok = true;
+    return ls;
+}
+
+// ---- Sphere (builtin/light/sphere.go; the same quadratic as builtin/geom/sphere/trace.go:54-103)
+
+__device__ bool ray_sphere(V3 Ro, V3 Rd, V3 C, float radius, float& t) {
+    const V3 L = sub(Ro, C);
+    const float a = dot(Rd, Rd);
+    const float b = 2 * dot(Rd, L);
+    const float c = dot(L, L) - radius * radius;
+    float x0, x1;
+    const float discr = b * b - 4 * a * c;
+    if (discr < 0) return false;
+    if (discr == 0) {
+        x1 = -0.5f * b / a;
+        x0 = x1;
+    } else {
+        const float q = b > 0 ? -0.5f * (b + sqrtf(discr)) : -0.5f * (b - sqrtf(discr));
+        x0 = q / a;
+        x1 = c / q;
+    }
+    if (x0 > x1) { const float tt = x0; x0 = x1; x1 = tt; }   // (raySphereIntersect's second swap is a no-op)
+    if (x0 < 0) {
+        x0 = x1;
+        if (x0 < 0) return false;
+    }
+    t = x0;
+    return true;
+}
+// Shirley 1996 cone pdf q_2 (sphere.go:170, :253)
+__device__ __forceinline__ float sphere_cone_pdf(float R, float l) {
+    return 1 / (2 * PI32 * (1 - sqrtf(1 - sqr32(R / l))));
+}
+// Sphere.ValidSample (sphere.go:129-177)
+__device__ bool sphere_valid_sample(const DevLight& L, const vgpu_material& lm, const SC& sg, BSample& s) {
+    const V3 C = from4(L.P0);
+    const float R = L.P0.w;
+    float t;
+    if (!ray_sphere(sg.P, s.d, C, R, t)) return false;
+    const V3 x = mad(sg.P, s.d, t);
+    const V3 D = sub(x, sg.P);
+    s.ldist = length(D);
+    s.ld = normalize(D);
+    s.liu = spec_from_rgb(sg.lambda, emission(lm));
+    const float l = length(sub(C, sg.P));
+    s.pdf_light = sphere_cone_pdf(R, l);
+    return true;
+}
+// Sphere.SampleArea for sample i of n (sphere.go:180-256); sg.Sample is 0 here
+__device__ LSample sphere_sample_area(const DevLight& L, const vgpu_material& lm, const SC& sg, int n, int i) {
+    LSample ls; ls.ok = false;
+    const V3 C = from4(L.P0);
+    const float R = L.P0.w;
+    const V3 V = sub(C, sg.P);
+    const float l = length(V);
+    const V3 w = normalize(V);
+    const V3 v = normalize(cross(w, sg.Ng));
+    const V3 u = cross(w, v);
+    const uint64_t idx = sg.I * (uint64_t)n + (uint64_t)i;
+    const double r0 = vdc(idx, sg.scr0), r1 = sobol(idx, sg.scr1);
+    const float theta = facos(1 - (float)r0 + (float)r0 * sqrtf(1 - sqr32(R / l)));
+    const float phi = 2 * PI32 * (float)r1;
+    const V3 a = v3(fcos(phi) * fsin(theta), fsin(phi) * fsin(theta), fcos(theta));
+    const V3 omega = basis_expand(u, v, w, a);
+    if (dot(omega, sg.Ng) < 0) return ls;
+    float t;
+    if (!ray_sphere(sg.P, omega, C, R, t)) return ls;
+    const V3 x = mad(sg.P, omega, t);
+    const V3 D = sub(x, sg.P);
+    ls.ldist = length(D);
+    ls.ld = normalize(D);
+    ls.liu = spec_from_rgb(sg.lambda, emission(lm));
+    ls.pdf = sphere_cone_pdf(R, l);
+    ls.ok = true;
+    return ls;
+}
+
+__device__ bool valid_sample(const DevLight& L, const vgpu_material& lm, const SC& sg, BSample& s) {
+    if (L.type == VGPU_LIGHT_DISK) return disk_valid_sample(L, lm, sg, s);
+    if (L.type == VGPU_LIGHT_SPHERE) return sphere_valid_sample(L, lm, sg, s);
+    return tri_valid_sample(L, lm, sg, s);
+}
+__device__ LSample sample_area(const DevLight& L, const vgpu_material& lm, const SC& sg, int n, int i) {
+    if (L.type == VGPU_LIGHT_DISK) return disk_sample_area(L, lm, sg, n, i);
+    if (L.type == VGPU_LIGHT_SPHERE) return sphere_sample_area(L, lm, sg, n, i);
+    return tri_sample_area(L, lm, sg, n, i);
 }
 
 // core/shader.go:139-163
@@ -559,6 +743,46 @@ __device__ float4 light_samples(const DevLight& L, const vgpu_material& lm, cons
 
 // ------------------------------------------------------------- kernels
 
+// Sampler.WarpSample (builtin/filter/filter.go:40-85): linear search of the marginal CDF, then of the
+// chosen row's conditional CDF, with the reference's lerps (including its i == 0 form).  A row search
+// that finds nothing returns (0, 0).
+__device__ void warp_sample(const RenderView& rv, double r0, double r1, double& u, double& v) {
+    const int n = rv.filt_n;
+    const double w = rv.filt_w;
+    const double* cdfV = rv.filt;
+    u = 0.0;
+    v = 0.0;
+    int uI = -1;
+    for (int i = 0; i < n; i++) {
+        uI = i;
+        if (r0 < cdfV[i]) {
+            if (i == 0) {
+                const double du = r0 / cdfV[i];
+                u = (-w / 2) + (du / w);
+            } else {
+                const double du = (r0 - cdfV[i - 1]) / (cdfV[i] - cdfV[i - 1]);
+                u = (-w / 2) + w * ((double)i + du) / (double)(n - 1);
+            }
+            break;
+        }
+    }
+    const double* row = rv.filt + n + (size_t)uI * n;
+    for (int i = 0; i < n; i++) {
+        if (r1 < row[i]) {
+            if (i == 0) {
+                const double dv = r1 / row[i];
+                v = (-w / 2) + (dv / w);
+            } else {
+                const double dv = (r1 - row[i - 1]) / (row[i] - row[i - 1]);
+                v = (-w / 2) + w * ((double)i + dv) / (double)(n - 1);
+            }
+            return;
+        }
+    }
+    u = 0.0;
+    v = 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter,
                                                 float4* __restrict__ rays, float4* __restrict__ pix) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -575,8 +799,15 @@ __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* _
     }
     uint64_t b = (((uint64_t)x << m) | (uint64_t)y) ^ delta;
     for (uint32_t c = 0; b != 0; b >>= 1, c++) if (b & 1) index ^= rv.raster_inv[c];
-    const double rx = (double)vdc_bits(index, 0) / (double)(1ull << (52 - m));
-    const double ry = (double)sobol_bits(index, 0) / (double)(1ull << (52 - m));
+    double rx = (double)vdc_bits(index, 0) / (double)(1ull << (52 - m));
+    double ry = (double)sobol_bits(index, 0) / (double)(1ull << (52 - m));
+    if (rv.filt_n > 0) {   // core/render.go:100-108
+        const double pixu = rx - floor(rx), pixv = ry - floor(ry);
+        double u, v;
+        warp_sample(rv, pixu, pixv, u, v);
+        rx = floor(rx) + 0.5 + u;
+        ry = floor(ry) + 0.5 + v;
+    }
     const double lambda = (720 - 450) * vdc(iter, s[3]) + 450;
     const float Sx = (float)(-1.0 + 2.0 * (rx / (double)rv.width));
     const float Sy = -(float)(-1.0 + 2.0 * (ry / (double)rv.height));
@@ -595,71 +826,126 @@ __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* _
     pix[p] = make_float4(D.x, D.y, D.z, (float)lambda);
 }
 
-__global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter,
-                                               const int4* __restrict__ hits32, const float4* __restrict__ pix,
-                                               float4* __restrict__ slot_rays, float4* __restrict__ slot_rgb,
-                                               float4* __restrict__ groups, float4* __restrict__ pixshade) {
+// builtin/shader/bsdf/specular.go: perfect mirror, reflect(omegaR, (0,0,1)) = 2 (N . omegaR) N - omegaR
+__device__ __forceinline__ V3 reflect_z(V3 wr) {
+    const V3 N = v3(0.0f, 0.0f, 1.0f);
+    return sub(scale(2.0f * dot(N, wr), N), wr);
+}
+struct Specular {
+    float lambda; V3 wr; Fresnel fres; Frame f;
+    __device__ V3 sample() const { return basis_expand(f.U, f.V, f.N, normalize(reflect_z(wr))); }
+    __device__ double pdf(V3 o_) const {
+        const V3 o = basis_project(f.U, f.V, f.N, o_);
+        return dot(o, reflect_z(wr)) < 0.9999f ? 0.0 : 1.0;
+    }
+    __device__ Spec eval(V3 o_) const {
+        const V3 o = basis_project(f.U, f.V, f.N, o_);
+        Spec rho;
+        rho.c[0] = rho.c[1] = rho.c[2] = rho.c[3] = 0.0f;
+        rho.lambda = 0.0f;
+        if (dot(o, reflect_z(wr)) < 0.9999f) return rho;
+        rho = spec_from_rgb(lambda, fres.kr(wr.z));
+        spec_scale(rho, dotabs(o, v3(0.0f, 0.0f, 1.0f)));
+        return rho;
+    }
+};
+
+__device__ __forceinline__ void ray_off(float4* rays, int p) {
+    rays[2 * p] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    rays[2 * p + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+}
+
+// One recursion level of core.Trace + ShaderStd.Eval for every pixel whose level-`level` ray hit.
+__global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter, int level,
+                                               const float4* __restrict__ rays_in, const int4* __restrict__ hits32,
+                                               const float4* __restrict__ pix, float4* __restrict__ slot_rays,
+                                               float4* __restrict__ slot_rgb, float4* __restrict__ groups,
+                                               float4* __restrict__ pixshade, float4* __restrict__ rays_out,
+                                               LevelRec* __restrict__ recs) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= rv.width * rv.height) return;
     const int SGS = 2 * rv.max_half;   // slots per group
     SlotOut so{slot_rays + (size_t)p * rv.slots * 2, slot_rgb + (size_t)p * rv.slots};
     float4* grp = groups + (size_t)p * rv.groups;
     float4* ps = pixshade + (size_t)p * 4;
+    LevelRec* rec = recs + (size_t)p * (kMaxLevel + 1) + level;
     const int4 h0 = hits32[2 * p], h1 = hits32[2 * p + 1];
     const int prim = h0.x, g = h0.y;
-    if (prim < 0) {
+    int mat = -1;
+    DevGeom geo = {0, 0, 1, 0.0f};
+    DevGeomShade gs = {-1, -1, 0, 0};
+    if (prim >= 0) {
+        geo = rv.sv.geoms[g];
+        gs = rv.gshade[g];
+        const int sidx = gs.sidx_base >= 0 ? (int)rv.shaderidx[gs.sidx_base + prim] : 0;
+        mat = gs.nshaders > 0 ? rv.shaders[gs.shader_base + sidx] : -1;
+    }
+    if (mat < 0) {   // a miss, or a geom without a shader: core.Trace reports no hit (trace.go:62-64)
         for (int s = 0; s < rv.slots; s++) slot_off(so, s);
         for (int q = 0; q < rv.groups; q++) grp[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         ps[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rec->a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rec->s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (rays_out) ray_off(rays_out, p);
         return;
     }
-    const float U = __int_as_float(h0.w), V = __int_as_float(h1.x), W = __int_as_float(h1.y);
-    const DevGeom geo = rv.sv.geoms[g];
-    const DevGeomShade gs = rv.gshade[g];
-    const float4* tp = rv.sv.tris + ((size_t)geo.tri_base + prim) * 3;
-    const V3 a = from4(tp[0]), b = from4(tp[1]), c = from4(tp[2]);
-    // hit attributes (builtin/geom/polymesh/trace.go:292-513)
-    const float xs = fabs32(U * a.x) + fabs32(V * b.x) + fabs32(W * c.x);
-    const float ys = fabs32(U * a.y) + fabs32(V * b.y) + fabs32(W * c.y);
-    const float zs = fabs32(U * a.z) + fabs32(V * b.z) + fabs32(W * c.z);
-    const float e00 = b.x - a.x, e01 = b.y - a.y, e02 = b.z - a.z;
-    const float e10 = c.x - a.x, e11 = c.y - a.y, e12 = c.z - a.z;
-    V3 Ng = normalize(v3(e01 * e12 - e02 * e11, e02 * e10 - e00 * e12, e00 * e11 - e01 * e10));
-    V3 N = Ng;
-    if (gs.nrm_base >= 0) {
-        const float4* np = rv.normals + gs.nrm_base + (size_t)prim * 3;
-        const V3 n0 = from4(np[0]), n1 = from4(np[1]), n2 = from4(np[2]);
-        N = normalize(v3(U * n0.x + V * n1.x + W * n2.x, U * n0.y + V * n1.y + W * n2.y, U * n0.z + V * n1.z + W * n2.z));
-    }
-    const float g7 = (7.0f * 5.9604645e-08f) / (1.0f - 7.0f * 5.9604645e-08f);   // Gamma(7), math/ferror.go:22-24
-    const float dd = g7 * xs * fabs32(Ng.x) + g7 * ys * fabs32(Ng.y) + g7 * zs * fabs32(Ng.z);
+    const float4 ra = rays_in[2 * p], rb = rays_in[2 * p + 1];
+    const V3 Ro = v3(ra.x, ra.y, ra.z), Rd = v3(rb.x, rb.y, rb.z);
+    const float thit = __int_as_float(h0.z);
     SC sc;
-    sc.Poffset = scale(dd, Ng);
-    sc.P = v3(U * a.x + V * b.x + W * c.x, U * a.y + V * b.y + W * c.y, U * a.z + V * b.z + W * c.z);
-    V3 axisu;
-    {
-        const float dx = 1 * Ng.x + 0 * Ng.y + 0 * Ng.z;
-        axisu = v3(1.0f - Ng.x * dx, 0.0f - Ng.y * dx, 0.0f - Ng.z * dx);
-        if (len2(axisu) < 0.1f || fabs32(dot(axisu, Ng)) > 0.3f) {
-            const float dz = 0 * Ng.x + 0 * Ng.y + 1 * Ng.z;
-            axisu = v3(0.0f - Ng.x * dz, 0.0f - Ng.y * dz, 1.0f - Ng.z * dz);
+    V3 N, Ng, dPdu, dPdv;
+    if (geo.nkeys == 0) {
+        // analytic sphere (builtin/geom/sphere/trace.go:13-52)
+        const float4 sp = rv.sv.spheres[geo.tri_base];
+        sc.P = mad(Ro, Rd, thit);
+        N = normalize(sub(sc.P, v3(sp.x, sp.y, sp.z)));
+        Ng = N;
+        sc.Poffset = scale(0.001f, N);
+        dPdu = v3(1.0f, 0.0f, 0.0f);
+        dPdv = v3(0.0f, 0.0f, 1.0f);
+    } else {
+        const float U = __int_as_float(h0.w), V = __int_as_float(h1.x), W = __int_as_float(h1.y);
+        const float4* tp = rv.sv.tris + ((size_t)geo.tri_base + prim) * 3;
+        const V3 a = from4(tp[0]), b = from4(tp[1]), c = from4(tp[2]);
+        // hit attributes (builtin/geom/polymesh/trace.go:292-513)
+        const float xs = fabs32(U * a.x) + fabs32(V * b.x) + fabs32(W * c.x);
+        const float ys = fabs32(U * a.y) + fabs32(V * b.y) + fabs32(W * c.y);
+        const float zs = fabs32(U * a.z) + fabs32(V * b.z) + fabs32(W * c.z);
+        const float e00 = b.x - a.x, e01 = b.y - a.y, e02 = b.z - a.z;
+        const float e10 = c.x - a.x, e11 = c.y - a.y, e12 = c.z - a.z;
+        Ng = normalize(v3(e01 * e12 - e02 * e11, e02 * e10 - e00 * e12, e00 * e11 - e01 * e10));
+        N = Ng;
+        if (gs.nrm_base >= 0) {
+            const float4* np = rv.normals + gs.nrm_base + (size_t)prim * 3;
+            const V3 n0 = from4(np[0]), n1 = from4(np[1]), n2 = from4(np[2]);
+            N = normalize(v3(U * n0.x + V * n1.x + W * n2.x, U * n0.y + V * n1.y + W * n2.y, U * n0.z + V * n1.z + W * n2.z));
         }
+        const float g7 = (7.0f * 5.9604645e-08f) / (1.0f - 7.0f * 5.9604645e-08f);   // Gamma(7), math/ferror.go:22-24
+        const float dd = g7 * xs * fabs32(Ng.x) + g7 * ys * fabs32(Ng.y) + g7 * zs * fabs32(Ng.z);
+        sc.Poffset = scale(dd, Ng);
+        sc.P = v3(U * a.x + V * b.x + W * c.x, U * a.y + V * b.y + W * c.y, U * a.z + V * b.z + W * c.z);
+        V3 axisu;
+        {
+            const float dx = 1 * Ng.x + 0 * Ng.y + 0 * Ng.z;
+            axisu = v3(1.0f - Ng.x * dx, 0.0f - Ng.y * dx, 0.0f - Ng.z * dx);
+            if (len2(axisu) < 0.1f || fabs32(dot(axisu, Ng)) > 0.3f) {
+                const float dz = 0 * Ng.x + 0 * Ng.y + 1 * Ng.z;
+                axisu = v3(0.0f - Ng.x * dz, 0.0f - Ng.y * dz, 1.0f - Ng.z * dz);
+            }
+        }
+        dPdu = normalize(axisu);
+        dPdv = cross(Ng, dPdu);
     }
-    V3 dPdu = normalize(axisu);
-    V3 dPdv = cross(Ng, dPdu);
     // ApplyTransform with identity matrices re-normalises (core/shader.go:129-135)
     sc.N = normalize(N);
     sc.Ng = normalize(Ng);
     dPdu = normalize(dPdu);
     dPdv = normalize(dPdv);
-    const float4 pv = pix[p];
-    const V3 Rd = v3(pv.x, pv.y, pv.z);
-    sc.lambda = pv.w;
+    sc.lambda = pix[p].w;
     sc.I = iter;
     sc.scr0 = scr[(size_t)p * 6 + 4];
     sc.scr1 = scr[(size_t)p * 6 + 5];
-    const int sidx = gs.sidx_base >= 0 ? (int)rv.shaderidx[gs.sidx_base + prim] : 0;
-    const vgpu_material mt = rv.materials[rv.shaders[gs.shader_base + sidx]];
+    const vgpu_material mt = rv.materials[mat];
     // ShaderStd.Eval (std.go:99-285)
     V3 Vv = cross(sc.N, dPdu);
     if (len2(Vv) < 0.1f) Vv = cross(sc.N, dPdv);
@@ -674,12 +960,14 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     sw /= total;
     const float srough = mt.has_spec1_roughness ? mt.spec1_roughness : 0.5f;
     const float ior = mt.has_ior ? mt.ior : 1.7f;
-    const bool diff_on = dw > 0.0f, spec_on = sw > 0.0f && srough > 0.0f;
-    const OrenNayar on{sc.lambda, basis_project(Uu, Vv, sc.N, neg(Rd)), drough * drough, fr};
-    const GGX gx{sc.lambda, basis_project(Uu, Vv, sc.N, neg(Rd)), srough * srough, ior, fr};
+    const Fresnel fres = make_fresnel(mt, ior);
+    const bool diff_on = dw > 0.0f, spec_on = sw > 0.0f && srough > 0.0f, mirror = sw > 0.0f && srough == 0.0f;
+    const V3 wr = basis_project(Uu, Vv, sc.N, neg(Rd));
+    const OrenNayar on{sc.lambda, wr, drough * drough, fr};
+    const GGX gx{sc.lambda, wr, srough * srough, fres, fr};
     for (int l = 0; l < rv.nlights; l++) {
         const DevLight L = rv.lights[l];
-        const int NS = 1 << L.samples;
+        const int NS = level == 0 ? 1 << L.samples : 1;   // NextLight: one sample per light below level 0 (shader.go:186-191)
         const bool skip = L.geom == g;   // LightsPrepare drops the light owning the hit geom (scene.go:100-116)
         const vgpu_material lm = rv.materials[L.material];
         const int b0 = (2 * l) * SGS, b1 = (2 * l + 1) * SGS;
@@ -695,49 +983,111 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     ps[2] = mt.has_spec1_colour ? make_float4(mt.spec1[0], mt.spec1[1], mt.spec1[2], 0.0f)
                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ps[3] = make_float4(diff_on ? 1.0f : 0.0f, spec_on ? 1.0f : 0.0f, 0.0f, 0.0f);
+    // Perfect mirror (std.go:209-261): one reflected ray at level + 1.  A level-4 hit shades to zero
+    // (std.go:97-99), so a level-3 mirror adds +0 whatever its ray hits, as an untraced one does.
+    float kind = 1.0f;
+    float4 srec = make_float4(0.0f, 0.0f, 0.0f, sw);
+    bool traced = false;
+    if (mirror) {
+        kind = 3.0f;
+        const Specular spc{sc.lambda, wr, fres, fr};
+        const V3 o = spc.sample();
+        const double pdf = spc.pdf(o);
+        if (dot(o, sc.Ng) > 0.0f && level < kMaxLevel && rays_out) {
+            Spec rho = spc.eval(o);
+            spec_scale(rho, 1.0f / (float)pdf);
+            RGB f = spec_to_rgb(rho);
+            const RGB scol = mt.has_spec1_colour ? RGB{mt.spec1[0], mt.spec1[1], mt.spec1[2]} : rgb0();
+            rgb_mul(f, scol);
+            srec = make_float4(f.r, f.g, f.b, sw);
+            const V3 po = offset_p(sc, 1);
+            rays_out[2 * p] = make_float4(po.x, po.y, po.z, __builtin_inff());
+            rays_out[2 * p + 1] = make_float4(o.x, o.y, o.z, 0.0f);
+            kind = 2.0f;
+            traced = true;
+        }
+    }
+    if (rays_out && !traced) ray_off(rays_out, p);
+    rec->a = make_float4(0.0f, 0.0f, 0.0f, kind);
+    rec->s = srec;
 }
 
-__global__ __launch_bounds__(256) void k_resolve(RenderView rv, uint32_t iter, const float4* __restrict__ slot_rgb,
+// Sums one level's unoccluded slot colours in the reference's order and applies the per-light / per-lobe
+// scales: a = (0 + emission) + diffuse, and for a rough specular lobe s = its direct lighting.
+__global__ __launch_bounds__(256) void k_resolve(RenderView rv, int level, const float4* __restrict__ slot_rgb,
                                                  const int4* __restrict__ slot_hits, const float4* __restrict__ groups,
-                                                 const float4* __restrict__ pixshade, float* __restrict__ film,
-                                                 float* __restrict__ sample_out) {
+                                                 const float4* __restrict__ pixshade, LevelRec* __restrict__ recs) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= rv.width * rv.height) return;
     const float4* ps = pixshade + (size_t)p * 4;
     const float4 e = ps[0];
-    RGB contrib = rgb0();
-    if (e.w != 0.0f) {
-        const int SGS = 2 * rv.max_half;
-        const float4 dc = ps[1], spc = ps[2], fl = ps[3];
-        const float4* rgb = slot_rgb + (size_t)p * rv.slots;
-        const int4* hit = slot_hits + (size_t)p * rv.slots;
-        const float4* grp = groups + (size_t)p * rv.groups;
-        RGB diff = rgb0(), spec = rgb0();
-        for (int lobe = 0; lobe < 2; lobe++) {
-            if ((lobe == 0 ? fl.x : fl.y) == 0.0f) continue;
-            RGB& acc = lobe == 0 ? diff : spec;
-            const RGB colour = lobe == 0 ? RGB{dc.x, dc.y, dc.z} : RGB{spc.x, spc.y, spc.z};
-            for (int l = 0; l < rv.nlights; l++) {
-                const float4 gr = grp[2 * l + lobe];
-                RGB col = rgb0();
-                if (gr.y != 0.0f) {
-                    const int base = (2 * l + lobe) * SGS;
-                    for (int s = 0; s < SGS; s++) {
-                        const float4 c = rgb[base + s];
-                        if (c.w != 0.0f && hit[base + s].x < 0) rgb_add(col, RGB{c.x, c.y, c.z});
-                    }
-                    if (gr.y == 1.0f) rgb_scale(col, gr.x);
+    if (e.w == 0.0f) return;
+    LevelRec* rec = recs + (size_t)p * (kMaxLevel + 1) + level;
+    const int SGS = 2 * rv.max_half;
+    const float4 dc = ps[1], spc = ps[2], fl = ps[3];
+    const float4* rgb = slot_rgb + (size_t)p * rv.slots;
+    const int4* hit = slot_hits + (size_t)p * rv.slots;
+    const float4* grp = groups + (size_t)p * rv.groups;
+    RGB diff = rgb0(), spec = rgb0();
+    for (int lobe = 0; lobe < 2; lobe++) {
+        if ((lobe == 0 ? fl.x : fl.y) == 0.0f) continue;
+        RGB& acc = lobe == 0 ? diff : spec;
+        const RGB colour = lobe == 0 ? RGB{dc.x, dc.y, dc.z} : RGB{spc.x, spc.y, spc.z};
+        for (int l = 0; l < rv.nlights; l++) {
+            const float4 gr = grp[2 * l + lobe];
+            RGB col = rgb0();
+            if (gr.y != 0.0f) {
+                const int base = (2 * l + lobe) * SGS;
+                for (int s = 0; s < SGS; s++) {
+                    const float4 c = rgb[base + s];
+                    if (c.w != 0.0f && hit[base + s].x < 0) rgb_add(col, RGB{c.x, c.y, c.z});
                 }
-                if (rv.lights[l].geom == -1) continue;
-                rgb_mul(col, colour);
-                rgb_add(acc, col);
+                if (gr.y == 1.0f) rgb_scale(col, gr.x);
             }
-            if (lobe == 0) rgb_scale(diff, dc.w);
+            if (rv.lights[l].geom == -1) continue;
+            rgb_mul(col, colour);
+            rgb_add(acc, col);
         }
-        rgb_add(contrib, RGB{e.x, e.y, e.z});
-        rgb_add(contrib, diff);
-        rgb_add(contrib, spec);
+        if (lobe == 0) rgb_scale(diff, dc.w);
     }
+    RGB a = rgb0();
+    rgb_add(a, RGB{e.x, e.y, e.z});
+    rgb_add(a, diff);
+    const float kind = rec->a.w;
+    rec->a = make_float4(a.r, a.g, a.b, kind);
+    if (kind == 1.0f) rec->s = make_float4(spec.r, spec.g, spec.b, rec->s.w);
+}
+
+// Folds the levels bottom-up (each mirror's colour is f * (the next level's colour), clamped at 0 and
+// NaN, scaled by its weight; std.go:226-258) and updates the film (core/render.go:127-129).
+__global__ __launch_bounds__(256) void k_combine(RenderView rv, uint32_t iter, const LevelRec* __restrict__ recs,
+                                                 float* __restrict__ film, float* __restrict__ sample_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= rv.width * rv.height) return;
+    const LevelRec* r = recs + (size_t)p * (kMaxLevel + 1);
+    RGB out = rgb0();
+    bool next_hit = false;
+    for (int L = rv.max_level; L >= 0; L--) {
+        const float4 a = r[L].a, sv = r[L].s;
+        if (a.w == 0.0f) { next_hit = false; continue; }
+        RGB S = rgb0();
+        if (a.w == 1.0f) {
+            S = RGB{sv.x, sv.y, sv.z};
+        } else if (a.w == 2.0f && next_hit) {
+            RGB col{sv.x, sv.y, sv.z};
+            rgb_mul(col, out);
+            if (col.r < 0 || __builtin_isnan(col.r)) col.r = 0;
+            if (col.g < 0 || __builtin_isnan(col.g)) col.g = 0;
+            if (col.b < 0 || __builtin_isnan(col.b)) col.b = 0;
+            rgb_add(S, col);
+            rgb_scale(S, sv.w / 1.0f);
+        }
+        RGB o{a.x, a.y, a.z};
+        rgb_add(o, S);
+        out = o;
+        next_hit = true;
+    }
+    const RGB contrib = (r[0].a.w != 0.0f) ? out : rgb0();
     float* f = film + (size_t)p * 3;
     f[0] = (f[0] * (float)iter + contrib.r) / (float)(iter + 1);
     f[1] = (f[1] * (float)iter + contrib.g) / (float)(iter + 1);
@@ -758,20 +1108,27 @@ hipError_t launch_camera(const RenderView& rv, const uint64_t* scr, uint32_t ite
     return hipGetLastError();
 }
 
-hipError_t launch_shade(const RenderView& rv, const uint64_t* scr, uint32_t iter, const int4* hits32, const float4* pix,
-                        float4* slot_rays, float4* slot_rgb, float4* groups, float4* pixshade, hipStream_t stream) {
+hipError_t launch_shade(const RenderView& rv, const uint64_t* scr, uint32_t iter, int level, const float4* rays_in,
+                        const int4* hits32, const float4* pix, float4* slot_rays, float4* slot_rgb, float4* groups,
+                        float4* pixshade, float4* rays_out, LevelRec* recs, hipStream_t stream) {
     const int n = rv.width * rv.height;
-    hipLaunchKernelGGL(k_shade, dim3((n + 127) / 128), dim3(128), 0, stream, rv, scr, iter, hits32, pix, slot_rays,
-                       slot_rgb, groups, pixshade);
+    hipLaunchKernelGGL(k_shade, dim3((n + 127) / 128), dim3(128), 0, stream, rv, scr, iter, level, rays_in, hits32, pix,
+                       slot_rays, slot_rgb, groups, pixshade, rays_out, recs);
     return hipGetLastError();
 }
 
-hipError_t launch_resolve(const RenderView& rv, uint32_t iter, const float4* slot_rgb, const int4* slot_hits,
-                          const float4* groups, const float4* pixshade, float* film, float* sample_out,
+hipError_t launch_resolve(const RenderView& rv, int level, const float4* slot_rgb, const int4* slot_hits,
+                          const float4* groups, const float4* pixshade, LevelRec* recs, hipStream_t stream) {
+    const int n = rv.width * rv.height;
+    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, stream, rv, level, slot_rgb, slot_hits, groups,
+                       pixshade, recs);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const RenderView& rv, uint32_t iter, const LevelRec* recs, float* film, float* sample_out,
                           hipStream_t stream) {
     const int n = rv.width * rv.height;
-    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, stream, rv, iter, slot_rgb, slot_hits, groups,
-                       pixshade, film, sample_out);
+    hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, stream, rv, iter, recs, film, sample_out);
     return hipGetLastError();
 }
 

@@ -18,12 +18,19 @@ struct alignas(16) DevGeomShade {
     int32_t nshaders;
 };
 
-struct alignas(16) DevLight {  // builtin/light/triangle.go:18-28
-    float4 P0, P1, P2;         // xyz
+// A light node: Tri (builtin/light/triangle.go:18-28), Disk (disk.go:17-31) or Sphere (sphere.go:16-28).
+struct alignas(16) DevLight {
+    float4 P0, P1, P2;         // tri: vertices; disk: P (w = radius), Up, LookAt; sphere: centre (w = radius)
     int32_t material;
     int32_t samples;           // NumSamples = 1 << samples
-    int32_t geom;              // post-sort geom slot of the light's own mesh
-    int32_t pad;
+    int32_t geom;              // post-sort geom slot of the light's own geometry
+    int32_t type;              // VGPU_LIGHT_TRI / _DISK / _SPHERE
+};
+
+// Per pixel and recursion level, what the level's shading left for the final combine.
+struct alignas(16) LevelRec {
+    float4 a;                  // xyz: (0 + emission) + diffuse; w: kind (0 miss, 1 rough/no spec, 2 mirror ray traced, 3 mirror not traced)
+    float4 s;                  // kind 1: specular direct lighting; kind 2: mirror factor rho.ToRGB() * Spec1Colour; w: spec weight
 };
 
 struct RenderView {
@@ -43,16 +50,24 @@ struct RenderView {
     float tan_theta_focal, aspect, focal;
     uint64_t raster_fwd[52];       // RasterXY enumeration tables for m = 12
     uint64_t raster_inv[24];
+    const double* filt;            // pixel filter sampler: cdfV (filt_n), then cdfVU (filt_n x filt_n); null: box
+    int32_t filt_n;
+    int32_t max_level;             // deepest recursion level shaded (0 without mirrors, else 3)
+    double filt_w;
 };
 
-// Launch wrappers (all enqueue on `stream`).
+constexpr int kMaxLevel = 3;       // ShaderStd.Eval returns at once above level 3 (std.go:97-99)
+
+// Launch wrappers (all enqueue on `stream`).  Levels: rays_in holds the level's rays (P, tmax; D,
+// time), rays_out receives the next level's mirror rays (inactive pixels get tmax -1).
 hipError_t launch_camera(const RenderView& rv, const uint64_t* scrambles, uint32_t iter, float4* rays, float4* pix,
                          hipStream_t stream);
-hipError_t launch_shade(const RenderView& rv, const uint64_t* scrambles, uint32_t iter, const int4* hits32,
-                        const float4* pix, float4* slot_rays, float4* slot_rgb, float4* groups, float4* pixshade,
-                        hipStream_t stream);
-hipError_t launch_resolve(const RenderView& rv, uint32_t iter, const float4* slot_rgb, const int4* slot_hits,
-                          const float4* groups, const float4* pixshade, float* film, float* sample_out,
+hipError_t launch_shade(const RenderView& rv, const uint64_t* scrambles, uint32_t iter, int level, const float4* rays_in,
+                        const int4* hits32, const float4* pix, float4* slot_rays, float4* slot_rgb, float4* groups,
+                        float4* pixshade, float4* rays_out, LevelRec* recs, hipStream_t stream);
+hipError_t launch_resolve(const RenderView& rv, int level, const float4* slot_rgb, const int4* slot_hits,
+                          const float4* groups, const float4* pixshade, LevelRec* recs, hipStream_t stream);
+hipError_t launch_combine(const RenderView& rv, uint32_t iter, const LevelRec* recs, float* film, float* sample_out,
                           hipStream_t stream);
 
 }  // namespace vgpu

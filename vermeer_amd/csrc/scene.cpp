@@ -340,6 +340,28 @@ int vgpu_scene_add_shaded_mesh(vgpu_scene* s, const float* verts, uint32_t nvert
     return VGPU_OK;
 }
 
+int vgpu_scene_add_sphere(vgpu_scene* s, const float* centre, float radius, int32_t material, int32_t* geom_out) {
+    if (!s || !centre) return fail(VGPU_ERR_ARG, "NULL argument");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (!(radius > 0.0f) || !std::isfinite(radius)) return fail(VGPU_ERR_ARG, "sphere radius must be > 0");
+    for (int k = 0; k < 3; k++) if (!std::isfinite(centre[k])) return fail(VGPU_ERR_ARG, "sphere centre must be finite");
+    if (material < 0) return fail(VGPU_ERR_ARG, "sphere material must be >= 0");
+    MeshHost m;
+    m.sphere = true;
+    m.built = true;
+    for (int k = 0; k < 3; k++) m.sphere_c[k] = centre[k];
+    m.sphere_r = radius;
+    // Sphere.Bounds (builtin/geom/sphere/sphere.go:68-77)
+    for (int k = 0; k < 3; k++) {
+        m.bounds[k] = centre[k] - radius;
+        m.bounds[3 + k] = centre[k] + radius;
+    }
+    m.shaders.assign(1, material);
+    s->meshes.push_back(std::move(m));
+    if (geom_out) *geom_out = (int32_t)s->meshes.size() - 1;
+    return VGPU_OK;
+}
+
 int vgpu_scene_add_mesh_prebuilt(vgpu_scene* s, const float* verts, uint32_t nverts, const uint32_t* idxp_sorted,
                                  uint32_t ntris, const vgpu_qbvh_node* nodes, uint32_t nnodes, const float* bounds,
                                  float ray_bias, int32_t* geom_out) {
@@ -444,7 +466,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
 
     // Flatten into the device layout.
     std::vector<vgpu::DevNode> dnodes;
-    std::vector<float4> dtris, dmnodes, dmtris;
+    std::vector<float4> dtris, dmnodes, dmtris, dspheres;
     std::vector<vgpu::DevGeom> dgeoms(ng);
     if (!s->top_motion) {
         for (auto& nd : s->top_nodes) { vgpu::DevNode d; to_dev(nd, d); dnodes.push_back(d); }
@@ -457,7 +479,12 @@ int vgpu_scene_commit(vgpu_scene* s) {
         vgpu::DevGeom& g = dgeoms[slot];
         g.nkeys = (int32_t)m.nkeys;
         g.ray_bias = m.ray_bias;
-        if (m.nkeys == 1) {
+        if (m.sphere) {   // nkeys 0 marks an analytic sphere; tri_base indexes the sphere table
+            g.nkeys = 0;
+            g.node_base = -1;
+            g.tri_base = (int32_t)dspheres.size();
+            dspheres.push_back(make_float4(m.sphere_c[0], m.sphere_c[1], m.sphere_c[2], m.sphere_r));
+        } else if (m.nkeys == 1) {
             g.node_base = (int32_t)dnodes.size();
             for (auto& nd : m.nodes) { vgpu::DevNode d; to_dev(nd, d); dnodes.push_back(d); }
             g.tri_base = (int32_t)(dtris.size() / 3);
@@ -490,6 +517,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
     if ((rc = upload(&s->d_mnodes, dmnodes))) return rc;
     if ((rc = upload(&s->d_mtris, dmtris))) return rc;
     if ((rc = upload(&s->d_geoms, dgeoms))) return rc;
+    if ((rc = upload(&s->d_spheres, dspheres))) return rc;
     // Shading data per slot: per-triangle vertex normals (the mesh's normalidx equals its idxp,
     // init.go:85-91), per-triangle shader index, shader tables.
     {
@@ -530,6 +558,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
     s->view.mnodes = s->d_mnodes;
     s->view.mtris = s->d_mtris;
     s->view.geoms = s->d_geoms;
+    s->view.spheres = s->d_spheres;
     s->view.top_motion = s->top_motion ? 1 : 0;
     s->view.top_keys = s->top_keys;
     s->view.nan_free_boxes = 1;
@@ -564,6 +593,7 @@ int vgpu_scene_mesh_accel(const vgpu_scene* s, int32_t slot, vgpu_qbvh_node* nod
     if (!s->committed) return fail(VGPU_ERR_STATE, "scene not committed");
     if (slot < 0 || slot >= (int32_t)s->meshes.size()) return fail(VGPU_ERR_ARG, "slot out of range");
     const MeshHost& m = s->meshes[s->order[slot]];
+    if (m.sphere) return fail(VGPU_ERR_ARG, "slot %d is a sphere", slot);
     if (m.nkeys != 1) return fail(VGPU_ERR_ARG, "slot %d is a motion mesh", slot);
     if (nodes) std::memcpy(nodes, m.nodes.data(), m.nodes.size() * sizeof(vgpu_qbvh_node));
     if (idxp_sorted) std::memcpy(idxp_sorted, m.idxp_sorted.data(), m.idxp_sorted.size() * sizeof(uint32_t));

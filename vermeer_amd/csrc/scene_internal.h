@@ -70,6 +70,8 @@ struct MeshHost {
     std::vector<uint8_t> shaderidx;    // per triangle (insertion order; reordered at build)
     std::vector<uint8_t> shaderidx_sorted;
     std::vector<int32_t> shaders;      // shader table: material ids
+    bool sphere = false;               // analytic sphere geom (vgpu_scene_add_sphere): sphere_c, sphere_r
+    float sphere_c[3] = {0, 0, 0}, sphere_r = 0;
 
     const float* vert(uint32_t key, uint32_t i) const { return &verts[((size_t)key * nverts + i) * 3]; }
 
@@ -103,6 +105,7 @@ struct vgpu_scene {
     float4* d_mnodes = nullptr;
     float4* d_mtris = nullptr;
     vgpu::DevGeom* d_geoms = nullptr;
+    float4* d_spheres = nullptr;
     int32_t* d_err = nullptr;
     float* d_spT = nullptr;
     int32_t* d_spN = nullptr;
@@ -128,7 +131,7 @@ struct vgpu_scene {
     ~vgpu_scene() {
         int prev = 0;
         if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(device);
-        void* ptrs[] = {d_nodes, d_tris, d_mnodes, d_mtris, d_geoms, d_err, d_spT, d_spN, d_stage, d_normals, d_shaderidx,
+        void* ptrs[] = {d_nodes, d_tris, d_mnodes, d_mtris, d_geoms, d_spheres, d_err, d_spT, d_spN, d_stage, d_normals, d_shaderidx,
                         d_shaders, d_gshade, sort.keys, sort.perm, sort.rays, sort.hist, sort.bsum};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);

@@ -10,6 +10,7 @@
 //   qbvh/intersect.go:91-246            Trace
 //   qbvh/motionintersect.go:22-127      TraceMotion
 //   builtin/scene/scene.go:30-97        Scene.Trace / TraceElems
+//   builtin/geom/sphere/trace.go:13-103  Sphere.Trace (a SphereLight's analytic sphere)
 //   builtin/geom/polymesh/trace.go:108-194, :520-669   TraceElems / TraceMotionElems
 //
 // The traversal stack (the reference's RenderTask.Traversal.Stack, shared by
@@ -258,6 +259,54 @@ __device__ __forceinline__ void motion_keys(float rtime, int nkeys, int& key, in
     key2 = (int)ceilf(k);
 }
 
+// Sphere.Trace (builtin/geom/sphere/trace.go:13-52): the closed-form quadratic (solveQuadratic,
+// raySphereIntersect :54-103), accepted when t < Tclosest, with no ray bias.
+__device__ bool sphere_geom_hit(const float* P, const float4& d, const float4& sp, float tclosest, float& t) {
+    const float Lx = P[0] - sp.x, Ly = P[1] - sp.y, Lz = P[2] - sp.z;
+    const float a = d.x * d.x + d.y * d.y + d.z * d.z;
+    const float b = 2 * (d.x * Lx + d.y * Ly + d.z * Lz);
+    const float c = (Lx * Lx + Ly * Ly + Lz * Lz) - sp.w * sp.w;
+    float x0, x1;
+    const float discr = b * b - 4 * a * c;
+    if (discr < 0) return false;
+    if (discr == 0) {
+        x1 = -0.5f * b / a;
+        x0 = x1;
+    } else {
+        const float q = b > 0 ? -0.5f * (b + sqrtf(discr)) : -0.5f * (b - sqrtf(discr));
+        x0 = q / a;
+        x1 = c / q;
+    }
+    if (x0 > x1) { const float tt = x0; x0 = x1; x1 = tt; }
+    if (x0 < 0) {
+        x0 = x1;
+        if (x0 < 0) return false;
+    }
+    if (!(x0 < tclosest)) return false;
+    t = x0;
+    return true;
+}
+
+// Scene.TraceElems over geoms [gcur, gend) of a top-level leaf (scene.go:61-78): analytic spheres
+// (DevGeom nkeys 0) are tested here, in order, up to the first mesh.  Returns 1 with g = that mesh,
+// 2 when a shadow ray hit a sphere (the traversal ends, qbvh/intersect.go:232-236), 0 at the leaf's end.
+template <bool SHADOW>
+__device__ __forceinline__ int next_geom(const SceneView& sv, const float4* __restrict__ rays, uint64_t ri, RayS& r,
+                                         int& gcur, int gend, DevGeom& g, int& best_geom, int& best_prim, float& bu,
+                                         float& bv, float& bw) {
+    for (; gcur < gend; gcur++) {
+        g = sv.geoms[gcur];
+        if (g.nkeys != 0) return 1;
+        float t;
+        if (sphere_geom_hit(r.P, rays[2 * ri + 1], sv.spheres[g.tri_base], r.tclosest, t)) {
+            r.tclosest = t;
+            best_geom = gcur; best_prim = 0; bu = 0.0f; bv = 0.0f; bw = 0.0f;
+            if (SHADOW) return 2;
+        }
+    }
+    return 0;
+}
+
 // EXT: hits are 32-byte records {prim, geom, t, U, V, W, 0, 0} for the shading kernels.
 template <int BLOCK, int LDS, bool SHADOW, bool MOTION, bool COUNTERS, bool EXT = false>
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __restrict__ rays, uint64_t n,
@@ -314,8 +363,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
             if (in_mesh && sp == mesh_base) {
                 // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
                 gcur++;
-                if (gcur < gend) {
-                    g = sv.geoms[gcur];
+                const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
+                if (ng == 2) break;
+                if (ng == 1) {
                     if (MOTION && g.nkeys > 1) motion_keys(r.time, g.nkeys, mkey, mkey2, mtime);
                     st.store(mesh_base, r.tclosest, 0);
                     sp = mesh_base + 1;
@@ -397,11 +447,14 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
                     mesh_base = top_motion ? sp : sp + 1;
                     if (mesh_base >= VGPU_MAX_STACK) { overflow = true; break; }
                     gcur = base; gend = base + count;
-                    g = sv.geoms[gcur];
-                    if (MOTION && g.nkeys > 1) motion_keys(r.time, g.nkeys, mkey, mkey2, mtime);
-                    st.store(mesh_base, r.tclosest, 0);
-                    sp = mesh_base + 1;
-                    in_mesh = true;
+                    const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
+                    if (ng == 2) break;
+                    if (ng == 1) {
+                        if (MOTION && g.nkeys > 1) motion_keys(r.time, g.nkeys, mkey, mkey2, mtime);
+                        st.store(mesh_base, r.tclosest, 0);
+                        sp = mesh_base + 1;
+                        in_mesh = true;
+                    }
                 }
             }
         }
@@ -530,8 +583,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 if (in_mesh && sp == mesh_base) {
                     // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
                     gcur++;
-                    if (gcur < gend) {
-                        g = sv.geoms[gcur];
+                    const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
+                    if (ng == 2) { done = true; break; }
+                    if (ng == 1) {
                         st.store(mesh_base, r.tclosest, 0);
                         sp = mesh_base + 1;
                         break;
@@ -583,10 +637,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                         mesh_base = sp + 1;
                         if (mesh_base >= VGPU_MAX_STACK) { overflow = true; done = true; break; }
                         gcur = base; gend = base + count;
-                        g = sv.geoms[gcur];
-                        st.store(mesh_base, r.tclosest, 0);
-                        sp = mesh_base + 1;
-                        in_mesh = true;
+                        const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
+                        if (ng == 2) { done = true; break; }
+                        if (ng == 1) {
+                            st.store(mesh_base, r.tclosest, 0);
+                            sp = mesh_base + 1;
+                            in_mesh = true;
+                        }
                     }
                 }
             } while (false);

@@ -44,6 +44,7 @@ struct SceneView {
     const float4* mnodes;     // motion nodes (topology + boxes)
     const float4* mtris;      // motion triangles
     const DevGeom* geoms;     // post-sort geom slots
+    const float4* spheres;    // analytic sphere geoms (centre, radius), indexed by DevGeom::tri_base
     int32_t top_motion;       // 1: the top level is a motion QBVH at mnodes[top_mbase]
     int32_t top_keys;         // motion keys of the top level
     int32_t top_mbase;

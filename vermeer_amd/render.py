@@ -1,8 +1,9 @@
 """Renderer: the reference's render pass loop on the GPU.
 
-Mirrors core.Render (core/render.go:140-218) for a scene of shaded meshes
-and triangle lights: create once per frame setup, then one pass() per
-iteration with the 1-based pass number the reference hands to render().
+Mirrors core.Render (core/render.go:140-218) for a scene of shaded meshes,
+analytic spheres and Tri / Disk / Sphere lights: create once per frame
+setup, then one pass() per iteration with the 1-based pass number the
+reference hands to render().
 """
 import ctypes as C
 import math
@@ -10,7 +11,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import CAMERA_DTYPE, MATERIAL_DTYPE, TRI_LIGHT_DTYPE, check, ptr
+from ._lib import CAMERA_DTYPE, LIGHT_DTYPE, MATERIAL_DTYPE, check, ptr
 from .accel import Scene
 
 
@@ -37,6 +38,28 @@ def lookat_camera(frm, to, up, fov_deg, focal, width, height):
     return cam
 
 
+_LIGHT_TYPES = {"tri": _lib.VGPU_LIGHT_TRI, "disk": _lib.VGPU_LIGHT_DISK, "sphere": _lib.VGPU_LIGHT_SPHERE}
+
+
+def light_records(lights):
+    """vgpu_light records from scene-description light dicts: type "tri" (P0, P1, P2), "disk" (P, Up,
+    LookAt, radius) or "sphere" (P, radius); material, samples, geom (insertion index)."""
+    out = np.zeros(len(lights), LIGHT_DTYPE)
+    for i, L in enumerate(lights):
+        kind = L.get("type", "tri")
+        out[i]["type"] = _LIGHT_TYPES[kind]
+        if kind == "tri":
+            out[i]["P0"], out[i]["P1"], out[i]["P2"] = L["P0"], L["P1"], L["P2"]
+        elif kind == "disk":
+            out[i]["P0"], out[i]["P1"], out[i]["P2"] = L["P"], L["Up"], L["LookAt"]
+        else:
+            out[i]["P0"] = L["P"]
+        if kind != "tri":
+            out[i]["radius"] = L["radius"]
+        out[i]["material"], out[i]["samples"], out[i]["geom"] = L["material"], L["samples"], L["geom"]
+    return out
+
+
 class Renderer:
     """GPU render passes over a scene description (see vermeer_amd.scenes.box_spheres_scene)."""
 
@@ -46,6 +69,9 @@ class Renderer:
         self.width, self.height = width, height
         self._keep = []
         for mesh in desc["meshes"]:
+            if mesh.get("sphere"):
+                self.scene.add_sphere(mesh["centre"], mesh["radius"], mesh["material"])
+                continue
             v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(-1, 3)
             ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
             nrm = mesh.get("normals")
@@ -59,10 +85,7 @@ class Renderer:
             self.scene.geoms.append(None)
         self.scene.pre_render()
         mats = np.ascontiguousarray(desc["materials"], MATERIAL_DTYPE)
-        lights = np.zeros(len(desc["lights"]), TRI_LIGHT_DTYPE)
-        for i, L in enumerate(desc["lights"]):
-            for k in ("P0", "P1", "P2", "material", "samples", "geom"):
-                lights[i][k] = L[k]
+        lights = light_records(desc["lights"])
         cam = np.ascontiguousarray(camera, CAMERA_DTYPE)
         h = C.c_void_p()
         check(self._lib.vgpu_render_create(C.byref(h), self.scene._h, ptr(mats), len(mats), ptr(lights), len(lights),
@@ -81,6 +104,11 @@ class Renderer:
             self.close()
         except Exception:
             pass
+
+    def set_filter(self, kind="box", width=0.0, res=0, peak=0.0):
+        """Pixel filter: "box" (none), "gaussian" or "airy"; zeros take the node defaults."""
+        t = {"box": _lib.VGPU_FILTER_BOX, "gaussian": _lib.VGPU_FILTER_GAUSSIAN, "airy": _lib.VGPU_FILTER_AIRY}[kind]
+        check(self._lib.vgpu_render_set_filter(self._h, t, float(width), int(res), float(peak)))
 
     def render_pass(self, scrambles, iteration, want_sample=False):
         """One pass with host arrays (synchronous); updates self.film.  Returns (shadow rays, sample)."""

@@ -251,10 +251,14 @@ def moving_spheres(count=2000, nseg=25, nring=6, extent=20.0, rmin=0.3, rmax=1.0
 from ._lib import MATERIAL_DTYPE  # noqa: E402
 
 FRESNEL_DIELECTRIC = 1
+FRESNEL_METAL = 2
 
 
 def material(diffuse=None, diffuse_strength=None, diffuse_roughness=None, spec=None, spec_strength=None,
-             spec_roughness=None, ior=None, emission=None, emission_strength=None):
+             spec_roughness=None, ior=None, emission=None, emission_strength=None, fresnel="dielectric",
+             fresnel_refl=None, fresnel_edge=None):
+    """ShaderStd parameters (builtin/shader/std.go:25-47); None leaves a parameter unset.  fresnel:
+    "dielectric" (uses ior) or "metal" (fresnel/conductor.go; fresnel_refl / fresnel_edge colours)."""
     m = np.zeros((), MATERIAL_DTYPE)
     for key, val, flag in (("diffuse", diffuse, "has_diffuse_colour"), ("spec1", spec, "has_spec1_colour"),
                            ("emission", emission, "has_emission_colour")):
@@ -267,7 +271,11 @@ def material(diffuse=None, diffuse_strength=None, diffuse_roughness=None, spec=N
         if val is not None:
             m[key] = val
             m["has_" + key] = 1
-    m["fresnel_model"] = FRESNEL_DIELECTRIC
+    m["fresnel_model"] = FRESNEL_METAL if fresnel == "metal" else FRESNEL_DIELECTRIC
+    for key, val in (("spec1_fresnel_refl", fresnel_refl), ("spec1_fresnel_edge", fresnel_edge)):
+        if val is not None:
+            m[key] = val
+            m["has_" + key] = 1
     return m
 
 
@@ -370,6 +378,112 @@ def box_spheres_scene(seed=3, nseg=64, nring=32, nspheres=8, amplitude=0.05):
     # up = -y: the reference's LookAt basis takes v = u x W (camera.go:164), which points down for up = +y
     camera = {"from": (0.0, 1.0, 3.4), "to": (0.0, 1.0, 0.0), "up": (0.0, -1.0, 0.0), "fov": 45.0, "focal": 12.0}
     return {"meshes": meshes, "materials": np.array(mats, MATERIAL_DTYPE), "lights": lights, "camera": camera}
+
+
+def _normalize32(a):
+    """math/vec3_amd64.s:11-42 with the IEEE 1/sqrt estimate, as the renderer's device code computes it."""
+    a = np.asarray(a, np.float32)
+    l2 = (a[1] * a[1] + a[0] * a[0]) + a[2] * a[2]
+    g = np.float32(1.0) / np.sqrt(l2)
+    gg = (g * g) * (l2 * np.float32(0.5))
+    r = (np.float32(1.5) - gg) * g
+    return a * r
+
+
+def _cross32(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], np.float32)
+
+
+def disk_light_mesh(P, up, look_at, radius, segments=20):
+    """The triangle fan a DiskLight inserts (Disk.createMesh, builtin/light/disk.go:224-262) in float32:
+    frame N = normalize(LookAt - P), T = normalize(N x Up), B = N x T (disk.go:84-86); triangle i is
+    (P, P + R cos(a) B + R sin(a) T, the same at a + 2 pi / segments), a accumulated in float32.
+    Returns (verts (3*segments, 3), idx (segments, 3), normals (3*segments, 3))."""
+    P = np.asarray(P, np.float32)
+    N = _normalize32(np.asarray(look_at, np.float32) - P)
+    T = _normalize32(_cross32(N, up))
+    B = _cross32(N, T)
+    pi32 = np.float32(3.14159265358)
+    R = np.float32(radius)
+    dv = (np.float32(2) * pi32) / np.float32(segments)
+    ang = np.float32(0)
+    verts = []
+    for _ in range(segments):
+        c0, s0 = np.float32(math.cos(float(ang))), np.float32(math.sin(float(ang)))
+        a1 = np.float32(ang + dv)
+        c1, s1 = np.float32(math.cos(float(a1))), np.float32(math.sin(float(a1)))
+        verts.append(P)
+        verts.append(P + (B * (R * c0) + T * (R * s0)))
+        verts.append(P + (B * (R * c1) + T * (R * s1)))
+        ang = np.float32(ang + dv)
+    v = np.array(verts, np.float32)
+    idx = np.arange(3 * segments, dtype=np.uint32).reshape(-1, 3)
+    return v, idx, np.tile(N, (3 * segments, 1)).astype(np.float32)
+
+
+def feature_scene(seed=5, nseg=48, nring=24, mirrors=True):
+    """Box room exercising the renderer paths beyond box_spheres_scene: a conductor ("Metal") GGX sphere
+    with Spec1FresnelEdge set, a dielectric GGX sphere, two facing perfect-mirror spheres (recursion to
+    level 3 when mirrors=True), a DiskLight in the ceiling (4 samples, MIS) and a SphereLight (an analytic
+    sphere geom, 2 samples).  Geom insertion order: the box+spheres mesh, the disk's fan, the light sphere."""
+    g = rng(seed)
+    mats = [material(diffuse=(0.7, 0.7, 0.7), diffuse_strength=1.0, diffuse_roughness=0.3)]
+    verts, faces, sidx = [], [], []
+    nv = 0
+    walls = [quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)),
+             quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),
+             quad((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)),
+             quad((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)),
+             quad((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1))]
+    for v, f in walls:
+        n = np.cross(v[f[0, 1]] - v[f[0, 0]], v[f[0, 2]] - v[f[0, 0]])
+        if np.dot(n, np.array([0.0, 1.0, 0.0]) - v[0]) < 0:
+            f = f[:, ::-1]
+        verts.append(v)
+        faces.append(f + nv)
+        sidx.append(np.zeros(len(f), np.int64))
+        nv += len(v)
+    sphere_mats = [
+        material(diffuse=(0.3, 0.3, 0.3), diffuse_strength=0.2, spec=(1.0, 0.9, 0.8), spec_strength=0.8,
+                 spec_roughness=0.3, fresnel="metal", fresnel_refl=(0.95, 0.64, 0.54), fresnel_edge=(0.9, 0.8, 0.6)),
+        material(diffuse=tuple(g.uniform(0.2, 0.8, 3)), diffuse_strength=0.6, diffuse_roughness=0.0,
+                 spec=(1.0, 1.0, 1.0), spec_strength=0.4, spec_roughness=0.15, ior=1.5),
+        material(spec=(1.0, 1.0, 1.0), spec_strength=1.0, spec_roughness=0.0 if mirrors else 0.05, ior=1.5),
+        material(diffuse=(0.2, 0.3, 0.6), diffuse_strength=0.3, spec=(0.9, 0.9, 0.9), spec_strength=0.7,
+                 spec_roughness=0.0 if mirrors else 0.05, fresnel="metal", fresnel_refl=(0.9, 0.9, 0.9)),
+    ]
+    centres = [(-0.5, 0.3, -0.2), (0.5, 0.3, -0.4), (-0.15, 0.75, 0.3), (0.45, 0.75, 0.35)]
+    sv, sf = uv_sphere(1.0, nseg, nring)
+    for k, (c, m) in enumerate(zip(centres, sphere_mats)):
+        r = 0.28
+        v = sv.astype(np.float64) * r + np.asarray(c)
+        mats.append(m)
+        verts.append(v)
+        faces.append(sf.astype(np.int64) + nv)
+        sidx.append(np.full(len(sf), 1 + k, np.int64))
+        nv += len(v)
+    light_mat = len(mats)
+    mats.append(material(emission=(1.0, 0.95, 0.9), emission_strength=8.0, diffuse=(0.0, 0.0, 0.0),
+                         diffuse_strength=1.0))
+    sphere_light_mat = len(mats)
+    mats.append(material(emission=(0.9, 0.95, 1.0), emission_strength=20.0, diffuse=(0.0, 0.0, 0.0),
+                         diffuse_strength=1.0))
+    main = {"verts": np.concatenate(verts).astype(np.float32), "idx": np.concatenate(faces).astype(np.uint32),
+            "shaderidx": np.concatenate(sidx).astype(np.uint8),
+            "shaders": np.arange(1 + len(sphere_mats), dtype=np.int32), "normals": None}
+    dP, dUp, dLook, dR = (0.0, 1.99, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.3
+    dv, di, dn = disk_light_mesh(dP, dUp, dLook, dR, 20)
+    disk = {"verts": dv, "idx": di, "shaderidx": None, "shaders": np.array([light_mat], np.int32), "normals": dn}
+    sph = {"sphere": True, "centre": (-0.6, 1.45, 0.55), "radius": 0.08, "material": sphere_light_mat}
+    lights = [{"type": "disk", "P": dP, "Up": dUp, "LookAt": dLook, "radius": dR, "material": light_mat,
+               "samples": 2, "geom": 1},
+              {"type": "sphere", "P": sph["centre"], "radius": sph["radius"], "material": sphere_light_mat,
+               "samples": 1, "geom": 2}]
+    camera = {"from": (0.0, 1.0, 3.4), "to": (0.0, 0.8, 0.0), "up": (0.0, -1.0, 0.0), "fov": 45.0, "focal": 12.0}
+    return {"meshes": [main, disk, sph], "materials": np.array(mats, MATERIAL_DTYPE), "lights": lights,
+            "camera": camera}
 
 
 def pixel_scrambles(width, height, seed=99):
