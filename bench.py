@@ -83,8 +83,9 @@ def _time_calls(torch, fn, reps):
 
 def other_workloads(torch, np):
     """The remaining configurations, measured briefly on the same GPU (rays resident on the device):
-    sphere + ground primary and uniform rays, the reference-semantics renderer on the box of spheres,
-    and motion-blurred traversal of 2000 moving spheres."""
+    sphere + ground primary and uniform rays, the reference-semantics renderer on the box of spheres
+    and on the feature scene (mirrors, disk and sphere lights, Gaussian filter), and motion-blurred
+    traversal of 2000 moving spheres."""
     import vermeer_amd as va
     from vermeer_amd import render, scenes
     dev = torch.device("cuda:0")
@@ -134,6 +135,28 @@ def other_workloads(torch, np):
                                  "triangles": int(sum(len(m["idx"]) for m in desc["meshes"]))}
     R.close()
     log(f"[bench] box+spheres renderer: {out['box_spheres_render']}")
+
+    # the same renderer on the feature scene: conductor and perfect-mirror spheres (four levels per pass),
+    # a disk light, a sphere light, Gaussian pixel filter; 512^2, 16 passes
+    desc = scenes.feature_scene()
+    c = desc["camera"]
+    cam = render.lookat_camera(c["from"], c["to"], c["up"], c["fov"], c["focal"], W, W)
+    R = render.Renderer(desc, W, W, cam)
+    R.set_filter("gaussian")
+    film.zero_()
+    R.render_pass_device(scr.data_ptr(), 1, film.data_ptr(), stream=st)
+    e0.record()
+    for it in range(1, 17):
+        R.render_pass_device(scr.data_ptr(), it, film.data_ptr(), stream=st)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    R.scene.status(st)
+    out["feature_scene_render"] = {"image": [W, W], "passes": 16, "ms": round(ms, 2),
+                                   "samples_per_s": round(W * W * 16 / (ms * 1e-3), 1), "mirror_levels": 4,
+                                   "filter": "gaussian"}
+    R.close()
+    log(f"[bench] feature-scene renderer: {out['feature_scene_render']}")
 
     # motion blur: 2000 spheres x 250 triangles, 2 keys; 1024^2 x 4 camera rays, time from the van der Corput
     # sequence; then one any-hit shadow ray per hit towards a point light (D = L - P, tmax 1 - 1e-4)
