@@ -140,6 +140,8 @@ typedef struct vgpu_camera {
     float tan_theta_focal;
     float aspect;
     float focal;
+    float radius;            /* lens radius: > 0 samples a thin lens with UniformDisk2D
+                                (camera.go:265-283, math/sample/sample.go:105-129); path mode needs 0 */
 } vgpu_camera;
 
 typedef struct vgpu_scene vgpu_scene;

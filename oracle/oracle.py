@@ -220,6 +220,8 @@ def _render_sigs(lib):
         "orc_render_free": (None, [P]),
         "orc_render_add_light": (None, [P, C.c_int32, P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32]),
         "orc_render_set_filter": (None, [P, C.c_int32, C.c_float, C.c_int32, C.c_float]),
+        "orc_render_set_lens": (None, [P, C.c_float]),
+        "orc_uniform_disk": (None, [C.c_float, C.c_float, C.c_float, P]),
         "orc_render_warp_sample": (None, [P, C.c_double, C.c_double, P]),
         "orc_render_filter_tables": (C.c_int32, [P, P]),
         "orc_fresnel_conductor": (C.c_float, [C.c_float, C.c_float, C.c_float]),
@@ -344,6 +346,10 @@ class OracleRenderer:
             self.lib.orc_render_free(self.h_r)
             self.h_r = None
 
+    def set_lens(self, radius):
+        """The camera node's Radius: a thin lens when > 0 (camera.go:265-283)."""
+        self.lib.orc_render_set_lens(self.h_r, float(radius))
+
     def set_filter(self, kind="box", width=0.0, res=0, peak=0.0):
         t = {"box": 0, "gaussian": 1, "airy": 2}[kind]
         self.lib.orc_render_set_filter(self.h_r, t, float(width), int(res), float(peak))
@@ -372,6 +378,13 @@ class OracleRenderer:
 def fresnel_dielectric(eta, c):
     lib = load(); _render_sigs(lib)
     return lib.orc_fresnel_dielectric(eta, c)
+
+
+def uniform_disk(radius, r0, r1):
+    lib = load(); _render_sigs(lib)
+    xy = np.zeros(2, np.float32)
+    lib.orc_uniform_disk(float(radius), float(r0), float(r1), _p(xy))
+    return float(xy[0]), float(xy[1])
 
 
 def fresnel_conductor(r, g, c):

@@ -6,7 +6,8 @@
 //   core/trace.go:40-83            Trace: ShaderContext setup, ApplyTransform, Shader.Eval
 //   core/shader.go:139-402         OffsetP, LightsPrepare, NextLight, EvaluateLightSamples
 //   core/ray.go:56-88              Ray.Init
-//   builtin/camera/camera.go:80-323  PreRender (TanThetaFocal), calcLookatMatrices, ComputeRay (pinhole)
+//   builtin/camera/camera.go:80-323  PreRender (TanThetaFocal), calcLookatMatrices, ComputeRay (pinhole, thin lens)
+//   math/sample/sample.go:105-129   UniformDisk2D
 //   builtin/shader/std.go:77-316     ShaderStd.Eval / EvalEmission, perfect mirrors by recursion to level 3
 //   builtin/shader/bsdf/orennayar.go:24-73, microfacetggx.go:26-146, specular.go:13-101
 //   builtin/shader/fresnel/dielectric.go:34-47, conductor.go:31-89
@@ -25,7 +26,7 @@
 // math package, both are within 1 ulp of the true value; the float32 results
 // they feed agree except on rare rounding ties).
 //
-// Not restated (absent from the workloads): thin-lens cameras, textures, quad lights.
+// Not restated (absent from the workloads): camera motion keys, textures, quad lights.
 
 #include "oracle_core.h"
 
@@ -749,7 +750,7 @@ struct Renderer {
     std::vector<double> filt;     // pixel filter sampler: cdfV (filtN) then cdfVU (filtN x filtN); empty: box
     int filtN = 0;
     double filtW = 0;
-    int maxLevel = 0;             // 3 when a material is a perfect mirror
+    float lensRadius = 0;         // camera Radius (thin lens when > 0)
     std::vector<std::vector<int32_t>> geomShaders;   // per post-sort geom slot: material id per shader index
     int shadowRays = 0;
 
@@ -1158,14 +1159,26 @@ struct Renderer {
         }
         float Sx = (float)(-1.0 + 2.0 * (rx / (double)width));
         float Sy = -(float)(-1.0 + 2.0 * (ry / (double)height));
-        // camera.ComputeRay (pinhole), camera.go:221-319
+        // camera.ComputeRay, camera.go:221-300: pinhole, or a thin lens when Radius > 0
         float camu = Sx * tanThetaFocal;
         float camv = Sy * (tanThetaFocal / aspect);
         Vec3 s = Sub(Add(Scale(camu, V3(1, 0, 0)), Scale(camv, V3(0, 1, 0))), Scale(focal, V3(0, 0, 1)));
-        Vec3 d = MulVec(M, s);
+        Vec3 e = V3(0, 0, 0);
+        if (lensRadius > 0.0f) {
+            // lensU, lensV (core/render.go:91-92) through sample.UniformDisk2D (math/sample/sample.go:105-129)
+            float r0 = (float)VanDerCorput(iter, scr[0]), r1 = (float)Sobol(iter, scr[1]);
+            float x = -1 + 2 * r0, y = -1 + 2 * r1, rr = 0, theta = 0;
+            if (x > -y && x > y) { rr = x; theta = (Pi32 / 4) * y / x; }
+            else if (x > -y && x < y) { rr = y; theta = (Pi32 / 4) * (2 - x / y); }
+            else if (x < y && x < -y) { rr = -x; theta = (Pi32 / 4) * (4 + y / x); }
+            else if (x > y && x < -y) { rr = -y; theta = (Pi32 / 4) * (6 - x / y); }
+            float xo = lensRadius * rr * fCos(theta), yo = lensRadius * rr * fSin(theta);
+            e = Add(Scale(xo, V3(1, 0, 0)), Scale(yo, V3(0, 1, 0)));
+        }
+        Vec3 d = MulVec(M, Sub(s, e));
         Ray r{};
         r.D = Normalize(d);
-        r.P = MulPoint(M, V3(0, 0, 0));
+        r.P = MulPoint(M, e);
         r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = 0;
         RaySetup(r);
         RGB out = TraceShade(tk, r, 0, (float)lambda, iter, scr + 4, hit);
@@ -1274,6 +1287,18 @@ void* orc_render_new(void* sp, int32_t w, int32_t h, const float* M, float tan_t
     return r;
 }
 void orc_render_free(void* r) { delete (Renderer*)r; }
+// the camera node's Radius (camera.go:265-283): a thin lens when > 0
+void orc_render_set_lens(void* rp, float radius) { ((Renderer*)rp)->lensRadius = radius; }
+// sample.UniformDisk2D (math/sample/sample.go:105-129) for known-answer tests
+void orc_uniform_disk(float radius, float r0, float r1, float* xy) {
+    float x = -1 + 2 * r0, y = -1 + 2 * r1, rr = 0, theta = 0;
+    if (x > -y && x > y) { rr = x; theta = (Pi32 / 4) * y / x; }
+    else if (x > -y && x < y) { rr = y; theta = (Pi32 / 4) * (2 - x / y); }
+    else if (x < y && x < -y) { rr = -x; theta = (Pi32 / 4) * (4 + y / x); }
+    else if (x > y && x < -y) { rr = -y; theta = (Pi32 / 4) * (6 - x / y); }
+    xy[0] = radius * rr * fCos(theta);
+    xy[1] = radius * rr * fSin(theta);
+}
 
 // light_geom_insertion: the insertion index of the light's mesh (mapped to its post-sort slot here)
 // type: 0 TriLight (P0, P1, P2 vertices), 1 DiskLight (P0 = P, P1 = Up, P2 = LookAt, radius),

@@ -226,3 +226,43 @@ def test_mirror_of_an_emitter_reflects_it_with_the_fresnel_weight():
     assert np.allclose(film[4, 4] * 2, want, rtol=2e-3, atol=1e-4)   # film = (0 * 1 + c) / 2 after pass 1
     # no light sources: everything else in the image is the mirror's reflection or the emitter
     assert np.all(np.isfinite(film))
+
+
+# ------------------------------------------------------------ thin-lens camera
+
+def test_uniform_disk_is_the_concentric_map():
+    """Shirley-Chiu concentric square-to-disk map (sample.go:105-129): points inside the radius, the
+    square's edges on the rim, equal areas (r^2 uniform, angle uniform), and the degenerate diagonals
+    x = +-y, which no branch takes, at the centre."""
+    R = 0.7
+    pts = np.array([oracle.uniform_disk(R, a, b) for a in (np.arange(96) + 0.5) / 96 for b in (np.arange(96) + 0.5) / 96])
+    rr = np.hypot(pts[:, 0], pts[:, 1])
+    assert rr.max() <= R * (1 + 1e-6)
+    assert abs(np.mean((rr / R) ** 2) - 0.5) < 0.01                  # uniform over the area
+    ang = np.arctan2(pts[:, 1], pts[:, 0])
+    h, _ = np.histogram(ang, bins=8, range=(-np.pi, np.pi))
+    assert h.min() > 0.95 * h.mean()
+    x, y = oracle.uniform_disk(R, 1.0, 0.5)                            # right edge midpoint -> (R, 0)
+    assert abs(x - R) < 1e-6 and abs(y) < 1e-6
+    assert oracle.uniform_disk(R, 0.75, 0.75) == (0.0, 0.0)           # x == y
+    assert oracle.uniform_disk(R, 0.25, 0.75) == (0.0, 0.0)           # x == -y
+
+
+def test_thin_lens_converges_and_reduces_to_pinhole():
+    d = scenes.feature_scene(nseg=16, nring=8)
+    c = d["camera"]
+    M, ttf = oracle.camera_lookat(c["from"], c["to"], c["up"], c["fov"], c["focal"])
+    scr = scenes.pixel_scrambles(32, 32, seed=4)
+    films = {}
+    for radius in (0.0, 0.08):
+        r = oracle.OracleRenderer(d, 32, 32, M, ttf, c["focal"])
+        r.set_lens(radius)
+        for it in (1, 2, 3):
+            r.render_pass(scr, it)
+        films[radius] = r.film.copy()
+    pin = oracle.OracleRenderer(d, 32, 32, M, ttf, c["focal"])
+    for it in (1, 2, 3):
+        pin.render_pass(scr, it)
+    assert np.array_equal(films[0.0], pin.film)                        # radius 0 is the pinhole path
+    assert not np.array_equal(films[0.08], pin.film)
+    assert abs(films[0.08].mean() - pin.film.mean()) < 0.25 * pin.film.mean()

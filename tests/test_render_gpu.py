@@ -151,6 +151,25 @@ def test_render_feature_scene_bitexact(feature_scene, filt):
         oracle.set_normalize_ieee(False)
 
 
+def test_render_thin_lens_bitexact(feature_scene):
+    oracle.set_normalize_ieee(True)
+    try:
+        w = h = 128
+        cam, M, ttf = _camera(feature_scene, w, h)
+        cam["radius"] = 0.06
+        g = render.Renderer(feature_scene, w, h, cam)
+        o = oracle.OracleRenderer(feature_scene, w, h, M, ttf, feature_scene["camera"]["focal"])
+        o.set_lens(0.06)
+        scr = scenes.pixel_scrambles(w, h, seed=13)
+        for it in range(1, 4):
+            print(f"[case] thin lens pass {it}", flush=True)
+            _, gs = g.render_pass(scr, it, want_sample=True)
+            _, os_ = o.render_pass(scr, it, want_sample=True)
+            assert _nan_equal(gs, os_), f"pass {it}: samples differ"
+    finally:
+        oracle.set_normalize_ieee(False)
+
+
 def test_render_feature_scene_device_pointer_path(feature_scene):
     import torch
     oracle.set_normalize_ieee(True)

@@ -111,7 +111,8 @@ MATERIAL_DTYPE = np.dtype([
 ])
 LIGHT_DTYPE = np.dtype([("P0", "<f4", 3), ("P1", "<f4", 3), ("P2", "<f4", 3), ("material", "<i4"),
                         ("samples", "<i4"), ("geom", "<i4"), ("type", "<i4"), ("radius", "<f4")])
-CAMERA_DTYPE = np.dtype([("M", "<f4", 16), ("tan_theta_focal", "<f4"), ("aspect", "<f4"), ("focal", "<f4")])
+CAMERA_DTYPE = np.dtype([("M", "<f4", 16), ("tan_theta_focal", "<f4"), ("aspect", "<f4"), ("focal", "<f4"),
+                         ("radius", "<f4")])
 
 _lib = None
 

@@ -126,6 +126,7 @@ int vgpu_pt_create(vgpu_pt** out, vgpu_scene* s, const vgpu_pt_material* materia
     v.width = (int32_t)width;
     v.height = (int32_t)height;
     v.max_depth = (int32_t)max_depth;
+    if (cam->radius != 0.0f) { delete pt; return fail(VGPU_ERR_ARG, "path mode takes a pinhole camera (radius 0)"); }
     std::memcpy(v.M, cam->M, sizeof(v.M));
     v.tan_theta_focal = cam->tan_theta_focal;
     v.aspect = cam->aspect;

@@ -224,6 +224,8 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     rv.tan_theta_focal = cam->tan_theta_focal;
     rv.aspect = cam->aspect;
     rv.focal = cam->focal;
+    if (!(cam->radius >= 0.0f) || !std::isfinite(cam->radius)) return fail(VGPU_ERR_ARG, "camera radius must be >= 0");
+    rv.lens_radius = cam->radius;
     rv.filt = nullptr;
     rv.filt_n = 0;
     rv.filt_w = 0.0;

@@ -811,16 +811,30 @@ __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* _
     const double lambda = (720 - 450) * vdc(iter, s[3]) + 450;
     const float Sx = (float)(-1.0 + 2.0 * (rx / (double)rv.width));
     const float Sy = -(float)(-1.0 + 2.0 * (ry / (double)rv.height));
-    // camera.ComputeRay, pinhole (camera.go:244-284)
+    // camera.ComputeRay (camera.go:221-300): pinhole, or a thin lens when Radius > 0
     const float camu = Sx * rv.tan_theta_focal;
     const float camv = Sy * (rv.tan_theta_focal / rv.aspect);
     const V3 sv = sub(add(scale(camu, v3(1, 0, 0)), scale(camv, v3(0, 1, 0))), scale(rv.focal, v3(0, 0, 1)));
+    V3 e = v3(0.0f, 0.0f, 0.0f);
+    if (rv.lens_radius > 0.0f) {
+        // lensU / lensV (core/render.go:91-92), sample.UniformDisk2D (math/sample/sample.go:105-129)
+        const float r0 = (float)vdc(iter, s[0]), r1 = (float)sobol(iter, s[1]);
+        const float x = -1 + 2 * r0, y = -1 + 2 * r1;
+        float r = 0.0f, th = 0.0f;
+        if (x > -y && x > y) { r = x; th = (PI32 / 4) * y / x; }
+        else if (x > -y && x < y) { r = y; th = (PI32 / 4) * (2 - x / y); }
+        else if (x < y && x < -y) { r = -x; th = (PI32 / 4) * (4 + y / x); }
+        else if (x > y && x < -y) { r = -y; th = (PI32 / 4) * (6 - x / y); }
+        const float xo = rv.lens_radius * r * fcos(th), yo = rv.lens_radius * r * fsin(th);
+        e = add(scale(xo, v3(1, 0, 0)), scale(yo, v3(0, 1, 0)));
+    }
+    const V3 se = sub(sv, e);
     const float* M = rv.M;
-    const V3 d = v3(M[0] * sv.x + M[4] * sv.y + M[8] * sv.z, M[1] * sv.x + M[5] * sv.y + M[9] * sv.z,
-                    M[2] * sv.x + M[6] * sv.y + M[10] * sv.z);
+    const V3 d = v3(M[0] * se.x + M[4] * se.y + M[8] * se.z, M[1] * se.x + M[5] * se.y + M[9] * se.z,
+                    M[2] * se.x + M[6] * se.y + M[10] * se.z);
     const V3 D = normalize(d);
-    const V3 P = v3(M[0] * 0.0f + M[4] * 0.0f + M[8] * 0.0f + M[12], M[1] * 0.0f + M[5] * 0.0f + M[9] * 0.0f + M[13],
-                    M[2] * 0.0f + M[6] * 0.0f + M[10] * 0.0f + M[14]);
+    const V3 P = v3(M[0] * e.x + M[4] * e.y + M[8] * e.z + M[12], M[1] * e.x + M[5] * e.y + M[9] * e.z + M[13],
+                    M[2] * e.x + M[6] * e.y + M[10] * e.z + M[14]);
     rays[2 * p] = make_float4(P.x, P.y, P.z, __builtin_inff());
     rays[2 * p + 1] = make_float4(D.x, D.y, D.z, 0.0f);
     pix[p] = make_float4(D.x, D.y, D.z, (float)lambda);

@@ -48,6 +48,7 @@ struct RenderView {
     int32_t max_half;              // max over lights of NSamples/2 (slots per group = 2 * max_half)
     float M[16];                   // camera matrix (column-major)
     float tan_theta_focal, aspect, focal;
+    float lens_radius;             // thin lens (camera Radius); 0: pinhole
     uint64_t raster_fwd[52];       // RasterXY enumeration tables for m = 12
     uint64_t raster_inv[24];
     const double* filt;            // pixel filter sampler: cdfV (filt_n), then cdfVU (filt_n x filt_n); null: box
