@@ -312,7 +312,9 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
         int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns, r->d_slot_hits, nullptr,
                                nullptr, st, s->cfg);
         if (rc) return rc;
-        e = vgpu::launch_resolve(r->rv, level, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, r->d_recs, st);
+        const bool direct = r->rv.max_level == 0;
+        e = vgpu::launch_resolve(r->rv, level, iter, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, r->d_recs,
+                                 direct ? d_film : nullptr, direct ? d_samp : nullptr, st);
         if (e != hipSuccess) return fail(VGPU_ERR_HIP, "resolve: %s", hipGetErrorString(e));
         if (host && shadow_rays_out) {
             // count active slots (diagnostic, host mode only)
@@ -322,8 +324,10 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
             for (const float4& v : rgb) shadow_count += v.w != 0.0f;
         }
     }
-    e = vgpu::launch_combine(r->rv, iter, r->d_recs, d_film, d_samp, st);
-    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "combine: %s", hipGetErrorString(e));
+    if (r->rv.max_level > 0) {
+        e = vgpu::launch_combine(r->rv, iter, r->d_recs, d_film, d_samp, st);
+        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "combine: %s", hipGetErrorString(e));
+    }
     if (host) {
         HIP_TRY(hipMemcpyAsync(film, d_film, np * 12, hipMemcpyDeviceToHost, st));
         if (sample_out) HIP_TRY(hipMemcpyAsync(sample_out, d_samp, np * 12, hipMemcpyDeviceToHost, st));

@@ -1028,14 +1028,32 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
 
 // Sums one level's unoccluded slot colours in the reference's order and applies the per-light / per-lobe
 // scales: a = (0 + emission) + diffuse, and for a rough specular lobe s = its direct lighting.
-__global__ __launch_bounds__(256) void k_resolve(RenderView rv, int level, const float4* __restrict__ slot_rgb,
+// Without mirrors (max_level 0) the single level is the sample: the film is updated here directly.
+__device__ __forceinline__ void film_update(float* __restrict__ film, float* __restrict__ sample_out, int p,
+                                            uint32_t iter, RGB contrib) {
+    float* f = film + (size_t)p * 3;
+    f[0] = (f[0] * (float)iter + contrib.r) / (float)(iter + 1);
+    f[1] = (f[1] * (float)iter + contrib.g) / (float)(iter + 1);
+    f[2] = (f[2] * (float)iter + contrib.b) / (float)(iter + 1);
+    if (sample_out) {
+        sample_out[(size_t)p * 3] = contrib.r;
+        sample_out[(size_t)p * 3 + 1] = contrib.g;
+        sample_out[(size_t)p * 3 + 2] = contrib.b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resolve(RenderView rv, int level, uint32_t iter, const float4* __restrict__ slot_rgb,
                                                  const int4* __restrict__ slot_hits, const float4* __restrict__ groups,
-                                                 const float4* __restrict__ pixshade, LevelRec* __restrict__ recs) {
+                                                 const float4* __restrict__ pixshade, LevelRec* __restrict__ recs,
+                                                 float* __restrict__ film, float* __restrict__ sample_out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= rv.width * rv.height) return;
     const float4* ps = pixshade + (size_t)p * 4;
     const float4 e = ps[0];
-    if (e.w == 0.0f) return;
+    if (e.w == 0.0f) {
+        if (film) film_update(film, sample_out, p, iter, rgb0());
+        return;
+    }
     LevelRec* rec = recs + (size_t)p * (kMaxLevel + 1) + level;
     const int SGS = 2 * rv.max_half;
     const float4 dc = ps[1], spc = ps[2], fl = ps[3];
@@ -1067,6 +1085,11 @@ __global__ __launch_bounds__(256) void k_resolve(RenderView rv, int level, const
     RGB a = rgb0();
     rgb_add(a, RGB{e.x, e.y, e.z});
     rgb_add(a, diff);
+    if (film) {   // no mirrors: contrib = ((0 + emission) + diffuse) + specular
+        rgb_add(a, spec);
+        film_update(film, sample_out, p, iter, a);
+        return;
+    }
     const float kind = rec->a.w;
     rec->a = make_float4(a.r, a.g, a.b, kind);
     if (kind == 1.0f) rec->s = make_float4(spec.r, spec.g, spec.b, rec->s.w);
@@ -1101,16 +1124,7 @@ __global__ __launch_bounds__(256) void k_combine(RenderView rv, uint32_t iter, c
         out = o;
         next_hit = true;
     }
-    const RGB contrib = (r[0].a.w != 0.0f) ? out : rgb0();
-    float* f = film + (size_t)p * 3;
-    f[0] = (f[0] * (float)iter + contrib.r) / (float)(iter + 1);
-    f[1] = (f[1] * (float)iter + contrib.g) / (float)(iter + 1);
-    f[2] = (f[2] * (float)iter + contrib.b) / (float)(iter + 1);
-    if (sample_out) {
-        sample_out[(size_t)p * 3] = contrib.r;
-        sample_out[(size_t)p * 3 + 1] = contrib.g;
-        sample_out[(size_t)p * 3 + 2] = contrib.b;
-    }
+    film_update(film, sample_out, p, iter, (r[0].a.w != 0.0f) ? out : rgb0());
 }
 
 }  // namespace
@@ -1131,11 +1145,12 @@ hipError_t launch_shade(const RenderView& rv, const uint64_t* scr, uint32_t iter
     return hipGetLastError();
 }
 
-hipError_t launch_resolve(const RenderView& rv, int level, const float4* slot_rgb, const int4* slot_hits,
-                          const float4* groups, const float4* pixshade, LevelRec* recs, hipStream_t stream) {
+hipError_t launch_resolve(const RenderView& rv, int level, uint32_t iter, const float4* slot_rgb, const int4* slot_hits,
+                          const float4* groups, const float4* pixshade, LevelRec* recs, float* film, float* sample_out,
+                          hipStream_t stream) {
     const int n = rv.width * rv.height;
-    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, stream, rv, level, slot_rgb, slot_hits, groups,
-                       pixshade, recs);
+    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, stream, rv, level, iter, slot_rgb, slot_hits,
+                       groups, pixshade, recs, film, sample_out);
     return hipGetLastError();
 }
 

@@ -66,8 +66,10 @@ hipError_t launch_camera(const RenderView& rv, const uint64_t* scrambles, uint32
 hipError_t launch_shade(const RenderView& rv, const uint64_t* scrambles, uint32_t iter, int level, const float4* rays_in,
                         const int4* hits32, const float4* pix, float4* slot_rays, float4* slot_rgb, float4* groups,
                         float4* pixshade, float4* rays_out, LevelRec* recs, hipStream_t stream);
-hipError_t launch_resolve(const RenderView& rv, int level, const float4* slot_rgb, const int4* slot_hits,
-                          const float4* groups, const float4* pixshade, LevelRec* recs, hipStream_t stream);
+// film null: record the level for launch_combine; otherwise (no mirrors) update the film directly.
+hipError_t launch_resolve(const RenderView& rv, int level, uint32_t iter, const float4* slot_rgb, const int4* slot_hits,
+                          const float4* groups, const float4* pixshade, LevelRec* recs, float* film, float* sample_out,
+                          hipStream_t stream);
 hipError_t launch_combine(const RenderView& rv, uint32_t iter, const LevelRec* recs, float* film, float* sample_out,
                           hipStream_t stream);
 
