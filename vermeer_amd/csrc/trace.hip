@@ -117,20 +117,23 @@ __device__ __forceinline__ void box4(const RayS& r, const float4& lx, const floa
     }
 }
 
+// The spill area is addressed from its (wavefront-uniform) bases plus one 32-bit per-lane offset,
+// so the lane keeps a single register for it.
 template <int BLOCK, int LDS>
 struct Stack {
     float* sT;
     int32_t* sN;
-    float* gT;
+    float* gT;          // spill bases (uniform)
     int32_t* gN;
     uint64_t gstride;
+    uint32_t gofs;      // this thread's column
     __device__ __forceinline__ void store(int i, float T, int32_t node) {
         if (i < LDS) { sT[i * BLOCK] = T; sN[i * BLOCK] = node; }
-        else { const uint64_t o = (uint64_t)(i - LDS) * gstride; gT[o] = T; gN[o] = node; }
+        else { const uint64_t o = (uint64_t)(i - LDS) * gstride + gofs; gT[o] = T; gN[o] = node; }
     }
     __device__ __forceinline__ void load(int i, float& T, int32_t& node) const {
         if (i < LDS) { T = sT[i * BLOCK]; node = sN[i * BLOCK]; }
-        else { const uint64_t o = (uint64_t)(i - LDS) * gstride; T = gT[o]; node = gN[o]; }
+        else { const uint64_t o = (uint64_t)(i - LDS) * gstride + gofs; T = gT[o]; node = gN[o]; }
     }
 };
 
@@ -154,6 +157,20 @@ __device__ __forceinline__ TriRay tri_ray(const RayS& r) {
 struct TriPre {
     float T, det, fU, fV, fW;
 };
+
+// The direction-sign bits and the permutation packed in one register: neg (bits 0-2, bit k set when
+// D[k] < 0), Kx (3-4), Ky (5-6), Kz (7-8).  The wide kernel keeps this instead of the separate fields
+// and the permuted origin, which it rebuilds from P when a leaf is tested.
+__device__ __forceinline__ int pack_perm(const RayS& r) { return r.neg | (r.Kx << 3) | (r.Ky << 5) | (r.Kz << 7); }
+__device__ __forceinline__ TriRay tri_ray_packed(const RayS& r, int kp) {
+    TriRay q;
+    q.S0 = r.S[0]; q.S1 = r.S[1]; q.S2 = r.S[2];
+    q.Kx = (kp >> 3) & 3; q.Ky = (kp >> 5) & 3; q.Kz = (kp >> 7) & 3;
+    q.Pkx = sel3(r.P[0], r.P[1], r.P[2], q.Kx);
+    q.Pky = sel3(r.P[0], r.P[1], r.P[2], q.Ky);
+    q.Pkz = sel3(r.P[0], r.P[1], r.P[2], q.Kz);
+    return q;
+}
 
 // Watertight ray/triangle test (builtin/geom/polymesh/trace.go:123-194 static, :556-624 motion),
 // the part that does not read the closest distance so far: edge functions (retried in double
@@ -206,6 +223,13 @@ __device__ __forceinline__ bool tri_accept(RayS& r, const TriPre& p, float& U, f
     W = p.fW * rcp;
     r.tclosest = p.T * rcp;
     return true;
+}
+
+// The whole test with the ray quantities given.
+__device__ __forceinline__ bool tri_test_q(RayS& r, const TriRay& q, const float4& v0, const float4& v1, const float4& v2,
+                                           float bias, float& U, float& V, float& W) {
+    TriPre p;
+    return tri_pre<false>(q, v0, v1, v2, bias, p) && tri_accept(r, p, U, V, W);
 }
 
 // The whole test.  Returns true when the triangle is accepted; updates tclosest, U, V, W.
@@ -322,7 +346,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
     const uint64_t nthr = (uint64_t)gridDim.x * BLOCK;
     Stack<BLOCK, LDS> st;
     st.sT = sT + tid; st.sN = sN + tid;
-    st.gT = spT + gtid; st.gN = spN + gtid; st.gstride = nthr;
+    st.gT = spT; st.gN = spN; st.gstride = nthr; st.gofs = (uint32_t)gtid;
 
     if (n_dev) n = min(n, (uint64_t)*n_dev);   // device-resident count (compacted queues)
     for (uint64_t ri = gtid; ri < n; ri += nthr) {
@@ -499,7 +523,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
     const uint64_t nthr = (uint64_t)gridDim.x * BLOCK;
     Stack<BLOCK, LDS> st;
     st.sT = sT + tid; st.sN = sN + tid;
-    st.gT = spT + gtid; st.gN = spN + gtid; st.gstride = nthr;
+    st.gT = spT; st.gN = spN; st.gstride = nthr; st.gofs = (uint32_t)gtid;
 
     if (n_dev) n = min(n, (uint64_t)*n_dev);
     // Rays of this wavefront: v = 0, 1, 2, ... -> ray wave0 + (v / 64) * nthr + v % 64 (increasing
@@ -509,7 +533,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
     const uint64_t wave0 = gtid - (uint64_t)lane;
     uint32_t cursor = 0;
     bool done = true, live = false, ok = true, fast = false;
-    uint64_t ri = 0;
+    uint32_t ri = 0;   // launches hold fewer than 2^32 rays
+    int kp = 0;   // pack_perm of the current ray
     RayS r;
     r.P[0] = 0.0f; r.P[1] = 0.0f; r.P[2] = 0.0f; r.tclosest = 0.0f; r.time = 0.0f;
     {
@@ -519,10 +544,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
     int nodesT = 0, leafsT = 0, trisT = 0, maxDepth = 0;
     int best_geom = -1, best_prim = -1;
     float bu = 0.0f, bv = 0.0f, bw = 0.0f;
-    bool in_mesh = false;
-    int mesh_base = 0, gcur = 0, gend = 0;
+    int mesh_base = 0, gcur = 0, gend = 0;   // mesh_base 0: in the scene level, else inside geom gcur
     DevGeom g = {0, 0, 1, 0.0f};
-    bool overflow = false;
     int sp = 0;
     while (true) {
         const uint64_t dm = __ballot(done);
@@ -531,11 +554,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
         if (more ? __popcll(dm) >= refill_min : dm == ~0ull) {
             if (done) {
                 if (live) {
-                    if (overflow) {
+                    if (best_prim == -2) {   // the stack overflowed
                         atomicOr(err, 1);
                         best_geom = -1; best_prim = -1;
                     }
-                    const uint64_t oi = perm ? (uint64_t)perm[ri] : ri;
+                    const uint64_t oi = perm ? (uint64_t)perm[ri] : (uint64_t)ri;
                     if (EXT) {
                         hits[2 * oi] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
                         hits[2 * oi + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
@@ -548,22 +571,23 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 }
                 if (more) {
                     const uint32_t v = cursor + (uint32_t)__popcll(dm & ((1ull << lane) - 1ull));
-                    ri = wave0 + (uint64_t)(v >> 6) * nthr + (v & 63u);
-                    live = ri < n;
+                    const uint64_t rnext = wave0 + (uint64_t)(v >> 6) * nthr + (v & 63u);
+                    live = rnext < n;
+                    ri = (uint32_t)rnext;
                     if (live) {
                         const float4 ra = rays[2 * ri], rb4 = rays[2 * ri + 1];
                         r.P[0] = ra.x; r.P[1] = ra.y; r.P[2] = ra.z; r.tclosest = ra.w;
                         const float D[3] = {rb4.x, rb4.y, rb4.z};
                         r.time = rb4.w;
                         ray_setup(r, D);
+                        kp = pack_perm(r);
                         ok = true;
                         for (int k = 0; k < 3; k++)
                             ok = ok && absf(r.P[k]) < INFINITY && absf(r.Dinv[k]) < INFINITY && r.Dinv[k] != 0.0f;
                         nodesT = 0; leafsT = 0; trisT = 0; maxDepth = 0;
                         best_geom = -1; best_prim = -1;
                         bu = 0.0f; bv = 0.0f; bw = 0.0f;
-                        in_mesh = false;
-                        overflow = false;
+                        mesh_base = 0;
                         st.store(0, r.tclosest, 0);
                         sp = 1;
                         done = false;
@@ -580,7 +604,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
         int lbase = 0, lcount = 0;
         if (!done) {
             do {
-                if (in_mesh && sp == mesh_base) {
+                if (mesh_base != 0 && sp == mesh_base) {
                     // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
                     gcur++;
                     const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
@@ -590,8 +614,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                         sp = mesh_base + 1;
                         break;
                     }
-                    in_mesh = false;
                     sp = mesh_base - 1;
+                    mesh_base = 0;
                 }
                 if (sp == 0) { done = true; break; }
                 sp--;
@@ -601,20 +625,20 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 if (node >= 0) {
                     if (COUNTERS) nodesT++;
                     NodeData nd;
-                    fetch_static(sv.nodes, in_mesh ? g.node_base + node : node, nd);
+                    fetch_static(sv.nodes, mesh_base != 0 ? g.node_base + node : node, nd);
                     Box4 b;
                     if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
                     else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
                     // push order of qbvh/intersect.go:137-216, as in k_trace
-                    const int n0 = (r.neg >> nd.ax[0]) & 1;
-                    const int n1 = (r.neg >> nd.ax[1]) & 1;
-                    const int n2 = (r.neg >> nd.ax[2]) & 1;
+                    const int n0 = (kp >> nd.ax[0]) & 1;
+                    const int n1 = (kp >> nd.ax[1]) & 1;
+                    const int n2 = (kp >> nd.ax[2]) & 1;
                     const int gA = n0 ? 0 : 2, gB = 2 - gA;
                     const int rk0 = gA + (n1 ^ 1), rk1 = gA + n1, rk2 = gB + (n2 ^ 1), rk3 = gB + n2;
                     const int m = (b.h[0] ? 1 << rk0 : 0) | (b.h[1] ? 1 << rk1 : 0) | (b.h[2] ? 1 << rk2 : 0) |
                                   (b.h[3] ? 1 << rk3 : 0);
                     if (sp + 4 > VGPU_MAX_STACK) {
-                        overflow = true;
+                        best_prim = -2;
                         done = true;
                         break;
                     }
@@ -627,22 +651,22 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 } else {
                     const int base = leaf_base(node), count = leaf_count(node);
                     if (COUNTERS) leafsT++;
-                    if (in_mesh) {
+                    if (mesh_base != 0) {
                         if (COUNTERS) trisT += count;
                         pend = true;
                         lbase = base;
                         lcount = count;
                     } else {
                         // Scene.TraceElems: geoms [base, base+count) (scene.go:61-78)
-                        mesh_base = sp + 1;
-                        if (mesh_base >= VGPU_MAX_STACK) { overflow = true; done = true; break; }
+                        const int mb = sp + 1;
+                        if (mb >= VGPU_MAX_STACK) { best_prim = -2; done = true; break; }
                         gcur = base; gend = base + count;
                         const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
                         if (ng == 2) { done = true; break; }
                         if (ng == 1) {
+                            mesh_base = mb;
                             st.store(mesh_base, r.tclosest, 0);
                             sp = mesh_base + 1;
-                            in_mesh = true;
                         }
                     }
                 }
@@ -656,9 +680,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
             if (pend) {
                 int idx = -1;
                 float U = 0.0f, V = 0.0f, W = 0.0f;
+                const TriRay q = tri_ray_packed(r, kp);
                 const float4* tp = sv.tris + ((size_t)g.tri_base + lbase) * 3;
                 for (int i = 0; i < lcount; i++)
-                    if (tri_test<false>(r, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
+                    if (tri_test_q(r, q, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
                         idx = lbase + i;
                 if (idx >= 0) {
                     best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
@@ -677,7 +702,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
             const int total = __shfl(pin, 63);
             const int pex = pin - c;
             const int tb = g.tri_base + lbase;
-            const TriRay q = tri_ray(r);
+            const TriRay q = tri_ray_packed(r, kp);
             const int kpack = q.Kx | (q.Ky << 2) | (q.Kz << 4);
             const float bias = g.ray_bias;
             uint32_t cand = 0;   // owner: leaf triangles that passed the distance-free part of the test
@@ -716,7 +741,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 while (cand != 0) {
                     const int i = __ffs(cand) - 1;
                     cand &= cand - 1;
-                    if (tri_test<false>(r, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
+                    if (tri_test_q(r, q, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
                         idx = lbase + i;
                 }
                 if (idx >= 0) {
@@ -780,6 +805,7 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
                             int32_t* spN, uint64_t spill_threads, int32_t* err, int num_cus, hipStream_t stream,
                             const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
+    if (n > 0xffffffffull) return hipErrorInvalidValue;
     constexpr int kBlock = 256, kLds = 16, kWaves = 16;   // the block size is the kernels' launch bound
     const TraceLaunch cfg{kBlock, kLds, kWaves};
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
@@ -809,6 +835,7 @@ hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const Tra
                         void* hits, int32_t* geoms, int32_t* counters, float* spT, int32_t* spN, uint64_t spill_threads,
                         int32_t* err, int num_cus, hipStream_t stream, const uint32_t* n_dev, const uint32_t* perm) {
     if (n == 0) return hipSuccess;
+    if (n > 0xffffffffull) return hipErrorInvalidValue;
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
     const uint64_t grid = threads / cfg.block;
