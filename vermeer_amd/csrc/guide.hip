@@ -506,39 +506,58 @@ __global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __res
     mask_pack_store<NT>(acc, X2, D, tb.dY[0], ldb, g, s);
 }
 
-// Weight-gradient partials: one wave per (16x16 tile of some dW_l, split of KSPLIT samples).
-constexpr int NJOB = 4 * 9 + 3 * 16 + 3 * 4;
+// Weight-gradient partials.  One wave per (strip of 16x16 tiles of some dW_l that share their output
+// rows, split of ks samples): the wave loads each dY fragment once for the 3 or 4 tiles of its strip,
+// so it issues one A and nit B loads per nit MFMAs.  Strips: dW1 (64 x 144) as 4 row tiles x 3
+// strips of 3 column tiles; dW2..dW4 (64 x 64) as 4 row tiles x 1 strip of 4; dW5 (40 x 64, rows
+// padded to 48) as 3 x 1 strip of 4.  Each tile accumulates its samples in the same order as one
+// wave per tile would.
+constexpr int NJOB = 4 * 3 + 3 * 4 + 3;
 
 __global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, uint32_t ks) {
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t job = w % NJOB, split = w / NJOB;
     if (split >= nsplit) return;
-    int l, ot, it, nin, nout;
-    if (job < 36) { l = 0; ot = job / 9; it = job % 9; nin = NENC; nout = NH; }
-    else if (job < 84) { l = 1 + (job - 36) / 16; ot = ((job - 36) % 16) / 4; it = (job - 36) % 4; nin = NH; nout = NH; }
-    else { l = 4; ot = (job - 84) / 4; it = (job - 84) % 4; nin = NH; nout = NOUT; }
+    int l, ot, it0, nit, nin, nout;
+    if (job < 12) { l = 0; ot = job / 3; it0 = 3 * (job % 3); nit = 3; nin = NENC; nout = NH; }
+    else if (job < 24) { l = 1 + (job - 12) / 4; ot = (job - 12) % 4; it0 = 0; nit = 4; nin = NH; nout = NH; }
+    else { l = 4; ot = job - 24; it0 = 0; nit = 4; nin = NH; nout = NOUT; }
     const uint32_t ldb = tb.ldb;
     const uint16_t* A = tb.dY[l] + (uint64_t)(16 * ot + c) * ldb;
-    const uint16_t* Bp = tb.X[l] + (uint64_t)(16 * it + c) * ldb;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t* Bp[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) Bp[k] = tb.X[l] + (uint64_t)(16 * (it0 + (k < nit ? k : 0)) + c) * ldb;
+    f32x4 acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float bs = 0.0f;
     const uint64_t s_end = (uint64_t)(split + 1) * ks;
+#pragma unroll 2
     for (uint64_t s0 = (uint64_t)split * ks; s0 < s_end; s0 += 32) {
         const u32x4 av = *reinterpret_cast<const u32x4*>(A + s0 + 8 * g);
-        const u32x4 bv = *reinterpret_cast<const u32x4*>(Bp + s0 + 8 * g);
+        u32x4 bv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < nit) bv[k] = *reinterpret_cast<const u32x4*>(Bp[k] + s0 + 8 * g);
         const bf16x8 a = __builtin_bit_cast(bf16x8, av);
-        acc = mfma(a, __builtin_bit_cast(bf16x8, bv), acc);
-        if (it == 0)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < nit) acc[k] = mfma(a, __builtin_bit_cast(bf16x8, bv[k]), acc[k]);
+        if (it0 == 0)
             for (int i = 0; i < 8; i++) bs += bf16_val(a[i]);
     }
     const int ow[5] = {OW1, OW2, OW3, OW4, OW5}, ob[5] = {OB1, OB2, OB3, OB4, OB5};
     float* Pp = tb.partial + (uint64_t)split * NPARAM;
-    for (int r = 0; r < 4; r++) {
-        const int out = 16 * ot + 4 * g + r;
-        if (out < nout) Pp[ow[l] + out * nin + 16 * it + c] = acc[r];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k >= nit) continue;
+        for (int r = 0; r < 4; r++) {
+            const int out = 16 * ot + 4 * g + r;
+            if (out < nout) Pp[ow[l] + out * nin + 16 * (it0 + k) + c] = acc[k][r];
+        }
     }
-    if (it == 0) {
+    if (it0 == 0) {
         bs = quad_sum(bs);
         if (g == 0 && 16 * ot + c < nout) Pp[ob[l] + 16 * ot + c] = bs;
     }
@@ -604,7 +623,7 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
     hipLaunchKernelGGL(k_csum_final, dim3(1), dim3(64), 0, st, (const float*)(tb.csum + 1), nsplit, tb.csum);
     hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb);
     // weight gradients in splits of 4096 samples when the batch allows (fewer partials to reduce)
-    const uint32_t ks = B % (4 * KSPLIT) == 0 ? 4 * KSPLIT : KSPLIT;
+    const uint32_t ks = B % (2 * KSPLIT) == 0 ? 2 * KSPLIT : KSPLIT;
     const uint32_t nsw = B / ks;
     const uint32_t waves = NJOB * nsw;
     hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsw, ks);
