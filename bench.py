@@ -130,8 +130,16 @@ def other_workloads(torch, np):
     e1.synchronize()
     ms = e0.elapsed_time(e1)
     R.scene.status(st)
+    # rays per pass as the reference counts them (core.TraceProbe calls: camera + shadow rays), from the
+    # same 16 passes rerun in host mode (untimed)
+    scr_h = scenes.pixel_scrambles(W, W)
+    nshadow = 0
+    for it in range(1, 17):
+        nshadow += R.render_pass(scr_h, it)[0]
+    nrays = 16 * W * W + nshadow
     out["box_spheres_render"] = {"image": [W, W], "passes": 16, "ms": round(ms, 2),
                                  "samples_per_s": round(W * W * 16 / (ms * 1e-3), 1),
+                                 "rays": int(nrays), "mrays_per_s": round(nrays / (ms * 1e-3) / 1e6, 1),
                                  "triangles": int(sum(len(m["idx"]) for m in desc["meshes"]))}
     R.close()
     log(f"[bench] box+spheres renderer: {out['box_spheres_render']}")
