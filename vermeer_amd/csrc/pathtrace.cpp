@@ -168,7 +168,7 @@ int vgpu_pt_create(vgpu_pt** out, vgpu_scene* s, const vgpu_pt_material* materia
     b.plen = nullptr;
     vgpu::TraceLaunch cfg{s->cfg.block, s->cfg.lds_stack, s->cfg.waves_per_cu};
     const int rc = ensure_spill(s, std::max(vgpu::trace_grid_threads(cfg, np, s->num_cus),
-                                            vgpu::trace_grid_threads(vgpu::TraceLaunch{256, 16, 16}, np, s->num_cus)));
+                                            vgpu::trace_grid_threads(vgpu::kExtLaunch, np, s->num_cus)));
     if (rc) { delete pt; return rc; }
     *out = pt;
     return VGPU_OK;

@@ -806,8 +806,8 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
                             const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
     if (n > 0xffffffffull) return hipErrorInvalidValue;
-    constexpr int kBlock = 256, kLds = 16, kWaves = 16;   // the block size is the kernels' launch bound
-    const TraceLaunch cfg{kBlock, kLds, kWaves};
+    constexpr int kBlock = kExtLaunch.block, kLds = kExtLaunch.lds_depth;   // the block size is the launch bound
+    const TraceLaunch cfg = kExtLaunch;
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
     const unsigned grid = (unsigned)(threads / kBlock);

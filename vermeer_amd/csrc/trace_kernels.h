@@ -53,9 +53,13 @@ struct SceneView {
 
 struct TraceLaunch {
     int block;          // 64, 128, 256
-    int lds_depth;      // 8, 16, 24, 32
+    int lds_depth;      // 8, 12, 16, 24, 32
     int waves_per_cu;   // resident waves per CU used to size the grid
 };
+
+// Launch shape of launch_trace_ext (the shading passes' closest-hit traversal); callers size the
+// spill area with it.
+constexpr TraceLaunch kExtLaunch{128, 12, 24};
 
 // Scratch of the ray sort (raysort.hip), sized for `capacity` rays.
 struct RaySortBuffers {
