@@ -238,6 +238,23 @@ int vgpu_scene_add_shaded_mesh(vgpu_scene* scene, const float* verts, uint32_t n
                                uint32_t ntris, const float* normals, const uint8_t* shaderidx,
                                const int32_t* shaders, uint32_t nshaders, int32_t* geom_out);
 
+/* Adds a moving triangle mesh with shading data (a PolyMesh whose P has
+ * nkeys >= 2 motion keys, builtin/geom/polymesh/trace.go:520-719): verts
+ * nkeys*nverts*3 floats, key-major, as vgpu_scene_add_mesh.  normals
+ * (nullable) hold normal_keys*ntris*3*3 floats, one normal per face corner
+ * and key, as TraceMotionElems indexes Normals (trace.go:694-703); they are
+ * used only when normal_keys == nkeys (otherwise N = Ng, as trace.go:694
+ * decides).  shaderidx (nullable) is per triangle in insertion order.  flags:
+ * 0 or VGPU_MESH_MOTION_REFERENCE_INDEXING (the leaf slot indexes the
+ * vertices, shader index and normals, trace.go:527-545).  A hit is shaded at
+ * the ray's time: vertices and normals interpolated between the two keys that
+ * bracket it, P, Ng, dPdu = v1 - v0, dPdv = v2 - v0 and the offset as
+ * trace.go:625-669 computes them. */
+int vgpu_scene_add_shaded_motion_mesh(vgpu_scene* scene, const float* verts, uint32_t nverts, uint32_t nkeys,
+                                      const uint32_t* idx, uint32_t ntris, const float* normals,
+                                      uint32_t normal_keys, const uint8_t* shaderidx, const int32_t* shaders,
+                                      uint32_t nshaders, uint32_t flags, int32_t* geom_out);
+
 /* Adds the analytic sphere a SphereLight inserts (builtin/geom/sphere:
  * sphere.go:15-95, trace.go:13-52) with its shader (a material id).  It is a
  * scene geom like a mesh: bounds P -/+ radius in the top-level BVH, hit by
@@ -250,7 +267,9 @@ int vgpu_scene_add_sphere(vgpu_scene* scene, const float* centre, float radius, 
  * direct lighting with MIS (std.go:77-296, core/shader.go:203-402), perfect
  * mirrors (Spec1Roughness 0) by recursion to level 3 with one light sample
  * per light below level 0, spectral hero-wavelength colour, film
- * fb = (fb*iter + c)/(iter+1). */
+ * fb = (fb*iter + c)/(iter+1).  Each sample has the time
+ * VanDerCorput(iter, time scramble) (render.go:93,118) that its camera,
+ * mirror and shadow rays carry; moving meshes are traced and shaded at it. */
 int vgpu_render_create(vgpu_render** out, vgpu_scene* scene, const vgpu_material* materials, uint32_t nmaterials,
                        const vgpu_light* lights, uint32_t nlights, const vgpu_camera* camera,
                        uint32_t width, uint32_t height);

@@ -216,6 +216,8 @@ def _render_sigs(lib):
         "orc_spectrum_to_rgb": (None, [P, C.c_float, P]),
         "orc_cie_tables": (None, [P, P, P]),
         "orc_scene_add_shaded_mesh": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, P, P, C.c_int32]),
+        "orc_scene_add_shaded_motion_mesh": (C.c_int, [P, P, C.c_int32, C.c_int32, P, C.c_int32, P, C.c_int32, P, P,
+                                                       C.c_int32, C.c_int32]),
         "orc_render_new": (P, [P, C.c_int32, C.c_int32, P, C.c_float, C.c_float, C.c_float, P, C.c_int32]),
         "orc_render_free": (None, [P]),
         "orc_render_add_light": (None, [P, C.c_int32, P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32]),
@@ -315,6 +317,9 @@ class OracleRenderer:
                 self.lib.orc_scene_add_sphere(self.scene.h, _p(c), float(mesh["radius"]), int(mesh["material"]))
                 self.scene.meshes.append(None)
                 continue
+            if int(mesh.get("keys", 1)) > 1:
+                self._add_motion_mesh(mesh)
+                continue
             v = np.ascontiguousarray(mesh["verts"], np.float32)
             ix = np.ascontiguousarray(mesh["idx"], np.uint32)
             nrm = mesh.get("normals")
@@ -340,6 +345,24 @@ class OracleRenderer:
                                           float(L.get("radius", 0.0)), int(L["material"]), int(L["samples"]),
                                           int(L["geom"]))
         self.film = np.zeros((height, width, 3), np.float32)
+
+    def _add_motion_mesh(self, mesh):
+        nk = int(mesh["keys"])
+        v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(nk, -1, 3)
+        ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
+        nrm = mesh.get("normals")
+        nrm_keys = 0
+        if nrm is not None:
+            nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, len(ix) * 3, 3)
+            nrm_keys = nrm.shape[0]
+        sidx = mesh.get("shaderidx")
+        sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
+        shaders = np.ascontiguousarray(mesh["shaders"], np.int32)
+        self._keep += [v, ix, nrm, sidx, shaders]
+        self.lib.orc_scene_add_shaded_motion_mesh(self.scene.h, _p(v), v.shape[1], nk, _p(ix), len(ix), _p(nrm),
+                                                  nrm_keys, _p(sidx), _p(shaders), len(shaders),
+                                                  1 if mesh.get("reference_indexing") else 0)
+        self.scene.meshes.append((v, ix))
 
     def __del__(self):
         if getattr(self, "h_r", None):

@@ -378,6 +378,8 @@ struct Mesh {
     std::vector<int32_t> accelIdx;    // accel.idx: leaf slot -> original face
     std::vector<Vec3> normals;        // Normals.Elems (per vertex), nullable
     std::vector<uint32_t> normalidx;  // reordered with idxp
+    std::vector<Vec3> mnormals;       // moving meshes: Normals.Elems per face corner, normalKeys keys
+    int normalKeys = 0;
     std::vector<uint8_t> shaderidx;   // per triangle, reordered with idxp
     std::vector<int32_t> shaders;     // shader table: material id per shader index
     int facecount = 0;
@@ -615,14 +617,37 @@ struct Mesh {
             ray.Tclosest = T * rcpDet;
             idx = faceidx;
             for (int k = 0; k < 3; k++) hr.Po[k] = U * V0[k] + V * V1[k] + W * V2[k];
+            float xAbsSum = Abs(U * V0[0]) + Abs(V * V1[0]) + Abs(W * V2[0]);
+            float yAbsSum = Abs(U * V0[1]) + Abs(V * V1[1]) + Abs(W * V2[1]);
+            float zAbsSum = Abs(U * V0[2]) + Abs(V * V1[2]) + Abs(W * V2[2]);
             float e00 = V1[0] - V0[0], e01 = V1[1] - V0[1], e02 = V1[2] - V0[2];
             float e10 = V2[0] - V0[0], e11 = V2[1] - V0[1], e12 = V2[2] - V0[2];
             Vec3 Ng; Ng[0] = e01 * e12 - e02 * e11; Ng[1] = e02 * e10 - e00 * e12; Ng[2] = e00 * e11 - e01 * e10;
             hr.Ng = Normalize(Ng);
+            hr.DdPdu[0] = e00; hr.DdPdu[1] = e01; hr.DdPdu[2] = e02;   // trace.go:653-659
+            hr.DdPdv[0] = e10; hr.DdPdv[1] = e11; hr.DdPdv[2] = e12;
+            float d = Gamma(7) * xAbsSum * Abs(hr.Ng[0]) + Gamma(7) * yAbsSum * Abs(hr.Ng[1]) +
+                      Gamma(7) * zAbsSum * Abs(hr.Ng[2]);
+            for (int k = 0; k < 3; k++) hr.Poffset[k] = d * hr.Ng[k];
             hr.Bu = U; hr.Bv = V;
         }
         if (idx == -1) return false;
         hr.elem = idx; hr.U = U; hr.V = V; hr.W = W;
+        // trace.go:673-717: shader index and normals of the hit face (the leaf slot itself with the
+        // reference's indexing; its accel.idx face otherwise)
+        const int sf = motionBug ? idx : accelIdx[idx];
+        hr.shaderIdx = shaderidx.empty() ? 0 : shaderidx[sf];
+        if (!mnormals.empty() && normalKeys == nkeys) {
+            const size_t perKey = (size_t)facecount * 3;   // Normals.ElemsPerKey
+            Vec3 N0 = Lerp(mnormals[sf * 3 + 0 + perKey * key], mnormals[sf * 3 + 0 + perKey * key2], time);
+            Vec3 N1 = Lerp(mnormals[sf * 3 + 1 + perKey * key], mnormals[sf * 3 + 1 + perKey * key2], time);
+            Vec3 N2 = Lerp(mnormals[sf * 3 + 2 + perKey * key], mnormals[sf * 3 + 2 + perKey * key2], time);
+            Vec3 Nn;
+            for (int k = 0; k < 3; k++) Nn[k] = U * N0[k] + V * N1[k] + W * N2[k];
+            hr.N = Normalize(Nn);
+        } else {
+            hr.N = hr.Ng;
+        }
         return true;
     }
 

@@ -224,10 +224,13 @@ static RasterTables makeRasterTables(int m) {
     return t;
 }
 static const RasterTables& rasterTables(int m) {
-    static RasterTables tabs[26];
-    static bool init = false;
-    if (!init) { for (int k = 1; k <= 25; k++) tabs[k] = makeRasterTables(k); init = true; }
-    return tabs[m];
+    // built once, before any thread reads them (a function-local static is initialised thread-safely)
+    struct All {
+        RasterTables t[26];
+        All() { for (int k = 1; k <= 25; k++) t[k] = makeRasterTables(k); }
+    };
+    static const All all;
+    return all.t[m];
 }
 // math/ldseq/raster.go:10-40
 static uint64_t rasterLookup(uint32_t m, uint32_t frame, uint32_t px, uint32_t py, uint64_t sx, uint64_t sy) {
@@ -472,6 +475,7 @@ struct SC {                    // the subset of core.ShaderContext the shading p
     int32_t geom;
     Vec3 P, Poffset, N, Ng, DdPdu, DdPdv, Rd;
     float Lambda;
+    float Time;                // sample time, carried by every ray the hit spawns (core/ray.go:65)
     uint64_t I;
     uint64_t Scramble[2];
     int32_t shaderIdx;
@@ -763,9 +767,9 @@ struct Renderer {
         return e;
     }
 
-    bool TraceShadow(Task* tk, const Vec3& P, const Vec3& D, float tmax) const {
+    bool TraceShadow(Task* tk, const Vec3& P, const Vec3& D, float tmax, float time) const {
         Ray r{};
-        r.P = P; r.D = D; r.Tclosest = tmax; r.Type = RayTypeShadow; r.Time = 0;
+        r.P = P; r.D = D; r.Tclosest = tmax; r.Type = RayTypeShadow; r.Time = time;
         RaySetup(r);
         tk->StackTop = 0; r.task = tk;
         SceneHit sh{}; sh.geom = -1;
@@ -977,7 +981,7 @@ struct Renderer {
                 if (Dot(l.Ld, sc.N) <= 0) continue;
                 Vec3 o = Dot(l.Ld, sc.Ng) < 0 ? OffsetP(sc, -1) : OffsetP(sc, 1);
                 shadowRays++;
-                if (!TraceShadow(tk, o, Scale(l.Ldist * k, l.Ld), 1.0f)) {
+                if (!TraceShadow(tk, o, Scale(l.Ldist * k, l.Ld), 1.0f, sc.Time)) {
                     Spectrum rho = bsdf.Eval(l.Ld);
                     rho.Mul(l.Liu);
                     float p_hat = (float)nB * (float)bsdf.PDF(l.Ld) / (float)total;
@@ -992,7 +996,7 @@ struct Renderer {
                 if (Dot(b.Ld, sc.N) <= 0) continue;
                 Vec3 o = Dot(b.Ld, sc.Ng) < 0 ? OffsetP(sc, -1) : OffsetP(sc, 1);
                 shadowRays++;
-                if (!TraceShadow(tk, o, Scale(b.Ldist * k, b.Ld), 1.0f)) {
+                if (!TraceShadow(tk, o, Scale(b.Ldist * k, b.Ld), 1.0f, sc.Time)) {
                     Spectrum rho = bsdf.Eval(b.Ld);
                     rho.Mul(b.Liu);
                     float p_hat = (float)nB * (float)b.Pdf / (float)total;
@@ -1011,7 +1015,7 @@ struct Renderer {
                 if (Dot(l.Ld, sc.N) <= 0) continue;
                 Vec3 o = Dot(l.Ld, sc.Ng) < 0 ? OffsetP(sc, -1) : OffsetP(sc, 1);
                 shadowRays++;
-                if (!TraceShadow(tk, o, Scale(l.Ldist * k, l.Ld), 1.0f)) {
+                if (!TraceShadow(tk, o, Scale(l.Ldist * k, l.Ld), 1.0f, sc.Time)) {
                     Spectrum rho = bsdf.Eval(l.Ld);
                     rho.Mul(l.Liu);
                     rho.Scale(1.0f / l.Pdf);
@@ -1063,7 +1067,7 @@ struct Renderer {
                 double pdf = spec.PDF(o);
                 if (Dot(o, sg.Ng) > 0.0f) {
                     Ray r{};
-                    r.P = OffsetP(sg, 1); r.D = o; r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = 0;
+                    r.P = OffsetP(sg, 1); r.D = o; r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = sg.Time;
                     RaySetup(r);
                     bool hit;
                     RGB sc = TraceShade(tk, r, level + 1, sg.Lambda, sg.I, sg.Scramble, hit);
@@ -1106,7 +1110,7 @@ struct Renderer {
         if (!hit) return out;
         const HitRecord& h = sh.hr;
         SC sc;
-        sc.geom = sh.geom; sc.Lambda = lambda; sc.I = I; sc.Scramble[0] = scramble[0]; sc.Scramble[1] = scramble[1];
+        sc.geom = sh.geom; sc.Lambda = lambda; sc.Time = r.Time; sc.I = I; sc.Scramble[0] = scramble[0]; sc.Scramble[1] = scramble[1];
         sc.Rd = r.D;
         // ApplyTransform with identity matrices (core/shader.go:129-135)
         sc.P = h.Po;
@@ -1150,6 +1154,7 @@ struct Renderer {
         uint64_t index; double rx, ry;
         RasterXY(12, iter, x, y, 0, 0, index, rx, ry);
         double lambda = (720 - 450) * VanDerCorput(iter, scr[3]) + 450;
+        double time = VanDerCorput(iter, scr[2]);   // core/render.go:93
         if (filtN > 0) {
             double pixu = rx - std::floor(rx), pixv = ry - std::floor(ry);
             double u, v;
@@ -1179,7 +1184,7 @@ struct Renderer {
         Ray r{};
         r.D = Normalize(d);
         r.P = MulPoint(M, e);
-        r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = 0;
+        r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = (float)time;   // sc.Time (render.go:118), Ray.Init
         RaySetup(r);
         RGB out = TraceShade(tk, r, 0, (float)lambda, iter, scr + 4, hit);
         hitGeom = -1;
@@ -1267,6 +1272,31 @@ int orc_scene_add_shaded_mesh(void* sp, const float* verts, int32_t nverts, cons
         m->normals.resize(nverts);
         std::memcpy(m->normals.data(), normals, sizeof(float) * 3 * (size_t)nverts);
         m->normalidx.assign(idx, idx + (size_t)ntris * 3);
+    }
+    if (shaderidx) m->shaderidx.assign(shaderidx, shaderidx + ntris);
+    m->shaders.assign(shaders, shaders + nshaders);
+    s->geoms.push_back(m);
+    return (int)s->geoms.size() - 1;
+}
+
+// Adds a moving mesh with shading data: verts nkeys*nverts*3 (key-major), normals normal_keys*ntris*3
+// per face corner (nullable; trace.go:694-703 uses them only when normal_keys == nkeys), shader index per
+// triangle (nullable), shader table.  motion_bug: the reference's slot indexing (trace.go:527-545).
+int orc_scene_add_shaded_motion_mesh(void* sp, const float* verts, int32_t nverts, int32_t nkeys, const uint32_t* idx,
+                                     int32_t ntris, const float* normals, int32_t normal_keys, const uint8_t* shaderidx,
+                                     const int32_t* shaders, int32_t nshaders, int32_t motion_bug) {
+    Scene* s = (Scene*)sp;
+    Mesh* m = new Mesh();
+    m->nkeys = nkeys; m->elemsPerKey = nverts;
+    m->verts.resize((size_t)nverts * nkeys);
+    std::memcpy(m->verts.data(), verts, sizeof(float) * 3 * (size_t)nverts * nkeys);
+    m->idxp.assign(idx, idx + (size_t)ntris * 3);
+    m->facecount = ntris;
+    m->motionBug = motion_bug != 0;
+    if (normals) {
+        m->normalKeys = normal_keys;
+        m->mnormals.resize((size_t)ntris * 3 * normal_keys);
+        std::memcpy(m->mnormals.data(), normals, sizeof(float) * 9 * (size_t)ntris * normal_keys);
     }
     if (shaderidx) m->shaderidx.assign(shaderidx, shaderidx + ntris);
     m->shaders.assign(shaders, shaders + nshaders);

@@ -76,6 +76,8 @@ SIGNATURES = [
     ("vgpu_scene_set_timing", C.c_int, [_P, _I32]),
     ("vgpu_scene_timing", C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_I32)]),
     ("vgpu_scene_add_shaded_mesh", C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _P, _U32, C.POINTER(_I32)]),
+    ("vgpu_scene_add_shaded_motion_mesh", C.c_int,
+     [_P, _P, _U32, _U32, _P, _U32, _P, _U32, _P, _P, _U32, _U32, C.POINTER(_I32)]),
     ("vgpu_scene_add_sphere", C.c_int, [_P, _P, _F, _I32, C.POINTER(_I32)]),
     ("vgpu_render_create", C.c_int, [C.POINTER(_P), _P, _P, _U32, _P, _U32, _P, _U32, _U32]),
     ("vgpu_render_set_filter", C.c_int, [_P, _I32, _F, _I32, _F]),

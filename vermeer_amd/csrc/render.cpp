@@ -166,7 +166,6 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     if (!out || !s || !materials || !cam) return fail(VGPU_ERR_ARG, "NULL argument");
     *out = nullptr;
     if (!s->committed) return fail(VGPU_ERR_STATE, "render_create before scene commit");
-    if (s->any_motion) return fail(VGPU_ERR_ARG, "the renderer supports static scenes only");
     if (width == 0 || height == 0 || width > 4096 || height > 4096)
         return fail(VGPU_ERR_ARG, "image size %ux%u outside [1,4096] (RasterXY uses 2^12 intervals)", width, height);
     if (s->max_material >= (int32_t)nmaterials)
@@ -303,7 +302,7 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
     for (int level = 0; level <= r->rv.max_level; level++) {
         float4* rin = (level & 1) ? r->d_rays2 : r->d_rays;
         float4* rout = level < r->rv.max_level ? ((level & 1) ? r->d_rays : r->d_rays2) : nullptr;
-        e = vgpu::launch_trace_ext(s->view, false, rin, np, r->d_hits32, s->d_spT, s->d_spN, s->spill_threads, s->d_err,
+        e = vgpu::launch_trace_ext(s->view, s->any_motion, rin, np, r->d_hits32, s->d_spT, s->d_spN, s->spill_threads, s->d_err,
                                    s->num_cus, st);
         if (e != hipSuccess) return fail(VGPU_ERR_HIP, "closest hit: %s", hipGetErrorString(e));
         e = vgpu::launch_shade(r->rv, d_scr, iter, level, rin, r->d_hits32, r->d_pix, r->d_slot_rays, r->d_slot_rgb,

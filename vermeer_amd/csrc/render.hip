@@ -659,6 +659,7 @@ __device__ V3 offset_p(const SC& sc, int dir) {
 struct SlotOut {
     float4* rays;   // 2 float4 per slot
     float4* rgb;    // 1 float4 per slot: colour if unoccluded, w = 1 when a shadow ray is traced
+    float time;     // ray time of the shaded hit, inherited by its shadow rays (core/ray.go:65)
 };
 
 __device__ __forceinline__ void slot_off(const SlotOut& so, int s) {
@@ -668,7 +669,7 @@ __device__ __forceinline__ void slot_off(const SlotOut& so, int s) {
 }
 __device__ __forceinline__ void slot_on(const SlotOut& so, int s, V3 o, V3 d, RGB c) {
     so.rays[2 * s] = make_float4(o.x, o.y, o.z, 1.0f);   // tmax 1: D spans the distance to the light
-    so.rays[2 * s + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    so.rays[2 * s + 1] = make_float4(d.x, d.y, d.z, so.time);
     so.rgb[s] = make_float4(c.r, c.g, c.b, 1.0f);
 }
 
@@ -809,6 +810,7 @@ __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* _
         ry = floor(ry) + 0.5 + v;
     }
     const double lambda = (720 - 450) * vdc(iter, s[3]) + 450;
+    const float time = (float)vdc(iter, s[2]);   // core/render.go:93,118
     const float Sx = (float)(-1.0 + 2.0 * (rx / (double)rv.width));
     const float Sy = -(float)(-1.0 + 2.0 * (ry / (double)rv.height));
     // camera.ComputeRay (camera.go:221-300): pinhole, or a thin lens when Radius > 0
@@ -836,8 +838,14 @@ __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* _
     const V3 P = v3(M[0] * e.x + M[4] * e.y + M[8] * e.z + M[12], M[1] * e.x + M[5] * e.y + M[9] * e.z + M[13],
                     M[2] * e.x + M[6] * e.y + M[10] * e.z + M[14]);
     rays[2 * p] = make_float4(P.x, P.y, P.z, __builtin_inff());
-    rays[2 * p + 1] = make_float4(D.x, D.y, D.z, 0.0f);
+    rays[2 * p + 1] = make_float4(D.x, D.y, D.z, time);
     pix[p] = make_float4(D.x, D.y, D.z, (float)lambda);
+}
+
+// m.Vec3Lerp (math/vec3.go:52-58): (1 - t) a + t b per component
+__device__ __forceinline__ V3 lerp3(const float4& a, const float4& b, float t) {
+    const float s = 1.0f - t;
+    return v3(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z);
 }
 
 // builtin/shader/bsdf/specular.go: perfect mirror, reflect(omegaR, (0,0,1)) = 2 (N . omegaR) N - omegaR
@@ -879,7 +887,8 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= rv.width * rv.height) return;
     const int SGS = 2 * rv.max_half;   // slots per group
-    SlotOut so{slot_rays + (size_t)p * rv.slots * 2, slot_rgb + (size_t)p * rv.slots};
+    const float4 ra = rays_in[2 * p], rb = rays_in[2 * p + 1];
+    SlotOut so{slot_rays + (size_t)p * rv.slots * 2, slot_rgb + (size_t)p * rv.slots, rb.w};
     float4* grp = groups + (size_t)p * rv.groups;
     float4* ps = pixshade + (size_t)p * 4;
     LevelRec* rec = recs + (size_t)p * (kMaxLevel + 1) + level;
@@ -903,7 +912,6 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
         if (rays_out) ray_off(rays_out, p);
         return;
     }
-    const float4 ra = rays_in[2 * p], rb = rays_in[2 * p + 1];
     const V3 Ro = v3(ra.x, ra.y, ra.z), Rd = v3(rb.x, rb.y, rb.z);
     const float thit = __int_as_float(h0.z);
     SC sc;
@@ -917,6 +925,40 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
         sc.Poffset = scale(0.001f, N);
         dPdu = v3(1.0f, 0.0f, 0.0f);
         dPdv = v3(0.0f, 0.0f, 1.0f);
+    } else if (geo.nkeys > 1) {
+        // moving mesh: attributes at the ray's time (builtin/geom/polymesh/trace.go:547-554, 625-703)
+        const float U = __int_as_float(h0.w), V = __int_as_float(h1.x), W = __int_as_float(h1.y);
+        int key, key2;
+        float mt;
+        {
+            const float k = rb.w * (float)(geo.nkeys - 1);   // trace.go:79-84
+            const float fk = floorf(k);
+            mt = k - fk;
+            key = (int)fk;
+            key2 = (int)ceilf(k);
+        }
+        const float4* q = rv.sv.mtris + geo.tri_base + (size_t)prim * 3 * geo.nkeys;
+        const V3 a = lerp3(q[3 * key], q[3 * key2], mt), b = lerp3(q[3 * key + 1], q[3 * key2 + 1], mt),
+                 c = lerp3(q[3 * key + 2], q[3 * key2 + 2], mt);
+        sc.P = v3(U * a.x + V * b.x + W * c.x, U * a.y + V * b.y + W * c.y, U * a.z + V * b.z + W * c.z);
+        const float xs = fabs32(U * a.x) + fabs32(V * b.x) + fabs32(W * c.x);
+        const float ys = fabs32(U * a.y) + fabs32(V * b.y) + fabs32(W * c.y);
+        const float zs = fabs32(U * a.z) + fabs32(V * b.z) + fabs32(W * c.z);
+        const float e00 = b.x - a.x, e01 = b.y - a.y, e02 = b.z - a.z;
+        const float e10 = c.x - a.x, e11 = c.y - a.y, e12 = c.z - a.z;
+        Ng = normalize(v3(e01 * e12 - e02 * e11, e02 * e10 - e00 * e12, e00 * e11 - e01 * e10));
+        dPdu = v3(e00, e01, e02);
+        dPdv = v3(e10, e11, e12);
+        const float g7 = (7.0f * 5.9604645e-08f) / (1.0f - 7.0f * 5.9604645e-08f);   // Gamma(7), math/ferror.go:22-24
+        const float dd = g7 * xs * fabs32(Ng.x) + g7 * ys * fabs32(Ng.y) + g7 * zs * fabs32(Ng.z);
+        sc.Poffset = scale(dd, Ng);
+        N = Ng;
+        if (gs.nrm_base >= 0) {   // per face corner and key, as trace.go:694-703 indexes them
+            const float4* nq = rv.normals + gs.nrm_base + (size_t)prim * 3 * geo.nkeys;
+            const V3 n0 = lerp3(nq[3 * key], nq[3 * key2], mt), n1 = lerp3(nq[3 * key + 1], nq[3 * key2 + 1], mt),
+                     n2 = lerp3(nq[3 * key + 2], nq[3 * key2 + 2], mt);
+            N = normalize(v3(U * n0.x + V * n1.x + W * n2.x, U * n0.y + V * n1.y + W * n2.y, U * n0.z + V * n1.z + W * n2.z));
+        }
     } else {
         const float U = __int_as_float(h0.w), V = __int_as_float(h1.x), W = __int_as_float(h1.y);
         const float4* tp = rv.sv.tris + ((size_t)geo.tri_base + prim) * 3;
@@ -1016,7 +1058,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
             srec = make_float4(f.r, f.g, f.b, sw);
             const V3 po = offset_p(sc, 1);
             rays_out[2 * p] = make_float4(po.x, po.y, po.z, __builtin_inff());
-            rays_out[2 * p + 1] = make_float4(o.x, o.y, o.z, 0.0f);
+            rays_out[2 * p + 1] = make_float4(o.x, o.y, o.z, rb.w);
             kind = 2.0f;
             traced = true;
         }

@@ -340,6 +340,29 @@ int vgpu_scene_add_shaded_mesh(vgpu_scene* s, const float* verts, uint32_t nvert
     return VGPU_OK;
 }
 
+int vgpu_scene_add_shaded_motion_mesh(vgpu_scene* s, const float* verts, uint32_t nverts, uint32_t nkeys,
+                                      const uint32_t* idx, uint32_t ntris, const float* normals, uint32_t normal_keys,
+                                      const uint8_t* shaderidx, const int32_t* shaders, uint32_t nshaders, uint32_t flags,
+                                      int32_t* geom_out) {
+    if (nkeys < 2) return fail(VGPU_ERR_ARG, "a moving mesh has at least 2 keys (vgpu_scene_add_shaded_mesh for static)");
+    if ((flags & ~VGPU_MESH_MOTION_REFERENCE_INDEXING) != 0) return fail(VGPU_ERR_ARG, "unknown mesh flags 0x%x", flags);
+    if (normals && normal_keys == 0) return fail(VGPU_ERR_ARG, "normals with 0 keys");
+    if (!shaders || nshaders == 0) return fail(VGPU_ERR_ARG, "a shaded mesh needs a shader table");
+    for (uint32_t i = 0; i < nshaders; i++)
+        if (shaders[i] < 0) return fail(VGPU_ERR_ARG, "shader %u has negative material id", i);
+    if (shaderidx && ntris)
+        for (uint32_t i = 0; i < ntris; i++)
+            if (shaderidx[i] >= nshaders) return fail(VGPU_ERR_ARG, "triangle %u uses shader %u of %u", i, shaderidx[i], nshaders);
+    int rc = vgpu_scene_add_mesh(s, verts, nverts, nkeys, idx, ntris, 0.0f, flags, geom_out);
+    if (rc) return rc;
+    MeshHost& m = s->meshes.back();
+    // trace.go:694: normals with another key count than the vertices are not used (N = Ng)
+    if (normals && normal_keys == nkeys) m.mnormals.assign(normals, normals + (size_t)nkeys * ntris * 9);
+    if (shaderidx) m.shaderidx.assign(shaderidx, shaderidx + ntris);
+    m.shaders.assign(shaders, shaders + nshaders);
+    return VGPU_OK;
+}
+
 int vgpu_scene_add_sphere(vgpu_scene* s, const float* centre, float radius, int32_t material, int32_t* geom_out) {
     if (!s || !centre) return fail(VGPU_ERR_ARG, "NULL argument");
     if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
@@ -538,7 +561,27 @@ int vgpu_scene_commit(vgpu_scene* s) {
                         dn.push_back(make_float4(nv[0], nv[1], nv[2], 0.0f));
                     }
             }
-            if (!m.shaderidx_sorted.empty()) {
+            if (m.nkeys > 1 && !m.sphere) {
+                // Moving mesh: leaf slot i shades face f = accel.idx[i], or f = i with the reference's
+                // indexing (trace.go:527-545 reads idxp, shaderidx and Normals at the slot).  Normals
+                // per slot: nkeys groups of the face's 3 corners, as the motion triangles are laid out.
+                const bool ref_idx = (m.flags & VGPU_MESH_MOTION_REFERENCE_INDEXING) != 0;
+                if (!m.mnormals.empty()) {
+                    gsd.nrm_base = (int32_t)dn.size();
+                    for (uint32_t i = 0; i < m.ntris; i++) {
+                        const uint32_t f = ref_idx ? i : (uint32_t)m.perm[i];
+                        for (uint32_t kk = 0; kk < m.nkeys; kk++)
+                            for (int j = 0; j < 3; j++) {
+                                const float* nv = &m.mnormals[(((size_t)kk * m.ntris + f) * 3 + j) * 3];
+                                dn.push_back(make_float4(nv[0], nv[1], nv[2], 0.0f));
+                            }
+                    }
+                }
+                if (!m.shaderidx.empty()) {
+                    gsd.sidx_base = (int32_t)dsi.size();
+                    for (uint32_t i = 0; i < m.ntris; i++) dsi.push_back(m.shaderidx[ref_idx ? i : (uint32_t)m.perm[i]]);
+                }
+            } else if (!m.shaderidx_sorted.empty()) {
                 gsd.sidx_base = (int32_t)dsi.size();
                 dsi.insert(dsi.end(), m.shaderidx_sorted.begin(), m.shaderidx_sorted.end());
             }
@@ -547,6 +590,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
             dsh.insert(dsh.end(), m.shaders.begin(), m.shaders.end());
             for (int32_t mid : m.shaders) s->max_material = std::max(s->max_material, mid);
         }
+        if (dn.size() > 0x7fffffffull) return fail(VGPU_ERR_ARG, "scene normals too large for 32-bit offsets");
         if (dsh.empty()) dsh.push_back(-1);
         if ((rc = upload(&s->d_normals, dn))) return rc;
         if ((rc = upload(&s->d_shaderidx, dsi))) return rc;

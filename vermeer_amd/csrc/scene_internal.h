@@ -67,6 +67,7 @@ struct MeshHost {
     float bounds[6];
     std::vector<float> motion_bounds;  // nkeys * 6
     std::vector<float> normals;        // nverts * 3 (shaded meshes), empty: N = Ng
+    std::vector<float> mnormals;       // moving shaded meshes: nkeys * ntris * 3 corners * 3, empty: N = Ng
     std::vector<uint8_t> shaderidx;    // per triangle (insertion order; reordered at build)
     std::vector<uint8_t> shaderidx_sorted;
     std::vector<int32_t> shaders;      // shader table: material ids

@@ -1,7 +1,7 @@
 """Renderer: the reference's render pass loop on the GPU.
 
-Mirrors core.Render (core/render.go:140-218) for a scene of shaded meshes,
-analytic spheres and Tri / Disk / Sphere lights: create once per frame
+Mirrors core.Render (core/render.go:140-218) for a scene of shaded meshes (static or
+moving), analytic spheres and Tri / Disk / Sphere lights: create once per frame
 setup, then one pass() per iteration with the 1-based pass number the
 reference hands to render().
 """
@@ -72,6 +72,9 @@ class Renderer:
             if mesh.get("sphere"):
                 self.scene.add_sphere(mesh["centre"], mesh["radius"], mesh["material"])
                 continue
+            if int(mesh.get("keys", 1)) > 1:
+                self._add_motion_mesh(mesh)
+                continue
             v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(-1, 3)
             ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
             nrm = mesh.get("normals")
@@ -92,6 +95,27 @@ class Renderer:
                                            ptr(cam), width, height))
         self._h = h
         self.film = np.zeros((height, width, 3), np.float32)
+
+    def _add_motion_mesh(self, mesh):
+        """A moving shaded mesh: verts (keys, nverts, 3); normals (normal keys, ntris * 3, 3) per face
+        corner or None; shaderidx, shaders; reference_indexing selects the reference's slot indexing."""
+        nk = int(mesh["keys"])
+        v = np.ascontiguousarray(mesh["verts"], np.float32).reshape(nk, -1, 3)
+        ix = np.ascontiguousarray(mesh["idx"], np.uint32).reshape(-1, 3)
+        nrm = mesh.get("normals")
+        nrm_keys = 0
+        if nrm is not None:
+            nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, len(ix) * 3, 3)
+            nrm_keys = nrm.shape[0]
+        sidx = mesh.get("shaderidx")
+        sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
+        sh = np.ascontiguousarray(mesh["shaders"], np.int32)
+        flags = _lib.VGPU_MESH_MOTION_REFERENCE_INDEXING if mesh.get("reference_indexing") else 0
+        g = C.c_int32(-1)
+        check(self._lib.vgpu_scene_add_shaded_motion_mesh(self.scene._h, ptr(v), v.shape[1], nk, ptr(ix), len(ix),
+                                                          ptr(nrm), nrm_keys, ptr(sidx), ptr(sh), len(sh), flags,
+                                                          C.byref(g)))
+        self.scene.geoms.append(None)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -486,6 +486,85 @@ def feature_scene(seed=5, nseg=48, nring=24, mirrors=True):
             "camera": camera}
 
 
+def _rot_y(theta):
+    c, s_ = np.cos(theta), np.sin(theta)
+    return np.array([[c, 0.0, s_], [0.0, 1.0, 0.0], [-s_, 0.0, c]])
+
+
+def moving_scene(seed=7, nseg=32, nring=16, keys=3, mirrors=True):
+    """Motion-blur room for the renderer: the feature scene's box (static) and lights, with three moving
+    meshes.  Mesh "spheres": three UV spheres (diffuse, conductor GGX, perfect mirror) that translate and
+    spin about y over `keys` keys, with per-corner normals for every key.  Mesh "blob": one sphere with a
+    single-key normal array (ignored: N = Ng) and the reference's slot indexing, two shaders by
+    hemisphere.  Geom insertion order: box, spheres, blob, disk fan."""
+    g = rng(seed)
+    base = feature_scene(seed=seed, nseg=8, nring=4, mirrors=mirrors)
+    mats = [base["materials"][0]]
+    box_v, box_f = [], []
+    nv = 0
+    for v, f in [quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)),
+                 quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),
+                 quad((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)),
+                 quad((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)),
+                 quad((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1))]:
+        n = np.cross(v[f[0, 1]] - v[f[0, 0]], v[f[0, 2]] - v[f[0, 0]])
+        if np.dot(n, np.array([0.0, 1.0, 0.0]) - v[0]) < 0:
+            f = f[:, ::-1]
+        box_v.append(v)
+        box_f.append(f + nv)
+        nv += len(v)
+    box = {"verts": np.concatenate(box_v).astype(np.float32), "idx": np.concatenate(box_f).astype(np.uint32),
+           "shaderidx": None, "shaders": np.array([0], np.int32), "normals": None}
+    sphere_mats = [
+        material(diffuse=(0.8, 0.3, 0.2), diffuse_strength=1.0, diffuse_roughness=0.2),
+        material(diffuse=(0.3, 0.3, 0.3), diffuse_strength=0.2, spec=(1.0, 0.9, 0.8), spec_strength=0.8,
+                 spec_roughness=0.25, fresnel="metal", fresnel_refl=(0.95, 0.64, 0.54)),
+        material(spec=(1.0, 1.0, 1.0), spec_strength=1.0, spec_roughness=0.0 if mirrors else 0.05, ior=1.5),
+    ]
+    sv, sf = uv_sphere(1.0, nseg, nring)
+    sv = sv.astype(np.float64)
+    corners = sf.reshape(-1).astype(np.int64)
+    kv, kn, faces, sidx = [[] for _ in range(keys)], [[] for _ in range(keys)], [], []
+    nv = 0
+    for k, m in enumerate(sphere_mats):
+        c0 = np.array([-0.5 + 0.5 * k, 0.35 + 0.1 * k, -0.2 + 0.1 * k])
+        move = g.uniform(-0.25, 0.25, 3)
+        spin = g.uniform(0.3, 1.2)
+        for key in range(keys):
+            t = key / (keys - 1)
+            R = _rot_y(spin * t)
+            r = 0.25 * (1.0 + 0.2 * t)
+            kv[key].append((sv @ R.T) * r + c0 + move * t)
+            kn[key].append((sv @ R.T)[corners])
+        mats.append(m)
+        faces.append(sf.astype(np.int64) + nv)
+        sidx.append(np.full(len(sf), k, np.int64))
+        nv += len(sv)
+    spheres = {"keys": keys, "verts": np.stack([np.concatenate(v) for v in kv]).astype(np.float32),
+               "idx": np.concatenate(faces).astype(np.uint32), "shaderidx": np.concatenate(sidx).astype(np.uint8),
+               "shaders": np.arange(1, 1 + len(sphere_mats), dtype=np.int32),
+               "normals": np.stack([np.concatenate(n) for n in kn]).astype(np.float32)}
+    blob_mats = [material(diffuse=(0.2, 0.7, 0.3), diffuse_strength=1.0),
+                 material(diffuse=(0.2, 0.3, 0.8), diffuse_strength=0.5, spec=(1.0, 1.0, 1.0), spec_strength=0.5,
+                          spec_roughness=0.4)]
+    bv = [sv * 0.18 + np.array([0.55, 0.25 + 0.3 * key / (keys - 1), 0.45]) for key in range(keys)]
+    blob = {"keys": keys, "verts": np.stack(bv).astype(np.float32), "idx": sf.astype(np.uint32),
+            "shaderidx": (sv[sf[:, 0], 1] > 0).astype(np.uint8),
+            "shaders": np.array([len(mats), len(mats) + 1], np.int32),
+            "normals": sv[corners][None].astype(np.float32), "reference_indexing": True}
+    mats += blob_mats
+    light_mat = len(mats)
+    mats.append(material(emission=(1.0, 0.95, 0.9), emission_strength=8.0, diffuse=(0.0, 0.0, 0.0),
+                         diffuse_strength=1.0))
+    dP, dUp, dLook, dR = (0.0, 1.99, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.35
+    dv, di, dn = disk_light_mesh(dP, dUp, dLook, dR, 20)
+    disk = {"verts": dv, "idx": di, "shaderidx": None, "shaders": np.array([light_mat], np.int32), "normals": dn}
+    lights = [{"type": "disk", "P": dP, "Up": dUp, "LookAt": dLook, "radius": dR, "material": light_mat,
+               "samples": 2, "geom": 3}]
+    return {"meshes": [box, spheres, blob, disk], "materials": np.array(mats, MATERIAL_DTYPE), "lights": lights,
+            "camera": base["camera"]}
+
+
 def pixel_scrambles(width, height, seed=99):
     """6 uint64 per pixel: lensU, lensV, time, lambda, scramble[0], scramble[1] (core/render.go:166-176)."""
     return rng(seed).integers(0, np.iinfo(np.uint64).max, (height, width, 6), dtype=np.uint64, endpoint=True)
