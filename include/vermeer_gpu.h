@@ -282,6 +282,16 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* scene, const vgpu_material
  * filter.CreateSampler does (filter.go:87-173) and samples are warped as
  * Sampler.WarpSample (filter.go:40-85), quirks included. */
 int vgpu_render_set_filter(vgpu_render* render, int32_t type, float width, int32_t res, float peak);
+
+/* Camera motion keys: nkeys column-major LocalToWorld matrices (the
+ * camera's LocalToWorld.Elems as calcLookatMatrices or matrixCalc leave
+ * them, builtin/camera/camera.go:109-193).  Each is decomposed on the host as
+ * TransformDecompMatrix4 (math/animdecomp.go:20-63).  With two or more keys
+ * every sample's camera matrix is the lerp of the two keys bracketing its
+ * time, recomposed as ComputeRay does (camera.go:224-236); one key sets the
+ * single matrix of every sample to that recomposition (replacing the
+ * vgpu_camera matrix); 0 keys keep the vgpu_camera matrix.  At most 64 keys. */
+int vgpu_render_set_camera_keys(vgpu_render* render, const float* local_to_world, uint32_t nkeys);
 #define VGPU_FILTER_BOX 0
 #define VGPU_FILTER_GAUSSIAN 1
 #define VGPU_FILTER_AIRY 2

@@ -216,6 +216,9 @@ def _render_sigs(lib):
         "orc_spectrum_to_rgb": (None, [P, C.c_float, P]),
         "orc_cie_tables": (None, [P, P, P]),
         "orc_scene_add_shaded_mesh": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, P, P, C.c_int32]),
+        "orc_camera_lookat_keys": (C.c_int, [P, C.c_int32, P, C.c_int32, P, P, C.c_int32, P]),
+        "orc_render_set_camera_keys": (None, [P, P, C.c_int32]),
+        "orc_decomp_lerp_matrix": (None, [P, P, C.c_float, P]),
         "orc_scene_add_shaded_motion_mesh": (C.c_int, [P, P, C.c_int32, C.c_int32, P, C.c_int32, P, C.c_int32, P, P,
                                                        C.c_int32, C.c_int32]),
         "orc_render_new": (P, [P, C.c_int32, C.c_int32, P, C.c_float, C.c_float, C.c_float, P, C.c_int32]),
@@ -249,6 +252,29 @@ def camera_lookat(frm, to, up, fov_deg, focal=12.0):
     u = np.asarray(up, np.float32)
     lib.orc_camera_lookat(_p(f), _p(t), _p(u), float(fov_deg), float(focal), _p(M), _p(ttf))
     return M, float(ttf[0])
+
+
+def camera_lookat_keys(from_keys, to_keys, up, roll=None):
+    """Camera.calcLookatMatrices (camera.go:109-193): (nkeys, 16) LocalToWorld matrices for keyed From /
+    To (and Roll, radians)."""
+    lib = load(); _render_sigs(lib)
+    f = np.ascontiguousarray(from_keys, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(to_keys, np.float32).reshape(-1, 3)
+    u = np.ascontiguousarray(up, np.float32)
+    r = None if roll is None else np.ascontiguousarray(roll, np.float32).reshape(-1)
+    out = np.zeros((max(len(f), len(t)), 16), np.float32)
+    n = lib.orc_camera_lookat_keys(_p(f), len(f), _p(t), len(t), _p(u), _p(r), 0 if r is None else len(r), _p(out))
+    return out[:n]
+
+
+def decomp_lerp_matrix(a, b, t):
+    """TransformDecompToMatrix4(TransformDecompLerp(decomp(a), decomp(b), t)) (animdecomp.go:20-80)."""
+    lib = load(); _render_sigs(lib)
+    a = np.ascontiguousarray(a, np.float32).reshape(16)
+    b = np.ascontiguousarray(b, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    lib.orc_decomp_lerp_matrix(_p(a), _p(b), float(t), _p(out))
+    return out
 
 
 def vdc(i, s=0):
@@ -390,6 +416,11 @@ class OracleRenderer:
         out = np.zeros(n + n * n, np.float64)
         self.lib.orc_render_filter_tables(self.h_r, _p(out))
         return out[:n], out[n:].reshape(n, n)
+
+    def set_camera_keys(self, mats):
+        m = np.ascontiguousarray(mats, np.float32).reshape(-1, 16)
+        self._keep.append(m)
+        self.lib.orc_render_set_camera_keys(self.h_r, _p(m), len(m))
 
     def render_pass(self, scrambles, iteration, nthreads=0, want_sample=False):
         scr = np.ascontiguousarray(scrambles, np.uint64)

@@ -413,22 +413,26 @@ static Decomp DecompM4(const M4 min) {
     float sgn = M4Det(m) >= 0.0f ? 1.0f : -1.0f;
     for (int i = 0; i < 3; i++) d.T[i] = m[3 * 4 + i];
     m[12] = 0; m[13] = 0; m[14] = 0;
-    if (sgn < 0.0f) for (int i = 0; i < 16; i++) m[i] = -1.0f * m[i];   // Matrix4Mul(Scale(-1, I), m)
+    // Matrix4Mul(Matrix4Scale(-1, Identity), m) as the reference multiplies it: the zeros of -I are -0,
+    // and the sums start from +0, which fixes the sign of the zero entries.
+    M4 negI; M4Identity(negI);
+    for (int i = 0; i < 16; i++) negI[i] = -1.0f * negI[i];
+    if (sgn < 0.0f) M4Mul(negI, m, m);
     M4 Q;
     if (!M4PolarFactor(m, Q)) { d.R = Quat{0, 0, 0, 1}; M4Identity(d.S); return d; }
     M4 QT; M4Transpose(Q, QT);
     M4 S; M4Mul(QT, m, S);
     d.R = M4ToQuat(Q);
-    if (sgn < 0.0f) { for (int i = 0; i < 16; i++) d.S[i] = -1.0f * S[i]; d.S[15] = 1; }
+    if (sgn < 0.0f) { M4Mul(negI, S, d.S); d.S[15] = 1; }
     else std::memcpy(d.S, S, sizeof(M4));
     return d;
 }
-// animdecomp.go:66-80 with a == b (a static camera: time t reduces to 0)
-static void DecompToM4(const Decomp& a, float t, M4 out) {
+// TransformDecompLerp then TransformDecompToMatrix4 (animdecomp.go:66-80); a static camera has a == b
+static void DecompToM4(const Decomp& a, const Decomp& b, float t, M4 out) {
     Decomp l;
-    l.T = Lerp(a.T, a.T, t);
-    l.R = QuatSlerp(a.R, a.R, t);
-    for (int i = 0; i < 16; i++) l.S[i] = (1.0f - t) * a.S[i] + t * a.S[i];
+    l.T = Lerp(a.T, b.T, t);
+    l.R = QuatSlerp(a.R, b.R, t);
+    for (int i = 0; i < 16; i++) l.S[i] = (1.0f - t) * a.S[i] + t * b.S[i];   // Matrix4Lerp
     M4 Tm; M4Identity(Tm); Tm[12] = l.T[0]; Tm[13] = l.T[1]; Tm[14] = l.T[2];
     M4 R; QuatToM4(l.R, R);
     M4 RS; M4Mul(R, l.S, RS);
@@ -748,6 +752,7 @@ struct Renderer {
     Scene* scene;
     int width, height;
     M4 M;
+    std::vector<Decomp> camKeys;  // camera.decomp with two or more motion keys; empty: M for every sample
     float tanThetaFocal, aspect, focal;
     std::vector<Material> mats;
     std::vector<Light> lights;
@@ -1180,10 +1185,19 @@ struct Renderer {
             float xo = lensRadius * rr * fCos(theta), yo = lensRadius * rr * fSin(theta);
             e = Add(Scale(xo, V3(1, 0, 0)), Scale(yo, V3(0, 1, 0)));
         }
-        Vec3 d = MulVec(M, Sub(s, e));
+        // camera.go:224-236: the matrix of the lerped decomposition at the sample's time
+        M4 Ms;
+        std::memcpy(Ms, M, sizeof(M4));
+        if (camKeys.size() >= 2) {
+            float k = (float)time * (float)(camKeys.size() - 1);
+            float t = k - Floor(k);
+            int key = (int)Floor(k), key2 = (int)Ceil(k);
+            DecompToM4(camKeys[key], camKeys[key2], t, Ms);
+        }
+        Vec3 d = MulVec(Ms, Sub(s, e));
         Ray r{};
         r.D = Normalize(d);
-        r.P = MulPoint(M, e);
+        r.P = MulPoint(Ms, e);
         r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = (float)time;   // sc.Time (render.go:118), Ray.Init
         RaySetup(r);
         RGB out = TraceShade(tk, r, 0, (float)lambda, iter, scr + 4, hit);
@@ -1216,8 +1230,64 @@ void orc_camera_lookat(const float* from, const float* to, const float* up, floa
     B[0] = U[0]; B[1] = U[1]; B[2] = U[2]; B[4] = V[0]; B[5] = V[1]; B[6] = V[2]; B[8] = W[0]; B[9] = W[1]; B[10] = W[2]; B[15] = 1;
     M4 mtx; M4Mul(Tr, B, mtx);
     Decomp d = DecompM4(mtx);
-    DecompToM4(d, 0.0f, M_out);
+    DecompToM4(d, d, 0.0f, M_out);
     *tan_theta_focal = fTan((fov_deg / 2) * Pi32 / 180.0f) * focal;
+}
+
+// Camera.calcLookatMatrices (camera.go:109-193) for keyed From / To / Roll: LocalToWorld per key of
+// the parameter with more keys (key i at time i / nkeys, as the reference samples the other one).
+// Writes max(nfrom, nto) column-major matrices and returns their count.
+int orc_camera_lookat_keys(const float* from, int32_t nfrom, const float* to, int32_t nto, const float* up,
+                           const float* roll, int32_t nroll, float* mats_out) {
+    Vec3 Up = V3(up[0], up[1], up[2]);
+    const bool byTarget = nto > nfrom;
+    const int n = byTarget ? nto : nfrom;
+    for (int i = 0; i < n; i++) {
+        float time = (float)i / (float)n;
+        const float* other = byTarget ? from : to;
+        int nother = byTarget ? nfrom : nto;
+        float k = time * (float)(nother - 1);
+        float t = k - Floor(k);
+        int key = (int)Floor(k), key2 = (int)Ceil(k);
+        Vec3 Pl = Lerp(V3(other[key * 3], other[key * 3 + 1], other[key * 3 + 2]),
+                       V3(other[key2 * 3], other[key2 * 3 + 1], other[key2 * 3 + 2]), t);
+        const float* own = byTarget ? to : from;
+        Vec3 Pi = V3(own[i * 3], own[i * 3 + 1], own[i * 3 + 2]);
+        Vec3 F = byTarget ? Pl : Pi, T = byTarget ? Pi : Pl;
+        Vec3 W = Normalize(Sub(F, T));
+        Vec3 u = Normalize(Cross(Up, W));
+        Vec3 v = Normalize(Cross(u, W));
+        float rl = 0;
+        if (roll && nroll > 0) {
+            float kr = time * (float)(nroll - 1);
+            float tr = kr - Floor(kr);
+            int r0 = (int)Floor(kr), r1 = (int)Ceil(kr);
+            rl = (1 - tr) * roll[r0] + tr * roll[r1];
+        }
+        Vec3 U = Add(Scale(fCos(rl), u), Scale(fSin(rl), v));
+        Vec3 V = Add(Scale(-fSin(rl), u), Scale(fCos(rl), v));
+        M4 Tr; M4Identity(Tr); Tr[12] = F[0]; Tr[13] = F[1]; Tr[14] = F[2];
+        M4 B; std::memset(B, 0, sizeof(M4));
+        B[0] = U[0]; B[1] = U[1]; B[2] = U[2]; B[4] = V[0]; B[5] = V[1]; B[6] = V[2]; B[8] = W[0]; B[9] = W[1]; B[10] = W[2]; B[15] = 1;
+        M4Mul(Tr, B, mats_out + (size_t)i * 16);
+    }
+    return n;
+}
+
+// The camera's LocalToWorld keys: TransformDecompMatrix4 of each (camera.go:188-192).  One key sets the
+// static matrix (the decomposition recomposed at t = 0); two or more make the matrix per sample.
+void orc_render_set_camera_keys(void* rp, const float* mats, int32_t nkeys) {
+    Renderer* r = (Renderer*)rp;
+    r->camKeys.clear();
+    for (int i = 0; i < nkeys; i++) r->camKeys.push_back(DecompM4(mats + (size_t)i * 16));
+    if (nkeys == 1) { DecompToM4(r->camKeys[0], r->camKeys[0], 0.0f, r->M); r->camKeys.clear(); }
+}
+
+// TransformDecompMatrix4 then TransformDecompToMatrix4 of the lerp of two decompositions at t
+// (animdecomp.go:20-80), for known-answer tests.
+void orc_decomp_lerp_matrix(const float* a, const float* b, float t, float* out) {
+    Decomp da = DecompM4(a), db = DecompM4(b);
+    DecompToM4(da, db, t, out);
 }
 
 // VdC / Sobol / RasterXY for known-answer tests

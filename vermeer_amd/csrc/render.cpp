@@ -7,6 +7,8 @@
 // buffers are allocated once at creation; a pass allocates nothing except,
 // in host-pointer mode, a staging area the first time it is needed.
 
+#include <array>
+#include <cmath>
 #include <new>
 
 #include "scene_internal.h"
@@ -126,6 +128,151 @@ void create_sampler(int n, double w, F f, std::vector<double>& out) {
     }
 }
 
+// ---- camera motion keys: TransformDecompMatrix4 (math/animdecomp.go:20-63) on the host, in float32
+// with the reference's operation order (element (i, j) of a column-major matrix at [j * 4 + i]).
+
+using Mat4 = std::array<float, 16>;
+
+Mat4 mat_mul(const Mat4& a, const Mat4& b) {   // Matrix4Mul (math/matrix4.go:92-102): sums from +0
+    Mat4 c{};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 4; k++) c[j * 4 + i] += a[k * 4 + i] * b[j * 4 + k];
+    return c;
+}
+
+Mat4 mat_transpose(const Mat4& a) {
+    Mat4 c{};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) c[i * 4 + j] = a[j * 4 + i];
+    return c;
+}
+
+// Cofactor expansion of Matrix4Inverse / Matrix4Det (math/matrix4.go:116-180); the terms and their
+// order fix the rounding, so they are written as the reference writes them.
+Mat4 mat_cofactors(const Mat4& m) {
+    Mat4 c{};
+    c[0] = m[5]*m[10]*m[15] - m[5]*m[11]*m[14] - m[9]*m[6]*m[15] + m[9]*m[7]*m[14] + m[13]*m[6]*m[11] - m[13]*m[7]*m[10];
+    c[4] = -m[4]*m[10]*m[15] + m[4]*m[11]*m[14] + m[8]*m[6]*m[15] - m[8]*m[7]*m[14] - m[12]*m[6]*m[11] + m[12]*m[7]*m[10];
+    c[8] = m[4]*m[9]*m[15] - m[4]*m[11]*m[13] - m[8]*m[5]*m[15] + m[8]*m[7]*m[13] + m[12]*m[5]*m[11] - m[12]*m[7]*m[9];
+    c[12] = -m[4]*m[9]*m[14] + m[4]*m[10]*m[13] + m[8]*m[5]*m[14] - m[8]*m[6]*m[13] - m[12]*m[5]*m[10] + m[12]*m[6]*m[9];
+    c[1] = -m[1]*m[10]*m[15] + m[1]*m[11]*m[14] + m[9]*m[2]*m[15] - m[9]*m[3]*m[14] - m[13]*m[2]*m[11] + m[13]*m[3]*m[10];
+    c[5] = m[0]*m[10]*m[15] - m[0]*m[11]*m[14] - m[8]*m[2]*m[15] + m[8]*m[3]*m[14] + m[12]*m[2]*m[11] - m[12]*m[3]*m[10];
+    c[9] = -m[0]*m[9]*m[15] + m[0]*m[11]*m[13] + m[8]*m[1]*m[15] - m[8]*m[3]*m[13] - m[12]*m[1]*m[11] + m[12]*m[3]*m[9];
+    c[13] = m[0]*m[9]*m[14] - m[0]*m[10]*m[13] - m[8]*m[1]*m[14] + m[8]*m[2]*m[13] + m[12]*m[1]*m[10] - m[12]*m[2]*m[9];
+    c[2] = m[1]*m[6]*m[15] - m[1]*m[7]*m[14] - m[5]*m[2]*m[15] + m[5]*m[3]*m[14] + m[13]*m[2]*m[7] - m[13]*m[3]*m[6];
+    c[6] = -m[0]*m[6]*m[15] + m[0]*m[7]*m[14] + m[4]*m[2]*m[15] - m[4]*m[3]*m[14] - m[12]*m[2]*m[7] + m[12]*m[3]*m[6];
+    c[10] = m[0]*m[5]*m[15] - m[0]*m[7]*m[13] - m[4]*m[1]*m[15] + m[4]*m[3]*m[13] + m[12]*m[1]*m[7] - m[12]*m[3]*m[5];
+    c[14] = -m[0]*m[5]*m[14] + m[0]*m[6]*m[13] + m[4]*m[1]*m[14] - m[4]*m[2]*m[13] - m[12]*m[1]*m[6] + m[12]*m[2]*m[5];
+    c[3] = -m[1]*m[6]*m[11] + m[1]*m[7]*m[10] + m[5]*m[2]*m[11] - m[5]*m[3]*m[10] - m[9]*m[2]*m[7] + m[9]*m[3]*m[6];
+    c[7] = m[0]*m[6]*m[11] - m[0]*m[7]*m[10] - m[4]*m[2]*m[11] + m[4]*m[3]*m[10] + m[8]*m[2]*m[7] - m[8]*m[3]*m[6];
+    c[11] = -m[0]*m[5]*m[11] + m[0]*m[7]*m[9] + m[4]*m[1]*m[11] - m[4]*m[3]*m[9] - m[8]*m[1]*m[7] + m[8]*m[3]*m[5];
+    c[15] = m[0]*m[5]*m[10] - m[0]*m[6]*m[9] - m[4]*m[1]*m[10] + m[4]*m[2]*m[9] + m[8]*m[1]*m[6] - m[8]*m[2]*m[5];
+    return c;
+}
+
+float mat_det(const Mat4& m, const Mat4& c) { return m[0] * c[0] + m[1] * c[4] + m[2] * c[8] + m[3] * c[12]; }
+
+// Matrix4PolarFactor (math/matrix4.go:343-363): Q <- (Q + Q^-T) / 2 until no entry grows by more than
+// 1e-6 (the reference's one-sided test), at most 10 steps.
+bool polar_factor(const Mat4& m, Mat4& Q) {
+    Q = m;
+    for (int it = 0; it < 10; it++) {
+        const Mat4 c = mat_cofactors(Q);
+        float det = mat_det(Q, c);
+        if (det == 0.0f) return false;
+        det = 1.0f / det;
+        Mat4 inv{};
+        for (int i = 0; i < 16; i++) inv[i] = c[i] * det;
+        const Mat4 invT = mat_transpose(inv);
+        Mat4 next{};
+        for (int i = 0; i < 16; i++) next[i] = 0.5f * (Q[i] + invT[i]);
+        bool same = true;
+        for (int i = 0; i < 16 && same; i++) same = !(next[i] - Q[i] > 0.000001f);
+        Q = next;
+        if (same) return true;
+    }
+    return false;
+}
+
+// TransformDecompMatrix4: T from column 3; a negative determinant factored out as -I; polar factor Q;
+// S = Q^T m; R = Matrix4ToQuat(Q) (math/quat.go:133-163).  Writes one kCamKeyFloats record.
+void decompose_key(Mat4 m, float* rec) {
+    const float sgn = mat_det(m, mat_cofactors(m)) >= 0.0f ? 1.0f : -1.0f;   // sign() (octanormal.go:16-23)
+    for (int i = 0; i < vgpu::kCamKeyFloats; i++) rec[i] = 0.0f;
+    rec[0] = m[12]; rec[1] = m[13]; rec[2] = m[14];
+    m[12] = m[13] = m[14] = 0.0f;
+    Mat4 negI{};
+    for (int i = 0; i < 16; i++) negI[i] = -1.0f * ((i % 5 == 0) ? 1.0f : 0.0f);   // Matrix4Scale(-1, Identity)
+    if (sgn < 0.0f) m = mat_mul(negI, m);
+    Mat4 Q;
+    float* R = rec + 3;
+    float* S = rec + 7;
+    if (!polar_factor(m, Q)) {   // the reference falls back to the identity rotation and scale
+        R[3] = 1.0f;
+        S[0] = S[5] = S[10] = S[15] = 1.0f;
+        return;
+    }
+    Mat4 Sm = mat_mul(mat_transpose(Q), m);
+    auto E = [&](int i, int j) { return Q[j * 4 + i]; };
+    const float tr = E(0, 0) + E(1, 1) + E(2, 2);
+    float& X = R[0]; float& Y = R[1]; float& Z = R[2]; float& W = R[3];
+    if (tr > 0.0f) {
+        const float q = std::sqrt(tr + 1.0f) * 2;
+        W = 0.25f * q; X = (E(2, 1) - E(1, 2)) / q; Y = (E(0, 2) - E(2, 0)) / q; Z = (E(1, 0) - E(0, 1)) / q;
+    } else if ((E(0, 0) > E(1, 1)) && (E(0, 0) > E(2, 2))) {
+        const float q = std::sqrt(1.0f + E(0, 0) - E(1, 1) - E(2, 2)) * 2;
+        W = (E(2, 1) - E(1, 2)) / q; X = 0.25f * q; Y = (E(0, 1) + E(1, 0)) / q; Z = (E(0, 2) + E(2, 0)) / q;
+    } else if (E(1, 1) > E(2, 2)) {
+        const float q = std::sqrt(1.0f + E(1, 1) - E(0, 0) - E(2, 2)) * 2;
+        W = (E(0, 2) - E(2, 0)) / q; X = (E(0, 1) + E(1, 0)) / q; Y = 0.25f * q; Z = (E(1, 2) + E(2, 1)) / q;
+    } else {
+        const float q = std::sqrt(1.0f + E(2, 2) - E(0, 0) - E(1, 1)) * 2;
+        W = (E(1, 0) - E(0, 1)) / q; X = (E(0, 2) + E(2, 0)) / q; Y = (E(1, 2) + E(2, 1)) / q; Z = 0.25f * q;
+    }
+    if (sgn < 0.0f) {
+        Sm = mat_mul(negI, Sm);
+        Sm[15] = 1.0f;
+    }
+    for (int i = 0; i < 16; i++) S[i] = Sm[i];
+}
+
+// TransformDecompLerp then TransformDecompToMatrix4 (math/animdecomp.go:66-80) of two key records, with
+// QuatSlerp (math/quat.go:30-67) and QuatToMatrix4 (quat.go:80-104); k_camera runs the same steps.
+Mat4 decomp_matrix(const float* a, const float* b, float t) {
+    float T[3], q[4], S[16];
+    for (int i = 0; i < 3; i++) T[i] = (1.0f - t) * a[i] + t * b[i];
+    const float* qa = a + 3;
+    const float* qb = b + 3;
+    const float c = qa[3] * qb[3] + qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2];
+    if (std::fabs(c) >= 1.0f) {
+        for (int i = 0; i < 4; i++) q[i] = qa[i];
+    } else {
+        const float half = (float)std::acos((double)c);
+        const float sh = std::sqrt(1.0f - c * c);
+        if (std::fabs(sh) < 0.001f) {
+            for (int i = 0; i < 4; i++) q[i] = qa[i] * 0.5f + qb[i] * 0.5f;
+        } else {
+            const float ra = (float)std::sin((double)((1 - t) * half)) / sh;
+            const float rb = (float)std::sin((double)(t * half)) / sh;
+            for (int i = 0; i < 4; i++) q[i] = qa[i] * ra + qb[i] * rb;
+        }
+    }
+    for (int i = 0; i < 16; i++) S[i] = (1.0f - t) * a[7 + i] + t * b[7 + i];
+    const float n = 1.0f / std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    const float X = q[0] * n, Y = q[1] * n, Z = q[2] * n, W = q[3] * n;
+    Mat4 R{};
+    R[0] = 1 - 2 * Y * Y - 2 * Z * Z; R[4] = 2 * X * Y - 2 * W * Z; R[8] = 2 * X * Z + 2 * W * Y;
+    R[1] = 2 * X * Y + 2 * W * Z; R[5] = 1 - 2 * X * X - 2 * Z * Z; R[9] = 2 * Y * Z - 2 * W * X;
+    R[2] = 2 * X * Z - 2 * W * Y; R[6] = 2 * Y * Z + 2 * W * X; R[10] = 1 - 2 * X * X - 2 * Y * Y;
+    R[15] = 1.0f;
+    Mat4 Sm, Tm{};
+    for (int i = 0; i < 16; i++) Sm[i] = S[i];
+    Tm[0] = Tm[5] = Tm[10] = Tm[15] = 1.0f;
+    Tm[12] = T[0]; Tm[13] = T[1]; Tm[14] = T[2];
+    return mat_mul(Tm, mat_mul(R, Sm));
+}
+
 }  // namespace
 
 struct vgpu_render {
@@ -138,6 +285,7 @@ struct vgpu_render {
     float4* d_rays2 = nullptr;     // odd levels' mirror rays
     vgpu::LevelRec* d_recs = nullptr;   // per pixel, per level
     double* d_filter = nullptr;    // pixel filter sampler tables
+    float* d_camkeys = nullptr;    // camera motion keys (kCamKeyFloats each)
     float4* d_pix = nullptr;       // per pixel: D, lambda
     int4* d_hits32 = nullptr;      // extended hits (2 int4 per pixel)
     float4* d_slot_rays = nullptr;
@@ -151,7 +299,7 @@ struct vgpu_render {
     ~vgpu_render() {
         int prev = 0;
         if (hipGetDevice(&prev) == hipSuccess && scene) (void)hipSetDevice(scene->device);
-        void* ptrs[] = {d_materials, d_lights, d_rays, d_rays2, d_recs, d_filter, d_pix, d_hits32, d_slot_rays,
+        void* ptrs[] = {d_materials, d_lights, d_rays, d_rays2, d_recs, d_filter, d_camkeys, d_pix, d_hits32, d_slot_rays,
                         d_slot_rgb, d_slot_hits, d_groups, d_pixshade, d_stage};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         (void)hipSetDevice(prev);
@@ -381,6 +529,35 @@ int vgpu_render_set_filter(vgpu_render* r, int32_t type, float width, int32_t re
     r->rv.filt = r->d_filter;
     r->rv.filt_n = n;
     r->rv.filt_w = (double)W;
+    return VGPU_OK;
+}
+
+int vgpu_render_set_camera_keys(vgpu_render* r, const float* local_to_world, uint32_t nkeys) {
+    if (!r) return fail(VGPU_ERR_ARG, "render is NULL");
+    if (nkeys > 0 && !local_to_world) return fail(VGPU_ERR_ARG, "camera keys are NULL");
+    if (nkeys > (uint32_t)vgpu::kMaxCamKeys) return fail(VGPU_ERR_ARG, "%u camera keys (at most %d)", nkeys, vgpu::kMaxCamKeys);
+    for (size_t i = 0; i < (size_t)nkeys * 16; i++)
+        if (!std::isfinite(local_to_world[i])) return fail(VGPU_ERR_ARG, "camera key matrices must be finite");
+    DeviceGuard dg(r->scene->device);
+    std::vector<float> recs((size_t)std::max<uint32_t>(nkeys, 1) * vgpu::kCamKeyFloats);
+    for (uint32_t k = 0; k < nkeys; k++) {
+        Mat4 m;
+        for (int i = 0; i < 16; i++) m[i] = local_to_world[(size_t)k * 16 + i];
+        decompose_key(m, &recs[(size_t)k * vgpu::kCamKeyFloats]);
+    }
+    if (nkeys == 1) {   // one key: the matrix of every sample (the key lerped with itself, camera.go:224-236)
+        const Mat4 M = decomp_matrix(recs.data(), recs.data(), 0.0f);
+        for (int i = 0; i < 16; i++) r->rv.M[i] = M[i];
+    }
+    if (r->d_camkeys) { HIP_TRY(hipFree(r->d_camkeys)); r->d_camkeys = nullptr; }
+    r->rv.cam_keys = nullptr;
+    r->rv.cam_nkeys = 0;
+    if (nkeys >= 2) {
+        HIP_TRY(hipMalloc((void**)&r->d_camkeys, recs.size() * sizeof(float)));
+        HIP_TRY(hipMemcpy(r->d_camkeys, recs.data(), recs.size() * sizeof(float), hipMemcpyHostToDevice));
+        r->rv.cam_keys = r->d_camkeys;
+        r->rv.cam_nkeys = (int32_t)nkeys;
+    }
     return VGPU_OK;
 }
 

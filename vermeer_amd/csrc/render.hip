@@ -784,6 +784,54 @@ __device__ void warp_sample(const RenderView& rv, double r0, double r1, double& 
     v = 0.0;
 }
 
+// Camera matrix at a sample's time from the motion keys (camera.go:224-236): TransformDecompLerp and
+// TransformDecompToMatrix4 (math/animdecomp.go:66-80) with QuatSlerp (math/quat.go:30-67),
+// QuatToMatrix4 (quat.go:80-104) and Matrix4Mul (math/matrix4.go:92-102, sums from +0).
+__device__ void camera_key_matrix(const RenderView& rv, float time, float* M) {
+    const float k = time * (float)(rv.cam_nkeys - 1);
+    const float t = k - floorf(k);
+    const float* a = rv.cam_keys + (int)floorf(k) * kCamKeyFloats;
+    const float* b = rv.cam_keys + (int)ceilf(k) * kCamKeyFloats;
+    float T[3], q[4];
+    for (int i = 0; i < 3; i++) T[i] = (1.0f - t) * a[i] + t * b[i];
+    const float* qa = a + 3;
+    const float* qb = b + 3;
+    const float c = qa[3] * qb[3] + qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2];
+    if (fabs32(c) >= 1.0f) {
+        for (int i = 0; i < 4; i++) q[i] = qa[i];
+    } else {
+        const float half = facos(c);
+        const float sh = sqrtf(1.0f - c * c);
+        if (fabs32(sh) < 0.001f) {
+            for (int i = 0; i < 4; i++) q[i] = qa[i] * 0.5f + qb[i] * 0.5f;
+        } else {
+            const float ra = fsin((1 - t) * half) / sh, rb = fsin(t * half) / sh;
+            for (int i = 0; i < 4; i++) q[i] = qa[i] * ra + qb[i] * rb;
+        }
+    }
+    const float n = 1.0f / sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    const float X = q[0] * n, Y = q[1] * n, Z = q[2] * n, W = q[3] * n;
+    float R[16] = {0};
+    R[0] = 1 - 2 * Y * Y - 2 * Z * Z; R[4] = 2 * X * Y - 2 * W * Z; R[8] = 2 * X * Z + 2 * W * Y;
+    R[1] = 2 * X * Y + 2 * W * Z; R[5] = 1 - 2 * X * X - 2 * Z * Z; R[9] = 2 * Y * Z - 2 * W * X;
+    R[2] = 2 * X * Z - 2 * W * Y; R[6] = 2 * Y * Z + 2 * W * X; R[10] = 1 - 2 * X * X - 2 * Y * Y;
+    R[15] = 1.0f;
+    float RS[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float acc = 0.0f;
+            for (int kk = 0; kk < 4; kk++) acc += R[kk * 4 + i] * ((1.0f - t) * a[7 + j * 4 + kk] + t * b[7 + j * 4 + kk]);
+            RS[j * 4 + i] = acc;
+        }
+    float Tm[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, T[0], T[1], T[2], 1};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float acc = 0.0f;
+            for (int kk = 0; kk < 4; kk++) acc += Tm[kk * 4 + i] * RS[j * 4 + kk];
+            M[j * 4 + i] = acc;
+        }
+}
+
 __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter,
                                                 float4* __restrict__ rays, float4* __restrict__ pix) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -831,7 +879,9 @@ __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* _
         e = add(scale(xo, v3(1, 0, 0)), scale(yo, v3(0, 1, 0)));
     }
     const V3 se = sub(sv, e);
-    const float* M = rv.M;
+    float M[16];
+    if (rv.cam_nkeys >= 2) camera_key_matrix(rv, time, M);
+    else for (int i = 0; i < 16; i++) M[i] = rv.M[i];
     const V3 d = v3(M[0] * se.x + M[4] * se.y + M[8] * se.z, M[1] * se.x + M[5] * se.y + M[9] * se.z,
                     M[2] * se.x + M[6] * se.y + M[10] * se.z);
     const V3 D = normalize(d);

@@ -49,6 +49,8 @@ struct RenderView {
     float M[16];                   // camera matrix (column-major)
     float tan_theta_focal, aspect, focal;
     float lens_radius;             // thin lens (camera Radius); 0: pinhole
+    const float* cam_keys;         // camera motion keys: cam_nkeys records of kCamKeyFloats (T, R, S)
+    int32_t cam_nkeys;             // 0: the matrix M for every sample
     uint64_t raster_fwd[52];       // RasterXY enumeration tables for m = 12
     uint64_t raster_inv[24];
     const double* filt;            // pixel filter sampler: cdfV (filt_n), then cdfVU (filt_n x filt_n); null: box
@@ -56,6 +58,10 @@ struct RenderView {
     int32_t max_level;             // deepest recursion level shaded (0 without mirrors, else 3)
     double filt_w;
 };
+
+// A camera motion key: TransformDecomp (math/animdecomp.go:13-17) as T[3], R (X, Y, Z, W), S[16], one pad.
+constexpr int kCamKeyFloats = 24;
+constexpr int kMaxCamKeys = 64;
 
 constexpr int kMaxLevel = 3;       // ShaderStd.Eval returns at once above level 3 (std.go:97-99)
 

@@ -134,6 +134,12 @@ class Renderer:
         t = {"box": _lib.VGPU_FILTER_BOX, "gaussian": _lib.VGPU_FILTER_GAUSSIAN, "airy": _lib.VGPU_FILTER_AIRY}[kind]
         check(self._lib.vgpu_render_set_filter(self._h, t, float(width), int(res), float(peak)))
 
+    def set_camera_keys(self, local_to_world):
+        """Camera motion keys: (nkeys, 16) column-major LocalToWorld matrices; each sample's camera matrix
+        is recomposed from the lerp of the two keys bracketing its time (one key: a single matrix)."""
+        m = np.ascontiguousarray(local_to_world, np.float32).reshape(-1, 16)
+        check(self._lib.vgpu_render_set_camera_keys(self._h, ptr(m), len(m)))
+
     def render_pass(self, scrambles, iteration, want_sample=False):
         """One pass with host arrays (synchronous); updates self.film.  Returns (shadow rays, sample)."""
         scr = np.ascontiguousarray(scrambles, np.uint64)
