@@ -85,7 +85,7 @@ def other_workloads(torch, np):
     """The remaining configurations, measured briefly on the same GPU (rays resident on the device):
     sphere + ground primary and uniform rays, the reference-semantics renderer on the box of spheres
     and on the feature scene (mirrors, disk and sphere lights, Gaussian filter), and motion-blurred
-    traversal of 2000 moving spheres."""
+    traversal and rendering of 2000 moving spheres."""
     import vermeer_amd as va
     from vermeer_amd import render, scenes
     dev = torch.device("cuda:0")
@@ -204,6 +204,33 @@ def other_workloads(torch, np):
                              "combined_mrays_per_s": round(2 * n / (ms_cam + ms_sh) / 1e3, 1),
                              "hit_fraction": round(float(hit.float().mean()), 4)}
     log(f"[bench] moving spheres: {out['moving_spheres']}")
+
+    # the same spheres rendered: each sample at its own time, camera + shadow rays as the reference counts
+    # them, 1024^2 x 4 passes
+    desc = scenes.moving_spheres_render_scene()
+    c = desc["camera"]
+    W = 1024
+    cam = render.lookat_camera(c["from"], c["to"], c["up"], c["fov"], c["focal"], W, W)
+    R = render.Renderer(desc, W, W, cam)
+    scr = torch.from_numpy(scenes.pixel_scrambles(W, W).view(np.int64).reshape(-1).copy()).to(dev)
+    film = torch.zeros(W * W * 3, dtype=torch.float32, device=dev)
+    R.render_pass_device(scr.data_ptr(), 1, film.data_ptr(), stream=st)
+    e0.record()
+    for it in range(1, 5):
+        R.render_pass_device(scr.data_ptr(), it, film.data_ptr(), stream=st)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    R.scene.status(st)
+    scr_h = scenes.pixel_scrambles(W, W)
+    nshadow = sum(R.render_pass(scr_h, it)[0] for it in range(1, 5))
+    nrays = 4 * W * W + nshadow
+    out["moving_spheres_render"] = {"image": [W, W], "passes": 4, "ms": round(ms, 2), "keys": 2,
+                                    "triangles": int(len(desc["meshes"][0]["idx"])), "rays": int(nrays),
+                                    "mrays_per_s": round(nrays / (ms * 1e-3) / 1e6, 1),
+                                    "samples_per_s": round(W * W * 4 / (ms * 1e-3), 1)}
+    R.close()
+    log(f"[bench] moving spheres renderer: {out['moving_spheres_render']}")
     return out
 
 

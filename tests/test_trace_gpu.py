@@ -321,3 +321,46 @@ def test_motion_blur_scene_bitexact():
         _same(hits, geoms, ref, f"motion shadow={shadow}")
         assert np.array_equal(cnt, rc)
     assert (hits["prim"] >= 0).mean() > 0.01
+
+
+@pytest.fixture(scope="module")
+def mixed_motion():
+    """Static and moving meshes in one scene (one moving mesh with the reference's slot indexing) and an
+    analytic sphere, under a motion top level, with 120k random rays and the oracle's answers."""
+    print("\n[case] mixed motion scene", flush=True)
+    mv, mi = scenes.moving_spheres(count=120, seed=5)
+    mv2, mi2 = scenes.moving_spheres(count=60, seed=9, extent=10.0)
+    sv_, sf = scenes.uv_sphere(3.0, 48, 24, center=(2.0, 1.0, -4.0))
+    gv, gf = scenes.ground_quad(size=30.0, y=-22.0)
+    g = va.Scene(0)
+    o = oracle.OracleScene()
+    for v, f, ref_idx in ((mv, mi, False), (sv_, sf, False), (mv2, mi2, True), (gv, gf, False)):
+        g.add_geom(va.PolyMesh(v, f, motion_reference_indexing=ref_idx))
+        o.add_mesh(v, f, motion_reference_indexing=ref_idx)
+    g.add_sphere((-3.0, 2.0, 5.0), 2.5, 0)
+    o.add_sphere((-3.0, 2.0, 5.0), 2.5)
+    g.pre_render()
+    o.commit()
+    gs = scenes.rng(17)
+    n = 120_000
+    rays = scenes.make_rays(gs.uniform(-25, 25, (n, 3)), scenes.uniform_sphere_dirs(gs, n), time=gs.random(n))
+    refs = {shadow: o.trace(rays, shadow=shadow, counters=True) for shadow in (False, True)}
+    yield g, rays, refs
+    g.close()
+
+
+@pytest.mark.parametrize("cfg", [(64, 8, 32), (128, 12, 24), (256, 16, 16), (64, 32, 16)])
+def test_mixed_motion_scene_every_config_bitexact(mixed_motion, cfg):
+    """Hits and per-ray counters equal the oracle's for closest and any hit in every launch shape."""
+    g, rays, refs = mixed_motion
+    print(f"[case] mixed motion scene, config {cfg}", flush=True)
+    g.set_trace_config(*cfg)
+    try:
+        for shadow in (False, True):
+            hits, geoms, cnt = g.trace(rays, shadow=shadow, want_counters=True)
+            ref, rc = refs[shadow]
+            _same(hits, geoms, ref, f"mixed motion cfg={cfg} shadow={shadow}")
+            assert np.array_equal(cnt, rc)
+            assert len(np.unique(geoms[geoms >= 0])) >= 4
+    finally:
+        g.set_trace_config(256, 16, 16)

@@ -790,8 +790,8 @@ __device__ void warp_sample(const RenderView& rv, double r0, double r1, double& 
 __device__ void camera_key_matrix(const RenderView& rv, float time, float* M) {
     const float k = time * (float)(rv.cam_nkeys - 1);
     const float t = k - floorf(k);
-    const float* a = rv.cam_keys + (int)floorf(k) * kCamKeyFloats;
-    const float* b = rv.cam_keys + (int)ceilf(k) * kCamKeyFloats;
+    const float* a = rv.cam_keys + min(max((int)floorf(k), 0), rv.cam_nkeys - 1) * kCamKeyFloats;
+    const float* b = rv.cam_keys + min(max((int)ceilf(k), 0), rv.cam_nkeys - 1) * kCamKeyFloats;
     float T[3], q[4];
     for (int i = 0; i < 3; i++) T[i] = (1.0f - t) * a[i] + t * b[i];
     const float* qa = a + 3;
@@ -984,8 +984,8 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
             const float k = rb.w * (float)(geo.nkeys - 1);   // trace.go:79-84
             const float fk = floorf(k);
             mt = k - fk;
-            key = (int)fk;
-            key2 = (int)ceilf(k);
+            key = min(max((int)fk, 0), geo.nkeys - 1);   // as the traversal holds a time outside [0, 1]
+            key2 = min(max((int)ceilf(k), 0), geo.nkeys - 1);
         }
         const float4* q = rv.sv.mtris + geo.tri_base + (size_t)prim * 3 * geo.nkeys;
         const V3 a = lerp3(q[3 * key], q[3 * key2], mt), b = lerp3(q[3 * key + 1], q[3 * key2 + 1], mt),

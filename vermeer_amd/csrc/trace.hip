@@ -35,7 +35,6 @@ struct RayS {
     float S[3];
     int Kx, Ky, Kz;
     int neg;          // bit k set when D[k] < 0
-    float Pkx, Pky, Pkz;
     float time;
     float tclosest;
 };
@@ -66,9 +65,6 @@ __device__ __forceinline__ void ray_setup(RayS& r, const float D[3]) {
     for (int k = 0; k < 3; k++) r.Dinv[k] = 1.0f / D[k];
     r.Kx = Kx; r.Ky = Ky; r.Kz = Kz;
     r.neg = (D[0] < 0.0f ? 1 : 0) | (D[1] < 0.0f ? 2 : 0) | (D[2] < 0.0f ? 4 : 0);
-    r.Pkx = sel3(r.P[0], r.P[1], r.P[2], Kx);
-    r.Pky = sel3(r.P[0], r.P[1], r.P[2], Ky);
-    r.Pkz = sel3(r.P[0], r.P[1], r.P[2], Kz);
 }
 
 // One axis slab of intersectBoxes (qbvh/intersect_amd64.s:26-81), NaN rules of MINPS/MAXPS.
@@ -146,13 +142,6 @@ struct TriRay {
     int Kx, Ky, Kz;
 };
 
-__device__ __forceinline__ TriRay tri_ray(const RayS& r) {
-    TriRay q;
-    q.S0 = r.S[0]; q.S1 = r.S[1]; q.S2 = r.S[2];
-    q.Pkx = r.Pkx; q.Pky = r.Pky; q.Pkz = r.Pkz;
-    q.Kx = r.Kx; q.Ky = r.Ky; q.Kz = r.Kz;
-    return q;
-}
 
 struct TriPre {
     float T, det, fU, fV, fW;
@@ -226,23 +215,30 @@ __device__ __forceinline__ bool tri_accept(RayS& r, const TriPre& p, float& U, f
 }
 
 // The whole test with the ray quantities given.
+template <bool MOTION_BOUND>
 __device__ __forceinline__ bool tri_test_q(RayS& r, const TriRay& q, const float4& v0, const float4& v1, const float4& v2,
                                            float bias, float& U, float& V, float& W) {
     TriPre p;
-    return tri_pre<false>(q, v0, v1, v2, bias, p) && tri_accept(r, p, U, V, W);
+    return tri_pre<MOTION_BOUND>(q, v0, v1, v2, bias, p) && tri_accept(r, p, U, V, W);
 }
 
-// The whole test.  Returns true when the triangle is accepted; updates tclosest, U, V, W.
-template <bool MOTION_BOUND>
-__device__ __forceinline__ bool tri_test(RayS& r, const float4& v0, const float4& v1, const float4& v2, float bias,
-                                         float& U, float& V, float& W) {
-    TriPre p;
-    return tri_pre<MOTION_BOUND>(tri_ray(r), v0, v1, v2, bias, p) && tri_accept(r, p, U, V, W);
-}
+
+
 
 __device__ __forceinline__ float4 lerp4(const float4& a, const float4& b, float t) {
     const float s = 1.0f - t;
     return make_float4(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z, s * a.w + t * b.w);
+}
+
+// Vertices of a motion triangle (its nkeys groups of 3 float4 at q) at time t between keys k0 and k1
+// (trace.go:547-554).
+__device__ __forceinline__ void motion_tri(const float4* __restrict__ q, int k0, int k1, float t, float4& v0, float4& v1,
+                                           float4& v2) {
+    const float4* a = q + 3 * k0;
+    const float4* b = q + 3 * k1;
+    v0 = lerp4(a[0], b[0], t);
+    v1 = lerp4(a[1], b[1], t);
+    v2 = lerp4(a[2], b[2], t);
 }
 
 struct NodeData {
@@ -275,12 +271,14 @@ __device__ __forceinline__ void fetch_motion(const float4* __restrict__ mnodes, 
 }
 
 // ray.Time -> (key, key2, t) (builtin/geom/polymesh/trace.go:79-84, scene.go:49-54)
+// A time outside [0, 1] (the reference indexes out of range and panics) is held to the end keys,
+// so no key index leaves the node or triangle.
 __device__ __forceinline__ void motion_keys(float rtime, int nkeys, int& key, int& key2, float& t) {
     const float k = rtime * (float)(nkeys - 1);
     const float fk = floorf(k);
     t = k - fk;
-    key = (int)fk;
-    key2 = (int)ceilf(k);
+    key = min(max((int)fk, 0), nkeys - 1);
+    key2 = min(max((int)ceilf(k), 0), nkeys - 1);
 }
 
 // Sphere.Trace (builtin/geom/sphere/trace.go:13-52): the closed-form quadratic (solveQuadratic,
@@ -331,175 +329,8 @@ __device__ __forceinline__ int next_geom(const SceneView& sv, const float4* __re
     return 0;
 }
 
-// EXT: hits are 32-byte records {prim, geom, t, U, V, W, 0, 0} for the shading kernels.
-template <int BLOCK, int LDS, bool SHADOW, bool MOTION, bool COUNTERS, bool EXT = false>
-__global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __restrict__ rays, uint64_t n,
-                                                 int4* __restrict__ hits, int32_t* __restrict__ geoms_out,
-                                                 int4* __restrict__ counters, float* __restrict__ spT,
-                                                 int32_t* __restrict__ spN, int32_t* __restrict__ err,
-                                                 const uint32_t* __restrict__ n_dev,
-                                                 const uint32_t* __restrict__ perm) {
-    __shared__ float sT[LDS * BLOCK];
-    __shared__ int32_t sN[LDS * BLOCK];
-    const int tid = threadIdx.x;
-    const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
-    const uint64_t nthr = (uint64_t)gridDim.x * BLOCK;
-    Stack<BLOCK, LDS> st;
-    st.sT = sT + tid; st.sN = sN + tid;
-    st.gT = spT; st.gN = spN; st.gstride = nthr; st.gofs = (uint32_t)gtid;
-
-    if (n_dev) n = min(n, (uint64_t)*n_dev);   // device-resident count (compacted queues)
-    for (uint64_t ri = gtid; ri < n; ri += nthr) {
-        const float4 ra = rays[2 * ri], rb = rays[2 * ri + 1];
-        RayS r;
-        r.P[0] = ra.x; r.P[1] = ra.y; r.P[2] = ra.z; r.tclosest = ra.w;
-        const float D[3] = {rb.x, rb.y, rb.z};
-        r.time = rb.w;
-        ray_setup(r, D);
-        // Wave-uniform choice of the box test (see box4): the fast form when no lane can make a
-        // NaN slab value.  Motion boxes are interpolated, and 0 * inf there gives NaN, so motion
-        // traversal always takes the exact form.
-        bool fast = false;
-        if (!MOTION && sv.nan_free_boxes) {
-            bool ok = true;
-            for (int k = 0; k < 3; k++) ok = ok && absf(r.P[k]) < INFINITY && absf(r.Dinv[k]) < INFINITY && r.Dinv[k] != 0.0f;
-            fast = __all(ok ? 1 : 0) != 0;
-        }
-
-        int nodesT = 0, leafsT = 0, trisT = 0, maxDepth = 0;
-        int best_geom = -1, best_prim = -1;
-        float bu = 0.0f, bv = 0.0f, bw = 0.0f;
-
-        // top level
-        const bool top_motion = MOTION && sv.top_motion;
-        int tkey = 0, tkey2 = 0; float ttime = 0.0f;
-        if (top_motion) motion_keys(r.time, sv.top_keys, tkey, tkey2, ttime);
-
-        bool in_mesh = false;
-        int mesh_base = 0, gcur = 0, gend = 0;
-        DevGeom g = {0, 0, 1, 0.0f};
-        int mkey = 0, mkey2 = 0; float mtime = 0.0f;
-        bool overflow = false;
-
-        st.store(0, r.tclosest, 0);
-        int sp = 1;
-        while (true) {
-            if (in_mesh && sp == mesh_base) {
-                // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
-                gcur++;
-                const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
-                if (ng == 2) break;
-                if (ng == 1) {
-                    if (MOTION && g.nkeys > 1) motion_keys(r.time, g.nkeys, mkey, mkey2, mtime);
-                    st.store(mesh_base, r.tclosest, 0);
-                    sp = mesh_base + 1;
-                    continue;
-                }
-                in_mesh = false;
-                sp = top_motion ? mesh_base : mesh_base - 1;
-            }
-            if (sp == 0) break;
-            sp--;
-            float T; int32_t node;
-            st.load(sp, T, node);
-            if (r.tclosest < T || node == -1) continue;
-            if (node >= 0) {
-                if (COUNTERS) nodesT++;
-                NodeData nd;
-                if (in_mesh) {
-                    if (MOTION && g.nkeys > 1) fetch_motion(sv.mnodes, g.node_base, node, g.nkeys, mkey, mkey2, mtime, nd);
-                    else fetch_static(sv.nodes, g.node_base + node, nd);
-                } else {
-                    if (top_motion) fetch_motion(sv.mnodes, sv.top_mbase, node, sv.top_keys, tkey, tkey2, ttime, nd);
-                    else fetch_static(sv.nodes, node, nd);
-                }
-                Box4 b;
-                if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
-                else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
-                // Push order of qbvh/intersect.go:137-216 (same as motionintersect.go:53-97): the
-                // children are pushed as two pairs; rank(c) is child c's position in that sequence.
-                // Hit children are stored compacted in rank order, i.e. exactly the entries the
-                // reference leaves below its stack top.
-                const int n0 = (r.neg >> nd.ax[0]) & 1;
-                const int n1 = (r.neg >> nd.ax[1]) & 1;
-                const int n2 = (r.neg >> nd.ax[2]) & 1;
-                const int gA = n0 ? 0 : 2, gB = 2 - gA;
-                const int rk0 = gA + (n1 ^ 1), rk1 = gA + n1, rk2 = gB + (n2 ^ 1), rk3 = gB + n2;
-                const int m = (b.h[0] ? 1 << rk0 : 0) | (b.h[1] ? 1 << rk1 : 0) | (b.h[2] ? 1 << rk2 : 0) |
-                              (b.h[3] ? 1 << rk3 : 0);
-                if (sp + 4 > VGPU_MAX_STACK) {   // conservative: the reference writes up to 4 slots here
-                    overflow = true;
-                    break;
-                }
-                if (b.h[0]) st.store(sp + __popc(m & ((1 << rk0) - 1)), b.t[0], nd.ch[0]);
-                if (b.h[1]) st.store(sp + __popc(m & ((1 << rk1) - 1)), b.t[1], nd.ch[1]);
-                if (b.h[2]) st.store(sp + __popc(m & ((1 << rk2) - 1)), b.t[2], nd.ch[2]);
-                if (b.h[3]) st.store(sp + __popc(m & ((1 << rk3) - 1)), b.t[3], nd.ch[3]);
-                sp += __popc(m);
-                if (COUNTERS) maxDepth = max(maxDepth, sp);
-            } else {
-                const int base = leaf_base(node), count = leaf_count(node);
-                if (COUNTERS) leafsT++;
-                if (in_mesh) {
-                    if (COUNTERS) trisT += count;
-                    int idx = -1;
-                    float U = 0.0f, V = 0.0f, W = 0.0f;
-                    if (MOTION && g.nkeys > 1) {
-                        const float4* tp = sv.mtris + g.tri_base + (size_t)base * 3 * g.nkeys;
-                        for (int i = 0; i < count; i++) {
-                            const float4* q = tp + (size_t)i * 3 * g.nkeys;
-                            const float4* q0 = q + 3 * mkey;
-                            const float4* q1 = q + 3 * mkey2;
-                            const float4 v0 = lerp4(q0[0], q1[0], mtime);
-                            const float4 v1 = lerp4(q0[1], q1[1], mtime);
-                            const float4 v2 = lerp4(q0[2], q1[2], mtime);
-                            if (tri_test<true>(r, v0, v1, v2, g.ray_bias, U, V, W)) idx = base + i;
-                        }
-                    } else {
-                        const float4* tp = sv.tris + ((size_t)g.tri_base + base) * 3;
-                        for (int i = 0; i < count; i++) {
-                            const float4 v0 = tp[3 * i], v1 = tp[3 * i + 1], v2 = tp[3 * i + 2];
-                            if (tri_test<false>(r, v0, v1, v2, g.ray_bias, U, V, W)) idx = base + i;
-                        }
-                    }
-                    if (idx >= 0) {
-                        best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
-                        if (SHADOW) break;   // qbvh/intersect.go:232-236 at both levels
-                    }
-                } else {
-                    // Scene.TraceElems: geoms [base, base+count) (scene.go:61-78)
-                    mesh_base = top_motion ? sp : sp + 1;
-                    if (mesh_base >= VGPU_MAX_STACK) { overflow = true; break; }
-                    gcur = base; gend = base + count;
-                    const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
-                    if (ng == 2) break;
-                    if (ng == 1) {
-                        if (MOTION && g.nkeys > 1) motion_keys(r.time, g.nkeys, mkey, mkey2, mtime);
-                        st.store(mesh_base, r.tclosest, 0);
-                        sp = mesh_base + 1;
-                        in_mesh = true;
-                    }
-                }
-            }
-        }
-        if (overflow) {
-            atomicOr(err, 1);
-            best_geom = -1; best_prim = -1;
-        }
-        const uint64_t oi = perm ? (uint64_t)perm[ri] : ri;
-        if (EXT) {
-            hits[2 * oi] = make_int4(best_prim, best_geom, __float_as_int(r.tclosest), __float_as_int(bu));
-            hits[2 * oi + 1] = make_int4(__float_as_int(bv), __float_as_int(bw), 0, 0);
-        } else {
-            hits[oi] = make_int4(best_prim, __float_as_int(r.tclosest), __float_as_int(bu), __float_as_int(bv));
-            if (geoms_out) geoms_out[oi] = best_geom;
-        }
-        if (COUNTERS) counters[oi] = make_int4(nodesT, leafsT, trisT, maxDepth);
-    }
-}
-
-// Static scenes.  The same traversal as k_trace, one step per loop iteration, except that
-// the triangles of the leaves popped in a step are tested by the whole wavefront: lane l
+// The traversal, one ray per lane and one traversal step (a node or a leaf) per loop iteration.
+// The triangles of the leaves popped in a step are tested by the whole wavefront: lane l
 // owns a leaf of c_l triangles, the sum of the c_l triangle tests is dealt out over the 64
 // lanes (consecutive lanes take consecutive triangles of one leaf, so the loads coalesce),
 // and each lane reads the owning ray's shear constants with a cross-lane shuffle.  That part
@@ -507,7 +338,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const float4* __r
 // test, in leaf order, on the triangles that passed it, which accepts the same triangles in
 // the same order as testing every triangle of the leaf in turn.  Lanes whose ray is finished
 // stay in the loop as workers until the whole wavefront is done.
-template <int BLOCK, int LDS, bool SHADOW, bool COUNTERS, bool EXT>
+// MOTION: the scene has motion keys (a motion top level and/or moving meshes); node boxes
+// (qbvh/motionintersect.go:47-49) and triangles of moving meshes (trace.go:547-554) are
+// interpolated at the ray's time.
+// EXT: hits are 32-byte records {prim, geom, t, U, V, W, 0, 0} for the shading kernels.
+template <int BLOCK, int LDS, bool SHADOW, bool MOTION, bool COUNTERS, bool EXT>
 __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4* __restrict__ rays, uint64_t n,
                                                       int4* __restrict__ hits, int32_t* __restrict__ geoms_out,
                                                       int4* __restrict__ counters, float* __restrict__ spT,
@@ -545,6 +380,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
     int best_geom = -1, best_prim = -1;
     float bu = 0.0f, bv = 0.0f, bw = 0.0f;
     int mesh_base = 0, gcur = 0, gend = 0;   // mesh_base 0: in the scene level, else inside geom gcur
+    // A motion top level starts a geom's stack at the popped leaf's own slot, which can be 0, so
+    // motion scenes keep a separate flag (qbvh/motionintersect.go:99-103 against intersect.go:219-226).
+    const bool top_motion = MOTION && sv.top_motion;
+    bool in_mesh_m = false;
+#define VGPU_IN_MESH (MOTION ? in_mesh_m : mesh_base != 0)
     DevGeom g = {0, 0, 1, 0.0f};
     int sp = 0;
     while (true) {
@@ -588,6 +428,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                         best_geom = -1; best_prim = -1;
                         bu = 0.0f; bv = 0.0f; bw = 0.0f;
                         mesh_base = 0;
+                        in_mesh_m = false;
                         st.store(0, r.tclosest, 0);
                         sp = 1;
                         done = false;
@@ -598,13 +439,14 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
             }
             if (!more) break;
             cursor += (uint32_t)__popcll(dm);
-            fast = sv.nan_free_boxes && __all(ok ? 1 : 0) != 0;
+            // motion boxes are interpolated, and 0 * inf there gives NaN: the exact form only
+            fast = !MOTION && sv.nan_free_boxes && __all(ok ? 1 : 0) != 0;
         }
         bool pend = false;   // this lane popped a triangle leaf: [lbase, lbase + lcount) of geom gcur
         int lbase = 0, lcount = 0;
         if (!done) {
             do {
-                if (mesh_base != 0 && sp == mesh_base) {
+                if (VGPU_IN_MESH && sp == mesh_base) {
                     // PolyMesh.Trace returned; Scene.TraceElems moves to the leaf's next geom (scene.go:65-75)
                     gcur++;
                     const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
@@ -614,8 +456,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                         sp = mesh_base + 1;
                         break;
                     }
-                    sp = mesh_base - 1;
+                    sp = top_motion ? mesh_base : mesh_base - 1;
                     mesh_base = 0;
+                    in_mesh_m = false;
                 }
                 if (sp == 0) { done = true; break; }
                 sp--;
@@ -625,11 +468,32 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 if (node >= 0) {
                     if (COUNTERS) nodesT++;
                     NodeData nd;
-                    fetch_static(sv.nodes, mesh_base != 0 ? g.node_base + node : node, nd);
+                    if constexpr (MOTION) {
+                        int k0, k1;
+                        float t;
+                        if (in_mesh_m) {
+                            if (g.nkeys > 1) {
+                                motion_keys(r.time, g.nkeys, k0, k1, t);
+                                fetch_motion(sv.mnodes, g.node_base, node, g.nkeys, k0, k1, t, nd);
+                            } else {
+                                fetch_static(sv.nodes, g.node_base + node, nd);
+                            }
+                        } else if (top_motion) {
+                            motion_keys(r.time, sv.top_keys, k0, k1, t);
+                            fetch_motion(sv.mnodes, sv.top_mbase, node, sv.top_keys, k0, k1, t, nd);
+                        } else {
+                            fetch_static(sv.nodes, node, nd);
+                        }
+                    } else {
+                        fetch_static(sv.nodes, mesh_base != 0 ? g.node_base + node : node, nd);
+                    }
                     Box4 b;
                     if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
                     else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
-                    // push order of qbvh/intersect.go:137-216, as in k_trace
+                    // Push order of qbvh/intersect.go:137-216 (same as motionintersect.go:53-97): the
+                    // children are pushed as two pairs; rank(c) is child c's position in that sequence.
+                    // Hit children are stored compacted in rank order, i.e. exactly the entries the
+                    // reference leaves below its stack top.
                     const int n0 = (kp >> nd.ax[0]) & 1;
                     const int n1 = (kp >> nd.ax[1]) & 1;
                     const int n2 = (kp >> nd.ax[2]) & 1;
@@ -637,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                     const int rk0 = gA + (n1 ^ 1), rk1 = gA + n1, rk2 = gB + (n2 ^ 1), rk3 = gB + n2;
                     const int m = (b.h[0] ? 1 << rk0 : 0) | (b.h[1] ? 1 << rk1 : 0) | (b.h[2] ? 1 << rk2 : 0) |
                                   (b.h[3] ? 1 << rk3 : 0);
-                    if (sp + 4 > VGPU_MAX_STACK) {
+                    if (sp + 4 > VGPU_MAX_STACK) {   // conservative: the reference writes up to 4 slots here
                         best_prim = -2;
                         done = true;
                         break;
@@ -651,20 +515,21 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 } else {
                     const int base = leaf_base(node), count = leaf_count(node);
                     if (COUNTERS) leafsT++;
-                    if (mesh_base != 0) {
+                    if (VGPU_IN_MESH) {
                         if (COUNTERS) trisT += count;
                         pend = true;
                         lbase = base;
                         lcount = count;
                     } else {
                         // Scene.TraceElems: geoms [base, base+count) (scene.go:61-78)
-                        const int mb = sp + 1;
+                        const int mb = top_motion ? sp : sp + 1;
                         if (mb >= VGPU_MAX_STACK) { best_prim = -2; done = true; break; }
                         gcur = base; gend = base + count;
                         const int ng = next_geom<SHADOW>(sv, rays, ri, r, gcur, gend, g, best_geom, best_prim, bu, bv, bw);
                         if (ng == 2) { done = true; break; }
                         if (ng == 1) {
                             mesh_base = mb;
+                            in_mesh_m = true;
                             st.store(mesh_base, r.tclosest, 0);
                             sp = mesh_base + 1;
                         }
@@ -681,10 +546,22 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 int idx = -1;
                 float U = 0.0f, V = 0.0f, W = 0.0f;
                 const TriRay q = tri_ray_packed(r, kp);
-                const float4* tp = sv.tris + ((size_t)g.tri_base + lbase) * 3;
-                for (int i = 0; i < lcount; i++)
-                    if (tri_test_q(r, q, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
-                        idx = lbase + i;
+                if (MOTION && g.nkeys > 1) {
+                    int k0, k1;
+                    float t;
+                    motion_keys(r.time, g.nkeys, k0, k1, t);
+                    const float4* tp = sv.mtris + g.tri_base + (size_t)lbase * 3 * g.nkeys;
+                    for (int i = 0; i < lcount; i++) {
+                        float4 v0, v1, v2;
+                        motion_tri(tp + (size_t)i * 3 * g.nkeys, k0, k1, t, v0, v1, v2);
+                        if (tri_test_q<true>(r, q, v0, v1, v2, g.ray_bias, U, V, W)) idx = lbase + i;
+                    }
+                } else {
+                    const float4* tp = sv.tris + ((size_t)g.tri_base + lbase) * 3;
+                    for (int i = 0; i < lcount; i++)
+                        if (tri_test_q<false>(r, q, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
+                            idx = lbase + i;
+                }
                 if (idx >= 0) {
                     best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
                     if (SHADOW) done = true;
@@ -701,7 +578,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
             }
             const int total = __shfl(pin, 63);
             const int pex = pin - c;
-            const int tb = g.tri_base + lbase;
+            // first triangle of the leaf: a triangle index (static) or a float4 offset (moving mesh)
+            const bool mv = MOTION && g.nkeys > 1;
+            const int tb = mv ? g.tri_base + lbase * 3 * g.nkeys : g.tri_base + lbase;
             const TriRay q = tri_ray_packed(r, kp);
             const int kpack = q.Kx | (q.Ky << 2) | (q.Kz << 4);
             const float bias = g.ray_bias;
@@ -721,7 +600,24 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 const float obias = __shfl(bias, o);
                 const int otb = __shfl(tb, o);
                 bool pass = false;
-                if (w < total) {
+                if constexpr (MOTION) {
+                    const int onk = __shfl(mv ? g.nkeys : 1, o);
+                    const float otime = __shfl(r.time, o);
+                    if (w < total) {
+                        TriPre pr;
+                        if (onk > 1) {
+                            int k0, k1;
+                            float t;
+                            motion_keys(otime, onk, k0, k1, t);
+                            float4 v0, v1, v2;
+                            motion_tri(sv.mtris + otb + (size_t)i * 3 * onk, k0, k1, t, v0, v1, v2);
+                            pass = tri_pre<true>(oq, v0, v1, v2, obias, pr);
+                        } else {
+                            const float4* tp = sv.tris + (size_t)(otb + i) * 3;
+                            pass = tri_pre<false>(oq, tp[0], tp[1], tp[2], obias, pr);
+                        }
+                    }
+                } else if (w < total) {
                     const float4* tp = sv.tris + (size_t)(otb + i) * 3;
                     TriPre pr;
                     pass = tri_pre<false>(oq, tp[0], tp[1], tp[2], obias, pr);
@@ -737,12 +633,25 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 // the owner's sequential pass over the candidates, in leaf order (trace.go:108-122)
                 int idx = -1;
                 float U = 0.0f, V = 0.0f, W = 0.0f;
-                const float4* tp = sv.tris + (size_t)tb * 3;
-                while (cand != 0) {
-                    const int i = __ffs(cand) - 1;
-                    cand &= cand - 1;
-                    if (tri_test_q(r, q, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
-                        idx = lbase + i;
+                if (mv) {
+                    int k0, k1;
+                    float t;
+                    motion_keys(r.time, g.nkeys, k0, k1, t);
+                    while (cand != 0) {
+                        const int i = __ffs(cand) - 1;
+                        cand &= cand - 1;
+                        float4 v0, v1, v2;
+                        motion_tri(sv.mtris + tb + (size_t)i * 3 * g.nkeys, k0, k1, t, v0, v1, v2);
+                        if (tri_test_q<true>(r, q, v0, v1, v2, g.ray_bias, U, V, W)) idx = lbase + i;
+                    }
+                } else {
+                    const float4* tp = sv.tris + (size_t)tb * 3;
+                    while (cand != 0) {
+                        const int i = __ffs(cand) - 1;
+                        cand &= cand - 1;
+                        if (tri_test_q<false>(r, q, tp[3 * i], tp[3 * i + 1], tp[3 * i + 2], g.ray_bias, U, V, W))
+                            idx = lbase + i;
+                    }
                 }
                 if (idx >= 0) {
                     best_geom = gcur; best_prim = idx; bu = U; bv = V; bw = W;
@@ -752,6 +661,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
         }
     }
 }
+
+#undef VGPU_IN_MESH
 
 // Idle lanes that trigger a refill of the wavefront (k_trace_wide); 48 of 64 measured fastest on
 // incoherent bounce rays (64: refill only when the whole wavefront is idle).
@@ -763,25 +674,14 @@ hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* h
                       float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev,
                       const uint32_t* perm, hipStream_t stream) {
     const int refill_min = refill_min_lanes();
-    if constexpr (!MOTION) {
-        if (counters)
-            hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, true, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
-                               stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err,
-                               n_dev, perm, refill_min);
-        else
-            hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, false, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
-                               stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err,
-                               n_dev, perm, refill_min);
-    } else {
-        if (counters)
-            hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, true, true>), dim3((unsigned)grid), dim3(BLOCK), 0, stream,
-                               sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err, n_dev,
-                               perm);
-        else
-            hipLaunchKernelGGL((k_trace<BLOCK, LDS, SHADOW, true, false>), dim3((unsigned)grid), dim3(BLOCK), 0, stream,
-                               sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err, n_dev,
-                               perm);
-    }
+    if (counters)
+        hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, MOTION, true, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
+                           stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err,
+                           n_dev, perm, refill_min);
+    else
+        hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, MOTION, false, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
+                           stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)nullptr, spT, spN, err,
+                           n_dev, perm, refill_min);
     return hipGetLastError();
 }
 
@@ -812,11 +712,11 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
     if (threads > spill_threads) return hipErrorInvalidValue;
     const unsigned grid = (unsigned)(threads / kBlock);
     if (motion)
-        hipLaunchKernelGGL((k_trace<kBlock, kLds, false, true, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace_wide<kBlock, kLds, false, true, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
-                           (const uint32_t*)nullptr);
+                           (const uint32_t*)nullptr, refill_min_lanes());
     else
-        hipLaunchKernelGGL((k_trace_wide<kBlock, kLds, false, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace_wide<kBlock, kLds, false, false, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr, refill_min_lanes());
     return hipGetLastError();

@@ -565,6 +565,31 @@ def moving_scene(seed=7, nseg=32, nring=16, keys=3, mirrors=True):
             "camera": base["camera"]}
 
 
+def moving_spheres_render_scene(count=2000, seed=21):
+    """The 2000 moving spheres (500k triangles, two keys) as a render: a diffuse shaded motion mesh in
+    front of a diffuse backdrop (z = -25), lit by a DiskLight (radius 5, one sample: one BSDF and one
+    light shadow ray per hit).  Camera at (0, 0, 45) looking at the origin, fov 60."""
+    verts, idx = moving_spheres(count=count, seed=seed)
+    mats = [material(diffuse=(0.7, 0.6, 0.5), diffuse_strength=1.0, diffuse_roughness=0.2),
+            material(diffuse=(0.5, 0.5, 0.55), diffuse_strength=1.0)]
+    spheres = {"keys": 2, "verts": verts, "idx": idx, "shaderidx": None, "shaders": np.array([0], np.int32),
+               "normals": None}
+    bv, bf = quad((-60, -60, -25), (60, -60, -25), (60, 60, -25), (-60, 60, -25))
+    backdrop = {"verts": bv.astype(np.float32), "idx": bf.astype(np.uint32), "shaderidx": None,
+                "shaders": np.array([1], np.int32), "normals": None}
+    light_mat = len(mats)
+    mats.append(material(emission=(1.0, 0.95, 0.9), emission_strength=30.0, diffuse=(0.0, 0.0, 0.0),
+                         diffuse_strength=1.0))
+    dP, dUp, dLook, dR = (0.0, 30.0, 30.0), (1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 5.0
+    dv, di, dn = disk_light_mesh(dP, dUp, dLook, dR, 20)
+    disk = {"verts": dv, "idx": di, "shaderidx": None, "shaders": np.array([light_mat], np.int32), "normals": dn}
+    lights = [{"type": "disk", "P": dP, "Up": dUp, "LookAt": dLook, "radius": dR, "material": light_mat,
+               "samples": 1, "geom": 2}]
+    camera = {"from": (0.0, 0.0, 45.0), "to": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0), "fov": 60.0, "focal": 12.0}
+    return {"meshes": [spheres, backdrop, disk], "materials": np.array(mats, MATERIAL_DTYPE), "lights": lights,
+            "camera": camera}
+
+
 def pixel_scrambles(width, height, seed=99):
     """6 uint64 per pixel: lensU, lensV, time, lambda, scramble[0], scramble[1] (core/render.go:166-176)."""
     return rng(seed).integers(0, np.iinfo(np.uint64).max, (height, width, 6), dtype=np.uint64, endpoint=True)
