@@ -17,7 +17,8 @@ vgpu_guide::~vgpu_guide() {
     int prev = 0;
     const bool ok = hipGetDevice(&prev) == hipSuccess;
     (void)hipSetDevice(device);
-    void* ptrs[] = {d_params, d_m, d_v, d_frag_fwd, d_frag_bwd, tb.partial, tb.loss, tb.csum, tb.step, tb.grad, d_stage};
+    void* ptrs[] = {d_params, d_m, d_v, d_frag_fwd, d_frag_bwd, tb.partial, tb.loss, tb.csum, tb.step, tb.grad,
+                    tb.slot_fwd, tb.slot_bwd, d_stage};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (int l = 0; l < 5; l++) {
         if (tb.X[l]) (void)hipFree(tb.X[l]);
@@ -111,6 +112,11 @@ int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_
     if (e == hipSuccess) e = hipMalloc(&g->tb.csum, sizeof(float) * (1 + nsplit) + 256);
     if (e == hipSuccess) e = hipMalloc(&g->tb.step, 256);
     if (e == hipSuccess) e = hipMalloc(&g->tb.grad, P4);
+    if (e == hipSuccess) e = hipMalloc(&g->tb.slot_fwd, sizeof(int32_t) * G::NPARAM);
+    if (e == hipSuccess) e = hipMalloc(&g->tb.slot_bwd, sizeof(int32_t) * G::NPARAM);
+    if (e == hipSuccess) e = hipMemset(g->tb.slot_fwd, 0xff, sizeof(int32_t) * G::NPARAM);
+    if (e == hipSuccess) e = hipMemset(g->tb.slot_bwd, 0xff, sizeof(int32_t) * G::NPARAM);
+    if (e == hipSuccess) e = G::launch_slot_map(g->tb.slot_fwd, g->tb.slot_bwd, nullptr);
     if (e != hipSuccess) { delete g; return fail(VGPU_ERR_HIP, "guide buffers: %s", hipGetErrorString(e)); }
     g->tb.ldb = max_batch;
     std::vector<float> P;

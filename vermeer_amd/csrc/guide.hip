@@ -12,11 +12,12 @@
 // Weight fragments are packed once per parameter update (k_pack) and staged
 // into LDS by every workgroup (50 KB forward, +32 KB transposed for backward).
 //
-// Training (one step over B samples): k_csum_part / k_csum_final sum the sample weights c;
-// k_train runs forward, the loss gradient and the backward pass in registers,
+// Training (one step over B samples): k_csum_part sums the sample weights c per split of
+// KSPLIT samples and k_train adds the split sums; k_train runs forward, the loss gradient and the backward pass in registers,
 // writing every layer input X and output gradient dY in feature-major bf16;
 // k_dw forms the weight gradients dW = dY X^T as split-K MFMA GEMMs into fp32
-// partials; k_adam reduces the partials in a fixed order and applies Adam.
+// partials; k_adam reduces the partials in a fixed order, applies Adam and writes each
+// updated weight's bf16 copy straight into its fragment slots.
 
 #include "guide_kernels.h"
 
@@ -55,32 +56,57 @@ __device__ __forceinline__ bf16x8 frag(const u32x4* s, int f, int lane) {
 
 // ---------------------------------------------------------------- packing
 
-__device__ float pack_fwd(const float* P, int f, int lane, int i) {
+// Parameter held in element i of lane `lane` of forward fragment f (-1: a zero pad).  Every weight
+// of the five layers is held exactly once.
+__device__ int pack_fwd_index(int f, int lane, int i) {
     const int r = lane & 15, g = lane >> 4;
     if (f < F_L2) {
         const int o = f / 5, j = f % 5, q = 32 * j + 8 * g + i;
-        return q < NENC ? P[OW1 + (16 * o + r) * NENC + q] : 0.0f;
+        return q < NENC ? OW1 + (16 * o + r) * NENC + q : -1;
     }
     if (f < F_L5) {
         const int l = (f - F_L2) / 8, ff = (f - F_L2) % 8, o = ff / 2, j = ff % 2;
         const int ow = l == 0 ? OW2 : (l == 1 ? OW3 : OW4);
-        return P[ow + (16 * o + r) * NH + hid_feature(j, g, i)];
+        return ow + (16 * o + r) * NH + hid_feature(j, g, i);
     }
     const int ff = f - F_L5, T = ff / 2, j = ff % 2;
     const int out = out5_index(r >> 2, 4 * T + (r & 3));
-    return out >= 0 ? P[OW5 + out * NH + hid_feature(j, g, i)] : 0.0f;
+    return out >= 0 ? OW5 + out * NH + hid_feature(j, g, i) : -1;
 }
 
-__device__ float pack_bwd(const float* P, int f, int lane, int i) {
+// The same for the transposed (backward) fragments of layers 5..2; each of their weights once.
+__device__ int pack_bwd_index(int f, int lane, int i) {
     const int r = lane & 15, g = lane >> 4;
     const int o = (f % 8) / 2, j = f % 2, row = 16 * o + r;
     if (f < FB_L4) {
         const int out = out5_index(g, 8 * j + i);
-        return out >= 0 ? P[OW5 + out * NH + row] : 0.0f;
+        return out >= 0 ? OW5 + out * NH + row : -1;
     }
     const int l = (f - FB_L4) / 8;
     const int ow = l == 0 ? OW4 : (l == 1 ? OW3 : OW2);
-    return P[ow + hid_feature(j, g, i) * NH + row];
+    return ow + hid_feature(j, g, i) * NH + row;
+}
+
+__device__ float pack_fwd(const float* P, int f, int lane, int i) {
+    const int k = pack_fwd_index(f, lane, i);
+    return k >= 0 ? P[k] : 0.0f;
+}
+__device__ float pack_bwd(const float* P, int f, int lane, int i) {
+    const int k = pack_bwd_index(f, lane, i);
+    return k >= 0 ? P[k] : 0.0f;
+}
+
+// Inverse of the packing: slot_fwd[p] / slot_bwd[p] = the bf16 element (index into the fragment
+// array as uint16) that holds parameter p, or -1 (biases; layer 1 has no backward fragment).
+// Both arrays are -1 on entry.
+__global__ void k_slot_map(int32_t* __restrict__ slot_fwd, int32_t* __restrict__ slot_bwd) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (NF_FWD + NF_BWD) * 64 * 8) return;
+    const int e = t & 7, fl = t >> 3;
+    const bool fwd = fl < NF_FWD * 64;
+    const int idx = fwd ? fl : fl - NF_FWD * 64;
+    const int k = fwd ? pack_fwd_index(idx >> 6, idx & 63, e) : pack_bwd_index(idx >> 6, idx & 63, e);
+    if (k >= 0) (fwd ? slot_fwd : slot_bwd)[k] = idx * 8 + e;
 }
 
 __global__ void k_pack(const float* __restrict__ P, u32x4* __restrict__ ffwd, u32x4* __restrict__ fbwd) {
@@ -361,11 +387,14 @@ __global__ __launch_bounds__(256) void k_csum_part(const TrainSample* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(64) void k_csum_final(const float* __restrict__ part, uint32_t nsplit, float* __restrict__ csum) {
-    if (threadIdx.x != 0) return;
+// Sum of the split sums part[0 .. nsplit): lane l adds part[l], part[l + 64], ... and the 64 lane sums
+// are added pairwise across the wavefront, the same order in every workgroup that calls it.
+__device__ __forceinline__ float batch_csum(const float* __restrict__ part, uint32_t nsplit, int lane) {
     float a = 0.0f;
-    for (uint32_t i = 0; i < nsplit; i++) a += part[i];
-    csum[0] = a;
+    for (uint32_t i = (uint32_t)lane; i < nsplit; i += 64) a += part[i];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+    return a;
 }
 
 // Masks the back-propagated tile with the layer input (ReLU derivative), stores it and repacks
@@ -405,7 +434,10 @@ __device__ __forceinline__ void layer_back(const u32x4* sB, int fbase, int lane,
 }
 
 // 8 waves x 32 samples per workgroup.
-__global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __restrict__ S, uint32_t B, TrainBuffers tb) {
+// update: a parameter update follows (k_adam); workgroup 0 then advances the Adam step counter
+// when the batch has a positive weight sum, before k_adam reads it.
+__global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __restrict__ S, uint32_t B, TrainBuffers tb,
+                                               int update) {
     constexpr int NT = 2;
     extern __shared__ u32x4 lds[];
     u32x4* sF = lds;
@@ -417,7 +449,13 @@ __global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __res
     const uint64_t base = (uint64_t)blockIdx.x * 256 + wave * 32;
     const float* P = net.params;
     const uint32_t ldb = tb.ldb;
-    const float csum = tb.csum[0];
+    // every wavefront adds the split sums itself (no separate launch); workgroup 0 keeps the
+    // total for k_adam and advances the step counter
+    const float csum = batch_csum(tb.csum + 1, B / KSPLIT, lane);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        tb.csum[0] = csum;
+        if (update && csum > 0.0f) tb.step[0] += 1;
+    }
     const float cscale = csum > 0.0f ? (float)B / csum : 0.0f;
 
     float x[NT][6], dir[NT][3], cw[NT];
@@ -563,31 +601,51 @@ __global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, ui
     }
 }
 
+// Adds the weight-gradient partials (split order), applies Adam and writes the updated parameter's
+// bf16 copy into its forward and backward fragment slots (so no separate repacking pass).
 __global__ void k_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
                        const float* __restrict__ partial, uint32_t nsplit, float invB, float lr,
-                       const float* __restrict__ csum, const int32_t* __restrict__ step, int update) {
+                       const float* __restrict__ csum, const int32_t* __restrict__ step, int update,
+                       const int32_t* __restrict__ slot_fwd, const int32_t* __restrict__ slot_bwd,
+                       uint16_t* __restrict__ ffwd, uint16_t* __restrict__ fbwd) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= NPARAM) return;
     float gs = 0.0f;
-    for (uint32_t s = 0; s < nsplit; s++) gs += partial[(uint64_t)s * NPARAM + p];
+    uint32_t s = 0;
+    for (; s + 16 <= nsplit; s += 16) {   // 16 loads in flight, then the adds in split order
+        float t[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) t[j] = partial[(uint64_t)(s + j) * NPARAM + p];
+#pragma unroll
+        for (int j = 0; j < 16; j++) gs += t[j];
+    }
+    for (; s < nsplit; s++) gs += partial[(uint64_t)s * NPARAM + p];
     const float gr = gs * invB;
     grad[p] = gr;
     if (!update || !(csum[0] > 0.0f)) return;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const int t = step[0] + 1;
+    const int t = step[0];   // advanced by k_train for this update
     const float mm = b1 * m[p] + (1.0f - b1) * gr;
     const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
     m[p] = mm;
     v[p] = vv;
     const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
-    P[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    const float np = P[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    P[p] = np;
+    const uint16_t h = bf16_bits((__bf16)np);
+    const int sf = slot_fwd[p], sb = slot_bwd[p];
+    if (sf >= 0) ffwd[sf] = h;
+    if (sb >= 0) fbwd[sb] = h;
 }
 
-__global__ void k_step(int32_t* step, const float* csum) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && csum[0] > 0.0f) step[0] += 1;
-}
 
 }  // namespace
+
+hipError_t launch_slot_map(int32_t* slot_fwd, int32_t* slot_bwd, hipStream_t st) {
+    const int total = (NF_FWD + NF_BWD) * 64 * 8;
+    hipLaunchKernelGGL(k_slot_map, dim3((total + 255) / 256), dim3(256), 0, st, slot_fwd, slot_bwd);
+    return hipGetLastError();
+}
 
 hipError_t launch_pack(const float* params, uint4* ffwd, uint4* fbwd, hipStream_t st) {
     const int total = (NF_FWD + NF_BWD) * 64;
@@ -608,6 +666,11 @@ hipError_t launch_infer(const Net& net, const float4* queries, uint64_t n, const
     return hipGetLastError();
 }
 
+// k_adam's workgroup size (small, so that its 24k parameters spread over many CUs) and the samples
+// per weight-gradient split (more splits: more waves in k_dw, more partials for k_adam to add).
+constexpr int kAdamBlock = 64;
+constexpr uint32_t kDwSplit = 2 * KSPLIT;
+
 hipError_t launch_train(const Net& net, float* params, float* m, float* v, const TrainSample* samples, uint32_t B,
                         float lr, bool update, const TrainBuffers& tb, hipStream_t st) {
     if (B == 0 || B % KSPLIT != 0 || B > tb.ldb) return hipErrorInvalidValue;
@@ -620,20 +683,16 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
     }
     const uint32_t nsplit = B / KSPLIT;
     hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, samples, tb.csum + 1);
-    hipLaunchKernelGGL(k_csum_final, dim3(1), dim3(64), 0, st, (const float*)(tb.csum + 1), nsplit, tb.csum);
-    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb);
-    // weight gradients in splits of 4096 samples when the batch allows (fewer partials to reduce)
-    const uint32_t ks = B % (2 * KSPLIT) == 0 ? 2 * KSPLIT : KSPLIT;
+    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb, update ? 1 : 0);
+    // weight gradients in splits of kDwSplit samples when the batch allows, else KSPLIT
+    const uint32_t ks = B % kDwSplit == 0 ? kDwSplit : KSPLIT;
     const uint32_t nsw = B / ks;
     const uint32_t waves = NJOB * nsw;
     hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsw, ks);
-    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, params, m, v, tb.grad, tb.partial, nsw,
-                       1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0);
-    if (update) {
-        hipLaunchKernelGGL(k_step, dim3(1), dim3(64), 0, st, tb.step, tb.csum);
-        hipLaunchKernelGGL(k_pack, dim3(((NF_FWD + NF_BWD) * 64 + 255) / 256), dim3(256), 0, st, (const float*)params,
-                           (u32x4*)net.frag_fwd, (u32x4*)net.frag_bwd);
-    }
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + kAdamBlock - 1) / kAdamBlock), dim3(kAdamBlock), 0, st, params, m, v,
+                       tb.grad, tb.partial, nsw, 1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0,
+                       (const int32_t*)tb.slot_fwd, (const int32_t*)tb.slot_bwd, (uint16_t*)net.frag_fwd,
+                       (uint16_t*)net.frag_bwd);
     return hipGetLastError();
 }
 

@@ -48,11 +48,15 @@ struct TrainBuffers {
     float* csum;        // [0]: sum of c over the batch; [1 .. 1 + max_batch/KSPLIT): per-split partial sums
     int32_t* step;      // Adam step count (device)
     float* grad;        // last mean gradient (NPARAM)
+    int32_t* slot_fwd;  // NPARAM: bf16 element of the forward fragments holding each parameter, -1: none
+    int32_t* slot_bwd;  // NPARAM: the same for the backward fragments
     uint32_t ldb;       // row stride (samples) of X / dY
 };
 
 // Rebuilds the bf16 fragments from the fp32 master copy.
 hipError_t launch_pack(const float* params, uint4* frag_fwd, uint4* frag_bwd, hipStream_t st);
+// Fills the parameter -> fragment slot maps (both arrays set to -1 beforehand).
+hipError_t launch_slot_map(int32_t* slot_fwd, int32_t* slot_bwd, hipStream_t st);
 
 // Inference on n queries (2 float4 each: position, normal).  When n_dev is not null the
 // count is min(n, *n_dev).  raw = true writes the 40 raw outputs in canonical order,
