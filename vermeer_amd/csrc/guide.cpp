@@ -107,7 +107,7 @@ int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_
         if (e == hipSuccess) e = hipMalloc(&g->tb.dY[l], (size_t)yrows[l] * max_batch * 2);
         if (e == hipSuccess) e = hipMemset(g->tb.dY[l], 0, (size_t)yrows[l] * max_batch * 2);
     }
-    if (e == hipSuccess) e = hipMalloc(&g->tb.partial, (size_t)nsplit * P4);
+    if (e == hipSuccess) e = hipMalloc(&g->tb.partial, (size_t)(max_batch / G::DW_SPLIT_MIN) * P4);
     if (e == hipSuccess) e = hipMalloc(&g->tb.loss, sizeof(float) * max_batch);
     if (e == hipSuccess) e = hipMalloc(&g->tb.csum, sizeof(float) * (1 + nsplit) + 256);
     if (e == hipSuccess) e = hipMalloc(&g->tb.step, 256);

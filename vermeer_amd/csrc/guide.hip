@@ -544,60 +544,72 @@ __global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __res
     mask_pack_store<NT>(acc, X2, D, tb.dY[0], ldb, g, s);
 }
 
-// Weight-gradient partials.  One wave per (strip of 16x16 tiles of some dW_l that share their output
-// rows, split of ks samples): the wave loads each dY fragment once for the 3 or 4 tiles of its strip,
-// so it issues one A and nit B loads per nit MFMAs.  Strips: dW1 (64 x 144) as 4 row tiles x 3
-// strips of 3 column tiles; dW2..dW4 (64 x 64) as 4 row tiles x 1 strip of 4; dW5 (40 x 64, rows
-// padded to 48) as 3 x 1 strip of 4.  Each tile accumulates its samples in the same order as one
-// wave per tile would.
-constexpr int NJOB = 4 * 3 + 3 * 4 + 3;
+// Weight-gradient partials.  One wave per (column strip of some dW_l, split of ks samples): the
+// wave holds every row tile of its strip (4, or 3 for dW5), so it loads each dY and X fragment once
+// per split: nor + nit loads per nor x nit MFMAs.  Strips: dW1 (64 x 144) as 3 strips of 3 column
+// tiles; dW2..dW4 (64 x 64) one strip of 4; dW5 (40 x 64, rows padded to 48) one strip of 4 with 3
+// row tiles.  Each tile accumulates its samples in the same order as one wave per tile would.
+constexpr int NJOB = 3 + 3 + 1;
 
 __global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, uint32_t ks) {
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t job = w % NJOB, split = w / NJOB;
     if (split >= nsplit) return;
-    int l, ot, it0, nit, nin, nout;
-    if (job < 12) { l = 0; ot = job / 3; it0 = 3 * (job % 3); nit = 3; nin = NENC; nout = NH; }
-    else if (job < 24) { l = 1 + (job - 12) / 4; ot = (job - 12) % 4; it0 = 0; nit = 4; nin = NH; nout = NH; }
-    else { l = 4; ot = job - 24; it0 = 0; nit = 4; nin = NH; nout = NOUT; }
+    int l, it0, nit, nin, nout, nor;
+    if (job < 3) { l = 0; it0 = 3 * job; nit = 3; nin = NENC; nout = NH; nor = 4; }
+    else if (job < 6) { l = job - 2; it0 = 0; nit = 4; nin = NH; nout = NH; nor = 4; }
+    else { l = 4; it0 = 0; nit = 4; nin = NH; nout = NOUT; nor = 3; }
     const uint32_t ldb = tb.ldb;
-    const uint16_t* A = tb.dY[l] + (uint64_t)(16 * ot + c) * ldb;
+    const uint16_t* Ap[4];
     const uint16_t* Bp[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) Bp[k] = tb.X[l] + (uint64_t)(16 * (it0 + (k < nit ? k : 0)) + c) * ldb;
-    f32x4 acc[4];
+    for (int k = 0; k < 4; k++) {
+        Ap[k] = tb.dY[l] + (uint64_t)(16 * (k < nor ? k : 0) + c) * ldb;
+        Bp[k] = tb.X[l] + (uint64_t)(16 * (it0 + (k < nit ? k : 0)) + c) * ldb;
+    }
+    f32x4 acc[4][4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    float bs = 0.0f;
+    for (int o = 0; o < 4; o++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[o][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     const uint64_t s_end = (uint64_t)(split + 1) * ks;
-#pragma unroll 2
     for (uint64_t s0 = (uint64_t)split * ks; s0 < s_end; s0 += 32) {
-        const u32x4 av = *reinterpret_cast<const u32x4*>(A + s0 + 8 * g);
-        u32x4 bv[4];
+        u32x4 av[4], bv[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < 4; k++) {
+            if (k < nor) av[k] = *reinterpret_cast<const u32x4*>(Ap[k] + s0 + 8 * g);
             if (k < nit) bv[k] = *reinterpret_cast<const u32x4*>(Bp[k] + s0 + 8 * g);
-        const bf16x8 a = __builtin_bit_cast(bf16x8, av);
+        }
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (k < nit) acc[k] = mfma(a, __builtin_bit_cast(bf16x8, bv[k]), acc[k]);
-        if (it0 == 0)
-            for (int i = 0; i < 8; i++) bs += bf16_val(a[i]);
+        for (int o = 0; o < 4; o++) {
+            if (o >= nor) continue;
+            const bf16x8 a = __builtin_bit_cast(bf16x8, av[o]);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < nit) acc[o][k] = mfma(a, __builtin_bit_cast(bf16x8, bv[k]), acc[o][k]);
+            if (it0 == 0)
+                for (int i = 0; i < 8; i++) bs[o] += bf16_val(a[i]);
+        }
     }
     const int ow[5] = {OW1, OW2, OW3, OW4, OW5}, ob[5] = {OB1, OB2, OB3, OB4, OB5};
     float* Pp = tb.partial + (uint64_t)split * NPARAM;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if (k >= nit) continue;
-        for (int r = 0; r < 4; r++) {
-            const int out = 16 * ot + 4 * g + r;
-            if (out < nout) Pp[ow[l] + out * nin + 16 * (it0 + k) + c] = acc[k][r];
+    for (int o = 0; o < 4; o++) {
+        if (o >= nor) continue;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k >= nit) continue;
+            for (int r = 0; r < 4; r++) {
+                const int out = 16 * o + 4 * g + r;
+                if (out < nout) Pp[ow[l] + out * nin + 16 * (it0 + k) + c] = acc[o][k][r];
+            }
         }
-    }
-    if (it0 == 0) {
-        bs = quad_sum(bs);
-        if (g == 0 && 16 * ot + c < nout) Pp[ob[l] + 16 * ot + c] = bs;
+        if (it0 == 0) {
+            const float b = quad_sum(bs[o]);
+            if (g == 0 && 16 * o + c < nout) Pp[ob[l] + 16 * o + c] = b;
+        }
     }
 }
 
@@ -666,10 +678,8 @@ hipError_t launch_infer(const Net& net, const float4* queries, uint64_t n, const
     return hipGetLastError();
 }
 
-// k_adam's workgroup size (small, so that its 24k parameters spread over many CUs) and the samples
-// per weight-gradient split (more splits: more waves in k_dw, more partials for k_adam to add).
+// k_adam's workgroup size: small, so that its 24k parameters spread over many CUs.
 constexpr int kAdamBlock = 64;
-constexpr uint32_t kDwSplit = 2 * KSPLIT;
 
 hipError_t launch_train(const Net& net, float* params, float* m, float* v, const TrainSample* samples, uint32_t B,
                         float lr, bool update, const TrainBuffers& tb, hipStream_t st) {
@@ -684,8 +694,8 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
     const uint32_t nsplit = B / KSPLIT;
     hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, samples, tb.csum + 1);
     hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb, update ? 1 : 0);
-    // weight gradients in splits of kDwSplit samples when the batch allows, else KSPLIT
-    const uint32_t ks = B % kDwSplit == 0 ? kDwSplit : KSPLIT;
+    // weight gradients in splits of DW_SPLIT samples when the batch allows, else KSPLIT
+    const uint32_t ks = B % DW_SPLIT == 0 ? DW_SPLIT : KSPLIT;
     const uint32_t nsw = B / ks;
     const uint32_t waves = NJOB * nsw;
     hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsw, ks);

@@ -22,7 +22,12 @@ static_assert(NPARAM == 24360, "parameter count");
 // bf16 operand fragments of the 16x16x32 MFMA, 64 lanes x 16 bytes each.
 constexpr int NF_FWD = 4 * 5 + 3 * 4 * 2 + 3 * 2;   // W1 (4 out tiles x 5 k-chunks), W2..W4, W5 (3 x 2)
 constexpr int NF_BWD = 4 * 2 * 4;                   // W5^T, W4^T, W3^T, W2^T (4 in tiles x 2 k-chunks)
-constexpr int KSPLIT = 1024;                        // batch granularity; weight-gradient partials cover 1024 or 2048
+constexpr int KSPLIT = 1024;                        // batch granularity (a batch is a multiple of it)
+// Samples per weight-gradient split (k_dw): more splits give k_dw more waves and k_adam more
+// partials to add.  A batch that is not a multiple of DW_SPLIT uses KSPLIT.
+constexpr int DW_SPLIT = 512;
+constexpr int DW_SPLIT_MIN = DW_SPLIT < KSPLIT ? DW_SPLIT : KSPLIT;
+static_assert(KSPLIT % DW_SPLIT_MIN == 0 && DW_SPLIT % 32 == 0, "splits tile the batch");
 constexpr float KAPPA_RAW_MIN = -5.0f, KAPPA_RAW_MAX = 7.0f;
 
 // One training sample: hit position + c, shading normal, sampled direction (unit).
