@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--pt-steps", type=int, default=32, help="Adam steps per pass")
     ap.add_argument("--pt-batch", type=int, default=65536)
     ap.add_argument("--pt-lr", type=float, default=1e-3)
-    ap.add_argument("--pt-leaf", type=int, default=2, help="BVH leaf size of the room (16 = the reference's)")
+    ap.add_argument("--pt-leaf", type=int, default=16, help="BVH leaf size of the room (16: the reference's)")
     ap.add_argument("--pt-warm", type=int, default=0,
                     help="guided passes left out of the image (their time still counts)")
     return ap.parse_args()
