@@ -40,7 +40,8 @@ extern "C" {
 
 /* Ray record, 32 bytes.  P/D as core.Ray.P/D (core/ray.go:28), tmax is the
  * initial core.Ray.Tclosest (Ray.Init maxdist, core/ray.go:59), time is
- * core.Ray.Time in [0,1) for motion blur. */
+ * core.Ray.Time in [0,1] for motion blur (a time outside it is held to the
+ * first or last key, where the reference would index out of range). */
 typedef struct vgpu_ray {
     float P[3];
     float tmax;

@@ -364,3 +364,20 @@ def test_mixed_motion_scene_every_config_bitexact(mixed_motion, cfg):
             assert len(np.unique(geoms[geoms >= 0])) >= 4
     finally:
         g.set_trace_config(256, 16, 16)
+
+
+def test_motion_times_outside_the_shutter_are_held_to_the_end_keys(mixed_motion):
+    """A ray time below 0 traces as time 0 and one above 1 as time 1 (the reference would index its
+    keys out of range); a NaN time traces without fault."""
+    g, rays, _ = mixed_motion
+    r = rays[:50_000].copy()
+    out = {}
+    for name, t in (("lo", -0.5), ("zero", 0.0), ("hi", 1.5), ("one", 1.0), ("nan", np.nan)):
+        r["time"] = np.float32(t)
+        print(f"[case] motion time {t}", flush=True)
+        out[name] = g.trace(r)
+    for a, b in (("lo", "zero"), ("hi", "one")):
+        (ha, ga, _), (hb, gb, _) = out[a], out[b]
+        assert np.array_equal(ha.view(np.uint32).reshape(len(ha), -1), hb.view(np.uint32).reshape(len(hb), -1))
+        assert np.array_equal(ga, gb)
+    assert (out["zero"][0]["prim"] >= 0).mean() > 0.01

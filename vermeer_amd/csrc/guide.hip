@@ -19,6 +19,8 @@
 // partials; k_adam reduces the partials in a fixed order, applies Adam and writes each
 // updated weight's bf16 copy straight into its fragment slots.
 
+#include <atomic>
+
 #include "guide_kernels.h"
 
 namespace vgpu {
@@ -551,44 +553,38 @@ __global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __res
 // row tiles.  Each tile accumulates its samples in the same order as one wave per tile would.
 constexpr int NJOB = 3 + 3 + 1;
 
-__global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, uint32_t ks) {
-    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t job = w % NJOB, split = w / NJOB;
-    if (split >= nsplit) return;
-    int l, it0, nit, nin, nout, nor;
-    if (job < 3) { l = 0; it0 = 3 * job; nit = 3; nin = NENC; nout = NH; nor = 4; }
-    else if (job < 6) { l = job - 2; it0 = 0; nit = 4; nin = NH; nout = NH; nor = 4; }
-    else { l = 4; it0 = 0; nit = 4; nin = NH; nout = NOUT; nor = 3; }
+// One strip: NOR row tiles x NIT column tiles starting at column tile it0 of dW_l.
+template <int NOR, int NIT>
+__device__ __forceinline__ void dw_strip(const TrainBuffers& tb, int l, int it0, int nin, int nout, uint32_t split,
+                                         uint32_t ks, int lane) {
+    const int g = lane >> 4, c = lane & 15;
     const uint32_t ldb = tb.ldb;
-    const uint16_t* Ap[4];
-    const uint16_t* Bp[4];
+    const uint16_t* Ap[NOR];
+    const uint16_t* Bp[NIT];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        Ap[k] = tb.dY[l] + (uint64_t)(16 * (k < nor ? k : 0) + c) * ldb;
-        Bp[k] = tb.X[l] + (uint64_t)(16 * (it0 + (k < nit ? k : 0)) + c) * ldb;
-    }
-    f32x4 acc[4][4];
+    for (int o = 0; o < NOR; o++) Ap[o] = tb.dY[l] + (uint64_t)(16 * o + c) * ldb;
 #pragma unroll
-    for (int o = 0; o < 4; o++)
+    for (int k = 0; k < NIT; k++) Bp[k] = tb.X[l] + (uint64_t)(16 * (it0 + k) + c) * ldb;
+    f32x4 acc[NOR][NIT];
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc[o][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    float bs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int o = 0; o < NOR; o++)
+#pragma unroll
+        for (int k = 0; k < NIT; k++) acc[o][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bs[NOR];
+#pragma unroll
+    for (int o = 0; o < NOR; o++) bs[o] = 0.0f;
     const uint64_t s_end = (uint64_t)(split + 1) * ks;
     for (uint64_t s0 = (uint64_t)split * ks; s0 < s_end; s0 += 32) {
-        u32x4 av[4], bv[4];
+        u32x4 av[NOR], bv[NIT];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (k < nor) av[k] = *reinterpret_cast<const u32x4*>(Ap[k] + s0 + 8 * g);
-            if (k < nit) bv[k] = *reinterpret_cast<const u32x4*>(Bp[k] + s0 + 8 * g);
-        }
+        for (int o = 0; o < NOR; o++) av[o] = *reinterpret_cast<const u32x4*>(Ap[o] + s0 + 8 * g);
 #pragma unroll
-        for (int o = 0; o < 4; o++) {
-            if (o >= nor) continue;
+        for (int k = 0; k < NIT; k++) bv[k] = *reinterpret_cast<const u32x4*>(Bp[k] + s0 + 8 * g);
+#pragma unroll
+        for (int o = 0; o < NOR; o++) {
             const bf16x8 a = __builtin_bit_cast(bf16x8, av[o]);
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (k < nit) acc[o][k] = mfma(a, __builtin_bit_cast(bf16x8, bv[k]), acc[o][k]);
+            for (int k = 0; k < NIT; k++) acc[o][k] = mfma(a, __builtin_bit_cast(bf16x8, bv[k]), acc[o][k]);
             if (it0 == 0)
                 for (int i = 0; i < 8; i++) bs[o] += bf16_val(a[i]);
         }
@@ -596,21 +592,28 @@ __global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, ui
     const int ow[5] = {OW1, OW2, OW3, OW4, OW5}, ob[5] = {OB1, OB2, OB3, OB4, OB5};
     float* Pp = tb.partial + (uint64_t)split * NPARAM;
 #pragma unroll
-    for (int o = 0; o < 4; o++) {
-        if (o >= nor) continue;
+    for (int o = 0; o < NOR; o++) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (k >= nit) continue;
+        for (int k = 0; k < NIT; k++)
             for (int r = 0; r < 4; r++) {
                 const int out = 16 * o + 4 * g + r;
                 if (out < nout) Pp[ow[l] + out * nin + 16 * (it0 + k) + c] = acc[o][k][r];
             }
-        }
         if (it0 == 0) {
             const float b = quad_sum(bs[o]);
             if (g == 0 && 16 * o + c < nout) Pp[ob[l] + 16 * o + c] = b;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, uint32_t ks) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t job = w % NJOB, split = w / NJOB;
+    if (split >= nsplit) return;
+    if (job < 3) dw_strip<4, 3>(tb, 0, 3 * (int)job, NENC, NH, split, ks, lane);
+    else if (job < 6) dw_strip<4, 4>(tb, (int)job - 2, 0, NH, NH, split, ks, lane);
+    else dw_strip<3, 4>(tb, 4, 0, NH, NOUT, split, ks, lane);
 }
 
 // Adds the weight-gradient partials (split order), applies Adam and writes the updated parameter's
@@ -678,18 +681,25 @@ hipError_t launch_infer(const Net& net, const float4* queries, uint64_t n, const
     return hipGetLastError();
 }
 
+constexpr int kMaxDevices = 64;
+
 // k_adam's workgroup size: small, so that its 24k parameters spread over many CUs.
 constexpr int kAdamBlock = 64;
 
 hipError_t launch_train(const Net& net, float* params, float* m, float* v, const TrainSample* samples, uint32_t B,
                         float lr, bool update, const TrainBuffers& tb, hipStream_t st) {
     if (B == 0 || B % KSPLIT != 0 || B > tb.ldb) return hipErrorInvalidValue;
-    static bool attr_set = false;
+    // k_train's dynamic LDS (84 KB) exceeds the default limit: raise it once per device
     const size_t lds = (size_t)(NF_FWD + NF_BWD) * 64 * 16;
-    if (!attr_set) {
-        const hipError_t e = hipFuncSetAttribute((const void*)k_train, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static std::atomic<bool> attr_set[kMaxDevices];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!attr_set[dev].load(std::memory_order_acquire)) {
+        e = hipFuncSetAttribute((const void*)k_train, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
-        attr_set = true;
+        attr_set[dev].store(true, std::memory_order_release);
     }
     const uint32_t nsplit = B / KSPLIT;
     hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, samples, tb.csum + 1);
