@@ -194,8 +194,8 @@ int vgpu_guide_train_step(vgpu_guide* g, uint32_t flags, const float* samples, u
         ds = (const G::TrainSample*)g->d_stage;
         HIP_TRY(hipMemcpyAsync((void*)ds, samples, (size_t)n * sizeof(G::TrainSample), hipMemcpyHostToDevice, st));
     }
-    HIP_TRY(G::launch_train(g->net(), g->d_params, g->d_m, g->d_v, ds, n, lr, (flags & VGPU_GUIDE_NO_UPDATE) == 0,
-                            g->tb, st));
+    HIP_TRY(G::launch_train(g->net(), g->d_params, g->d_m, g->d_v, G::SampleSource{ds, nullptr, 0, 0}, n, lr,
+                            (flags & VGPU_GUIDE_NO_UPDATE) == 0, g->tb, st));
     if (host) {
         if (loss_out) {
             std::vector<float> L(n);

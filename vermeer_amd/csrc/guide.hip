@@ -375,11 +375,26 @@ __global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict
 
 // Sum of the sample weights in two deterministic stages: one partial per KSPLIT samples, then the
 // partials in order (the partial area is the tail of the weight-gradient partial buffer's first row).
-__global__ __launch_bounds__(256) void k_csum_part(const TrainSample* __restrict__ S, float* __restrict__ part) {
+__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {   // C. Wellons' integer hash
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+// Batch slot i of a training step (SampleSource in guide_kernels.h).
+__device__ __forceinline__ TrainSample fetch_sample(const SampleSource& src, uint32_t i) {
+    if (!src.pick_count) return src.S[i];
+    uint32_t n = *src.pick_count;
+    if (n > src.pick_cap) n = src.pick_cap;
+    TrainSample t{};
+    if (n > 0) t = src.S[lowbias32(i ^ lowbias32(src.seed)) % n];
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_csum_part(SampleSource src, float* __restrict__ part) {
     __shared__ float red[256];
     const uint32_t base = blockIdx.x * KSPLIT;
     float a = 0.0f;
-    for (uint32_t i = threadIdx.x; i < (uint32_t)KSPLIT; i += 256) a += S[base + i].c;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)KSPLIT; i += 256) a += fetch_sample(src, base + i).c;
     red[threadIdx.x] = a;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -438,8 +453,7 @@ __device__ __forceinline__ void layer_back(const u32x4* sB, int fbase, int lane,
 // 8 waves x 32 samples per workgroup.
 // update: a parameter update follows (k_adam); workgroup 0 then advances the Adam step counter
 // when the batch has a positive weight sum, before k_adam reads it.
-__global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __restrict__ S, uint32_t B, TrainBuffers tb,
-                                               int update) {
+__global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32_t B, TrainBuffers tb, int update) {
     constexpr int NT = 2;
     extern __shared__ u32x4 lds[];
     u32x4* sF = lds;
@@ -465,7 +479,7 @@ __global__ __launch_bounds__(512) void k_train(Net net, const TrainSample* __res
 #pragma unroll
     for (int t = 0; t < NT; t++) {
         s[t] = base + 16 * t + c;
-        const TrainSample& ts = S[s[t]];
+        const TrainSample ts = fetch_sample(src, (uint32_t)s[t]);
         normalise(make_float4(ts.px, ts.py, ts.pz, 0.0f), make_float4(ts.nx, ts.ny, ts.nz, 0.0f), net.bounds, x[t]);
         dir[t][0] = ts.dx; dir[t][1] = ts.dy; dir[t][2] = ts.dz;
         cw[t] = ts.c;
@@ -686,7 +700,7 @@ constexpr int kMaxDevices = 64;
 // k_adam's workgroup size: small, so that its 24k parameters spread over many CUs.
 constexpr int kAdamBlock = 64;
 
-hipError_t launch_train(const Net& net, float* params, float* m, float* v, const TrainSample* samples, uint32_t B,
+hipError_t launch_train(const Net& net, float* params, float* m, float* v, const SampleSource& src, uint32_t B,
                         float lr, bool update, const TrainBuffers& tb, hipStream_t st) {
     if (B == 0 || B % KSPLIT != 0 || B > tb.ldb) return hipErrorInvalidValue;
     // k_train's dynamic LDS (84 KB) exceeds the default limit: raise it once per device
@@ -702,8 +716,9 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
         attr_set[dev].store(true, std::memory_order_release);
     }
     const uint32_t nsplit = B / KSPLIT;
-    hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, samples, tb.csum + 1);
-    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, samples, B, tb, update ? 1 : 0);
+    if (!src.S) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, src, tb.csum + 1);
+    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, src, B, tb, update ? 1 : 0);
     // weight gradients in splits of DW_SPLIT samples when the batch allows, else KSPLIT
     const uint32_t ks = B % DW_SPLIT == 0 ? DW_SPLIT : KSPLIT;
     const uint32_t nsw = B / ks;

@@ -69,9 +69,19 @@ hipError_t launch_slot_map(int32_t* slot_fwd, int32_t* slot_bwd, hipStream_t st)
 hipError_t launch_infer(const Net& net, const float4* queries, uint64_t n, const uint32_t* n_dev, float* out, bool raw,
                         int num_cus, hipStream_t st);
 
+// Where a training step reads its batch: S[i] for slot i < B; or, when pick_count is set, a
+// random draw from the first n = min(*pick_count, pick_cap) records of S: slot i takes record
+// lowbias32(i ^ lowbias32(seed)) mod n (no records: an all-zero sample, which trains nothing).
+struct SampleSource {
+    const TrainSample* S;
+    const uint32_t* pick_count;   // device; null: S is the batch itself
+    uint32_t pick_cap;
+    uint32_t seed;
+};
+
 // One training step on a batch of B samples (B a multiple of KSPLIT).  update = false computes
 // the mean gradient only.
-hipError_t launch_train(const Net& net, float* params, float* adam_m, float* adam_v, const TrainSample* samples,
+hipError_t launch_train(const Net& net, float* params, float* adam_m, float* adam_v, const SampleSource& src,
                         uint32_t B, float lr, bool update, const TrainBuffers& tb, hipStream_t st);
 
 }  // namespace guide

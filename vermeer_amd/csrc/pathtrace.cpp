@@ -211,7 +211,6 @@ int vgpu_pt_set_guide(vgpu_pt* pt, vgpu_guide* g, float alpha, uint32_t record_s
             e = pt->alloc((void**)&b.tsamples, cap * sizeof(vgpu::guide::TrainSample));
             if (e == hipSuccess) b.tcap = (uint32_t)cap;
         }
-        if (e == hipSuccess && !b.tbatch) e = pt->alloc((void**)&b.tbatch, (size_t)g->max_batch * sizeof(vgpu::guide::TrainSample));
     }
     if (e != hipSuccess) return fail(VGPU_ERR_HIP, "guide buffers: %s", hipGetErrorString(e));
     pt->guide = g;
@@ -276,13 +275,10 @@ int vgpu_pt_pass(vgpu_pt* pt, uint32_t flags, uint32_t pass, double* accum, void
         e = vgpu::pt_launch_gather(v, pass, bb, st);
         vgpu_guide* g = pt->guide;
         for (uint32_t k = 0; k < pt->train_steps && e == hipSuccess; k++) {
-            stage = "training batch";
-            e = vgpu::pt_launch_batch(bb, pt->batch, pass * 7919u + k * 104729u, st);
-            if (e == hipSuccess) {
-                stage = "training step";
-                e = vgpu::guide::launch_train(g->net(), g->d_params, g->d_m, g->d_v, bb.tbatch, pt->batch, pt->lr, true,
-                                              g->tb, st);
-            }
+            // each step draws its batch from the pass's training samples inside the training kernels
+            stage = "training step";
+            const vgpu::guide::SampleSource src{bb.tsamples, bb.count + 3, bb.tcap, pass * 7919u + k * 104729u};
+            e = vgpu::guide::launch_train(g->net(), g->d_params, g->d_m, g->d_v, src, pt->batch, pt->lr, true, g->tb, st);
         }
     }
     if (e != hipSuccess) return fail(VGPU_ERR_HIP, "path pass (%s): %s", stage, hipGetErrorString(e));

@@ -11,7 +11,8 @@
 //   (trace)        shadow any-hit over the compacted shadow queue
 //   k_pt_shadow    adds unoccluded NEE contributions
 //   k_pt_accumulate  adds the pass radiance to the FP64 accumulation
-//   k_pt_gather / k_pt_batch  training samples for the guide (guided mode)
+//   k_pt_gather     training samples for the guide (guided mode); each training step draws
+//                   its batch from them inside the guide's kernels
 //
 // Every float operation is written out in a fixed order and the file is
 // compiled with -ffp-contract=off, so that the CPU restatement
@@ -419,15 +420,6 @@ __global__ __launch_bounds__(256) void k_pt_gather(PTView v, uint32_t pass, PTBu
     }
 }
 
-__global__ __launch_bounds__(256) void k_pt_batch(PTBuffers b, uint32_t B, uint32_t seed) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= B) return;
-    uint32_t cnt = b.count[3];
-    if (cnt > b.tcap) cnt = b.tcap;
-    guide::TrainSample s{};
-    if (cnt > 0) s = b.tsamples[mix32(i ^ mix32(seed)) % cnt];
-    b.tbatch[i] = s;
-}
 
 }  // namespace
 
@@ -455,10 +447,6 @@ hipError_t pt_launch_accumulate(const PTView& v, const PTBuffers& b, double* acc
 }
 hipError_t pt_launch_gather(const PTView& v, uint32_t pass, const PTBuffers& b, hipStream_t st) {
     hipLaunchKernelGGL(k_pt_gather, dim3(grid_for((uint64_t)v.width * v.height)), dim3(256), 0, st, v, pass, b);
-    return hipGetLastError();
-}
-hipError_t pt_launch_batch(const PTBuffers& b, uint32_t B, uint32_t seed, hipStream_t st) {
-    hipLaunchKernelGGL(k_pt_batch, dim3(grid_for(B)), dim3(256), 0, st, b, B, seed);
     return hipGetLastError();
 }
 

@@ -77,7 +77,6 @@ struct PTBuffers {
     float4* rec;            // training records [pixel][depth][PT_REC_F4]
     int32_t* plen;          // recorded path length per pixel (-1: not recorded this pass)
     guide::TrainSample* tsamples;  // gathered training samples
-    guide::TrainSample* tbatch;    // one training batch
     uint32_t tcap;          // capacity of tsamples
 };
 
@@ -87,6 +86,5 @@ hipError_t pt_launch_scatter(const PTView& v, uint32_t pass, int cur, const PTBu
 hipError_t pt_launch_shadow(const PTView& v, const PTBuffers& b, hipStream_t st);
 hipError_t pt_launch_accumulate(const PTView& v, const PTBuffers& b, double* accum, hipStream_t st);
 hipError_t pt_launch_gather(const PTView& v, uint32_t pass, const PTBuffers& b, hipStream_t st);
-hipError_t pt_launch_batch(const PTBuffers& b, uint32_t B, uint32_t step_seed, hipStream_t st);
 
 }  // namespace vgpu
