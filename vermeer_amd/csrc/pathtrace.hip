@@ -233,17 +233,37 @@ __global__ __launch_bounds__(256) void k_pt_hit(PTView v, uint32_t pass, int cur
     b.query[2 * i + 1] = f4(N, 0.0f);
 }
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool take) {
-    const unsigned long long mask = __ballot(take);
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader < 0 ? 0 : leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+
+// Appends n entries per lane for the whole 256-thread workgroup with one atomic on `counter`
+// (same-address device atomics are slow when every wavefront issues its own); returns this lane's
+// first slot.  Lanes take consecutive slots in thread order.  Every thread of the workgroup calls it.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, uint32_t n, uint32_t* s_tot) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_tot[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < 4; w++) {
+            const uint32_t t = s_tot[w];
+            s_tot[w] = run;
+            run += t;
+        }
+        s_tot[4] = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_tot[4] + s_tot[wave] + (x - n);
+    __syncthreads();
+    return slot;
 }
 
 __global__ __launch_bounds__(256) void k_pt_scatter(PTView v, uint32_t pass, int cur, PTBuffers b) {
+    __shared__ uint32_t s_tot[5];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.count[cur];
     bool alive = false, shadow = false;
@@ -335,7 +355,7 @@ __global__ __launch_bounds__(256) void k_pt_scatter(PTView v, uint32_t pass, int
             }
         }
     }
-    const uint32_t si = wave_append(b.count + 2, shadow);
+    const uint32_t si = block_append(b.count + 2, shadow ? 1u : 0u, s_tot);
     if (shadow) {
         b.sray[2 * si] = f4(sorg, stmax);
         b.sray[2 * si + 1] = f4(sdir, 0.0f);
@@ -344,7 +364,7 @@ __global__ __launch_bounds__(256) void k_pt_scatter(PTView v, uint32_t pass, int
         b.scontrib[2 * si + 1] = f4(E, __int_as_float(R ? pixel * v.max_depth + depth : -1));
     }
     const int nxt = 1 - cur;
-    const uint32_t qi = wave_append(b.count + nxt, alive);
+    const uint32_t qi = block_append(b.count + nxt, alive ? 1u : 0u, s_tot);
     if (alive) {
         b.qray[nxt][2 * qi] = f4(offset(P, N), __builtin_inff());
         b.qray[nxt][2 * qi + 1] = f4(wdir, 0.0f);
@@ -382,41 +402,58 @@ __global__ __launch_bounds__(256) void k_pt_accumulate(PTView v, PTBuffers b, do
 //   R_j = E_j + w_j (wB_{j+1} Le_{j+1} + R_{j+1})        (radiance leaving j, emission excluded)
 //   L_in(j) = Le_{j+1} + R_{j+1}                         (radiance arriving at j along its sample)
 // Vertex j yields the sample (P_j, N_j, dir_j) with c = lum(L_in(j)) cos_j / p_j when c > 0.
-// The wave walks depth max_depth-1 .. 0 in lockstep (lanes past their path's end idle) so that each
-// depth's samples are appended with one atomic per wave.
+//
+// gather_step: one step of the backward walk over record j (next: j + 1 < n, a later vertex
+// exists); true when vertex j yields a sample, written to s.
+__device__ __forceinline__ bool gather_step(const float4* __restrict__ r, bool next, F3& Rn, F3& Len, float& wBn,
+                                            guide::TrainSample& s) {
+    bool emit = false;
+    const float4 r5 = r[5];
+    if (next && r5.w != 0.0f) {
+        const F3 Lin = add(Len, Rn);
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+        const float c = lum(Lin) * r1.w / r0.w;
+        if (c > 0.0f && c < 3.0e38f) {
+            emit = true;
+            s.px = r0.x; s.py = r0.y; s.pz = r0.z; s.c = c;
+            s.nx = r1.x; s.ny = r1.y; s.nz = r1.z;
+            s.dx = r2.x; s.dy = r2.y; s.dz = r2.z;
+        }
+    }
+    const F3 inner = next ? add(mul(Len, wBn), Rn) : f3(0, 0, 0);
+    const float4 r3 = r[3], r4 = r[4];
+    Rn = add(f3(r4), mul(f3(r5), inner));
+    Len = f3(r3);
+    wBn = r3.w;
+    return emit;
+}
+
+// Walks each recorded path backwards twice: once to count its training samples, then, after one
+// append for the whole workgroup, to write them to consecutive slots.
 __global__ __launch_bounds__(256) void k_pt_gather(PTView v, uint32_t pass, PTBuffers b) {
+    __shared__ uint32_t s_tot[5];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = p < v.width * v.height ? b.plen[p] : 0;
     const float4* R = b.rec + (size_t)(p < v.width * v.height ? p : 0) * v.max_depth * PT_REC_F4;
+    uint32_t cnt = 0;
+    {
+        F3 Rn = f3(0, 0, 0), Len = f3(0, 0, 0);
+        float wBn = 0.0f;
+        for (int j = n - 1; j >= 0; j--) {
+            guide::TrainSample s{};
+            if (gather_step(R + (size_t)j * PT_REC_F4, j + 1 < n, Rn, Len, wBn, s)) cnt++;
+        }
+    }
+    uint32_t k = block_append(b.count + 3, cnt, s_tot);
+    if (cnt == 0) return;
     F3 Rn = f3(0, 0, 0), Len = f3(0, 0, 0);
     float wBn = 0.0f;
-    for (int j = v.max_depth - 1; j >= 0; j--) {
-        const bool live = j < n;
-        bool emit = false;
+    for (int j = n - 1; j >= 0; j--) {
         guide::TrainSample s{};
-        if (live) {
-            const float4* r = R + (size_t)j * PT_REC_F4;
-            const bool next = j + 1 < n;
-            const float4 r5 = r[5];
-            if (next && r5.w != 0.0f) {
-                const F3 Lin = add(Len, Rn);
-                const float4 r0 = r[0], r1 = r[1], r2 = r[2];
-                const float c = lum(Lin) * r1.w / r0.w;
-                if (c > 0.0f && c < 3.0e38f) {
-                    emit = true;
-                    s.px = r0.x; s.py = r0.y; s.pz = r0.z; s.c = c;
-                    s.nx = r1.x; s.ny = r1.y; s.nz = r1.z;
-                    s.dx = r2.x; s.dy = r2.y; s.dz = r2.z;
-                }
-            }
-            const F3 inner = next ? add(mul(Len, wBn), Rn) : f3(0, 0, 0);
-            const float4 r3 = r[3], r4 = r[4];
-            Rn = add(f3(r4), mul(f3(r5), inner));
-            Len = f3(r3);
-            wBn = r3.w;
+        if (gather_step(R + (size_t)j * PT_REC_F4, j + 1 < n, Rn, Len, wBn, s)) {
+            if (k < b.tcap) b.tsamples[k] = s;
+            k++;
         }
-        const uint32_t k = wave_append(b.count + 3, emit);
-        if (emit && k < b.tcap) b.tsamples[k] = s;
     }
 }
 
