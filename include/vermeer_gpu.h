@@ -291,7 +291,8 @@ int vgpu_render_set_filter(vgpu_render* render, int32_t type, float width, int32
  * every sample's camera matrix is the lerp of the two keys bracketing its
  * time, recomposed as ComputeRay does (camera.go:224-236); one key sets the
  * single matrix of every sample to that recomposition (replacing the
- * vgpu_camera matrix); 0 keys keep the vgpu_camera matrix.  At most 64 keys. */
+ * vgpu_camera matrix); 0 keys go back to the vgpu_camera matrix given at
+ * creation.  At most 64 keys. */
 int vgpu_render_set_camera_keys(vgpu_render* render, const float* local_to_world, uint32_t nkeys);
 #define VGPU_FILTER_BOX 0
 #define VGPU_FILTER_GAUSSIAN 1

@@ -302,3 +302,26 @@ def test_render_camera_motion_keys_bitexact(moving_scene):
             g1.set_camera_keys(np.full((1, 16), np.nan, np.float32))
     finally:
         oracle.set_normalize_ieee(False)
+
+
+def test_camera_keys_reset_and_limits(feature_scene):
+    """Zero keys return to the creation-time camera matrix (also after a one-key call replaced it);
+    more than 64 keys are refused."""
+    oracle.set_normalize_ieee(True)
+    try:
+        w = h = 64
+        c = feature_scene["camera"]
+        cam, M, ttf = _camera(feature_scene, w, h)
+        keys = oracle.camera_lookat_keys([(0.3, 1.2, 3.2)], [c["to"]], c["up"])
+        g = render.Renderer(feature_scene, w, h, cam)
+        o = oracle.OracleRenderer(feature_scene, w, h, M, ttf, c["focal"])
+        scr = scenes.pixel_scrambles(w, h, seed=41)
+        g.set_camera_keys(keys)
+        g.set_camera_keys(np.zeros((0, 16), np.float32))
+        _, gs = g.render_pass(scr, 1, want_sample=True)
+        _, os_ = o.render_pass(scr, 1, want_sample=True)
+        assert _nan_equal(gs, os_)
+        with pytest.raises(Exception):
+            g.set_camera_keys(np.repeat(keys, 65, axis=0))
+    finally:
+        oracle.set_normalize_ieee(False)

@@ -286,6 +286,7 @@ struct vgpu_render {
     vgpu::LevelRec* d_recs = nullptr;   // per pixel, per level
     double* d_filter = nullptr;    // pixel filter sampler tables
     float* d_camkeys = nullptr;    // camera motion keys (kCamKeyFloats each)
+    float cam_M[16] = {};          // the vgpu_camera matrix given at creation
     float4* d_pix = nullptr;       // per pixel: D, lambda
     int4* d_hits32 = nullptr;      // extended hits (2 int4 per pixel)
     float4* d_slot_rays = nullptr;
@@ -368,6 +369,7 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     rv.max_half = std::max(1, max_ns / 2);
     rv.slots = std::max(1, rv.groups * 2 * rv.max_half);
     std::memcpy(rv.M, cam->M, sizeof(rv.M));
+    std::memcpy(r->cam_M, cam->M, sizeof(r->cam_M));
     rv.tan_theta_focal = cam->tan_theta_focal;
     rv.aspect = cam->aspect;
     rv.focal = cam->focal;
@@ -548,6 +550,8 @@ int vgpu_render_set_camera_keys(vgpu_render* r, const float* local_to_world, uin
     if (nkeys == 1) {   // one key: the matrix of every sample (the key lerped with itself, camera.go:224-236)
         const Mat4 M = decomp_matrix(recs.data(), recs.data(), 0.0f);
         for (int i = 0; i < 16; i++) r->rv.M[i] = M[i];
+    } else {            // none: the vgpu_camera matrix; two or more: per sample from the keys
+        std::memcpy(r->rv.M, r->cam_M, sizeof(r->rv.M));
     }
     if (r->d_camkeys) { HIP_TRY(hipFree(r->d_camkeys)); r->d_camkeys = nullptr; }
     r->rv.cam_keys = nullptr;
