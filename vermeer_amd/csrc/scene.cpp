@@ -486,6 +486,10 @@ int vgpu_scene_commit(vgpu_scene* s) {
         }
     }
     s->any_motion = maxKeys > 1;
+    // Default launch shape unless the caller chose one: static scenes run best at block 128 with a
+    // 12-entry LDS stack and 24 waves per CU; the motion kernel (about 120 VGPRs, four waves per
+    // SIMD) at block 256, 16 entries, 16 waves per CU.
+    if (!s->cfg_set) s->cfg = s->any_motion ? vgpu_trace_config{256, 16, 16, 0} : vgpu_trace_config{128, 12, 24, 0};
 
     // Flatten into the device layout.
     std::vector<vgpu::DevNode> dnodes;
@@ -742,6 +746,7 @@ int vgpu_scene_set_trace_config(vgpu_scene* s, const vgpu_trace_config* cfg) {
     int rc = check_config(*cfg);
     if (rc) return rc;
     s->cfg = *cfg;
+    s->cfg_set = true;
     return VGPU_OK;
 }
 
@@ -790,6 +795,7 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
     (void)hipEventDestroy(e1);
     vgpu::clear_error();
     s->cfg = bc;
+    s->cfg_set = true;
     if (best_out) *best_out = bc;
     if (ms_out) *ms_out = best;
     return vgpu_scene_status(s, stream);

@@ -121,6 +121,7 @@ struct vgpu_scene {
     vgpu::DevGeomShade* d_gshade = nullptr;
     int32_t max_material = -1;
     vgpu_trace_config cfg{256, 16, 16, 0};
+    bool cfg_set = false;               // the caller chose cfg (set_trace_config or autotune)
     int32_t leaf_max = 16;              // leafMax of the static meshes' BuildAccel (buildqbvh.go:32: 16)
     float box[6] = {0, 0, 0, 0, 0, 0};   // scene bounds over all keys (ray-sort grid)
     vgpu::RaySortBuffers sort{};
