@@ -325,7 +325,10 @@ int vgpu_scene_set_leaf_size(vgpu_scene* scene, int32_t leaf_max);
 int vgpu_scene_set_timing(vgpu_scene* scene, int32_t enable);
 int vgpu_scene_timing(vgpu_scene* scene, float* trace_ms, float* sort_ms, int32_t* launches);
 
-/* Traversal launch configuration (the autotuner's knobs). */
+/* Traversal launch configuration (the autotuner's knobs).  Until the caller
+ * sets one (or runs the autotuner), vgpu_scene_commit picks block 128, LDS
+ * stack 12, 24 waves per CU, no sort for a static scene, and 256 / 16 / 16 /
+ * no sort for a scene with motion keys. */
 typedef struct vgpu_trace_config {
     int32_t block;        /* threads per workgroup: 64, 128 or 256 */
     int32_t lds_stack;    /* stack entries per ray held in LDS: 8, 12, 16, 24 or 32; the rest spill to HBM */

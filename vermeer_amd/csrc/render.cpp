@@ -407,8 +407,9 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     rv.lights = r->d_lights;
     // size the traversal spill area for the larger (shadow) launch
     vgpu::TraceLaunch L{s->cfg.block, s->cfg.lds_stack, s->cfg.waves_per_cu};
+    const vgpu::TraceLaunch ext = s->any_motion ? vgpu::kExtLaunchMotion : vgpu::kExtLaunch;
     int rc = ensure_spill(s, std::max(vgpu::trace_grid_threads(L, ns, s->num_cus),
-                                      vgpu::trace_grid_threads(vgpu::kExtLaunch, np, s->num_cus)));
+                                      vgpu::trace_grid_threads(ext, np, s->num_cus)));
     if (rc) { delete r; return rc; }
     *out = r;
     return VGPU_OK;

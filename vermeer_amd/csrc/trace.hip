@@ -706,17 +706,19 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
                             const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
     if (n > 0xffffffffull) return hipErrorInvalidValue;
-    constexpr int kBlock = kExtLaunch.block, kLds = kExtLaunch.lds_depth;   // the block size is the launch bound
-    const TraceLaunch cfg = kExtLaunch;
+    // the block size is the kernel's launch bound: each shape is a template constant
+    constexpr int kB = kExtLaunch.block, kL = kExtLaunch.lds_depth;
+    constexpr int kBM = kExtLaunchMotion.block, kLM = kExtLaunchMotion.lds_depth;
+    const TraceLaunch cfg = motion ? kExtLaunchMotion : kExtLaunch;
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
-    const unsigned grid = (unsigned)(threads / kBlock);
+    const unsigned grid = (unsigned)(threads / cfg.block);
     if (motion)
-        hipLaunchKernelGGL((k_trace_wide<kBlock, kLds, false, true, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace_wide<kBM, kLM, false, true, false, true>), dim3(grid), dim3(kBM), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr, refill_min_lanes());
     else
-        hipLaunchKernelGGL((k_trace_wide<kBlock, kLds, false, false, false, true>), dim3(grid), dim3(kBlock), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace_wide<kB, kL, false, false, false, true>), dim3(grid), dim3(kB), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr, refill_min_lanes());
     return hipGetLastError();

@@ -57,9 +57,11 @@ struct TraceLaunch {
     int waves_per_cu;   // resident waves per CU used to size the grid
 };
 
-// Launch shape of launch_trace_ext (the shading passes' closest-hit traversal); callers size the
-// spill area with it.
+// Launch shapes of launch_trace_ext (the shading passes' closest-hit traversal) for static and
+// motion scenes (the motion kernel holds four waves per SIMD); callers size the spill area for the
+// larger grid.
 constexpr TraceLaunch kExtLaunch{128, 12, 24};
+constexpr TraceLaunch kExtLaunchMotion{256, 16, 16};
 
 // Scratch of the ray sort (raysort.hip), sized for `capacity` rays.
 struct RaySortBuffers {
