@@ -309,7 +309,7 @@ __device__ __forceinline__ float dlog_norm(float k) {
 template <bool RAW>
 __global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict__ q, uint64_t n,
                                                const uint32_t* __restrict__ n_dev, float* __restrict__ out) {
-    constexpr int NT = 4;
+    constexpr int NT = 2;   // 16-query tiles per wave and iteration: 2 keeps three waves per SIMD
     __shared__ u32x4 sF[NF_FWD * 64];
     for (int k = threadIdx.x; k < NF_FWD * 64; k += 256) sF[k] = reinterpret_cast<const u32x4*>(net.frag_fwd)[k];
     __syncthreads();
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
     const uint64_t nw = (uint64_t)gridDim.x * 4;
     const float* P = net.params;
-    for (uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * 64; base < nn; base += nw * 64) {
+    for (uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * (16 * NT); base < nn; base += nw * (16 * NT)) {
         // keeps the weight fragments' LDS reads inside the loop: hoisted, they would pin 200 registers
         __asm__ volatile("" ::: "memory");
         float x[NT][6];
