@@ -234,9 +234,14 @@ __global__ __launch_bounds__(256) void k_pt_hit(PTView v, uint32_t pass, int cur
 }
 
 
-// Appends n entries per lane for the whole 256-thread workgroup with one atomic on `counter`
-// (same-address device atomics are slow when every wavefront issues its own); returns this lane's
-// first slot.  Lanes take consecutive slots in thread order.  Every thread of the workgroup calls it.
+// Workgroup size of the kernels that append to the path queues (k_pt_scatter, k_pt_gather).
+constexpr int kAppendBlock = 1024;
+constexpr int kAppendWaves = kAppendBlock / 64;
+
+// Appends n entries per lane for the whole workgroup (kAppendBlock threads) with one atomic on
+// `counter` (same-address device atomics are slow when every wavefront issues its own); returns
+// this lane's first slot.  Lanes take consecutive slots in thread order.  Every thread of the
+// workgroup calls it; s_tot holds kAppendWaves + 1 entries.
 __device__ __forceinline__ uint32_t block_append(uint32_t* counter, uint32_t n, uint32_t* s_tot) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = n;
@@ -249,21 +254,21 @@ __device__ __forceinline__ uint32_t block_append(uint32_t* counter, uint32_t n, 
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0;
-        for (int w = 0; w < 4; w++) {
+        for (int w = 0; w < kAppendWaves; w++) {
             const uint32_t t = s_tot[w];
             s_tot[w] = run;
             run += t;
         }
-        s_tot[4] = run ? atomicAdd(counter, run) : 0u;
+        s_tot[kAppendWaves] = run ? atomicAdd(counter, run) : 0u;
     }
     __syncthreads();
-    const uint32_t slot = s_tot[4] + s_tot[wave] + (x - n);
+    const uint32_t slot = s_tot[kAppendWaves] + s_tot[wave] + (x - n);
     __syncthreads();
     return slot;
 }
 
-__global__ __launch_bounds__(256) void k_pt_scatter(PTView v, uint32_t pass, int cur, PTBuffers b) {
-    __shared__ uint32_t s_tot[5];
+__global__ __launch_bounds__(kAppendBlock) void k_pt_scatter(PTView v, uint32_t pass, int cur, PTBuffers b) {
+    __shared__ uint32_t s_tot[kAppendWaves + 1];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = b.count[cur];
     bool alive = false, shadow = false;
@@ -430,8 +435,8 @@ __device__ __forceinline__ bool gather_step(const float4* __restrict__ r, bool n
 
 // Walks each recorded path backwards twice: once to count its training samples, then, after one
 // append for the whole workgroup, to write them to consecutive slots.
-__global__ __launch_bounds__(256) void k_pt_gather(PTView v, uint32_t pass, PTBuffers b) {
-    __shared__ uint32_t s_tot[5];
+__global__ __launch_bounds__(kAppendBlock) void k_pt_gather(PTView v, uint32_t pass, PTBuffers b) {
+    __shared__ uint32_t s_tot[kAppendWaves + 1];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = p < v.width * v.height ? b.plen[p] : 0;
     const float4* R = b.rec + (size_t)(p < v.width * v.height ? p : 0) * v.max_depth * PT_REC_F4;
@@ -471,7 +476,9 @@ hipError_t pt_launch_hit(const PTView& v, uint32_t pass, int cur, const PTBuffer
     return hipGetLastError();
 }
 hipError_t pt_launch_scatter(const PTView& v, uint32_t pass, int cur, const PTBuffers& b, hipStream_t st) {
-    hipLaunchKernelGGL(k_pt_scatter, dim3(grid_for((uint64_t)v.width * v.height)), dim3(256), 0, st, v, pass, cur, b);
+    const uint64_t np = (uint64_t)v.width * v.height;
+    hipLaunchKernelGGL(k_pt_scatter, dim3((unsigned)((np + kAppendBlock - 1) / kAppendBlock)), dim3(kAppendBlock), 0, st, v,
+                       pass, cur, b);
     return hipGetLastError();
 }
 hipError_t pt_launch_shadow(const PTView& v, const PTBuffers& b, hipStream_t st) {
@@ -483,7 +490,9 @@ hipError_t pt_launch_accumulate(const PTView& v, const PTBuffers& b, double* acc
     return hipGetLastError();
 }
 hipError_t pt_launch_gather(const PTView& v, uint32_t pass, const PTBuffers& b, hipStream_t st) {
-    hipLaunchKernelGGL(k_pt_gather, dim3(grid_for((uint64_t)v.width * v.height)), dim3(256), 0, st, v, pass, b);
+    const uint64_t np = (uint64_t)v.width * v.height;
+    hipLaunchKernelGGL(k_pt_gather, dim3((unsigned)((np + kAppendBlock - 1) / kAppendBlock)), dim3(kAppendBlock), 0, st, v,
+                       pass, b);
     return hipGetLastError();
 }
 
