@@ -92,7 +92,8 @@ hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const Tra
                         const uint32_t* perm = nullptr);
 
 // Closest hit with 32-byte records {prim, geom, t, U, V, W, 0, 0} (U, V, W: barycentric weights of the
-// triangle's three vertices), for the shading kernels.  Fixed configuration (block 256, LDS stack 16).
+// triangle's three vertices), for the shading kernels.  Launch shape kExtLaunch, or kExtLaunchMotion
+// with motion.
 hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, uint64_t n, void* hits32, float* spill_T,
                             int32_t* spill_N, uint64_t spill_threads, int32_t* err_flag, int num_cus, hipStream_t stream,
                             const uint32_t* n_dev = nullptr);
