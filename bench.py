@@ -60,10 +60,14 @@ def parse():
     ap.add_argument("--pt-plain-spp", type=int, default=64)
     ap.add_argument("--pt-alpha", type=float, default=0.5)
     ap.add_argument("--pt-stride", type=int, default=1, help="record the paths of one pixel in this many")
-    ap.add_argument("--pt-steps", type=int, default=32, help="Adam steps per pass")
+    ap.add_argument("--pt-steps", type=int, default=48, help="Adam steps per pass (until --pt-late-pass)")
     ap.add_argument("--pt-batch", type=int, default=65536)
     ap.add_argument("--pt-lr", type=float, default=1e-3)
     ap.add_argument("--pt-leaf", type=int, default=16, help="BVH leaf size of the room (16: the reference's)")
+    ap.add_argument("--pt-late-pass", type=int, default=10,
+                    help="guided pass from which --pt-steps-late Adam steps per pass run (-1: never)")
+    ap.add_argument("--pt-steps-late", type=int, default=0,
+                    help="Adam steps per pass from --pt-late-pass on (0: the guide is frozen, no paths recorded)")
     ap.add_argument("--pt-warm", type=int, default=0,
                     help="guided passes left out of the image (their time still counts)")
     return ap.parse_args()
@@ -299,6 +303,9 @@ def guided_measurement(args, torch, np):
         if p == args.pt_warm and p > 0:
             acc_g.zero_()
             n_img = 0
+        if p == args.pt_late_pass:
+            gp.set_guide(gd, alpha=args.pt_alpha, record_stride=args.pt_stride, train_steps=args.pt_steps_late,
+                         batch=args.pt_batch, lr=args.pt_lr)
         t_g += passes(gp, acc_g, p, 1)
         p += 1
         n_img += 1
@@ -328,7 +335,8 @@ def guided_measurement(args, torch, np):
         "guided_rmse_at_half_plain_time": e_half,
         "guided_mrays_per_s": round(guided_mrays, 1),
         "guide": {"alpha": args.pt_alpha, "record_stride": args.pt_stride, "adam_steps_per_pass": args.pt_steps,
-                  "batch": args.pt_batch, "lr": args.pt_lr, "warm_passes": args.pt_warm},
+                  "batch": args.pt_batch, "lr": args.pt_lr, "warm_passes": args.pt_warm,
+                  "late_pass": args.pt_late_pass, "adam_steps_per_pass_late": args.pt_steps_late},
         "rmse_trace_ms": trace[:: max(1, len(trace) // 16)],
     }
 
