@@ -68,6 +68,8 @@ def parse():
                     help="guided pass from which --pt-steps-late Adam steps per pass run (-1: never)")
     ap.add_argument("--pt-steps-late", type=int, default=0,
                     help="Adam steps per pass from --pt-late-pass on (0: the guide is frozen, no paths recorded)")
+    ap.add_argument("--pt-alpha-late", type=float, default=0.65,
+                    help="guide sampling probability from --pt-late-pass on")
     ap.add_argument("--pt-warm", type=int, default=0,
                     help="guided passes left out of the image (their time still counts)")
     return ap.parse_args()
@@ -304,7 +306,8 @@ def guided_measurement(args, torch, np):
             acc_g.zero_()
             n_img = 0
         if p == args.pt_late_pass:
-            gp.set_guide(gd, alpha=args.pt_alpha, record_stride=args.pt_stride, train_steps=args.pt_steps_late,
+            a_late = args.pt_alpha_late
+            gp.set_guide(gd, alpha=a_late, record_stride=args.pt_stride, train_steps=args.pt_steps_late,
                          batch=args.pt_batch, lr=args.pt_lr)
         t_g += passes(gp, acc_g, p, 1)
         p += 1
@@ -336,7 +339,8 @@ def guided_measurement(args, torch, np):
         "guided_mrays_per_s": round(guided_mrays, 1),
         "guide": {"alpha": args.pt_alpha, "record_stride": args.pt_stride, "adam_steps_per_pass": args.pt_steps,
                   "batch": args.pt_batch, "lr": args.pt_lr, "warm_passes": args.pt_warm,
-                  "late_pass": args.pt_late_pass, "adam_steps_per_pass_late": args.pt_steps_late},
+                  "late_pass": args.pt_late_pass, "adam_steps_per_pass_late": args.pt_steps_late,
+                  "alpha_late": args.pt_alpha_late},
         "rmse_trace_ms": trace[:: max(1, len(trace) // 16)],
     }
 
