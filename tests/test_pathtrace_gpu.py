@@ -68,6 +68,39 @@ def test_furnace_closed_form(guided):
     assert abs(m - want) < 4 * se + 1e-3
 
 
+def test_frozen_guide_keeps_its_weights_and_stays_unbiased():
+    """Train online, then attach the same guide again with no training steps and a larger sampling
+    probability (0.65). Re-attaching keeps the trained weights, a frozen pass records no samples and
+    leaves the weights unchanged, and the furnace estimate still meets the closed form."""
+    rho, depth, W = 0.5, 4, 32
+    print("\n[case] furnace, guide trained 6 passes then frozen at alpha 0.65", flush=True)
+    d = scenes.furnace_box_scene(albedo=rho, emission=1.0)
+    pt = pathtrace.PathTracer(d, W, W, max_depth=depth, seed=11)
+    gd = guide.Guide(np.array([-1, -1, -1, 1, 1, 1], np.float32), seed=3, max_batch=1024)
+    p0 = gd.params
+    pt.set_guide(gd, alpha=0.5, record_stride=1, train_steps=2, batch=1024, lr=3e-3)
+    per = []
+    for p in range(64):
+        if p == 6:
+            trained = gd.params
+            assert not np.array_equal(trained, p0)
+            pt.set_guide(gd, alpha=0.65, record_stride=1, train_steps=0, batch=1024, lr=3e-3)
+            assert np.array_equal(gd.params, trained)
+        before = pt.accum.copy()
+        pt.render_pass(p)
+        per.append((pt.accum - before).mean())
+        if p >= 6:
+            assert pt.stats()[1] == 0
+    assert np.array_equal(gd.params, trained)
+    pt.close()
+    gd.close()
+    per = np.array(per)
+    want = (1 - rho ** depth) / (1 - rho)
+    m, se = per.mean(), per.std() / math.sqrt(len(per))
+    print(f"[case] frozen guide furnace: {m:.5f} +- {se:.5f} (want {want:.5f})", flush=True)
+    assert abs(m - want) < 4 * se + 1e-3
+
+
 @pytest.fixture(scope="module")
 def room():
     return scenes.guided_room_scene(res=10, nboxes=20000)
