@@ -421,7 +421,8 @@ int vgpu_pt_destroy(vgpu_pt* pt);
  * records the paths of one pixel in record_stride, turns their vertices
  * into training samples (c = luminance of the incident radiance estimate x
  * cos / sampling pdf) and runs train_steps Adam steps on batches of `batch`
- * samples drawn from them. */
+ * samples drawn from them.  Attaching the same guide again changes only these
+ * settings: its weights and Adam state stay, so train_steps = 0 freezes it. */
 int vgpu_pt_set_guide(vgpu_pt* pt, vgpu_guide* guide, float alpha, uint32_t record_stride, uint32_t train_steps,
                       uint32_t batch, float lr);
 /* One pass (pass index selects the random numbers).  accum: width*height*3
