@@ -350,7 +350,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
-              "multi-GPU traversal shards rays across GPUs with a replicated BVH and is not part of this build",
+              "sharding rays across GPUs over a replicated BVH is not implemented",
               file=sys.stderr)
         sys.exit(2)
 
