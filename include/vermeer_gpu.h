@@ -13,8 +13,11 @@
  *   - The caller owns every host buffer.  The library copies what it needs
  *     and keeps no caller pointer after a call returns (cgo pointer rules).
  *   - Device memory is owned by the scene object.
- *   - A scene is not internally synchronised: use one scene per host thread,
- *     or serialise calls on it.
+ *   - Calls on one scene (and on the renderers and path tracers made over it)
+ *     are serialised by a per-scene lock: host-memory calls may come from many
+ *     host threads at once.  Work enqueued with VGPU_TRACE_DEVICE_PTRS shares
+ *     the scene's traversal spill area and error flag while it runs, so all
+ *     device-pointer work on one scene must be enqueued on one stream.
  */
 #ifndef VERMEER_GPU_H
 #define VERMEER_GPU_H

@@ -97,3 +97,39 @@ def test_motion_builder_topology_matches_oracle_motion_mesh():
     mn = va.build_accel_motion(boxes, cent, ids, 16)
     assert np.array_equal(ids, perm)
     assert np.array_equal(mn["children"], topo[:, 3:7]) and np.array_equal(mn["axis"], topo[:, 0:3])
+
+
+_NAN_BUILD = """
+import sys
+import numpy as np
+sys.path.insert(0, %r)
+import vermeer_amd as va
+from vermeer_amd import scenes
+g = scenes.rng(7)
+n = 400_000
+c = g.uniform(-10, 10, (n, 3)).astype(np.float32)
+c[:, 0] *= 4.0
+c[123_457, 0] = 38.0
+boxes = np.ascontiguousarray(np.concatenate([c - 0.01, c + 0.01], 1), np.float32)
+cent = np.ascontiguousarray(c, np.float32)
+cent[123_457, 1] = np.nan
+try:
+    va.build_accel(boxes, cent, np.arange(n, dtype=np.int32), 16, %d)
+    print("built")
+except va.VgpuError as e:
+    print("code", e.code)
+"""
+
+
+@pytest.mark.parametrize("nthreads", [1, 4])
+def test_builder_reports_nan_centroid_with_threaded_subtrees(nthreads):
+    # 400k elements split on x first; the NaN y-centroid sits in the highest-x quarter, the subtree
+    # the calling thread builds while its three siblings run on their own threads.  The failure must
+    # come back as VGPU_ERR_BUILD after every thread is joined (a subprocess, because unwinding past
+    # a joinable std::thread aborts the process).
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", _NAN_BUILD % (ROOT, nthreads)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip().splitlines()[-1] == "code %d" % _lib.VGPU_ERR_BUILD

@@ -397,3 +397,43 @@ def test_motion_times_outside_the_shutter_are_held_to_the_end_keys(mixed_motion)
         assert np.array_equal(ha.view(np.uint32).reshape(len(ha), -1), hb.view(np.uint32).reshape(len(hb), -1))
         assert np.array_equal(ga, gb)
     assert (out["zero"][0]["prim"] >= 0).mean() > 0.01
+
+
+def test_concurrent_host_traces_on_one_scene(sphere_ground):
+    """Host-memory traces from several threads at once share the scene's staging buffer and spill
+    area under its lock: every thread's result equals the oracle's for its own rays (different sizes,
+    so the staging buffer is also reallocated while other threads wait)."""
+    import threading
+    g, o = sphere_ground
+    sets = [scenes.uniform_rays(n, seed=40 + i) for i, n in enumerate((300_000, 50_000, 700_000, 120_000))]
+    refs = [o.trace(r)[0] for r in sets]
+    results = [None] * len(sets)
+    errors = []
+
+    def work(i):
+        try:
+            for _ in range(3):
+                results[i] = g.trace(sets[i])
+        except Exception as e:  # noqa: BLE001 (reported below)
+            errors.append(e)
+
+    print("[case] 4 host threads tracing one scene", flush=True)
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(sets))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for i, (hits, geoms, _) in enumerate(results):
+        _same(hits, geoms, refs[i], f"thread {i}")
+
+
+def test_mesh_with_non_finite_vertex_is_rejected():
+    v, f = scenes.uv_sphere(1.0, 16, 8)
+    v = v.copy()
+    v[5, 1] = np.nan
+    g = va.Scene(0)
+    with pytest.raises(va.VgpuError) as ei:
+        g.add_geom(va.PolyMesh(v, f))
+    assert ei.value.code == va._lib.VGPU_ERR_ARG
+    g.close()

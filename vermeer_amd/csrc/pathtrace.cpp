@@ -135,6 +135,7 @@ int vgpu_pt_create(vgpu_pt** out, vgpu_scene* s, const vgpu_pt_material* materia
     v.seed_hi = (uint32_t)(seed >> 32);
     v.alpha = 0.0f;
     v.rec_stride = 0;
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     const size_t np = (size_t)width * height;
     vgpu::PTBuffers& b = pt->b;
@@ -225,6 +226,7 @@ int vgpu_pt_set_guide(vgpu_pt* pt, vgpu_guide* g, float alpha, uint32_t record_s
 int vgpu_pt_pass(vgpu_pt* pt, uint32_t flags, uint32_t pass, double* accum, void* stream) {
     if (!pt || !accum) return fail(VGPU_ERR_ARG, "NULL argument");
     vgpu_scene* s = pt->scene;
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     hipStream_t st = (hipStream_t)stream;
     const size_t np = (size_t)pt->v.width * pt->v.height;

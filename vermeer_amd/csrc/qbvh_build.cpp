@@ -182,7 +182,10 @@ struct Builder {
                     });
                 } else {
                     if (cn >= kParallelMin) spare_threads.fetch_add(1);
-                    build(sub[c], e + lo[c], cn, base + lo[c], cbox[c], depth + 1);
+                    // A failure here must not unwind past the joinable sibling threads (and the
+                    // sub/cbox/failed arrays they write): record it and rethrow after every join.
+                    try { build(sub[c], e + lo[c], cn, base + lo[c], cbox[c], depth + 1); }
+                    catch (...) { failed[c] = true; }
                 }
             }
             for (int c = 0; c < 4; c++) if (spawned[c]) th[c].join();

@@ -380,6 +380,7 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     rv.filt_w = 0.0;
     rv.max_level = any_mirror ? vgpu::kMaxLevel : 0;
     if (!raster_tables(12, rv.raster_fwd, 52, rv.raster_inv)) { delete r; return fail(VGPU_ERR_ARG, "RasterXY tables"); }
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     const size_t np = (size_t)width * height, ns = np * (size_t)rv.slots;
     auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)); };
@@ -425,6 +426,7 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
     if (!r || !scrambles || !film) return fail(VGPU_ERR_ARG, "NULL argument");
     if (iter == 0) return fail(VGPU_ERR_ARG, "iter is 1-based");
     vgpu_scene* s = r->scene;
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     hipStream_t st = (hipStream_t)stream;
     const size_t np = (size_t)r->width * r->height, ns = np * (size_t)r->rv.slots;

@@ -313,6 +313,11 @@ int vgpu_scene_add_mesh(vgpu_scene* s, const float* verts, uint32_t nverts, uint
     if (ntris >= (1u << 23)) return fail(VGPU_ERR_ARG, "mesh has %u triangles; the leaf encoding holds < 2^23 (qbvh/qbvh.go:64)", ntris);
     for (uint64_t i = 0; i < (uint64_t)ntris * 3; i++)
         if (idx[i] >= nverts) return fail(VGPU_ERR_ARG, "index %u out of range at %llu", idx[i], (unsigned long long)i);
+    // A NaN or infinite coordinate gives a centroid no SAH bin holds: the reference's BuildAccel
+    // indexes out of range or recurses without end on it.
+    for (uint64_t i = 0; i < (uint64_t)nverts * nkeys * 3; i++)
+        if (!std::isfinite(verts[i]))
+            return fail(VGPU_ERR_ARG, "vertex coordinate %llu is not finite", (unsigned long long)i);
     MeshHost m;
     m.nverts = nverts; m.nkeys = nkeys; m.ntris = ntris; m.ray_bias = ray_bias; m.flags = flags;
     m.verts.assign(verts, verts + (size_t)nverts * nkeys * 3);
@@ -658,6 +663,7 @@ int vgpu_trace(vgpu_scene* s, uint32_t flags, const vgpu_ray* rays, uint64_t n, 
     if (!s->committed) return fail(VGPU_ERR_STATE, "trace before commit");
     if (n == 0) return VGPU_OK;
     if (!rays || !hits) return fail(VGPU_ERR_ARG, "rays/hits are NULL");
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     hipStream_t st = (hipStream_t)stream;
     if (flags & VGPU_TRACE_DEVICE_PTRS)
@@ -730,6 +736,7 @@ int vgpu_scene_timing(vgpu_scene* s, float* trace_ms, float* sort_ms, int32_t* l
 
 int vgpu_scene_status(vgpu_scene* s, void* stream) {
     if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     hipStream_t st = (hipStream_t)stream;
     int32_t err = 0;
@@ -745,6 +752,7 @@ int vgpu_scene_set_trace_config(vgpu_scene* s, const vgpu_trace_config* cfg) {
     if (!s || !cfg) return fail(VGPU_ERR_ARG, "NULL argument");
     int rc = check_config(*cfg);
     if (rc) return rc;
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     s->cfg = *cfg;
     s->cfg_set = true;
     return VGPU_OK;
@@ -760,6 +768,7 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
                         vgpu_trace_config* best_out, float* ms_out) {
     if (!s || !d_rays || !d_hits) return fail(VGPU_ERR_ARG, "NULL argument");
     if (!s->committed) return fail(VGPU_ERR_STATE, "autotune before commit");
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     hipStream_t st = (hipStream_t)stream;
     const int blocks[] = {64, 128, 256};

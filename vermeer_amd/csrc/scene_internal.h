@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -127,6 +128,9 @@ struct vgpu_scene {
     vgpu::RaySortBuffers sort{};
     // kernel timing (vgpu_scene_set_timing): events [sort start, trace start, trace end] per launch
     bool timing = false;
+    // Serialises the calls that share this scene's device buffers (host staging, spill area,
+    // ray-sort scratch, error flag) and may reallocate them.  Recursive: autotune ends in status.
+    std::recursive_mutex mu;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
 
