@@ -228,6 +228,26 @@ int vgpu_scene_mesh_accel(const vgpu_scene* scene, int32_t slot, vgpu_qbvh_node*
 int vgpu_trace(vgpu_scene* scene, uint32_t flags, const vgpu_ray* rays, uint64_t n,
                vgpu_hit* hits, int32_t* geoms, int32_t* counters, void* stream);
 
+/* Allocation-free variant of vgpu_trace.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (work under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * array is device memory; the work is enqueued on `stream` and the call
+ * returns without synchronising (flags: VGPU_TRACE_SHADOW, and
+ * VGPU_TRACE_DEVICE_PTRS, which is implied).  Results equal vgpu_trace's.
+ * The traversal stack spill lives in a 128 MiB array that the library holds
+ * on every GPU it runs on until the process ends; the launch takes the
+ * scene's shape with fewer resident waves where the spill would not fit, and
+ * rays are never sorted.  Any n below 2^32 fits (the persistent grid reuses
+ * its spill columns); larger n is rejected with VGPU_ERR_ARG.  d_status
+ * (nullable, one device int32) is cleared by a kernel at the start of the
+ * call and set non-zero if a ray needed more than VGPU_MAX_STACK entries (its
+ * hit is written as a miss).  The array is shared by every call on one GPU
+ * and keeps the last call's contents: calls that overlap on one GPU, from any
+ * scene or stream, race.  It calls no hipMalloc, hipFree or other allocating
+ * routine. */
+int vgpu_trace_noalloc(vgpu_scene* scene, uint32_t flags, const vgpu_ray* d_rays, uint64_t n, vgpu_hit* d_hits,
+                       int32_t* d_geoms, int32_t* d_counters, int32_t* d_status, void* stream);
+
 /* Synchronises `stream` and reports (then clears) an error latched by
  * traversal work enqueued with VGPU_TRACE_DEVICE_PTRS: VGPU_ERR_STACK if a
  * ray overflowed VGPU_MAX_STACK (its hit was written as a miss).  No

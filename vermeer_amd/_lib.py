@@ -69,6 +69,7 @@ SIGNATURES = [
     ("vgpu_scene_mesh_accel", C.c_int, [_P, _I32, _P, _P, _P]),
     ("vgpu_trace", C.c_int, [_P, _U32, _P, _U64, _P, _P, _P, _P]),
     ("vgpu_scene_status", C.c_int, [_P, _P]),
+    ("vgpu_trace_noalloc", C.c_int, [_P, _U32, _P, _U64, _P, _P, _P, _P, _P]),
     ("vgpu_scene_set_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
     ("vgpu_scene_get_trace_config", C.c_int, [_P, C.POINTER(TraceConfig)]),
     ("vgpu_scene_autotune", C.c_int, [_P, _U32, _P, _U64, _P, _P, C.POINTER(TraceConfig), C.POINTER(_F)]),

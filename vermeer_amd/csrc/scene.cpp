@@ -296,6 +296,12 @@ int vgpu_scene_create(vgpu_scene** out, int device) {
         delete s;
         return fail(VGPU_ERR_HIP, "cannot allocate the error flag");
     }
+    // Once per device: the address of vgpu_trace_noalloc's scratch array on this device.
+    const hipError_t ne = vgpu::noalloc_init(device);
+    if (ne != hipSuccess) {
+        delete s;
+        return fail(VGPU_ERR_HIP, "allocation-free traversal scratch on device %d: %s", device, hipGetErrorString(ne));
+    }
     *out = s;
     return VGPU_OK;
 }
@@ -693,6 +699,30 @@ int vgpu_trace(vgpu_scene* s, uint32_t flags, const vgpu_ray* rays, uint64_t n, 
     HIP_TRY(hipMemcpyAsync(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
+    return VGPU_OK;
+}
+
+int vgpu_trace_noalloc(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, uint64_t n, vgpu_hit* d_hits,
+                       int32_t* d_geoms, int32_t* d_counters, int32_t* d_status, void* stream) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (!s->committed) return fail(VGPU_ERR_STATE, "trace before commit");
+    if ((flags & ~(VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS)) != 0) return fail(VGPU_ERR_ARG, "unknown trace flags 0x%x", flags);
+    if (n > 0xffffffffull) return fail(VGPU_ERR_ARG, "%llu rays: one call traces at most 2^32-1", (unsigned long long)n);
+    if (n > 0 && (!d_rays || !d_hits)) return fail(VGPU_ERR_ARG, "rays/hits are NULL");
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
+    DeviceGuard dg(s->device);
+    // The scene's launch shape with fewer resident waves where its spill columns would not fit the
+    // scratch array; never sorted (the sort's buffers grow with n).
+    vgpu::TraceLaunch L{s->cfg.block, s->cfg.lds_stack, s->cfg.waves_per_cu};
+    const uint64_t cap = vgpu::noalloc_max_threads(L.lds_depth);
+    while (L.waves_per_cu > 1 && vgpu::trace_grid_threads(L, n, s->num_cus) > cap) L.waves_per_cu--;
+    if (vgpu::trace_grid_threads(L, n, s->num_cus) > cap)
+        return fail(VGPU_ERR_ARG, "the %llu-byte scratch holds the stack spill of fewer than one wavefront per CU on %d CUs",
+                    (unsigned long long)vgpu::kNoAllocScratchBytes, s->num_cus);
+    const hipError_t e = vgpu::launch_trace_noalloc(s->device, s->view, (flags & VGPU_TRACE_SHADOW) != 0, s->any_motion, L,
+                                                    d_rays, n, d_hits, d_geoms, d_counters, d_status, s->num_cus,
+                                                    (hipStream_t)stream);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "allocation-free trace launch: %s", hipGetErrorString(e));
     return VGPU_OK;
 }
 

@@ -22,6 +22,8 @@
 // Compiled with -ffp-contract=off: every float operation rounds as the
 // reference's unfused SSE code does.
 
+#include <mutex>
+
 #include "trace_kernels.h"
 #include "../../include/vermeer_gpu.h"
 
@@ -750,6 +752,69 @@ hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const Tra
     VGPU_CASE(256, 8) VGPU_CASE(256, 12) VGPU_CASE(256, 16) VGPU_CASE(256, 24) VGPU_CASE(256, 32)
 #undef VGPU_CASE
     return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------- allocation-free traversal
+//
+// vgpu_trace_noalloc keeps its traversal spill area in this file-scope array instead of device
+// memory the scene allocates.  Layout: the error flag in the first 256 bytes, then the spill
+// depths T and nodes N, each (VGPU_MAX_STACK - lds) entries per launched thread, depth-major as
+// the Stack struct addresses them.  The array's host-side name is not a device address: host code
+// uses only the pointer hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+__global__ void k_clear_flag(int32_t* flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
+}
+
+}  // namespace
+
+hipError_t noalloc_init(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;   // the caller makes `device` current
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+uint64_t noalloc_max_threads(int lds_depth) {
+    const uint64_t per = (uint64_t)(VGPU_MAX_STACK - lds_depth) * (sizeof(float) + sizeof(int32_t));
+    return (kNoAllocScratchBytes - 256) / per;
+}
+
+hipError_t launch_trace_noalloc(int device, const SceneView& sv, bool shadow, bool motion, const TraceLaunch& cfg,
+                                const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
+                                int32_t* status, int num_cus, hipStream_t stream) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = (unsigned char*)g_scratch_ptr[device];
+    }
+    if (!base) return hipErrorNotInitialized;
+    const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
+    if (threads > noalloc_max_threads(cfg.lds_depth)) return hipErrorInvalidValue;
+    int32_t* flag = status ? status : (int32_t*)base;
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, stream, flag);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n == 0) return e;
+    const uint64_t per = (uint64_t)(VGPU_MAX_STACK - cfg.lds_depth);
+    float* spT = (float*)(base + 256);
+    int32_t* spN = (int32_t*)(base + 256 + per * threads * sizeof(float));
+    return launch_trace(sv, shadow, motion, cfg, rays, n, hits, geoms, counters, spT, spN, threads, flag, num_cus,
+                        stream, nullptr, nullptr);
 }
 
 }  // namespace vgpu

@@ -101,4 +101,19 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
 // Threads the launch above will use (so the caller can size the spill area).
 uint64_t trace_grid_threads(const TraceLaunch& cfg, uint64_t n, int num_cus);
 
+// Allocation-free traversal (vgpu_trace_noalloc): the spill area and error flag live in a
+// file-scope device array of kNoAllocScratchBytes in trace.hip, one copy per GPU.
+constexpr uint64_t kNoAllocScratchBytes = 128ull << 20;
+constexpr int kNoAllocMaxDevices = 64;
+// Looks the array up on `device`, which must be current; once per device (later calls return at once).
+hipError_t noalloc_init(int device);
+// Threads whose spill columns fit the array at LDS depth lds_depth.
+uint64_t noalloc_max_threads(int lds_depth);
+// Clears the error flag (status, or the array's own flag word when status is NULL) with a kernel,
+// then enqueues launch_trace with the array as its spill area.  The grid of cfg must fit
+// noalloc_max_threads.  hipErrorNotInitialized: noalloc_init has not run on `device`.
+hipError_t launch_trace_noalloc(int device, const SceneView& sv, bool shadow, bool motion, const TraceLaunch& cfg,
+                                const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
+                                int32_t* status, int num_cus, hipStream_t stream);
+
 }  // namespace vgpu
