@@ -422,6 +422,26 @@ class OracleRenderer:
         self._keep.append(m)
         self.lib.orc_render_set_camera_keys(self.h_r, _p(m), len(m))
 
+    def light_samples(self, li, P, Ng, lam, s0, s1, I0, count, n=1):
+        """Light.SampleArea of light li at (P, Ng) for sample indices I0..I0+count-1, n each:
+        (m, 5) float32 rows of (Ld xyz, Ldist, Pdf), accepted samples only."""
+        _pin_sigs(self.lib)
+        out = np.zeros((count * n, 5), np.float32)
+        Pv, Nv = _v3(P), _v3(Ng)      # held until the call returns
+        m = self.lib.orc_light_sample_range(self.h_r, li, _p(Pv), _p(Nv), float(lam), int(s0), int(s1),
+                                            int(I0), int(count), int(n), _p(out))
+        return out[:m]
+
+    def eval_light_samples(self, li, P, N, dPdu, Rd, lam, roughness, scrambles, I0, count, nsamples):
+        """EvaluateLightSamples of light li with ShaderStd's diffuse lobe: (count, 3) RGB per index."""
+        _pin_sigs(self.lib)
+        scr = np.ascontiguousarray(scrambles, np.uint64).reshape(2)
+        out = np.zeros((count, 3), np.float32)
+        vs = [_v3(P), _v3(N), _v3(dPdu), _v3(Rd)]      # held until the call returns
+        self.lib.orc_eval_light_samples(self.h_r, li, _p(vs[0]), _p(vs[1]), _p(vs[2]), _p(vs[3]), float(lam),
+                                        float(roughness), _p(scr), int(I0), int(count), int(nsamples), _p(out))
+        return out
+
     def render_pass(self, scrambles, iteration, nthreads=0, want_sample=False):
         scr = np.ascontiguousarray(scrambles, np.uint64)
         samp = np.zeros((self.h, self.w, 3), np.float32) if want_sample else None
@@ -458,6 +478,91 @@ def oren_nayar_eval(lam, wi, wo, roughness):
     b = np.ascontiguousarray(wo, np.float32)
     out = np.zeros(4, np.float32)
     lib.orc_oren_nayar_eval(lam, _p(a), _p(b), roughness, _p(out))
+    return out
+
+
+def _pin_sigs(lib):
+    P = C.c_void_p
+    sig = {
+        "orc_sample_spherical_triangle_n": (None, [P, P, P, P, C.c_int64, P, P, P, P]),
+        "orc_light_sample_range": (C.c_int64, [P, C.c_int32, P, P, C.c_float, C.c_uint64, C.c_uint64, C.c_uint64,
+                                               C.c_int64, C.c_int32, P]),
+        "orc_eval_light_samples": (None, [P, C.c_int32, P, P, P, P, C.c_float, C.c_float, P, C.c_uint64, C.c_int64,
+                                          C.c_int32, P]),
+        "orc_spectrum_product_rgb": (None, [P, P, C.c_float, P]),
+        "orc_ggx_sample_n": (None, [C.c_float, P, C.c_int64, P, P, P]),
+        "orc_ggx_pdf_eval_n": (None, [C.c_float, C.c_float, C.c_float, P, C.c_int64, P, P, P]),
+        "orc_ggx_d_n": (None, [P, C.c_int64, C.c_float, P]),
+        "orc_ggx_g1_n": (None, [P, P, C.c_int64, C.c_float, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def _v3(a):
+    return np.ascontiguousarray(a, np.float32).reshape(3)
+
+
+def sample_spherical_triangle(p0, p1, p2, p, r0, r1):
+    """sampleSphericalTriangle (builtin/light/triangle.go:474-535) for arrays r0, r1: (dirs (n,3), pdf (n,))."""
+    lib = _pin_sigs(load())
+    r0 = np.ascontiguousarray(r0, np.float64)
+    r1 = np.ascontiguousarray(r1, np.float64)
+    n = len(r0)
+    dirs = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float64)
+    a, b, c, o = _v3(p0), _v3(p1), _v3(p2), _v3(p)
+    lib.orc_sample_spherical_triangle_n(_p(a), _p(b), _p(c), _p(o), n, _p(r0), _p(r1), _p(dirs), _p(pdf))
+    return dirs, pdf
+
+
+def spectrum_product_rgb(a, b, lam):
+    """ToRGB(FromRGB(a) * FromRGB(b)) at hero wavelength lam."""
+    lib = _pin_sigs(load())
+    out = np.zeros(3, np.float32)
+    x, y = _v3(a), _v3(b)
+    lib.orc_spectrum_product_rgb(_p(x), _p(y), float(lam), _p(out))
+    return out
+
+
+def ggx_sample(roughness, omega_r, r0, r1):
+    lib = _pin_sigs(load())
+    r0 = np.ascontiguousarray(r0, np.float64)
+    r1 = np.ascontiguousarray(r1, np.float64)
+    out = np.zeros((len(r0), 3), np.float32)
+    w = _v3(omega_r)
+    lib.orc_ggx_sample_n(float(roughness), _p(w), len(r0), _p(r0), _p(r1), _p(out))
+    return out
+
+
+def ggx_pdf_eval(roughness, omega_r, dirs, eta=1.5, lam=550.0):
+    """MicrofacetGGX PDF (float64) and Eval (4 spectral values) of dirs in the tangent frame."""
+    lib = _pin_sigs(load())
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    pdf = np.zeros(len(d), np.float64)
+    ev = np.zeros((len(d), 4), np.float32)
+    w = _v3(omega_r)
+    lib.orc_ggx_pdf_eval_n(float(roughness), float(eta), float(lam), _p(w), len(d), _p(d), _p(pdf), _p(ev))
+    return pdf, ev
+
+
+def ggx_d_n(m, alpha):
+    lib = _pin_sigs(load())
+    mm = np.ascontiguousarray(m, np.float32).reshape(-1, 3)
+    out = np.zeros(len(mm), np.float32)
+    lib.orc_ggx_d_n(_p(mm), len(mm), float(alpha), _p(out))
+    return out
+
+
+def ggx_g1_n(w, m, alpha):
+    lib = _pin_sigs(load())
+    mm = np.ascontiguousarray(m, np.float32).reshape(-1, 3)
+    out = np.zeros(len(mm), np.float32)
+    ww = _v3(w)
+    lib.orc_ggx_g1_n(_p(ww), _p(mm), len(mm), float(alpha), _p(out))
     return out
 
 

@@ -1511,3 +1511,126 @@ void orc_oren_nayar_eval(float lambda, const float* wi, const float* wo, float r
     for (int k = 0; k < 4; k++) out4[k] = s.C[k];
 }
 }
+
+extern "C" {
+// Hooks for the tests that pin the light samplers, EvaluateLightSamples and the GGX lobe against
+// closed forms and invariants.  They call the restated functions above unchanged.
+
+// sampleSphericalTriangle (triangle.go:474-535) for n (r0, r1) pairs: unit directions and pdfs.
+void orc_sample_spherical_triangle_n(const float* p0, const float* p1, const float* p2, const float* p, int64_t n,
+                                     const double* r0, const double* r1, float* dirs, double* pdf) {
+    const Vec3 a = V3(p0[0], p0[1], p0[2]), b = V3(p1[0], p1[1], p1[2]), c = V3(p2[0], p2[1], p2[2]);
+    const Vec3 o = V3(p[0], p[1], p[2]);
+    for (int64_t i = 0; i < n; i++) {
+        Vec3 x = sampleSphericalTriangle(a, b, c, o, r0[i], r1[i], pdf[i]);
+        for (int k = 0; k < 3; k++) dirs[i * 3 + k] = x[k];
+    }
+}
+
+// Light.SampleArea of light `li` of a renderer at a shading point (P, Ng, with N = Ng) for the
+// sample indices I = I0 .. I0+count-1, n samples each (ldseq VdC/Sobol with scrambles s0, s1).
+// out: 5 floats per accepted sample (Ld xyz, Ldist, Pdf); returns the number accepted.
+int64_t orc_light_sample_range(void* rp, int32_t li, const float* P, const float* Ng, float lambda, uint64_t s0,
+                               uint64_t s1, uint64_t I0, int64_t count, int32_t n, float* out) {
+    Renderer* r = (Renderer*)rp;
+    SC sg{};
+    sg.P = V3(P[0], P[1], P[2]);
+    sg.Ng = V3(Ng[0], Ng[1], Ng[2]);
+    sg.N = sg.Ng;
+    sg.Lambda = lambda;
+    sg.Scramble[0] = s0; sg.Scramble[1] = s1;
+    int64_t m = 0;
+    std::vector<LightSample> ls;
+    for (int64_t i = 0; i < count; i++) {
+        sg.I = I0 + (uint64_t)i;
+        ls.clear();
+        r->SampleArea(r->lights[li], sg, n, ls);
+        for (const auto& l : ls) {
+            for (int k = 0; k < 3; k++) out[m * 5 + k] = l.Ld[k];
+            out[m * 5 + 3] = l.Ldist;
+            out[m * 5 + 4] = l.Pdf;
+            m++;
+        }
+    }
+    return m;
+}
+
+// EvaluateLightSamples (core/shader.go:203-402) of light `li` with ShaderStd's diffuse lobe
+// (Oren-Nayar with sigma = roughness^2, frame built as std.go:99-104) at a point P with normal
+// N = Ng, tangent dPdu, view direction Rd, for the sample indices I0 .. I0+count-1 of one pixel
+// (scrambles[0..1]); rgb: 3 floats per index.  The renderer's scene supplies the shadow rays.
+void orc_eval_light_samples(void* rp, int32_t li, const float* P, const float* N, const float* dPdu, const float* Rd,
+                            float lambda, float roughness, const uint64_t* scrambles, uint64_t I0, int64_t count,
+                            int32_t nsamples, float* rgb) {
+    Renderer* r = (Renderer*)rp;
+    Task* tk = new Task();
+    SC sg{};
+    sg.geom = -1;
+    sg.P = V3(P[0], P[1], P[2]);
+    sg.N = V3(N[0], N[1], N[2]);
+    sg.Ng = sg.N;
+    sg.Poffset = V3(0, 0, 0);
+    sg.DdPdu = V3(dPdu[0], dPdu[1], dPdu[2]);
+    sg.DdPdv = Cross(sg.N, sg.DdPdu);
+    sg.Rd = V3(Rd[0], Rd[1], Rd[2]);
+    sg.Lambda = lambda;
+    sg.Scramble[0] = scrambles[0]; sg.Scramble[1] = scrambles[1];
+    Vec3 V = Normalize(Cross(sg.N, sg.DdPdu));
+    Vec3 U = Normalize(Cross(sg.N, V));
+    OrenNayar diff{sg.Lambda, BasisProject(U, V, sg.N, Neg(sg.Rd)), roughness * roughness, U, V, sg.N};
+    for (int64_t i = 0; i < count; i++) {
+        sg.I = I0 + (uint64_t)i;
+        RGB c = r->EvaluateLightSamples(tk, sg, r->lights[li], nsamples, diff);
+        for (int k = 0; k < 3; k++) rgb[i * 3 + k] = c.c[k];
+    }
+    delete tk;
+}
+
+// ToRGB(FromRGB(a) * FromRGB(b)) at hero wavelength lambda (colour/spectrum.go): the colour factor of a
+// product of two RGB reflectances/emissions, which shading applies per light sample.
+void orc_spectrum_product_rgb(const float* a, const float* b, float lambda, float* rgb) {
+    Spectrum sa, sb;
+    sa.Lambda = lambda; sb.Lambda = lambda;
+    RGB ra, rb;
+    for (int k = 0; k < 3; k++) { ra.c[k] = a[k]; rb.c[k] = b[k]; }
+    sa.FromRGB(ra); sb.FromRGB(rb);
+    sa.Mul(sb);
+    RGB o = sa.ToRGB();
+    for (int k = 0; k < 3; k++) rgb[k] = o.c[k];
+}
+
+// MicrofacetGGX (microfacetggx.go) in the tangent frame (U, V, N = x, y, z), dielectric Fresnel with eta:
+// Sample for n (r0, r1) pairs, PDF and Eval (4 spectral values) of given directions.
+void orc_ggx_sample_n(float roughness, const float* omegaR, int64_t n, const double* r0, const double* r1, float* dirs) {
+    GGX g{500.0f, V3(omegaR[0], omegaR[1], omegaR[2]), roughness, FresnelModel(), V3(1, 0, 0), V3(0, 1, 0), V3(0, 0, 1)};
+    for (int64_t i = 0; i < n; i++) {
+        Vec3 o = g.Sample(r0[i], r1[i]);
+        for (int k = 0; k < 3; k++) dirs[i * 3 + k] = o[k];
+    }
+}
+void orc_ggx_pdf_eval_n(float roughness, float eta, float lambda, const float* omegaR, int64_t n, const float* dirs,
+                        double* pdf, float* eval4) {
+    FresnelModel f;
+    f.eta = eta;
+    GGX g{lambda, V3(omegaR[0], omegaR[1], omegaR[2]), roughness, f, V3(1, 0, 0), V3(0, 1, 0), V3(0, 0, 1)};
+    for (int64_t i = 0; i < n; i++) {
+        Vec3 o = V3(dirs[i * 3], dirs[i * 3 + 1], dirs[i * 3 + 2]);
+        if (pdf) pdf[i] = g.PDF(o);
+        if (eval4) { Spectrum s = g.Eval(o); for (int k = 0; k < 4; k++) eval4[i * 4 + k] = s.C[k]; }
+    }
+}
+float orc_ggx_g1(const float* w, const float* m, float alpha) {
+    return ggxG1(V3(w[0], w[1], w[2]), V3(m[0], m[1], m[2]), alpha);
+}
+}
+
+extern "C" {
+// The GGX D and Smith G1 terms (microfacetggx.go:26-80) over n directions.
+void orc_ggx_d_n(const float* m, int64_t n, float alpha, float* out) {
+    for (int64_t i = 0; i < n; i++) out[i] = ggxD(V3(m[i * 3], m[i * 3 + 1], m[i * 3 + 2]), alpha);
+}
+void orc_ggx_g1_n(const float* w, const float* m, int64_t n, float alpha, float* out) {
+    const Vec3 ww = V3(w[0], w[1], w[2]);
+    for (int64_t i = 0; i < n; i++) out[i] = ggxG1(ww, V3(m[i * 3], m[i * 3 + 1], m[i * 3 + 2]), alpha);
+}
+}
