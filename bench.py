@@ -178,28 +178,14 @@ def other_workloads(torch, np):
     sc = va.Scene(0)
     sc.add_geom(va.PolyMesh(verts, idx))
     sc.pre_render()
-    base = scenes.pinhole_rays(1024, 1024, fov_deg=60.0, eye=(0.0, 0.0, 45.0), target=(0.0, 0.0, 0.0))
-    rays = np.concatenate([base] * 4)
-    k = np.arange(len(rays), dtype=np.uint64)
-    rev = np.zeros(len(rays), np.float64)
-    for b in range(32):
-        rev += ((k >> np.uint64(b)) & np.uint64(1)).astype(np.float64) * 2.0 ** -(b + 1)
-    rays["time"] = rev.astype(np.float32)
+    rays = scenes.moving_spheres_camera_rays()
     n = len(rays)
     d = dev_rays(rays)
     h = torch.empty(n * 16, dtype=torch.uint8, device=dev)
     ms_cam = _time_calls(torch, lambda: sc.trace_device(d.data_ptr(), n, h.data_ptr(), stream=st), 5)
-    hits = h.view(torch.int32).view(n, 4)
-    dr = d.view(torch.float32).view(n, 8)
-    t = hits[:, 1].view(torch.float32)
-    hit = hits[:, 0] >= 0
-    P = dr[:, 0:3] + dr[:, 4:7] * torch.where(hit, t, torch.zeros_like(t))[:, None]
-    L = torch.tensor([10.0, 30.0, 20.0], device=dev)
-    srays = torch.zeros_like(dr)
-    srays[:, 0:3] = P + (L - P) * 1e-4
-    srays[:, 4:7] = L - P
-    srays[:, 3] = torch.where(hit, torch.full_like(t, 1.0 - 1e-4), torch.full_like(t, -1.0))
-    srays[:, 7] = dr[:, 7]
+    hits_h = h.cpu().numpy().view(va.HIT_DTYPE)
+    hit = torch.from_numpy(hits_h["prim"] >= 0)
+    srays = dev_rays(scenes.point_light_shadow_rays(rays, hits_h["prim"], hits_h["t"]))
     sh = torch.empty(n * 16, dtype=torch.uint8, device=dev)
     ms_sh = _time_calls(torch, lambda: sc.trace_device(srays.data_ptr(), n, sh.data_ptr(), shadow=True, stream=st), 5)
     sc.status(st)

@@ -8,7 +8,7 @@
 //
 // Path mode has no reference counterpart (the reference shades one camera hit
 // with direct light).  Its definition, restated here:
-//   u(pixel, pass, dim)  lowbias32 hash chain: k = mix(mix(pixel ^ s_lo) ^ (pass + s_hi)),
+//   u(pixel, pass, dim)  lowbias32 hash chain: k = mix(mix(pixel ^ mix(s_lo)) ^ (pass + s_hi)),
 //                        u = (mix(k + dim * 0x9e3779b9) >> 8) * 2^-24
 //   dims                 0, 1: pixel jitter; vertex d uses 2 + 8d + {0: light pick,
 //                        1, 2: point on the light, 3: mixture pick, 4, 5: direction}
@@ -172,7 +172,7 @@ struct PT {
 
     // One path (float arithmetic in the GPU's order); returns this pass's pixel radiance.
     F3 path(orc::Task* tk, int pixel, uint32_t pass, int64_t& rays) const {
-        const uint32_t key = mix32(mix32((uint32_t)pixel ^ s_lo) ^ (pass + s_hi));
+        const uint32_t key = mix32(mix32((uint32_t)pixel ^ mix32(s_lo)) ^ (pass + s_hi));
         const float fx = (float)(pixel % W) + rnd(key, 0);
         const float fy = (float)(pixel / W) + rnd(key, 1);
         const float Sx = 2.0f * fx / (float)W - 1.0f;
@@ -397,7 +397,7 @@ void orc_pt_set_no_nee(void* pp, int32_t on) { ((PT*)pp)->no_nee = on != 0; }
 void orc_pt_sincos2pi(float u, float* s, float* c) { opt::sincos2pi(u, *s, *c); }
 
 float orc_pt_rnd(uint32_t pixel, uint32_t pass, uint32_t dim, uint64_t seed) {
-    const uint32_t key = opt::mix32(opt::mix32(pixel ^ (uint32_t)seed) ^ (pass + (uint32_t)(seed >> 32)));
+    const uint32_t key = opt::mix32(opt::mix32(pixel ^ opt::mix32((uint32_t)seed)) ^ (pass + (uint32_t)(seed >> 32)));
     return opt::rnd(key, dim);
 }
 

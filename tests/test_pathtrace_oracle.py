@@ -37,6 +37,19 @@ def test_hash_random_numbers_are_uniform():
     assert np.abs(c - np.eye(4)).max() < 0.05
 
 
+def test_small_seeds_give_unrelated_streams():
+    """Seeds 1 and 2 (or any two seeds) must not share streams: XOR-ing the raw seed into the pixel
+    index would give pixel p under seed 2 the numbers of pixel p ^ 3 under seed 1."""
+    W = 1 << 12
+    for s1, s2 in ((1, 2), (1, 3), (7, 8)):
+        a = np.array([oracle.pt_rnd(p, 0, 2, s1) for p in range(W)])
+        b = np.array([oracle.pt_rnd(p, 0, 2, s2) for p in range(W)])
+        delta = s1 ^ s2
+        assert not np.any(b == a[np.arange(W) ^ delta])
+        assert len(np.intersect1d(a, b)) < 8            # 2^24 possible values: chance matches only
+        assert abs(np.corrcoef(a, b)[0, 1]) < 0.06
+
+
 @pytest.mark.parametrize("rho,depth", [(0.5, 4), (0.8, 3)])
 def test_furnace_closed_form(rho, depth):
     d = scenes.furnace_box_scene(albedo=rho, emission=1.0)

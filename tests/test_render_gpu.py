@@ -67,7 +67,9 @@ def test_render_box_spheres_bitexact(box_scene):
 
 def test_render_within_1e4_of_reference_normalize(box_scene):
     """Oracle on RSQRTSS (the reference's arithmetic on this host CPU): the film agrees within 1e-4
-    relative (absolute floor 1e-3) on all but a small fraction of pixels."""
+    relative (absolute floor 1e-3) on all but a small fraction of pixels: 1.84% measured on the MI355X
+    hosts, where a 1-ulp direction change flips a shadow ray grazing an edge (the IEEE-mode test above
+    is bit-exact)."""
     cam, M, ttf = _camera(box_scene, 256, 256)
     g = render.Renderer(box_scene, 256, 256, cam)
     o = oracle.OracleRenderer(box_scene, 256, 256, M, ttf, box_scene["camera"]["focal"])
@@ -80,7 +82,7 @@ def test_render_within_1e4_of_reference_normalize(box_scene):
     rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
     frac_bad = float(np.mean(~(ok | (rel <= 1e-4))))
     print(f"[case] RSQRTSS oracle: fraction of film values beyond 1e-4: {frac_bad:.4f}", flush=True)
-    assert frac_bad < 0.05
+    assert frac_bad < 0.025
 
 
 def test_render_device_pointer_path(box_scene):

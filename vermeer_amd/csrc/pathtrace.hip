@@ -46,8 +46,10 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {   // lowbias32 (C. Wello
     x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
     return x;
 }
+// The seed is hashed before it meets the pixel index: XOR with the raw seed would only permute the
+// pixels, so two small seeds would share the same streams at different pixels.
 __device__ __forceinline__ uint32_t path_key(const PTView& v, uint32_t pixel, uint32_t pass) {
-    return mix32(mix32(pixel ^ v.seed_lo) ^ (pass + v.seed_hi));
+    return mix32(mix32(pixel ^ mix32(v.seed_lo)) ^ (pass + v.seed_hi));
 }
 __device__ __forceinline__ float rnd(uint32_t key, uint32_t dim) {
     return (float)(mix32(key + dim * 0x9e3779b9u) >> 8) * 5.9604645e-08f;

@@ -245,6 +245,43 @@ def moving_spheres(count=2000, nseg=25, nring=6, extent=20.0, rmin=0.3, rmax=1.0
     return np.stack([k0, k1]).astype(np.float32), idx
 
 
+def van_der_corput(n):
+    """Base-2 radical inverse of 0..n-1 (32 bits), as float32."""
+    k = np.arange(n, dtype=np.uint64)
+    rev = np.zeros(n, np.float64)
+    for b in range(32):
+        rev += ((k >> np.uint64(b)) & np.uint64(1)).astype(np.float64) * 2.0 ** -(b + 1)
+    return rev.astype(np.float32)
+
+
+def moving_spheres_camera_rays(width=1024, spp=4):
+    """Camera rays of the motion-blur workload: width^2 pinhole rays (60 degree FOV from (0, 0, 45)
+    towards the origin) repeated spp times, ray k at time VanDerCorput(k)."""
+    base = pinhole_rays(width, width, fov_deg=60.0, eye=(0.0, 0.0, 45.0), target=(0.0, 0.0, 0.0))
+    rays = np.concatenate([base] * spp)
+    rays["time"] = van_der_corput(len(rays))
+    return rays
+
+
+def point_light_shadow_rays(rays, prim, t, light=(10.0, 30.0, 20.0)):
+    """One shadow ray per camera ray towards a point light, at the camera ray's time: from the hit
+    point P = O + t D, origin P + 1e-4 (L - P), direction L - P, tmax 1 - 1e-4; a ray whose camera ray
+    missed gets tmax -1 (it can hit nothing).  float32 throughout."""
+    f = np.float32
+    O = rays["P"].astype(f)
+    D = rays["D"].astype(f)
+    hit = np.asarray(prim) >= 0
+    tt = np.where(hit, np.asarray(t, f), f(0))
+    P = O + D * tt[:, None]
+    L = np.asarray(light, f)
+    out = np.zeros(len(rays), RAY_DTYPE)
+    out["P"] = P + (L - P) * f(1e-4)
+    out["D"] = L - P
+    out["tmax"] = np.where(hit, f(1.0 - 1e-4), f(-1.0))
+    out["time"] = rays["time"]
+    return out
+
+
 # ------------------------------------------------------------ shaded scenes
 
 # ShaderStd parameters (builtin/shader/std.go:25-47); has_* flags mark parameters that are set.
