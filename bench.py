@@ -72,6 +72,9 @@ def parse():
                     help="guide sampling probability from --pt-late-pass on")
     ap.add_argument("--pt-warm", type=int, default=0,
                     help="guided passes left out of the image (their time still counts)")
+    ap.add_argument("--pt-seeds", default="2,3,4",
+                    help="path seeds of the guided runs (the schedule was tuned on another seed); the time "
+                         "ratio is their mean")
     return ap.parse_args()
 
 
@@ -85,6 +88,61 @@ def _time_calls(torch, fn, reps):
     e1.record()
     e1.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def _traversal_roofline(torch, sc, d_rays, n, shadow, motion, reps=10):
+    """Roofline of one traversal launch over n device-resident rays: per-ray node visits and triangle
+    tests from a counting launch give the algorithmic bytes in the reference layout (static
+    48 + 128 N + 48 T, motion 48 + 224 N + 84 T: ray and hit, nodes, triangles), divided by the
+    traversal kernel's average duration from the library's HIP events."""
+    st = torch.cuda.current_stream().cuda_stream
+    d_h = torch.empty(n * 16, dtype=torch.uint8, device=d_rays.device)
+    d_c = torch.empty(n * 4, dtype=torch.int32, device=d_rays.device)
+    sc.trace_device(d_rays.data_ptr(), n, d_h.data_ptr(), shadow=shadow, d_counters=d_c.data_ptr(), stream=st)
+    sc.status(st)
+    c = d_c.view(n, 4).to(torch.float64)
+    nodes, tris = float(c[:, 0].mean()), float(c[:, 2].mean())
+    del c, d_c
+    bpr = (48.0 + 224.0 * nodes + 84.0 * tris) if motion else (48.0 + 128.0 * nodes + 48.0 * tris)
+    sc.trace_device(d_rays.data_ptr(), n, d_h.data_ptr(), shadow=shadow, stream=st)
+    sc.set_timing(True)
+    for _ in range(reps):
+        sc.trace_device(d_rays.data_ptr(), n, d_h.data_ptr(), shadow=shadow, stream=st)
+    tms, _, nl = sc.timing()
+    sc.set_timing(False)
+    sc.status(st)
+    kms = tms / max(nl, 1)
+    ach = bpr * n / (kms * 1e-3) / 1e9
+    return {"mean_node_visits": round(nodes, 3), "mean_triangle_tests": round(tris, 3),
+            "algorithmic_bytes_per_ray": round(bpr, 1), "kernel_avg_launch_ms": round(kms, 4),
+            "kernel_mrays_per_s": round(n / kms / 1e3, 1), "achieved_GBps": round(ach, 1),
+            "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
+def _converged_rmse(torch, R, d_scr, W, passes, conv_passes):
+    """RMSE of a passes-pass film against an FP64 mean of conv_passes passes' samples (the same QMC
+    sequence continued): for the film as the reference normalises it (sum / (passes + 1),
+    core/render.go:127-129) and for the plain mean sum / passes."""
+    st = torch.cuda.current_stream().cuda_stream
+    dev = d_scr.device
+    film = torch.zeros(W * W * 3, dtype=torch.float32, device=dev)
+    samp = torch.empty(W * W * 3, dtype=torch.float32, device=dev)
+    acc = torch.zeros(W * W * 3, dtype=torch.float64, device=dev)
+    acc_n = torch.zeros_like(acc)
+    for it in range(1, conv_passes + 1):
+        R.render_pass_device(d_scr.data_ptr(), it, film.data_ptr(), d_sample=samp.data_ptr(), stream=st)
+        acc += samp.double()
+        if it == passes:
+            acc_n.copy_(acc)
+    R.scene.status(st)
+    conv = acc / conv_passes
+    fin = torch.isfinite(conv) & torch.isfinite(acc_n)
+    film_ref = acc_n / (passes + 1)
+    mean_n = acc_n / passes
+    return {"passes": passes, "converged_passes": conv_passes, "converged_mean": round(float(conv[fin].mean()), 6),
+            "rmse_film": round(float(torch.sqrt(torch.mean((film_ref - conv)[fin] ** 2))), 6),
+            "rmse_mean": round(float(torch.sqrt(torch.mean((mean_n - conv)[fin] ** 2))), 6),
+            "nonfinite_values": int((~fin).sum())}
 
 
 def other_workloads(torch, np):
@@ -106,17 +164,19 @@ def other_workloads(torch, np):
     sc.add_geom(va.PolyMesh(*scenes.uv_sphere()))
     sc.add_geom(va.PolyMesh(*scenes.ground_quad()))
     sc.pre_render()
-    res = {}
+    res, roof = {}, {}
     for name, rays in (("pinhole_1024x1024", scenes.pinhole_rays()), ("uniform_1M", scenes.uniform_rays())):
         d = dev_rays(rays)
         n = len(rays)
         h = torch.empty(n * 16, dtype=torch.uint8, device=dev)
         for shadow in (False, True):
+            key = f"{name}_{'anyhit' if shadow else 'closest'}"
             ms = _time_calls(torch, lambda: sc.trace_device(d.data_ptr(), n, h.data_ptr(), shadow=shadow, stream=st), 20)
-            res[f"{name}_{'anyhit' if shadow else 'closest'}_mrays_per_s"] = round(n / ms / 1e3, 1)
+            res[f"{key}_mrays_per_s"] = round(n / ms / 1e3, 1)
+            roof[key] = _traversal_roofline(torch, sc, d, n, shadow, motion=False)
     sc.status(st)
     sc.close()
-    out["sphere_ground"] = dict(res, triangles=65026)
+    out["sphere_ground"] = dict(res, triangles=65026, roofline=roof)
     log(f"[bench] sphere+ground: {res}")
 
     # reference-semantics renderer: box + 8 displaced spheres, 512^2, 16 passes of RasterXY QMC
@@ -146,7 +206,8 @@ def other_workloads(torch, np):
     out["box_spheres_render"] = {"image": [W, W], "passes": 16, "ms": round(ms, 2),
                                  "samples_per_s": round(W * W * 16 / (ms * 1e-3), 1),
                                  "rays": int(nrays), "mrays_per_s": round(nrays / (ms * 1e-3) / 1e6, 1),
-                                 "triangles": int(sum(len(m["idx"]) for m in desc["meshes"]))}
+                                 "triangles": int(sum(len(m["idx"]) for m in desc["meshes"])),
+                                 "rmse_vs_converged": _converged_rmse(torch, R, scr, W, 16, 1024)}
     R.close()
     log(f"[bench] box+spheres renderer: {out['box_spheres_render']}")
 
@@ -189,12 +250,14 @@ def other_workloads(torch, np):
     sh = torch.empty(n * 16, dtype=torch.uint8, device=dev)
     ms_sh = _time_calls(torch, lambda: sc.trace_device(srays.data_ptr(), n, sh.data_ptr(), shadow=True, stream=st), 5)
     sc.status(st)
-    sc.close()
+    roof = {"camera_closest": _traversal_roofline(torch, sc, d, n, False, motion=True, reps=5),
+            "shadow_anyhit": _traversal_roofline(torch, sc, srays, n, True, motion=True, reps=5)}
     out["moving_spheres"] = {"triangles": int(len(idx)), "keys": 2, "camera_rays": n,
                              "camera_closest_mrays_per_s": round(n / ms_cam / 1e3, 1),
                              "shadow_anyhit_mrays_per_s": round(n / ms_sh / 1e3, 1),
                              "combined_mrays_per_s": round(2 * n / (ms_cam + ms_sh) / 1e3, 1),
-                             "hit_fraction": round(float(hit.float().mean()), 4)}
+                             "hit_fraction": round(float(hit.float().mean()), 4), "roofline": roof}
+    sc.close()
     log(f"[bench] moving spheres: {out['moving_spheres']}")
 
     # the same spheres rendered: each sample at its own time, camera + shadow rays as the reference counts
@@ -220,10 +283,43 @@ def other_workloads(torch, np):
     out["moving_spheres_render"] = {"image": [W, W], "passes": 4, "ms": round(ms, 2), "keys": 2,
                                     "triangles": int(len(desc["meshes"][0]["idx"])), "rays": int(nrays),
                                     "mrays_per_s": round(nrays / (ms * 1e-3) / 1e6, 1),
-                                    "samples_per_s": round(W * W * 4 / (ms * 1e-3), 1)}
+                                    "samples_per_s": round(W * W * 4 / (ms * 1e-3), 1),
+                                    "rmse_vs_converged": _converged_rmse(torch, R, scr, W, 4, 256)}
     R.close()
     log(f"[bench] moving spheres renderer: {out['moving_spheres_render']}")
     return out
+
+
+def path_traversal_roofline(torch, desc, scene, pt, acc, first_pass, passes=4):
+    """Roofline of path mode's traversal launches (closest hit per bounce, plus the NEE shadow rays):
+    the kernels' time from the library's HIP events over `passes` plain passes, and their
+    algorithmic bytes from the path oracle, whose plain paths are the GPU's bit for bit, on a
+    256^2 pass of the same scene: node visits and triangle tests per traced segment, bytes
+    48 + 128 N + 48 T, averaged over closest-hit and shadow segments by their counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from vermeer_amd import pathtrace
+    st = torch.cuda.current_stream().cuda_stream
+    o = oracle.OraclePathTracer(desc, 256, 256, pathtrace.camera_for(desc, 256, 256), max_depth=8, seed=1)
+    o.render_pass(0)
+    w = o.traversal_work()
+    del o
+    (nc, nnc, ntc), (ns, nns, nts) = w["closest"], w["shadow"]
+    bytes_per_ray = ((48.0 * nc + 128.0 * nnc + 48.0 * ntc) + (48.0 * ns + 128.0 * nns + 48.0 * nts)) / (nc + ns)
+    r0, _ = pt.stats()
+    scene.set_timing(True)
+    for p in range(first_pass, first_pass + passes):
+        pt.render_pass_device(acc.data_ptr(), p, stream=st)
+    tms, _, nl = scene.timing()
+    scene.set_timing(False)
+    r1, _ = pt.stats()
+    rays = r1 - r0
+    achieved = bytes_per_ray * rays / (tms * 1e-3) / 1e9
+    return {"launches": nl, "kernel_ms_per_pass": round(tms / passes, 4), "rays_per_pass": rays // passes,
+            "closest_mean_node_visits": round(nnc / nc, 3), "closest_mean_triangle_tests": round(ntc / nc, 3),
+            "shadow_mean_node_visits": round(nns / max(ns, 1), 3), "shadow_mean_triangle_tests": round(nts / max(ns, 1), 3),
+            "shadow_fraction_of_rays": round(ns / (nc + ns), 4), "algorithmic_bytes_per_ray": round(bytes_per_ray, 1),
+            "achieved_GBps": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4)}
 
 
 def guided_measurement(args, torch, np):
@@ -273,61 +369,69 @@ def guided_measurement(args, torch, np):
     acc_p = torch.zeros_like(acc_ref)
     t_plain = passes(pl, acc_p, 0, args.pt_plain_spp)
     e_plain = rmse(acc_p, args.pt_plain_spp, ref)
+    traversal = path_traversal_roofline(torch, desc, scene, pl, acc_p, args.pt_plain_spp)
     pl.close()
     log(f"[bench] guided: plain {args.pt_plain_spp} spp rmse {e_plain:.5f} in {t_plain:.1f} ms")
 
-    # guided, online training, until its RMSE reaches the plain one (or 4x the plain time)
-    gd = guide.Guide(pathtrace.scene_bounds(desc), seed=1, max_batch=args.pt_batch)
-    gp = pathtrace.PathTracer(desc, W, W, max_depth=8, seed=2, scene=scene)
-    gp.set_guide(gd, alpha=args.pt_alpha, record_stride=args.pt_stride, train_steps=args.pt_steps,
-                 batch=args.pt_batch, lr=args.pt_lr)
-    acc_g = torch.zeros_like(acc_ref)
-    t_g, n_img, p = 0.0, 0, 0
-    t_equal = None
-    e_half = None
-    trace = []
-    g0, _ = gp.stats()
-    while t_g < 4 * t_plain:
-        if p == args.pt_warm and p > 0:
-            acc_g.zero_()
-            n_img = 0
-        if p == args.pt_late_pass:
-            a_late = args.pt_alpha_late
-            gp.set_guide(gd, alpha=a_late, record_stride=args.pt_stride, train_steps=args.pt_steps_late,
-                         batch=args.pt_batch, lr=args.pt_lr)
-        t_g += passes(gp, acc_g, p, 1)
-        p += 1
-        n_img += 1
-        e = rmse(acc_g, n_img, ref)
-        trace.append((round(t_g, 2), round(e, 5)))
-        if t_g <= 0.5 * t_plain and p > args.pt_warm:
-            e_half = round(e, 6)
-        if t_equal is None and e <= e_plain and p > args.pt_warm:
-            t_equal = t_g
-            break
-    g1, nsamp = gp.stats()
-    guided_mrays = (g1 - g0) / (t_g * 1e-3) / 1e6
-    gp.close()
-    gd.close()
+    # guided, online training, until its RMSE reaches the plain one (or 4x the plain time); once per seed
+    def guided_run(seed):
+        gd = guide.Guide(pathtrace.scene_bounds(desc), seed=1, max_batch=args.pt_batch)
+        gp = pathtrace.PathTracer(desc, W, W, max_depth=8, seed=seed, scene=scene)
+        gp.set_guide(gd, alpha=args.pt_alpha, record_stride=args.pt_stride, train_steps=args.pt_steps,
+                     batch=args.pt_batch, lr=args.pt_lr)
+        acc_g = torch.zeros_like(acc_ref)
+        t_g, n_img, p = 0.0, 0, 0
+        t_equal = None
+        e_half = None
+        trace = []
+        g0, _ = gp.stats()
+        while t_g < 4 * t_plain:
+            if p == args.pt_warm and p > 0:
+                acc_g.zero_()
+                n_img = 0
+            if p == args.pt_late_pass:
+                gp.set_guide(gd, alpha=args.pt_alpha_late, record_stride=args.pt_stride,
+                             train_steps=args.pt_steps_late, batch=args.pt_batch, lr=args.pt_lr)
+            t_g += passes(gp, acc_g, p, 1)
+            p += 1
+            n_img += 1
+            e = rmse(acc_g, n_img, ref)
+            trace.append((round(t_g, 2), round(e, 5)))
+            if t_g <= 0.5 * t_plain and p > args.pt_warm:
+                e_half = round(e, 6)
+            if t_equal is None and e <= e_plain and p > args.pt_warm:
+                t_equal = t_g
+                break
+        g1, _ = gp.stats()
+        gp.close()
+        gd.close()
+        return {"seed": seed, "passes": p, "ms_to_equal_rmse": None if t_equal is None else round(t_equal, 2),
+                "time_ratio": None if t_equal is None else round(t_equal / t_plain, 4),
+                "rmse_at_half_plain_time": e_half, "mrays_per_s": round((g1 - g0) / (t_g * 1e-3) / 1e6, 1),
+                "rmse_trace_ms": trace[:: max(1, len(trace) // 16)]}
+
+    runs = [guided_run(int(x)) for x in args.pt_seeds.split(",")]
+    for r in runs:
+        log(f"[bench] guided seed {r['seed']}: reached plain rmse after {r['passes']} passes, "
+            f"{r['ms_to_equal_rmse']} ms (ratio {r['time_ratio']}); {r['mrays_per_s']:.0f} Mrays/s")
+    ratios = [r["time_ratio"] for r in runs]
+    done = [x for x in ratios if x is not None]
     scene.close()
-    ratio = None if t_equal is None else t_equal / t_plain
-    log(f"[bench] guided: reached plain rmse after {p} passes, {t_g:.1f} ms (ratio {ratio}); "
-        f"{guided_mrays:.0f} Mrays/s")
+    ratio = sum(done) / len(done) if len(done) == len(ratios) else None
     return {
         "scene": "occluded-light room", "triangles": ntri, "image": [W, W], "max_depth": 8, "bvh_leaf_size": args.pt_leaf,
         "reference_spp": args.pt_ref_spp, "reference_mean": round(float(ref.mean()), 6),
         "plain_spp": args.pt_plain_spp, "plain_rmse": round(e_plain, 6), "plain_ms": round(t_plain, 2),
         "plain_samples_per_s": round(npx * args.pt_plain_spp / (t_plain * 1e-3), 1),
         "plain_mrays_per_s": round(plain_mrays, 1),
-        "guided_passes": p, "guided_ms_to_equal_rmse": None if t_equal is None else round(t_equal, 2),
+        "plain_traversal_roofline": traversal,
         "time_ratio": None if ratio is None else round(ratio, 4),
-        "guided_rmse_at_half_plain_time": e_half,
-        "guided_mrays_per_s": round(guided_mrays, 1),
+        "time_ratio_min_max": [min(done), max(done)] if done else None,
+        "guided_runs": runs,
         "guide": {"alpha": args.pt_alpha, "record_stride": args.pt_stride, "adam_steps_per_pass": args.pt_steps,
                   "batch": args.pt_batch, "lr": args.pt_lr, "warm_passes": args.pt_warm,
                   "late_pass": args.pt_late_pass, "adam_steps_per_pass_late": args.pt_steps_late,
                   "alpha_late": args.pt_alpha_late},
-        "rmse_trace_ms": trace[:: max(1, len(trace) // 16)],
     }
 
 
@@ -347,6 +451,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    from vermeer_amd import hbmcopy
+    achievable = hbmcopy.measure(torch)
+    log(f"[bench] streaming kernels over 4 GiB: copy {achievable['copy_GBps']} GB/s (read + write), "
+        f"read {achievable['read_GBps']} GB/s")
 
     t0 = time.time()
     verts, idx = scenes.icosphere_field(count=args.spheres)[:2]
@@ -471,8 +579,16 @@ def main():
         o = oracle.OracleScene()
         o.add_mesh(verts, idx)
         o.commit()
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = min(threads, 16)
+        visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = min(visible, 16)      # the host CPU share that goes with one GPU on these machines
+        cpu_model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         probe = 20000
         t0 = time.perf_counter()
         o.trace(d_rays[: probe * 32].cpu().numpy().view(va.RAY_DTYPE), nthreads=threads)
@@ -484,8 +600,10 @@ def main():
         o.trace(sample, nthreads=threads)
         dt = time.perf_counter() - t0
         cpu = {"value": round(m / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model, "visible_cores": visible,
                "sample": f"{m} of the {n} bounce rays (evenly strided across all bounces), closest hit, "
-                         f"C++ restatement of the Go traversal, OpenMP {threads} threads, {dt:.1f}s"}
+                         f"C++ restatement of the Go traversal, OpenMP {threads} threads (the host share of one "
+                         f"GPU; {visible} cores visible), {cpu_model}, {dt:.1f}s"}
         log(f"[bench] CPU baseline {cpu['value']} Mrays/s on {threads} threads")
 
     # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a
@@ -541,6 +659,9 @@ def main():
             "traffic_config": traffic_cfg,
             "algorithmic_bytes_per_ray": round(bytes_per_ray, 1),
             "kernel": "k_trace_wide (traversal)",
+            "achievable_copy_GBps": achievable["copy_GBps"],
+            "achievable_read_GBps": achievable["read_GBps"],
+            "frac_of_achievable_read": round(achieved / achievable["read_GBps"], 4),
             "kernel_ms_per_step": round(kernel_ms_per_step, 4),
             "kernel_avg_launch_ms": round(trace_ms / max(nlaunch, 1), 4),
             "sort_ms_per_step": round(sort_ms_per_step, 4),

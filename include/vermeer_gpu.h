@@ -346,6 +346,17 @@ int vgpu_scene_set_leaf_size(vgpu_scene* scene, int32_t leaf_max);
  * sort milliseconds and the number of traversals since the last call, and
  * starts a new interval. */
 int vgpu_scene_set_timing(vgpu_scene* scene, int32_t enable);
+
+/* Streaming diagnostic, no reference counterpart: measures the HBM bandwidth a
+ * plain access pattern reaches and calibrates the FETCH_SIZE / WRITE_SIZE
+ * counters.  Enqueues on `stream` (current device) one kernel of
+ * CUs x blocks_per_cu workgroups of 256 threads over `bytes` (a multiple of
+ * 16; 16-byte aligned device buffers).  VGPU_DIAG_COPY copies src to dst;
+ * VGPU_DIAG_READ reads src and writes one float per thread to dst
+ * (CUs x blocks_per_cu x 256 floats). */
+#define VGPU_DIAG_COPY 0
+#define VGPU_DIAG_READ 1
+int vgpu_diag_stream(int32_t mode, const void* src, void* dst, uint64_t bytes, int32_t blocks_per_cu, void* stream);
 int vgpu_scene_timing(vgpu_scene* scene, float* trace_ms, float* sort_ms, int32_t* launches);
 
 /* Traversal launch configuration (the autotuner's knobs).  Until the caller

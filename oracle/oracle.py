@@ -673,6 +673,7 @@ def _pt_sigs(lib):
         "orc_pt_set_fixed_guide": (None, [P, C.c_float, P]),
         "orc_pt_set_no_nee": (None, [P, C.c_int32]),
         "orc_pt_rnd": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
+        "orc_pt_work": (None, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -730,6 +731,13 @@ class OraclePathTracer:
     def set_no_nee(self, on=True):
         """Diagnostic: sample directions only (no next-event estimation)."""
         self.lib.orc_pt_set_no_nee(self.h_pt, int(bool(on)))
+
+    def traversal_work(self):
+        """Traversal work since the last call (then reset): {"closest": (rays, node visits, triangle
+        tests), "shadow": (...)}."""
+        w = np.zeros(6, np.int64)
+        self.lib.orc_pt_work(self.h_pt, _p(w))
+        return {"closest": tuple(int(x) for x in w[:3]), "shadow": tuple(int(x) for x in w[3:])}
 
     def render_pass(self, pass_index, nthreads=0, want_sample=False):
         samp = np.zeros((self.h, self.w, 3), np.float32) if want_sample else None

@@ -24,6 +24,7 @@
 //   depth                up to max_depth segments from the camera
 // Closest-hit and shadow traversal go through the traversal oracle (oracle_core.h).
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -132,6 +133,8 @@ struct PT {
     float alpha = 0.0f;
     float lobes[40];
     bool no_nee = false;   // diagnostic: BSDF/guide sampling only, emission weight 1
+    // traversal work of the traced segments (closest hit, then shadow): rays, node visits, triangle tests
+    mutable std::atomic<int64_t> work[6] = {};
 
     void finalize() {
         if (finalized) return;
@@ -156,7 +159,9 @@ struct PT {
         orc::RaySetup(r);
         tk->StackTop = 0; r.task = tk;
         orc::SceneHit sh{}; sh.geom = -1; sh.hr.elem = -1;
-        if (!scene->Trace(r, sh)) return false;
+        const bool hit = scene->Trace(r, sh);
+        work[0]++; work[1] += r.NodesT; work[2] += r.TrisT;
+        if (!hit) return false;
         geom = sh.geom; elem = sh.hr.elem; t = r.Tclosest; U = sh.hr.U; V = sh.hr.V; Wb = sh.hr.W;
         return true;
     }
@@ -167,7 +172,9 @@ struct PT {
         orc::RaySetup(r);
         tk->StackTop = 0; r.task = tk;
         orc::SceneHit sh{}; sh.geom = -1;
-        return scene->Trace(r, sh);
+        const bool hit = scene->Trace(r, sh);
+        work[3]++; work[4] += r.NodesT; work[5] += r.TrisT;
+        return hit;
     }
 
     // One path (float arithmetic in the GPU's order); returns this pass's pixel radiance.
@@ -382,6 +389,13 @@ int64_t orc_pt_pass(void* pp, uint32_t pass, double* accum, float* sample_out, i
         delete tk;
     }
     return rays;
+}
+
+// Traversal work since the last call: closest-hit rays, node visits, triangle tests, then the same for
+// shadow rays (the counters qbvh.Trace keeps, qbvh/intersect.go:114,222); resets them.
+void orc_pt_work(void* pp, int64_t* out6) {
+    PT* p = (PT*)pp;
+    for (int k = 0; k < 6; k++) out6[k] = p->work[k].exchange(0);
 }
 
 // Fixed-proposal guided sampling (alpha > 0): lobes = 8 x (mu xyz, kappa, weight).

@@ -33,6 +33,8 @@ VGPU_LIGHT_SPHERE = 2
 VGPU_FILTER_BOX = 0
 VGPU_FILTER_GAUSSIAN = 1
 VGPU_FILTER_AIRY = 2
+VGPU_DIAG_COPY = 0
+VGPU_DIAG_READ = 1
 
 # vgpu_qbvh_node (128 bytes, qbvh.Node layout)
 NODE_DTYPE = np.dtype([("boxes", "<f4", 24), ("axis", "<u4", 3), ("children", "<i4", 4), ("parent", "<i4")])
@@ -76,6 +78,7 @@ SIGNATURES = [
     ("vgpu_scene_set_leaf_size", C.c_int, [_P, _I32]),
     ("vgpu_scene_set_timing", C.c_int, [_P, _I32]),
     ("vgpu_scene_timing", C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_I32)]),
+    ("vgpu_diag_stream", C.c_int, [_I32, _P, _P, _U64, _I32, _P]),
     ("vgpu_scene_add_shaded_mesh", C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _P, _U32, C.POINTER(_I32)]),
     ("vgpu_scene_add_shaded_motion_mesh", C.c_int,
      [_P, _P, _U32, _U32, _P, _U32, _P, _U32, _P, _P, _U32, _U32, C.POINTER(_I32)]),

@@ -253,8 +253,16 @@ int vgpu_pt_pass(vgpu_pt* pt, uint32_t flags, uint32_t pass, double* accum, void
     for (int d = 0; d < v.max_depth && e == hipSuccess; d++) {
         const int cur = d & 1;
         stage = "closest hit";
+        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+        if (s->timing) {   // the same [sort start, trace start, trace end] records as enqueue_trace
+            for (auto& x : ev) x = timing_event(s);
+            if (!ev[0] || !ev[1] || !ev[2]) return fail(VGPU_ERR_HIP, "timing events");
+            HIP_TRY(hipEventRecord(ev[0], st));
+            HIP_TRY(hipEventRecord(ev[1], st));
+        }
         e = vgpu::launch_trace_ext(s->view, false, bb.qray[cur], np, bb.hits, s->d_spT, s->d_spN, s->spill_threads,
                                    s->d_err, s->num_cus, st, bb.count + cur);
+        if (ev[2] && e == hipSuccess) HIP_TRY(hipEventRecord(ev[2], st));
         if (e == hipSuccess) { stage = "hit"; e = vgpu::pt_launch_hit(v, pass, cur, bb, st); }
         if (e != hipSuccess || d + 1 >= v.max_depth) continue;
         if (guided) {

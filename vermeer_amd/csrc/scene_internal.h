@@ -154,6 +154,10 @@ struct DeviceGuard {
     ~DeviceGuard() { if (prev >= 0) (void)hipSetDevice(prev); }
 };
 
+// Next free timing event of vgpu_scene_set_timing (the pool grows; vgpu_scene_timing recycles it);
+// NULL when the runtime cannot create one.
+hipEvent_t timing_event(vgpu_scene* s);
+
 // Ensures the spill area covers `threads` traversal threads.
 int ensure_spill(vgpu_scene* s, uint64_t threads);
 // Enqueues one traversal (device pointers) with configuration cfg.  n_dev (nullable): device-resident
