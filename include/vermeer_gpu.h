@@ -467,6 +467,30 @@ int vgpu_pt_pass(vgpu_pt* pt, uint32_t flags, uint32_t pass, double* accum, void
  * the last pass.  Synchronises the device. */
 int vgpu_pt_stats(vgpu_pt* pt, uint64_t* rays, uint32_t* train_samples);
 
+/* ------------------------------------------------------------------------
+ * Output drivers.  The film is width*height*3 floats in pixel order (row 0
+ * first), as core.FrameBuf() holds it; with VGPU_TRACE_DEVICE_PTRS it is
+ * device memory on `device` (work on `stream`), otherwise host memory.
+ * ------------------------------------------------------------------------ */
+
+/* The Radiance RGBE bytes of the film as image/hdr/writer.go:62-107 and
+ * hdr.go:24-48 produce them on amd64: 4 bytes per pixel, the bottom film row
+ * first; rgbe (width*height*4 bytes) is device memory with
+ * VGPU_TRACE_DEVICE_PTRS, else host memory.  Encoded on the GPU. */
+int vgpu_film_to_rgbe(int device, uint32_t flags, const float* film, uint32_t width, uint32_t height, uint8_t* rgbe,
+                      void* stream);
+
+/* Replaces OutputHDR.PostRender (builtin/driver/outputhdr.go:30-57): writes
+ * the file byte for byte as the reference does (its header, no run-length
+ * encoding).  Synchronous. */
+int vgpu_write_hdr(int device, uint32_t flags, const float* film, uint32_t width, uint32_t height, const char* path,
+                   void* stream);
+
+/* Replaces OutputFloat.PostRender (builtin/driver/outputfloat.go:30-42): the
+ * film's float32 values, little-endian, in pixel order.  Synchronous. */
+int vgpu_write_float(int device, uint32_t flags, const float* film, uint32_t width, uint32_t height, const char* path,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -756,3 +756,40 @@ def pt_sincos2pi(u):
 
 def pt_rnd(pixel, pass_index, dim, seed):
     return _pt_sigs(load()).orc_pt_rnd(pixel, pass_index, dim, seed)
+
+
+# ------------------------------------------------------------ output drivers (oracle_output.cpp)
+
+def _output_sigs(lib):
+    P = C.c_void_p
+    for name, res, args in (("orc_film_to_rgbe", None, [P, C.c_int32, C.c_int32, P]),
+                            ("orc_write_hdr", C.c_int, [P, C.c_int32, C.c_int32, C.c_char_p]),
+                            ("orc_write_float", C.c_int, [P, C.c_int32, C.c_int32, C.c_char_p])):
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def film_to_rgbe(film):
+    """convertRGBToRGBE over a (h, w, 3) float32 film, in the file's scanline order: (h, w, 4) uint8."""
+    lib = _output_sigs(load())
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    out = np.zeros((h, w, 4), np.uint8)
+    lib.orc_film_to_rgbe(_p(f), w, h, _p(out))
+    return out
+
+
+def write_hdr(film, path):
+    lib = _output_sigs(load())
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    assert lib.orc_write_hdr(_p(f), w, h, str(path).encode()) == 0
+
+
+def write_float(film, path):
+    lib = _output_sigs(load())
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    assert lib.orc_write_float(_p(f), w, h, str(path).encode()) == 0

@@ -79,6 +79,9 @@ SIGNATURES = [
     ("vgpu_scene_set_timing", C.c_int, [_P, _I32]),
     ("vgpu_scene_timing", C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_I32)]),
     ("vgpu_diag_stream", C.c_int, [_I32, _P, _P, _U64, _I32, _P]),
+    ("vgpu_film_to_rgbe", C.c_int, [C.c_int, _U32, _P, _U32, _U32, _P, _P]),
+    ("vgpu_write_hdr", C.c_int, [C.c_int, _U32, _P, _U32, _U32, C.c_char_p, _P]),
+    ("vgpu_write_float", C.c_int, [C.c_int, _U32, _P, _U32, _U32, C.c_char_p, _P]),
     ("vgpu_scene_add_shaded_mesh", C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _P, _U32, C.POINTER(_I32)]),
     ("vgpu_scene_add_shaded_motion_mesh", C.c_int,
      [_P, _P, _U32, _U32, _P, _U32, _P, _U32, _P, _P, _U32, _U32, C.POINTER(_I32)]),
