@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--spheres", type=int, default=4096)
-    ap.add_argument("--trace-config", default="", help="block,lds_stack,waves_per_cu[,sort_bits] (default: autotune)")
+    ap.add_argument("--trace-config", default="",
+                    help="block,lds_stack,waves_per_cu[,sort_bits[,refill]] (default: autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work for the baseline sample")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay measurement")
@@ -485,8 +486,8 @@ def main():
     hp = d_hits.data_ptr()
 
     if args.trace_config:
-        b, l, w, so = (int(x) for x in (args.trace_config.split(",") + ["0"])[:4])
-        sc.set_trace_config(b, l, w, so)
+        b, l, w, so, rf = (int(x) for x in (args.trace_config.split(",") + ["0", "0"])[:5])
+        sc.set_trace_config(b, l, w, so, rf)
         tuned_ms = None
     else:
         # tuned on one bounce's launch (the size every step issues), sort time included

@@ -87,6 +87,11 @@ typedef struct vgpu_mqbvh_node {
 #define VGPU_TRACE_DEVICE_PTRS 2u   /* rays/hits/geoms/counters are device pointers; the call only
                                        enqueues work on `stream` and returns without synchronising */
 
+#define VGPU_GRAPH 32u                /* vgpu_render_pass / vgpu_pt_pass with VGPU_TRACE_DEVICE_PTRS: the
+                                       pass's launches are captured into a HIP graph and replayed as one
+                                       (the graph is updated in place from pass to pass; a pass whose
+                                       launch sequence differs re-instantiates it).  Not with timing on. */
+
 /* vgpu_scene_add_mesh flags */
 #define VGPU_MESH_MOTION_REFERENCE_INDEXING 1u
     /* motion meshes only: test the triangle idxp[leaf slot] as the reference does
@@ -370,6 +375,8 @@ typedef struct vgpu_trace_config {
     int32_t sort;         /* 0: trace in the given order; k in 1..6: first reorder the rays by bucket =
                            * (Morton cell of the origin in a 2^k-per-axis grid over the scene box) x
                            * (direction octant); results still land in the callers' order */
+    int32_t refill;       /* 1..64: a wavefront's finished lanes wait until this many are idle, then take
+                           * new rays together (ray set-up runs once for them); 0: the default, 48 */
 } vgpu_trace_config;
 
 int vgpu_scene_set_trace_config(vgpu_scene* scene, const vgpu_trace_config* cfg);
@@ -377,8 +384,10 @@ int vgpu_scene_get_trace_config(const vgpu_scene* scene, vgpu_trace_config* cfg)
 
 /* Runs the autotuner: times each candidate configuration (block x LDS depth x
  * waves x sort in {0, 4, 5, 6}, the sort included in the time) on the given rays
- * (device pointers, n rays) and keeps the fastest.  best_out (nullable)
- * receives the chosen configuration; ms_out (nullable) its time. */
+ * (device pointers, n rays), then the refill threshold (16, 32, 40, 48, 56, 64)
+ * in the fastest of those, and keeps the fastest.  The results of a trace never
+ * depend on its configuration.  best_out (nullable) receives the chosen
+ * configuration; ms_out (nullable) its time. */
 int vgpu_scene_autotune(vgpu_scene* scene, uint32_t flags, const vgpu_ray* d_rays, uint64_t n,
                         vgpu_hit* d_hits, void* stream, vgpu_trace_config* best_out, float* ms_out);
 

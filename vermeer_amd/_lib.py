@@ -27,6 +27,7 @@ VGPU_GUIDE_RAW = 4
 VGPU_GUIDE_NO_UPDATE = 8
 VGPU_GUIDE_NPARAMS = 24360
 VGPU_PT_NO_TRAIN = 16
+VGPU_GRAPH = 32
 VGPU_LIGHT_TRI = 0
 VGPU_LIGHT_DISK = 1
 VGPU_LIGHT_SPHERE = 2
@@ -45,7 +46,8 @@ assert NODE_DTYPE.itemsize == 128 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.ite
 
 
 class TraceConfig(C.Structure):
-    _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("waves_per_cu", C.c_int32), ("sort", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("waves_per_cu", C.c_int32), ("sort", C.c_int32),
+                ("refill", C.c_int32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

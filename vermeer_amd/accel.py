@@ -193,8 +193,8 @@ class Scene:
     def status(self, stream=0):
         check(self._lib.vgpu_scene_status(self._h, C.c_void_p(stream or None)))
 
-    def set_trace_config(self, block=256, lds_stack=16, waves_per_cu=16, sort=0):
-        cfg = _lib.TraceConfig(block, lds_stack, waves_per_cu, sort)
+    def set_trace_config(self, block=256, lds_stack=16, waves_per_cu=16, sort=0, refill=0):
+        cfg = _lib.TraceConfig(block, lds_stack, waves_per_cu, sort, refill)
         check(self._lib.vgpu_scene_set_trace_config(self._h, C.byref(cfg)))
 
     def set_leaf_size(self, leaf_max):

@@ -26,6 +26,7 @@ struct vgpu_pt {
     vgpu::PTLight* d_lights = nullptr;
     double* d_stage = nullptr;
     std::vector<void*> owned;
+    vgpu_detail::GraphCache graph;   // VGPU_GRAPH passes
 
     ~vgpu_pt() {
         int prev = 0;
@@ -241,57 +242,67 @@ int vgpu_pt_pass(vgpu_pt* pt, uint32_t flags, uint32_t pass, double* accum, void
         HIP_TRY(hipMemcpyAsync(dacc, accum, np * 3 * sizeof(double), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemsetAsync(s->d_err, 0, sizeof(int32_t), st));
     }
-    vgpu::PTView v = pt->v;
-    vgpu::PTBuffers& b = pt->b;
-    const bool guided = pt->guide != nullptr && v.alpha > 0.0f;
-    const bool train = guided && pt->train_steps > 0 && v.rec_stride > 0 && (flags & VGPU_PT_NO_TRAIN) == 0;
-    if (!train) v.rec_stride = 0;
-    vgpu::PTBuffers bb = b;
-    if (!train) bb.plen = nullptr;
-    const char* stage = "camera";
-    hipError_t e = vgpu::pt_launch_camera(v, pass, bb, st);
-    for (int d = 0; d < v.max_depth && e == hipSuccess; d++) {
-        const int cur = d & 1;
-        stage = "closest hit";
-        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-        if (s->timing) {   // the same [sort start, trace start, trace end] records as enqueue_trace
-            for (auto& x : ev) x = timing_event(s);
-            if (!ev[0] || !ev[1] || !ev[2]) return fail(VGPU_ERR_HIP, "timing events");
-            HIP_TRY(hipEventRecord(ev[0], st));
-            HIP_TRY(hipEventRecord(ev[1], st));
+    const bool graph = (flags & VGPU_GRAPH) != 0;
+    if (graph && host) return fail(VGPU_ERR_ARG, "VGPU_GRAPH needs VGPU_TRACE_DEVICE_PTRS");
+    if (graph && s->timing) return fail(VGPU_ERR_STATE, "VGPU_GRAPH with kernel timing enabled");
+    if (graph && s->cfg.sort > 0) { const int rc = ensure_sort(s, np); if (rc) return rc; }
+    // The pass's launches, on the caller's stream or on the capture stream of the pass's graph.
+    auto enqueue = [&](hipStream_t st) -> int {
+        vgpu::PTView v = pt->v;
+        vgpu::PTBuffers& b = pt->b;
+        const bool guided = pt->guide != nullptr && v.alpha > 0.0f;
+        const bool train = guided && pt->train_steps > 0 && v.rec_stride > 0 && (flags & VGPU_PT_NO_TRAIN) == 0;
+        if (!train) v.rec_stride = 0;
+        vgpu::PTBuffers bb = b;
+        if (!train) bb.plen = nullptr;
+        const char* stage = "camera";
+        hipError_t e = vgpu::pt_launch_camera(v, pass, bb, st);
+        for (int d = 0; d < v.max_depth && e == hipSuccess; d++) {
+            const int cur = d & 1;
+            stage = "closest hit";
+            hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+            if (s->timing) {   // the same [sort start, trace start, trace end] records as enqueue_trace
+                for (auto& x : ev) x = timing_event(s);
+                if (!ev[0] || !ev[1] || !ev[2]) return fail(VGPU_ERR_HIP, "timing events");
+                HIP_TRY(hipEventRecord(ev[0], st));
+                HIP_TRY(hipEventRecord(ev[1], st));
+            }
+            e = vgpu::launch_trace_ext(s->view, false, bb.qray[cur], np, bb.hits, s->d_spT, s->d_spN, s->spill_threads,
+                                       s->d_err, s->num_cus, st, bb.count + cur);
+            if (ev[2] && e == hipSuccess) HIP_TRY(hipEventRecord(ev[2], st));
+            if (e == hipSuccess) { stage = "hit"; e = vgpu::pt_launch_hit(v, pass, cur, bb, st); }
+            if (e != hipSuccess || d + 1 >= v.max_depth) continue;
+            if (guided) {
+                stage = "guide inference";
+                e = vgpu::guide::launch_infer(pt->guide->net(), bb.query, np, bb.count + cur, bb.lobes, false,
+                                              pt->guide->num_cus, st);
+            }
+            if (e == hipSuccess) { stage = "scatter"; e = vgpu::pt_launch_scatter(v, pass, cur, bb, st); }
+            if (e == hipSuccess) {
+                const int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, bb.sray, np, bb.shits, nullptr,
+                                             nullptr, st, s->cfg, bb.count + 2);
+                if (rc) return rc;
+                stage = "shadow";
+                e = vgpu::pt_launch_shadow(v, bb, st);
+            }
         }
-        e = vgpu::launch_trace_ext(s->view, false, bb.qray[cur], np, bb.hits, s->d_spT, s->d_spN, s->spill_threads,
-                                   s->d_err, s->num_cus, st, bb.count + cur);
-        if (ev[2] && e == hipSuccess) HIP_TRY(hipEventRecord(ev[2], st));
-        if (e == hipSuccess) { stage = "hit"; e = vgpu::pt_launch_hit(v, pass, cur, bb, st); }
-        if (e != hipSuccess || d + 1 >= v.max_depth) continue;
-        if (guided) {
-            stage = "guide inference";
-            e = vgpu::guide::launch_infer(pt->guide->net(), bb.query, np, bb.count + cur, bb.lobes, false,
-                                          pt->guide->num_cus, st);
+        if (e == hipSuccess) { stage = "accumulate"; e = vgpu::pt_launch_accumulate(v, bb, dacc, st); }
+        if (e == hipSuccess && train) {
+            stage = "gather";
+            e = vgpu::pt_launch_gather(v, pass, bb, st);
+            vgpu_guide* g = pt->guide;
+            for (uint32_t k = 0; k < pt->train_steps && e == hipSuccess; k++) {
+                // each step draws its batch from the pass's training samples inside the training kernels
+                stage = "training step";
+                const vgpu::guide::SampleSource src{bb.tsamples, bb.count + 3, bb.tcap, pass * 7919u + k * 104729u};
+                e = vgpu::guide::launch_train(g->net(), g->d_params, g->d_m, g->d_v, src, pt->batch, pt->lr, true, g->tb, st);
+            }
         }
-        if (e == hipSuccess) { stage = "scatter"; e = vgpu::pt_launch_scatter(v, pass, cur, bb, st); }
-        if (e == hipSuccess) {
-            const int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, bb.sray, np, bb.shits, nullptr,
-                                         nullptr, st, s->cfg, bb.count + 2);
-            if (rc) return rc;
-            stage = "shadow";
-            e = vgpu::pt_launch_shadow(v, bb, st);
-        }
-    }
-    if (e == hipSuccess) { stage = "accumulate"; e = vgpu::pt_launch_accumulate(v, bb, dacc, st); }
-    if (e == hipSuccess && train) {
-        stage = "gather";
-        e = vgpu::pt_launch_gather(v, pass, bb, st);
-        vgpu_guide* g = pt->guide;
-        for (uint32_t k = 0; k < pt->train_steps && e == hipSuccess; k++) {
-            // each step draws its batch from the pass's training samples inside the training kernels
-            stage = "training step";
-            const vgpu::guide::SampleSource src{bb.tsamples, bb.count + 3, bb.tcap, pass * 7919u + k * 104729u};
-            e = vgpu::guide::launch_train(g->net(), g->d_params, g->d_m, g->d_v, src, pt->batch, pt->lr, true, g->tb, st);
-        }
-    }
-    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "path pass (%s): %s", stage, hipGetErrorString(e));
+        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "path pass (%s): %s", stage, hipGetErrorString(e));
+        return VGPU_OK;
+    };
+    const int rc = graph ? pt->graph.run(st, enqueue) : enqueue(st);
+    if (rc) return rc;
     if (host) {
         HIP_TRY(hipMemcpyAsync(accum, dacc, np * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
         int32_t err = 0;

@@ -296,6 +296,7 @@ struct vgpu_render {
     float4* d_pixshade = nullptr;
     void* d_stage = nullptr;       // host-pointer mode: scrambles, film, samples
     size_t stage_bytes = 0;
+    vgpu_detail::GraphCache graph; // VGPU_GRAPH passes
 
     ~vgpu_render() {
         int prev = 0;
@@ -449,37 +450,47 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
         HIP_TRY(hipMemcpyAsync(d_film, film, np * 12, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemsetAsync(s->d_err, 0, sizeof(int32_t), st));
     }
-    hipError_t e = vgpu::launch_camera(r->rv, d_scr, iter, r->d_rays, r->d_pix, st);
-    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "camera: %s", hipGetErrorString(e));
+    const bool graph = (flags & VGPU_GRAPH) != 0;
+    if (graph && host) return fail(VGPU_ERR_ARG, "VGPU_GRAPH needs VGPU_TRACE_DEVICE_PTRS");
+    if (graph && s->timing) return fail(VGPU_ERR_STATE, "VGPU_GRAPH with kernel timing enabled");
+    if (graph && s->cfg.sort > 0) { const int rc = ensure_sort(s, ns); if (rc) return rc; }
     uint64_t shadow_count = 0;
-    for (int level = 0; level <= r->rv.max_level; level++) {
-        float4* rin = (level & 1) ? r->d_rays2 : r->d_rays;
-        float4* rout = level < r->rv.max_level ? ((level & 1) ? r->d_rays : r->d_rays2) : nullptr;
-        e = vgpu::launch_trace_ext(s->view, s->any_motion, rin, np, r->d_hits32, s->d_spT, s->d_spN, s->spill_threads, s->d_err,
-                                   s->num_cus, st);
-        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "closest hit: %s", hipGetErrorString(e));
-        e = vgpu::launch_shade(r->rv, d_scr, iter, level, rin, r->d_hits32, r->d_pix, r->d_slot_rays, r->d_slot_rgb,
-                               r->d_groups, r->d_pixshade, rout, r->d_recs, st);
-        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "shade: %s", hipGetErrorString(e));
-        int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns, r->d_slot_hits, nullptr,
-                               nullptr, st, s->cfg);
-        if (rc) return rc;
-        const bool direct = r->rv.max_level == 0;
-        e = vgpu::launch_resolve(r->rv, level, iter, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, r->d_recs,
-                                 direct ? d_film : nullptr, direct ? d_samp : nullptr, st);
-        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "resolve: %s", hipGetErrorString(e));
-        if (host && shadow_rays_out) {
-            // count active slots (diagnostic, host mode only)
-            std::vector<float4> rgb(ns);
-            HIP_TRY(hipMemcpyAsync(rgb.data(), r->d_slot_rgb, ns * 16, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            for (const float4& v : rgb) shadow_count += v.w != 0.0f;
+    // The pass's launches, on the caller's stream or on the capture stream of the pass's graph.
+    auto enqueue = [&](hipStream_t st) -> int {
+        hipError_t e = vgpu::launch_camera(r->rv, d_scr, iter, r->d_rays, r->d_pix, st);
+        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "camera: %s", hipGetErrorString(e));
+        for (int level = 0; level <= r->rv.max_level; level++) {
+            float4* rin = (level & 1) ? r->d_rays2 : r->d_rays;
+            float4* rout = level < r->rv.max_level ? ((level & 1) ? r->d_rays : r->d_rays2) : nullptr;
+            e = vgpu::launch_trace_ext(s->view, s->any_motion, rin, np, r->d_hits32, s->d_spT, s->d_spN, s->spill_threads, s->d_err,
+                                       s->num_cus, st);
+            if (e != hipSuccess) return fail(VGPU_ERR_HIP, "closest hit: %s", hipGetErrorString(e));
+            e = vgpu::launch_shade(r->rv, d_scr, iter, level, rin, r->d_hits32, r->d_pix, r->d_slot_rays, r->d_slot_rgb,
+                                   r->d_groups, r->d_pixshade, rout, r->d_recs, st);
+            if (e != hipSuccess) return fail(VGPU_ERR_HIP, "shade: %s", hipGetErrorString(e));
+            int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns, r->d_slot_hits, nullptr,
+                                   nullptr, st, s->cfg);
+            if (rc) return rc;
+            const bool direct = r->rv.max_level == 0;
+            e = vgpu::launch_resolve(r->rv, level, iter, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, r->d_recs,
+                                     direct ? d_film : nullptr, direct ? d_samp : nullptr, st);
+            if (e != hipSuccess) return fail(VGPU_ERR_HIP, "resolve: %s", hipGetErrorString(e));
+            if (host && shadow_rays_out) {
+                // count active slots (diagnostic, host mode only)
+                std::vector<float4> rgb(ns);
+                HIP_TRY(hipMemcpyAsync(rgb.data(), r->d_slot_rgb, ns * 16, hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                for (const float4& v : rgb) shadow_count += v.w != 0.0f;
+            }
         }
-    }
-    if (r->rv.max_level > 0) {
-        e = vgpu::launch_combine(r->rv, iter, r->d_recs, d_film, d_samp, st);
-        if (e != hipSuccess) return fail(VGPU_ERR_HIP, "combine: %s", hipGetErrorString(e));
-    }
+        if (r->rv.max_level > 0) {
+            e = vgpu::launch_combine(r->rv, iter, r->d_recs, d_film, d_samp, st);
+            if (e != hipSuccess) return fail(VGPU_ERR_HIP, "combine: %s", hipGetErrorString(e));
+        }
+        return VGPU_OK;
+    };
+    const int rc = graph ? r->graph.run(st, enqueue) : enqueue(st);
+    if (rc) return rc;
     if (host) {
         HIP_TRY(hipMemcpyAsync(film, d_film, np * 12, hipMemcpyDeviceToHost, st));
         if (sample_out) HIP_TRY(hipMemcpyAsync(sample_out, d_samp, np * 12, hipMemcpyDeviceToHost, st));

@@ -188,6 +188,7 @@ int check_config(const vgpu_trace_config& c) {
     if (c.block * c.lds_stack * 8 > 65536) return fail(VGPU_ERR_ARG, "block*lds_stack exceeds 64 KiB of LDS");
     if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return fail(VGPU_ERR_ARG, "waves_per_cu must be in [1,32]");
     if (c.sort < 0 || c.sort > 6) return fail(VGPU_ERR_ARG, "sort must be 0 (off) or 1..6 grid bits per axis");
+    if (c.refill < 0 || c.refill > 64) return fail(VGPU_ERR_ARG, "refill must be 0 (default) or 1..64 idle lanes");
     return VGPU_OK;
 }
 
@@ -210,6 +211,40 @@ int ensure_sort(vgpu_scene* s, uint64_t n) {
     return VGPU_OK;
 }
 
+GraphCache::~GraphCache() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (cap) (void)hipStreamDestroy(cap);
+}
+
+int GraphCache::run(hipStream_t st, const std::function<int(hipStream_t)>& body) {
+    if (!cap) HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    const int rc = body(cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cap, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (ec != hipSuccess) return fail(VGPU_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+    hipError_t e = hipSuccess;
+    if (exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(exec, graph, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+    }
+    if (!exec) {
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) instantiations++;
+        else exec = nullptr;
+    }
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(VGPU_ERR_HIP, "graph instantiation: %s", hipGetErrorString(e));
+    HIP_TRY(hipGraphLaunch(exec, st));
+    return VGPU_OK;
+}
+
 // Next free timing event (the pool grows; vgpu_scene_timing recycles it).
 hipEvent_t timing_event(vgpu_scene* s) {
     if (s->ev_used == s->ev_pool.size()) {
@@ -222,7 +257,7 @@ hipEvent_t timing_event(vgpu_scene* s) {
 
 int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,
                   int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg, const uint32_t* n_dev) {
-    vgpu::TraceLaunch L{cfg.block, cfg.lds_stack, cfg.waves_per_cu};
+    vgpu::TraceLaunch L{cfg.block, cfg.lds_stack, cfg.waves_per_cu, cfg.refill > 0 ? cfg.refill : vgpu::kRefillDefault};
     const uint64_t threads = vgpu::trace_grid_threads(L, n, s->num_cus);
     int rc = ensure_spill(s, threads);
     if (rc) return rc;
@@ -500,7 +535,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
     // Default launch shape unless the caller chose one: static scenes run best at block 128 with a
     // 12-entry LDS stack and 24 waves per CU; the motion kernel (about 120 VGPRs, four waves per
     // SIMD) at block 256, 16 entries, 16 waves per CU.
-    if (!s->cfg_set) s->cfg = s->any_motion ? vgpu_trace_config{256, 16, 16, 0} : vgpu_trace_config{128, 12, 24, 0};
+    if (!s->cfg_set) s->cfg = s->any_motion ? vgpu_trace_config{256, 16, 16, 0, 0} : vgpu_trace_config{128, 12, 24, 0, 0};
 
     // Flatten into the device layout.
     std::vector<vgpu::DevNode> dnodes;
@@ -713,7 +748,8 @@ int vgpu_trace_noalloc(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, ui
     DeviceGuard dg(s->device);
     // The scene's launch shape with fewer resident waves where its spill columns would not fit the
     // scratch array; never sorted (the sort's buffers grow with n).
-    vgpu::TraceLaunch L{s->cfg.block, s->cfg.lds_stack, s->cfg.waves_per_cu};
+    vgpu::TraceLaunch L{s->cfg.block, s->cfg.lds_stack, s->cfg.waves_per_cu,
+                        s->cfg.refill > 0 ? s->cfg.refill : vgpu::kRefillDefault};
     const uint64_t cap = vgpu::noalloc_max_threads(L.lds_depth);
     while (L.waves_per_cu > 1 && vgpu::trace_grid_threads(L, n, s->num_cus) > cap) L.waves_per_cu--;
     if (vgpu::trace_grid_threads(L, n, s->num_cus) > cap)
@@ -801,37 +837,49 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
     std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
     hipStream_t st = (hipStream_t)stream;
-    const int blocks[] = {64, 128, 256};
-    const int ldss[] = {8, 12, 16, 24, 32};
-    const int waves[] = {8, 16, 32};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
     vgpu_trace_config bc = s->cfg;
-    const int sorts[] = {0, 4, 5, 6};
+    // one warm launch, then the mean of two timed ones; returns a status code
+    auto time_config = [&](const vgpu_trace_config& c) -> int {
+        int rc = enqueue_trace(s, flags | VGPU_TRACE_DEVICE_PTRS, d_rays, n, d_hits, nullptr, nullptr, st, c);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int rep = 0; rep < 2 && !rc; rep++)
+            rc = enqueue_trace(s, flags | VGPU_TRACE_DEVICE_PTRS, d_rays, n, d_hits, nullptr, nullptr, st, c);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        ms *= 0.5f;
+        if (ms < best) { best = ms; bc = c; }
+        return VGPU_OK;
+    };
+    int rc = VGPU_OK;
+    const int sorts[] = {0, 4, 5, 6}, blocks[] = {64, 128, 256}, ldss[] = {8, 12, 16, 24, 32}, waves[] = {8, 16, 32};
     for (int so : sorts)
-    for (int b : blocks)
-        for (int l : ldss)
-            for (int w : waves) {
-                vgpu_trace_config c{b, l, w, so};
-                if (check_config(c)) continue;
-                int rc = enqueue_trace(s, flags | VGPU_TRACE_DEVICE_PTRS, d_rays, n, d_hits, nullptr, nullptr, st, c);
-                if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
-                HIP_TRY(hipEventRecord(e0, st));
-                for (int rep = 0; rep < 2; rep++) {
-                    rc = enqueue_trace(s, flags | VGPU_TRACE_DEVICE_PTRS, d_rays, n, d_hits, nullptr, nullptr, st, c);
-                    if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+        for (int b : blocks)
+            for (int l : ldss)
+                for (int w : waves) {
+                    const vgpu_trace_config c{b, l, w, so, 0};
+                    if (rc || check_config(c)) continue;
+                    rc = time_config(c);
                 }
-                HIP_TRY(hipEventRecord(e1, st));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                ms *= 0.5f;
-                if (ms < best) { best = ms; bc = c; }
-            }
+    // second stage: the refill threshold in the fastest shape
+    const int refills[] = {16, 32, 40, 48, 56, 64};
+    const vgpu_trace_config shape = bc;
+    for (int r : refills) {
+        vgpu_trace_config c = shape;
+        c.refill = r;
+        if (!rc) rc = time_config(c);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (bc.refill == 0) bc.refill = vgpu::kRefillDefault;   // report the threshold the shape ran with
     vgpu::clear_error();
     s->cfg = bc;
     s->cfg_set = true;

@@ -121,7 +121,7 @@ struct vgpu_scene {
     int32_t* d_shaders = nullptr;
     vgpu::DevGeomShade* d_gshade = nullptr;
     int32_t max_material = -1;
-    vgpu_trace_config cfg{256, 16, 16, 0};
+    vgpu_trace_config cfg{256, 16, 16, 0, 0};
     bool cfg_set = false;               // the caller chose cfg (set_trace_config or autotune)
     int32_t leaf_max = 16;              // leafMax of the static meshes' BuildAccel (buildqbvh.go:32: 16)
     float box[6] = {0, 0, 0, 0, 0, 0};   // scene bounds over all keys (ray-sort grid)
@@ -160,6 +160,24 @@ hipEvent_t timing_event(vgpu_scene* s);
 
 // Ensures the spill area covers `threads` traversal threads.
 int ensure_spill(vgpu_scene* s, uint64_t threads);
+// Ensures the ray-sort buffers hold n rays.
+int ensure_sort(vgpu_scene* s, uint64_t n);
+
+// A sequence of launches replayed as one HIP graph.  run() captures `body` on a private stream
+// (thread-local capture mode: body may only enqueue work on the stream it is given, no allocation,
+// no synchronisation, no hipMemsetAsync), then updates the executable graph in place when only
+// kernel arguments changed since the last capture (re-instantiating it when the launch sequence
+// differs), and launches it on `st`.
+struct GraphCache {
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int instantiations = 0;
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache();
+    int run(hipStream_t st, const std::function<int(hipStream_t)>& body);
+};
 // Enqueues one traversal (device pointers) with configuration cfg.  n_dev (nullable): device-resident
 // count of rays to trace, at most n.
 int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,

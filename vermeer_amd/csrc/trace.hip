@@ -666,16 +666,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
 
 #undef VGPU_IN_MESH
 
-// Idle lanes that trigger a refill of the wavefront (k_trace_wide); 48 of 64 measured fastest on
-// incoherent bounce rays (64: refill only when the whole wavefront is idle).
-constexpr int kRefillMin = 48;
-int refill_min_lanes() { return kRefillMin; }
 
 template <int BLOCK, int LDS, bool SHADOW, bool MOTION>
 hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
                       float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev,
-                      const uint32_t* perm, hipStream_t stream) {
-    const int refill_min = refill_min_lanes();
+                      const uint32_t* perm, hipStream_t stream, int refill_min) {
     if (counters)
         hipLaunchKernelGGL((k_trace_wide<BLOCK, LDS, SHADOW, MOTION, true, false>), dim3((unsigned)grid), dim3(BLOCK), 0,
                            stream, sv, (const float4*)rays, n, (int4*)hits, geoms, (int4*)counters, spT, spN, err,
@@ -690,15 +685,15 @@ hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* h
 template <int BLOCK, int LDS>
 hipError_t launch_mode(const SceneView& sv, bool shadow, bool motion, const void* rays, uint64_t n, void* hits,
                        int32_t* geoms, int32_t* counters, float* spT, int32_t* spN, uint64_t grid, int32_t* err,
-                       const uint32_t* n_dev, const uint32_t* perm, hipStream_t stream) {
+                       const uint32_t* n_dev, const uint32_t* perm, hipStream_t stream, int refill_min) {
     if (shadow) {
         if (motion)
-            return launch_cfg<BLOCK, LDS, true, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
-        return launch_cfg<BLOCK, LDS, true, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
+            return launch_cfg<BLOCK, LDS, true, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream, refill_min);
+        return launch_cfg<BLOCK, LDS, true, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream, refill_min);
     }
     if (motion)
-        return launch_cfg<BLOCK, LDS, false, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
-    return launch_cfg<BLOCK, LDS, false, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream);
+        return launch_cfg<BLOCK, LDS, false, true>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream, refill_min);
+    return launch_cfg<BLOCK, LDS, false, false>(sv, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, stream, refill_min);
 }
 
 }  // namespace
@@ -718,11 +713,11 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
     if (motion)
         hipLaunchKernelGGL((k_trace_wide<kBM, kLM, false, true, false, true>), dim3(grid), dim3(kBM), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
-                           (const uint32_t*)nullptr, refill_min_lanes());
+                           (const uint32_t*)nullptr, kRefillDefault);
     else
         hipLaunchKernelGGL((k_trace_wide<kB, kL, false, false, false, true>), dim3(grid), dim3(kB), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
-                           (const uint32_t*)nullptr, refill_min_lanes());
+                           (const uint32_t*)nullptr, kRefillDefault);
     return hipGetLastError();
 }
 
@@ -746,7 +741,7 @@ hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const Tra
 #define VGPU_CASE(B, L)                                                                                         \
     if (cfg.block == B && cfg.lds_depth == L)                                                                   \
         return launch_mode<B, L>(sv, shadow, motion, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, \
-                                 stream);
+                                 stream, cfg.refill_min);
     VGPU_CASE(64, 8) VGPU_CASE(64, 12) VGPU_CASE(64, 16) VGPU_CASE(64, 24) VGPU_CASE(64, 32)
     VGPU_CASE(128, 8) VGPU_CASE(128, 12) VGPU_CASE(128, 16) VGPU_CASE(128, 24) VGPU_CASE(128, 32)
     VGPU_CASE(256, 8) VGPU_CASE(256, 12) VGPU_CASE(256, 16) VGPU_CASE(256, 24) VGPU_CASE(256, 32)

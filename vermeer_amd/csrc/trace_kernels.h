@@ -7,6 +7,10 @@
 
 namespace vgpu {
 
+// Idle lanes that trigger a refill of the wavefront in the traversal kernel; 48 of 64 measured
+// fastest on incoherent bounce rays (64: refill only when the whole wavefront is idle).
+constexpr int kRefillDefault = 48;
+
 // Device node, 128 bytes = one cache line.  Same content as qbvh.Node
 // (qbvh/qbvh.go:31-36) re-laid for aligned 16-byte loads: the six box rows
 // (min x/y/z, max x/y/z; lane c = child c) are the reference's Boxes[24]
@@ -55,6 +59,7 @@ struct TraceLaunch {
     int block;          // 64, 128, 256
     int lds_depth;      // 8, 12, 16, 24, 32
     int waves_per_cu;   // resident waves per CU used to size the grid
+    int refill_min = kRefillDefault;   // idle lanes that make a wavefront take new rays together (1..64)
 };
 
 // Launch shapes of launch_trace_ext (the shading passes' closest-hit traversal) for static and

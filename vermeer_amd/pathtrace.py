@@ -85,8 +85,10 @@ class PathTracer:
         check(self._lib.vgpu_pt_pass(self._h, flags, int(pass_index), ptr(self.accum), None))
         self.passes += 1
 
-    def render_pass_device(self, d_accum, pass_index, stream=None, train=True):
-        flags = _lib.VGPU_TRACE_DEVICE_PTRS | (0 if train else _lib.VGPU_PT_NO_TRAIN)
+    def render_pass_device(self, d_accum, pass_index, stream=None, train=True, graph=False):
+        """One pass on a device accumulation buffer; graph=True replays the pass as a HIP graph."""
+        flags = (_lib.VGPU_TRACE_DEVICE_PTRS | (0 if train else _lib.VGPU_PT_NO_TRAIN) |
+                 (_lib.VGPU_GRAPH if graph else 0))
         check(self._lib.vgpu_pt_pass(self._h, flags, int(pass_index), C.c_void_p(d_accum), C.c_void_p(stream)))
 
     def stats(self):

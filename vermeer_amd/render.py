@@ -148,8 +148,10 @@ class Renderer:
         check(self._lib.vgpu_render_pass(self._h, 0, ptr(scr), iteration, ptr(self.film), ptr(samp), None, C.byref(n)))
         return n.value, samp
 
-    def render_pass_device(self, d_scrambles, iteration, d_film, d_sample=0, stream=0):
-        """One pass on device buffers (ints: data pointers), enqueued on `stream`."""
-        check(self._lib.vgpu_render_pass(self._h, _lib.VGPU_TRACE_DEVICE_PTRS, C.c_void_p(d_scrambles), iteration,
+    def render_pass_device(self, d_scrambles, iteration, d_film, d_sample=0, stream=0, graph=False):
+        """One pass on device buffers (ints: data pointers), enqueued on `stream`; graph=True replays the
+        pass as a HIP graph."""
+        flags = _lib.VGPU_TRACE_DEVICE_PTRS | (_lib.VGPU_GRAPH if graph else 0)
+        check(self._lib.vgpu_render_pass(self._h, flags, C.c_void_p(d_scrambles), iteration,
                                          C.c_void_p(d_film), C.c_void_p(d_sample or None), C.c_void_p(stream or None),
                                          None))
