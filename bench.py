@@ -73,6 +73,8 @@ def parse():
                     help="guide sampling probability from --pt-late-pass on")
     ap.add_argument("--pt-warm", type=int, default=0,
                     help="guided passes left out of the image (their time still counts)")
+    ap.add_argument("--pt-eager", action="store_true",
+                    help="path-mode passes as eager launches (default: each pass replayed as a HIP graph)")
     ap.add_argument("--pt-seeds", default="2,3,4",
                     help="path seeds of the guided runs (the schedule was tuned on another seed); the time "
                          "ratio is their mean")
@@ -345,7 +347,7 @@ def guided_measurement(args, torch, np):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for p in range(first, first + count):
-            pt.render_pass_device(acc.data_ptr(), p, stream=st, train=train)
+            pt.render_pass_device(acc.data_ptr(), p, stream=st, train=train, graph=not args.pt_eager)
         e1.record(stream)
         e1.synchronize()
         return e0.elapsed_time(e1)
