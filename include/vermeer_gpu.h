@@ -477,6 +477,45 @@ int vgpu_pt_pass(vgpu_pt* pt, uint32_t flags, uint32_t pass, double* accum, void
 int vgpu_pt_stats(vgpu_pt* pt, uint64_t* rays, uint32_t* train_samples);
 
 /* ------------------------------------------------------------------------
+ * Scene files.  A .vnf file (docs/quickstart.rst) is parsed as nodes.Parse
+ * does (nodes/lex.go, nodes/parser.go), quirks included, and built as
+ * core.PreRender builds it (core/core.go:38-61; PolyMesh.init's fan
+ * triangulation of PolyCount polygons, builtin/geom/polymesh/init.go:12-134;
+ * the light nodes' geometry; ShaderStd; the LookAt or matrix camera; the
+ * pixel filter) into a scene and a renderer.  Node types whose rendering this
+ * library does not provide (DebugShader, QuadLight, Proc, GeomInstance,
+ * rgbtex textures, non-identity mesh transforms) fail the build.
+ * ------------------------------------------------------------------------ */
+typedef struct vgpu_vnf vgpu_vnf;
+
+#define VGPU_VNF_MESSAGES 0   /* vgpu_vnf_text: the parse errors the reference prints */
+#define VGPU_VNF_NODES 1      /* vgpu_vnf_text: every parsed node's present parameters (floats as bits) */
+#define VGPU_OUTPUT_HDR 0     /* an OutputHDR node */
+#define VGPU_OUTPUT_FLOAT 1   /* an OutputFloat node */
+
+/* Replaces nodes.Parse (nodes/parser.go:108-130) plus core.AddNode: the file's
+ * nodes, and the frame size of its last Globals node (1024 x 1024 without
+ * one).  Parse errors are reported as the reference prints them (see
+ * vgpu_vnf_text) and parsing goes on, as it does; more than 10 errors fail
+ * the load (the reference exits). */
+int vgpu_vnf_load(const char* path, vgpu_vnf** out);
+int vgpu_vnf_destroy(vgpu_vnf* vnf);
+/* Frame size, Globals MaxIter and the number of parse errors. */
+int vgpu_vnf_info(const vgpu_vnf* vnf, uint32_t* width, uint32_t* height, int32_t* max_iter, int32_t* nerrors);
+/* Text of VGPU_VNF_MESSAGES or VGPU_VNF_NODES into buf (len bytes, NUL
+ * terminated, truncated); *needed (nullable) the full size. */
+int vgpu_vnf_text(const vgpu_vnf* vnf, int32_t what, char* buf, uint64_t len, uint64_t* needed);
+/* Replaces core.PreRender and the setup of core.Render (core/render.go:140-176):
+ * runs the nodes' PreRender in the reference's rounds (Include nodes parse
+ * their file, lights add their geometry), creates the scene on `device` with
+ * every geom in core.AddNode order, commits it and creates the renderer (its
+ * camera from Globals Camera or "camera").  Once per loaded file. */
+int vgpu_vnf_build(vgpu_vnf* vnf, int device, vgpu_scene** scene_out, vgpu_render** render_out);
+/* After build: *count (nullable) the output nodes; for i >= 0 the i-th one's
+ * type (VGPU_OUTPUT_*) and file name (len bytes, NUL terminated). */
+int vgpu_vnf_output(const vgpu_vnf* vnf, int32_t i, int32_t* type, char* path, uint64_t len, int32_t* count);
+
+/* ------------------------------------------------------------------------
  * Output drivers.  The film is width*height*3 floats in pixel order (row 0
  * first), as core.FrameBuf() holds it; with VGPU_TRACE_DEVICE_PTRS it is
  * device memory on `device` (work on `stream`), otherwise host memory.

@@ -226,6 +226,8 @@ def _render_sigs(lib):
         "orc_render_add_light": (None, [P, C.c_int32, P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32]),
         "orc_render_set_filter": (None, [P, C.c_int32, C.c_float, C.c_int32, C.c_float]),
         "orc_render_set_lens": (None, [P, C.c_float]),
+        "orc_scene_mesh_set_normals": (None, [P, C.c_int32, P, C.c_int32, P]),
+        "orc_scene_mesh_set_ray_bias": (None, [P, C.c_int32, C.c_float]),
         "orc_uniform_disk": (None, [C.c_float, C.c_float, C.c_float, P]),
         "orc_render_warp_sample": (None, [P, C.c_double, C.c_double, P]),
         "orc_render_filter_tables": (C.c_int32, [P, P]),
@@ -330,7 +332,7 @@ class OracleRenderer:
     scene_desc: dict from vermeer_amd.scenes (meshes with shader tables, materials, tri lights, camera).
     """
 
-    def __init__(self, desc, width, height, M, tan_theta_focal, focal=12.0):
+    def __init__(self, desc, width, height, M, tan_theta_focal, focal=12.0, aspect=None):
         self.lib = load()
         _render_sigs(self.lib)
         self.scene = OracleScene()
@@ -354,14 +356,23 @@ class OracleRenderer:
             sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
             shaders = np.ascontiguousarray(mesh["shaders"], np.int32)
             self._keep += [v, ix, nrm, sidx, shaders]
-            self.lib.orc_scene_add_shaded_mesh(self.scene.h, _p(v), len(v), _p(ix), len(ix), _p(nrm), _p(sidx),
-                                               _p(shaders), len(shaders))
+            g = self.lib.orc_scene_add_shaded_mesh(self.scene.h, _p(v), len(v), _p(ix), len(ix), _p(nrm), _p(sidx),
+                                                   _p(shaders), len(shaders))
+            if mesh.get("normal_list") is not None:
+                # Normals with their own index per face corner (PolyMesh.normalidx)
+                nl = np.ascontiguousarray(mesh["normal_list"], np.float32).reshape(-1, 3)
+                ni = np.ascontiguousarray(mesh["normal_idx"], np.uint32)
+                self._keep += [nl, ni]
+                self.lib.orc_scene_mesh_set_normals(self.scene.h, g, _p(nl), len(nl), _p(ni))
+            if mesh.get("ray_bias"):
+                self.lib.orc_scene_mesh_set_ray_bias(self.scene.h, g, float(mesh["ray_bias"]))
             self.scene.meshes.append((v, ix))
         self.scene.commit()
         mats = np.ascontiguousarray(desc["materials"], MATERIAL_DTYPE)
         Mx = np.ascontiguousarray(M, np.float32)
+        asp = float(width) / float(height) if aspect is None else float(aspect)
         self.h_r = self.lib.orc_render_new(self.scene.h, width, height, _p(Mx), float(tan_theta_focal),
-                                           float(width) / float(height), float(focal), _p(mats), len(mats))
+                                           asp, float(focal), _p(mats), len(mats))
         types = {"tri": 0, "disk": 1, "sphere": 2}
         for L in desc["lights"]:
             kind = L.get("type", "tri")
@@ -385,9 +396,11 @@ class OracleRenderer:
         sidx = None if sidx is None else np.ascontiguousarray(sidx, np.uint8)
         shaders = np.ascontiguousarray(mesh["shaders"], np.int32)
         self._keep += [v, ix, nrm, sidx, shaders]
-        self.lib.orc_scene_add_shaded_motion_mesh(self.scene.h, _p(v), v.shape[1], nk, _p(ix), len(ix), _p(nrm),
-                                                  nrm_keys, _p(sidx), _p(shaders), len(shaders),
-                                                  1 if mesh.get("reference_indexing") else 0)
+        g = self.lib.orc_scene_add_shaded_motion_mesh(self.scene.h, _p(v), v.shape[1], nk, _p(ix), len(ix), _p(nrm),
+                                                      nrm_keys, _p(sidx), _p(shaders), len(shaders),
+                                                      1 if mesh.get("reference_indexing") else 0)
+        if mesh.get("ray_bias"):
+            self.lib.orc_scene_mesh_set_ray_bias(self.scene.h, g, float(mesh["ray_bias"]))
         self.scene.meshes.append((v, ix))
 
     def __del__(self):

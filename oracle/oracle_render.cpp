@@ -1634,3 +1634,15 @@ void orc_ggx_g1_n(const float* w, const float* m, int64_t n, float alpha, float*
     for (int64_t i = 0; i < n; i++) out[i] = ggxG1(ww, V3(m[i * 3], m[i * 3 + 1], m[i * 3 + 2]), alpha);
 }
 }
+
+extern "C" {
+// A mesh's normals indexed by its own NormalIdx (normalidx, builtin/geom/polymesh/init.go:47-57),
+// and its RayBias (polymesh.go:19); geom is the insertion index.
+void orc_scene_mesh_set_normals(void* sp, int32_t geom, const float* normals, int32_t nnormals, const uint32_t* nidx) {
+    Mesh* m = ((Scene*)sp)->geoms[geom];
+    m->normals.resize(nnormals);
+    std::memcpy(m->normals.data(), normals, sizeof(float) * 3 * (size_t)nnormals);
+    m->normalidx.assign(nidx, nidx + (size_t)m->facecount * 3);
+}
+void orc_scene_mesh_set_ray_bias(void* sp, int32_t geom, float bias) { ((Scene*)sp)->geoms[geom]->RayBias = bias; }
+}

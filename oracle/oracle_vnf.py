@@ -1,0 +1,812 @@
+"""Oracle (test infrastructure only): .vnf parsing and PreRender restated in Python.
+
+  nodes/lex.go          Lex: next (line reads, '#' comments), lex, token, num, str
+  nodes/parser.go       parse, node, parseParam and the typed readers (int32slice, rgb, floatmap,
+                        vec3, pointarray, vec3array, vec2array, float32array, matrixarray)
+  nodes/register.go     createNode; the node types' constructor defaults from their Register calls
+  core/core.go:38-89    PreRender rounds, AddNode (geoms, lights, filter, globals, node names)
+  builtin/geom/polymesh/init.go:12-134   fan triangulation of PolyCount polygons
+  builtin/light/{triangle,disk,sphere}.go PreRender: the geometry each light adds
+  builtin/camera/camera.go:80-207        Aspect, TanThetaFocal, LocalToWorld keys
+
+dump() writes the parsed nodes in the same text form as the library's vgpu_vnf_text(VGPU_VNF_NODES),
+so the two parsers can be compared; build() gives a scene description, camera and filter for
+OracleRenderer.
+"""
+import math
+import os
+import struct
+
+import numpy as np
+
+EOF, TOK, STR, FLT, INT, OBRACE, CBRACE, OCURLY, CCURLY, COMMA = range(10)
+
+KEYWORDS = ("int", "float", "vec2", "vec3", "point", "rgb", "rgbtex", "matrix")
+
+# Field name, kind, required: the exported, settable fields of each node struct in declaration order
+# (a `node:"-"` tag skips a field, `node:",opt"` makes it optional, `node:"Name"` renames it).
+SCHEMAS = {
+    "Globals": [("NodeDef", "struct", False), ("XRes", "int", False), ("YRes", "int", False),
+                ("UseProgress", "bool", False), ("MaxGoRoutines", "int", False), ("Camera", "string", False),
+                ("MaxIter", "int", False), ("Output", "string", False)],
+    "PolyMesh": [("Name", "string", True), ("RayBias", "float", False), ("Verts", "point", True),
+                 ("PolyCount", "ints", False), ("FaceIdx", "ints", False), ("Shader", "strings", True),
+                 ("ShaderIdx", "ints", False), ("CalcNormals", "bool", False), ("IsVisible", "bool", False),
+                 ("Transform", "matrix", False), ("UV", "vec2", False), ("UVIdx", "ints", False),
+                 ("Normals", "vec3", False), ("NormalIdx", "ints", False)],
+    "Camera": [("Name", "string", True), ("Type", "string", True), ("From", "point", True), ("To", "point", True),
+               ("Roll", "float32array", True), ("Up", "v3", True), ("Aspect", "float", False), ("Fov", "float", False),
+               ("Focal", "float", False), ("WorldToLocal", "matrix", False), ("LocalToWorld", "matrix", False),
+               ("L", "float", False), ("R", "float", False), ("T", "float", False), ("B", "float", False),
+               ("Radius", "float", False)],
+    "ShaderStd": [("Name", "string", True), ("EmissionColour", "map", False), ("EmissionStrength", "map", False),
+                  ("Sides", "int", False), ("DiffuseColour", "map", False), ("DiffuseStrength", "map", False),
+                  ("DiffuseRoughness", "map", False), ("Spec1Colour", "map", False), ("Spec1Strength", "map", False),
+                  ("Spec1Roughness", "map", False), ("Spec1FresnelModel", "string", False),
+                  ("Spec1FresnelRefl", "map", False), ("Spec1FresnelEdge", "map", False), ("IOR", "map", False)],
+    "DebugShader": [("Name", "string", True), ("Sides", "int", False), ("Colour", "map", True)],
+    "TriLight": [("Name", "string", True), ("P0", "v3", True), ("P1", "v3", True), ("P2", "v3", True),
+                 ("Shader", "string", True), ("Samples", "int", True)],
+    "DiskLight": [("Name", "string", True), ("P", "v3", True), ("Up", "v3", True), ("LookAt", "v3", True),
+                  ("Radius", "float", True), ("Shader", "string", True), ("Segments", "int", False),
+                  ("Samples", "int", False)],
+    "SphereLight": [("Name", "string", True), ("P", "v3", True), ("Radius", "float", True), ("Shader", "string", True),
+                    ("Samples", "int", False)],
+    "QuadLight": [("Name", "string", True), ("P", "v3", True), ("U", "v3", True), ("V", "v3", True),
+                  ("Shader", "string", True), ("Samples", "int", False)],
+    "Sphere": [("Name", "string", True), ("RayBias", "float", False), ("P", "v3", True), ("Radius", "float", True),
+               ("Shader", "string", True)],
+    "GaussianFilter": [("NodeDef", "struct", True), ("Name", "string", True), ("Width", "float", True),
+                       ("Res", "int", True)],
+    "AiryFilter": [("Name", "string", True), ("Width", "float", False), ("Res", "int", False), ("Peak", "float", False)],
+    "OutputHDR": [("Filename", "string", True)],
+    "OutputFloat": [("Filename", "string", True)],
+    "Include": [("NodeDef", "struct", False), ("Name", "string", False), ("Filename", "string", True)],
+    "Proc": [("Name", "string", True), ("Handler", "string", True), ("Data", "string", True),
+             ("BMin", "point", True), ("BMax", "point", True), ("Transform", "matrix", False)],
+    "GeomInstance": [("Name", "string", True), ("Geom", "string", True), ("BMin", "point", True),
+                     ("BMax", "point", True), ("Transform", "matrix", True)],
+}
+
+ARRAY_WIDTH = {"point": 3, "vec3": 3, "vec2": 2, "float32array": 1, "matrix": 16}
+ARRAY_TYPE = {"point": "point", "vec3": "vec3", "vec2": "vec2", "float32array": "float", "matrix": "matrix"}
+
+
+def f32(x):
+    with np.errstate(over="ignore"):
+        return float(np.float32(x))
+
+
+def i64_to_f32(v):
+    """float32(int64) on amd64 (CVTSI2SS): one rounding, to nearest even."""
+    a = abs(v)
+    if a < 1 << 24:
+        return float(v)
+    shift = a.bit_length() - 24
+    q, r = divmod(a, 1 << shift)
+    half = 1 << (shift - 1)
+    if r > half or (r == half and q & 1):
+        q += 1
+    return float(q << shift) * (-1 if v < 0 else 1)
+
+
+def go_int64(f):
+    """int64(float64) on amd64 (CVTTSD2SQ): truncation, -2^63 for NaN or out of range."""
+    return int(f) if -2.0 ** 63 <= f < 2.0 ** 63 else -2 ** 63
+
+
+def wrap32(v):
+    return (v + 2 ** 31) % 2 ** 32 - 2 ** 31
+
+
+class Lex:
+    def __init__(self, data):
+        self.lines = data.split(b"\n")
+        self.nl = [True] * (len(self.lines) - 1) + [False]   # the last piece has no newline
+        self.li = -1
+        self.line = b""
+        self.peek = 0
+        self.peek_token = None
+        self.line_number = 0
+
+    def read_line(self):
+        self.line_number += 1
+        self.li += 1
+        if self.li >= len(self.lines):
+            return False
+        piece = self.lines[self.li] + (b"\n" if self.nl[self.li] else b"")
+        if not piece:
+            return False
+        self.line = piece
+        return True
+
+    def next(self):
+        if self.peek:
+            r, self.peek = self.peek, 0
+            return r
+        while not self.line:
+            if not self.read_line():
+                return 0
+        c, self.line = self.line[0], self.line[1:]
+        if c == ord("#"):
+            if not self.read_line():
+                return 0
+            return ord("\n")
+        return c
+
+    @staticmethod
+    def alpha(c):
+        return c == ord("_") or (c < 128 and chr(c).isalpha())
+
+    def lex(self):
+        while True:
+            c = self.next()
+            if self.alpha(c):
+                return self.token(c)
+            if c == 0:
+                return EOF, None
+            if c == ord("-") or ord("0") <= c <= ord("9"):
+                return self.num(c)
+            ch = chr(c)
+            if ch == "[":
+                return OBRACE, None
+            if ch == "]":
+                return CBRACE, None
+            if ch == "{":
+                return OCURLY, None
+            if ch == "}":
+                return CCURLY, None
+            if ch == ",":
+                return COMMA, None
+            if ch == '"':
+                return self.string()
+
+    def string(self):
+        b = bytearray()
+        while True:
+            c = self.next()
+            if c == 0:
+                return EOF, None
+            if c == ord('"'):
+                return STR, b.decode("utf-8", "replace")
+            b.append(c)
+
+    def token(self, c):
+        b = bytearray([c])
+        while True:
+            c = self.next()
+            if c == 0:
+                return EOF, None
+            if chr(c) in '"[]{},':
+                self.peek = c
+                break
+            if chr(c) in " \t\r\n":
+                break
+            b.append(c)
+        return TOK, b.decode("utf-8", "replace")
+
+    def num(self, c):
+        b = chr(c)
+        is_float = False
+        while True:
+            c = self.next()
+            if c and (chr(c).isdigit() and c < 128 or chr(c) == "-"):
+                b += chr(c)
+            elif c and chr(c) in ".eE":
+                is_float = True
+                b += chr(c)
+            else:
+                break
+        if c != 0:
+            self.peek = c
+        if is_float:
+            v = go_parse_float(b)
+            return (EOF, None) if v is None else (FLT, v)
+        v = go_parse_int(b)
+        return (EOF, None) if v is None else (INT, v)
+
+    def Lex(self):
+        if self.peek_token is not None:
+            t, self.peek_token = self.peek_token, None
+            return t
+        return self.lex()
+
+    def Peek(self):
+        t = self.lex()
+        self.peek_token = t
+        return t
+
+
+def go_parse_int(s):
+    """strconv.ParseInt(s, 10, 64)."""
+    body = s[1:] if s.startswith("-") else s
+    if not body or not body.isdigit():
+        return None
+    v = int(s)
+    return v if -2 ** 63 <= v < 2 ** 63 else None
+
+
+def go_parse_float(s):
+    """strconv.ParseFloat(s, 64) over digits, '-', '.', 'e', 'E'."""
+    import re
+    if not re.fullmatch(r"-?(\d+\.?\d*|\.\d+)([eE]-?\d+)?", s):
+        return None
+    v = float(s)
+    return None if math.isinf(v) else v
+
+
+class Parser:
+    def __init__(self, data, filename, counters):
+        self.lex = Lex(data)
+        self.filename = filename
+        self.nerrors = 0
+        self.log = []
+        self.fatal = False
+        self.counters = counters   # {"camera": n}
+
+    def errorf(self, msg):
+        self.log.append(f"{self.filename}:{self.lex.line_number}: {msg}")
+        self.nerrors += 1
+        if self.nerrors > 10:
+            self.fatal = True
+
+    def number(self):
+        t, v = self.lex.Lex()
+        if t == INT:
+            return i64_to_f32(v)
+        if t == FLT:
+            return f32(v)
+        return None
+
+    def slice_(self, p, strings):
+        t, v = self.lex.Lex()
+        if t != INT:
+            return
+        count = v
+        t, v = self.lex.Lex()
+        if t != TOK and v != ("string" if strings else "int"):
+            return
+        out = []
+        for _ in range(count):
+            t, v = self.lex.Lex()
+            if t != (STR if strings else INT):
+                return
+            out.append(v if strings else wrap32(v))
+        p["value"] = out
+
+    def map_(self, p, what):
+        self.lex.Lex()
+        if what == "rgbtex":
+            t, v = self.lex.Lex()
+            if t != STR:
+                return
+            p["value"] = ("tex", v)
+            return
+        c = []
+        for _ in range(3 if what == "rgb" else 1):
+            x = self.number()
+            if x is None:
+                return
+            c.append(x)
+        if len(c) == 1:
+            c = c * 3
+        p["value"] = ("c", c)
+
+    def array(self, p, kind):
+        t, v = self.lex.Lex()
+        if t != INT:
+            return False
+        keys, per = v, 1
+        if kind != "matrix":
+            t, v = self.lex.Lex()
+            if t != INT:
+                return False
+            per = v
+        t, v = self.lex.Lex()
+        if t != TOK and v != ARRAY_TYPE[kind]:
+            return False
+        w = ARRAY_WIDTH[kind]
+        e = []
+        for _ in range((keys or 1) * per):
+            m = []
+            for _ in range(w):
+                x = self.number()
+                if x is None:
+                    return False
+                m.append(x)
+            if kind == "matrix":   # the file lists rows; Matrix4Transpose stores columns
+                m = [m[c * 4 + r] for r in range(4) for c in range(4)]
+            e += m
+        p["value"] = (keys, per, e)
+        return True
+
+    def param(self, p):
+        kind = p["kind"]
+        if kind == "int":
+            t, v = self.lex.Lex()
+            if t == FLT:
+                p["value"] = go_int64(v)
+            elif t == INT:
+                p["value"] = v
+        elif kind == "float":
+            t, v = self.lex.Lex()
+            if t in (FLT, INT):
+                p["value"] = float(v)
+        elif kind == "bool":
+            t, v = self.lex.Lex()
+            if t == INT:
+                p["value"] = v != 0
+        elif kind == "string":
+            t, v = self.lex.Lex()
+            if t == STR:
+                p["value"] = v
+        elif kind in ("ints", "strings"):
+            t, v = self.lex.Peek()
+            if t == INT:
+                self.slice_(p, kind == "strings")
+            else:
+                self.errorf("Invalid token for param (expecting length of slice)")
+                self.lex.Lex()
+        elif kind == "map":
+            t, v = self.lex.Peek()
+            if t == TOK and v in ("rgb", "float", "rgbtex"):
+                self.map_(p, v)
+        elif kind == "v3":
+            c = []
+            for _ in range(3):
+                x = self.number()
+                if x is None:
+                    return False
+                c.append(x)
+            p["value"] = (0, 0, c)
+        elif kind in ARRAY_WIDTH:
+            return self.array(p, kind)
+        elif kind == "struct":
+            self.errorf("Invalid type for param (core.NodeDef)")
+            self.lex.Lex()
+        return True
+
+    def skip_to_token(self):
+        while True:
+            t, v = self.lex.Peek()
+            if t == CCURLY or (t == TOK and v not in KEYWORDS) or t == EOF:
+                return
+            self.lex.Lex()
+
+    def node(self, type_):
+        if type_ not in SCHEMAS:
+            return None
+        n = {"type": type_, "file": self.filename, "line": self.lex.line_number,
+             "p": {name: {"kind": k, "req": r, "present": False, "value": None} for name, k, r in SCHEMAS[type_]}}
+        defaults(type_, n["p"], self.counters)
+        while True:
+            if self.fatal:
+                return None
+            t, v = self.lex.Lex()
+            if t == TOK:
+                f = n["p"].get(v)
+                if f is None:
+                    self.errorf(f'Field "{v}" not found in node {type_}')
+                    self.skip_to_token()
+                    continue
+                if f["present"]:
+                    self.errorf(f"Field {v} already found in {type_}")
+                    return None
+                if not self.param(f):
+                    self.errorf(f"Error parsing field {v}")
+                f["present"] = True
+            elif t == CCURLY:
+                for name, f in n["p"].items():
+                    if f["req"] and not f["present"]:
+                        self.errorf(f"node: required field {name} not found in {type_}")
+                        return None
+                return n
+            elif t == EOF:
+                self.errorf(f"node: end of file inside node {type_}")
+                self.fatal = True
+                return None
+            else:
+                self.errorf(f'node: Parse error, invalid token in object "{t}"')
+
+    def parse(self):
+        out = []
+        while not self.fatal:
+            t, v = self.lex.Lex()
+            if t != TOK:
+                break
+            type_ = v
+            if self.lex.Lex()[0] != OCURLY:
+                self.errorf("Invalid token in node preamble")
+            n = self.node(type_)
+            if n is None:
+                self.errorf(f"Node is nil: {'<nil>' if type_ in SCHEMAS else 'Node type ' + type_ + ' not registered.'}")
+                while True:
+                    u = self.lex.Lex()[0]
+                    if u in (CCURLY, EOF):
+                        break
+                continue
+            out.append(n)
+        return out
+
+
+def defaults(type_, p, counters):
+    """The constructors' field values (their nodes.Register functions)."""
+    def put(k, v):
+        p[k]["value"] = v
+    if type_ == "Globals":
+        put("XRes", 256), put("YRes", 256), put("MaxGoRoutines", 5)
+    elif type_ == "PolyMesh":
+        put("IsVisible", True)
+    elif type_ == "Camera":
+        put("Focal", 12.0), put("Fov", 90.0), put("Name", f"camera<{counters['camera']}>")
+        counters["camera"] += 1
+    elif type_ == "DiskLight":
+        put("Segments", 20), put("Samples", 1)
+    elif type_ == "SphereLight":
+        put("Radius", 1.0), put("Samples", 1)
+    elif type_ == "QuadLight":
+        put("Samples", 1)
+    elif type_ == "TriLight":
+        put("Samples", 1)
+    elif type_ == "Sphere":
+        put("Radius", 1.0)
+    elif type_ == "GaussianFilter":
+        put("Res", 17), put("Width", 2.0)
+    elif type_ == "AiryFilter":
+        put("Res", 49), put("Width", 6.0), put("Peak", 4.0)
+    elif type_ == "OutputHDR":
+        put("Filename", "out.hdr")
+    elif type_ == "OutputFloat":
+        put("Filename", "out.float")
+
+
+def parse_file(path, counters=None):
+    """nodes.Parse: (nodes, nerrors, log lines, fatal)."""
+    counters = counters if counters is not None else {"camera": 0}
+    with open(path, "rb") as f:
+        p = Parser(f.read(), str(path), counters)
+    nodes = p.parse()
+    return nodes, p.nerrors, p.log, p.fatal
+
+
+def _hex(x):
+    return "%08x" % struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def dump(nodes):
+    out = []
+    for n in nodes:
+        out.append(n["type"])
+        for name, k, _ in SCHEMAS[n["type"]]:
+            f = n["p"][name]
+            if not f["present"]:
+                continue
+            v = f["value"]
+            s = f"  {name}:"
+            if k == "int":
+                s += f" {v if v is not None else 0}"
+            elif k == "float":
+                s += " " + _hex(f32(v if v is not None else 0.0))
+            elif k == "bool":
+                s += " 1" if v else " 0"
+            elif k == "string":
+                s += f' "{v if v is not None else ""}"'
+            elif k == "ints":
+                s += "".join(f" {x}" for x in (v or []))
+            elif k == "strings":
+                s += "".join(f' "{x}"' for x in (v or []))
+            elif k == "map":
+                if v is None:
+                    s += " nil"
+                elif v[0] == "tex":
+                    s += f' tex "{v[1]}"'
+                else:
+                    s += "".join(" " + _hex(x) for x in v[1])
+            elif k == "struct":
+                pass
+            else:
+                keys, per, e = v if v is not None else (0, 0, [])
+                s += f" keys {keys} per {per}" + "".join(" " + _hex(x) for x in e)
+            out.append(s)
+    return "\n".join(out) + ("\n" if out else "")
+
+
+# ------------------------------------------------------------------ PreRender
+
+def _val(n, k, default=None):
+    v = n["p"][k]["value"] if k in n["p"] else None
+    return default if v is None else v
+
+
+def _name(n):
+    t = n["type"]
+    if t == "Globals":
+        return "<globals>"
+    if t in ("OutputHDR", "OutputFloat"):
+        return t + "<>"
+    return _val(n, "Name", "")
+
+
+def _triangulate(n):
+    """PolyMesh.init (init.go:12-134): (idxp, normalidx or None, shaderidx or None)."""
+    verts = _val(n, "Verts")
+    pc, fi = _val(n, "PolyCount"), _val(n, "FaceIdx")     # None: not read (nil in Go)
+    ni, si = _val(n, "NormalIdx", []), _val(n, "ShaderIdx", [])
+    normals = _val(n, "Normals")
+    has_n = normals is not None and len(normals[2]) > 0
+    idxp, nidx, sidx = [], [], []
+    if pc is not None:
+        base = 0
+        for k, cnt in enumerate(pc):
+            i = 0
+            for _ in range(cnt - 2):
+                i += 1
+                for q in (base, base + i, base + i + 1):
+                    idxp.append(fi[q])
+                    if has_n:
+                        nidx.append(ni[q] if ni else fi[q])
+                if si:
+                    sidx.append(si[k] & 0xFF)
+            base += cnt
+    elif fi is not None:
+        for j, x in enumerate(fi):
+            idxp.append(x)
+            if has_n:
+                nidx.append(ni[j] if ni else x)
+        sidx = [x & 0xFF for x in si]
+    else:
+        for j in range(verts[1]):
+            idxp.append(j)
+            if has_n:
+                nidx.append(ni[j] if ni else j)
+        sidx = [x & 0xFF for x in si]
+    return idxp, (nidx if has_n else None), (sidx if sidx else None)
+
+
+def _light_mesh(L):
+    """Tri.createMesh (triangle.go:537-567) / Disk.createMesh (disk.go:224-262) as a PolyMesh node."""
+    from vermeer_amd import scenes
+    if L["type"] == "TriLight":
+        P0, P1, P2 = (np.array(_val(L, k)[2], np.float32) for k in ("P0", "P1", "P2"))
+        N = scenes._normalize32(scenes._cross32(P1 - P0, P2 - P0))
+        V = np.stack([P0, P1, P2])
+        NN = np.stack([N, N, N])
+    else:
+        V, _, NN = scenes.disk_light_mesh(_val(L, "P")[2], _val(L, "Up")[2], _val(L, "LookAt")[2], _val(L, "Radius"),
+                                          segments=_val(L, "Segments"))
+    m = {"type": "PolyMesh", "file": L["file"], "line": L["line"],
+         "p": {name: {"kind": k, "req": r, "present": False, "value": None} for name, k, r in SCHEMAS["PolyMesh"]}}
+    m["p"]["Name"]["value"] = _val(L, "Name", "") + ":<mesh>"
+    m["p"]["Shader"]["value"] = [_val(L, "Shader")]
+    m["p"]["IsVisible"]["value"] = True
+    m["p"]["Verts"]["value"] = (1, len(V), [float(x) for x in np.asarray(V, np.float32).reshape(-1)])
+    m["p"]["Normals"]["value"] = (1, len(NN), [float(x) for x in np.asarray(NN, np.float32).reshape(-1)])
+    return m
+
+
+class Unsupported(Exception):
+    pass
+
+
+def build(path, cwd=None):
+    """core.PreRender plus Render's set-up for a .vnf file, as inputs of OracleRenderer: a dict with
+    desc (meshes, materials, lights), width, height, camera (aspect, tan_theta_focal, focal, radius,
+    keys: LocalToWorld matrices), filter (kind, width, res, peak) or None, max_iter, nerrors, outputs."""
+    import oracle
+    counters = {"camera": 0}
+    nodes, nerrors, log, fatal = parse_file(path, counters)
+    if fatal:
+        raise Unsupported("too many parse errors")
+    allnodes, by_name, geoms, lights = [], {}, [], []
+    state = {"filter": None, "globals": None}
+    light_geom = {}
+
+    def add(n):
+        allnodes.append(n)
+        by_name[_name(n)] = n
+        t = n["type"]
+        if t in ("PolyMesh", "Sphere"):
+            geoms.append(n)
+        elif t in ("TriLight", "DiskLight", "SphereLight", "QuadLight"):
+            lights.append(n)
+        elif t in ("GaussianFilter", "AiryFilter"):
+            state["filter"] = n
+        elif t == "Globals":
+            state["globals"] = n
+
+    for n in nodes:
+        add(n)
+    g0 = state["globals"]
+    width, height = (_val(g0, "XRes"), _val(g0, "YRes")) if g0 else (1024, 1024)
+
+    def shader(name):
+        s = by_name.get(name)
+        return s if s is not None and s["type"] in ("ShaderStd", "DebugShader") else None
+
+    rnd = list(allnodes)
+    while rnd:
+        added = []
+        for n in rnd:
+            t = n["type"]
+            if t == "Include":
+                sub, ne, lg, fa = parse_file(_val(n, "Filename"), counters)
+                if fa:
+                    raise Unsupported("too many parse errors in an included file")
+                nerrors += ne
+                for s in sub:
+                    add(s)
+                    added.append(s)
+                continue
+            if t in ("Proc", "GeomInstance", "QuadLight"):
+                raise Unsupported(t)
+            if t in ("PolyMesh", "Sphere", "TriLight", "DiskLight", "SphereLight"):
+                for s in (_val(n, "Shader") if t == "PolyMesh" else [_val(n, "Shader")]):
+                    if shader(s) is None:
+                        raise Unsupported(f"Unable to find node (shader {s})")
+            if t in ("TriLight", "DiskLight"):
+                m = _light_mesh(n)
+                light_geom[id(n)] = m
+                add(m)
+                added.append(m)
+            elif t == "SphereLight":
+                g = {"type": "Sphere", "file": n["file"], "line": n["line"],
+                     "p": {name: {"kind": k, "req": r, "present": False, "value": None} for name, k, r in SCHEMAS["Sphere"]}}
+                g["p"]["Name"]["value"] = _val(n, "Name", "") + ":<geom>"
+                g["p"]["P"]["value"] = _val(n, "P")
+                g["p"]["Radius"]["value"] = _val(n, "Radius")
+                g["p"]["Shader"]["value"] = _val(n, "Shader")
+                light_geom[id(n)] = g
+                add(g)
+                added.append(g)
+        rnd = added
+
+    mats = []
+
+    def material(name):
+        s = shader(name)
+        if s["type"] == "DebugShader":
+            raise Unsupported("DebugShader")
+        for i, m in enumerate(mats):
+            if m is s:
+                return i
+        mats.append(s)
+        return len(mats) - 1
+
+    meshes, geom_index = [], {}
+    for g in geoms:
+        geom_index[id(g)] = len(meshes)
+        if g["type"] == "Sphere":
+            meshes.append({"sphere": True, "centre": np.array(_val(g, "P")[2], np.float32),
+                           "radius": _val(g, "Radius"), "material": material(_val(g, "Shader"))})
+            continue
+        tr = _val(g, "Transform")
+        if tr is not None:
+            e = tr[2]
+            for k in range(0, len(e) // 16 * 16, 16):
+                if e[k:k + 16] != [1.0 if i % 5 == 0 else 0.0 for i in range(16)]:
+                    raise Unsupported("mesh transform")
+        keys, per, ev = _val(g, "Verts")
+        keys = max(keys, 1)
+        idxp, nidx, sidx = _triangulate(g)
+        shaders = np.array([material(s) for s in _val(g, "Shader")], np.int32)
+        nv = _val(g, "Normals")
+        V = np.array(ev[: keys * per * 3], np.float32).reshape(keys, per, 3)
+        mesh = {"idx": np.array(idxp, np.uint32).reshape(-1, 3), "shaders": shaders,
+                "shaderidx": None if sidx is None else np.array(sidx, np.uint8),
+                "ray_bias": _val(g, "RayBias", 0.0)}
+        if keys == 1:
+            mesh["verts"] = V[0]
+            mesh["normals"] = None
+            if nidx is not None:
+                mesh["normal_list"] = np.array(nv[2], np.float32).reshape(-1, 3)
+                mesh["normal_idx"] = np.array(nidx, np.uint32)
+        else:
+            mesh["keys"] = keys
+            mesh["verts"] = V
+            mesh["reference_indexing"] = True
+            mesh["normals"] = None
+            if nv is not None and nv[2]:
+                nk, nper = max(nv[0], 1), nv[1]
+                arr = np.array(nv[2], np.float32).reshape(nk, nper, 3)[:, : len(idxp)]
+                mesh["normals"] = arr.reshape(nk, -1, 3) if nv[0] == keys else None
+        meshes.append(mesh)
+
+    from vermeer_amd import scenes
+    from vermeer_amd._lib import MATERIAL_DTYPE
+    mtab = np.zeros(max(len(mats), 1), MATERIAL_DTYPE)
+    for i, s in enumerate(mats):
+        def mp(k):
+            v = _val(s, k)
+            if v is None:
+                return None
+            if v[0] == "tex":
+                raise Unsupported("texture")
+            return v[1]
+        for key, rec, one in (("EmissionColour", "emission", False), ("EmissionStrength", "emission_strength", True),
+                              ("DiffuseColour", "diffuse", False), ("DiffuseStrength", "diffuse_strength", True),
+                              ("DiffuseRoughness", "diffuse_roughness", True), ("Spec1Colour", "spec1", False),
+                              ("Spec1Strength", "spec1_strength", True), ("Spec1Roughness", "spec1_roughness", True),
+                              ("IOR", "ior", True), ("Spec1FresnelRefl", "spec1_fresnel_refl", False),
+                              ("Spec1FresnelEdge", "spec1_fresnel_edge", False)):
+            c = mp(key)
+            if c is None:
+                continue
+            mtab[i][rec] = c[0] if one else c
+            flag = {"emission": "has_emission_colour", "emission_strength": "has_emission_strength",
+                    "diffuse": "has_diffuse_colour", "spec1": "has_spec1_colour"}.get(rec, "has_" + rec)
+            mtab[i][flag] = 1
+        fm = _val(s, "Spec1FresnelModel", "")
+        mtab[i]["fresnel_model"] = 2 if fm == "Metal" else 1 if fm == "Dielectric" else 0
+
+    lrec = []
+    for L in lights:
+        d = {"material": material(_val(L, "Shader")), "samples": _val(L, "Samples"),
+             "geom": geom_index[id(light_geom[id(L)])]}
+        if L["type"] == "TriLight":
+            d.update(type="tri", P0=np.array(_val(L, "P0")[2], np.float32), P1=np.array(_val(L, "P1")[2], np.float32),
+                     P2=np.array(_val(L, "P2")[2], np.float32))
+        elif L["type"] == "DiskLight":
+            d.update(type="disk", P=np.array(_val(L, "P")[2], np.float32), Up=np.array(_val(L, "Up")[2], np.float32),
+                     LookAt=np.array(_val(L, "LookAt")[2], np.float32), radius=_val(L, "Radius"))
+        else:
+            d.update(type="sphere", P=np.array(_val(L, "P")[2], np.float32), radius=_val(L, "Radius"))
+        lrec.append(d)
+
+    gl = state["globals"]
+    cam_name = "camera"
+    max_iter = 16
+    if gl is not None:
+        if _val(gl, "Camera"):
+            cam_name = _val(gl, "Camera")
+        max_iter = _val(gl, "MaxIter", 0)
+    cam = by_name.get(cam_name)
+    if cam is None or cam["type"] != "Camera":
+        raise Unsupported("no camera")
+    aspect = f32(_val(cam, "Aspect", 0.0))
+    if aspect == 0.0:
+        aspect = f32(np.float32(width) / np.float32(height))
+    fov, focal = np.float32(_val(cam, "Fov")), np.float32(_val(cam, "Focal"))
+    ang = (fov / np.float32(2)) * np.float32(3.14159265358) / np.float32(180)
+    ttf = f32(np.float32(math.tan(float(ang))) * focal)
+    keys_out = []
+    ltw = _val(cam, "LocalToWorld")
+    if ltw is not None:
+        e = ltw[2]
+        keys_out += [e[k:k + 16] for k in range(0, len(e) // 16 * 16, 16)]
+    if _val(cam, "Type") == "LookAt":
+        fr, to, ro = _val(cam, "From"), _val(cam, "To"), _val(cam, "Roll")
+        fk = np.array(fr[2], np.float32).reshape(-1, 3)[: fr[0]]
+        tk = np.array(to[2], np.float32).reshape(-1, 3)[: to[0]]
+        rk = None if ro is None or not ro[2] else np.array(ro[2], np.float32)[: ro[0]]
+        mats_k = oracle.camera_lookat_keys(fk, tk, np.array(_val(cam, "Up")[2], np.float32), rk)
+        keys_out += [list(np.asarray(m, np.float32).reshape(16)) for m in mats_k]
+    else:
+        raise Unsupported("matrix camera")
+    filt = None
+    fn = state["filter"]
+    if fn is not None:
+        if fn["type"] == "GaussianFilter":
+            filt = ("gaussian", _val(fn, "Width"), _val(fn, "Res"), 0.0)
+        else:
+            filt = ("airy", _val(fn, "Width"), _val(fn, "Res"), _val(fn, "Peak"))
+    outputs = [("hdr" if n["type"] == "OutputHDR" else "float", _val(n, "Filename"))
+               for n in allnodes if n["type"] in ("OutputHDR", "OutputFloat")]
+    return {"desc": {"meshes": meshes, "materials": mtab, "lights": lrec}, "width": width, "height": height,
+            "camera": {"aspect": aspect, "tan_theta_focal": ttf, "focal": f32(focal), "radius": f32(_val(cam, "Radius", 0.0)),
+                       "keys": np.array(keys_out, np.float32)},
+            "filter": filt, "max_iter": max_iter, "nerrors": nerrors, "outputs": outputs}
+
+
+def renderer(built):
+    """An OracleRenderer for build()'s result."""
+    import oracle
+    c = built["camera"]
+    r = oracle.OracleRenderer(built["desc"], built["width"], built["height"], np.eye(4, dtype=np.float32).reshape(-1),
+                              c["tan_theta_focal"], c["focal"], aspect=c["aspect"])
+    r.set_camera_keys(c["keys"])
+    if c["radius"] > 0:
+        r.set_lens(c["radius"])
+    if built["filter"] is not None:
+        kind, w, res, peak = built["filter"]
+        r.set_filter(kind, w, res, peak)
+    return r

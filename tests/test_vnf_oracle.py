@@ -1,0 +1,106 @@
+"""Pins the oracle's restatement of PreRender for .vnf files (oracle/oracle_vnf.py build()) with known
+answers worked from the Go source:
+
+  polymesh/init.go:12-134   PolyCount fans (0, i, i+1), ShaderIdx per polygon, NormalIdx per corner
+                            (FaceIdx when absent), FaceIdx-only and index-free meshes
+  camera.go:80-98           Aspect from the frame (float32 XRes / YRes), TanThetaFocal =
+                            float32(tan(float64(Fov/2 * Pi / 180))) * Focal with Pi = 3.14159265358
+  core/core.go:69-89        AddNode order decides geoms and lights; a light's PreRender adds its
+                            geometry as a new node, prerendered in the next round
+  core/render.go:146-163    the camera is the node named by Globals.Camera, else "camera"
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_vnf
+
+
+def _build(tmp_path, text):
+    p = tmp_path / "s.vnf"
+    p.write_text(text)
+    return oracle_vnf.build(str(p))
+
+
+CAM = ('Camera { Name "camera" Type "LookAt" From 1 1 point 0 0 5 To 1 1 point 0 0 0 Roll 0 0 float '
+       'Up 0 1 0 Fov 50 Focal 3.5 }\n')
+SH = 'ShaderStd { Name "a" DiffuseStrength float 1 }\nShaderStd { Name "b" Spec1Strength float 1 }\n'
+
+
+def test_polycount_fans_and_indices(tmp_path):
+    b = _build(tmp_path, 'Globals { XRes 30 YRes 20 }\n' + CAM + SH +
+               'PolyMesh { Name "m" Verts 1 7 point 0 0 0 1 0 0 1 1 0 0 1 0 2 0 0 3 0 0 3 1 0 '
+               'PolyCount 2 int 4 3 FaceIdx 7 int 0 1 2 3 4 5 6 ShaderIdx 2 int 1 0 '
+               'Shader 2 string "a" "b" Normals 1 2 vec3 0 0 1 0 0 -1 NormalIdx 7 int 0 1 0 1 1 1 0 }\n')
+    m = b["desc"]["meshes"][0]
+    assert m["idx"].tolist() == [[0, 1, 2], [0, 2, 3], [4, 5, 6]]
+    assert m["shaderidx"].tolist() == [1, 1, 0]                      # per polygon, repeated per triangle
+    assert m["normal_idx"].tolist() == [0, 1, 0, 0, 0, 1, 1, 1, 0]   # NormalIdx at corners 0,1,2 0,2,3 4,5,6
+    assert m["shaders"].tolist() == [0, 1]                           # materials in first-use order
+    assert b["camera"]["aspect"] == np.float32(1.5)
+    ttf = np.float32(math.tan(float(np.float32(np.float32(25.0) * np.float32(3.14159265358)) / np.float32(180))))
+    assert b["camera"]["tan_theta_focal"] == np.float32(ttf * np.float32(3.5))
+    assert abs(b["camera"]["tan_theta_focal"] - math.tan(math.radians(25)) * 3.5) < 1e-6
+    assert (b["width"], b["height"], b["max_iter"]) == (30, 20, 0)
+
+
+def test_faceidx_only_and_implicit(tmp_path):
+    b = _build(tmp_path, CAM + SH +
+               'PolyMesh { Name "f" Verts 1 4 point 0 0 0 1 0 0 1 1 0 0 1 0 FaceIdx 6 int 0 1 2 2 3 0 '
+               'ShaderIdx 2 int 0 257 Shader 1 string "a" Normals 1 4 vec3 0 0 1 0 0 1 0 0 1 0 0 1 }\n'
+               'PolyMesh { Name "v" Verts 1 3 point 0 0 1 1 0 1 1 1 1 Shader 1 string "b" }\n')
+    f, v = b["desc"]["meshes"]
+    assert f["idx"].tolist() == [[0, 1, 2], [2, 3, 0]]
+    assert f["normal_idx"].tolist() == [0, 1, 2, 2, 3, 0]              # FaceIdx when NormalIdx is absent
+    assert f["shaderidx"].tolist() == [0, 1]                         # uint8(257) = 1
+    assert v["idx"].tolist() == [[0, 1, 2]] and "normal_list" not in v
+    assert (b["width"], b["height"], b["max_iter"]) == (1024, 1024, 16)   # defaultGlobals without Globals
+
+
+def test_lights_add_geometry_in_order(tmp_path):
+    b = _build(tmp_path, CAM + SH +
+               'SphereLight { Name "s" P 0 3 0 Radius 0.5 Shader "a" }\n'
+               'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "b" }\n'
+               'TriLight { Name "t" P0 0 2 0 P1 1 2 0 P2 0 2 1 Shader "a" Samples 2 }\n'
+               'DiskLight { Name "d" P 0 2 0 Up 1 0 0 LookAt 0 0 0 Radius 1 Shader "a" Segments 4 }\n')
+    meshes, lights = b["desc"]["meshes"], b["desc"]["lights"]
+    # the file's geoms first, then the light geometry in the order the lights were prerendered
+    assert [("sphere" if m.get("sphere") else len(m["idx"])) for m in meshes] == [1, "sphere", 1, 4]
+    assert [(L["type"], L["geom"], L["samples"]) for L in lights] == [("sphere", 1, 1), ("tri", 2, 2), ("disk", 3, 1)]
+    assert meshes[1]["radius"] == 0.5 and meshes[1]["centre"].tolist() == [0, 3, 0]
+    tri = meshes[2]
+    assert tri["verts"].tolist() == [[0, 2, 0], [1, 2, 0], [0, 2, 1]]
+    # Tri.createMesh's normal: normalize((P1 - P0) x (P2 - P0)) = (0, -1, 0) on every corner
+    assert tri["normal_list"].tolist() == [[0, -1, 0]] * 3
+    disk = meshes[3]
+    rim = np.linalg.norm(disk["verts"][1::3] - np.array([0, 2, 0], np.float32), axis=1)
+    assert np.allclose(rim, 1.0, atol=1e-6) and np.allclose(disk["verts"][0::3], [0, 2, 0])
+    # materials in first-use order over the geoms: "b" on the file's mesh, then "a" on the sphere
+    assert b["desc"]["materials"]["has_spec1_strength"].tolist() == [1, 0]
+    assert b["desc"]["materials"]["has_diffuse_strength"].tolist() == [0, 1]
+
+
+def test_camera_selection_and_errors(tmp_path):
+    two = CAM.replace('"camera"', '"c2"').replace("Fov 50", "Fov 90")
+    b = _build(tmp_path, 'Globals { Camera "c2" MaxIter 5 }\n' + CAM + two + SH)
+    assert b["camera"]["tan_theta_focal"] == np.float32(np.float32(math.tan(float(np.float32(45) * np.float32(3.14159265358) / np.float32(180)))) * np.float32(3.5))
+    assert b["max_iter"] == 5
+    with pytest.raises(oracle_vnf.Unsupported, match="no camera"):
+        _build(tmp_path, 'Globals { Camera "none" }\n' + CAM + SH)
+    with pytest.raises(oracle_vnf.Unsupported, match="shader"):
+        _build(tmp_path, CAM + 'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "x" }\n')
+    with pytest.raises(oracle_vnf.Unsupported, match="transform"):
+        _build(tmp_path, CAM + SH + 'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "a" '
+                                    'Transform 1 matrix 1 0 0 1 0 1 0 0 0 0 1 0 0 0 0 1 }\n')
+
+
+def test_include_is_read_at_prerender(tmp_path):
+    inc = tmp_path / "inc.vnf"
+    inc.write_text(SH + 'Globals { XRes 8 YRes 8 MaxIter 2 }\n')
+    b = _build(tmp_path, 'Globals { XRes 40 YRes 10 }\n' + CAM + f'Include {{ Filename "{inc}" }}\n'
+               'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "b" }\n')
+    # the frame was created from the first file's Globals; the included Globals still sets MaxIter
+    assert (b["width"], b["height"], b["max_iter"]) == (40, 10, 2)
+    assert b["camera"]["aspect"] == np.float32(4.0)
+    assert b["desc"]["materials"]["has_spec1_strength"].tolist() == [1]

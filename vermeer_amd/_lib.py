@@ -34,6 +34,10 @@ VGPU_LIGHT_SPHERE = 2
 VGPU_FILTER_BOX = 0
 VGPU_FILTER_GAUSSIAN = 1
 VGPU_FILTER_AIRY = 2
+VGPU_VNF_MESSAGES = 0
+VGPU_VNF_NODES = 1
+VGPU_OUTPUT_HDR = 0
+VGPU_OUTPUT_FLOAT = 1
 VGPU_DIAG_COPY = 0
 VGPU_DIAG_READ = 1
 
@@ -82,6 +86,12 @@ SIGNATURES = [
     ("vgpu_scene_timing", C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_I32)]),
     ("vgpu_diag_stream", C.c_int, [_I32, _P, _P, _U64, _I32, _P]),
     ("vgpu_film_to_rgbe", C.c_int, [C.c_int, _U32, _P, _U32, _U32, _P, _P]),
+    ("vgpu_vnf_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    ("vgpu_vnf_destroy", C.c_int, [_P]),
+    ("vgpu_vnf_info", C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_I32), C.POINTER(_I32)]),
+    ("vgpu_vnf_text", C.c_int, [_P, _I32, C.c_char_p, _U64, C.POINTER(_U64)]),
+    ("vgpu_vnf_build", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P)]),
+    ("vgpu_vnf_output", C.c_int, [_P, _I32, C.POINTER(_I32), C.c_char_p, _U64, C.POINTER(_I32)]),
     ("vgpu_write_hdr", C.c_int, [C.c_int, _U32, _P, _U32, _U32, C.c_char_p, _P]),
     ("vgpu_write_float", C.c_int, [C.c_int, _U32, _P, _U32, _U32, C.c_char_p, _P]),
     ("vgpu_scene_add_shaded_mesh", C.c_int, [_P, _P, _U32, _P, _U32, _P, _P, _P, _U32, C.POINTER(_I32)]),

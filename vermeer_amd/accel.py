@@ -105,6 +105,17 @@ class Scene:
         self.geoms = []
         self.committed = False
 
+    @classmethod
+    def adopt(cls, handle, device=0):
+        """A Scene owning a committed vgpu_scene the library created (vgpu_vnf_build)."""
+        sc = cls.__new__(cls)
+        sc._lib = _lib.load()
+        sc._h = handle
+        sc.device = device
+        sc.geoms = []
+        sc.committed = True
+        return sc
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.vgpu_scene_destroy(self._h)

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvermeer_gpu.so")
 
-SOURCES = ["trace.hip", "raysort.hip", "hbmcopy.hip", "output.hip", "render.hip", "guide.hip", "pathtrace.hip", "scene.cpp", "render.cpp", "guide.cpp",
+SOURCES = ["trace.hip", "raysort.hip", "hbmcopy.hip", "output.hip", "vnf.cpp", "render.hip", "guide.hip", "pathtrace.hip", "scene.cpp", "render.cpp", "guide.cpp",
            "pathtrace.cpp", "qbvh_build.cpp"]
 HEADERS = ["trace_kernels.h", "render_kernels.h", "guide_kernels.h", "guide_internal.h", "pathtrace_kernels.h",
            "scene_internal.h", "qbvh_build.h"]

@@ -273,7 +273,86 @@ Mat4 decomp_matrix(const float* a, const float* b, float t) {
     return mat_mul(Tm, mat_mul(R, Sm));
 }
 
+
+// Vec3Normalize (math/vec3_amd64.s:11-42) with the IEEE 1/sqrt estimate under the reference's
+// Newton-Raphson step, as the renderer's device code computes it.
+void normalize3(const float* a, float* out) {
+    const float l2 = (a[1] * a[1] + a[0] * a[0]) + a[2] * a[2];
+    const float g = 1.0f / std::sqrt(l2);
+    const float gg = (g * g) * (l2 * 0.5f);
+    const float r = (1.5f - gg) * g;
+    for (int k = 0; k < 3; k++) out[k] = a[k] * r;
+}
+void cross3(const float* a, const float* b, float* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
 }  // namespace
+
+namespace vgpu_detail {
+
+void vec3_normalize(const float* a, float* out) { normalize3(a, out); }
+void vec3_cross(const float* a, const float* b, float* o) { cross3(a, b, o); }
+
+int lookat_keys(const float* from, int nfrom, const float* to, int nto, const float* up, const float* roll, int nroll,
+                std::vector<float>& out) {
+    // Camera.calcLookatMatrices (builtin/camera/camera.go:109-186): one LocalToWorld key per key of the
+    // array with more keys, the other one (and Roll) lerped at time i / n.
+    const bool by_target = nto > nfrom;
+    const int n = by_target ? nto : nfrom;
+    out.assign((size_t)n * 16, 0.0f);
+    auto key_lerp = [](const float* arr, int nk, float time, float* o, int width) {
+        const float k = time * (float)(nk - 1);
+        const float t = k - std::floor(k);
+        const int k0 = (int)std::floor(k), k1 = (int)std::ceil(k);
+        for (int c = 0; c < width; c++) o[c] = (1.0f - t) * arr[k0 * width + c] + t * arr[k1 * width + c];
+    };
+    for (int i = 0; i < n; i++) {
+        const float time = (float)i / (float)n;
+        float other[3], F[3], T[3];
+        key_lerp(by_target ? from : to, by_target ? nfrom : nto, time, other, 3);
+        const float* own = (by_target ? to : from) + (size_t)i * 3;
+        for (int c = 0; c < 3; c++) { F[c] = by_target ? other[c] : own[c]; T[c] = by_target ? own[c] : other[c]; }
+        float d[3] = {F[0] - T[0], F[1] - T[1], F[2] - T[2]}, W[3], u0[3], u[3], v0[3], v[3];
+        normalize3(d, W);
+        cross3(up, W, u0);
+        normalize3(u0, u);
+        cross3(u, W, v0);
+        normalize3(v0, v);
+        float rl = 0.0f;
+        if (roll && nroll > 0) key_lerp(roll, nroll, time, &rl, 1);
+        const float c = (float)std::cos((double)rl), sn = (float)std::sin((double)rl);
+        float U[3], V[3];
+        for (int k = 0; k < 3; k++) {
+            U[k] = c * u[k] + sn * v[k];
+            V[k] = -sn * u[k] + c * v[k];
+        }
+        Mat4 Tr{}, B{};
+        Tr[0] = Tr[5] = Tr[10] = Tr[15] = 1.0f;
+        Tr[12] = F[0]; Tr[13] = F[1]; Tr[14] = F[2];
+        B[0] = U[0]; B[1] = U[1]; B[2] = U[2]; B[4] = V[0]; B[5] = V[1]; B[6] = V[2];
+        B[8] = W[0]; B[9] = W[1]; B[10] = W[2]; B[15] = 1.0f;
+        const Mat4 M = mat_mul(Tr, B);
+        std::copy(M.begin(), M.end(), out.begin() + (size_t)i * 16);
+    }
+    return n;
+}
+
+bool mat4_inverse(const float* m, float* out) {
+    // Matrix4Inverse (math/matrix4.go:116-176): cofactors scaled by 1 / det; false when det == 0
+    Mat4 a{};
+    std::copy(m, m + 16, a.begin());
+    const Mat4 c = mat_cofactors(a);
+    float det = mat_det(a, c);
+    if (det == 0.0f) return false;
+    det = 1.0f / det;
+    for (int i = 0; i < 16; i++) out[i] = c[i] * det;
+    return true;
+}
+
+}  // namespace vgpu_detail
 
 struct vgpu_render {
     vgpu_scene* scene = nullptr;

@@ -160,6 +160,14 @@ hipEvent_t timing_event(vgpu_scene* s);
 
 // Ensures the spill area covers `threads` traversal threads.
 int ensure_spill(vgpu_scene* s, uint64_t threads);
+// Host math of the camera (render.cpp), in the reference's float32 operation order.
+void vec3_normalize(const float* a, float* out);
+void vec3_cross(const float* a, const float* b, float* o);
+// LocalToWorld keys of a LookAt camera (builtin/camera/camera.go:109-186), 16 floats each; returns the count.
+int lookat_keys(const float* from, int nfrom, const float* to, int nto, const float* up, const float* roll, int nroll,
+                std::vector<float>& out);
+bool mat4_inverse(const float* m, float* out);
+
 // Ensures the ray-sort buffers hold n rays.
 int ensure_sort(vgpu_scene* s, uint64_t n);
 

@@ -96,6 +96,18 @@ class Renderer:
         self._h = h
         self.film = np.zeros((height, width, 3), np.float32)
 
+    @classmethod
+    def adopt(cls, scene, handle, width, height):
+        """A Renderer owning a vgpu_render the library created (vgpu_vnf_build) over `scene`."""
+        r = cls.__new__(cls)
+        r._lib = _lib.load()
+        r.scene = scene
+        r.width, r.height = width, height
+        r._keep = []
+        r._h = handle
+        r.film = np.zeros((height, width, 3), np.float32)
+        return r
+
     def _add_motion_mesh(self, mesh):
         """A moving shaded mesh: verts (keys, nverts, 3); normals (normal keys, ntris * 3, 3) per face
         corner or None; shaderidx, shaders; reference_indexing selects the reference's slot indexing."""
