@@ -258,9 +258,10 @@ __device__ __forceinline__ void fetch_static(const DevNode* __restrict__ nodes, 
     nd.ax[0] = a.x; nd.ax[1] = a.y; nd.ax[2] = a.z;
 }
 
-// Motion node: boxes lerped per key (qbvh/motionintersect.go:47-49).
-__device__ __forceinline__ void fetch_motion(const float4* __restrict__ mnodes, int base, int node, int nkeys, int key,
-                                             int key2, float time, NodeData& nd) {
+// Motion node (qbvh/motionintersect.go:47-49): the boxes lerped per key and tested one axis at a
+// time, in the slab order of box4<false>, so only one axis of both keys is live at once.
+__device__ __forceinline__ void box_motion(const RayS& r, const float4* __restrict__ mnodes, int base, int node,
+                                           int nkeys, int key, int key2, float time, NodeData& nd, Box4& b) {
     const float4* p = mnodes + base + (size_t)node * (2 + 6 * nkeys);
     const int4 c = reinterpret_cast<const int4*>(p)[0];
     const int4 a = reinterpret_cast<const int4*>(p)[1];
@@ -268,8 +269,20 @@ __device__ __forceinline__ void fetch_motion(const float4* __restrict__ mnodes, 
     nd.ax[0] = a.x; nd.ax[1] = a.y; nd.ax[2] = a.z;
     const float4* b0 = p + 2 + 6 * key;
     const float4* b1 = p + 2 + 6 * key2;
-    nd.lx = lerp4(b0[0], b1[0], time); nd.ly = lerp4(b0[1], b1[1], time); nd.lz = lerp4(b0[2], b1[2], time);
-    nd.hx = lerp4(b0[3], b1[3], time); nd.hy = lerp4(b0[4], b1[4], time); nd.hz = lerp4(b0[5], b1[5], time);
+    float tmin[4], tmax[4];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        const float4 lo = lerp4(b0[ax], b1[ax], time), hi = lerp4(b0[3 + ax], b1[3 + ax], time);
+        const float la[4] = {lo.x, lo.y, lo.z, lo.w}, ha[4] = {hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) slab(la[ch], ha[ch], r.P[ax], r.Dinv[ax], tmin[ch], tmax[ch], ax == 0);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        const float tn = (0.0f > tmin[ch]) ? 0.0f : tmin[ch];
+        b.t[ch] = tn;
+        b.h[ch] = tn <= tmax[ch];
+    }
 }
 
 // ray.Time -> (key, key2, t) (builtin/geom/polymesh/trace.go:79-84, scene.go:49-54)
@@ -470,28 +483,24 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
                 if (node >= 0) {
                     if (COUNTERS) nodesT++;
                     NodeData nd;
+                    Box4 b;
                     if constexpr (MOTION) {
-                        int k0, k1;
-                        float t;
-                        if (in_mesh_m) {
-                            if (g.nkeys > 1) {
-                                motion_keys(r.time, g.nkeys, k0, k1, t);
-                                fetch_motion(sv.mnodes, g.node_base, node, g.nkeys, k0, k1, t, nd);
-                            } else {
-                                fetch_static(sv.nodes, g.node_base + node, nd);
-                            }
-                        } else if (top_motion) {
-                            motion_keys(r.time, sv.top_keys, k0, k1, t);
-                            fetch_motion(sv.mnodes, sv.top_mbase, node, sv.top_keys, k0, k1, t, nd);
+                        const bool mnode = in_mesh_m ? g.nkeys > 1 : top_motion;
+                        if (mnode) {
+                            const int nk = in_mesh_m ? g.nkeys : sv.top_keys;
+                            int k0, k1;
+                            float t;
+                            motion_keys(r.time, nk, k0, k1, t);
+                            box_motion(r, sv.mnodes, in_mesh_m ? g.node_base : sv.top_mbase, node, nk, k0, k1, t, nd, b);
                         } else {
-                            fetch_static(sv.nodes, node, nd);
+                            fetch_static(sv.nodes, in_mesh_m ? g.node_base + node : node, nd);
+                            box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
                         }
                     } else {
                         fetch_static(sv.nodes, mesh_base != 0 ? g.node_base + node : node, nd);
+                        if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                        else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
                     }
-                    Box4 b;
-                    if (fast) box4<true>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
-                    else box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
                     // Push order of qbvh/intersect.go:137-216 (same as motionintersect.go:53-97): the
                     // children are pushed as two pairs; rank(c) is child c's position in that sequence.
                     // Hit children are stored compacted in rank order, i.e. exactly the entries the
