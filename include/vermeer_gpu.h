@@ -322,6 +322,24 @@ int vgpu_render_set_filter(vgpu_render* render, int32_t type, float width, int32
  * vgpu_camera matrix); 0 keys go back to the vgpu_camera matrix given at
  * creation.  At most 64 keys. */
 int vgpu_render_set_camera_keys(vgpu_render* render, const float* local_to_world, uint32_t nkeys);
+/* Path mode (no reference counterpart; core.Render and ShaderStd.Eval,
+ * core/render.go:66-137 and builtin/shader/std.go:77-296, shade one camera
+ * hit with direct light and recurse only through perfect mirrors).  depth 0
+ * keeps the reference's shading.  depth D in [1,16] traces paths of up to D
+ * segments: every vertex is shaded by ShaderStd as at level 0 of the
+ * reference (EvaluateLightSamples per light and lobe, one sample per light
+ * below the camera vertex) and, before the last segment, continues with one
+ * ray: the diffuse (Oren-Nayar), mirror or GGX lobe is picked with its
+ * normalised weight and sampled with the lobe's own Sample; the continuation
+ * carries (Eval / pdf).ToRGB() x the lobe colour (for GGX, pdf is the density
+ * of the sampled direction), clamped at 0 as the reference clamps its mirror
+ * ray.  Emission of a light's own geometry reached by an Oren-Nayar or GGX
+ * continuation is not added (the light's EvaluateLightSamples at the vertex
+ * before covers it).  QMC: the camera vertex's light samples use the pixel's
+ * scramble0/1 as the reference does; every other level and dimension pair uses
+ * a SplitMix64 hash of them.  Depth 1 gives the reference's pixels for scenes
+ * without mirrors. */
+int vgpu_render_set_path_depth(vgpu_render* render, uint32_t depth);
 #define VGPU_FILTER_BOX 0
 #define VGPU_FILTER_GAUSSIAN 1
 #define VGPU_FILTER_AIRY 2

@@ -226,6 +226,7 @@ def _render_sigs(lib):
         "orc_render_add_light": (None, [P, C.c_int32, P, P, P, C.c_float, C.c_int32, C.c_int32, C.c_int32]),
         "orc_render_set_filter": (None, [P, C.c_int32, C.c_float, C.c_int32, C.c_float]),
         "orc_render_set_lens": (None, [P, C.c_float]),
+        "orc_render_set_path_depth": (None, [P, C.c_int32]),
         "orc_scene_mesh_set_normals": (None, [P, C.c_int32, P, C.c_int32, P]),
         "orc_scene_mesh_set_ray_bias": (None, [P, C.c_int32, C.c_float]),
         "orc_uniform_disk": (None, [C.c_float, C.c_float, C.c_float, P]),
@@ -412,6 +413,11 @@ class OracleRenderer:
         """The camera node's Radius: a thin lens when > 0 (camera.go:265-283)."""
         self.lib.orc_render_set_lens(self.h_r, float(radius))
 
+    def set_path_depth(self, depth):
+        """Path mode (no reference counterpart): paths of up to `depth` segments; 0 keeps the
+        reference's shading."""
+        self.lib.orc_render_set_path_depth(self.h_r, int(depth))
+
     def set_filter(self, kind="box", width=0.0, res=0, peak=0.0):
         t = {"box": 0, "gaussian": 1, "airy": 2}[kind]
         self.lib.orc_render_set_filter(self.h_r, t, float(width), int(res), float(peak))
@@ -507,6 +513,8 @@ def _pin_sigs(lib):
         "orc_ggx_pdf_eval_n": (None, [C.c_float, C.c_float, C.c_float, P, C.c_int64, P, P, P]),
         "orc_ggx_d_n": (None, [P, C.c_int64, C.c_float, P]),
         "orc_ggx_g1_n": (None, [P, P, C.c_int64, C.c_float, P]),
+        "orc_spectrum_rgb_n": (None, [P, C.c_int64, P, P]),
+        "orc_path_lobe_weight_n": (None, [C.c_int32, C.c_float, C.c_float, C.c_float, P, C.c_int64, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -560,6 +568,30 @@ def ggx_pdf_eval(roughness, omega_r, dirs, eta=1.5, lam=550.0):
     w = _v3(omega_r)
     lib.orc_ggx_pdf_eval_n(float(roughness), float(eta), float(lam), _p(w), len(d), _p(d), _p(pdf), _p(ev))
     return pdf, ev
+
+
+def spectrum_rgb(rgb, lambdas):
+    """ToRGB(FromRGB(rgb)) at each hero wavelength (float32 lambdas): (n, 3)."""
+    lib = _pin_sigs(load())
+    lam = np.ascontiguousarray(lambdas, np.float32).reshape(-1)
+    out = np.zeros((len(lam), 3), np.float32)
+    c = _v3(rgb)
+    lib.orc_spectrum_rgb_n(_p(c), len(lam), _p(lam), _p(out))
+    return out
+
+
+def path_lobe_weight(kind, roughness, omega, r0, r1, eta=1.5, lam=550.0):
+    """Path mode's continuation weight Eval/pdf of one lobe (kind 0 Oren-Nayar, 1 GGX) for sample pairs
+    (r0, r1) in the tangent frame: (dirs (n, 3), weights (n, 4))."""
+    lib = _pin_sigs(load())
+    r0 = np.ascontiguousarray(r0, np.float64)
+    r1 = np.ascontiguousarray(r1, np.float64)
+    dirs = np.zeros((len(r0), 3), np.float32)
+    out = np.zeros((len(r0), 4), np.float32)
+    w = _v3(omega)
+    lib.orc_path_lobe_weight_n(int(kind), float(roughness), float(eta), float(lam), _p(w), len(r0), _p(r0), _p(r1),
+                               _p(dirs), _p(out))
+    return dirs, out
 
 
 def ggx_d_n(m, alpha):

@@ -1,7 +1,9 @@
 // ORACLE — test infrastructure only (same scope rules as oracle_accel.cpp).
 //
 // CPU restatement of the reference's per-pixel render/shade loop in its
-// reference semantics (direct lighting with MIS; indirect light only through perfect mirrors):
+// reference semantics (direct lighting with MIS; indirect light only through perfect mirrors),
+// and of path mode (set_path_depth; no reference counterpart): the same shading at every vertex
+// of a path plus one continuation ray per vertex (Renderer::Continue, LevelScramble):
 //   core/render.go:66-137          render: RasterXY / VdC / Sobol sample generation, film update
 //   core/trace.go:40-83            Trace: ShaderContext setup, ApplyTransform, Shader.Eval
 //   core/shader.go:139-402         OffsetP, LightsPrepare, NextLight, EvaluateLightSamples
@@ -481,9 +483,37 @@ struct SC {                    // the subset of core.ShaderContext the shading p
     float Lambda;
     float Time;                // sample time, carried by every ray the hit spawns (core/ray.go:65)
     uint64_t I;
-    uint64_t Scramble[2];
+    uint64_t Scramble[2];      // the scrambles EvaluateLightSamples reads at this vertex
+    uint64_t PixScramble[2];   // the pixel's scramble0/1 (path mode derives its other dimensions from them)
     int32_t shaderIdx;
 };
+
+// Path mode (not in the reference): a continuation ray's origin moved off the surface along the unit
+// geometric normal n by 256 ulps per component, or by 2^-16 n within 1/32 of zero (Waechter and Binder,
+// "A Fast and Robust Method for Avoiding Self-Intersection", Ray Tracing Gems, 2019).  OffsetP's error bound
+// is zero for a surface lying exactly in a coordinate plane through the origin, where a continuation ray
+// would otherwise find its own surface again at t ~ 0.
+static float OffsetAlong1(float p, float n) {
+    const int32_t of = (int32_t)(256.0f * n);
+    int32_t bits;
+    std::memcpy(&bits, &p, 4);
+    bits += p < 0.0f ? -of : of;
+    float pi;
+    std::memcpy(&pi, &bits, 4);
+    return std::fabs(p) < 0.03125f ? p + 1.52587891e-05f * n : pi;
+}
+static Vec3 OffsetAlong(const Vec3& p, const Vec3& n) {
+    return V3(OffsetAlong1(p[0], n[0]), OffsetAlong1(p[1], n[1]), OffsetAlong1(p[2], n[2]));
+}
+
+// Path mode (not in the reference): the scramble of dimension pair k at recursion level `level`, the
+// SplitMix64 finaliser (Steele, Lea, Flood 2014) of a pixel scramble xor a per-(level, k) constant.
+static uint64_t LevelScramble(uint64_t s, int level, int k) {
+    uint64_t z = s ^ ((uint64_t)(8 * level + k + 1) * 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
 
 struct OrenNayar {             // builtin/shader/bsdf/orennayar.go
     float Lambda; Vec3 OmegaI; float Roughness; Vec3 U, V, N;
@@ -592,6 +622,16 @@ struct GGX {                   // builtin/shader/bsdf/microfacetggx.go
         float alpha = sqr32(Roughness);
         Vec3 m = Scale(signf(OmegaR[2]), Normalize(Add(OmegaR, o)));
         double pdf = (double)(ggxD(m, alpha) * m[2]);
+        if (std::isnan(pdf)) return 0;
+        return pdf;
+    }
+    // Path mode: density of the direction Sample returns.  PDF above is the half vector's density
+    // D(m) m_z (microfacetggx.go:108-122); the reflection about m divides it by 4 |omegaR . m|.
+    double PDFDir(const Vec3& o_) const {
+        Vec3 o = BasisProject(U, V, N, o_);
+        float alpha = sqr32(Roughness);
+        Vec3 m = Scale(signf(OmegaR[2]), Normalize(Add(OmegaR, o)));
+        double pdf = (double)(ggxD(m, alpha) * m[2] / (4.0f * DotAbs(OmegaR, m)));
         if (std::isnan(pdf)) return 0;
         return pdf;
     }
@@ -762,6 +802,7 @@ struct Renderer {
     float lensRadius = 0;         // camera Radius (thin lens when > 0)
     std::vector<std::vector<int32_t>> geomShaders;   // per post-sort geom slot: material id per shader index
     int shadowRays = 0;
+    int pathDepth = 0;            // path mode (not in the reference): segments per path; 0: the reference's shading
 
     // EvalEmission (std.go:299-316)
     RGB Emission(const Material& m) const {
@@ -1032,8 +1073,8 @@ struct Renderer {
     }
 
     // ShaderStd.Eval (std.go:77-296) at recursion level `level`
-    RGB Shade(Task* tk, const SC& sg, const Material& sh, int level) {
-        if (level > 3) return RGB();
+    RGB Shade(Task* tk, const SC& sg, const Material& sh, int level, bool smooth) {
+        if (pathDepth == 0 && level > 3) return RGB();
         Vec3 V = Cross(sg.N, sg.DdPdu);
         if (Length2(V) < 0.1f) V = Cross(sg.N, sg.DdPdv);
         V = Normalize(V);
@@ -1065,7 +1106,7 @@ struct Renderer {
             float rough = sh.has_spec1_roughness ? sh.spec1_roughness : 0.5f;
             RGB specColour;
             if (sh.has_spec1_colour) for (int c = 0; c < 3; c++) specColour.c[c] = sh.spec1[c];
-            if (rough == 0.0f) {
+            if (rough == 0.0f && pathDepth == 0) {
                 // perfect mirror (std.go:209-261): one reflected ray, recursively shaded at level + 1
                 Specular spec{sg.Lambda, BasisProject(U, V, sg.N, Neg(sg.Rd)), fres, U, V, sg.N};
                 Vec3 o = spec.Sample();
@@ -1099,15 +1140,81 @@ struct Renderer {
             }
         }
         RGB contrib;
-        contrib.Add(Emission(sh));
+        RGB em = Emission(sh);
+        if (pathDepth > 0 && level > 0 && smooth) {
+            // path mode: a light's own geometry reached by an Oren-Nayar or GGX sample was covered by the
+            // previous vertex's EvaluateLightSamples
+            for (const auto& L : lights) if (L.geom == sg.geom) em = RGB();
+        }
+        contrib.Add(em);
         contrib.Add(diffContrib);
         contrib.Add(specContrib);
+        if (pathDepth > 0 && level + 1 < pathDepth) contrib.Add(Continue(tk, sg, sh, level, dw, sw, fres));
         return contrib;
+    }
+
+    // Path mode's continuation at a vertex (not in the reference): the diffuse, mirror or GGX lobe picked
+    // with its normalised weight and sampled by the lobe's Sample; the next vertex's colour weighted by
+    // (Eval / pdf).ToRGB() * lobe colour and clamped at 0 as the reference's mirror ray (std.go:226-258).
+    RGB Continue(Task* tk, const SC& sg, const Material& sh, int level, float dw, float sw, const FresnelModel& fres) {
+        Vec3 V = Cross(sg.N, sg.DdPdu);
+        if (Length2(V) < 0.1f) V = Cross(sg.N, sg.DdPdv);
+        V = Normalize(V);
+        Vec3 U = Normalize(Cross(sg.N, V));
+        const Vec3 wr = BasisProject(U, V, sg.N, Neg(sg.Rd));
+        const double u = VanDerCorput(sg.I, LevelScramble(sg.PixScramble[0], level, 2));
+        const double r0 = VanDerCorput(sg.I, LevelScramble(sg.PixScramble[0], level, 3));
+        const double r1 = Sobol(sg.I, LevelScramble(sg.PixScramble[1], level, 4));
+        const float rough = sh.has_spec1_roughness ? sh.spec1_roughness : 0.5f;
+        Vec3 o = V3(0, 0, 0);
+        double pdf = 0;
+        Spectrum rho;
+        RGB colour;
+        bool smooth = true, ok = false;
+        if (u < (double)dw) {
+            float dr = sh.has_diffuse_roughness ? sh.diffuse_roughness : 0.5f;
+            OrenNayar diff{sg.Lambda, wr, dr * dr, U, V, sg.N};
+            o = Normalize(diff.Sample(r0, r1));
+            pdf = diff.PDF(o);
+            rho = diff.Eval(o);
+            if (sh.has_diffuse_colour) for (int c = 0; c < 3; c++) colour.c[c] = sh.diffuse[c];
+            ok = true;
+        } else if (sw > 0.0f && rough == 0.0f) {
+            Specular spec{sg.Lambda, wr, fres, U, V, sg.N};
+            o = spec.Sample();
+            pdf = spec.PDF(o);
+            rho = spec.Eval(o);
+            if (sh.has_spec1_colour) for (int c = 0; c < 3; c++) colour.c[c] = sh.spec1[c];
+            smooth = false;
+            ok = true;
+        } else if (sw > 0.0f && rough > 0.0f) {
+            GGX ggx{sg.Lambda, wr, rough * rough, fres, U, V, sg.N};
+            o = Normalize(ggx.Sample(r0, r1));
+            pdf = ggx.PDFDir(o);
+            rho = ggx.Eval(o);
+            if (sh.has_spec1_colour) for (int c = 0; c < 3; c++) colour.c[c] = sh.spec1[c];
+            ok = true;
+        }
+        RGB out;
+        if (!ok || !(pdf > 0) || !(Dot(o, sg.Ng) > 0.0f)) return out;
+        Ray r{};
+        r.P = OffsetAlong(OffsetP(sg, 1), sg.Ng); r.D = o; r.Tclosest = kInf; r.Type = RayTypeCamera; r.Time = sg.Time;
+        RaySetup(r);
+        bool hit;
+        RGB next = TraceShade(tk, r, level + 1, sg.Lambda, sg.I, sg.PixScramble, hit, smooth);
+        if (!hit) return out;
+        rho.Scale(1.0f / (float)pdf);
+        RGB col = rho.ToRGB();
+        col.Mul(colour);
+        col.Mul(next);
+        for (int k = 0; k < 3; k++) if (col.c[k] < 0 || std::isnan(col.c[k])) col.c[k] = 0;
+        return col;
     }
 
     // core.Trace (trace.go:40-83): closest hit, then the hit's shader; hit = false on a miss or a geom
     // without a shader.
-    RGB TraceShade(Task* tk, Ray& r, int level, float lambda, uint64_t I, const uint64_t* scramble, bool& hit) {
+    RGB TraceShade(Task* tk, Ray& r, int level, float lambda, uint64_t I, const uint64_t* scramble, bool& hit,
+                   bool smooth = false) {
         tk->StackTop = 0; r.task = tk;
         SceneHit sh{}; sh.geom = -1; sh.hr.elem = -1;
         RGB out;
@@ -1116,6 +1223,11 @@ struct Renderer {
         const HitRecord& h = sh.hr;
         SC sc;
         sc.geom = sh.geom; sc.Lambda = lambda; sc.Time = r.Time; sc.I = I; sc.Scramble[0] = scramble[0]; sc.Scramble[1] = scramble[1];
+        sc.PixScramble[0] = scramble[0]; sc.PixScramble[1] = scramble[1];
+        if (pathDepth > 0 && level > 0) {
+            sc.Scramble[0] = LevelScramble(scramble[0], level, 0);
+            sc.Scramble[1] = LevelScramble(scramble[1], level, 1);
+        }
         sc.Rd = r.D;
         // ApplyTransform with identity matrices (core/shader.go:129-135)
         sc.P = h.Po;
@@ -1125,7 +1237,7 @@ struct Renderer {
         const auto& shaders = geomShaders[sh.geom];
         int mat = shaders.empty() ? -1 : shaders[h.shaderIdx];
         if (mat < 0) { hit = false; return out; }
-        return Shade(tk, sc, mats[mat], level);
+        return Shade(tk, sc, mats[mat], level, smooth);
     }
 
     // Sampler.WarpSample (builtin/filter/filter.go:40-85)
@@ -1389,6 +1501,7 @@ void* orc_render_new(void* sp, int32_t w, int32_t h, const float* M, float tan_t
 void orc_render_free(void* r) { delete (Renderer*)r; }
 // the camera node's Radius (camera.go:265-283): a thin lens when > 0
 void orc_render_set_lens(void* rp, float radius) { ((Renderer*)rp)->lensRadius = radius; }
+void orc_render_set_path_depth(void* rp, int32_t depth) { ((Renderer*)rp)->pathDepth = depth; }
 // sample.UniformDisk2D (math/sample/sample.go:105-129) for known-answer tests
 void orc_uniform_disk(float radius, float r0, float r1, float* xy) {
     float x = -1 + 2 * r0, y = -1 + 2 * r1, rr = 0, theta = 0;
@@ -1625,6 +1738,40 @@ float orc_ggx_g1(const float* w, const float* m, float alpha) {
 }
 
 extern "C" {
+// ToRGB(FromRGB(rgb)) at each hero wavelength lambdas[i] (colour/spectrum.go): the RGB factor a white
+// (or any) reflectance contributes at one bounce of a sample at that wavelength.
+void orc_spectrum_rgb_n(const float* rgb, int64_t n, const float* lambdas, float* out) {
+    RGB c;
+    for (int k = 0; k < 3; k++) c.c[k] = rgb[k];
+    for (int64_t i = 0; i < n; i++) {
+        Spectrum s;
+        s.Lambda = lambdas[i];
+        s.FromRGB(c);
+        RGB o = s.ToRGB();
+        for (int k = 0; k < 3; k++) out[i * 3 + k] = o.c[k];
+    }
+}
+
+// Path mode's continuation weight Eval(o) / pdf(o) (4 spectral values) of one lobe in the tangent frame
+// (U, V, N = x, y, z) for n (r0, r1) pairs, as Renderer::Continue forms it: kind 0 Oren-Nayar (field
+// Roughness = sigma) sampled by Sample with PDF; kind 1 GGX (field Roughness, alpha = Roughness^2,
+// dielectric Fresnel eta) sampled by Sample with PDFDir.  0 where the pdf is 0 or o is below the surface.
+void orc_path_lobe_weight_n(int32_t kind, float roughness, float eta, float lambda, const float* omega, int64_t n,
+                            const double* r0, const double* r1, float* dirs, float* out4) {
+    const Vec3 w = V3(omega[0], omega[1], omega[2]);
+    FresnelModel f;
+    f.eta = eta;
+    OrenNayar on{lambda, w, roughness, V3(1, 0, 0), V3(0, 1, 0), V3(0, 0, 1)};
+    GGX g{lambda, w, roughness, f, V3(1, 0, 0), V3(0, 1, 0), V3(0, 0, 1)};
+    for (int64_t i = 0; i < n; i++) {
+        Vec3 o = Normalize(kind == 0 ? on.Sample(r0[i], r1[i]) : g.Sample(r0[i], r1[i]));
+        double pdf = kind == 0 ? on.PDF(o) : g.PDFDir(o);
+        Spectrum e = kind == 0 ? on.Eval(o) : g.Eval(o);
+        for (int k = 0; k < 3; k++) dirs[i * 3 + k] = o[k];
+        for (int k = 0; k < 4; k++) out4[i * 4 + k] = (pdf > 0 && o[2] > 0) ? e.C[k] * (1.0f / (float)pdf) : 0.0f;
+    }
+}
+
 // The GGX D and Smith G1 terms (microfacetggx.go:26-80) over n directions.
 void orc_ggx_d_n(const float* m, int64_t n, float alpha, float* out) {
     for (int64_t i = 0; i < n; i++) out[i] = ggxD(V3(m[i * 3], m[i * 3 + 1], m[i * 3 + 2]), alpha);

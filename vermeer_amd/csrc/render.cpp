@@ -376,6 +376,7 @@ struct vgpu_render {
     void* d_stage = nullptr;       // host-pointer mode: scrambles, film, samples
     size_t stage_bytes = 0;
     vgpu_detail::GraphCache graph; // VGPU_GRAPH passes
+    bool any_mirror = false;       // a material has a perfect-mirror lobe (the reference recurses to level 3)
 
     ~vgpu_render() {
         int prev = 0;
@@ -459,6 +460,9 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     rv.filt_n = 0;
     rv.filt_w = 0.0;
     rv.max_level = any_mirror ? vgpu::kMaxLevel : 0;
+    rv.path_depth = 0;
+    rv.rec_levels = vgpu::kMaxLevel + 1;
+    r->any_mirror = any_mirror;
     if (!raster_tables(12, rv.raster_fwd, 52, rv.raster_inv)) { delete r; return fail(VGPU_ERR_ARG, "RasterXY tables"); }
     std::lock_guard<std::recursive_mutex> lk(s->mu);
     DeviceGuard dg(s->device);
@@ -547,14 +551,16 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
             e = vgpu::launch_shade(r->rv, d_scr, iter, level, rin, r->d_hits32, r->d_pix, r->d_slot_rays, r->d_slot_rgb,
                                    r->d_groups, r->d_pixshade, rout, r->d_recs, st);
             if (e != hipSuccess) return fail(VGPU_ERR_HIP, "shade: %s", hipGetErrorString(e));
-            int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns, r->d_slot_hits, nullptr,
-                                   nullptr, st, s->cfg);
-            if (rc) return rc;
+            if (r->rv.groups > 0) {   // without lights no slot holds a shadow ray
+                const int rc = enqueue_trace(s, VGPU_TRACE_SHADOW | VGPU_TRACE_DEVICE_PTRS, r->d_slot_rays, ns,
+                                             r->d_slot_hits, nullptr, nullptr, st, s->cfg);
+                if (rc) return rc;
+            }
             const bool direct = r->rv.max_level == 0;
             e = vgpu::launch_resolve(r->rv, level, iter, r->d_slot_rgb, r->d_slot_hits, r->d_groups, r->d_pixshade, r->d_recs,
                                      direct ? d_film : nullptr, direct ? d_samp : nullptr, st);
             if (e != hipSuccess) return fail(VGPU_ERR_HIP, "resolve: %s", hipGetErrorString(e));
-            if (host && shadow_rays_out) {
+            if (host && shadow_rays_out && r->rv.groups > 0) {
                 // count active slots (diagnostic, host mode only)
                 std::vector<float4> rgb(ns);
                 HIP_TRY(hipMemcpyAsync(rgb.data(), r->d_slot_rgb, ns * 16, hipMemcpyDeviceToHost, st));
@@ -579,6 +585,28 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
         if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
         if (shadow_rays_out) *shadow_rays_out = shadow_count;
     }
+    return VGPU_OK;
+}
+
+int vgpu_render_set_path_depth(vgpu_render* r, uint32_t depth) {
+    if (!r) return fail(VGPU_ERR_ARG, "render is NULL");
+    if (depth > (uint32_t)vgpu::kMaxPathDepth)
+        return fail(VGPU_ERR_ARG, "path depth %u outside [0,%d]", depth, vgpu::kMaxPathDepth);
+    vgpu_scene* s = r->scene;
+    std::lock_guard<std::recursive_mutex> lk(s->mu);
+    DeviceGuard dg(s->device);
+    const size_t np = (size_t)r->width * r->height;
+    const int levels = std::max(vgpu::kMaxLevel + 1, (int)depth);
+    if (levels > r->rv.rec_levels) {
+        vgpu::LevelRec* recs = nullptr;
+        HIP_TRY(hipMalloc((void**)&recs, np * sizeof(vgpu::LevelRec) * (size_t)levels));
+        if (r->d_recs) (void)hipFree(r->d_recs);
+        r->d_recs = recs;
+        r->rv.rec_levels = levels;
+    }
+    if (depth > 1 && !r->d_rays2) HIP_TRY(hipMalloc((void**)&r->d_rays2, np * 32));
+    r->rv.path_depth = (int32_t)depth;
+    r->rv.max_level = depth > 0 ? (int32_t)depth - 1 : (r->any_mirror ? vgpu::kMaxLevel : 0);
     return VGPU_OK;
 }
 

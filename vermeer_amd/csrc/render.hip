@@ -237,6 +237,17 @@ __device__ __forceinline__ uint64_t sobol_bits(uint64_t i, uint64_t scramble) {
 }
 __device__ __forceinline__ double sobol(uint64_t i, uint64_t s) { return (double)sobol_bits(i, s) / 4503599627370496.0; }
 
+// Path mode: the scramble of QMC dimension pair k at recursion level `level`, derived from one of the
+// pixel's two shading scrambles by the SplitMix64 finaliser (Steele, Lea, Flood 2014).  Level 0's light
+// samples keep the pixel's own scrambles, as the reference's renderer uses them; every other (level, k)
+// gets an independent random XOR scramble, so the dimensions of different bounces are not correlated.
+__device__ __forceinline__ uint64_t level_scramble(uint64_t s, int level, int k) {
+    uint64_t z = s ^ ((uint64_t)(8 * level + k + 1) * 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
 // ------------------------------------------------------------- BSDFs
 
 struct Frame { V3 U, V, N; };
@@ -340,6 +351,16 @@ struct GGX {
         const float alpha = sqr32(rough);
         const V3 m = scale(signf(wr.z), normalize(add(wr, o)));
         const double p = (double)(ggx_d(m, alpha) * m.z);
+        return __builtin_isnan(p) ? 0.0 : p;
+    }
+    // Density of the direction sample() returns: pdf() is the half vector's density D(m) m_z
+    // (microfacetggx.go:108-122); reflecting about m scales it by 1 / (4 |wr . m|).  Path mode weighs
+    // its GGX continuation with this (the reference never continues a GGX sample).
+    __device__ double pdf_dir(V3 o_) const {
+        const V3 o = basis_project(f.U, f.V, f.N, o_);
+        const float alpha = sqr32(rough);
+        const V3 m = scale(signf(wr.z), normalize(add(wr, o)));
+        const double p = (double)(ggx_d(m, alpha) * m.z / (4.0f * dotabs(wr, m)));
         return __builtin_isnan(p) ? 0.0 : p;
     }
     __device__ Spec eval(V3 o_) const {
@@ -924,6 +945,19 @@ struct Specular {
     }
 };
 
+// Path mode: a continuation ray's origin moved off the surface along the unit geometric normal n by 256
+// ulps per component, or by 2^-16 n within 1/32 of zero (Waechter and Binder, Ray Tracing Gems, 2019).
+// offset_p's error bound is zero for a surface lying exactly in a coordinate plane through the origin,
+// where a continuation ray would otherwise find its own surface again at t ~ 0.
+__device__ __forceinline__ float offset_along1(float p, float n) {
+    const int of = (int)(256.0f * n);
+    const float pi = __int_as_float(__float_as_int(p) + (p < 0.0f ? -of : of));
+    return fabs32(p) < 0.03125f ? p + 1.52587891e-05f * n : pi;
+}
+__device__ __forceinline__ V3 offset_along(V3 p, V3 n) {
+    return v3(offset_along1(p.x, n.x), offset_along1(p.y, n.y), offset_along1(p.z, n.z));
+}
+
 __device__ __forceinline__ void ray_off(float4* rays, int p) {
     rays[2 * p] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     rays[2 * p + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
@@ -943,7 +977,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     SlotOut so{slot_rays + (size_t)p * rv.slots * 2, slot_rgb + (size_t)p * rv.slots, rb.w};
     float4* grp = groups + (size_t)p * rv.groups;
     float4* ps = pixshade + (size_t)p * 4;
-    LevelRec* rec = recs + (size_t)p * (kMaxLevel + 1) + level;
+    LevelRec* rec = recs + (size_t)p * rv.rec_levels + level;
     const int4 h0 = hits32[2 * p], h1 = hits32[2 * p + 1];
     const int prim = h0.x, g = h0.y;
     int mat = -1;
@@ -955,12 +989,14 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
         const int sidx = gs.sidx_base >= 0 ? (int)rv.shaderidx[gs.sidx_base + prim] : 0;
         mat = gs.nshaders > 0 ? rv.shaders[gs.shader_base + sidx] : -1;
     }
+    const bool path = rv.path_depth > 0;
     if (mat < 0) {   // a miss, or a geom without a shader: core.Trace reports no hit (trace.go:62-64)
         for (int s = 0; s < rv.slots; s++) slot_off(so, s);
         for (int q = 0; q < rv.groups; q++) grp[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         ps[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         rec->a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         rec->s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (path) rec->c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (rays_out) ray_off(rays_out, p);
         return;
     }
@@ -1051,8 +1087,9 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     dPdv = normalize(dPdv);
     sc.lambda = pix[p].w;
     sc.I = iter;
-    sc.scr0 = scr[(size_t)p * 6 + 4];
-    sc.scr1 = scr[(size_t)p * 6 + 5];
+    const uint64_t scr4 = scr[(size_t)p * 6 + 4], scr5 = scr[(size_t)p * 6 + 5];
+    sc.scr0 = (path && level > 0) ? level_scramble(scr4, level, 0) : scr4;
+    sc.scr1 = (path && level > 0) ? level_scramble(scr5, level, 1) : scr5;
     const vgpu_material mt = rv.materials[mat];
     // ShaderStd.Eval (std.go:99-285)
     V3 Vv = cross(sc.N, dPdu);
@@ -1084,7 +1121,12 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
         if (!skip && spec_on) grp[2 * l + 1] = light_samples(L, lm, sc, gx, NS, so, b1, SGS);
         else { for (int s = 0; s < SGS; s++) slot_off(so, b1 + s); grp[2 * l + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
     }
-    const RGB e = emission(mt);
+    RGB e = emission(mt);
+    if (path && level > 0 && rec[-1].c.w == kPathSmooth) {
+        // Reached by an Oren-Nayar or GGX sample: a light's own geometry was sampled at the previous
+        // vertex by EvaluateLightSamples (with its BSDF samples), so its emission is not added twice.
+        for (int l = 0; l < rv.nlights; l++) if (rv.lights[l].geom == g) e = rgb0();
+    }
     ps[0] = make_float4(e.r, e.g, e.b, 1.0f);
     ps[1] = mt.has_diffuse_colour ? make_float4(mt.diffuse[0], mt.diffuse[1], mt.diffuse[2], dw)
                                   : make_float4(0.0f, 0.0f, 0.0f, dw);
@@ -1096,7 +1138,53 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     float kind = 1.0f;
     float4 srec = make_float4(0.0f, 0.0f, 0.0f, sw);
     bool traced = false;
-    if (mirror) {
+    if (path) {
+        // Path mode: one continuation ray per vertex.  The lobe is picked with probability equal to its
+        // normalised weight (diffuse dw, specular sw), so the weight over the pick probability is 1 and
+        // the factor is (Eval / pdf).ToRGB() * colour, as the reference weighs its mirror ray
+        // (std.go:226-258).  Not traced at the last segment, below the geometric surface, or at pdf 0.
+        float4 crec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (level + 1 < rv.path_depth && rays_out) {
+            const double u = vdc(iter, level_scramble(scr4, level, 2));
+            const double r0 = vdc(iter, level_scramble(scr4, level, 3)), r1 = sobol(iter, level_scramble(scr5, level, 4));
+            V3 o = v3(0.0f, 0.0f, 0.0f);
+            double pdf = 0.0;
+            Spec rho;
+            RGB col = rgb0();
+            float flag = 0.0f;
+            if (u < (double)dw) {
+                o = normalize(on.sample(r0, r1));
+                pdf = on.pdf(o);
+                rho = on.eval(o);
+                if (mt.has_diffuse_colour) col = RGB{mt.diffuse[0], mt.diffuse[1], mt.diffuse[2]};
+                flag = kPathSmooth;
+            } else if (mirror) {
+                const Specular spc{sc.lambda, wr, fres, fr};
+                o = spc.sample();
+                pdf = spc.pdf(o);
+                rho = spc.eval(o);
+                if (mt.has_spec1_colour) col = RGB{mt.spec1[0], mt.spec1[1], mt.spec1[2]};
+                flag = kPathDelta;
+            } else if (spec_on) {
+                o = normalize(gx.sample(r0, r1));
+                pdf = gx.pdf_dir(o);
+                rho = gx.eval(o);
+                if (mt.has_spec1_colour) col = RGB{mt.spec1[0], mt.spec1[1], mt.spec1[2]};
+                flag = kPathSmooth;
+            }
+            if (flag != 0.0f && pdf > 0.0 && dot(o, sc.Ng) > 0.0f) {
+                spec_scale(rho, 1.0f / (float)pdf);
+                RGB f = spec_to_rgb(rho);
+                rgb_mul(f, col);
+                crec = make_float4(f.r, f.g, f.b, flag);
+                const V3 po = offset_along(offset_p(sc, 1), sc.Ng);
+                rays_out[2 * p] = make_float4(po.x, po.y, po.z, __builtin_inff());
+                rays_out[2 * p + 1] = make_float4(o.x, o.y, o.z, rb.w);
+                traced = true;
+            }
+        }
+        rec->c = crec;
+    } else if (mirror) {
         kind = 3.0f;
         const Specular spc{sc.lambda, wr, fres, fr};
         const V3 o = spc.sample();
@@ -1148,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_resolve(RenderView rv, int level, uint3
         if (film) film_update(film, sample_out, p, iter, rgb0());
         return;
     }
-    LevelRec* rec = recs + (size_t)p * (kMaxLevel + 1) + level;
+    LevelRec* rec = recs + (size_t)p * rv.rec_levels + level;
     const int SGS = 2 * rv.max_half;
     const float4 dc = ps[1], spc = ps[2], fl = ps[3];
     const float4* rgb = slot_rgb + (size_t)p * rv.slots;
@@ -1195,9 +1283,30 @@ __global__ __launch_bounds__(256) void k_combine(RenderView rv, uint32_t iter, c
                                                  float* __restrict__ film, float* __restrict__ sample_out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= rv.width * rv.height) return;
-    const LevelRec* r = recs + (size_t)p * (kMaxLevel + 1);
+    const LevelRec* r = recs + (size_t)p * rv.rec_levels;
     RGB out = rgb0();
     bool next_hit = false;
+    if (rv.path_depth > 0) {
+        // path mode: ((0 + emission + diffuse) + specular direct) + max(0, f * next level's colour)
+        for (int L = rv.max_level; L >= 0; L--) {
+            const float4 a = r[L].a, sv = r[L].s, c = r[L].c;
+            if (a.w == 0.0f) { next_hit = false; continue; }
+            RGB o{a.x, a.y, a.z};
+            rgb_add(o, RGB{sv.x, sv.y, sv.z});
+            if (c.w != 0.0f && next_hit) {
+                RGB col{c.x, c.y, c.z};
+                rgb_mul(col, out);
+                if (col.r < 0 || __builtin_isnan(col.r)) col.r = 0;
+                if (col.g < 0 || __builtin_isnan(col.g)) col.g = 0;
+                if (col.b < 0 || __builtin_isnan(col.b)) col.b = 0;
+                rgb_add(o, col);
+            }
+            out = o;
+            next_hit = true;
+        }
+        film_update(film, sample_out, p, iter, (r[0].a.w != 0.0f) ? out : rgb0());
+        return;
+    }
     for (int L = rv.max_level; L >= 0; L--) {
         const float4 a = r[L].a, sv = r[L].s;
         if (a.w == 0.0f) { next_hit = false; continue; }

@@ -31,7 +31,11 @@ struct alignas(16) DevLight {
 struct alignas(16) LevelRec {
     float4 a;                  // xyz: (0 + emission) + diffuse; w: kind (0 miss, 1 rough/no spec, 2 mirror ray traced, 3 mirror not traced)
     float4 s;                  // kind 1: specular direct lighting; kind 2: mirror factor rho.ToRGB() * Spec1Colour; w: spec weight
+    float4 c;                  // path mode: continuation factor (Eval / pdf).ToRGB() * lobe colour; w: 0 none traced,
+                               // kPathDelta (perfect mirror) or kPathSmooth (Oren-Nayar or GGX sample)
 };
+constexpr float kPathDelta = 2.0f;
+constexpr float kPathSmooth = 4.0f;
 
 struct RenderView {
     SceneView sv;
@@ -55,7 +59,9 @@ struct RenderView {
     uint64_t raster_inv[24];
     const double* filt;            // pixel filter sampler: cdfV (filt_n), then cdfVU (filt_n x filt_n); null: box
     int32_t filt_n;
-    int32_t max_level;             // deepest recursion level shaded (0 without mirrors, else 3)
+    int32_t max_level;             // deepest recursion level shaded (0 without mirrors, else 3; path mode: depth - 1)
+    int32_t path_depth;            // 0: the reference's shading; D > 0: path mode with D segments per path
+    int32_t rec_levels;            // LevelRec entries per pixel
     double filt_w;
 };
 
@@ -64,6 +70,7 @@ constexpr int kCamKeyFloats = 24;
 constexpr int kMaxCamKeys = 64;
 
 constexpr int kMaxLevel = 3;       // ShaderStd.Eval returns at once above level 3 (std.go:97-99)
+constexpr int kMaxPathDepth = 16;  // path mode: segments per path
 
 // Launch wrappers (all enqueue on `stream`).  Levels: rays_in holds the level's rays (P, tmax; D,
 // time), rays_out receives the next level's mirror rays (inactive pixels get tmax -1).

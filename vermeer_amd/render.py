@@ -141,6 +141,11 @@ class Renderer:
         except Exception:
             pass
 
+    def set_path_depth(self, depth):
+        """Path mode: paths of up to `depth` segments shaded by ShaderStd at every vertex, one
+        continuation ray per vertex (vgpu_render_set_path_depth); 0 keeps the reference's shading."""
+        check(self._lib.vgpu_render_set_path_depth(self._h, int(depth)))
+
     def set_filter(self, kind="box", width=0.0, res=0, peak=0.0):
         """Pixel filter: "box" (none), "gaussian" or "airy"; zeros take the node defaults."""
         t = {"box": _lib.VGPU_FILTER_BOX, "gaussian": _lib.VGPU_FILTER_GAUSSIAN, "airy": _lib.VGPU_FILTER_AIRY}[kind]
