@@ -214,6 +214,31 @@ def other_workloads(torch, np):
     R.close()
     log(f"[bench] box+spheres renderer: {out['box_spheres_render']}")
 
+    # the same scene and passes in path mode at max depth 5: ShaderStd at every vertex, one continuation per
+    # vertex; rays counted on the device (camera + continuation closest hits, shadow rays)
+    R = render.Renderer(desc, W, W, cam)
+    R.set_path_depth(5)
+    film.zero_()
+    R.render_pass_device(scr.data_ptr(), 1, film.data_ptr(), stream=st)
+    c0, s0 = R.stats()
+    film.zero_()
+    e0.record()
+    for it in range(1, 17):
+        R.render_pass_device(scr.data_ptr(), it, film.data_ptr(), stream=st, graph=True)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    R.scene.status(st)
+    c1, s1 = R.stats()
+    nrays = (c1 - c0) + (s1 - s0)
+    out["box_spheres_path_depth5"] = {"image": [W, W], "passes": 16, "max_depth": 5, "ms": round(ms, 2),
+                                      "samples_per_s": round(W * W * 16 / (ms * 1e-3), 1),
+                                      "closest_rays": int(c1 - c0), "shadow_rays": int(s1 - s0),
+                                      "mrays_per_s": round(nrays / (ms * 1e-3) / 1e6, 1),
+                                      "rmse_vs_converged": _converged_rmse(torch, R, scr, W, 16, 1024)}
+    R.close()
+    log(f"[bench] box+spheres path mode depth 5: {out['box_spheres_path_depth5']}")
+
     # the same renderer on the feature scene: conductor and perfect-mirror spheres (four levels per pass),
     # a disk light, a sphere light, Gaussian pixel filter; 512^2, 16 passes
     desc = scenes.feature_scene()

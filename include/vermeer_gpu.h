@@ -340,6 +340,12 @@ int vgpu_render_set_camera_keys(vgpu_render* render, const float* local_to_world
  * a SplitMix64 hash of them.  Depth 1 gives the reference's pixels for scenes
  * without mirrors. */
 int vgpu_render_set_path_depth(vgpu_render* render, uint32_t depth);
+
+/* Rays traced by the renderer's passes since creation, as core.RenderStats
+ * counts core.TraceProbe calls (core/stats.go:14-45, core/trace.go:26-35):
+ * closest hit (camera rays, mirror rays, path mode's continuations) and
+ * shadow rays.  Synchronises the device. */
+int vgpu_render_stats(vgpu_render* render, uint64_t* closest_rays, uint64_t* shadow_rays);
 #define VGPU_FILTER_BOX 0
 #define VGPU_FILTER_GAUSSIAN 1
 #define VGPU_FILTER_AIRY 2

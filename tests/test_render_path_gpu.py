@@ -51,6 +51,7 @@ def _diff(a, b):
 
 
 def _run(g, o, scr, passes, label):
+    nshadow = 0
     for it in range(1, passes + 1):
         print(f"[case] {label} pass {it}", flush=True)
         ng, gs = g.render_pass(scr, it, want_sample=True)
@@ -58,7 +59,13 @@ def _run(g, o, scr, passes, label):
         bad = _diff(gs, os_)
         assert not bad.any(), f"{label} pass {it}: {int(bad.any(axis=2).sum())} pixels differ, first {np.argwhere(bad)[:3]}"
         assert ng == no, f"{label} pass {it}: shadow rays {ng} vs {no}"
+        nshadow += no
     assert not _diff(g.film, o.film).any()
+    # the device ray counters (vgpu_render_stats) agree with the oracle's shadow rays, and every pass
+    # traced at least one camera ray per pixel
+    closest, shadow = g.stats()
+    assert shadow == nshadow, f"{label}: device shadow rays {shadow} vs {nshadow}"
+    assert closest >= passes * g.width * g.height
 
 
 def test_path_box_spheres_full_size_depth5_bitexact():
@@ -137,6 +144,9 @@ def test_path_depth_switches_on_one_renderer():
             assert not _diff(g.film, o.film).any(), f"depth {depth}"
             films[depth] = g.film.copy()
         assert not _diff(films[0], films[1]).any()
+        # depth 0 and 1 trace exactly one closest-hit ray per pixel and pass (2 + 2 passes), depth 5 more
+        closest, _ = g.stats()
+        assert closest > 6 * w * h
         with pytest.raises(va.VgpuError):
             g.set_path_depth(17)
     finally:

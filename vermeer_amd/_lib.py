@@ -101,6 +101,7 @@ SIGNATURES = [
     ("vgpu_render_create", C.c_int, [C.POINTER(_P), _P, _P, _U32, _P, _U32, _P, _U32, _U32]),
     ("vgpu_render_set_filter", C.c_int, [_P, _I32, _F, _I32, _F]),
     ("vgpu_render_set_path_depth", C.c_int, [_P, C.c_uint32]),
+    ("vgpu_render_stats", C.c_int, [_P, _P, _P]),
     ("vgpu_render_set_camera_keys", C.c_int, [_P, _P, _U32]),
     ("vgpu_render_destroy", C.c_int, [_P]),
     ("vgpu_render_pass", C.c_int, [_P, _U32, _P, _U32, _P, _P, _P, C.POINTER(_U64)]),

@@ -363,6 +363,7 @@ struct vgpu_render {
     float4* d_rays = nullptr;      // camera rays and even levels' mirror rays (2 float4 per pixel)
     float4* d_rays2 = nullptr;     // odd levels' mirror rays
     vgpu::LevelRec* d_recs = nullptr;   // per pixel, per level
+    unsigned long long* d_counts = nullptr;   // rays traced: closest hit, shadow (RenderView::counts)
     double* d_filter = nullptr;    // pixel filter sampler tables
     float* d_camkeys = nullptr;    // camera motion keys (kCamKeyFloats each)
     float cam_M[16] = {};          // the vgpu_camera matrix given at creation
@@ -382,7 +383,7 @@ struct vgpu_render {
         int prev = 0;
         if (hipGetDevice(&prev) == hipSuccess && scene) (void)hipSetDevice(scene->device);
         void* ptrs[] = {d_materials, d_lights, d_rays, d_rays2, d_recs, d_filter, d_camkeys, d_pix, d_hits32, d_slot_rays,
-                        d_slot_rgb, d_slot_hits, d_groups, d_pixshade, d_stage};
+                        d_slot_rgb, d_slot_hits, d_groups, d_pixshade, d_stage, d_counts};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         (void)hipSetDevice(prev);
     }
@@ -481,6 +482,8 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     if (e == hipSuccess) e = alloc((void**)&r->d_slot_hits, ns * 16);
     if (e == hipSuccess) e = alloc((void**)&r->d_groups, np * (size_t)rv.groups * 16);
     if (e == hipSuccess) e = alloc((void**)&r->d_pixshade, np * 64);
+    if (e == hipSuccess) e = alloc((void**)&r->d_counts, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(r->d_counts, 0, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemcpy(r->d_materials, materials, sizeof(vgpu_material) * nmaterials, hipMemcpyHostToDevice);
     if (e == hipSuccess && nlights) e = hipMemcpy(r->d_lights, dl.data(), sizeof(vgpu::DevLight) * nlights, hipMemcpyHostToDevice);
     if (e != hipSuccess) { delete r; return fail(VGPU_ERR_HIP, "render buffers: %s", hipGetErrorString(e)); }
@@ -490,6 +493,7 @@ int vgpu_render_create(vgpu_render** out, vgpu_scene* s, const vgpu_material* ma
     rv.gshade = s->d_gshade;
     rv.materials = r->d_materials;
     rv.lights = r->d_lights;
+    rv.counts = r->d_counts;
     // size the traversal spill area for the larger (shadow) launch
     vgpu::TraceLaunch L{s->cfg.block, s->cfg.lds_stack, s->cfg.waves_per_cu};
     const vgpu::TraceLaunch ext = s->any_motion ? vgpu::kExtLaunchMotion : vgpu::kExtLaunch;
@@ -585,6 +589,17 @@ int vgpu_render_pass(vgpu_render* r, uint32_t flags, const uint64_t* scrambles, 
         if (err) return fail(VGPU_ERR_STACK, "a ray needed more than %d traversal stack entries", VGPU_MAX_STACK);
         if (shadow_rays_out) *shadow_rays_out = shadow_count;
     }
+    return VGPU_OK;
+}
+
+int vgpu_render_stats(vgpu_render* r, uint64_t* closest_rays, uint64_t* shadow_rays) {
+    if (!r) return fail(VGPU_ERR_ARG, "render is NULL");
+    DeviceGuard dg(r->scene->device);
+    HIP_TRY(hipDeviceSynchronize());
+    unsigned long long c[2] = {0, 0};
+    HIP_TRY(hipMemcpy(c, r->d_counts, sizeof(c), hipMemcpyDeviceToHost));
+    if (closest_rays) *closest_rays = c[0];
+    if (shadow_rays) *shadow_rays = c[1];
     return VGPU_OK;
 }
 

@@ -702,6 +702,7 @@ template <class B>
 __device__ __noinline__ float4 light_samples(const DevLight& L, const vgpu_material& lm, const SC& sc, const B& bsdf, int NS,
                                 const SlotOut& so, int base, int sg_slots) {
     const float k = 1.0f - 0.0001f;   // 1 - ShadowRayEpsilon (core/ray.go:22)
+    int on = 0;                       // shadow rays set up (returned in .z)
     for (int s = 0; s < sg_slots; s++) slot_off(so, base + s);
     if (NS > 1) {
         const int half = NS / 2;
@@ -731,6 +732,7 @@ __device__ __noinline__ float4 light_samples(const DevLight& L, const vgpu_mater
             if (c.g < 0) c.g = 0;
             if (c.b < 0) c.b = 0;
             slot_on(so, base + i, o, scale(l.ldist * k, l.ld), c);
+            on++;
         }
         for (int i = 0; i < half; i++) {
             const uint64_t idx = sc.I * (uint64_t)half + (uint64_t)i;
@@ -750,8 +752,9 @@ __device__ __noinline__ float4 light_samples(const DevLight& L, const vgpu_mater
             if (c.g < 0) c.g = 0;
             if (c.b < 0) c.b = 0;
             slot_on(so, base + half + i, o, scale(b.ldist * k, b.ld), c);
+            on++;
         }
-        return make_float4(1.0f / (float)total, 1.0f, 0.0f, 0.0f);
+        return make_float4(1.0f / (float)total, 1.0f, (float)on, 0.0f);
     }
     for (int i = 0; i < NS; i++) {
         const LSample l = sample_area(L, lm, sc, NS, i);
@@ -761,8 +764,9 @@ __device__ __noinline__ float4 light_samples(const DevLight& L, const vgpu_mater
         spec_mul(rho, l.liu);
         spec_scale(rho, 1.0f / l.pdf);
         slot_on(so, base + i, o, scale(l.ldist * k, l.ld), spec_to_rgb(rho));
+        on++;
     }
-    return make_float4(1.0f, 2.0f, 0.0f, 0.0f);
+    return make_float4(1.0f, 2.0f, (float)on, 0.0f);
 }
 
 // ------------------------------------------------------------- kernels
@@ -858,6 +862,7 @@ __device__ void camera_key_matrix(const RenderView& rv, float time, float* M) {
 __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter,
                                                 float4* __restrict__ rays, float4* __restrict__ pix) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && rv.counts) atomicAdd(rv.counts, (unsigned long long)rv.width * (unsigned long long)rv.height);
     if (p >= rv.width * rv.height) return;
     const uint32_t x = (uint32_t)(p % rv.width), y = (uint32_t)(p / rv.width);
     const uint64_t* s = scr + (size_t)p * 6;
@@ -963,15 +968,18 @@ __device__ __forceinline__ void ray_off(float4* rays, int p) {
     rays[2 * p + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
 }
 
-// One recursion level of core.Trace + ShaderStd.Eval for every pixel whose level-`level` ray hit.
-__global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter, int level,
+// One recursion level of core.Trace + ShaderStd.Eval for one pixel whose level-`level` ray hit; returns the
+// shadow rays it sets up, and in `next` the rays it sends to the next level (a mirror ray or path mode's
+// continuation: 0 or 1).
+__device__ __forceinline__ int shade_pixel(int& next, const RenderView& rv, const uint64_t* __restrict__ scr, uint32_t iter, int level,
                                                const float4* __restrict__ rays_in, const int4* __restrict__ hits32,
                                                const float4* __restrict__ pix, float4* __restrict__ slot_rays,
                                                float4* __restrict__ slot_rgb, float4* __restrict__ groups,
                                                float4* __restrict__ pixshade, float4* __restrict__ rays_out,
                                                LevelRec* __restrict__ recs) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= rv.width * rv.height) return;
+    next = 0;
+    if (p >= rv.width * rv.height) return 0;
     const int SGS = 2 * rv.max_half;   // slots per group
     const float4 ra = rays_in[2 * p], rb = rays_in[2 * p + 1];
     SlotOut so{slot_rays + (size_t)p * rv.slots * 2, slot_rgb + (size_t)p * rv.slots, rb.w};
@@ -998,7 +1006,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
         rec->s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (path) rec->c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (rays_out) ray_off(rays_out, p);
-        return;
+        return 0;
     }
     const V3 Ro = v3(ra.x, ra.y, ra.z), Rd = v3(rb.x, rb.y, rb.z);
     const float thit = __int_as_float(h0.z);
@@ -1110,16 +1118,21 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     const V3 wr = basis_project(Uu, Vv, sc.N, neg(Rd));
     const OrenNayar on{sc.lambda, wr, drough * drough, fr};
     const GGX gx{sc.lambda, wr, srough * srough, fres, fr};
+    int nrays = 0;
     for (int l = 0; l < rv.nlights; l++) {
         const DevLight L = rv.lights[l];
         const int NS = level == 0 ? 1 << L.samples : 1;   // NextLight: one sample per light below level 0 (shader.go:186-191)
         const bool skip = L.geom == g;   // LightsPrepare drops the light owning the hit geom (scene.go:100-116)
         const vgpu_material lm = rv.materials[L.material];
         const int b0 = (2 * l) * SGS, b1 = (2 * l + 1) * SGS;
-        if (!skip && diff_on) grp[2 * l] = light_samples(L, lm, sc, on, NS, so, b0, SGS);
-        else { for (int s = 0; s < SGS; s++) slot_off(so, b0 + s); grp[2 * l] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
-        if (!skip && spec_on) grp[2 * l + 1] = light_samples(L, lm, sc, gx, NS, so, b1, SGS);
-        else { for (int s = 0; s < SGS; s++) slot_off(so, b1 + s); grp[2 * l + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
+        float4 g0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g1 = g0;
+        if (!skip && diff_on) g0 = light_samples(L, lm, sc, on, NS, so, b0, SGS);
+        else for (int s = 0; s < SGS; s++) slot_off(so, b0 + s);
+        if (!skip && spec_on) g1 = light_samples(L, lm, sc, gx, NS, so, b1, SGS);
+        else for (int s = 0; s < SGS; s++) slot_off(so, b1 + s);
+        grp[2 * l] = g0;
+        grp[2 * l + 1] = g1;
+        nrays += (int)g0.z + (int)g1.z;
     }
     RGB e = emission(mt);
     if (path && level > 0 && rec[-1].c.w == kPathSmooth) {
@@ -1206,7 +1219,32 @@ __global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __
     if (rays_out && !traced) ray_off(rays_out, p);
     rec->a = make_float4(0.0f, 0.0f, 0.0f, kind);
     rec->s = srec;
+    next = traced ? 1 : 0;
+    return nrays;
 }
+
+// Adds every lane's count to *counter with one atomic per wavefront (all 64 lanes call it).
+__device__ __forceinline__ void wave_count(unsigned long long* counter, int n) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) n += __shfl_xor(n, d);
+    if ((threadIdx.x & 63) == 0 && n != 0) atomicAdd(counter, (unsigned long long)n);
+}
+
+__global__ __launch_bounds__(128) void k_shade(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter, int level,
+                                               const float4* __restrict__ rays_in, const int4* __restrict__ hits32,
+                                               const float4* __restrict__ pix, float4* __restrict__ slot_rays,
+                                               float4* __restrict__ slot_rgb, float4* __restrict__ groups,
+                                               float4* __restrict__ pixshade, float4* __restrict__ rays_out,
+                                               LevelRec* __restrict__ recs) {
+    int next;
+    const int nshadow = shade_pixel(next, rv, scr, iter, level, rays_in, hits32, pix, slot_rays, slot_rgb, groups, pixshade,
+                                    rays_out, recs);
+    if (rv.counts) {
+        wave_count(rv.counts, next);
+        wave_count(rv.counts + 1, nshadow);
+    }
+}
+
 
 // Sums one level's unoccluded slot colours in the reference's order and applies the per-light / per-lobe
 // scales: a = (0 + emission) + diffuse, and for a rough specular lobe s = its direct lighting.

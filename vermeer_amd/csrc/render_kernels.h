@@ -61,6 +61,7 @@ struct RenderView {
     int32_t filt_n;
     int32_t max_level;             // deepest recursion level shaded (0 without mirrors, else 3; path mode: depth - 1)
     int32_t path_depth;            // 0: the reference's shading; D > 0: path mode with D segments per path
+    unsigned long long* counts;    // rays traced since creation: [0] closest hit (camera, mirror, continuation), [1] shadow
     int32_t rec_levels;            // LevelRec entries per pixel
     double filt_w;
 };

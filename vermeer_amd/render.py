@@ -146,6 +146,12 @@ class Renderer:
         continuation ray per vertex (vgpu_render_set_path_depth); 0 keeps the reference's shading."""
         check(self._lib.vgpu_render_set_path_depth(self._h, int(depth)))
 
+    def stats(self):
+        """(closest-hit rays, shadow rays) traced since creation (vgpu_render_stats; synchronises)."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        check(self._lib.vgpu_render_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def set_filter(self, kind="box", width=0.0, res=0, peak=0.0):
         """Pixel filter: "box" (none), "gaussian" or "airy"; zeros take the node defaults."""
         t = {"box": _lib.VGPU_FILTER_BOX, "gaussian": _lib.VGPU_FILTER_GAUSSIAN, "airy": _lib.VGPU_FILTER_AIRY}[kind]
