@@ -201,6 +201,37 @@ int vgpu_scene_add_mesh_prebuilt(vgpu_scene* scene, const float* verts, uint32_t
                                  const vgpu_qbvh_node* nodes, uint32_t nnodes,
                                  const float* bounds, float ray_bias, int32_t* geom_out);
 
+/* A PolyMesh's Transform (builtin/geom/polymesh/trace.go:20-74): nkeys
+ * column-major 4x4 matrices.  Rays are moved into the mesh's space by the
+ * inverse of the transform at their time (one key: the matrix as given; more:
+ * the TransformDecomp keys interpolated, math/animdecomp.go:66-80).  As in the
+ * reference, the mesh's bounds ignore it (bounds.go:26), so the scene culls the
+ * mesh by its untransformed box.  nkeys 0 removes it.  Before commit. */
+int vgpu_scene_set_transform(vgpu_scene* scene, int32_t geom, const float* matrices, uint32_t nkeys);
+
+/* Leaves geom (insertion index) out of the scene's top level: it is traced only
+ * through the instances and procs that hold it (the geoms a Proc handler
+ * creates, builtin/geom/proc/vnf/vnf.go:33-47).  Before commit. */
+int vgpu_scene_set_hidden(vgpu_scene* scene, int32_t geom, int32_t hidden);
+
+/* GeomInstance (builtin/geom/instance/instance.go:36-160): geom (a mesh or
+ * sphere, insertion index) traced under nkeys transform keys (TransformSRT,
+ * interpolated at the ray's time; one key recomposed from its decomposition),
+ * with bounds the box of bmin, bmax (BMin[0], BMax[0]; the reference ignores
+ * later keys).  Hits report the instance's slot.  Scenes holding instances,
+ * procs or mesh transforms are traced by a per-lane kernel without per-ray
+ * counters.  *geom_out: insertion index. */
+int vgpu_scene_add_instance(vgpu_scene* scene, int32_t geom, const float* matrices, uint32_t nkeys, const float* bmin,
+                            const float* bmax, int32_t* geom_out);
+
+/* Proc (builtin/geom/proc/proc.go:40-196) over geoms[0..ngeoms) (meshes or
+ * spheres; usually hidden), in that order, under nkeys transform keys (0: the
+ * identity).  As the reference does, a closest-hit ray takes the first of its
+ * geoms that reports a hit, closer or not (proc.go:112-120); shadow rays try
+ * them all.  Bounds as for instances. */
+int vgpu_scene_add_proc(vgpu_scene* scene, const int32_t* geoms, uint32_t ngeoms, const float* matrices, uint32_t nkeys,
+                        const float* bmin, const float* bmax, int32_t* geom_out);
+
 /* Builds the per-mesh accels not yet built, the scene top level
  * (Scene.initAccel, scene.go:135-203, leafMax 1) and uploads everything to
  * the device.  Replaces Scene.PreRender (scene.go:131-133). */

@@ -44,6 +44,10 @@ def load():
         "orc_scene_add_mesh": (C.c_int, [P, P, C.c_int32, C.c_int32, P, C.c_int32, C.c_float, C.c_int32]),
         "orc_scene_add_sphere": (C.c_int, [P, P, C.c_float, C.c_int32]),
         "orc_scene_commit": (C.c_int, [P]),
+        "orc_scene_set_transform": (None, [P, C.c_int32, P, C.c_int32]),
+        "orc_scene_set_hidden": (None, [P, C.c_int32, C.c_int32]),
+        "orc_scene_add_instance": (C.c_int, [P, C.c_int32, P, C.c_int32, P, P]),
+        "orc_scene_add_proc": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, P]),
         "orc_scene_set_leaf_size": (None, [P, C.c_int32]),
         "orc_scene_info": (C.c_int, [P, P, P, P, P, P]),
         "orc_scene_mesh_accel": (C.c_int, [P, C.c_int32, P, P, P]),
@@ -133,6 +137,28 @@ class OracleScene:
         g = self.lib.orc_scene_add_sphere(self.h, _p(c), float(radius), int(material))
         self.meshes.append(None)
         return g
+
+    def set_transform(self, geom, mats):
+        """A mesh's Transform keys: (nkeys, 16) column-major matrices (builtin/geom/polymesh/trace.go:20-74)."""
+        m = np.ascontiguousarray(mats, np.float32).reshape(-1, 16)
+        self.lib.orc_scene_set_transform(self.h, int(geom), _p(m), len(m))
+
+    def set_hidden(self, geom, hidden=True):
+        """Leaves a geom out of the scene's top level (reached only through instances or procs)."""
+        self.lib.orc_scene_set_hidden(self.h, int(geom), 1 if hidden else 0)
+
+    def add_instance(self, proto, mats, bmin, bmax):
+        """GeomInstance of geom `proto` under transform keys (nkeys, 16); returns its insertion index."""
+        m = np.ascontiguousarray(mats, np.float32).reshape(-1, 16)
+        lo, hi = np.ascontiguousarray(bmin, np.float32), np.ascontiguousarray(bmax, np.float32)
+        return self.lib.orc_scene_add_instance(self.h, int(proto), _p(m), len(m), _p(lo), _p(hi))
+
+    def add_proc(self, children, mats, bmin, bmax):
+        """Proc over geoms `children` (in order) under transform keys (nkeys, 16) or None (identity)."""
+        ch = np.ascontiguousarray(children, np.int32)
+        m = np.ascontiguousarray(mats if mats is not None else np.zeros((0, 16)), np.float32).reshape(-1, 16)
+        lo, hi = np.ascontiguousarray(bmin, np.float32), np.ascontiguousarray(bmax, np.float32)
+        return self.lib.orc_scene_add_proc(self.h, _p(ch), len(ch), _p(m), len(m), _p(lo), _p(hi))
 
     def commit(self, leaf_size=16):
         self.lib.orc_scene_set_leaf_size(self.h, int(leaf_size))
@@ -368,6 +394,15 @@ class OracleRenderer:
             if mesh.get("ray_bias"):
                 self.lib.orc_scene_mesh_set_ray_bias(self.scene.h, g, float(mesh["ray_bias"]))
             self.scene.meshes.append((v, ix))
+        for gi, mesh in enumerate(desc["meshes"]):
+            if mesh.get("transform") is not None:
+                self.scene.set_transform(gi, mesh["transform"])
+            if mesh.get("hidden"):
+                self.scene.set_hidden(gi)
+        for it in desc.get("instances", []):
+            self.scene.add_instance(it["geom"], it["matrices"], it["bmin"], it["bmax"])
+        for pr in desc.get("procs", []):
+            self.scene.add_proc(pr["geoms"], pr.get("matrices"), pr["bmin"], pr["bmax"])
         self.scene.commit()
         mats = np.ascontiguousarray(desc["materials"], MATERIAL_DTYPE)
         Mx = np.ascontiguousarray(M, np.float32)

@@ -121,6 +121,57 @@ int orc_scene_add_sphere(void* sp, const float* centre, float radius, int32_t ma
     return (int)s->geoms.size() - 1;
 }
 
+// A geom's transform keys (nkeys column-major 4x4 matrices) and their TransformDecompMatrix4.
+static void set_keys(Mesh* m, const float* mats, int32_t nkeys) {
+    m->xform.clear(); m->xformSRT.clear();
+    for (int k = 0; k < nkeys; k++) {
+        m->xform.emplace_back(mats + 16 * k, mats + 16 * k + 16);
+        m->xformSRT.push_back(DecompM4(mats + 16 * k));
+    }
+}
+static Box user_box(const float* bmin, const float* bmax) {
+    Box b; b.Reset();
+    Vec3 a, c;
+    for (int k = 0; k < 3; k++) { a[k] = bmin[k]; c[k] = bmax[k]; }
+    b.GrowVec3(a); b.GrowVec3(c);
+    return b;
+}
+
+// A PolyMesh's Transform (builtin/geom/polymesh/trace.go:20-74): nkeys matrices (one: used as given;
+// more: TransformDecomp keys interpolated at the ray's time).
+void orc_scene_set_transform(void* sp, int32_t geom, const float* mats, int32_t nkeys) {
+    set_keys(((Scene*)sp)->geoms[geom], mats, nkeys);
+}
+// A geom left out of the scene's top level (reached only through instances or procs).
+void orc_scene_set_hidden(void* sp, int32_t geom, int32_t hidden) { ((Scene*)sp)->geoms[geom]->hidden = hidden != 0; }
+
+// GeomInstance (builtin/geom/instance/instance.go): geom `proto` (insertion index) under nkeys
+// transform keys; bounds the box of bmin, bmax.  Returns the insertion index.
+int orc_scene_add_instance(void* sp, int32_t proto, const float* mats, int32_t nkeys, const float* bmin, const float* bmax) {
+    Scene* s = (Scene*)sp;
+    Mesh* m = new Mesh();
+    m->kind = GEOM_INSTANCE;
+    m->children.push_back(s->geoms[proto]);
+    set_keys(m, mats, nkeys);
+    m->userBounds = user_box(bmin, bmax);
+    s->geoms.push_back(m);
+    return (int)s->geoms.size() - 1;
+}
+// Proc (builtin/geom/proc/proc.go) holding geoms children[0..nchild) in that order, under nkeys
+// transform keys (0: the identity, proc.go:154-157); bounds the box of bmin, bmax.
+int orc_scene_add_proc(void* sp, const int32_t* children, int32_t nchild, const float* mats, int32_t nkeys,
+                       const float* bmin, const float* bmax) {
+    Scene* s = (Scene*)sp;
+    Mesh* m = new Mesh();
+    m->kind = GEOM_PROC;
+    for (int i = 0; i < nchild; i++) m->children.push_back(s->geoms[children[i]]);
+    if (nkeys > 0) set_keys(m, mats, nkeys);
+    else { M4 I; M4Identity(I); set_keys(m, I, 1); }
+    m->userBounds = user_box(bmin, bmax);
+    s->geoms.push_back(m);
+    return (int)s->geoms.size() - 1;
+}
+
 // PreRender: per-mesh accel, then the scene's top level.
 // leafMax of the static meshes' build (16: the reference's); call before orc_scene_commit.
 void orc_scene_set_leaf_size(void* sp, int32_t leaf_max) {
@@ -130,7 +181,7 @@ void orc_scene_set_leaf_size(void* sp, int32_t leaf_max) {
 int orc_scene_commit(void* sp) {
     Scene* s = (Scene*)sp;
     try {
-        for (auto* g : s->geoms) g->initAccel();
+        for (auto* g : s->geoms) if (g->kind == GEOM_MESH) g->initAccel();
         s->initAccel();
     } catch (BuildError&) { return -1; }
     return 0;

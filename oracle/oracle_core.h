@@ -356,6 +356,150 @@ static void intersectBoxes(const Ray& ray, const float* boxes, int32_t* hits, fl
     }
 }
 
+// float32 trigonometry through the float64 C library (the reference calls Go's float64 math)
+static inline float fSin(float x) { return (float)std::sin((double)x); }
+static inline float fCos(float x) { return (float)std::cos((double)x); }
+static inline float fAcos(float x) { return (float)std::acos((double)x); }
+static inline Vec3 V3(float x, float y, float z) { Vec3 v; v[0] = x; v[1] = y; v[2] = z; return v; }
+
+// --------------------------------------------------------------- matrices
+
+typedef float M4[16];   // column-major, Elt(i,j) = m[j*4+i] (math/matrix4.go:50-57)
+
+static void M4Identity(M4 m) { std::memset(m, 0, sizeof(M4)); m[0] = m[5] = m[10] = m[15] = 1.0f; }
+static void M4Mul(const M4 a, const M4 b, M4 c) {
+    float t[16] = {0};
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) for (int k = 0; k < 4; k++) t[j * 4 + i] += a[k * 4 + i] * b[j * 4 + k];
+    std::memcpy(c, t, sizeof(t));
+}
+static void M4Transpose(const M4 a, M4 c) { float t[16]; for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) t[i * 4 + j] = a[j * 4 + i]; std::memcpy(c, t, sizeof(t)); }
+static void M4Cof(const float* m, float* inv) {
+    inv[0] = m[5]*m[10]*m[15] - m[5]*m[11]*m[14] - m[9]*m[6]*m[15] + m[9]*m[7]*m[14] + m[13]*m[6]*m[11] - m[13]*m[7]*m[10];
+    inv[4] = -m[4]*m[10]*m[15] + m[4]*m[11]*m[14] + m[8]*m[6]*m[15] - m[8]*m[7]*m[14] - m[12]*m[6]*m[11] + m[12]*m[7]*m[10];
+    inv[8] = m[4]*m[9]*m[15] - m[4]*m[11]*m[13] - m[8]*m[5]*m[15] + m[8]*m[7]*m[13] + m[12]*m[5]*m[11] - m[12]*m[7]*m[9];
+    inv[12] = -m[4]*m[9]*m[14] + m[4]*m[10]*m[13] + m[8]*m[5]*m[14] - m[8]*m[6]*m[13] - m[12]*m[5]*m[10] + m[12]*m[6]*m[9];
+    inv[1] = -m[1]*m[10]*m[15] + m[1]*m[11]*m[14] + m[9]*m[2]*m[15] - m[9]*m[3]*m[14] - m[13]*m[2]*m[11] + m[13]*m[3]*m[10];
+    inv[5] = m[0]*m[10]*m[15] - m[0]*m[11]*m[14] - m[8]*m[2]*m[15] + m[8]*m[3]*m[14] + m[12]*m[2]*m[11] - m[12]*m[3]*m[10];
+    inv[9] = -m[0]*m[9]*m[15] + m[0]*m[11]*m[13] + m[8]*m[1]*m[15] - m[8]*m[3]*m[13] - m[12]*m[1]*m[11] + m[12]*m[3]*m[9];
+    inv[13] = m[0]*m[9]*m[14] - m[0]*m[10]*m[13] - m[8]*m[1]*m[14] + m[8]*m[2]*m[13] + m[12]*m[1]*m[10] - m[12]*m[2]*m[9];
+    inv[2] = m[1]*m[6]*m[15] - m[1]*m[7]*m[14] - m[5]*m[2]*m[15] + m[5]*m[3]*m[14] + m[13]*m[2]*m[7] - m[13]*m[3]*m[6];
+    inv[6] = -m[0]*m[6]*m[15] + m[0]*m[7]*m[14] + m[4]*m[2]*m[15] - m[4]*m[3]*m[14] - m[12]*m[2]*m[7] + m[12]*m[3]*m[6];
+    inv[10] = m[0]*m[5]*m[15] - m[0]*m[7]*m[13] - m[4]*m[1]*m[15] + m[4]*m[3]*m[13] + m[12]*m[1]*m[7] - m[12]*m[3]*m[5];
+    inv[14] = -m[0]*m[5]*m[14] + m[0]*m[6]*m[13] + m[4]*m[1]*m[14] - m[4]*m[2]*m[13] - m[12]*m[1]*m[6] + m[12]*m[2]*m[5];
+    inv[3] = -m[1]*m[6]*m[11] + m[1]*m[7]*m[10] + m[5]*m[2]*m[11] - m[5]*m[3]*m[10] - m[9]*m[2]*m[7] + m[9]*m[3]*m[6];
+    inv[7] = m[0]*m[6]*m[11] - m[0]*m[7]*m[10] - m[4]*m[2]*m[11] + m[4]*m[3]*m[10] + m[8]*m[2]*m[7] - m[8]*m[3]*m[6];
+    inv[11] = -m[0]*m[5]*m[11] + m[0]*m[7]*m[9] + m[4]*m[1]*m[11] - m[4]*m[3]*m[9] - m[8]*m[1]*m[7] + m[8]*m[3]*m[5];
+    inv[15] = m[0]*m[5]*m[10] - m[0]*m[6]*m[9] - m[4]*m[1]*m[10] + m[4]*m[2]*m[9] + m[8]*m[1]*m[6] - m[8]*m[2]*m[5];
+}
+// math/matrix4.go:116-149
+static bool M4Inverse(const M4 m, M4 c) {
+    float inv[16];
+    M4Cof(m, inv);
+    float det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    if (det == 0.0f) return false;
+    det = 1.0f / det;
+    for (int i = 0; i < 16; i++) c[i] = inv[i] * det;
+    return true;
+}
+static float M4Det(const M4 m) { float inv[16]; M4Cof(m, inv); return m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12]; }
+// math/matrix4.go:343-363
+static bool M4PolarFactor(const M4 m, M4 Q) {
+    std::memcpy(Q, m, sizeof(M4));
+    for (int it = 0; it < 10; it++) {
+        M4 Qinv, QinvT, Qnew;
+        if (!M4Inverse(Q, Qinv)) return false;
+        M4Transpose(Qinv, QinvT);
+        for (int i = 0; i < 16; i++) Qnew[i] = 0.5f * (Q[i] + QinvT[i]);
+        bool eq = true;
+        for (int i = 0; i < 16; i++) if (Qnew[i] - Q[i] > 0.000001f) { eq = false; break; }
+        if (eq) { std::memcpy(Q, Qnew, sizeof(M4)); return true; }
+        std::memcpy(Q, Qnew, sizeof(M4));
+    }
+    return false;
+}
+struct Quat { float X, Y, Z, W; };
+// math/quat.go:133-163
+static Quat M4ToQuat(const M4 m) {
+    auto E = [&](int i, int j) { return m[j * 4 + i]; };
+    Quat q;
+    float tr = E(0, 0) + E(1, 1) + E(2, 2);
+    if (tr > 0.0f) {
+        float S = std::sqrt(tr + 1.0f) * 2;
+        q.W = 0.25f * S; q.X = (E(2, 1) - E(1, 2)) / S; q.Y = (E(0, 2) - E(2, 0)) / S; q.Z = (E(1, 0) - E(0, 1)) / S;
+    } else if ((E(0, 0) > E(1, 1)) && (E(0, 0) > E(2, 2))) {
+        float S = std::sqrt(1.0f + E(0, 0) - E(1, 1) - E(2, 2)) * 2;
+        q.W = (E(2, 1) - E(1, 2)) / S; q.X = 0.25f * S; q.Y = (E(0, 1) + E(1, 0)) / S; q.Z = (E(0, 2) + E(2, 0)) / S;
+    } else if (E(1, 1) > E(2, 2)) {
+        float S = std::sqrt(1.0f + E(1, 1) - E(0, 0) - E(2, 2)) * 2;
+        q.W = (E(0, 2) - E(2, 0)) / S; q.X = (E(0, 1) + E(1, 0)) / S; q.Y = 0.25f * S; q.Z = (E(1, 2) + E(2, 1)) / S;
+    } else {
+        float S = std::sqrt(1.0f + E(2, 2) - E(0, 0) - E(1, 1)) * 2;
+        q.W = (E(1, 0) - E(0, 1)) / S; q.X = (E(0, 2) + E(2, 0)) / S; q.Y = (E(1, 2) + E(2, 1)) / S; q.Z = 0.25f * S;
+    }
+    return q;
+}
+// math/quat.go:80-104
+static void QuatToM4(const Quat& q, M4 m) {
+    std::memset(m, 0, sizeof(M4));
+    float n = 1.0f / std::sqrt(q.X * q.X + q.Y * q.Y + q.Z * q.Z + q.W * q.W);
+    float X = q.X * n, Y = q.Y * n, Z = q.Z * n, W = q.W * n;
+    auto S = [&](int i, int j, float v) { m[j * 4 + i] = v; };
+    S(0, 0, 1 - 2 * Y * Y - 2 * Z * Z); S(0, 1, 2 * X * Y - 2 * W * Z); S(0, 2, 2 * X * Z + 2 * W * Y);
+    S(1, 0, 2 * X * Y + 2 * W * Z); S(1, 1, 1 - 2 * X * X - 2 * Z * Z); S(1, 2, 2 * Y * Z - 2 * W * X);
+    S(2, 0, 2 * X * Z - 2 * W * Y); S(2, 1, 2 * Y * Z + 2 * W * X); S(2, 2, 1 - 2 * X * X - 2 * Y * Y);
+    S(3, 3, 1.0f);
+}
+// math/quat.go:30-67
+static Quat QuatSlerp(const Quat& a, const Quat& b, float t) {
+    float c = a.W * b.W + a.X * b.X + a.Y * b.Y + a.Z * b.Z;
+    if (Abs(c) >= 1.0f) return a;
+    float half = fAcos(c);
+    float s = std::sqrt(1.0f - c * c);
+    if (Abs(s) < 0.001f) return Quat{a.X * 0.5f + b.X * 0.5f, a.Y * 0.5f + b.Y * 0.5f, a.Z * 0.5f + b.Z * 0.5f, a.W * 0.5f + b.W * 0.5f};
+    float ra = fSin((1 - t) * half) / s, rb = fSin(t * half) / s;
+    return Quat{a.X * ra + b.X * rb, a.Y * ra + b.Y * rb, a.Z * ra + b.Z * rb, a.W * ra + b.W * rb};
+}
+struct Decomp { Vec3 T; Quat R; M4 S; };
+// math/animdecomp.go:20-63 (octanormal.go:16 sign)
+static Decomp DecompM4(const M4 min) {
+    Decomp d;
+    M4 m; std::memcpy(m, min, sizeof(M4));
+    float sgn = M4Det(m) >= 0.0f ? 1.0f : -1.0f;
+    for (int i = 0; i < 3; i++) d.T[i] = m[3 * 4 + i];
+    m[12] = 0; m[13] = 0; m[14] = 0;
+    // Matrix4Mul(Matrix4Scale(-1, Identity), m) as the reference multiplies it: the zeros of -I are -0,
+    // and the sums start from +0, which fixes the sign of the zero entries.
+    M4 negI; M4Identity(negI);
+    for (int i = 0; i < 16; i++) negI[i] = -1.0f * negI[i];
+    if (sgn < 0.0f) M4Mul(negI, m, m);
+    M4 Q;
+    if (!M4PolarFactor(m, Q)) { d.R = Quat{0, 0, 0, 1}; M4Identity(d.S); return d; }
+    M4 QT; M4Transpose(Q, QT);
+    M4 S; M4Mul(QT, m, S);
+    d.R = M4ToQuat(Q);
+    if (sgn < 0.0f) { M4Mul(negI, S, d.S); d.S[15] = 1; }
+    else std::memcpy(d.S, S, sizeof(M4));
+    return d;
+}
+// TransformDecompLerp then TransformDecompToMatrix4 (animdecomp.go:66-80); a static camera has a == b
+static void DecompToM4(const Decomp& a, const Decomp& b, float t, M4 out) {
+    Decomp l;
+    l.T = Lerp(a.T, b.T, t);
+    l.R = QuatSlerp(a.R, b.R, t);
+    for (int i = 0; i < 16; i++) l.S[i] = (1.0f - t) * a.S[i] + t * b.S[i];   // Matrix4Lerp
+    M4 Tm; M4Identity(Tm); Tm[12] = l.T[0]; Tm[13] = l.T[1]; Tm[14] = l.T[2];
+    M4 R; QuatToM4(l.R, R);
+    M4 RS; M4Mul(R, l.S, RS);
+    M4Mul(Tm, RS, out);
+}
+static Vec3 MulVec(const M4 a, const Vec3& b) {
+    return V3(a[0] * b[0] + a[4] * b[1] + a[8] * b[2], a[1] * b[0] + a[5] * b[1] + a[9] * b[2], a[2] * b[0] + a[6] * b[1] + a[10] * b[2]);
+}
+static Vec3 MulPoint(const M4 a, const Vec3& b) {
+    return V3(a[0] * b[0] + a[4] * b[1] + a[8] * b[2] + a[12], a[1] * b[0] + a[5] * b[1] + a[9] * b[2] + a[13],
+              a[2] * b[0] + a[6] * b[1] + a[10] * b[2] + a[14]);
+}
+
 // ------------------------------------------------------------------- meshes
 
 struct HitRecord {
@@ -368,7 +512,14 @@ struct HitRecord {
     float Bu, Bv;
     Vec3 N;                // shading normal (interpolated normals, else Ng) (trace.go:324-338)
     int32_t shaderIdx;     // mesh.shaderidx[idx] (trace.go:300-306)
+    const struct Mesh* leaf;   // the mesh or sphere whose element was hit (inside an instance or proc)
+    bool hasXform;         // sg.Transform / sg.InvTransform set by the hit's geoms
+    bool meshXform;        // the hit mesh's own Transform applied to N in PolyMesh.Trace (trace.go:65-70)
+    M4 xform, xformInv;    // sg.Transform, sg.InvTransform
+    M4 meshInv;            // the inverse of the hit mesh's own transform
 };
+
+enum { GEOM_MESH = 0, GEOM_INSTANCE = 1, GEOM_PROC = 2 };
 
 struct Mesh {
     int nkeys = 1;
@@ -392,6 +543,18 @@ struct Mesh {
     std::vector<Box> motionBounds;
     bool sphere = false;              // analytic sphere geom (builtin/geom/sphere), no triangles
     Vec3 sphereC; float sphereR = 0;
+    // Geoms that wrap others under a transform: GeomInstance (builtin/geom/instance/instance.go) wraps
+    // one geom, Proc (builtin/geom/proc/proc.go) a list; a mesh may carry its own Transform
+    // (builtin/geom/polymesh/trace.go:20-74).  xform: the raw matrices (column-major); xformSRT their
+    // TransformDecompMatrix4 (animdecomp.go:20-63).  userBounds: Bounds of an instance or proc, the
+    // box of BMin[0], BMax[0] whatever the time (instance.go:63-67, proc.go:76-80).  hidden: not in
+    // the scene's top level (a proc's or instance's geoms that the scene does not hold itself).
+    int kind = 0;                     // GEOM_MESH, GEOM_INSTANCE, GEOM_PROC
+    std::vector<std::vector<float>> xform;
+    std::vector<Decomp> xformSRT;
+    std::vector<Mesh*> children;
+    Box userBounds;
+    bool hidden = false;
 
     // builtin/geom/polymesh/buildqbvh.go:14-144
     int leafMax = 16;                 // BuildAccel leafMax (buildqbvh.go:32)
@@ -481,15 +644,37 @@ struct Mesh {
         return nodebox;
     }
 
-    // polymesh/bounds.go:159-186
+    // polymesh/bounds.go:159-186 (a mesh's Transform is ignored: bounds.go:26 reads `if false && ...`);
+    // instance and proc: their given box
     Box Bounds(float time) const {
+        if (kind != GEOM_MESH) return userBounds;
         if (nkeys == 1) return bounds;
         float k = time * (float)(motionBounds.size() - 1);
         float t = k - Floor(k);
         int key = (int)Floor(k), key2 = (int)Ceil(k);
         return BoxLerp(motionBounds[key], motionBounds[key2], t);
     }
-    int MotionKeys() const { return nkeys == 1 ? 1 : (int)mboxes.size(); }
+    // polymesh.go:102-112; instance.go:151-156 (the wrapped geom's); proc.go:188-193 (its transform keys)
+    int MotionKeys() const {
+        if (kind == GEOM_INSTANCE) return children[0]->MotionKeys();
+        if (kind == GEOM_PROC) return (int)xformSRT.size();
+        return nkeys == 1 ? 1 : (int)mboxes.size();
+    }
+    // The transform at a ray's time: TransformSRTArray.TimeKey then TransformDecompToMatrix4
+    // (instance.go:17-34, proc.go:20-37; a single key recomposed too); a mesh's own Transform with one
+    // key is its raw matrix (trace.go:33-47).  Times outside [0, 1] are held to the end keys.
+    void TransformAt(float time, M4 out) const {
+        const int nk = (int)xformSRT.size();
+        if (nk == 1) {
+            if (kind == GEOM_MESH) std::memcpy(out, xform[0].data(), sizeof(M4));
+            else DecompToM4(xformSRT[0], xformSRT[0], 0.0f, out);
+            return;
+        }
+        float k = time * (float)(nk - 1);
+        float t = k - Floor(k);
+        int key = std::min(std::max((int)Floor(k), 0), nk - 1), key2 = std::min(std::max((int)Ceil(k), 0), nk - 1);
+        DecompToM4(xformSRT[key], xformSRT[key2], t, out);
+    }
 
     // builtin/geom/polymesh/trace.go:108-194 (idxp branch) — returns true on an improving hit.
     bool TraceElems(Ray& ray, HitRecord& hr, int base, int count) const {
@@ -776,7 +961,8 @@ struct SceneHit {
 };
 
 struct Scene {
-    std::vector<Mesh*> geoms;        // post-sort order after commit
+    std::vector<Mesh*> geoms;        // insertion order; after commit the top level's geoms in post-sort order
+    std::vector<Mesh*> all;          // every geom in insertion order (owned), set at commit
     std::vector<int32_t> geomOrder;  // post-sort slot -> insertion index
     std::vector<Node> qbvh;
     std::vector<MNode> mnodes;
@@ -784,10 +970,15 @@ struct Scene {
     bool motion = false;
     Box bounds;
 
-    ~Scene() { for (auto* g : geoms) delete g; }
+    ~Scene() { for (auto* g : (all.empty() ? geoms : all)) delete g; }
 
     // builtin/scene/scene.go:135-203
     void initAccel() {
+        all = geoms;
+        std::vector<int32_t> vis;   // insertion indices of the top level's geoms
+        for (int i = 0; i < (int)all.size(); i++) if (!all[i]->hidden) vis.push_back(i);
+        geoms.clear();
+        for (int i : vis) geoms.push_back(all[i]);
         int n = (int)geoms.size();
         std::vector<Box> boxes; std::vector<int32_t> indices; std::vector<Vec3> centroids;
         int maxKeys = 0;
@@ -803,7 +994,8 @@ struct Scene {
         std::vector<Mesh*> ng(n);
         for (int i = 0; i < n; i++) ng[i] = geoms[indices[i]];
         geoms.swap(ng);
-        geomOrder = indices;
+        geomOrder.resize(n);
+        for (int i = 0; i < n; i++) geomOrder[i] = vis[indices[i]];
         if (motion) {
             // scene.go:207-268
             mboxes.assign(maxKeys, std::vector<float>(mnodes.size() * 24, 0.0f));
@@ -873,18 +1065,91 @@ struct Scene {
         return true;
     }
 
-    // builtin/geom/polymesh/trace.go:15-102 without transforms
-    bool GeomTrace(const Mesh& mesh, Ray& ray, SceneHit& sh) const {
+    // The ray moved into a geom's object space (instance.go:90-94, trace.go:49-53): P and D by the
+    // inverse transform (D not renormalised, so t keeps its meaning), then Setup.  Restore puts the
+    // ray back, Tclosest excepted (instance.go:98-104).
+    struct SavedRay { Vec3 P, D, Dinv; float S[3]; int32_t Kx, Ky, Kz; };
+    static SavedRay ToObject(Ray& ray, const M4 inv) {
+        SavedRay sv{ray.P, ray.D, ray.Dinv, {ray.S[0], ray.S[1], ray.S[2]}, ray.Kx, ray.Ky, ray.Kz};
+        ray.P = MulPoint(inv, sv.P);
+        ray.D = MulVec(inv, sv.D);
+        RaySetup(ray);
+        return sv;
+    }
+    static void Restore(Ray& ray, const SavedRay& sv) {
+        ray.P = sv.P; ray.D = sv.D; ray.Dinv = sv.Dinv;
+        for (int k = 0; k < 3; k++) ray.S[k] = sv.S[k];
+        ray.Kx = sv.Kx; ray.Ky = sv.Ky; ray.Kz = sv.Kz;
+    }
+
+    // builtin/geom/polymesh/trace.go:15-102 (and the analytic sphere).  A mesh's own Transform takes
+    // the ray into object space; after a hit sg.Transform = that matrix, sg.P = M Po and sg.N =
+    // M^-T N (ApplyTransform then transforms N once more).  The reference sets those after every
+    // transformed mesh it traces, hit or not, so a transformed mesh traced after the closest hit
+    // re-transforms that hit's frame; here they follow the mesh that holds the closest hit.
+    bool MeshTrace(const Mesh& mesh, Ray& ray, SceneHit& sh) const {
         HitRecord& hr = sh.hr;
-        if (mesh.sphere) return SphereTrace(mesh, ray, hr);
-        if (mesh.nkeys == 1) {
-            return TraceQBVH(mesh.qbvh.data(), ray, [&](int base, int count) { return mesh.TraceElems(ray, hr, base, count); });
+        if (mesh.sphere) {
+            if (!SphereTrace(mesh, ray, hr)) return false;
+            hr.leaf = &mesh; hr.hasXform = false; hr.meshXform = false;
+            return true;
         }
-        float k = ray.Time * (float)(mesh.nkeys - 1);
-        float time = k - Floor(k);
-        int key = (int)Floor(k), key2 = (int)Ceil(k);
-        return TraceMotionQBVH(mesh.mnodes.data(), mesh.mboxes, time, key, key2, ray,
-                               [&](int base, int count) { return mesh.TraceMotionElems(time, key, key2, ray, hr, base, count); });
+        M4 M, Minv;
+        SavedRay sv{};
+        const bool xf = !mesh.xform.empty();
+        if (xf) {
+            mesh.TransformAt(ray.Time, M);
+            M4Inverse(M, Minv);
+            sv = ToObject(ray, Minv);
+        }
+        bool hit;
+        if (mesh.nkeys == 1) {
+            hit = TraceQBVH(mesh.qbvh.data(), ray, [&](int base, int count) { return mesh.TraceElems(ray, hr, base, count); });
+        } else {
+            float k = ray.Time * (float)(mesh.nkeys - 1);
+            float time = k - Floor(k);
+            int key = (int)Floor(k), key2 = (int)Ceil(k);
+            hit = TraceMotionQBVH(mesh.mnodes.data(), mesh.mboxes, time, key, key2, ray,
+                                  [&](int base, int count) { return mesh.TraceMotionElems(time, key, key2, ray, hr, base, count); });
+        }
+        if (xf) Restore(ray, sv);
+        if (hit) {
+            hr.leaf = &mesh;
+            hr.hasXform = xf;
+            hr.meshXform = xf;
+            if (xf) {
+                std::memcpy(hr.xform, M, sizeof(M4));
+                std::memcpy(hr.xformInv, Minv, sizeof(M4));
+                std::memcpy(hr.meshInv, Minv, sizeof(M4));
+            }
+        }
+        return hit;
+    }
+
+    // Geom.Trace: meshes and spheres, GeomInstance.Trace (instance.go:73-114) and Proc.Trace
+    // (proc.go:88-139).  Both move the ray by the inverse of their transform at the ray's time and set
+    // sg.Transform only on a hit.  A proc tries its geoms in order and, for any ray but a shadow ray,
+    // stops at the first one that reports a hit (proc.go:112-120), closer or not.
+    bool GeomTrace(const Mesh& g, Ray& ray, SceneHit& sh) const {
+        if (g.kind == GEOM_MESH) return MeshTrace(g, ray, sh);
+        M4 M, Minv;
+        g.TransformAt(ray.Time, M);
+        M4Inverse(M, Minv);
+        SavedRay sv = ToObject(ray, Minv);
+        bool hit = false;
+        for (const Mesh* c : g.children) {
+            if (GeomTrace(*c, ray, sh)) {
+                hit = true;
+                if (g.kind == GEOM_INSTANCE || (ray.Type & RayTypeShadow) == 0) break;
+            }
+        }
+        Restore(ray, sv);
+        if (hit) {
+            sh.hr.hasXform = true;
+            std::memcpy(sh.hr.xform, M, sizeof(M4));
+            std::memcpy(sh.hr.xformInv, Minv, sizeof(M4));
+        }
+        return hit;
     }
 
     // scene.go:61-97

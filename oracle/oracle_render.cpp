@@ -263,7 +263,6 @@ static void RasterXY(uint32_t m, uint32_t frame, uint32_t px, uint32_t py, uint6
 
 // ----------------------------------------------------------------- vectors
 
-static inline Vec3 V3(float x, float y, float z) { Vec3 v; v[0] = x; v[1] = y; v[2] = z; return v; }
 static inline Vec3 Sub(const Vec3& a, const Vec3& b) { return V3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
 static inline Vec3 Add(const Vec3& a, const Vec3& b) { return V3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
 static inline Vec3 Scale(float s, const Vec3& a) { return V3(a[0] * s, a[1] * s, a[2] * s); }
@@ -287,10 +286,7 @@ static inline Vec3 BasisExpand(const Vec3& U, const Vec3& V, const Vec3& W, cons
 }
 static const float Pi32 = 3.14159265358f;
 static inline float sqr32(float x) { return x * x; }
-static inline float fSin(float x) { return (float)std::sin((double)x); }
-static inline float fCos(float x) { return (float)std::cos((double)x); }
 static inline float fTan(float x) { return (float)std::tan((double)x); }
-static inline float fAcos(float x) { return (float)std::acos((double)x); }
 static inline float fAtan(float x) { return (float)std::atan((double)x); }
 static inline float fAtan2(float y, float x) { return (float)std::atan2((double)y, (double)x); }
 
@@ -308,144 +304,6 @@ static float NextFloatDown(float v) {
     uint32_t ui = f2bits(v);
     if (v >= 0.0f) ui--; else ui++;
     return bits2f(ui);
-}
-
-// --------------------------------------------------------------- matrices
-
-typedef float M4[16];   // column-major, Elt(i,j) = m[j*4+i] (math/matrix4.go:50-57)
-
-static void M4Identity(M4 m) { std::memset(m, 0, sizeof(M4)); m[0] = m[5] = m[10] = m[15] = 1.0f; }
-static void M4Mul(const M4 a, const M4 b, M4 c) {
-    float t[16] = {0};
-    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) for (int k = 0; k < 4; k++) t[j * 4 + i] += a[k * 4 + i] * b[j * 4 + k];
-    std::memcpy(c, t, sizeof(t));
-}
-static void M4Transpose(const M4 a, M4 c) { float t[16]; for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) t[i * 4 + j] = a[j * 4 + i]; std::memcpy(c, t, sizeof(t)); }
-static void M4Cof(const float* m, float* inv) {
-    inv[0] = m[5]*m[10]*m[15] - m[5]*m[11]*m[14] - m[9]*m[6]*m[15] + m[9]*m[7]*m[14] + m[13]*m[6]*m[11] - m[13]*m[7]*m[10];
-    inv[4] = -m[4]*m[10]*m[15] + m[4]*m[11]*m[14] + m[8]*m[6]*m[15] - m[8]*m[7]*m[14] - m[12]*m[6]*m[11] + m[12]*m[7]*m[10];
-    inv[8] = m[4]*m[9]*m[15] - m[4]*m[11]*m[13] - m[8]*m[5]*m[15] + m[8]*m[7]*m[13] + m[12]*m[5]*m[11] - m[12]*m[7]*m[9];
-    inv[12] = -m[4]*m[9]*m[14] + m[4]*m[10]*m[13] + m[8]*m[5]*m[14] - m[8]*m[6]*m[13] - m[12]*m[5]*m[10] + m[12]*m[6]*m[9];
-    inv[1] = -m[1]*m[10]*m[15] + m[1]*m[11]*m[14] + m[9]*m[2]*m[15] - m[9]*m[3]*m[14] - m[13]*m[2]*m[11] + m[13]*m[3]*m[10];
-    inv[5] = m[0]*m[10]*m[15] - m[0]*m[11]*m[14] - m[8]*m[2]*m[15] + m[8]*m[3]*m[14] + m[12]*m[2]*m[11] - m[12]*m[3]*m[10];
-    inv[9] = -m[0]*m[9]*m[15] + m[0]*m[11]*m[13] + m[8]*m[1]*m[15] - m[8]*m[3]*m[13] - m[12]*m[1]*m[11] + m[12]*m[3]*m[9];
-    inv[13] = m[0]*m[9]*m[14] - m[0]*m[10]*m[13] - m[8]*m[1]*m[14] + m[8]*m[2]*m[13] + m[12]*m[1]*m[10] - m[12]*m[2]*m[9];
-    inv[2] = m[1]*m[6]*m[15] - m[1]*m[7]*m[14] - m[5]*m[2]*m[15] + m[5]*m[3]*m[14] + m[13]*m[2]*m[7] - m[13]*m[3]*m[6];
-    inv[6] = -m[0]*m[6]*m[15] + m[0]*m[7]*m[14] + m[4]*m[2]*m[15] - m[4]*m[3]*m[14] - m[12]*m[2]*m[7] + m[12]*m[3]*m[6];
-    inv[10] = m[0]*m[5]*m[15] - m[0]*m[7]*m[13] - m[4]*m[1]*m[15] + m[4]*m[3]*m[13] + m[12]*m[1]*m[7] - m[12]*m[3]*m[5];
-    inv[14] = -m[0]*m[5]*m[14] + m[0]*m[6]*m[13] + m[4]*m[1]*m[14] - m[4]*m[2]*m[13] - m[12]*m[1]*m[6] + m[12]*m[2]*m[5];
-    inv[3] = -m[1]*m[6]*m[11] + m[1]*m[7]*m[10] + m[5]*m[2]*m[11] - m[5]*m[3]*m[10] - m[9]*m[2]*m[7] + m[9]*m[3]*m[6];
-    inv[7] = m[0]*m[6]*m[11] - m[0]*m[7]*m[10] - m[4]*m[2]*m[11] + m[4]*m[3]*m[10] + m[8]*m[2]*m[7] - m[8]*m[3]*m[6];
-    inv[11] = -m[0]*m[5]*m[11] + m[0]*m[7]*m[9] + m[4]*m[1]*m[11] - m[4]*m[3]*m[9] - m[8]*m[1]*m[7] + m[8]*m[3]*m[5];
-    inv[15] = m[0]*m[5]*m[10] - m[0]*m[6]*m[9] - m[4]*m[1]*m[10] + m[4]*m[2]*m[9] + m[8]*m[1]*m[6] - m[8]*m[2]*m[5];
-}
-// math/matrix4.go:116-149
-static bool M4Inverse(const M4 m, M4 c) {
-    float inv[16];
-    M4Cof(m, inv);
-    float det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
-    if (det == 0.0f) return false;
-    det = 1.0f / det;
-    for (int i = 0; i < 16; i++) c[i] = inv[i] * det;
-    return true;
-}
-static float M4Det(const M4 m) { float inv[16]; M4Cof(m, inv); return m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12]; }
-// math/matrix4.go:343-363
-static bool M4PolarFactor(const M4 m, M4 Q) {
-    std::memcpy(Q, m, sizeof(M4));
-    for (int it = 0; it < 10; it++) {
-        M4 Qinv, QinvT, Qnew;
-        if (!M4Inverse(Q, Qinv)) return false;
-        M4Transpose(Qinv, QinvT);
-        for (int i = 0; i < 16; i++) Qnew[i] = 0.5f * (Q[i] + QinvT[i]);
-        bool eq = true;
-        for (int i = 0; i < 16; i++) if (Qnew[i] - Q[i] > 0.000001f) { eq = false; break; }
-        if (eq) { std::memcpy(Q, Qnew, sizeof(M4)); return true; }
-        std::memcpy(Q, Qnew, sizeof(M4));
-    }
-    return false;
-}
-struct Quat { float X, Y, Z, W; };
-// math/quat.go:133-163
-static Quat M4ToQuat(const M4 m) {
-    auto E = [&](int i, int j) { return m[j * 4 + i]; };
-    Quat q;
-    float tr = E(0, 0) + E(1, 1) + E(2, 2);
-    if (tr > 0.0f) {
-        float S = std::sqrt(tr + 1.0f) * 2;
-        q.W = 0.25f * S; q.X = (E(2, 1) - E(1, 2)) / S; q.Y = (E(0, 2) - E(2, 0)) / S; q.Z = (E(1, 0) - E(0, 1)) / S;
-    } else if ((E(0, 0) > E(1, 1)) && (E(0, 0) > E(2, 2))) {
-        float S = std::sqrt(1.0f + E(0, 0) - E(1, 1) - E(2, 2)) * 2;
-        q.W = (E(2, 1) - E(1, 2)) / S; q.X = 0.25f * S; q.Y = (E(0, 1) + E(1, 0)) / S; q.Z = (E(0, 2) + E(2, 0)) / S;
-    } else if (E(1, 1) > E(2, 2)) {
-        float S = std::sqrt(1.0f + E(1, 1) - E(0, 0) - E(2, 2)) * 2;
-        q.W = (E(0, 2) - E(2, 0)) / S; q.X = (E(0, 1) + E(1, 0)) / S; q.Y = 0.25f * S; q.Z = (E(1, 2) + E(2, 1)) / S;
-    } else {
-        float S = std::sqrt(1.0f + E(2, 2) - E(0, 0) - E(1, 1)) * 2;
-        q.W = (E(1, 0) - E(0, 1)) / S; q.X = (E(0, 2) + E(2, 0)) / S; q.Y = (E(1, 2) + E(2, 1)) / S; q.Z = 0.25f * S;
-    }
-    return q;
-}
-// math/quat.go:80-104
-static void QuatToM4(const Quat& q, M4 m) {
-    std::memset(m, 0, sizeof(M4));
-    float n = 1.0f / std::sqrt(q.X * q.X + q.Y * q.Y + q.Z * q.Z + q.W * q.W);
-    float X = q.X * n, Y = q.Y * n, Z = q.Z * n, W = q.W * n;
-    auto S = [&](int i, int j, float v) { m[j * 4 + i] = v; };
-    S(0, 0, 1 - 2 * Y * Y - 2 * Z * Z); S(0, 1, 2 * X * Y - 2 * W * Z); S(0, 2, 2 * X * Z + 2 * W * Y);
-    S(1, 0, 2 * X * Y + 2 * W * Z); S(1, 1, 1 - 2 * X * X - 2 * Z * Z); S(1, 2, 2 * Y * Z - 2 * W * X);
-    S(2, 0, 2 * X * Z - 2 * W * Y); S(2, 1, 2 * Y * Z + 2 * W * X); S(2, 2, 1 - 2 * X * X - 2 * Y * Y);
-    S(3, 3, 1.0f);
-}
-// math/quat.go:30-67
-static Quat QuatSlerp(const Quat& a, const Quat& b, float t) {
-    float c = a.W * b.W + a.X * b.X + a.Y * b.Y + a.Z * b.Z;
-    if (Abs(c) >= 1.0f) return a;
-    float half = fAcos(c);
-    float s = std::sqrt(1.0f - c * c);
-    if (Abs(s) < 0.001f) return Quat{a.X * 0.5f + b.X * 0.5f, a.Y * 0.5f + b.Y * 0.5f, a.Z * 0.5f + b.Z * 0.5f, a.W * 0.5f + b.W * 0.5f};
-    float ra = fSin((1 - t) * half) / s, rb = fSin(t * half) / s;
-    return Quat{a.X * ra + b.X * rb, a.Y * ra + b.Y * rb, a.Z * ra + b.Z * rb, a.W * ra + b.W * rb};
-}
-struct Decomp { Vec3 T; Quat R; M4 S; };
-// math/animdecomp.go:20-63 (octanormal.go:16 sign)
-static Decomp DecompM4(const M4 min) {
-    Decomp d;
-    M4 m; std::memcpy(m, min, sizeof(M4));
-    float sgn = M4Det(m) >= 0.0f ? 1.0f : -1.0f;
-    for (int i = 0; i < 3; i++) d.T[i] = m[3 * 4 + i];
-    m[12] = 0; m[13] = 0; m[14] = 0;
-    // Matrix4Mul(Matrix4Scale(-1, Identity), m) as the reference multiplies it: the zeros of -I are -0,
-    // and the sums start from +0, which fixes the sign of the zero entries.
-    M4 negI; M4Identity(negI);
-    for (int i = 0; i < 16; i++) negI[i] = -1.0f * negI[i];
-    if (sgn < 0.0f) M4Mul(negI, m, m);
-    M4 Q;
-    if (!M4PolarFactor(m, Q)) { d.R = Quat{0, 0, 0, 1}; M4Identity(d.S); return d; }
-    M4 QT; M4Transpose(Q, QT);
-    M4 S; M4Mul(QT, m, S);
-    d.R = M4ToQuat(Q);
-    if (sgn < 0.0f) { M4Mul(negI, S, d.S); d.S[15] = 1; }
-    else std::memcpy(d.S, S, sizeof(M4));
-    return d;
-}
-// TransformDecompLerp then TransformDecompToMatrix4 (animdecomp.go:66-80); a static camera has a == b
-static void DecompToM4(const Decomp& a, const Decomp& b, float t, M4 out) {
-    Decomp l;
-    l.T = Lerp(a.T, b.T, t);
-    l.R = QuatSlerp(a.R, b.R, t);
-    for (int i = 0; i < 16; i++) l.S[i] = (1.0f - t) * a.S[i] + t * b.S[i];   // Matrix4Lerp
-    M4 Tm; M4Identity(Tm); Tm[12] = l.T[0]; Tm[13] = l.T[1]; Tm[14] = l.T[2];
-    M4 R; QuatToM4(l.R, R);
-    M4 RS; M4Mul(R, l.S, RS);
-    M4Mul(Tm, RS, out);
-}
-static Vec3 MulVec(const M4 a, const Vec3& b) {
-    return V3(a[0] * b[0] + a[4] * b[1] + a[8] * b[2], a[1] * b[0] + a[5] * b[1] + a[9] * b[2], a[2] * b[0] + a[6] * b[1] + a[10] * b[2]);
-}
-static Vec3 MulPoint(const M4 a, const Vec3& b) {
-    return V3(a[0] * b[0] + a[4] * b[1] + a[8] * b[2] + a[12], a[1] * b[0] + a[5] * b[1] + a[9] * b[2] + a[13],
-              a[2] * b[0] + a[6] * b[1] + a[10] * b[2] + a[14]);
 }
 
 // --------------------------------------------------------------- materials
@@ -1229,12 +1087,26 @@ struct Renderer {
             sc.Scramble[1] = LevelScramble(scramble[1], level, 1);
         }
         sc.Rd = r.D;
-        // ApplyTransform with identity matrices (core/shader.go:129-135)
-        sc.P = h.Po;
-        sc.N = Normalize(h.N); sc.Ng = Normalize(h.Ng);
-        sc.DdPdu = Normalize(h.DdPdu); sc.DdPdv = Normalize(h.DdPdv);
+        if (h.hasXform) {
+            // PolyMesh.Trace's own transform of N (trace.go:65-70), then ApplyTransform with sg.Transform
+            // (core/shader.go:129-135); Poffset stays in object space
+            M4 T;
+            Vec3 N1 = h.N;
+            if (h.meshXform) { M4Transpose(h.meshInv, T); N1 = MulVec(T, h.N); }
+            M4Transpose(h.xformInv, T);
+            sc.P = MulPoint(h.xform, h.Po);
+            sc.N = Normalize(MulVec(T, N1));
+            sc.Ng = Normalize(MulVec(T, h.Ng));
+            sc.DdPdu = Normalize(MulVec(h.xform, h.DdPdu));
+            sc.DdPdv = Normalize(MulVec(h.xform, h.DdPdv));
+        } else {
+            // ApplyTransform with identity matrices (core/shader.go:129-135)
+            sc.P = h.Po;
+            sc.N = Normalize(h.N); sc.Ng = Normalize(h.Ng);
+            sc.DdPdu = Normalize(h.DdPdu); sc.DdPdv = Normalize(h.DdPdv);
+        }
         sc.Poffset = h.Poffset;
-        const auto& shaders = geomShaders[sh.geom];
+        const auto& shaders = h.leaf ? h.leaf->shaders : geomShaders[sh.geom];
         int mat = shaders.empty() ? -1 : shaders[h.shaderIdx];
         if (mat < 0) { hit = false; return out; }
         return Shade(tk, sc, mats[mat], level, smooth);

@@ -101,6 +101,10 @@ SIGNATURES = [
     ("vgpu_render_create", C.c_int, [C.POINTER(_P), _P, _P, _U32, _P, _U32, _P, _U32, _U32]),
     ("vgpu_render_set_filter", C.c_int, [_P, _I32, _F, _I32, _F]),
     ("vgpu_render_set_path_depth", C.c_int, [_P, C.c_uint32]),
+    ("vgpu_scene_set_transform", C.c_int, [_P, _I32, _P, C.c_uint32]),
+    ("vgpu_scene_set_hidden", C.c_int, [_P, _I32, _I32]),
+    ("vgpu_scene_add_instance", C.c_int, [_P, _I32, _P, C.c_uint32, _P, _P, _P]),
+    ("vgpu_scene_add_proc", C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, _P, _P]),
     ("vgpu_render_stats", C.c_int, [_P, _P, _P]),
     ("vgpu_render_set_camera_keys", C.c_int, [_P, _P, _U32]),
     ("vgpu_render_destroy", C.c_int, [_P]),

@@ -155,6 +155,49 @@ class Scene:
         self.geoms.append(None)
         return g.value
 
+    def set_transform(self, geom, matrices):
+        """A mesh's Transform keys, (nkeys, 16) column-major (vgpu_scene_set_transform); geom: insertion index."""
+        m = np.ascontiguousarray(matrices, np.float32).reshape(-1, 16)
+        check(self._lib.vgpu_scene_set_transform(self._h, int(geom), ptr(m), len(m)))
+
+    def set_hidden(self, geom, hidden=True):
+        """Leaves a geom out of the top level: traced only through instances and procs holding it."""
+        check(self._lib.vgpu_scene_set_hidden(self._h, int(geom), 1 if hidden else 0))
+
+    def add_instance(self, geom, matrices, bmin, bmax):
+        """GeomInstance of geom (insertion index) under transform keys (nkeys, 16); returns its insertion index."""
+        m = np.ascontiguousarray(matrices, np.float32).reshape(-1, 16)
+        lo, hi = np.ascontiguousarray(bmin, np.float32), np.ascontiguousarray(bmax, np.float32)
+        g = C.c_int32(-1)
+        check(self._lib.vgpu_scene_add_instance(self._h, int(geom), ptr(m), len(m), ptr(lo), ptr(hi), C.byref(g)))
+        self.geoms.append(None)
+        return g.value
+
+    def add_proc(self, geoms, matrices, bmin, bmax):
+        """Proc over geoms (insertion indices, in order) under transform keys (nkeys, 16) or None (identity)."""
+        ch = np.ascontiguousarray(geoms, np.int32)
+        m = None if matrices is None else np.ascontiguousarray(matrices, np.float32).reshape(-1, 16)
+        lo, hi = np.ascontiguousarray(bmin, np.float32), np.ascontiguousarray(bmax, np.float32)
+        g = C.c_int32(-1)
+        check(self._lib.vgpu_scene_add_proc(self._h, ptr(ch), len(ch), ptr(m), 0 if m is None else len(m), ptr(lo),
+                                            ptr(hi), C.byref(g)))
+        self.geoms.append(None)
+        return g.value
+
+    def add_transforms(self, desc):
+        """The transforms of a scene description: per mesh "transform" (nkeys, 16) and "hidden", then
+        desc["instances"] ({geom, matrices, bmin, bmax}) and desc["procs"] ({geoms, matrices or None, bmin,
+        bmax}), added after the meshes in that order."""
+        for gi, mesh in enumerate(desc["meshes"]):
+            if mesh.get("transform") is not None:
+                self.set_transform(gi, mesh["transform"])
+            if mesh.get("hidden"):
+                self.set_hidden(gi)
+        for it in desc.get("instances", []):
+            self.add_instance(it["geom"], it["matrices"], it["bmin"], it["bmax"])
+        for pr in desc.get("procs", []):
+            self.add_proc(pr["geoms"], pr.get("matrices"), pr["bmin"], pr["bmax"])
+
     def pre_render(self):
         check(self._lib.vgpu_scene_commit(self._h))
         self.committed = True

@@ -152,22 +152,7 @@ Mat4 mat_transpose(const Mat4& a) {
 // order fix the rounding, so they are written as the reference writes them.
 Mat4 mat_cofactors(const Mat4& m) {
     Mat4 c{};
-    c[0] = m[5]*m[10]*m[15] - m[5]*m[11]*m[14] - m[9]*m[6]*m[15] + m[9]*m[7]*m[14] + m[13]*m[6]*m[11] - m[13]*m[7]*m[10];
-    c[4] = -m[4]*m[10]*m[15] + m[4]*m[11]*m[14] + m[8]*m[6]*m[15] - m[8]*m[7]*m[14] - m[12]*m[6]*m[11] + m[12]*m[7]*m[10];
-    c[8] = m[4]*m[9]*m[15] - m[4]*m[11]*m[13] - m[8]*m[5]*m[15] + m[8]*m[7]*m[13] + m[12]*m[5]*m[11] - m[12]*m[7]*m[9];
-    c[12] = -m[4]*m[9]*m[14] + m[4]*m[10]*m[13] + m[8]*m[5]*m[14] - m[8]*m[6]*m[13] - m[12]*m[5]*m[10] + m[12]*m[6]*m[9];
-    c[1] = -m[1]*m[10]*m[15] + m[1]*m[11]*m[14] + m[9]*m[2]*m[15] - m[9]*m[3]*m[14] - m[13]*m[2]*m[11] + m[13]*m[3]*m[10];
-    c[5] = m[0]*m[10]*m[15] - m[0]*m[11]*m[14] - m[8]*m[2]*m[15] + m[8]*m[3]*m[14] + m[12]*m[2]*m[11] - m[12]*m[3]*m[10];
-    c[9] = -m[0]*m[9]*m[15] + m[0]*m[11]*m[13] + m[8]*m[1]*m[15] - m[8]*m[3]*m[13] - m[12]*m[1]*m[11] + m[12]*m[3]*m[9];
-    c[13] = m[0]*m[9]*m[14] - m[0]*m[10]*m[13] - m[8]*m[1]*m[14] + m[8]*m[2]*m[13] + m[12]*m[1]*m[10] - m[12]*m[2]*m[9];
-    c[2] = m[1]*m[6]*m[15] - m[1]*m[7]*m[14] - m[5]*m[2]*m[15] + m[5]*m[3]*m[14] + m[13]*m[2]*m[7] - m[13]*m[3]*m[6];
-    c[6] = -m[0]*m[6]*m[15] + m[0]*m[7]*m[14] + m[4]*m[2]*m[15] - m[4]*m[3]*m[14] - m[12]*m[2]*m[7] + m[12]*m[3]*m[6];
-    c[10] = m[0]*m[5]*m[15] - m[0]*m[7]*m[13] - m[4]*m[1]*m[15] + m[4]*m[3]*m[13] + m[12]*m[1]*m[7] - m[12]*m[3]*m[5];
-    c[14] = -m[0]*m[5]*m[14] + m[0]*m[6]*m[13] + m[4]*m[1]*m[14] - m[4]*m[2]*m[13] - m[12]*m[1]*m[6] + m[12]*m[2]*m[5];
-    c[3] = -m[1]*m[6]*m[11] + m[1]*m[7]*m[10] + m[5]*m[2]*m[11] - m[5]*m[3]*m[10] - m[9]*m[2]*m[7] + m[9]*m[3]*m[6];
-    c[7] = m[0]*m[6]*m[11] - m[0]*m[7]*m[10] - m[4]*m[2]*m[11] + m[4]*m[3]*m[10] + m[8]*m[2]*m[7] - m[8]*m[3]*m[6];
-    c[11] = -m[0]*m[5]*m[11] + m[0]*m[7]*m[9] + m[4]*m[1]*m[11] - m[4]*m[3]*m[9] - m[8]*m[1]*m[7] + m[8]*m[3]*m[5];
-    c[15] = m[0]*m[5]*m[10] - m[0]*m[6]*m[9] - m[4]*m[1]*m[10] + m[4]*m[2]*m[9] + m[8]*m[1]*m[6] - m[8]*m[2]*m[5];
+    vgpu::xf_cofactors(m.data(), c.data());
     return c;
 }
 
@@ -240,37 +225,9 @@ void decompose_key(Mat4 m, float* rec) {
 // TransformDecompLerp then TransformDecompToMatrix4 (math/animdecomp.go:66-80) of two key records, with
 // QuatSlerp (math/quat.go:30-67) and QuatToMatrix4 (quat.go:80-104); k_camera runs the same steps.
 Mat4 decomp_matrix(const float* a, const float* b, float t) {
-    float T[3], q[4], S[16];
-    for (int i = 0; i < 3; i++) T[i] = (1.0f - t) * a[i] + t * b[i];
-    const float* qa = a + 3;
-    const float* qb = b + 3;
-    const float c = qa[3] * qb[3] + qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2];
-    if (std::fabs(c) >= 1.0f) {
-        for (int i = 0; i < 4; i++) q[i] = qa[i];
-    } else {
-        const float half = (float)std::acos((double)c);
-        const float sh = std::sqrt(1.0f - c * c);
-        if (std::fabs(sh) < 0.001f) {
-            for (int i = 0; i < 4; i++) q[i] = qa[i] * 0.5f + qb[i] * 0.5f;
-        } else {
-            const float ra = (float)std::sin((double)((1 - t) * half)) / sh;
-            const float rb = (float)std::sin((double)(t * half)) / sh;
-            for (int i = 0; i < 4; i++) q[i] = qa[i] * ra + qb[i] * rb;
-        }
-    }
-    for (int i = 0; i < 16; i++) S[i] = (1.0f - t) * a[7 + i] + t * b[7 + i];
-    const float n = 1.0f / std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    const float X = q[0] * n, Y = q[1] * n, Z = q[2] * n, W = q[3] * n;
-    Mat4 R{};
-    R[0] = 1 - 2 * Y * Y - 2 * Z * Z; R[4] = 2 * X * Y - 2 * W * Z; R[8] = 2 * X * Z + 2 * W * Y;
-    R[1] = 2 * X * Y + 2 * W * Z; R[5] = 1 - 2 * X * X - 2 * Z * Z; R[9] = 2 * Y * Z - 2 * W * X;
-    R[2] = 2 * X * Z - 2 * W * Y; R[6] = 2 * Y * Z + 2 * W * X; R[10] = 1 - 2 * X * X - 2 * Y * Y;
-    R[15] = 1.0f;
-    Mat4 Sm, Tm{};
-    for (int i = 0; i < 16; i++) Sm[i] = S[i];
-    Tm[0] = Tm[5] = Tm[10] = Tm[15] = 1.0f;
-    Tm[12] = T[0]; Tm[13] = T[1]; Tm[14] = T[2];
-    return mat_mul(Tm, mat_mul(R, Sm));
+    Mat4 M{};
+    vgpu::xf_decomp_matrix(a, b, t, M.data());
+    return M;
 }
 
 
@@ -338,6 +295,12 @@ int lookat_keys(const float* from, int nfrom, const float* to, int nto, const fl
         std::copy(M.begin(), M.end(), out.begin() + (size_t)i * 16);
     }
     return n;
+}
+
+void decompose_transform(const float* m, float* rec) {
+    Mat4 a{};
+    std::copy(m, m + 16, a.begin());
+    decompose_key(a, rec);
 }
 
 bool mat4_inverse(const float* m, float* out) {

@@ -23,6 +23,7 @@
 // step (math/vec3_amd64.s:11-42) on the IEEE estimate 1/sqrt(x).
 
 #include "render_kernels.h"
+#include "xform.h"
 
 #include <cstdint>
 
@@ -819,44 +820,7 @@ __device__ void camera_key_matrix(const RenderView& rv, float time, float* M) {
     const float t = k - floorf(k);
     const float* a = rv.cam_keys + min(max((int)floorf(k), 0), rv.cam_nkeys - 1) * kCamKeyFloats;
     const float* b = rv.cam_keys + min(max((int)ceilf(k), 0), rv.cam_nkeys - 1) * kCamKeyFloats;
-    float T[3], q[4];
-    for (int i = 0; i < 3; i++) T[i] = (1.0f - t) * a[i] + t * b[i];
-    const float* qa = a + 3;
-    const float* qb = b + 3;
-    const float c = qa[3] * qb[3] + qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2];
-    if (fabs32(c) >= 1.0f) {
-        for (int i = 0; i < 4; i++) q[i] = qa[i];
-    } else {
-        const float half = facos(c);
-        const float sh = sqrtf(1.0f - c * c);
-        if (fabs32(sh) < 0.001f) {
-            for (int i = 0; i < 4; i++) q[i] = qa[i] * 0.5f + qb[i] * 0.5f;
-        } else {
-            const float ra = fsin((1 - t) * half) / sh, rb = fsin(t * half) / sh;
-            for (int i = 0; i < 4; i++) q[i] = qa[i] * ra + qb[i] * rb;
-        }
-    }
-    const float n = 1.0f / sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    const float X = q[0] * n, Y = q[1] * n, Z = q[2] * n, W = q[3] * n;
-    float R[16] = {0};
-    R[0] = 1 - 2 * Y * Y - 2 * Z * Z; R[4] = 2 * X * Y - 2 * W * Z; R[8] = 2 * X * Z + 2 * W * Y;
-    R[1] = 2 * X * Y + 2 * W * Z; R[5] = 1 - 2 * X * X - 2 * Z * Z; R[9] = 2 * Y * Z - 2 * W * X;
-    R[2] = 2 * X * Z - 2 * W * Y; R[6] = 2 * Y * Z + 2 * W * X; R[10] = 1 - 2 * X * X - 2 * Y * Y;
-    R[15] = 1.0f;
-    float RS[16];
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) {
-            float acc = 0.0f;
-            for (int kk = 0; kk < 4; kk++) acc += R[kk * 4 + i] * ((1.0f - t) * a[7 + j * 4 + kk] + t * b[7 + j * 4 + kk]);
-            RS[j * 4 + i] = acc;
-        }
-    float Tm[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, T[0], T[1], T[2], 1};
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) {
-            float acc = 0.0f;
-            for (int kk = 0; kk < 4; kk++) acc += Tm[kk * 4 + i] * RS[j * 4 + kk];
-            M[j * 4 + i] = acc;
-        }
+    xf_decomp_matrix(a, b, t, M);
 }
 
 __global__ __launch_bounds__(256) void k_camera(RenderView rv, const uint64_t* __restrict__ scr, uint32_t iter,
@@ -988,12 +952,14 @@ __device__ __forceinline__ int shade_pixel(int& next, const RenderView& rv, cons
     LevelRec* rec = recs + (size_t)p * rv.rec_levels + level;
     const int4 h0 = hits32[2 * p], h1 = hits32[2 * p + 1];
     const int prim = h0.x, g = h0.y;
+    // the mesh or sphere whose element was hit: the hit geom itself, or the geom an instance or proc holds
+    const int lg = rv.sv.any_xform ? h1.z : g;
     int mat = -1;
     DevGeom geo = {0, 0, 1, 0.0f};
     DevGeomShade gs = {-1, -1, 0, 0};
     if (prim >= 0) {
-        geo = rv.sv.geoms[g];
-        gs = rv.gshade[g];
+        geo = rv.sv.geoms[lg];
+        gs = rv.gshade[lg];
         const int sidx = gs.sidx_base >= 0 ? (int)rv.shaderidx[gs.sidx_base + prim] : 0;
         mat = gs.nshaders > 0 ? rv.shaders[gs.shader_base + sidx] : -1;
     }
@@ -1009,13 +975,38 @@ __device__ __forceinline__ int shade_pixel(int& next, const RenderView& rv, cons
         return 0;
     }
     const V3 Ro = v3(ra.x, ra.y, ra.z), Rd = v3(rb.x, rb.y, rb.z);
+    V3 Ro_obj = Ro, Rd_obj = Rd;   // the ray in the space of the hit geom (for the analytic sphere)
     const float thit = __int_as_float(h0.z);
+    // Transforms of the hit (instance.go:108-111, polymesh/trace.go:57-70): M / Minv end as
+    // sg.Transform / sg.InvTransform (an instance's or proc's over its mesh's own); Mmi is the mesh's own
+    // inverse, which PolyMesh.Trace applies to N before ApplyTransform applies Minv^T again.
+    bool xf = false, mesh_xf = false;
+    float M[16], Minv[16], Mmi[16];
+    if (rv.sv.any_xform) {
+        const DevXform xl = rv.sv.xforms[lg], xt = rv.sv.xforms[g];
+        if (xl.kind == kXfMesh) {
+            xform_at(xl, rv.sv.xkeys, rb.w, M, Minv);
+            for (int i = 0; i < 16; i++) Mmi[i] = Minv[i];
+            xf = mesh_xf = true;
+        }
+        if (xt.kind == kXfInstance || xt.kind == kXfProc) {
+            xform_at(xt, rv.sv.xkeys, rb.w, M, Minv);
+            xf = true;
+            // the analytic sphere's hit point is found in the instance's space (sphere/trace.go:30-35)
+            float o[3], d[3];
+            const float Pw[3] = {Ro.x, Ro.y, Ro.z}, Dw[3] = {Rd.x, Rd.y, Rd.z};
+            xf_point(Minv, Pw, o);
+            xf_vec(Minv, Dw, d);
+            Ro_obj = v3(o[0], o[1], o[2]);
+            Rd_obj = v3(d[0], d[1], d[2]);
+        }
+    }
     SC sc;
     V3 N, Ng, dPdu, dPdv;
     if (geo.nkeys == 0) {
         // analytic sphere (builtin/geom/sphere/trace.go:13-52)
         const float4 sp = rv.sv.spheres[geo.tri_base];
-        sc.P = mad(Ro, Rd, thit);
+        sc.P = mad(Ro_obj, Rd_obj, thit);
         N = normalize(sub(sc.P, v3(sp.x, sp.y, sp.z)));
         Ng = N;
         sc.Poffset = scale(0.001f, N);
@@ -1088,11 +1079,31 @@ __device__ __forceinline__ int shade_pixel(int& next, const RenderView& rv, cons
         dPdu = normalize(axisu);
         dPdv = cross(Ng, dPdu);
     }
-    // ApplyTransform with identity matrices re-normalises (core/shader.go:129-135)
-    sc.N = normalize(N);
-    sc.Ng = normalize(Ng);
-    dPdu = normalize(dPdu);
-    dPdv = normalize(dPdv);
+    if (xf) {
+        // ApplyTransform (core/shader.go:129-135) with the hit's transform; Poffset stays in object space
+        float a[3], b[3];
+        const float Po[3] = {sc.P.x, sc.P.y, sc.P.z};
+        xf_point(M, Po, a);
+        sc.P = v3(a[0], a[1], a[2]);
+        float n1[3] = {N.x, N.y, N.z};
+        if (mesh_xf) { xf_vec_t(Mmi, n1, b); n1[0] = b[0]; n1[1] = b[1]; n1[2] = b[2]; }
+        xf_vec_t(Minv, n1, a);
+        sc.N = normalize(v3(a[0], a[1], a[2]));
+        const float ng[3] = {Ng.x, Ng.y, Ng.z};
+        xf_vec_t(Minv, ng, a);
+        sc.Ng = normalize(v3(a[0], a[1], a[2]));
+        const float du[3] = {dPdu.x, dPdu.y, dPdu.z}, dv[3] = {dPdv.x, dPdv.y, dPdv.z};
+        xf_vec(M, du, a);
+        dPdu = normalize(v3(a[0], a[1], a[2]));
+        xf_vec(M, dv, a);
+        dPdv = normalize(v3(a[0], a[1], a[2]));
+    } else {
+        // ApplyTransform with identity matrices re-normalises (core/shader.go:129-135)
+        sc.N = normalize(N);
+        sc.Ng = normalize(Ng);
+        dPdu = normalize(dPdu);
+        dPdv = normalize(dPdv);
+    }
     sc.lambda = pix[p].w;
     sc.I = iter;
     const uint64_t scr4 = scr[(size_t)p * 6 + 4], scr5 = scr[(size_t)p * 6 + 5];

@@ -7,6 +7,7 @@
 
 #include "../../include/vermeer_gpu.h"
 #include "trace_kernels.h"
+#include "xform.h"
 
 namespace vgpu {
 
@@ -67,7 +68,7 @@ struct RenderView {
 };
 
 // A camera motion key: TransformDecomp (math/animdecomp.go:13-17) as T[3], R (X, Y, Z, W), S[16], one pad.
-constexpr int kCamKeyFloats = 24;
+constexpr int kCamKeyFloats = kXfKeyFloats;
 constexpr int kMaxCamKeys = 64;
 
 constexpr int kMaxLevel = 3;       // ShaderStd.Eval returns at once above level 3 (std.go:97-99)

@@ -257,6 +257,8 @@ hipEvent_t timing_event(vgpu_scene* s) {
 
 int enqueue_trace(vgpu_scene* s, uint32_t flags, const void* d_rays, uint64_t n, void* d_hits, int32_t* d_geoms,
                   int32_t* d_counters, hipStream_t st, const vgpu_trace_config& cfg, const uint32_t* n_dev) {
+    if (d_counters && s->view.any_xform)
+        return fail(VGPU_ERR_ARG, "per-ray counters are not kept for scenes with instances, procs or mesh transforms");
     vgpu::TraceLaunch L{cfg.block, cfg.lds_stack, cfg.waves_per_cu, cfg.refill > 0 ? cfg.refill : vgpu::kRefillDefault};
     const uint64_t threads = vgpu::trace_grid_threads(L, n, s->num_cus);
     int rc = ensure_spill(s, threads);
@@ -431,6 +433,99 @@ int vgpu_scene_add_sphere(vgpu_scene* s, const float* centre, float radius, int3
     return VGPU_OK;
 }
 
+namespace {
+// Checks nkeys column-major matrices of a transform (finite; one key: invertible, as every ray inverts it).
+int check_transform(const float* mats, uint32_t nkeys, bool allow_none) {
+    if (nkeys == 0) return allow_none ? VGPU_OK : fail(VGPU_ERR_ARG, "a transform needs at least one key");
+    if (!mats) return fail(VGPU_ERR_ARG, "transform matrices are NULL");
+    if (nkeys > 64) return fail(VGPU_ERR_ARG, "at most 64 transform keys");
+    for (size_t i = 0; i < (size_t)nkeys * 16; i++)
+        if (!std::isfinite(mats[i])) return fail(VGPU_ERR_ARG, "transform matrices must be finite");
+    float inv[16];
+    if (nkeys == 1 && !vgpu::xf_inverse(mats, inv)) return fail(VGPU_ERR_ARG, "transform matrix is singular");
+    return VGPU_OK;
+}
+int check_box(const float* bmin, const float* bmax) {
+    if (!bmin || !bmax) return fail(VGPU_ERR_ARG, "bounds are NULL");
+    for (int k = 0; k < 3; k++)
+        if (!std::isfinite(bmin[k]) || !std::isfinite(bmax[k])) return fail(VGPU_ERR_ARG, "bounds must be finite");
+    return VGPU_OK;
+}
+void set_user_box(MeshHost& m, const float* bmin, const float* bmax) {
+    // BoundingBox.GrowVec3 of BMin[0] then BMax[0] from an empty box (instance.go:126-133)
+    Box6 b;
+    b.reset();
+    b.grow_point(bmin);
+    b.grow_point(bmax);
+    std::memcpy(m.ubox, b.v, sizeof(m.ubox));
+}
+}  // namespace
+
+int vgpu_scene_set_transform(vgpu_scene* s, int32_t geom, const float* mats, uint32_t nkeys) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (geom < 0 || geom >= (int32_t)s->meshes.size()) return fail(VGPU_ERR_ARG, "geom %d out of range", geom);
+    MeshHost& m = s->meshes[geom];
+    if (m.kind != vgpu::kXfNone && m.kind != vgpu::kXfMesh) return fail(VGPU_ERR_ARG, "geom %d is an instance or proc", geom);
+    if (m.sphere) return fail(VGPU_ERR_ARG, "geom %d is an analytic sphere (no Transform)", geom);
+    int rc = check_transform(mats, nkeys, true);
+    if (rc) return rc;
+    m.xmats.assign(mats, mats + (size_t)nkeys * 16);
+    m.kind = nkeys ? vgpu::kXfMesh : vgpu::kXfNone;
+    return VGPU_OK;
+}
+
+int vgpu_scene_set_hidden(vgpu_scene* s, int32_t geom, int32_t hidden) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (geom < 0 || geom >= (int32_t)s->meshes.size()) return fail(VGPU_ERR_ARG, "geom %d out of range", geom);
+    s->meshes[geom].hidden = hidden != 0;
+    return VGPU_OK;
+}
+
+namespace {
+// GeomInstance (kind kXfInstance, one child) or Proc (kXfProc): nkeys transform keys (a proc with none
+// holds the identity, proc.go:154-157), bounds the box of bmin, bmax.
+int add_wrapper(vgpu_scene* s, int kind, const int32_t* children, uint32_t nchild, const float* mats, uint32_t nkeys,
+                const float* bmin, const float* bmax, int32_t* geom_out) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (nchild == 0 || !children) return fail(VGPU_ERR_ARG, "no geoms to hold");
+    for (uint32_t i = 0; i < nchild; i++) {
+        const int32_t c = children[i];
+        if (c < 0 || c >= (int32_t)s->meshes.size()) return fail(VGPU_ERR_ARG, "geom %d out of range", c);
+        if (s->meshes[c].kind >= vgpu::kXfInstance)
+            return fail(VGPU_ERR_ARG, "geom %d is an instance or proc (they hold meshes and spheres only)", c);
+    }
+    int rc = check_transform(mats, nkeys, kind == vgpu::kXfProc);
+    if (!rc) rc = check_box(bmin, bmax);
+    if (rc) return rc;
+    MeshHost m;
+    m.kind = kind;
+    m.built = true;
+    m.children.assign(children, children + nchild);
+    if (nkeys) m.xmats.assign(mats, mats + (size_t)nkeys * 16);
+    else {
+        m.xmats.assign(16, 0.0f);
+        m.xmats[0] = m.xmats[5] = m.xmats[10] = m.xmats[15] = 1.0f;
+    }
+    set_user_box(m, bmin, bmax);
+    s->meshes.push_back(std::move(m));
+    if (geom_out) *geom_out = (int32_t)s->meshes.size() - 1;
+    return VGPU_OK;
+}
+}  // namespace
+
+int vgpu_scene_add_instance(vgpu_scene* s, int32_t geom, const float* mats, uint32_t nkeys, const float* bmin,
+                            const float* bmax, int32_t* geom_out) {
+    return add_wrapper(s, vgpu::kXfInstance, &geom, 1, mats, nkeys, bmin, bmax, geom_out);
+}
+
+int vgpu_scene_add_proc(vgpu_scene* s, const int32_t* geoms, uint32_t ngeoms, const float* mats, uint32_t nkeys,
+                        const float* bmin, const float* bmax, int32_t* geom_out) {
+    return add_wrapper(s, vgpu::kXfProc, geoms, ngeoms, mats, nkeys, bmin, bmax, geom_out);
+}
+
 int vgpu_scene_add_mesh_prebuilt(vgpu_scene* s, const float* verts, uint32_t nverts, const uint32_t* idxp_sorted,
                                  uint32_t ntris, const vgpu_qbvh_node* nodes, uint32_t nnodes, const float* bounds,
                                  float ray_bias, int32_t* geom_out) {
@@ -473,16 +568,29 @@ int vgpu_scene_commit(vgpu_scene* s) {
         int rc = m.nkeys == 1 ? build_static_mesh(m, s->leaf_max) : build_motion_mesh(m);
         if (rc) return rc;
     }
-    // Scene.initAccel (scene.go:135-203)
-    const int ng = (int)s->meshes.size();
+    // Scene.initAccel (scene.go:135-203) over the geoms the scene holds itself; hidden geoms (reached only
+    // through instances and procs) take the slots after them, in insertion order.
+    const int nall = (int)s->meshes.size();
+    std::vector<int32_t> vis;
+    for (int i = 0; i < nall; i++) if (!s->meshes[i].hidden) vis.push_back(i);
+    if (vis.empty()) return fail(VGPU_ERR_STATE, "every geom is hidden");
+    const int ng = (int)vis.size();
+    // Geom.MotionKeys: a mesh's vertex keys (polymesh.go:102-112), an instance's geom's
+    // (instance.go:151-156), a proc's transform keys (proc.go:188-193)
+    auto motion_keys_of = [&](int i) -> int {
+        const MeshHost& m = s->meshes[i];
+        if (m.kind == vgpu::kXfInstance) return (int)s->meshes[m.children[0]].nkeys;
+        if (m.kind == vgpu::kXfProc) return (int)(m.xmats.size() / 16);
+        return (int)m.nkeys;
+    };
     int maxKeys = 0;
-    for (auto& m : s->meshes) maxKeys = std::max<int>(maxKeys, (int)m.nkeys);
+    for (int i : vis) maxKeys = std::max(maxKeys, motion_keys_of(i));
     std::vector<float> boxes((size_t)ng * 6), cent((size_t)ng * 3);
     std::vector<int32_t> ids(ng);
     const float t_build = maxKeys == 1 ? 0.0f : 0.5f;
     for (int i = 0; i < ng; i++) {
         float b[6];
-        s->meshes[i].bounds_at(t_build, b);
+        s->meshes[vis[i]].bounds_at(t_build, b);
         std::memcpy(&boxes[(size_t)i * 6], b, sizeof(b));
         for (int k = 0; k < 3; k++) cent[(size_t)i * 3 + k] = (b[3 + k] + b[k]) * 0.5f;   // BoundingBox.Centroid
         ids[i] = i;
@@ -490,8 +598,10 @@ int vgpu_scene_commit(vgpu_scene* s) {
     {
         Box6 all;
         all.reset();
-        for (auto& m : s->meshes) {
-            if (m.nkeys == 1) all.grow_box(m.bounds);
+        for (int i : vis) {
+            const MeshHost& m = s->meshes[i];
+            float b[6];
+            if (m.kind >= vgpu::kXfInstance || m.nkeys == 1) { m.bounds_at(0.0f, b); all.grow_box(b); }
             else for (uint32_t k = 0; k < m.nkeys; k++) all.grow_box(&m.motion_bounds[(size_t)k * 6]);
         }
         std::memcpy(s->box, all.v, sizeof(s->box));
@@ -514,7 +624,12 @@ int vgpu_scene_commit(vgpu_scene* s) {
         s->top_motion = true;
         s->top_keys = maxKeys;
     }
-    s->order = ids;
+    s->order.resize(nall);
+    for (int i = 0; i < ng; i++) s->order[i] = vis[ids[i]];
+    {
+        int slot = ng;
+        for (int i = 0; i < nall; i++) if (s->meshes[i].hidden) s->order[slot++] = i;
+    }
     if (s->top_motion) {
         // scene.go:207-268
         s->top_mboxes.assign((size_t)maxKeys * nn * 24, 0.0f);
@@ -540,19 +655,23 @@ int vgpu_scene_commit(vgpu_scene* s) {
     // Flatten into the device layout.
     std::vector<vgpu::DevNode> dnodes;
     std::vector<float4> dtris, dmnodes, dmtris, dspheres;
-    std::vector<vgpu::DevGeom> dgeoms(ng);
+    std::vector<vgpu::DevGeom> dgeoms(nall);
     if (!s->top_motion) {
         for (auto& nd : s->top_nodes) { vgpu::DevNode d; to_dev(nd, d); dnodes.push_back(d); }
     } else {
         s->view.top_mbase = 0;
         append_motion_nodes(dmnodes, s->top_mnodes, s->top_mboxes, maxKeys);
     }
-    for (int slot = 0; slot < ng; slot++) {
+    for (int slot = 0; slot < nall; slot++) {
         MeshHost& m = s->meshes[s->order[slot]];
         vgpu::DevGeom& g = dgeoms[slot];
         g.nkeys = (int32_t)m.nkeys;
         g.ray_bias = m.ray_bias;
-        if (m.sphere) {   // nkeys 0 marks an analytic sphere; tri_base indexes the sphere table
+        if (m.kind >= vgpu::kXfInstance) {   // instance or proc: nothing of its own to trace
+            g.nkeys = -1;
+            g.node_base = -1;
+            g.tri_base = -1;
+        } else if (m.sphere) {   // nkeys 0 marks an analytic sphere; tri_base indexes the sphere table
             g.nkeys = 0;
             g.node_base = -1;
             g.tri_base = (int32_t)dspheres.size();
@@ -597,8 +716,8 @@ int vgpu_scene_commit(vgpu_scene* s) {
         std::vector<float4> dn;
         std::vector<uint8_t> dsi;
         std::vector<int32_t> dsh;
-        std::vector<vgpu::DevGeomShade> dgs(ng);
-        for (int slot = 0; slot < ng; slot++) {
+        std::vector<vgpu::DevGeomShade> dgs(nall);
+        for (int slot = 0; slot < nall; slot++) {
             MeshHost& m = s->meshes[s->order[slot]];
             vgpu::DevGeomShade& gsd = dgs[slot];
             gsd.nrm_base = -1;
@@ -646,6 +765,59 @@ int vgpu_scene_commit(vgpu_scene* s) {
         if ((rc = upload(&s->d_shaderidx, dsi))) return rc;
         if ((rc = upload(&s->d_shaders, dsh))) return rc;
         if ((rc = upload(&s->d_gshade, dgs))) return rc;
+    }
+    // Transforms per slot (xform.h records; one key precomputed with its inverse)
+    {
+        bool any = false;
+        for (auto& m : s->meshes) any = any || m.kind != vgpu::kXfNone;
+        s->view.any_xform = any ? 1 : 0;
+        if (any) {
+            std::vector<int32_t> slot_of(nall);
+            for (int slot = 0; slot < nall; slot++) slot_of[s->order[slot]] = slot;
+            std::vector<vgpu::DevXform> dx(nall);
+            std::vector<float> keys;
+            std::vector<int32_t> kids;
+            for (int slot = 0; slot < nall; slot++) {
+                const MeshHost& m = s->meshes[s->order[slot]];
+                vgpu::DevXform& x = dx[slot];
+                x = vgpu::DevXform{m.kind, 0, -1, -1, 0, 0, 0, 0};
+                if (m.kind == vgpu::kXfNone) continue;
+                const int nk = (int)(m.xmats.size() / 16);
+                x.nkeys = nk;
+                if (nk == 1) {
+                    float M[16], Minv[16];
+                    if (m.kind == vgpu::kXfMesh) std::memcpy(M, m.xmats.data(), sizeof(M));   // trace.go:45-47: as given
+                    else {   // TimeKey returns the one decomposition, recomposed (instance.go:17-34)
+                        float rec[vgpu::kXfKeyFloats];
+                        decompose_transform(m.xmats.data(), rec);
+                        vgpu::xf_decomp_matrix(rec, rec, 0.0f, M);
+                    }
+                    if (!vgpu::xf_inverse(M, Minv)) std::fill(Minv, Minv + 16, 0.0f);
+                    x.mat_base = (int32_t)keys.size();
+                    keys.insert(keys.end(), M, M + 16);
+                    keys.insert(keys.end(), Minv, Minv + 16);
+                } else {
+                    x.key_base = (int32_t)keys.size();
+                    for (int k = 0; k < nk; k++) {
+                        float rec[vgpu::kXfKeyFloats];
+                        decompose_transform(m.xmats.data() + 16 * k, rec);
+                        keys.insert(keys.end(), rec, rec + vgpu::kXfKeyFloats);
+                    }
+                }
+                if (m.kind >= vgpu::kXfInstance) {
+                    x.child_base = (int32_t)kids.size();
+                    x.nchild = (int32_t)m.children.size();
+                    for (int32_t c : m.children) kids.push_back(slot_of[c]);
+                }
+            }
+            if (kids.empty()) kids.push_back(-1);
+            if ((rc = upload(&s->d_xforms, dx))) return rc;
+            if ((rc = upload(&s->d_xkeys, keys))) return rc;
+            if ((rc = upload(&s->d_xchildren, kids))) return rc;
+        }
+        s->view.xforms = s->d_xforms;
+        s->view.xkeys = s->d_xkeys;
+        s->view.xchildren = s->d_xchildren;
     }
     s->view.nodes = s->d_nodes;
     s->view.tris = s->d_tris;

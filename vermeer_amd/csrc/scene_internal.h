@@ -74,11 +74,20 @@ struct MeshHost {
     std::vector<int32_t> shaders;      // shader table: material ids
     bool sphere = false;               // analytic sphere geom (vgpu_scene_add_sphere): sphere_c, sphere_r
     float sphere_c[3] = {0, 0, 0}, sphere_r = 0;
+    // Transforms: a mesh's own Transform (kind kXfNone/kXfMesh), or a GeomInstance / Proc (kXfInstance,
+    // kXfProc) over `children` (insertion indices) with bounds ubox (lo xyz, hi xyz).
+    int kind = 0;
+    std::vector<float> xmats;          // transform keys, 16 floats each (column-major)
+    std::vector<int32_t> children;
+    float ubox[6] = {0, 0, 0, 0, 0, 0};
+    bool hidden = false;               // left out of the scene's top level
 
     const float* vert(uint32_t key, uint32_t i) const { return &verts[((size_t)key * nverts + i) * 3]; }
 
-    // PolyMesh.Bounds (builtin/geom/polymesh/bounds.go:159-186)
+    // PolyMesh.Bounds (builtin/geom/polymesh/bounds.go:159-186; a mesh's Transform is ignored there,
+    // bounds.go:26); an instance's or proc's given box whatever the time (instance.go:63-67)
     void bounds_at(float time, float* out) const {
+        if (kind >= 2) { std::memcpy(out, ubox, sizeof(ubox)); return; }
         if (nkeys == 1) { std::memcpy(out, bounds, sizeof(bounds)); return; }
         const float k = time * (float)(nkeys - 1);
         const float t = k - std::floor(k);
@@ -120,6 +129,9 @@ struct vgpu_scene {
     uint8_t* d_shaderidx = nullptr;
     int32_t* d_shaders = nullptr;
     vgpu::DevGeomShade* d_gshade = nullptr;
+    vgpu::DevXform* d_xforms = nullptr;  // transforms (scenes with instances, procs or mesh transforms)
+    float* d_xkeys = nullptr;
+    int32_t* d_xchildren = nullptr;
     int32_t max_material = -1;
     vgpu_trace_config cfg{256, 16, 16, 0, 0};
     bool cfg_set = false;               // the caller chose cfg (set_trace_config or autotune)
@@ -138,7 +150,8 @@ struct vgpu_scene {
         int prev = 0;
         if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(device);
         void* ptrs[] = {d_nodes, d_tris, d_mnodes, d_mtris, d_geoms, d_spheres, d_err, d_spT, d_spN, d_stage, d_normals, d_shaderidx,
-                        d_shaders, d_gshade, sort.keys, sort.perm, sort.rays, sort.hist, sort.bsum};
+                        d_shaders, d_gshade, sort.keys, sort.perm, sort.rays, sort.hist, sort.bsum,
+                        d_xforms, d_xkeys, d_xchildren};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         (void)hipSetDevice(prev);
@@ -167,6 +180,8 @@ void vec3_cross(const float* a, const float* b, float* o);
 int lookat_keys(const float* from, int nfrom, const float* to, int nto, const float* up, const float* roll, int nroll,
                 std::vector<float>& out);
 bool mat4_inverse(const float* m, float* out);
+// TransformDecompMatrix4 (math/animdecomp.go:20-63) of a column-major matrix: one kXfKeyFloats record.
+void decompose_transform(const float* m, float* rec);
 
 // Ensures the ray-sort buffers hold n rays.
 int ensure_sort(vgpu_scene* s, uint64_t n);

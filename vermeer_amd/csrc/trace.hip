@@ -25,6 +25,7 @@
 #include <mutex>
 
 #include "trace_kernels.h"
+#include "xform.h"
 #include "../../include/vermeer_gpu.h"
 
 namespace vgpu {
@@ -676,6 +677,241 @@ __global__ __launch_bounds__(BLOCK) void k_trace_wide(SceneView sv, const float4
 #undef VGPU_IN_MESH
 
 
+// ---------------------------------------------------------------- scenes with transforms
+//
+// Scenes holding GeomInstances, Procs or meshes with their own Transform run this kernel: one ray
+// per lane, the reference's recursion written out (Scene.Trace -> Geom.Trace -> qbvh.Trace ->
+// TraceElems), each level on the same stack above the level that called it.  A geom with a
+// transform moves the ray into its object space (P and D by the inverse transform at the ray's time,
+// D not renormalised, then Setup), traces what it holds and restores the ray, Tclosest excepted
+// (instance.go:78-104, proc.go:93-131, polymesh/trace.go:20-74).
+
+// The ray in object space: P and D moved by Minv, then Ray.Setup.
+__device__ __forceinline__ void ray_to_object(const RayS& r, const float* D, const float* Minv, RayS& o, float* Do) {
+    xf_point(Minv, r.P, o.P);
+    xf_vec(Minv, D, Do);
+    ray_setup(o, Do);
+    o.time = r.time;
+    o.tclosest = r.tclosest;
+}
+
+struct XHit {
+    int prim;
+    float U, V, W;
+};
+
+// qbvh.Trace / TraceMotion over one mesh (or the analytic sphere test) with the ray in the mesh's
+// space, its stack starting at entry `base`; true when it improved the hit.  -1 in *over: the stack
+// overflowed.
+template <bool SHADOW>
+__device__ bool trace_mesh(const SceneView& sv, const DevGeom& g, RayS& r, const float* D, Stack<64, 8>& st, int base,
+                           XHit& h, int* over) {
+    if (g.nkeys == 0) {   // analytic sphere (builtin/geom/sphere/trace.go:13-52)
+        float t;
+        const float4 d = make_float4(D[0], D[1], D[2], 0.0f);
+        if (!sphere_geom_hit(r.P, d, sv.spheres[g.tri_base], r.tclosest, t)) return false;
+        r.tclosest = t;
+        h.prim = 0; h.U = 0.0f; h.V = 0.0f; h.W = 0.0f;
+        return true;
+    }
+    const int kp = pack_perm(r);
+    const TriRay q = tri_ray_packed(r, kp);
+    int k0 = 0, k1 = 0;
+    float mt = 0.0f;
+    if (g.nkeys > 1) motion_keys(r.time, g.nkeys, k0, k1, mt);
+    bool hit = false;
+    st.store(base, r.tclosest, 0);
+    int sp = base + 1;
+    while (sp > base) {
+        sp--;
+        float T; int32_t node;
+        st.load(sp, T, node);
+        if (r.tclosest < T || node == -1) continue;
+        if (node >= 0) {
+            NodeData nd;
+            Box4 b;
+            if (g.nkeys > 1) {
+                box_motion(r, sv.mnodes, g.node_base, node, g.nkeys, k0, k1, mt, nd, b);
+            } else {
+                fetch_static(sv.nodes, g.node_base + node, nd);
+                box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+            }
+            const int n0 = (kp >> nd.ax[0]) & 1, n1 = (kp >> nd.ax[1]) & 1, n2 = (kp >> nd.ax[2]) & 1;
+            const int gA = n0 ? 0 : 2, gB = 2 - gA;
+            const int rk[4] = {gA + (n1 ^ 1), gA + n1, gB + (n2 ^ 1), gB + n2};
+            if (sp + 4 > VGPU_MAX_STACK) { *over = 1; return hit; }
+            int m = 0;
+            for (int c = 0; c < 4; c++) if (b.h[c]) m |= 1 << rk[c];
+            for (int c = 0; c < 4; c++)
+                if (b.h[c]) st.store(sp + __popc(m & ((1 << rk[c]) - 1)), b.t[c], nd.ch[c]);
+            sp += __popc(m);
+        } else {
+            const int lb = leaf_base(node), lc = leaf_count(node);
+            bool leaf_hit = false;
+            for (int i = 0; i < lc; i++) {
+                float U, V, W;
+                bool ok;
+                if (g.nkeys > 1) {
+                    float4 v0, v1, v2;
+                    motion_tri(sv.mtris + g.tri_base + (size_t)(lb + i) * 3 * g.nkeys, k0, k1, mt, v0, v1, v2);
+                    ok = tri_test_q<true>(r, q, v0, v1, v2, g.ray_bias, U, V, W);
+                } else {
+                    const float4* tp = sv.tris + ((size_t)g.tri_base + lb + i) * 3;
+                    ok = tri_test_q<false>(r, q, tp[0], tp[1], tp[2], g.ray_bias, U, V, W);
+                }
+                if (ok) { h.prim = lb + i; h.U = U; h.V = V; h.W = W; leaf_hit = true; }
+            }
+            if (leaf_hit) {
+                hit = true;
+                if (SHADOW) return true;   // qbvh/intersect.go:232-236
+            }
+        }
+    }
+    return hit;
+}
+
+// Geom.Trace of a mesh or sphere slot, through its own Transform when it has one.
+template <bool SHADOW>
+__device__ bool trace_leaf_geom(const SceneView& sv, int slot, RayS& r, const float* D, Stack<64, 8>& st, int base,
+                                XHit& h, int* over) {
+    const DevGeom g = sv.geoms[slot];
+    const DevXform x = sv.xforms[slot];
+    if (x.kind != kXfMesh) return trace_mesh<SHADOW>(sv, g, r, D, st, base, h, over);
+    float M[16], Minv[16];
+    xform_at(x, sv.xkeys, r.time, M, Minv);
+    RayS o;
+    float Do[3];
+    ray_to_object(r, D, Minv, o, Do);
+    const bool hit = trace_mesh<SHADOW>(sv, g, o, Do, st, base, h, over);
+    r.tclosest = o.tclosest;
+    return hit;
+}
+
+// Geom.Trace of a top-level slot: a mesh or sphere, or an instance or proc over its geoms.  *leaf:
+// the slot whose element was hit.  A proc stops at the first geom reporting a hit unless the ray is
+// a shadow ray (proc.go:112-120).
+template <bool SHADOW>
+__device__ bool trace_geom(const SceneView& sv, int slot, RayS& r, const float* D, Stack<64, 8>& st, int base, XHit& h,
+                           int& leaf, int* over) {
+    const DevXform x = sv.xforms[slot];
+    if (x.kind != kXfInstance && x.kind != kXfProc) {
+        const bool hit = trace_leaf_geom<SHADOW>(sv, slot, r, D, st, base, h, over);
+        if (hit) leaf = slot;
+        return hit;
+    }
+    float M[16], Minv[16];
+    xform_at(x, sv.xkeys, r.time, M, Minv);
+    RayS o;
+    float Do[3];
+    ray_to_object(r, D, Minv, o, Do);
+    bool hit = false;
+    for (int c = 0; c < x.nchild && !*over; c++) {
+        const int child = sv.xchildren[x.child_base + c];
+        if (trace_leaf_geom<SHADOW>(sv, child, o, Do, st, base, h, over)) {
+            hit = true;
+            leaf = child;
+            if (x.kind == kXfInstance || !SHADOW) break;
+        }
+    }
+    r.tclosest = o.tclosest;
+    return hit;
+}
+
+// The whole traversal for one ray per lane.  EXT: 32-byte records {prim, geom, t, U, V, W, leaf slot, 0}.
+template <bool SHADOW, bool EXT>
+__global__ __launch_bounds__(64) void k_trace_xform(SceneView sv, const float4* __restrict__ rays, uint64_t n,
+                                                    int4* __restrict__ hits, int32_t* __restrict__ geoms_out,
+                                                    float* __restrict__ spT, int32_t* __restrict__ spN,
+                                                    int32_t* __restrict__ err, const uint32_t* __restrict__ n_dev,
+                                                    const uint32_t* __restrict__ perm) {
+    __shared__ float sT[8 * 64];
+    __shared__ int32_t sN[8 * 64];
+    const int tid = threadIdx.x;
+    const uint64_t gtid = (uint64_t)blockIdx.x * 64 + tid;
+    const uint64_t nthr = (uint64_t)gridDim.x * 64;
+    Stack<64, 8> st;
+    st.sT = sT + tid; st.sN = sN + tid;
+    st.gT = spT; st.gN = spN; st.gstride = nthr; st.gofs = (uint32_t)gtid;
+    if (n_dev) n = min(n, (uint64_t)*n_dev);
+    for (uint64_t ri = gtid; ri < n; ri += nthr) {
+        const float4 ra = rays[2 * ri], rb = rays[2 * ri + 1];
+        RayS r;
+        r.P[0] = ra.x; r.P[1] = ra.y; r.P[2] = ra.z;
+        r.tclosest = ra.w;
+        r.time = rb.w;
+        const float D[3] = {rb.x, rb.y, rb.z};
+        ray_setup(r, D);
+        const int kp = pack_perm(r);
+        XHit h{-1, 0.0f, 0.0f, 0.0f};
+        int best_geom = -1, best_leaf = -1, over = 0;
+        const bool top_motion = sv.top_motion != 0;
+        int k0 = 0, k1 = 0;
+        float mt = 0.0f;
+        if (top_motion) motion_keys(r.time, sv.top_keys, k0, k1, mt);
+        st.store(0, r.tclosest, 0);
+        int sp = 1;
+        bool done = false;
+        while (sp > 0 && !done && !over) {
+            sp--;
+            float T; int32_t node;
+            st.load(sp, T, node);
+            if (r.tclosest < T || node == -1) continue;
+            if (node >= 0) {
+                NodeData nd;
+                Box4 b;
+                if (top_motion) {
+                    box_motion(r, sv.mnodes, sv.top_mbase, node, sv.top_keys, k0, k1, mt, nd, b);
+                } else {
+                    fetch_static(sv.nodes, node, nd);
+                    box4<false>(r, nd.lx, nd.ly, nd.lz, nd.hx, nd.hy, nd.hz, b);
+                }
+                const int n0 = (kp >> nd.ax[0]) & 1, n1 = (kp >> nd.ax[1]) & 1, n2 = (kp >> nd.ax[2]) & 1;
+                const int gA = n0 ? 0 : 2, gB = 2 - gA;
+                const int rk[4] = {gA + (n1 ^ 1), gA + n1, gB + (n2 ^ 1), gB + n2};
+                if (sp + 4 > VGPU_MAX_STACK) { over = 1; break; }
+                int m = 0;
+                for (int c = 0; c < 4; c++) if (b.h[c]) m |= 1 << rk[c];
+                for (int c = 0; c < 4; c++)
+                    if (b.h[c]) st.store(sp + __popc(m & ((1 << rk[c]) - 1)), b.t[c], nd.ch[c]);
+                sp += __popc(m);
+            } else {
+                // Scene.TraceElems over the leaf's geoms (scene.go:61-78)
+                const int lb = leaf_base(node), lc = leaf_count(node);
+                for (int gi = lb; gi < lb + lc && !over; gi++) {
+                    int leaf = -1;
+                    if (trace_geom<SHADOW>(sv, gi, r, D, st, sp, h, leaf, &over)) {
+                        best_geom = gi;
+                        best_leaf = leaf;
+                        if (SHADOW) { done = true; break; }
+                    }
+                }
+            }
+        }
+        if (over) {
+            atomicOr(err, 1);
+            best_geom = -1; best_leaf = -1; h.prim = -1;
+        }
+        if (best_geom < 0) h.prim = -1;
+        const uint64_t oi = perm ? (uint64_t)perm[ri] : ri;
+        if (EXT) {
+            hits[2 * oi] = make_int4(h.prim, best_geom, __float_as_int(r.tclosest), __float_as_int(h.U));
+            hits[2 * oi + 1] = make_int4(__float_as_int(h.V), __float_as_int(h.W), best_leaf, 0);
+        } else {
+            hits[oi] = make_int4(h.prim, __float_as_int(r.tclosest), __float_as_int(h.U), __float_as_int(h.V));
+            if (geoms_out) geoms_out[oi] = best_geom;
+        }
+    }
+}
+
+template <bool SHADOW, bool EXT>
+hipError_t launch_xform(const SceneView& sv, const void* rays, uint64_t n, void* hits, int32_t* geoms, float* spT,
+                        int32_t* spN, uint64_t threads, int32_t* err, const uint32_t* n_dev, const uint32_t* perm,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL((k_trace_xform<SHADOW, EXT>), dim3((unsigned)(threads / 64)), dim3(64), 0, stream, sv,
+                       (const float4*)rays, n, (int4*)hits, geoms, spT, spN, err, n_dev, perm);
+    return hipGetLastError();
+}
+
 template <int BLOCK, int LDS, bool SHADOW, bool MOTION>
 hipError_t launch_cfg(const SceneView& sv, const void* rays, uint64_t n, void* hits, int32_t* geoms, int32_t* counters,
                       float* spT, int32_t* spN, uint64_t grid, int32_t* err, const uint32_t* n_dev,
@@ -719,6 +955,8 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
     const unsigned grid = (unsigned)(threads / cfg.block);
+    if (sv.any_xform)
+        return launch_xform<false, true>(sv, rays, n, hits32, nullptr, spT, spN, threads, err, n_dev, nullptr, stream);
     if (motion)
         hipLaunchKernelGGL((k_trace_wide<kBM, kLM, false, true, false, true>), dim3(grid), dim3(kBM), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
@@ -747,6 +985,11 @@ hipError_t launch_trace(const SceneView& sv, bool shadow, bool motion, const Tra
     const uint64_t threads = trace_grid_threads(cfg, n, num_cus);
     if (threads > spill_threads) return hipErrorInvalidValue;
     const uint64_t grid = threads / cfg.block;
+    if (sv.any_xform) {   // the per-lane kernel (block 64; threads is a multiple of 64); no counters
+        if (counters) return hipErrorInvalidValue;
+        return shadow ? launch_xform<true, false>(sv, rays, n, hits, geoms, spT, spN, threads, err, n_dev, perm, stream)
+                      : launch_xform<false, false>(sv, rays, n, hits, geoms, spT, spN, threads, err, n_dev, perm, stream);
+    }
 #define VGPU_CASE(B, L)                                                                                         \
     if (cfg.block == B && cfg.lds_depth == L)                                                                   \
         return launch_mode<B, L>(sv, shadow, motion, rays, n, hits, geoms, counters, spT, spN, grid, err, n_dev, perm, \

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "xform.h"
+
 namespace vgpu {
 
 // Idle lanes that trigger a refill of the wavefront in the traversal kernel; 48 of 64 measured
@@ -42,6 +44,43 @@ struct alignas(16) DevGeom {
     float ray_bias;     // PolyMesh.RayBias (builtin/geom/polymesh/polymesh.go:19)
 };
 
+// Per geom slot: the transform a geom applies to rays before tracing what it holds.  kind:
+//   kXfNone      a mesh or sphere without a transform
+//   kXfMesh      a PolyMesh with its own Transform (builtin/geom/polymesh/trace.go:20-74)
+//   kXfInstance  a GeomInstance of child 0 (builtin/geom/instance/instance.go:73-114)
+//   kXfProc      a Proc over its children, in order (builtin/geom/proc/proc.go:88-139)
+// One key: M and M^-1 precomputed at mat_base (32 floats; a mesh's single key is its raw matrix, an
+// instance's or proc's the recomposed TransformDecomp).  More: nkeys TransformDecomp records
+// (kXfKeyFloats each) at key_base, interpolated at the ray's time.
+constexpr int kXfNone = 0, kXfMesh = 1, kXfInstance = 2, kXfProc = 3;
+struct alignas(16) DevXform {
+    int32_t kind;
+    int32_t nkeys;
+    int32_t key_base;      // float offset in SceneView::xkeys (more than one key)
+    int32_t mat_base;      // float offset in SceneView::xkeys (one key)
+    int32_t child_base;    // first entry in SceneView::xchildren (instances and procs: geom slots)
+    int32_t nchild;
+    int32_t pad0, pad1;
+};
+
+// The transform of a geom at a ray's time and its inverse (a singular matrix inverts to zero, as
+// Matrix4Inverse leaves it).  Keys as polymesh/trace.go:27-44 and TransformSRTArray.TimeKey
+// (instance.go:17-34) pick them; a time outside [0, 1] is held to the end keys.
+__device__ inline void xform_at(const DevXform& x, const float* xkeys, float time, float* M, float* Minv) {
+    if (x.nkeys == 1) {
+        const float* m = xkeys + x.mat_base;
+        for (int i = 0; i < 16; i++) { M[i] = m[i]; Minv[i] = m[16 + i]; }
+        return;
+    }
+    const float k = time * (float)(x.nkeys - 1);
+    const float fk = floorf(k);
+    const float t = k - fk;
+    const int k0 = min(max((int)fk, 0), x.nkeys - 1), k1 = min(max((int)ceilf(k), 0), x.nkeys - 1);
+    xf_decomp_matrix(xkeys + x.key_base + k0 * kXfKeyFloats, xkeys + x.key_base + k1 * kXfKeyFloats, t, M);
+    if (!xf_inverse(M, Minv))
+        for (int i = 0; i < 16; i++) Minv[i] = 0.0f;
+}
+
 struct SceneView {
     const DevNode* nodes;     // static nodes: top level at [0, ntop) when the top is static
     const float4* tris;       // static triangles
@@ -53,6 +92,10 @@ struct SceneView {
     int32_t top_keys;         // motion keys of the top level
     int32_t top_mbase;
     int32_t nan_free_boxes;   // 1: no static node box holds a NaN bound
+    int32_t any_xform;        // 1: some geom applies a transform (the per-lane kernel traces every ray)
+    const DevXform* xforms;   // per geom slot (any_xform only)
+    const float* xkeys;       // transform records
+    const int32_t* xchildren; // geom slots held by instances and procs
 };
 
 struct TraceLaunch {

@@ -86,6 +86,7 @@ class Renderer:
             check(self._lib.vgpu_scene_add_shaded_mesh(self.scene._h, ptr(v), len(v), ptr(ix), len(ix), ptr(nrm),
                                                        ptr(sidx), ptr(sh), len(sh), C.byref(g)))
             self.scene.geoms.append(None)
+        self.scene.add_transforms(desc)
         self.scene.pre_render()
         mats = np.ascontiguousarray(desc["materials"], MATERIAL_DTYPE)
         lights = light_records(desc["lights"])

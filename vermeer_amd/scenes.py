@@ -523,6 +523,93 @@ def feature_scene(seed=5, nseg=48, nring=24, mirrors=True):
             "camera": camera}
 
 
+def trs_matrix(t=(0.0, 0.0, 0.0), yaw=0.0, pitch=0.0, scale=(1.0, 1.0, 1.0)):
+    """Column-major 4x4 (16 float32): translate(t) * rot_y(yaw) * rot_x(pitch) * scale."""
+    cy, sy, cp, sp = np.cos(yaw), np.sin(yaw), np.cos(pitch), np.sin(pitch)
+    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    rx = np.array([[1, 0, 0], [0, cp, -sp], [0, sp, cp]])
+    m = np.eye(4)
+    m[:3, :3] = ry @ rx @ np.diag(scale)
+    m[:3, 3] = t
+    return m.astype(np.float32).T.reshape(16)
+
+
+def instance_scene(seed=11, count=12, nseg=24, nring=12):
+    """Transforms in the scene graph: the feature scene's box room and lights (its spheres dropped), with
+    three hidden prototype meshes (a displaced UV sphere, an icosphere, a box) held by `count`
+    GeomInstances of random rotation, non-uniform scale and position (every third one with two keys: it
+    moves and turns over the shutter); a Proc holding two hidden meshes (a small sphere and a slab) under
+    a rotation; and a visible mesh with its own two-key Transform (a plank sliding inside its box, so the
+    reference's untransformed bounds still cover it).  Materials: Oren-Nayar, dielectric GGX, conductor
+    GGX and a perfect mirror.  Geom insertion order: box, disk fan, the three prototypes, the proc's two
+    meshes, the plank; then the instances, then the proc."""
+    g = rng(seed)
+    base = feature_scene(seed=seed, nseg=8, nring=4, mirrors=True)
+    mats = [base["materials"][0], base["materials"][5]]   # walls, disk light
+    box_v, box_f = [], []
+    nv = 0
+    for v, f in [quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)),
+                 quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),
+                 quad((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)),
+                 quad((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)),
+                 quad((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1))]:
+        n = np.cross(v[f[0, 1]] - v[f[0, 0]], v[f[0, 2]] - v[f[0, 0]])
+        if np.dot(n, np.array([0.0, 1.0, 0.0]) - v[0]) < 0:
+            f = f[:, ::-1]
+        box_v.append(v)
+        box_f.append(f + nv)
+        nv += len(v)
+    box = {"verts": np.concatenate(box_v).astype(np.float32), "idx": np.concatenate(box_f).astype(np.uint32),
+           "shaderidx": None, "shaders": np.array([0], np.int32), "normals": None}
+    disk = dict(base["meshes"][1], shaders=np.array([1], np.int32))
+    lights = [dict(base["lights"][0], material=1, geom=1)]
+    m_on = len(mats); mats.append(material(diffuse=(0.75, 0.45, 0.3), diffuse_strength=1.0, diffuse_roughness=0.4))
+    m_gx = len(mats); mats.append(material(diffuse=(0.2, 0.5, 0.3), diffuse_strength=0.5, spec=(1, 1, 1),
+                                           spec_strength=0.5, spec_roughness=0.25, ior=1.5))
+    m_mt = len(mats); mats.append(material(diffuse=(0.3, 0.3, 0.3), diffuse_strength=0.2, spec=(1.0, 0.9, 0.8),
+                                           spec_strength=0.8, spec_roughness=0.3, fresnel="metal",
+                                           fresnel_refl=(0.95, 0.64, 0.54)))
+    m_mr = len(mats); mats.append(material(spec=(1, 1, 1), spec_strength=1.0, spec_roughness=0.0, ior=1.5))
+    sv, sf = uv_sphere(1.0, nseg, nring)
+    sv = sv * (1.0 + 0.08 * (2.0 * value_noise(sv, seed) - 1.0))[:, None]
+    blob = {"verts": sv.astype(np.float32), "idx": sf.astype(np.uint32), "shaderidx": None,
+            "shaders": np.array([m_on], np.int32), "normals": None, "hidden": True}
+    iv, i_f = icosphere(2)
+    ico = {"verts": iv.astype(np.float32), "idx": i_f.astype(np.uint32), "shaderidx": None,
+           "shaders": np.array([m_mt], np.int32), "normals": None, "hidden": True}
+    cube_v = np.array([(x, y, z) for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float32)
+    cube_f = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1],
+                       [2, 3, 7], [2, 7, 6], [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.uint32)
+    cube = {"verts": cube_v, "idx": cube_f, "shaderidx": None, "shaders": np.array([m_gx], np.int32),
+            "normals": None, "hidden": True}
+    small = {"verts": (sv * 0.12 + np.array([0.0, 0.15, 0.0])).astype(np.float32), "idx": sf.astype(np.uint32),
+             "shaderidx": None, "shaders": np.array([m_mr], np.int32), "normals": None, "hidden": True}
+    slab_v = cube_v * np.array([0.25, 0.02, 0.12], np.float32)
+    slab = {"verts": slab_v, "idx": cube_f, "shaderidx": None, "shaders": np.array([m_gx], np.int32),
+            "normals": None, "hidden": True}
+    plank_v = cube_v * np.array([0.3, 0.03, 0.1], np.float32) + np.array([0.0, 0.05, 0.6], np.float32)
+    plank = {"verts": plank_v, "idx": cube_f, "shaderidx": None, "shaders": np.array([m_on], np.int32),
+             "normals": None,
+             "transform": np.stack([trs_matrix((-0.05, 0.0, 0.0), yaw=0.05), trs_matrix((0.05, 0.0, 0.0), yaw=-0.05)])}
+    meshes = [box, disk, blob, ico, cube, small, slab, plank]
+    instances = []
+    for k in range(count):
+        proto = 2 + k % 3
+        sc = g.uniform(0.08, 0.2, 3) * (np.array([1.0, 1.0, 1.0]) if proto != 4 else np.array([1.0, 0.6, 1.4]))
+        pos = np.array([g.uniform(-0.75, 0.75), g.uniform(0.15, 1.6), g.uniform(-0.75, 0.5)])
+        yaw, pitch = g.uniform(0, 2 * np.pi), g.uniform(-0.6, 0.6)
+        mats_k = [trs_matrix(pos, yaw, pitch, sc)]
+        if k % 3 == 0:
+            mats_k.append(trs_matrix(pos + g.uniform(-0.08, 0.08, 3), yaw + 0.4, pitch, sc))
+        r = float(np.max(sc)) * 1.75 + 0.1
+        instances.append({"geom": proto, "matrices": np.stack(mats_k), "bmin": pos - r, "bmax": pos + r})
+    procs = [{"geoms": [5, 6], "matrices": trs_matrix((0.55, 0.35, 0.25), yaw=0.7, pitch=0.2)[None, :],
+              "bmin": (0.2, 0.0, -0.1), "bmax": (0.9, 0.7, 0.6)}]
+    camera = dict(base["camera"])
+    return {"meshes": meshes, "materials": np.array(mats, MATERIAL_DTYPE), "lights": lights, "camera": camera,
+            "instances": instances, "procs": procs}
+
+
 def _rot_y(theta):
     c, s_ = np.cos(theta), np.sin(theta)
     return np.array([[c, 0.0, s_], [0.0, 1.0, 0.0], [-s_, 0.0, c]])
