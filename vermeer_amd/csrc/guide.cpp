@@ -20,10 +20,6 @@ vgpu_guide::~vgpu_guide() {
     void* ptrs[] = {d_params, d_m, d_v, d_frag_fwd, d_frag_bwd, tb.partial, tb.loss, tb.csum, tb.step, tb.grad,
                     tb.slot_fwd, tb.slot_bwd, d_stage};
     for (void* p : ptrs) if (p) (void)hipFree(p);
-    for (int l = 0; l < 5; l++) {
-        if (tb.X[l]) (void)hipFree(tb.X[l]);
-        if (tb.dY[l]) (void)hipFree(tb.dY[l]);
-    }
     if (ok) (void)hipSetDevice(prev);
 }
 
@@ -95,19 +91,12 @@ int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_
     g->num_cus = prop.multiProcessorCount;
     const size_t P4 = sizeof(float) * G::NPARAM;
     const uint32_t nsplit = max_batch / G::KSPLIT;
-    const int xrows[5] = {G::NENC, G::NH, G::NH, G::NH, G::NH};
-    const int yrows[5] = {G::NH, G::NH, G::NH, G::NH, 48};
     if (e == hipSuccess) e = hipMalloc(&g->d_params, P4);
     if (e == hipSuccess) e = hipMalloc(&g->d_m, P4);
     if (e == hipSuccess) e = hipMalloc(&g->d_v, P4);
     if (e == hipSuccess) e = hipMalloc(&g->d_frag_fwd, (size_t)G::NF_FWD * 64 * 16);
     if (e == hipSuccess) e = hipMalloc(&g->d_frag_bwd, (size_t)G::NF_BWD * 64 * 16);
-    for (int l = 0; l < 5 && e == hipSuccess; l++) {
-        e = hipMalloc(&g->tb.X[l], (size_t)xrows[l] * max_batch * 2);
-        if (e == hipSuccess) e = hipMalloc(&g->tb.dY[l], (size_t)yrows[l] * max_batch * 2);
-        if (e == hipSuccess) e = hipMemset(g->tb.dY[l], 0, (size_t)yrows[l] * max_batch * 2);
-    }
-    if (e == hipSuccess) e = hipMalloc(&g->tb.partial, (size_t)(max_batch / G::DW_SPLIT_MIN) * P4);
+    if (e == hipSuccess) e = hipMalloc(&g->tb.partial, (size_t)(max_batch / G::TRAIN_WG) * P4);
     if (e == hipSuccess) e = hipMalloc(&g->tb.loss, sizeof(float) * max_batch);
     if (e == hipSuccess) e = hipMalloc(&g->tb.csum, sizeof(float) * (1 + nsplit) + 256);
     if (e == hipSuccess) e = hipMalloc(&g->tb.step, 256);
@@ -118,7 +107,7 @@ int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_
     if (e == hipSuccess) e = hipMemset(g->tb.slot_bwd, 0xff, sizeof(int32_t) * G::NPARAM);
     if (e == hipSuccess) e = G::launch_slot_map(g->tb.slot_fwd, g->tb.slot_bwd, nullptr);
     if (e != hipSuccess) { delete g; return fail(VGPU_ERR_HIP, "guide buffers: %s", hipGetErrorString(e)); }
-    g->tb.ldb = max_batch;
+    g->tb.max_batch = max_batch;
     std::vector<float> P;
     init_params(seed, P);
     const int rc = upload_params(g, P.data());

@@ -13,11 +13,11 @@
 // into LDS by every workgroup (50 KB forward, +32 KB transposed for backward).
 //
 // Training (one step over B samples): k_csum_part sums the sample weights c per split of
-// KSPLIT samples and k_train adds the split sums; k_train runs forward, the loss gradient and the backward pass in registers,
-// writing every layer input X and output gradient dY in feature-major bf16;
-// k_dw forms the weight gradients dW = dY X^T as split-K MFMA GEMMs into fp32
-// partials; k_adam reduces the partials in a fixed order, applies Adam and writes each
-// updated weight's bf16 copy straight into its fragment slots.
+// KSPLIT samples and k_train adds the split sums; k_train runs forward, the loss gradient and
+// the backward pass in registers, and after each backward layer stages that layer's input X and
+// output gradient dY for its 256 samples in LDS and forms dW = dY X^T on the matrix cores (samples
+// as the k dimension) into one fp32 partial per workgroup; k_adam reduces the partials in a fixed
+// order, applies Adam and writes each updated weight's bf16 copy straight into its fragment slots.
 
 #include <atomic>
 
@@ -207,20 +207,13 @@ __device__ __forceinline__ void layer_hidden(const u32x4* sF, int fbase, const f
 
 template <int NT>
 __device__ __forceinline__ void layer_first(const u32x4* sF, const float* bias, int g, int lane, const float (&x)[NT][6],
-                                            f32x4 (&acc)[4][NT], uint16_t* X1, uint32_t ldb, const uint64_t (&s)[NT]) {
+                                            f32x4 (&acc)[4][NT]) {
     bias_hidden<NT>(acc, bias, g);
 #pragma unroll
     for (int j = 0; j < 5; j++) {
         bf16x8 b[NT];
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-            b[t] = enc_chunk(x[t], j, g);
-            if (X1) {
-                const int q0 = 32 * j + 8 * g;
-                if (q0 < NENC)
-                    for (int i = 0; i < 8; i++) X1[(uint64_t)(q0 + i) * ldb + s[t]] = bf16_bits(b[t][i]);
-            }
-        }
+        for (int t = 0; t < NT; t++) b[t] = enc_chunk(x[t], j, g);
 #pragma unroll
         for (int o = 0; o < 4; o++) {
             const bf16x8 a = frag(sF, F_L1 + 5 * o + j, lane);
@@ -250,17 +243,6 @@ __device__ __forceinline__ void layer_out(const u32x4* sF, const float* b5, int 
 #pragma unroll
             for (int t = 0; t < NT; t++) acc[T][t] = mfma(a, Xin[j][t], acc[T][t]);
         }
-}
-
-template <int NT>
-__device__ __forceinline__ void store_hidden(uint16_t* X, uint32_t ldb, const bf16x8 (&Xf)[2][NT], int g,
-                                             const uint64_t (&s)[NT]) {
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-#pragma unroll
-            for (int i = 0; i < 8; i++) X[(uint64_t)hid_feature(j, g, i) * ldb + s[t]] = bf16_bits(Xf[j][t][i]);
 }
 
 // ---------------------------------------------------------------- lobes
@@ -334,7 +316,7 @@ __global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict
         }
         f32x4 acc[4][NT];
         bf16x8 X[2][NT];
-        layer_first<NT>(sF, P + OB1, g, lane, x, acc, nullptr, 0, s);
+        layer_first<NT>(sF, P + OB1, g, lane, x, acc);
         relu_pack<NT>(acc, X);
         layer_hidden<NT>(sF, F_L2, P + OB2, g, lane, X, acc);
         relu_pack<NT>(acc, X);
@@ -414,12 +396,10 @@ __device__ __forceinline__ float batch_csum(const float* __restrict__ part, uint
     return a;
 }
 
-// Masks the back-propagated tile with the layer input (ReLU derivative), stores it and repacks
-// it as the next B operand.
+// Masks the back-propagated tile with the layer input (ReLU derivative) and repacks it as the next
+// B operand.
 template <int NT>
-__device__ __forceinline__ void mask_pack_store(const f32x4 (&acc)[4][NT], const bf16x8 (&Xf)[2][NT],
-                                                bf16x8 (&D)[2][NT], uint16_t* dY, uint32_t ldb, int g,
-                                                const uint64_t (&s)[NT]) {
+__device__ __forceinline__ void mask_pack(const f32x4 (&acc)[4][NT], const bf16x8 (&Xf)[2][NT], bf16x8 (&D)[2][NT]) {
 #pragma unroll
     for (int j = 0; j < 2; j++)
 #pragma unroll
@@ -427,9 +407,7 @@ __device__ __forceinline__ void mask_pack_store(const f32x4 (&acc)[4][NT], const
 #pragma unroll
             for (int i = 0; i < 8; i++) {
                 const float v = acc[2 * j + (i >> 2)][t][i & 3];
-                const __bf16 b = (__bf16)(bf16_val(Xf[j][t][i]) > 0.0f ? v : 0.0f);
-                D[j][t][i] = b;
-                dY[(uint64_t)hid_feature(j, g, i) * ldb + s[t]] = bf16_bits(b);
+                D[j][t][i] = (__bf16)(bf16_val(Xf[j][t][i]) > 0.0f ? v : 0.0f);
             }
 }
 
@@ -450,21 +428,113 @@ __device__ __forceinline__ void layer_back(const u32x4* sB, int fbase, int lane,
         }
 }
 
-// 8 waves x 32 samples per workgroup.
+// ---- weight gradients inside the training kernel
+//
+// A workgroup trains TRAIN_WG samples as TRAIN_CHUNKS chunks of TRAIN_CHUNK (8 waves x 16 samples).
+// After each backward layer it stages that layer's input X and output gradient dY for the chunk in
+// LDS, feature-major with rows of STG bf16 (the 8-element pad makes the 16-byte operand reads of 16
+// consecutive rows conflict-free), and adds dY X^T to its dW tiles on the matrix cores with the
+// samples as the k dimension.  The tiles of every layer stay in the waves' accumulators over the
+// chunks; the workgroup then writes one fp32 partial of every parameter, which k_adam adds in
+// workgroup order.  X and dY never reach HBM.
+constexpr int TRAIN_CHUNK = 128;
+constexpr int TRAIN_CHUNKS = TRAIN_WG / TRAIN_CHUNK;
+static_assert(TRAIN_CHUNKS * TRAIN_CHUNK == TRAIN_WG, "chunks tile the workgroup's samples");
+constexpr int STG = TRAIN_CHUNK + 8;           // staged row stride, bf16 (272 bytes)
+constexpr size_t FWD_BYTES = (size_t)NF_FWD * 64 * 16, BWD_BYTES = (size_t)NF_BWD * 64 * 16;
+constexpr size_t STAGE_BYTES = (size_t)(NENC + NH) * STG * 2;   // X1 (144 rows) and dY1 (64 rows): the largest
+constexpr size_t TRAIN_LDS = FWD_BYTES + BWD_BYTES + STAGE_BYTES;
+static_assert(STAGE_BYTES % 16 == 0 && TRAIN_LDS <= 160 * 1024, "LDS layout");
+
+// Tiles of one layer's dW held by a wave: tiles wave, wave + 8, ... of NOR x NIT.
+template <int NOR, int NIT>
+struct DwTiles {
+    static constexpr int N = (NOR * NIT + 7) / 8;
+    f32x4 acc[N];
+};
+
+// Writes this wave's 16 sample columns of a hidden-layer tensor (result-tile order, hid_feature).
+__device__ __forceinline__ void stage_hidden(uint16_t* rows, const bf16x8 (&V)[2][1], int g, int col) {
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) rows[hid_feature(j, g, i) * STG + col] = bf16_bits(V[j][0][i]);
+}
+
+// Adds the staged chunk's dY X^T to the wave's tiles of dW_l and the chunk's dY row sums to bsum
+// (8 threads per row, 16 samples each).
+template <int NOR, int NIT>
+__device__ __forceinline__ void dw_chunk(const uint16_t* Xs, const uint16_t* Ys, DwTiles<NOR, NIT>& T, float& bsum,
+                                         int nout) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int q = 0; q < DwTiles<NOR, NIT>::N; q++) {
+        const int tile = wave + 8 * q;
+        if (tile < NOR * NIT) {
+            const int o = tile / NIT, it = tile % NIT;
+            const uint16_t* a0 = Ys + (16 * o + c) * STG + 8 * g;
+            const uint16_t* b0 = Xs + (16 * it + c) * STG + 8 * g;
+#pragma unroll
+            for (int k0 = 0; k0 < TRAIN_CHUNK; k0 += 32)
+                T.acc[q] = mfma(__builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(a0 + k0)),
+                                __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(b0 + k0)), T.acc[q]);
+        }
+    }
+    const int row = threadIdx.x >> 3, part = threadIdx.x & 7;
+    if (row < nout) {
+        const uint16_t* y = Ys + row * STG + 16 * part;
+        for (int k = 0; k < 16; k++) bsum += bf16_val(__builtin_bit_cast(__bf16, y[k]));
+    }
+}
+
+// The wave's tiles and the bias sums of dW_l into the workgroup's partial Pp.
+template <int NOR, int NIT>
+__device__ __forceinline__ void dw_write(const DwTiles<NOR, NIT>& T, float bsum, float* Pp, int ow, int ob, int nin,
+                                         int nout) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int q = 0; q < DwTiles<NOR, NIT>::N; q++) {
+        const int tile = wave + 8 * q;
+        if (tile < NOR * NIT) {
+            const int o = tile / NIT, it = tile % NIT;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int out = 16 * o + 4 * g + r;
+                if (out < nout) Pp[ow + out * nin + 16 * it + c] = T.acc[q][r];
+            }
+        }
+    }
+    float b = bsum;
+    b += __shfl_xor(b, 1);
+    b += __shfl_xor(b, 2);
+    b += __shfl_xor(b, 4);
+    const int row = threadIdx.x >> 3;
+    if (row < nout && (threadIdx.x & 7) == 0) Pp[ob + row] = b;
+}
+
+template <int NOR, int NIT>
+__device__ __forceinline__ void dw_zero(DwTiles<NOR, NIT>& T) {
+#pragma unroll
+    for (int q = 0; q < DwTiles<NOR, NIT>::N; q++) T.acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+// One training step's forward, loss gradient, backward and weight-gradient partials for TRAIN_WG
+// samples per workgroup.
 // update: a parameter update follows (k_adam); workgroup 0 then advances the Adam step counter
 // when the batch has a positive weight sum, before k_adam reads it.
 __global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32_t B, TrainBuffers tb, int update) {
-    constexpr int NT = 2;
+    constexpr int NT = 1;
     extern __shared__ u32x4 lds[];
     u32x4* sF = lds;
-    u32x4* sB = lds + NF_FWD * 64;
+    u32x4* sB = lds + FWD_BYTES / 16;
+    uint16_t* Xs = reinterpret_cast<uint16_t*>(lds + (FWD_BYTES + BWD_BYTES) / 16);
+    uint16_t* Ys = Xs + NENC * STG;
     for (int k = threadIdx.x; k < NF_FWD * 64; k += 512) sF[k] = reinterpret_cast<const u32x4*>(net.frag_fwd)[k];
     for (int k = threadIdx.x; k < NF_BWD * 64; k += 512) sB[k] = reinterpret_cast<const u32x4*>(net.frag_bwd)[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
-    const uint64_t base = (uint64_t)blockIdx.x * 256 + wave * 32;
+    const int col = wave * 16 + c;                          // this lane's staged column
     const float* P = net.params;
-    const uint32_t ldb = tb.ldb;
     // every wavefront adds the split sums itself (no separate launch); workgroup 0 keeps the
     // total for k_adam and advances the step counter
     const float csum = batch_csum(tb.csum + 1, B / KSPLIT, lane);
@@ -474,160 +544,132 @@ __global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32
     }
     const float cscale = csum > 0.0f ? (float)B / csum : 0.0f;
 
-    float x[NT][6], dir[NT][3], cw[NT];
-    uint64_t s[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-        s[t] = base + 16 * t + c;
-        const TrainSample ts = fetch_sample(src, (uint32_t)s[t]);
-        normalise(make_float4(ts.px, ts.py, ts.pz, 0.0f), make_float4(ts.nx, ts.ny, ts.nz, 0.0f), net.bounds, x[t]);
-        dir[t][0] = ts.dx; dir[t][1] = ts.dy; dir[t][2] = ts.dz;
-        cw[t] = ts.c;
-    }
-    f32x4 acc[4][NT];
-    bf16x8 X2[2][NT], X3[2][NT], X4[2][NT], X5[2][NT];
-    layer_first<NT>(sF, P + OB1, g, lane, x, acc, tb.X[0], ldb, s);
-    relu_pack<NT>(acc, X2);
-    store_hidden<NT>(tb.X[1], ldb, X2, g, s);
-    layer_hidden<NT>(sF, F_L2, P + OB2, g, lane, X2, acc);
-    relu_pack<NT>(acc, X3);
-    store_hidden<NT>(tb.X[2], ldb, X3, g, s);
-    layer_hidden<NT>(sF, F_L3, P + OB3, g, lane, X3, acc);
-    relu_pack<NT>(acc, X4);
-    store_hidden<NT>(tb.X[3], ldb, X4, g, s);
-    layer_hidden<NT>(sF, F_L4, P + OB4, g, lane, X4, acc);
-    relu_pack<NT>(acc, X5);
-    store_hidden<NT>(tb.X[4], ldb, X5, g, s);
-    f32x4 o5[3][NT];
-    layer_out<NT>(sF, P + OB5, g, lane, X5, o5);
+    DwTiles<4, 9> T1;
+    DwTiles<4, 4> T2, T3, T4;
+    DwTiles<3, 4> T5;
+    dw_zero(T1); dw_zero(T2); dw_zero(T3); dw_zero(T4); dw_zero(T5);
+    float bs1 = 0.0f, bs2 = 0.0f, bs3 = 0.0f, bs4 = 0.0f, bs5 = 0.0f;
 
-    // loss gradient d(-(c/cbar) log q)/d raw for this lane's two lobes
-    bf16x8 G5[2][NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-        float v[12];
-#pragma unroll
-        for (int T = 0; T < 3; T++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) v[4 * T + r] = o5[T][t][r];
-        LobeD L[2];
-        decode_lobe(v, L[0]);
-        decode_lobe(v + 5, L[1]);
-        const float mlog = quad_max(fmaxf(L[0].logit, L[1].logit));
-        const float e0 = expf(L[0].logit - mlog), e1 = expf(L[1].logit - mlog);
-        const float z = quad_sum(e0 + e1);
-        const float w[2] = {e0 / z, e1 / z};
-        float a[2], cosv[2];
-        for (int k = 0; k < 2; k++) {
-            cosv[k] = L[k].mu[0] * dir[t][0] + L[k].mu[1] * dir[t][1] + L[k].mu[2] * dir[t][2];
-            a[k] = logf(w[k]) + log_vmf_norm(L[k].kappa) + L[k].kappa * (cosv[k] - 1.0f);
+    for (int chunk = 0; chunk < TRAIN_CHUNKS; chunk++) {
+        // keeps the weight fragments' LDS reads inside the loop (hoisted, they would pin ~330 registers)
+        __asm__ volatile("" ::: "memory");
+        const uint64_t s0 = (uint64_t)blockIdx.x * TRAIN_WG + (uint64_t)chunk * TRAIN_CHUNK + wave * 16 + c;
+        float x[NT][6], dir[3], cw;
+        {
+            const TrainSample ts = fetch_sample(src, (uint32_t)s0);
+            normalise(make_float4(ts.px, ts.py, ts.pz, 0.0f), make_float4(ts.nx, ts.ny, ts.nz, 0.0f), net.bounds, x[0]);
+            dir[0] = ts.dx; dir[1] = ts.dy; dir[2] = ts.dz;
+            cw = ts.c;
         }
-        const float am = quad_max(fmaxf(a[0], a[1]));
-        const float ex0 = expf(a[0] - am), ex1 = expf(a[1] - am);
-        const float se = quad_sum(ex0 + ex1);
-        const float logq = am + logf(se);
-        const float rr[2] = {ex0 / se, ex1 / se};
-        const float sc = cw[t] * cscale;
-        float gr[12];
-        for (int k = 0; k < 2; k++) {
-            const float kap = L[k].kappa;
-            gr[5 * k + 4] = sc * (w[k] - rr[k]);
-            const float dl = dlog_norm(kap) + (cosv[k] - 1.0f);
-            gr[5 * k + 3] = L[k].kin ? -sc * rr[k] * dl * kap : 0.0f;
-            for (int jj = 0; jj < 3; jj++)
-                gr[5 * k + jj] =
-                    L[k].ulen > 0.0f ? -sc * rr[k] * kap * (dir[t][jj] - cosv[k] * L[k].mu[jj]) / L[k].ulen : 0.0f;
-        }
-        gr[10] = 0.0f; gr[11] = 0.0f;
-        if (g == 0) tb.loss[s[t]] = -sc * logq;
-#pragma unroll
-        for (int i = 0; i < 8; i++) G5[0][t][i] = (__bf16)gr[i];
-#pragma unroll
-        for (int i = 0; i < 8; i++) G5[1][t][i] = (__bf16)(i < 4 ? gr[8 + i] : 0.0f);
-        for (int sl = 0; sl < 10; sl++)
-            tb.dY[4][(uint64_t)out5_index(g, sl) * ldb + s[t]] = bf16_bits(sl < 8 ? G5[0][t][sl] : G5[1][t][sl - 8]);
-    }
+        f32x4 acc[4][NT];
+        bf16x8 X2[2][NT], X3[2][NT], X4[2][NT], X5[2][NT];
+        layer_first<NT>(sF, P + OB1, g, lane, x, acc);
+        relu_pack<NT>(acc, X2);
+        layer_hidden<NT>(sF, F_L2, P + OB2, g, lane, X2, acc);
+        relu_pack<NT>(acc, X3);
+        layer_hidden<NT>(sF, F_L3, P + OB3, g, lane, X3, acc);
+        relu_pack<NT>(acc, X4);
+        layer_hidden<NT>(sF, F_L4, P + OB4, g, lane, X4, acc);
+        relu_pack<NT>(acc, X5);
+        f32x4 o5[3][NT];
+        layer_out<NT>(sF, P + OB5, g, lane, X5, o5);
 
-    // backward: dX_l = W_l^T dY_l, dY_{l-1} = dX_l * (X_l > 0)
-    bf16x8 D[2][NT];
-    layer_back<NT>(sB, FB_L5, lane, G5, acc);
-    mask_pack_store<NT>(acc, X5, D, tb.dY[3], ldb, g, s);
-    layer_back<NT>(sB, FB_L4, lane, D, acc);
-    mask_pack_store<NT>(acc, X4, D, tb.dY[2], ldb, g, s);
-    layer_back<NT>(sB, FB_L3, lane, D, acc);
-    mask_pack_store<NT>(acc, X3, D, tb.dY[1], ldb, g, s);
-    layer_back<NT>(sB, FB_L2, lane, D, acc);
-    mask_pack_store<NT>(acc, X2, D, tb.dY[0], ldb, g, s);
-}
-
-// Weight-gradient partials.  One wave per (column strip of some dW_l, split of ks samples): the
-// wave holds every row tile of its strip (4, or 3 for dW5), so it loads each dY and X fragment once
-// per split: nor + nit loads per nor x nit MFMAs.  Strips: dW1 (64 x 144) as 3 strips of 3 column
-// tiles; dW2..dW4 (64 x 64) one strip of 4; dW5 (40 x 64, rows padded to 48) one strip of 4 with 3
-// row tiles.  Each tile accumulates its samples in the same order as one wave per tile would.
-constexpr int NJOB = 3 + 3 + 1;
-
-// One strip: NOR row tiles x NIT column tiles starting at column tile it0 of dW_l.
-template <int NOR, int NIT>
-__device__ __forceinline__ void dw_strip(const TrainBuffers& tb, int l, int it0, int nin, int nout, uint32_t split,
-                                         uint32_t ks, int lane) {
-    const int g = lane >> 4, c = lane & 15;
-    const uint32_t ldb = tb.ldb;
-    const uint16_t* Ap[NOR];
-    const uint16_t* Bp[NIT];
+        // loss gradient d(-(c/cbar) log q)/d raw for this lane's two lobes, staged as dY5 rows (out5_index)
+        bf16x8 G5[2][NT];
+        {
+            float v[12];
 #pragma unroll
-    for (int o = 0; o < NOR; o++) Ap[o] = tb.dY[l] + (uint64_t)(16 * o + c) * ldb;
+            for (int T = 0; T < 3; T++)
 #pragma unroll
-    for (int k = 0; k < NIT; k++) Bp[k] = tb.X[l] + (uint64_t)(16 * (it0 + k) + c) * ldb;
-    f32x4 acc[NOR][NIT];
-#pragma unroll
-    for (int o = 0; o < NOR; o++)
-#pragma unroll
-        for (int k = 0; k < NIT; k++) acc[o][k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    float bs[NOR];
-#pragma unroll
-    for (int o = 0; o < NOR; o++) bs[o] = 0.0f;
-    const uint64_t s_end = (uint64_t)(split + 1) * ks;
-    for (uint64_t s0 = (uint64_t)split * ks; s0 < s_end; s0 += 32) {
-        u32x4 av[NOR], bv[NIT];
-#pragma unroll
-        for (int o = 0; o < NOR; o++) av[o] = *reinterpret_cast<const u32x4*>(Ap[o] + s0 + 8 * g);
-#pragma unroll
-        for (int k = 0; k < NIT; k++) bv[k] = *reinterpret_cast<const u32x4*>(Bp[k] + s0 + 8 * g);
-#pragma unroll
-        for (int o = 0; o < NOR; o++) {
-            const bf16x8 a = __builtin_bit_cast(bf16x8, av[o]);
-#pragma unroll
-            for (int k = 0; k < NIT; k++) acc[o][k] = mfma(a, __builtin_bit_cast(bf16x8, bv[k]), acc[o][k]);
-            if (it0 == 0)
-                for (int i = 0; i < 8; i++) bs[o] += bf16_val(a[i]);
-        }
-    }
-    const int ow[5] = {OW1, OW2, OW3, OW4, OW5}, ob[5] = {OB1, OB2, OB3, OB4, OB5};
-    float* Pp = tb.partial + (uint64_t)split * NPARAM;
-#pragma unroll
-    for (int o = 0; o < NOR; o++) {
-#pragma unroll
-        for (int k = 0; k < NIT; k++)
-            for (int r = 0; r < 4; r++) {
-                const int out = 16 * o + 4 * g + r;
-                if (out < nout) Pp[ow[l] + out * nin + 16 * (it0 + k) + c] = acc[o][k][r];
+                for (int r = 0; r < 4; r++) v[4 * T + r] = o5[T][0][r];
+            LobeD L[2];
+            decode_lobe(v, L[0]);
+            decode_lobe(v + 5, L[1]);
+            const float mlog = quad_max(fmaxf(L[0].logit, L[1].logit));
+            const float e0 = expf(L[0].logit - mlog), e1 = expf(L[1].logit - mlog);
+            const float z = quad_sum(e0 + e1);
+            const float w[2] = {e0 / z, e1 / z};
+            float a[2], cosv[2];
+            for (int k = 0; k < 2; k++) {
+                cosv[k] = L[k].mu[0] * dir[0] + L[k].mu[1] * dir[1] + L[k].mu[2] * dir[2];
+                a[k] = logf(w[k]) + log_vmf_norm(L[k].kappa) + L[k].kappa * (cosv[k] - 1.0f);
             }
-        if (it0 == 0) {
-            const float b = quad_sum(bs[o]);
-            if (g == 0 && 16 * o + c < nout) Pp[ob[l] + 16 * o + c] = b;
+            const float am = quad_max(fmaxf(a[0], a[1]));
+            const float ex0 = expf(a[0] - am), ex1 = expf(a[1] - am);
+            const float se = quad_sum(ex0 + ex1);
+            const float logq = am + logf(se);
+            const float rr[2] = {ex0 / se, ex1 / se};
+            const float sc = cw * cscale;
+            float gr[12];
+            for (int k = 0; k < 2; k++) {
+                const float kap = L[k].kappa;
+                gr[5 * k + 4] = sc * (w[k] - rr[k]);
+                const float dl = dlog_norm(kap) + (cosv[k] - 1.0f);
+                gr[5 * k + 3] = L[k].kin ? -sc * rr[k] * dl * kap : 0.0f;
+                for (int jj = 0; jj < 3; jj++)
+                    gr[5 * k + jj] =
+                        L[k].ulen > 0.0f ? -sc * rr[k] * kap * (dir[jj] - cosv[k] * L[k].mu[jj]) / L[k].ulen : 0.0f;
+            }
+            gr[10] = 0.0f; gr[11] = 0.0f;
+            if (g == 0) tb.loss[s0] = -sc * logq;
+#pragma unroll
+            for (int i = 0; i < 8; i++) G5[0][0][i] = (__bf16)gr[i];
+#pragma unroll
+            for (int i = 0; i < 8; i++) G5[1][0][i] = (__bf16)(i < 4 ? gr[8 + i] : 0.0f);
         }
-    }
-}
 
-__global__ __launch_bounds__(256) void k_dw(TrainBuffers tb, uint32_t nsplit, uint32_t ks) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t job = w % NJOB, split = w / NJOB;
-    if (split >= nsplit) return;
-    if (job < 3) dw_strip<4, 3>(tb, 0, 3 * (int)job, NENC, NH, split, ks, lane);
-    else if (job < 6) dw_strip<4, 4>(tb, (int)job - 2, 0, NH, NH, split, ks, lane);
-    else dw_strip<3, 4>(tb, 4, 0, NH, NOUT, split, ks, lane);
+        // backward: dX_l = W_l^T dY_l, dY_{l-1} = dX_l * (X_l > 0); dW_l from the staged X_l and dY_l
+        __syncthreads();   // the previous chunk's layer-1 tiles are done with the staging area
+        for (int sl = 0; sl < 10; sl++)
+            Ys[out5_index(g, sl) * STG + col] = bf16_bits(sl < 8 ? G5[0][0][sl] : G5[1][0][sl - 8]);
+        stage_hidden(Xs, X5, g, col);
+        bf16x8 D[2][NT];
+        layer_back<NT>(sB, FB_L5, lane, G5, acc);
+        mask_pack<NT>(acc, X5, D);
+        __syncthreads();
+        dw_chunk(Xs, Ys, T5, bs5, NOUT);
+        __syncthreads();
+        stage_hidden(Xs, X4, g, col);
+        stage_hidden(Ys, D, g, col);
+        layer_back<NT>(sB, FB_L4, lane, D, acc);
+        mask_pack<NT>(acc, X4, D);
+        __syncthreads();
+        dw_chunk(Xs, Ys, T4, bs4, NH);
+        __syncthreads();
+        stage_hidden(Xs, X3, g, col);
+        stage_hidden(Ys, D, g, col);
+        layer_back<NT>(sB, FB_L3, lane, D, acc);
+        mask_pack<NT>(acc, X3, D);
+        __syncthreads();
+        dw_chunk(Xs, Ys, T3, bs3, NH);
+        __syncthreads();
+        stage_hidden(Xs, X2, g, col);
+        stage_hidden(Ys, D, g, col);
+        layer_back<NT>(sB, FB_L2, lane, D, acc);
+        mask_pack<NT>(acc, X2, D);
+        __syncthreads();
+        dw_chunk(Xs, Ys, T2, bs2, NH);
+        __syncthreads();
+        // layer 1: its input is the encoding, recomputed (the forward pass keeps it in no register)
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            const int q0 = 32 * j + 8 * g;
+            if (q0 < NENC) {
+                const bf16x8 b = enc_chunk(x[0], j, g);
+#pragma unroll
+                for (int i = 0; i < 8; i++) Xs[(q0 + i) * STG + col] = bf16_bits(b[i]);
+            }
+        }
+        stage_hidden(Ys, D, g, col);
+        __syncthreads();
+        dw_chunk(Xs, Ys, T1, bs1, NH);
+    }
+    float* Pp = tb.partial + (uint64_t)blockIdx.x * NPARAM;
+    dw_write(T1, bs1, Pp, OW1, OB1, NENC, NH);
+    dw_write(T2, bs2, Pp, OW2, OB2, NH, NH);
+    dw_write(T3, bs3, Pp, OW3, OB3, NH, NH);
+    dw_write(T4, bs4, Pp, OW4, OB4, NH, NH);
+    dw_write(T5, bs5, Pp, OW5, OB5, NH, NOUT);
 }
 
 // Adds the weight-gradient partials (split order), applies Adam and writes the updated parameter's
@@ -702,30 +744,26 @@ constexpr int kAdamBlock = 64;
 
 hipError_t launch_train(const Net& net, float* params, float* m, float* v, const SampleSource& src, uint32_t B,
                         float lr, bool update, const TrainBuffers& tb, hipStream_t st) {
-    if (B == 0 || B % KSPLIT != 0 || B > tb.ldb) return hipErrorInvalidValue;
-    // k_train's dynamic LDS (84 KB) exceeds the default limit: raise it once per device
-    const size_t lds = (size_t)(NF_FWD + NF_BWD) * 64 * 16;
+    if (B == 0 || B % KSPLIT != 0 || B > tb.max_batch) return hipErrorInvalidValue;
+    // k_train's dynamic LDS (staging plus the backward fragments) exceeds the default limit: raise it once
+    // per device
     static std::atomic<bool> attr_set[kMaxDevices];
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
     if (!attr_set[dev].load(std::memory_order_acquire)) {
-        e = hipFuncSetAttribute((const void*)k_train, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        e = hipFuncSetAttribute((const void*)k_train, hipFuncAttributeMaxDynamicSharedMemorySize, (int)TRAIN_LDS);
         if (e != hipSuccess) return e;
         attr_set[dev].store(true, std::memory_order_release);
     }
     const uint32_t nsplit = B / KSPLIT;
     if (!src.S) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, src, tb.csum + 1);
-    hipLaunchKernelGGL(k_train, dim3(B / 256), dim3(512), lds, st, net, src, B, tb, update ? 1 : 0);
-    // weight gradients in splits of DW_SPLIT samples when the batch allows, else KSPLIT
-    const uint32_t ks = B % DW_SPLIT == 0 ? DW_SPLIT : KSPLIT;
-    const uint32_t nsw = B / ks;
-    const uint32_t waves = NJOB * nsw;
-    hipLaunchKernelGGL(k_dw, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nsw, ks);
+    const uint32_t nwg = B / TRAIN_WG;
+    hipLaunchKernelGGL(k_train, dim3(nwg), dim3(512), TRAIN_LDS, st, net, src, B, tb, update ? 1 : 0);
     hipLaunchKernelGGL(k_adam, dim3((NPARAM + kAdamBlock - 1) / kAdamBlock), dim3(kAdamBlock), 0, st, params, m, v,
-                       tb.grad, tb.partial, nsw, 1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0,
+                       tb.grad, tb.partial, nwg, 1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0,
                        (const int32_t*)tb.slot_fwd, (const int32_t*)tb.slot_bwd, (uint16_t*)net.frag_fwd,
                        (uint16_t*)net.frag_bwd);
     return hipGetLastError();

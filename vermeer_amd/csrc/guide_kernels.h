@@ -23,11 +23,9 @@ static_assert(NPARAM == 24360, "parameter count");
 constexpr int NF_FWD = 4 * 5 + 3 * 4 * 2 + 3 * 2;   // W1 (4 out tiles x 5 k-chunks), W2..W4, W5 (3 x 2)
 constexpr int NF_BWD = 4 * 2 * 4;                   // W5^T, W4^T, W3^T, W2^T (4 in tiles x 2 k-chunks)
 constexpr int KSPLIT = 1024;                        // batch granularity (a batch is a multiple of it)
-// Samples per weight-gradient split (k_dw): more splits give k_dw more waves and k_adam more
-// partials to add.  A batch that is not a multiple of DW_SPLIT uses KSPLIT.
-constexpr int DW_SPLIT = 512;
-constexpr int DW_SPLIT_MIN = DW_SPLIT < KSPLIT ? DW_SPLIT : KSPLIT;
-static_assert(KSPLIT % DW_SPLIT_MIN == 0 && DW_SPLIT % 32 == 0, "splits tile the batch");
+// Samples per training workgroup: each writes one fp32 partial of every parameter's gradient.
+constexpr int TRAIN_WG = 256;
+static_assert(KSPLIT % TRAIN_WG == 0, "workgroups tile the batch");
 constexpr float KAPPA_RAW_MIN = -5.0f, KAPPA_RAW_MAX = 7.0f;
 
 // One training sample: hit position + c, shading normal, sampled direction (unit).
@@ -46,16 +44,14 @@ struct Net {
 };
 
 struct TrainBuffers {
-    uint16_t* X[5];     // layer inputs, feature-major [features][ldb] bf16: 144, 64, 64, 64, 64
-    uint16_t* dY[5];    // output gradients, feature-major bf16: 64, 64, 64, 64, 48 (rows 40..47 zero)
-    float* partial;     // [nsplit][NPARAM]
+    float* partial;     // [batch / TRAIN_WG][NPARAM]: each training workgroup's gradient sums
     float* loss;        // per sample
     float* csum;        // [0]: sum of c over the batch; [1 .. 1 + max_batch/KSPLIT): per-split partial sums
     int32_t* step;      // Adam step count (device)
     float* grad;        // last mean gradient (NPARAM)
     int32_t* slot_fwd;  // NPARAM: bf16 element of the forward fragments holding each parameter, -1: none
     int32_t* slot_bwd;  // NPARAM: the same for the backward fragments
-    uint32_t ldb;       // row stride (samples) of X / dY
+    uint32_t max_batch; // largest batch the buffers hold
 };
 
 // Rebuilds the bf16 fragments from the fp32 master copy.
