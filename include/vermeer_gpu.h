@@ -172,6 +172,19 @@ int vgpu_qbvh_build(float* boxes, float* centroids, int32_t* idx, int32_t n, int
                     vgpu_qbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out,
                     float* bounds_out, int32_t nthreads);
 
+/* GPU builder.  Replaces qbvh.BuildAccel (qbvh/build.go:293-307) as
+ * vgpu_qbvh_build does, with the same output node for node and the same
+ * permutation of boxes / centroids / idx, computed on the current device with
+ * work enqueued on `stream` (a hipStream_t, NULL = the null stream; the call
+ * returns after the results are in the host arrays).  Non-finite centroids
+ * fail with VGPU_ERR_BUILD as on the host; so does a NaN box coordinate,
+ * which vgpu_qbvh_build accepts.  A tree more than 192 levels deep below
+ * its 2048-element subtrees fails with VGPU_ERR_STACK (vgpu_qbvh_build has no
+ * such limit). */
+int vgpu_qbvh_build_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
+                        vgpu_qbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out,
+                        float* bounds_out, void* stream);
+
 /* Replaces qbvh.BuildAccelMotion (qbvh/motionbuild.go:108-122): topology only. */
 int vgpu_qbvh_build_motion(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
                            vgpu_mqbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out,

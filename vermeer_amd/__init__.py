@@ -5,8 +5,8 @@ builder) behind the C ABI declared in include/vermeer_gpu.h; this package is
 the Python mirror of the reference's plugin interface over that ABI.
 """
 from . import _lib
-from .accel import PolyMesh, Scene, build_accel, build_accel_motion
+from .accel import PolyMesh, Scene, build_accel, build_accel_gpu, build_accel_motion
 from ._lib import HIT_DTYPE, MNODE_DTYPE, NODE_DTYPE, RAY_DTYPE, VgpuError
 
-__all__ = ["PolyMesh", "Scene", "build_accel", "build_accel_motion", "HIT_DTYPE", "NODE_DTYPE", "MNODE_DTYPE",
+__all__ = ["PolyMesh", "Scene", "build_accel", "build_accel_gpu", "build_accel_motion", "HIT_DTYPE", "NODE_DTYPE", "MNODE_DTYPE",
            "RAY_DTYPE", "VgpuError"]

@@ -67,6 +67,7 @@ SIGNATURES = [
     ("vgpu_last_error", C.c_char_p, []),
     ("vgpu_device_count", C.c_int, [C.POINTER(C.c_int)]),
     ("vgpu_qbvh_build", C.c_int, [_P, _P, _P, _I32, _I32, _P, _I32, C.POINTER(_I32), _P, _I32]),
+    ("vgpu_qbvh_build_gpu", C.c_int, [_P, _P, _P, _I32, _I32, _P, _I32, C.POINTER(_I32), _P, _P]),
     ("vgpu_qbvh_build_motion", C.c_int, [_P, _P, _P, _I32, _I32, _P, _I32, C.POINTER(_I32), _I32]),
     ("vgpu_scene_create", C.c_int, [C.POINTER(_P), C.c_int]),
     ("vgpu_scene_destroy", C.c_int, [_P]),

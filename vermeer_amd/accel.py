@@ -3,6 +3,7 @@
 The names follow the reference so that code (and tests) read alike:
 
   build_accel(boxes, centroids, indxs, leaf_max)   qbvh.BuildAccel      (qbvh/build.go:293)
+  build_accel_gpu(...)                             qbvh.BuildAccel, built on the GPU
   build_accel_motion(...)                          qbvh.BuildAccelMotion (qbvh/motionbuild.go:108)
   PolyMesh                                         builtin/geom/polymesh/polymesh.go:17-60
   Scene.add_geom / pre_render / trace              builtin/scene/scene.go:119-133, :30-58 and
@@ -44,6 +45,23 @@ def build_accel(boxes, centroids, indxs, leaf_max=16, nthreads=0):
     bounds = np.zeros(6, dtype=np.float32)
     check(lib.vgpu_qbvh_build(ptr(boxes), ptr(centroids), ptr(indxs), n, leaf_max, ptr(nodes), len(nodes),
                               C.byref(nn), ptr(bounds), nthreads))
+    return nodes[: nn.value].copy(), bounds
+
+
+def build_accel_gpu(boxes, centroids, indxs, leaf_max=16, stream=None):
+    """qbvh.BuildAccel computed on the current GPU (vgpu_qbvh_build_gpu): the same (nodes, bounds)
+    and the same in-place permutation as build_accel.  stream: a hipStream_t as an int, or None."""
+    n = len(indxs)
+    for a, dt in ((boxes, np.float32), (centroids, np.float32), (indxs, np.int32)):
+        if not (isinstance(a, np.ndarray) and a.dtype == dt and a.flags.c_contiguous):
+            raise TypeError("build_accel_gpu permutes its inputs in place: pass C-contiguous numpy arrays "
+                            "(float32 boxes/centroids, int32 indxs)")
+    lib = _lib.load()
+    nodes = np.zeros(max(n, 1), dtype=NODE_DTYPE)
+    nn = C.c_int32(0)
+    bounds = np.zeros(6, dtype=np.float32)
+    check(lib.vgpu_qbvh_build_gpu(ptr(boxes), ptr(centroids), ptr(indxs), n, leaf_max, ptr(nodes), len(nodes),
+                                  C.byref(nn), ptr(bounds), C.c_void_p(stream or 0)))
     return nodes[: nn.value].copy(), bounds
 
 

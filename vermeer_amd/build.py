@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvermeer_gpu.so")
 
 SOURCES = ["trace.hip", "raysort.hip", "hbmcopy.hip", "output.hip", "vnf.cpp", "render.hip", "guide.hip", "pathtrace.hip", "scene.cpp", "render.cpp", "guide.cpp",
-           "pathtrace.cpp", "qbvh_build.cpp"]
+           "pathtrace.cpp", "qbvh_build.cpp", "bvh_build.hip"]
 HEADERS = ["trace_kernels.h", "render_kernels.h", "guide_kernels.h", "guide_internal.h", "pathtrace_kernels.h",
            "scene_internal.h", "qbvh_build.h"]
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <hip/hip_runtime.h>
 #include "../../include/vermeer_gpu.h"
 
 namespace vgpu {
@@ -12,5 +13,11 @@ int build_qbvh(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t 
                int32_t max_nodes, int32_t* nnodes_out, float* bounds_out, int32_t nthreads);
 int build_qbvh_motion(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leafMax,
                       vgpu_mqbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out, int32_t nthreads);
+
+// The GPU builder (bvh_build.hip): the same output as build_qbvh, built on the current device with
+// work enqueued on `st`.  Returns 0, -1, -2 (HIP error), -3 (degenerate or non-finite input) or -4
+// (a subtree deeper than the builder's stack; build_qbvh handles any depth).
+int build_qbvh_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leafMax, vgpu_qbvh_node* nodes_out,
+                   int32_t max_nodes, int32_t* nnodes_out, float* bounds_out, hipStream_t st);
 
 }  // namespace vgpu

@@ -309,6 +309,18 @@ int vgpu_qbvh_build(float* boxes, float* centroids, int32_t* idx, int32_t n, int
     return VGPU_OK;
 }
 
+int vgpu_qbvh_build_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
+                        vgpu_qbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out, float* bounds_out,
+                        void* stream) {
+    int rc = vgpu::build_qbvh_gpu(boxes, centroids, idx, n, leaf_max, nodes_out, max_nodes, nnodes_out, bounds_out,
+                                  (hipStream_t)stream);
+    if (rc == -3) return fail(VGPU_ERR_BUILD, "degenerate or non-finite input: the reference build fails or does not terminate");
+    if (rc == -4) return fail(VGPU_ERR_STACK, "tree deeper than the GPU builder's stack (vgpu_qbvh_build handles it)");
+    if (rc == -2) return fail(VGPU_ERR_HIP, "HIP error in the GPU builder");
+    if (rc != 0) return fail(VGPU_ERR_ARG, "bad arguments or max_nodes too small (need %d)", nnodes_out ? *nnodes_out : -1);
+    return VGPU_OK;
+}
+
 int vgpu_qbvh_build_motion(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
                            vgpu_mqbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out, int32_t nthreads) {
     int rc = vgpu::build_qbvh_motion(boxes, centroids, idx, n, leaf_max, nodes_out, max_nodes, nnodes_out, nthreads);
