@@ -39,8 +39,34 @@ using namespace vgpu_detail;
 
 namespace vgpu_detail {
 
+// Meshes with at least this many triangles build their BVH on the scene's GPU (vgpu::build_qbvh_gpu);
+// smaller ones on the host, where the launch and transfer overheads would dominate.  Both give the
+// reference's tree.  The host builder also takes any mesh whose tree is deeper than the GPU
+// builder's stack (-4).
+constexpr uint32_t kGpuBuildMinTris = 1u << 15;
+
+int build_nodes(float* boxes, float* cent, int32_t* ids, uint32_t n, int leaf_max, std::vector<vgpu_qbvh_node>& nodes,
+                float* bounds, int device) {
+    int32_t nn = 0;
+    nodes.assign(std::max<uint32_t>(n, 1), vgpu_qbvh_node{});
+    int rc = -4;
+    if (device >= 0 && n >= kGpuBuildMinTris) {
+        DeviceGuard dg(device);
+        rc = vgpu::build_qbvh_gpu(boxes, cent, ids, (int32_t)n, leaf_max, nodes.data(), (int32_t)nodes.size(), &nn,
+                                  bounds, nullptr);
+        if (rc == -2) return fail(VGPU_ERR_HIP, "HIP error in the GPU BVH builder");
+    }
+    if (rc == -4)
+        rc = vgpu::build_qbvh(boxes, cent, ids, (int32_t)n, leaf_max, nodes.data(), (int32_t)nodes.size(), &nn,
+                              bounds, 0);
+    if (rc == -3) return fail(VGPU_ERR_BUILD, "BVH build failed: degenerate mesh (coincident or non-finite centroids)");
+    if (rc != 0) return fail(VGPU_ERR_BUILD, "BVH build failed (%d)", rc);
+    nodes.resize(nn);
+    return VGPU_OK;
+}
+
 // Static mesh accel, as PolyMesh.initAccel (buildqbvh.go:60-129).
-int build_static_mesh(MeshHost& m, int leaf_max) {
+int build_static_mesh(MeshHost& m, int leaf_max, int device) {
     const uint32_t n = m.ntris;
     std::vector<float> boxes((size_t)n * 6), cent((size_t)n * 3);
     std::vector<int32_t> ids(n);
@@ -53,13 +79,9 @@ int build_static_mesh(MeshHost& m, int leaf_max) {
         for (int k = 0; k < 3; k++) cent[(size_t)i * 3 + k] = (V0[k] + V1[k] + V2[k]) / 3.0f;
         ids[i] = (int32_t)i;
     }
-    int32_t nn = 0;
-    std::vector<vgpu_qbvh_node> nodes(std::max<uint32_t>(n, 1));
-    int rc = vgpu::build_qbvh(boxes.data(), cent.data(), ids.data(), (int32_t)n, leaf_max, nodes.data(),
-                              (int32_t)nodes.size(), &nn, m.bounds, 0);
-    if (rc == -3) return fail(VGPU_ERR_BUILD, "BVH build failed: degenerate mesh (coincident centroids)");
-    if (rc != 0) return fail(VGPU_ERR_BUILD, "BVH build failed (%d)", rc);
-    nodes.resize(nn);
+    std::vector<vgpu_qbvh_node> nodes;
+    const int rc = build_nodes(boxes.data(), cent.data(), ids.data(), n, leaf_max, nodes, m.bounds, device);
+    if (rc) return rc;
     m.nodes.swap(nodes);
     m.perm = ids;
     m.idxp_sorted.resize((size_t)n * 3);
@@ -94,8 +116,10 @@ void motion_boxes_rec(const std::vector<vgpu_mqbvh_node>& topo, int32_t node, fl
     }
 }
 
-// Motion mesh accel (buildqbvh.go:19-57 + initMotionBoxes :148-169).
-int build_motion_mesh(MeshHost& m) {
+// Motion mesh accel (buildqbvh.go:19-57 + initMotionBoxes :148-169).  BuildAccelMotion's topology is
+// BuildAccel's with the boxes left out (the same splits, qbvh/motionbuild.go:12-106), so a large mesh
+// takes it from the GPU builder.
+int build_motion_mesh(MeshHost& m, int device) {
     const uint32_t n = m.ntris;
     std::vector<float> boxes((size_t)n * 6), cent((size_t)n * 3);
     std::vector<int32_t> ids(n);
@@ -110,15 +134,33 @@ int build_motion_mesh(MeshHost& m) {
         for (int c = 0; c < 3; c++) cent[(size_t)i * 3 + c] = (V[0][c] + V[1][c] + V[2][c]) / 3.0f;
         ids[i] = (int32_t)i;
     }
-    int32_t nn = 0;
-    std::vector<vgpu_mqbvh_node> nodes(std::max<uint32_t>(n, 1));
-    int rc = vgpu::build_qbvh_motion(boxes.data(), cent.data(), ids.data(), (int32_t)n, 16, nodes.data(),
-                                     (int32_t)nodes.size(), &nn, 0);
-    if (rc == -3) return fail(VGPU_ERR_BUILD, "motion BVH build failed: degenerate mesh");
-    if (rc != 0) return fail(VGPU_ERR_BUILD, "motion BVH build failed (%d)", rc);
-    nodes.resize(nn);
+    std::vector<vgpu_mqbvh_node> nodes;
+    if (device >= 0 && n >= kGpuBuildMinTris) {
+        std::vector<vgpu_qbvh_node> full;
+        float bounds[6];
+        const int rc = build_nodes(boxes.data(), cent.data(), ids.data(), n, 16, full, bounds, device);
+        if (rc) return rc;
+        nodes.resize(full.size());
+        for (size_t i = 0; i < full.size(); i++) {
+            vgpu_mqbvh_node& d = nodes[i];
+            d.axis0 = (int32_t)full[i].axis0;
+            d.axis1 = (int32_t)full[i].axis1;
+            d.axis2 = (int32_t)full[i].axis2;
+            for (int c = 0; c < 4; c++) d.children[c] = full[i].children[c];
+            d.pad = 0;
+        }
+    } else {
+        int32_t nn = 0;
+        nodes.resize(std::max<uint32_t>(n, 1));
+        int rc = vgpu::build_qbvh_motion(boxes.data(), cent.data(), ids.data(), (int32_t)n, 16, nodes.data(),
+                                         (int32_t)nodes.size(), &nn, 0);
+        if (rc == -3) return fail(VGPU_ERR_BUILD, "motion BVH build failed: degenerate mesh");
+        if (rc != 0) return fail(VGPU_ERR_BUILD, "motion BVH build failed (%d)", rc);
+        nodes.resize(nn);
+    }
     m.mnodes.swap(nodes);
     m.perm = ids;
+    const size_t nn = m.mnodes.size();
     m.mboxes.assign((size_t)m.nkeys * nn * 24, 0.0f);
     m.motion_bounds.assign((size_t)m.nkeys * 6, 0.0f);
     Box6 full; full.reset();
@@ -577,7 +619,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
     if (s->meshes.empty()) return fail(VGPU_ERR_STATE, "scene has no geometry");
     for (auto& m : s->meshes) {
         if (m.built) continue;
-        int rc = m.nkeys == 1 ? build_static_mesh(m, s->leaf_max) : build_motion_mesh(m);
+        int rc = m.nkeys == 1 ? build_static_mesh(m, s->leaf_max, s->device) : build_motion_mesh(m, s->device);
         if (rc) return rc;
     }
     // Scene.initAccel (scene.go:135-203) over the geoms the scene holds itself; hidden geoms (reached only
