@@ -318,6 +318,36 @@ def other_workloads(torch, np):
     return out
 
 
+def bvh_build_measurement(np, verts, idx, leaf_max=16):
+    """qbvh.BuildAccel over the workload's triangles: the GPU builder (vgpu_qbvh_build_gpu, host arrays in
+    and out, PCIe transfers included; best of 3 calls after a first one) against the host builder on the
+    host share of one GPU (16 threads), and whether the two trees are identical."""
+    import vermeer_amd as va
+    V = verts[idx.astype(np.int64)]
+    boxes = np.ascontiguousarray(np.concatenate([np.minimum(np.minimum(V[:, 0], V[:, 1]), V[:, 2]),
+                                                 np.maximum(np.maximum(V[:, 0], V[:, 1]), V[:, 2])], 1), np.float32)
+    cent = np.ascontiguousarray(((V[:, 0] + V[:, 1]) + V[:, 2]) / np.float32(3), np.float32)
+    ids = np.arange(len(idx), dtype=np.int32)
+    del V
+    gpu_ms = []
+    for _ in range(4):
+        b, c, i = boxes.copy(), cent.copy(), ids.copy()
+        t0 = time.perf_counter()
+        ng, _ = va.build_accel_gpu(b, c, i, leaf_max)
+        gpu_ms.append((time.perf_counter() - t0) * 1e3)
+    b2, c2, i2 = boxes.copy(), cent.copy(), ids.copy()
+    t0 = time.perf_counter()
+    nh, _ = va.build_accel(b2, c2, i2, leaf_max, 16)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    same = ng.tobytes() == nh.tobytes() and np.array_equal(i, i2)
+    out = {"triangles": int(len(idx)), "leaf_max": leaf_max, "nodes": int(len(ng)),
+           "gpu_ms": round(min(gpu_ms[1:]), 2), "gpu_first_call_ms": round(gpu_ms[0], 2),
+           "host_ms": round(host_ms, 2), "host_threads": 16, "identical": bool(same)}
+    log(f"[bench] BVH build over {len(idx)} triangles: GPU {out['gpu_ms']} ms, host (16 threads) "
+        f"{out['host_ms']} ms, identical {same}")
+    return out
+
+
 def path_traversal_roofline(torch, desc, scene, pt, acc, first_pass, passes=4):
     """Roofline of path mode's traversal launches (closest hit per bounce, plus the NEE shadow rays):
     the kernels' time from the library's HIP events over `passes` plain passes, and their
@@ -502,7 +532,9 @@ def main():
     sc.pre_render()
     build_s = time.time() - t0
     info = sc.info()
-    log(f"[bench] host BVH build + upload {build_s:.2f}s, {int(info['geom_nodes'][0])} nodes")
+    log(f"[bench] scene commit (BVH built on the GPU, flattened and uploaded) {build_s:.2f}s, "
+        f"{int(info['geom_nodes'][0])} nodes")
+    bvh_build = bvh_build_measurement(np, verts, idx)
 
     n = args.rays
     d_hits = torch.empty(n * 16, dtype=torch.uint8, device=dev)
@@ -696,7 +728,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "hipgraph": graph,
-        "host_build_s": round(build_s, 2),
+        "scene_commit_s": round(build_s, 2),
+        "bvh_build": bvh_build,
         "guided": None,
     }
     if not args.no_guided or not args.no_others:

@@ -214,6 +214,23 @@ int vgpu_scene_add_mesh_prebuilt(vgpu_scene* scene, const float* verts, uint32_t
                                  const vgpu_qbvh_node* nodes, uint32_t nnodes,
                                  const float* bounds, float ray_bias, int32_t* geom_out);
 
+/* Adds a moving mesh whose accel the caller already built with the
+ * reference's own code (the cgo path: Go's PolyMesh.initMotionAccel output,
+ * buildqbvh.go:19-57, and initMotionBoxes, :148-212).  verts: nkeys*nverts*3
+ * floats, key-major; idxp: ntris*3 vertex indices in face order (mesh.idxp,
+ * which the reference leaves unsorted for moving meshes); accel_idx: ntris
+ * faces in leaf order (mesh.accel.idx); nodes: mesh.accel.mqbvh.Nodes
+ * (axes and children; Parent is not used); boxes: mesh.accel.mqbvh.Boxes,
+ * nkeys*nnodes*24 floats, key-major.  mesh.motionBounds and mesh.bounds
+ * are derived from the root's boxes as initMotionBoxes derives them.  flags:
+ * as vgpu_scene_add_mesh (VGPU_MESH_MOTION_REFERENCE_INDEXING reads idxp by
+ * leaf slot, as TraceMotionElems does). */
+int vgpu_scene_add_motion_mesh_prebuilt(vgpu_scene* scene, const float* verts, uint32_t nverts,
+                                        uint32_t nkeys, const uint32_t* idxp, uint32_t ntris,
+                                        const int32_t* accel_idx, const vgpu_mqbvh_node* nodes,
+                                        uint32_t nnodes, const float* boxes, float ray_bias,
+                                        uint32_t flags, int32_t* geom_out);
+
 /* A PolyMesh's Transform (builtin/geom/polymesh/trace.go:20-74): nkeys
  * column-major 4x4 matrices.  Rays are moved into the mesh's space by the
  * inverse of the transform at their time (one key: the matrix as given; more:

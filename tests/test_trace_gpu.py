@@ -437,3 +437,66 @@ def test_mesh_with_non_finite_vertex_is_rejected():
         g.add_geom(va.PolyMesh(v, f))
     assert ei.value.code == va._lib.VGPU_ERR_ARG
     g.close()
+
+
+@pytest.mark.parametrize("ref_idx", [False, True])
+def test_prebuilt_motion_mesh_equals_built_mesh_and_oracle(ref_idx):
+    """vgpu_scene_add_motion_mesh_prebuilt fed with the oracle's BuildAccelMotion + initMotionBoxes output
+    (topology, per-key boxes, accel.idx) and the unsorted idxp, as a Go host hands them over: traversal
+    bit-identical to the same mesh built by the library and to the oracle, in both idxp indexings."""
+    verts, idx = scenes.moving_spheres(count=300)
+    o = oracle.OracleScene()
+    o.add_mesh(verts, idx, motion_reference_indexing=ref_idx)
+    o.commit()
+    topo, boxes, perm = o.mesh_motion_accel(0)
+    pre = va.Scene(0)
+    pre.add_motion_prebuilt(verts, idx, perm, topo, boxes, reference_indexing=ref_idx)
+    pre.pre_render()
+    built = va.Scene(0)
+    built.add_geom(va.PolyMesh(verts, idx, motion_reference_indexing=ref_idx))
+    built.pre_render()
+    gs = scenes.rng(4)
+    n = 100_000
+    rays = scenes.make_rays(gs.uniform(-25, 25, (n, 3)), scenes.uniform_sphere_dirs(gs, n), time=gs.random(n))
+    print(f"[case] prebuilt motion mesh ref_idx={ref_idx}", flush=True)
+    for shadow in (False, True):
+        h1, g1, _ = pre.trace(rays, shadow=shadow)
+        h2, g2, _ = built.trace(rays, shadow=shadow)
+        ref, _ = o.trace(rays, shadow=shadow)
+        _same(h1, g1, ref, f"prebuilt motion shadow={shadow}")
+        assert h1.tobytes() == h2.tobytes() and np.array_equal(g1, g2)
+    if not ref_idx:   # leaf-slot indexing tests other faces than the boxes hold: few rays hit
+        assert (h1["prim"] >= 0).mean() > 0.01
+    pre.close()
+    built.close()
+
+
+def test_prebuilt_motion_mesh_rejects_bad_accels():
+    verts, idx = scenes.moving_spheres(count=4)
+    o = oracle.OracleScene()
+    o.add_mesh(verts, idx, motion_reference_indexing=False)
+    o.commit()
+    topo, boxes, perm = o.mesh_motion_accel(0)
+    sc = va.Scene(0)
+    with pytest.raises(va.VgpuError):            # one key, through the C ABI
+        va._lib.check(sc._lib.vgpu_scene_add_motion_mesh_prebuilt(
+            sc._h, va._lib.ptr(np.ascontiguousarray(verts[:1])), verts.shape[1], 1, va._lib.ptr(idx), len(idx),
+            va._lib.ptr(perm), va._lib.ptr(topo), len(topo), va._lib.ptr(np.ascontiguousarray(boxes[:1])), 0.0, 0,
+            None))
+    bad = topo.copy()
+    bad[0, 3] = len(topo)                        # child beyond the node count
+    with pytest.raises(va.VgpuError):
+        sc.add_motion_prebuilt(verts, idx, perm, bad, boxes)
+    if len(topo) > 1:
+        cyc = topo.copy()
+        j = int(np.flatnonzero(cyc[0, 3:7] > 0)[0]) + 3
+        child = cyc[0, j]
+        k = int(np.flatnonzero(cyc[child, 3:7] >= -1)[0]) + 3 if (cyc[child, 3:7] >= -1).any() else 3
+        cyc[child, k] = 0                        # points back at the root
+        with pytest.raises(va.VgpuError):
+            sc.add_motion_prebuilt(verts, idx, perm, cyc, boxes)
+    bp = perm.copy()
+    bp[0] = len(perm)
+    with pytest.raises(va.VgpuError):
+        sc.add_motion_prebuilt(verts, idx, bp, topo, boxes)
+    sc.close()

@@ -73,6 +73,8 @@ SIGNATURES = [
     ("vgpu_scene_destroy", C.c_int, [_P]),
     ("vgpu_scene_add_mesh", C.c_int, [_P, _P, _U32, _U32, _P, _U32, _F, _U32, C.POINTER(_I32)]),
     ("vgpu_scene_add_mesh_prebuilt", C.c_int, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _F, C.POINTER(_I32)]),
+    ("vgpu_scene_add_motion_mesh_prebuilt", C.c_int,
+     [_P, _P, _U32, _U32, _P, _U32, _P, _P, _U32, _P, _F, _U32, C.POINTER(_I32)]),
     ("vgpu_scene_commit", C.c_int, [_P]),
     ("vgpu_scene_info", C.c_int, [_P, C.POINTER(_I32), C.POINTER(_I32), _P, _P, _P]),
     ("vgpu_scene_mesh_accel", C.c_int, [_P, _I32, _P, _P, _P]),

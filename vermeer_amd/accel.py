@@ -165,6 +165,28 @@ class Scene:
         self.geoms.append(None)
         return g.value
 
+    def add_motion_prebuilt(self, verts, idxp, accel_idx, nodes, boxes, ray_bias=0.0, reference_indexing=False):
+        """Moving mesh with an accel built by the reference's own code (the cgo path): verts (nkeys, nverts, 3),
+        idxp (ntris, 3) in face order, accel_idx (ntris,) faces in leaf order, nodes (MNODE_DTYPE or an
+        (nnodes, 8) int32 array: axis0..2, children[4], pad), boxes (nkeys, nnodes, 24)."""
+        v = np.ascontiguousarray(verts, dtype=np.float32)
+        if v.ndim != 3 or v.shape[2] != 3 or v.shape[0] < 2:
+            raise ValueError("verts must be (nkeys >= 2, nverts, 3)")
+        ix = np.ascontiguousarray(idxp, dtype=np.uint32).reshape(-1, 3)
+        ai = np.ascontiguousarray(accel_idx, dtype=np.int32).reshape(-1)
+        nd = np.ascontiguousarray(nodes)
+        if nd.dtype != MNODE_DTYPE:
+            nd = np.ascontiguousarray(nd, dtype=np.int32).reshape(-1, 8).view(MNODE_DTYPE).reshape(-1)
+        bx = np.ascontiguousarray(boxes, dtype=np.float32)
+        if bx.shape != (v.shape[0], len(nd), 24):
+            raise ValueError(f"boxes must be (nkeys, nnodes, 24), got {bx.shape}")
+        g = C.c_int32(-1)
+        check(self._lib.vgpu_scene_add_motion_mesh_prebuilt(
+            self._h, ptr(v), v.shape[1], v.shape[0], ptr(ix), len(ix), ptr(ai), ptr(nd), len(nd), ptr(bx),
+            float(ray_bias), 1 if reference_indexing else 0, C.byref(g)))
+        self.geoms.append(None)
+        return g.value
+
     def add_sphere(self, centre, radius, material):
         """The analytic sphere a SphereLight inserts (builtin/geom/sphere) with its shader (a material id)."""
         c = np.ascontiguousarray(centre, dtype=np.float32).reshape(3)

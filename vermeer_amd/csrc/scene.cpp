@@ -613,6 +613,58 @@ int vgpu_scene_add_mesh_prebuilt(vgpu_scene* s, const float* verts, uint32_t nve
     return VGPU_OK;
 }
 
+int vgpu_scene_add_motion_mesh_prebuilt(vgpu_scene* s, const float* verts, uint32_t nverts, uint32_t nkeys,
+                                        const uint32_t* idxp, uint32_t ntris, const int32_t* accel_idx,
+                                        const vgpu_mqbvh_node* nodes, uint32_t nnodes, const float* boxes,
+                                        float ray_bias, uint32_t flags, int32_t* geom_out) {
+    if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
+    if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
+    if (nkeys < 2) return fail(VGPU_ERR_ARG, "a moving mesh has at least 2 keys (vgpu_scene_add_mesh_prebuilt for static)");
+    if ((flags & ~VGPU_MESH_MOTION_REFERENCE_INDEXING) != 0) return fail(VGPU_ERR_ARG, "unknown mesh flags 0x%x", flags);
+    if (!accel_idx || !nodes || !boxes || nnodes == 0 || ntris == 0) return fail(VGPU_ERR_ARG, "empty accel");
+    for (uint32_t i = 0; i < ntris; i++)
+        if (accel_idx[i] < 0 || (uint32_t)accel_idx[i] >= ntris)
+            return fail(VGPU_ERR_ARG, "accel index %d at %u out of range", accel_idx[i], i);
+    for (uint32_t i = 0; i < nnodes; i++)
+        for (int c = 0; c < 4; c++) {
+            const int32_t ch = nodes[i].children[c];
+            if (ch >= (int32_t)nnodes) return fail(VGPU_ERR_ARG, "node %u child %d out of range", i, ch);
+            // pre-order numbering (BuildAccelMotion's recursion): a child follows its parent, so no cycles
+            if (ch >= 0 && ch <= (int32_t)i) return fail(VGPU_ERR_ARG, "node %u child %d does not follow its parent", i, ch);
+            if (ch < -1) {
+                const uint32_t b = (uint32_t)((ch & 0x7ffffff) >> 4), cnt = (uint32_t)((ch & 0xf) + 1);
+                if (b + cnt > ntris) return fail(VGPU_ERR_ARG, "node %u leaf [%u,+%u) beyond %u triangles", i, b, cnt, ntris);
+            }
+        }
+    for (uint64_t i = 0; i < (uint64_t)nkeys * nnodes * 24; i++)
+        if (std::isnan(boxes[i])) return fail(VGPU_ERR_ARG, "motion box value %llu is NaN", (unsigned long long)i);
+    const int rc = vgpu_scene_add_mesh(s, verts, nverts, nkeys, idxp, ntris, ray_bias, flags, geom_out);
+    if (rc) return rc;
+    MeshHost& m = s->meshes.back();
+    m.mnodes.assign(nodes, nodes + nnodes);
+    m.mboxes.assign(boxes, boxes + (size_t)nkeys * nnodes * 24);
+    m.perm.assign(accel_idx, accel_idx + ntris);
+    // mesh.motionBounds[k] is the root's box in key k and mesh.bounds their union (initMotionBoxes,
+    // buildqbvh.go:151-169): the stored child boxes of node 0 grown in child order
+    m.motion_bounds.assign((size_t)nkeys * 6, 0.0f);
+    Box6 full; full.reset();
+    for (uint32_t k = 0; k < nkeys; k++) {
+        Box6 rb; rb.reset();
+        const float* nb = &m.mboxes[(size_t)k * nnodes * 24];
+        for (int c = 0; c < 4; c++) {
+            if (nodes[0].children[c] == -1) continue;
+            float cb[6];
+            for (int a = 0; a < 3; a++) { cb[a] = nb[c + a * 4]; cb[3 + a] = nb[c + 12 + a * 4]; }
+            rb.grow_box(cb);
+        }
+        std::memcpy(&m.motion_bounds[(size_t)k * 6], rb.v, sizeof(rb.v));
+        full.grow_box(rb.v);
+    }
+    std::memcpy(m.bounds, full.v, sizeof(full.v));
+    m.built = true;
+    return VGPU_OK;
+}
+
 int vgpu_scene_commit(vgpu_scene* s) {
     if (!s) return fail(VGPU_ERR_ARG, "scene is NULL");
     if (s->committed) return fail(VGPU_ERR_STATE, "scene already committed");
