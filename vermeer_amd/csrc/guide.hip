@@ -435,8 +435,8 @@ __device__ __forceinline__ void layer_back(const u32x4* sB, int fbase, int lane,
 // LDS, feature-major with rows of STG bf16 (the 8-element pad makes the 16-byte operand reads of 16
 // consecutive rows conflict-free), and adds dY X^T to its dW tiles on the matrix cores with the
 // samples as the k dimension.  The tiles of every layer stay in the waves' accumulators over the
-// chunks; the workgroup then writes one fp32 partial of every parameter, which k_adam adds in
-// workgroup order.  X and dY never reach HBM.
+// chunks; the workgroup then writes one fp32 partial of every parameter, which k_adam adds in a fixed
+// order.  X and dY never reach HBM.
 constexpr int TRAIN_CHUNK = 128;
 constexpr int TRAIN_CHUNKS = TRAIN_WG / TRAIN_CHUNK;
 static_assert(TRAIN_CHUNKS * TRAIN_CHUNK == TRAIN_WG, "chunks tile the workgroup's samples");
@@ -672,25 +672,41 @@ __global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32
     dw_write(T5, bs5, Pp, OW5, OB5, NH, NOUT);
 }
 
-// Adds the weight-gradient partials (split order), applies Adam and writes the updated parameter's
-// bf16 copy into its forward and backward fragment slots (so no separate repacking pass).
-__global__ void k_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
-                       const float* __restrict__ partial, uint32_t nsplit, float invB, float lr,
-                       const float* __restrict__ csum, const int32_t* __restrict__ step, int update,
-                       const int32_t* __restrict__ slot_fwd, const int32_t* __restrict__ slot_bwd,
-                       uint16_t* __restrict__ ffwd, uint16_t* __restrict__ fbwd) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= NPARAM) return;
+// k_adam: ADAM_PARAMS parameters per workgroup, each summed by ADAM_GROUPS threads over contiguous
+// runs of partials; the run sums are then added in run order, so the reduction order is fixed.
+constexpr int ADAM_PARAMS = 32, ADAM_GROUPS = 8;
+
+// Adds the weight-gradient partials (workgroup order within each run, then run order), applies Adam
+// and writes the updated parameter's bf16 copy into its forward and backward fragment slots (so no
+// separate repacking pass).
+__global__ __launch_bounds__(ADAM_PARAMS * ADAM_GROUPS) void k_adam(
+        float* __restrict__ P, float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
+        const float* __restrict__ partial, uint32_t nsplit, float invB, float lr, const float* __restrict__ csum,
+        const int32_t* __restrict__ step, int update, const int32_t* __restrict__ slot_fwd,
+        const int32_t* __restrict__ slot_bwd, uint16_t* __restrict__ ffwd, uint16_t* __restrict__ fbwd) {
+    __shared__ float run_sum[ADAM_GROUPS][ADAM_PARAMS];
+    const int q = threadIdx.x % ADAM_PARAMS, grp = threadIdx.x / ADAM_PARAMS;
+    const int p = blockIdx.x * ADAM_PARAMS + q;
+    const uint32_t per = (nsplit + ADAM_GROUPS - 1) / ADAM_GROUPS;
+    const uint32_t s0 = min(nsplit, (uint32_t)grp * per), s1 = min(nsplit, s0 + per);
     float gs = 0.0f;
-    uint32_t s = 0;
-    for (; s + 16 <= nsplit; s += 16) {   // 16 loads in flight, then the adds in split order
-        float t[16];
+    if (p < NPARAM) {
+        uint32_t s = s0;
+        for (; s + 8 <= s1; s += 8) {   // 8 loads in flight, then the adds in order
+            float t[8];
 #pragma unroll
-        for (int j = 0; j < 16; j++) t[j] = partial[(uint64_t)(s + j) * NPARAM + p];
+            for (int j = 0; j < 8; j++) t[j] = partial[(uint64_t)(s + j) * NPARAM + p];
 #pragma unroll
-        for (int j = 0; j < 16; j++) gs += t[j];
+            for (int j = 0; j < 8; j++) gs += t[j];
+        }
+        for (; s < s1; s++) gs += partial[(uint64_t)s * NPARAM + p];
     }
-    for (; s < nsplit; s++) gs += partial[(uint64_t)s * NPARAM + p];
+    run_sum[grp][q] = gs;
+    __syncthreads();
+    if (grp != 0 || p >= NPARAM) return;
+    gs = run_sum[0][q];
+#pragma unroll
+    for (int g = 1; g < ADAM_GROUPS; g++) gs += run_sum[g][q];
     const float gr = gs * invB;
     grad[p] = gr;
     if (!update || !(csum[0] > 0.0f)) return;
@@ -739,8 +755,6 @@ hipError_t launch_infer(const Net& net, const float4* queries, uint64_t n, const
 
 constexpr int kMaxDevices = 64;
 
-// k_adam's workgroup size: small, so that its 24k parameters spread over many CUs.
-constexpr int kAdamBlock = 64;
 
 hipError_t launch_train(const Net& net, float* params, float* m, float* v, const SampleSource& src, uint32_t B,
                         float lr, bool update, const TrainBuffers& tb, hipStream_t st) {
@@ -762,7 +776,8 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
     hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, src, tb.csum + 1);
     const uint32_t nwg = B / TRAIN_WG;
     hipLaunchKernelGGL(k_train, dim3(nwg), dim3(512), TRAIN_LDS, st, net, src, B, tb, update ? 1 : 0);
-    hipLaunchKernelGGL(k_adam, dim3((NPARAM + kAdamBlock - 1) / kAdamBlock), dim3(kAdamBlock), 0, st, params, m, v,
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + ADAM_PARAMS - 1) / ADAM_PARAMS), dim3(ADAM_PARAMS * ADAM_GROUPS), 0, st,
+                       params, m, v,
                        tb.grad, tb.partial, nwg, 1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0,
                        (const int32_t*)tb.slot_fwd, (const int32_t*)tb.slot_bwd, (uint16_t*)net.frag_fwd,
                        (uint16_t*)net.frag_bwd);
