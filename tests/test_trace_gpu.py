@@ -365,7 +365,7 @@ def mixed_motion():
     g.close()
 
 
-@pytest.mark.parametrize("cfg", [(64, 8, 32), (128, 12, 24), (256, 16, 16), (64, 32, 16)])
+@pytest.mark.parametrize("cfg", [(64, 8, 32), (128, 12, 24), (256, 16, 16), (64, 32, 16), (64, 8, 20)])
 def test_mixed_motion_scene_every_config_bitexact(mixed_motion, cfg):
     """Hits and per-ray counters equal the oracle's for closest and any hit in every launch shape."""
     g, rays, refs = mixed_motion

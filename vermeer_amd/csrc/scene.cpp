@@ -756,7 +756,7 @@ int vgpu_scene_commit(vgpu_scene* s) {
     // Default launch shape unless the caller chose one: static scenes run best at block 128 with a
     // 12-entry LDS stack and 24 waves per CU; the motion kernel (about 120 VGPRs, four waves per
     // SIMD) at block 256, 16 entries, 16 waves per CU.
-    if (!s->cfg_set) s->cfg = s->any_motion ? vgpu_trace_config{256, 16, 16, 0, 0} : vgpu_trace_config{128, 12, 24, 0, 0};
+    if (!s->cfg_set) s->cfg = s->any_motion ? vgpu_trace_config{64, 8, 20, 0, 0} : vgpu_trace_config{128, 12, 24, 0, 0};
 
     // Flatten into the device layout.
     std::vector<vgpu::DevNode> dnodes;
@@ -1137,7 +1137,7 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
         return VGPU_OK;
     };
     int rc = VGPU_OK;
-    const int sorts[] = {0, 4, 5, 6}, blocks[] = {64, 128, 256}, ldss[] = {8, 12, 16, 24, 32}, waves[] = {8, 16, 32};
+    const int sorts[] = {0, 4, 5, 6}, blocks[] = {64, 128, 256}, ldss[] = {8, 12, 16, 24, 32}, waves[] = {8, 16, 20, 32};
     for (int so : sorts)
         for (int b : blocks)
             for (int l : ldss)
