@@ -678,10 +678,13 @@ def main():
             ents = [e for e in tj.get("entries", []) if e["sort"] == cfg["sort"]]
             if ents:
                 ent = min(ents, key=lambda e: (e["lds_stack"] != cfg["lds_stack"], e["block"] != cfg["block"],
+                                               e.get("waves_per_cu") != cfg["waves_per_cu"],
                                                abs(e["lds_stack"] - cfg["lds_stack"]), abs(e["block"] - cfg["block"])))
                 traffic = round(ent["hbm_bytes_per_ray"] * n / len(segs))
                 traffic_cfg = {"block": ent["block"], "lds_stack": ent["lds_stack"], "sort": ent["sort"],
-                               "same_as_run": (ent["block"], ent["lds_stack"]) == (cfg["block"], cfg["lds_stack"])}
+                               "waves_per_cu": ent.get("waves_per_cu"),
+                               "same_as_run": (ent["block"], ent["lds_stack"], ent.get("waves_per_cu")) ==
+                                              (cfg["block"], cfg["lds_stack"], cfg["waves_per_cu"])}
 
     out = {
         "metric": METRIC,
@@ -724,6 +727,11 @@ def main():
             "frac_of_achievable_read": round(achieved / achievable["read_GBps"], 4),
             "kernel_ms_per_step": round(kernel_ms_per_step, 4),
             "kernel_avg_launch_ms": round(trace_ms / max(nlaunch, 1), 4),
+            # the HBM side of the same launches: measured bytes per launch over the measured launch time
+            "hbm_achieved_GBps": (round(traffic / (trace_ms / max(nlaunch, 1) * 1e-3) / 1e9, 1)
+                                  if traffic else None),
+            "hbm_frac": (round(traffic / (trace_ms / max(nlaunch, 1) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                         if traffic else None),
             "sort_ms_per_step": round(sort_ms_per_step, 4),
         },
         "cpu_baseline": cpu,

