@@ -366,6 +366,16 @@ class OracleRenderer:
         self.w, self.h = width, height
         self._keep = []
         for mesh in desc["meshes"]:
+            if mesh.get("instance") is not None:   # an instance or proc in scene order (a .vnf's AddNode order)
+                it = mesh["instance"]
+                self.scene.add_instance(it["geom"], it["matrices"], it["bmin"], it["bmax"])
+                self.scene.meshes.append(None)
+                continue
+            if mesh.get("proc") is not None:
+                pr = mesh["proc"]
+                self.scene.add_proc(pr["geoms"], pr.get("matrices"), pr["bmin"], pr["bmax"])
+                self.scene.meshes.append(None)
+                continue
             if mesh.get("sphere"):
                 c = np.ascontiguousarray(mesh["centre"], np.float32)
                 self._keep.append(c)

@@ -600,12 +600,13 @@ def build(path, cwd=None):
     allnodes, by_name, geoms, lights = [], {}, [], []
     state = {"filter": None, "globals": None}
     light_geom = {}
+    proc_geoms = {}   # id(Proc) -> the geoms its handler made (Proc.Geom; not in the scene itself)
 
     def add(n):
         allnodes.append(n)
         by_name[_name(n)] = n
         t = n["type"]
-        if t in ("PolyMesh", "Sphere"):
+        if t in ("PolyMesh", "Sphere", "GeomInstance", "Proc"):
             geoms.append(n)
         elif t in ("TriLight", "DiskLight", "SphereLight", "QuadLight"):
             lights.append(n)
@@ -637,8 +638,38 @@ def build(path, cwd=None):
                     add(s)
                     added.append(s)
                 continue
-            if t in ("Proc", "GeomInstance", "QuadLight"):
+            if t == "QuadLight":
                 raise Unsupported(t)
+            if t == "GeomInstance":   # Instance.PreRender (builtin/geom/instance/instance.go:116-146)
+                g = by_name.get(_val(n, "Geom"))
+                if g is None or g["type"] not in ("PolyMesh", "Sphere", "GeomInstance", "Proc"):
+                    raise Unsupported(f"Instance: Unable to find geom {_val(n, 'Geom')}")
+                continue
+            if t == "Proc":   # Proc.PreRender (builtin/geom/proc/proc.go:140-172) with the "vnf" handler
+                if _val(n, "Handler") != "vnf":
+                    raise Unsupported(f"Proc handler {_val(n, 'Handler')}")
+                sub, ne, lg, fa = parse_file(_val(n, "Data"), counters)
+                if fa:
+                    raise Unsupported("too many parse errors in a Proc's file")
+                nerrors += ne
+                # VNFFile.Init (builtin/geom/proc/vnf/vnf.go:19-56): the file's shaders join the scene's
+                # nodes; its geoms, prerendered, belong to the Proc alone; other nodes are dropped
+                for x in sub:
+                    if x["type"] in ("ShaderStd", "DebugShader"):
+                        add(x)
+                kids = []
+                for x in sub:
+                    if x["type"] in ("PolyMesh", "Sphere"):
+                        for s_ in (_val(x, "Shader") if x["type"] == "PolyMesh" else [_val(x, "Shader")]):
+                            if shader(s_) is None:
+                                raise Unsupported(f"Unable to find node (shader {s_})")
+                        kids.append(x)
+                    elif x["type"] in ("GeomInstance", "Proc"):
+                        raise Unsupported(f"{x['type']} inside a Proc")
+                if not kids:
+                    raise Unsupported(f"GeomProc: {_name(n)} no geoms.")
+                proc_geoms[id(n)] = kids
+                continue
             if t in ("PolyMesh", "Sphere", "TriLight", "DiskLight", "SphereLight"):
                 for s in (_val(n, "Shader") if t == "PolyMesh" else [_val(n, "Shader")]):
                     if shader(s) is None:
@@ -673,18 +704,19 @@ def build(path, cwd=None):
         return len(mats) - 1
 
     meshes, geom_index = [], {}
-    for g in geoms:
-        geom_index[id(g)] = len(meshes)
+
+    def matrices(tr):
+        e = tr[2]
+        return np.array(e[: len(e) // 16 * 16], np.float32).reshape(-1, 16)
+
+    def box_of(n):   # BMin[0] and BMax[0]: the instance and proc use bounds[0] only (instance.go:64-69)
+        return np.array(_val(n, "BMin")[2][:3], np.float32), np.array(_val(n, "BMax")[2][:3], np.float32)
+
+    def add_mesh(g):   # a PolyMesh or Sphere entry (PolyMesh.init, builtin/geom/polymesh/init.go:12-134)
         if g["type"] == "Sphere":
             meshes.append({"sphere": True, "centre": np.array(_val(g, "P")[2], np.float32),
                            "radius": _val(g, "Radius"), "material": material(_val(g, "Shader"))})
-            continue
-        tr = _val(g, "Transform")
-        if tr is not None:
-            e = tr[2]
-            for k in range(0, len(e) // 16 * 16, 16):
-                if e[k:k + 16] != [1.0 if i % 5 == 0 else 0.0 for i in range(16)]:
-                    raise Unsupported("mesh transform")
+            return
         keys, per, ev = _val(g, "Verts")
         keys = max(keys, 1)
         idxp, nidx, sidx = _triangulate(g)
@@ -709,7 +741,37 @@ def build(path, cwd=None):
                 nk, nper = max(nv[0], 1), nv[1]
                 arr = np.array(nv[2], np.float32).reshape(nk, nper, 3)[:, : len(idxp)]
                 mesh["normals"] = arr.reshape(nk, -1, 3) if nv[0] == keys else None
+        tr = _val(g, "Transform")
+        if tr is not None:   # PolyMesh.Trace moves rays whenever Transform is given (trace.go:22)
+            mesh["transform"] = matrices(tr)
         meshes.append(mesh)
+
+    for g in geoms:
+        if g["type"] == "GeomInstance":
+            tgt = by_name[_val(g, "Geom")]
+            if id(tgt) not in geom_index:
+                raise Unsupported("GeomInstance of a geom that follows it")
+            if tgt["type"] not in ("PolyMesh", "Sphere"):
+                raise Unsupported("GeomInstance of an instance or proc")
+            lo, hi = box_of(g)
+            geom_index[id(g)] = len(meshes)
+            meshes.append({"instance": {"geom": geom_index[id(tgt)], "matrices": matrices(_val(g, "Transform")),
+                                        "bmin": lo, "bmax": hi}})
+            continue
+        if g["type"] == "Proc":   # its geoms are held by the Proc only: hidden from the top level
+            kids = []
+            for x in proc_geoms[id(g)]:
+                kids.append(len(meshes))
+                add_mesh(x)
+                meshes[-1]["hidden"] = True
+            lo, hi = box_of(g)
+            tr = _val(g, "Transform")
+            geom_index[id(g)] = len(meshes)
+            meshes.append({"proc": {"geoms": kids, "matrices": None if tr is None else matrices(tr),
+                                    "bmin": lo, "bmax": hi}})
+            continue
+        geom_index[id(g)] = len(meshes)
+        add_mesh(g)
 
     from vermeer_amd import scenes
     from vermeer_amd._lib import MATERIAL_DTYPE

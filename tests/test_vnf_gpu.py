@@ -6,7 +6,9 @@ The scene exercises what PreRender decides: a PolyMesh with PolyCount fans, per-
 and Normals on their own NormalIdx; a moving PolyMesh with per-corner normals on both keys; a Sphere
 geom; Tri, Disk and Sphere lights (each adding its own geometry); an Included file (read at
 PreRender) holding a ShaderStd; a LookAt camera with two From keys, a Roll and a thin lens; a
-GaussianFilter made parseable by a NodeDef token; OutputHDR and OutputFloat nodes.
+GaussianFilter made parseable by a NodeDef token; OutputHDR and OutputFloat nodes.  A second scene
+holds a GeomInstance with two transform keys, a PolyMesh with its own Transform and a Proc reading
+another file through the "vnf" handler.
 
 Bar: with both sides on the IEEE 1/sqrt estimate, every pass's samples and the film are
 bit-identical, and the files written are byte-identical to the oracle's writers on the oracle's film.
@@ -129,13 +131,112 @@ def test_vnf_render_file_writes_outputs(tmp_path):
     assert (tmp_path / "out.float").read_bytes() == ref_float.read_bytes()
 
 
+CAMERA = ('Camera { Name "camera" Type "LookAt" From 1 1 point 0 1.2 4 To 1 1 point 0 0.5 0 Roll 0 0 float '
+          'Up 0 1 0 Fov 50 Focal 3.5 }\n')
+
+
+def _instance_scene(tmp_path):
+    sub = tmp_path / "held.vnf"
+    sub.write_text('ShaderStd { Name "blue" DiffuseColour rgb 0.1 0.2 0.8 DiffuseStrength float 1 '
+                   'Spec1Strength float 0.3 Spec1Roughness float 0.3 }\n'
+                   'PolyMesh { Name "fin_a" Verts 1 3 point -0.3 0 0  0.3 0 0  0 0.7 0 Shader 1 string "blue" }\n'
+                   'PolyMesh { Name "fin_b" Verts 1 3 point -0.3 0 -0.4  0.3 0 -0.4  0 0.5 -0.4 Shader 1 string "white" }\n'
+                   'Sphere { Name "knob" P 0 0.9 -0.2 Radius 0.15 Shader "blue" }\n'
+                   'TriLight { Name "dropped" P0 0 3 0 P1 1 3 0 P2 0 3 1 Shader "blue" Samples 1 }\n')
+    text = f"""Globals {{ XRes {W} YRes {H} MaxIter {PASSES} }}
+{CAMERA}
+ShaderStd {{ Name "white" DiffuseColour rgb 0.8 0.8 0.8 DiffuseStrength float 1 }}
+ShaderStd {{ Name "red" DiffuseColour rgb 0.8 0.2 0.1 DiffuseStrength float 1 }}
+ShaderStd {{ Name "glow" EmissionColour rgb 1 0.9 0.8 EmissionStrength float 12 DiffuseStrength float 1 }}
+PolyMesh {{
+    Name "floor"
+    Verts 1 4 point -3 0 -3  3 0 -3  3 0 3  -3 0 3
+    PolyCount 1 int 4
+    FaceIdx 4 int 0 3 2 1
+    Shader 1 string "white"
+}}
+PolyMesh {{
+    Name "proto"
+    Verts 1 4 point 0 0 0  0.5 0 0  0 0.5 0  0 0 0.5
+    FaceIdx 12 int 0 2 1  0 1 3  0 3 2  1 2 3
+    Shader 1 string "red"
+}}
+GeomInstance {{
+    Name "spun"
+    Geom "proto"
+    BMin 1 1 point -2 -0.5 -1
+    BMax 1 1 point 2 1.5 2
+    Transform 2 matrix 1 0 0 0.8  0 1 0 0.1  0 0 1 0.4  0 0 0 1
+                       0.866 0 0.5 1.1  0 1 0 0.3  -0.5 0 0.866 0.5  0 0 0 1
+}}
+PolyMesh {{
+    Name "turned"
+    Verts 1 3 point -1.5 0.1 -0.5  -0.9 0.1 -0.5  -1.5 0.7 -0.5
+    FaceIdx 3 int 0 1 2
+    Shader 1 string "white"
+    Transform 1 matrix 0 -1 0 -0.8  1 0 0 1.6  0 0 1 0  0 0 0 1
+}}
+Proc {{
+    Name "held"
+    Handler "vnf"
+    Data "{sub}"
+    BMin 1 1 point -1 -0.5 -1.5
+    BMax 1 1 point 1 1.5 1.5
+    Transform 1 matrix 1 0 0 -0.2  0 1 0 0  0 0 1 1.2  0 0 0 1
+}}
+TriLight {{ Name "tri" P0 -0.5 2.9 -0.5 P1 0.5 2.9 -0.5 P2 0 2.9 0.5 Shader "glow" Samples 1 }}
+"""
+    p = tmp_path / "instances.vnf"
+    p.write_text(text)
+    return p
+
+
+def test_vnf_instances_procs_transforms_bitexact(tmp_path):
+    """A GeomInstance with two transform keys, a PolyMesh with its own Transform (a quarter turn about
+    its box centre, so that rays entering its untransformed bounds find it) and a Proc whose "vnf" handler
+    reads a file of two meshes and a sphere (its TriLight dropped), under a moved Proc transform.  Camera
+    rays hit each of them."""
+    print("\n[case] vnf instances, procs and transforms", flush=True)
+    path = _instance_scene(tmp_path)
+    f = VnfFile(path)
+    R = f.build(0)
+    oracle.set_normalize_ieee(True)
+    try:
+        built = oracle_vnf.build(str(path))
+        kinds = [("instance" if "instance" in m else "proc" if "proc" in m else "hidden" if m.get("hidden")
+                  else "geom") for m in built["desc"]["meshes"]]
+        assert kinds == ["geom", "geom", "instance", "geom", "hidden", "hidden", "hidden", "proc", "geom"]
+        o = oracle_vnf.renderer(built)
+        # camera rays reach the floor, the prototype, the instance, the turned mesh and the proc
+        rays = scenes.pinhole_rays(W, H, fov_deg=50.0, eye=(0.0, 1.2, 4.0), target=(0.0, 0.5, 0.0))
+        rays["time"] = np.linspace(0, 1, len(rays), dtype=np.float32)
+        hit_slots = set(np.unique(o.scene.trace(rays)[0]["geom"]).tolist())
+        assert {0, 1, 2, 3, 4} <= hit_slots
+        scr = scenes.pixel_scrambles(W, H, seed=6)
+        for it in range(1, PASSES + 1):
+            print(f"[case] vnf instance pass {it}", flush=True)
+            ng, gs = R.render_pass(scr, it, want_sample=True)
+            no, os_ = o.render_pass(scr, it, want_sample=True)
+            assert ng == no, f"pass {it}: shadow rays {ng} vs {no}"
+            assert _nan_equal(gs, os_), f"pass {it}: samples differ"
+        assert _nan_equal(R.film, o.film)
+        assert (R.film > 0).mean() > 0.3
+    finally:
+        oracle.set_normalize_ieee(False)
+    R.close()
+    f.close()
+
+
 def test_vnf_build_errors(tmp_path):
     from vermeer_amd import _lib
     cases = {
         "nocam.vnf": 'ShaderStd { Name "s" }\n',
         "noshader.vnf": 'TriLight { Name "t" P0 0 0 0 P1 1 0 0 P2 0 1 0 Shader "none" Samples 1 }\n',
-        "transform.vnf": ('ShaderStd { Name "s" }\nPolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 '
-                          'Shader 1 string "s" Transform 1 matrix 2 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1 }\n'),
+        "wfobj.vnf": ('ShaderStd { Name "s" }\nProc { Name "p" Handler "wfobj" Data "x.obj" '
+                      'BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 }\n'),
+        "forward.vnf": (CAMERA + 'ShaderStd { Name "s" }\nGeomInstance { Name "i" Geom "m" BMin 1 1 point 0 0 0 '
+                        'BMax 1 1 point 1 1 1 Transform 1 matrix 1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1 }\n'
+                        'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "s" }\n'),
         "quad.vnf": 'ShaderStd { Name "s" }\nQuadLight { Name "q" P 0 0 0 U 1 0 0 V 0 1 0 Shader "s" }\n',
     }
     for name, text in cases.items():

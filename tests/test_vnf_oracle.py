@@ -8,6 +8,10 @@ answers worked from the Go source:
   core/core.go:69-89        AddNode order decides geoms and lights; a light's PreRender adds its
                             geometry as a new node, prerendered in the next round
   core/render.go:146-163    the camera is the node named by Globals.Camera, else "camera"
+  instance.go:116-146,      a GeomInstance takes its geom by name and its bounds from BMin[0] / BMax[0];
+  proc.go:140-172,          a Proc with the "vnf" handler reads its file at PreRender: the file's shaders
+  proc/vnf/vnf.go:19-56     join the scene, its geoms belong to the Proc alone (hidden from the top
+                            level), its other nodes are dropped; a Proc without Transform holds the identity
 """
 import math
 
@@ -90,9 +94,11 @@ def test_camera_selection_and_errors(tmp_path):
         _build(tmp_path, 'Globals { Camera "none" }\n' + CAM + SH)
     with pytest.raises(oracle_vnf.Unsupported, match="shader"):
         _build(tmp_path, CAM + 'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "x" }\n')
-    with pytest.raises(oracle_vnf.Unsupported, match="transform"):
-        _build(tmp_path, CAM + SH + 'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "a" '
+    # a PolyMesh Transform is kept as given (row-major in the file, stored column-major)
+    b = _build(tmp_path, CAM + SH + 'PolyMesh { Name "m" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "a" '
                                     'Transform 1 matrix 1 0 0 1 0 1 0 0 0 0 1 0 0 0 0 1 }\n')
+    t = b["desc"]["meshes"][0]["transform"]
+    assert t.shape == (1, 16) and t[0, 12] == 1.0 and t[0, 3] == 0.0
 
 
 def test_include_is_read_at_prerender(tmp_path):
@@ -104,3 +110,34 @@ def test_include_is_read_at_prerender(tmp_path):
     assert (b["width"], b["height"], b["max_iter"]) == (40, 10, 2)
     assert b["camera"]["aspect"] == np.float32(4.0)
     assert b["desc"]["materials"]["has_spec1_strength"].tolist() == [1]
+
+
+def test_instances_and_procs(tmp_path):
+    sub = tmp_path / "sub.vnf"
+    sub.write_text('ShaderStd { Name "c" DiffuseStrength float 1 }\n'
+                   'PolyMesh { Name "k1" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "c" }\n'
+                   'Sphere { Name "k2" P 0 0 1 Radius 0.5 Shader "a" }\n'
+                   'TriLight { Name "dropped" P0 0 2 0 P1 1 2 0 P2 0 2 1 Shader "c" Samples 1 }\n')
+    b = _build(tmp_path, CAM + SH +
+               'PolyMesh { Name "proto" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "a" }\n'
+               'GeomInstance { Name "i" Geom "proto" BMin 1 1 point 3 -1 -1 BMax 1 1 point -1 3 1 '
+               'Transform 2 matrix 1 0 0 1 0 1 0 0 0 0 1 0 0 0 0 1  1 0 0 2 0 1 0 0 0 0 1 0 0 0 0 1 }\n'
+               f'Proc {{ Name "p" Handler "vnf" Data "{sub}" BMin 1 1 point -2 -2 -2 BMax 1 1 point 2 2 2 }}\n')
+    m = b["desc"]["meshes"]
+    assert "idx" in m[0]                                                   # proto
+    inst = m[1]["instance"]
+    assert inst["geom"] == 0 and inst["matrices"].shape == (2, 16) and inst["matrices"][1, 12] == 2.0
+    assert inst["bmin"].tolist() == [3, -1, -1] and inst["bmax"].tolist() == [-1, 3, 1]   # grown at commit
+    assert m[2].get("hidden") and m[3].get("hidden") and m[3]["sphere"]   # the Proc's geoms, in file order
+    assert m[4]["proc"]["geoms"] == [2, 3] and m[4]["proc"]["matrices"] is None
+    assert len(m) == 5 and b["desc"]["lights"] == []                       # the sub-file's TriLight is dropped
+    assert len(b["desc"]["materials"]) == 2                                # "a" and the sub-file's "c"
+    with pytest.raises(oracle_vnf.Unsupported, match="handler"):
+        _build(tmp_path, CAM + SH + 'Proc { Name "p" Handler "wfobj" Data "x.obj" BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 }\n')
+    with pytest.raises(oracle_vnf.Unsupported, match="Unable to find geom"):
+        _build(tmp_path, CAM + SH + 'GeomInstance { Name "i" Geom "none" BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 '
+                                    'Transform 1 matrix 1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1 }\n')
+    with pytest.raises(oracle_vnf.Unsupported, match="follows"):
+        _build(tmp_path, CAM + SH + 'GeomInstance { Name "i" Geom "later" BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 '
+                                    'Transform 1 matrix 1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1 }\n'
+                                    'PolyMesh { Name "later" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "a" }\n')

@@ -13,8 +13,10 @@
 //   - a node missing a required parameter, or repeating one, is dropped, and the parser then skips to
 //     the next '}', which is the end of the following node (the reference does this too);
 //   - more than 10 errors stop the reference (os.Exit); here the load fails.
-// Every node type of the reference is recognised by name.  Types whose rendering this library does
-// not provide (DebugShader, QuadLight, Proc, GeomInstance, textures) fail the build with an error.
+// Every node type of the reference is recognised by name.  What this library does not render fails
+// the build with an error: DebugShader, QuadLight, textures, Proc handlers other than "vnf" (wfobj),
+// instances of instances or procs, an instance placed before its geom, instances and procs inside a
+// Proc's file.
 #include <cerrno>
 #include <cmath>
 #include <cstdint>
@@ -597,7 +599,9 @@ int parse_file(vgpu_vnf* v, const std::string& path, std::vector<std::unique_ptr
     return VGPU_OK;
 }
 
-bool is_geom(const Node& n) { return n.type == "PolyMesh" || n.type == "Sphere"; }
+bool is_geom(const Node& n) {
+    return n.type == "PolyMesh" || n.type == "Sphere" || n.type == "GeomInstance" || n.type == "Proc";
+}
 bool is_light(const Node& n) {
     return n.type == "TriLight" || n.type == "DiskLight" || n.type == "SphereLight" || n.type == "QuadLight";
 }
@@ -611,6 +615,7 @@ struct Build {
     Node* globals = nullptr;
     std::vector<std::unique_ptr<Node>> made;   // nodes created during PreRender (light geometry)
     std::map<Node*, Node*> light_geom;         // light -> the geom its PreRender added
+    std::map<Node*, std::vector<Node*>> proc_geoms;   // Proc -> the geoms its handler made (Proc.Geom)
 
     void add_node(Node* n) {   // core.AddNode (core/core.go:69-89)
         all.push_back(n);
@@ -695,8 +700,46 @@ struct Build {
             for (auto& s : sub) { add_node(s.get()); added.push_back(s.get()); made.push_back(std::move(s)); }
             return VGPU_OK;
         }
-        if (t == "Proc" || t == "GeomInstance")
-            return fail(VGPU_ERR_ARG, "%s:%d: %s nodes are not supported by this library", n->file.c_str(), n->line, t.c_str());
+        if (t == "GeomInstance") {   // Instance.PreRender (builtin/geom/instance/instance.go:116-146)
+            auto it = by_name.find(n->s("Geom"));
+            if (it == by_name.end() || !is_geom(*it->second))
+                return fail(VGPU_ERR_ARG, "%s:%d: Instance %s: Unable to find geom %s", n->file.c_str(), n->line,
+                            n->s("Name").c_str(), n->s("Geom").c_str());
+            return VGPU_OK;
+        }
+        if (t == "Proc") {   // Proc.PreRender (builtin/geom/proc/proc.go:140-172)
+            if (n->s("Handler") != "vnf")
+                return fail(VGPU_ERR_ARG, "%s:%d: Proc %s: handler \"%s\" is not supported by this library (\"vnf\" is)",
+                            n->file.c_str(), n->line, n->s("Name").c_str(), n->s("Handler").c_str());
+            // VNFFile.Init (builtin/geom/proc/vnf/vnf.go:19-56): the file's shaders join the scene's nodes;
+            // its geoms belong to the Proc alone; its other nodes are dropped
+            std::vector<std::unique_ptr<Node>> sub;
+            int rc = parse_file(v, n->s("Data"), sub);
+            if (rc) return rc;
+            std::vector<Node*> subn;
+            for (auto& x : sub) { subn.push_back(x.get()); made.push_back(std::move(x)); }
+            for (Node* x : subn)
+                if (x->type == "ShaderStd" || x->type == "DebugShader") { add_node(x); added.push_back(x); }
+            std::vector<Node*> kids;
+            for (Node* x : subn) {
+                if (x->type == "PolyMesh" || x->type == "Sphere") {
+                    std::vector<std::string> names =
+                        x->type == "PolyMesh" ? x->get("Shader")->strs : std::vector<std::string>{x->s("Shader")};
+                    for (const std::string& sn : names)
+                        if (!shader_node(sn))
+                            return fail(VGPU_ERR_ARG, "%s:%d: Unable to find node (shader %s)", x->file.c_str(), x->line,
+                                        sn.c_str());
+                    kids.push_back(x);
+                } else if (x->type == "GeomInstance" || x->type == "Proc") {
+                    return fail(VGPU_ERR_ARG, "%s:%d: %s inside a Proc's file is not supported by this library",
+                                x->file.c_str(), x->line, x->type.c_str());
+                }
+            }
+            if (kids.empty())
+                return fail(VGPU_ERR_ARG, "%s:%d: GeomProc: %s no geoms.", n->file.c_str(), n->line, n->s("Name").c_str());
+            proc_geoms[n] = kids;
+            return VGPU_OK;
+        }
         if (t == "QuadLight")
             return fail(VGPU_ERR_ARG, "%s:%d: QuadLight cannot render (its SampleArea panics in the reference, "
                         "builtin/light/quad.go:87-95)", n->file.c_str(), n->line);
@@ -778,11 +821,6 @@ int to_material(const Node& s, vgpu_material& m) {
     return VGPU_OK;
 }
 
-bool is_identity(const float* m) {
-    for (int i = 0; i < 16; i++) if (m[i] != ((i % 5 == 0) ? 1.0f : 0.0f)) return false;
-    return true;
-}
-
 // PolyMesh.init (builtin/geom/polymesh/init.go:12-134): fan triangulation of PolyCount polygons,
 // or FaceIdx as triangles, or the vertices in order; shader indices per polygon or per triangle.
 int triangulate(const Node& n, std::vector<uint32_t>& idx, std::vector<uint32_t>& nidx, std::vector<uint8_t>& sidx,
@@ -848,12 +886,6 @@ int add_geom(Build& B, vgpu_scene* sc, Node& n, std::vector<Node*>& mats, int32_
         if (rc) return rc;
         return vgpu_scene_add_sphere(sc, n.get("P")->elems.data(), (float)n.f("Radius"), mat, &geom_out);
     }
-    const Value* tr = n.get("Transform");
-    if (tr && tr->present)
-        for (size_t k = 0; k + 16 <= tr->elems.size(); k += 16)
-            if (!is_identity(&tr->elems[k]))
-                return fail(VGPU_ERR_ARG, "%s:%d: PolyMesh %s: mesh transforms are not supported by this library",
-                            n.file.c_str(), n.line, n.s("Name").c_str());
     const Value* vv = n.get("Verts");
     const int keys = vv->motion_keys < 1 ? 1 : vv->motion_keys;
     const int nverts = vv->per_key;
@@ -925,6 +957,22 @@ int add_geom(Build& B, vgpu_scene* sc, Node& n, std::vector<Node*>& mats, int32_
     }
     if (rc) return rc;
     sc->meshes.back().ray_bias = (float)n.f("RayBias");   // PolyMesh.RayBias (polymesh.go:19)
+    // PolyMesh.Trace moves rays into the mesh's space whenever Transform is given (trace.go:22-53)
+    const Value* tr = n.get("Transform");
+    if (tr && tr->present && tr->elems.size() >= 16)
+        return vgpu_scene_set_transform(sc, geom_out, tr->elems.data(), (uint32_t)(tr->elems.size() / 16));
+    return VGPU_OK;
+}
+
+// GeomInstance and Proc bounds: BMin[0] and BMax[0] (only bounds[0] is used, instance.go:64-69)
+int wrapper_box(const Node& n, const float** lo, const float** hi) {
+    const Value* a = n.get("BMin");
+    const Value* b = n.get("BMax");
+    if (!a || !b || a->elems.size() < 3 || b->elems.size() < 3)
+        return fail(VGPU_ERR_ARG, "%s:%d: %s %s needs BMin and BMax", n.file.c_str(), n.line, n.type.c_str(),
+                    n.s("Name").c_str());
+    *lo = a->elems.data();
+    *hi = b->elems.data();
     return VGPU_OK;
 }
 
@@ -961,7 +1009,42 @@ int build(vgpu_vnf* v, int device, vgpu_scene** scene_out, vgpu_render** render_
     std::map<Node*, int32_t> geom_index;
     for (Node* g : B.geoms) {
         int32_t gi = -1;
-        if ((rc = add_geom(B, sc, *g, mats, gi))) return rc;
+        const float *lo, *hi;
+        if (g->type == "GeomInstance") {
+            Node* tgt = B.by_name[g->s("Geom")];
+            auto it = geom_index.find(tgt);
+            if (it == geom_index.end())
+                return fail(VGPU_ERR_ARG, "%s:%d: GeomInstance %s: its geom %s follows it in the scene (not supported "
+                            "by this library)", g->file.c_str(), g->line, g->s("Name").c_str(), g->s("Geom").c_str());
+            if (tgt->type != "PolyMesh" && tgt->type != "Sphere")
+                return fail(VGPU_ERR_ARG, "%s:%d: GeomInstance %s: instances of instances or procs are not supported "
+                            "by this library", g->file.c_str(), g->line, g->s("Name").c_str());
+            const Value* tr = g->get("Transform");
+            if ((rc = wrapper_box(*g, &lo, &hi))) return rc;
+            if (!tr || tr->elems.size() < 16)
+                return fail(VGPU_ERR_ARG, "%s:%d: GeomInstance %s has no Transform", g->file.c_str(), g->line,
+                            g->s("Name").c_str());
+            if ((rc = vgpu_scene_add_instance(sc, it->second, tr->elems.data(), (uint32_t)(tr->elems.size() / 16), lo,
+                                              hi, &gi)))
+                return rc;
+        } else if (g->type == "Proc") {
+            // the Proc's geoms: in the scene for the Proc to hold, hidden from the top level
+            std::vector<int32_t> kids;
+            for (Node* x : B.proc_geoms[g]) {
+                int32_t k = -1;
+                if ((rc = add_geom(B, sc, *x, mats, k))) return rc;
+                if ((rc = vgpu_scene_set_hidden(sc, k, 1))) return rc;
+                kids.push_back(k);
+            }
+            const Value* tr = g->get("Transform");
+            const bool has_tr = tr && tr->present && tr->elems.size() >= 16;
+            if ((rc = wrapper_box(*g, &lo, &hi))) return rc;
+            if ((rc = vgpu_scene_add_proc(sc, kids.data(), (uint32_t)kids.size(), has_tr ? tr->elems.data() : nullptr,
+                                          has_tr ? (uint32_t)(tr->elems.size() / 16) : 0u, lo, hi, &gi)))
+                return rc;
+        } else if ((rc = add_geom(B, sc, *g, mats, gi))) {
+            return rc;
+        }
         geom_index[g] = gi;
     }
     if ((rc = vgpu_scene_commit(sc))) return rc;

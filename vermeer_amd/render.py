@@ -69,6 +69,14 @@ class Renderer:
         self.width, self.height = width, height
         self._keep = []
         for mesh in desc["meshes"]:
+            if mesh.get("instance") is not None:   # an instance or proc in scene order
+                it = mesh["instance"]
+                self.scene.add_instance(it["geom"], it["matrices"], it["bmin"], it["bmax"])
+                continue
+            if mesh.get("proc") is not None:
+                pr = mesh["proc"]
+                self.scene.add_proc(pr["geoms"], pr.get("matrices"), pr["bmin"], pr["bmax"])
+                continue
             if mesh.get("sphere"):
                 self.scene.add_sphere(mesh["centre"], mesh["radius"], mesh["material"])
                 continue
