@@ -8,6 +8,8 @@
   builtin/geom/polymesh/init.go:12-134   fan triangulation of PolyCount polygons
   builtin/light/{triangle,disk,sphere}.go PreRender: the geometry each light adds
   builtin/camera/camera.go:80-207        Aspect, TanThetaFocal, LocalToWorld keys
+  builtin/geom/instance, proc, proc/vnf  GeomInstance, Proc with the vnf and wfobj handlers
+  builtin/geom/proc/wfobj/*.go           the .obj and .mtl readers
 
 dump() writes the parsed nodes in the same text form as the library's vgpu_vnf_text(VGPU_VNF_NODES),
 so the two parsers can be compared; build() gives a scene description, camera and filter for
@@ -563,6 +565,267 @@ def _triangulate(n):
     return idxp, (nidx if has_n else None), (sidx if sidx else None)
 
 
+def _blank(type_, path, line=1):
+    return {"type": type_, "file": str(path), "line": line,
+            "p": {name: {"kind": k, "req": r, "present": False, "value": None} for name, k, r in SCHEMAS[type_]}}
+
+
+class _ObjLine:
+    """lineScanner (wfobj/mtlparser.go:33-88) over bytes: token() and rest()."""
+
+    def __init__(self, line):
+        self.s, self.pos = line, 0
+
+    def token(self):
+        s, n = self.s, len(self.s)
+        while self.pos < n and s[self.pos] in " \t":
+            self.pos += 1
+        if self.pos >= n or s[self.pos] in "#\0":
+            self.pos = n
+            return ""
+        b = self.pos
+        while self.pos < n and s[self.pos] not in " \t\0":
+            self.pos += 1
+        tok = s[b:self.pos]
+        if self.pos < n:
+            self.pos += 1
+        return tok
+
+    def rest(self):
+        return self.s[self.pos:]
+
+
+def _go_atoi(t):
+    """strconv.Atoi: (ok, value); 0 on a syntax error, the int64 limit out of range."""
+    import re
+    if not re.fullmatch(r"[+-]?\d+", t):
+        return False, 0
+    v = int(t)
+    if v > 2 ** 63 - 1:
+        return False, 2 ** 63 - 1
+    if v < -2 ** 63:
+        return False, -2 ** 63
+    return True, v
+
+
+def _go_float32(t):
+    """strconv.ParseFloat(t, 32): (ok, float32 value); 0 on a syntax error, +-Inf out of range.  The
+    decimal value reaches float32 by way of float64 (ParseFloat rounds once; they differ only for
+    values within 2^-29 relative of a float32 halfway point)."""
+    import re
+    body = t[1:] if t[:1] in "+-" else t
+    low = body.lower()
+    if low in ("inf", "infinity"):
+        return True, np.float32(-np.inf if t.startswith("-") else np.inf)
+    if low == "nan":
+        return (t == body), (np.float32(np.nan) if t == body else np.float32(0))
+    if re.fullmatch(r"0x([0-9a-f]+\.?[0-9a-f]*|\.[0-9a-f]+)p[+-]?\d+", low):
+        v = float.fromhex(t)
+    elif re.fullmatch(r"(\d+\.?\d*|\.\d+)(e[+-]?\d+)?", low):
+        v = float(t)
+    else:
+        return False, np.float32(0)
+    with np.errstate(over="ignore"):
+        f = np.float32(v)
+    return bool(np.isfinite(f)), f
+
+
+def _go_clean(path):
+    """path/filepath.Clean for '/'-separated paths."""
+    if not path:
+        return "."
+    rooted = path.startswith("/")
+    parts = []
+    for e in path.split("/"):
+        if e in ("", "."):
+            continue
+        if e == "..":
+            if parts and parts[-1] != "..":
+                parts.pop()
+            elif not rooted:
+                parts.append("..")
+        else:
+            parts.append(e)
+    out = ("/" if rooted else "") + "/".join(parts)
+    return out or "."
+
+
+def _go_dir(path):
+    k = path.rfind("/")
+    return _go_clean(path[: k + 1] if k >= 0 else "")
+
+
+def _go_join(a, b):
+    return _go_clean(a + "/" + b if a and b else (a or b)) if (a or b) else ""
+
+
+def _set_map(n, k, c):
+    n["p"][k]["value"] = ("c", [float(np.float32(x)) for x in c])
+    n["p"][k]["present"] = True
+
+
+def _parse_mtl(path, dirname):
+    """parseMTL (wfobj/mtlparser.go:93-296): ShaderStd nodes; raises Unsupported where the reference
+    returns an error (its caller then drops the file's shaders) or dereferences nil."""
+    with open(path, "rb") as f:
+        lines = f.read().decode("latin-1").split("\n")
+    out, mtl = [], None
+
+    def need(cmd):
+        if mtl is None:
+            raise Unsupported(f"{cmd} before newmtl")
+
+    for lineno, line in enumerate(lines, 1):
+        if line.endswith("\r"):
+            line = line[:-1]
+        L = _ObjLine(line)
+        cmd = L.token()
+        if cmd == "newmtl":
+            name = L.token()
+            mtl = _blank("ShaderStd", path, lineno)
+            mtl["p"]["Name"]["value"] = name
+            mtl["p"]["Name"]["present"] = True
+            out.append(mtl)
+        elif cmd == "d":
+            ok, d = _go_float32(L.token())
+            if not ok:
+                raise _MtlError(cmd)
+            if d == 0:
+                need(cmd)
+                _set_map(mtl, "DiffuseColour", (0.5, 0.5, 0.5))
+                _set_map(mtl, "DiffuseStrength", (0.5, 0.5, 0.5))
+        elif cmd in ("Ke", "Kd", "Ks"):
+            c = [_go_float32(L.token()) for _ in range(3)]
+            if not c[2][0]:            # only the last value's error is tested (err is reassigned)
+                raise _MtlError(cmd)
+            r, g, b = (float(x[1]) for x in c)
+            if cmd == "Ke":
+                if r == 0 and g == 0 and b == 0:
+                    continue
+                need(cmd)
+                st = math.sqrt(r * r + g * g + b * b)
+                _set_map(mtl, "EmissionColour", (r / st, g / st, b / st))
+                _set_map(mtl, "EmissionStrength", (st, st, st))
+            else:
+                need(cmd)
+                _set_map(mtl, "DiffuseColour" if cmd == "Kd" else "Spec1Colour", (r, g, b))
+        elif cmd == "Ns":
+            ok, ns = _go_float32(L.token())
+            if not ok:
+                raise _MtlError(cmd)
+            need(cmd)
+            ns = np.float32(ns)    # NsToSpecRoughness (mtlparser.go:25-31); m.Max as MAXSS
+            if ns <= np.float32(1000):
+                r = np.float32(1) - ns / np.float32(1000)
+            else:
+                y = np.float32(1) - ns / np.float32(2048)
+                r = np.float32(0) if np.float32(0) > y else y
+            _set_map(mtl, "Spec1Roughness", (r, r, r))
+        elif cmd == "illum":
+            ok, m = _go_atoi(L.token())
+            if not ok:
+                raise _MtlError(cmd)
+            if 0 <= m <= 10:
+                need(cmd)
+                dstr, sstr = {0: (0.5, None), 1: (0.5, None), 4: (0.2, 0.5), 5: (0.2, 0.5)}.get(m, (0.5, 0.5))
+                _set_map(mtl, "DiffuseStrength", (dstr,) * 3)
+                if sstr is not None:
+                    _set_map(mtl, "Spec1Strength", (sstr,) * 3)
+                if m == 5:
+                    mtl["p"]["Spec1FresnelModel"]["value"] = "Metal"
+        elif cmd == "map_Kd":
+            need(cmd)
+            mtl["p"]["DiffuseColour"]["value"] = ("tex", _go_join(dirname, L.rest()))
+            mtl["p"]["DiffuseColour"]["present"] = True
+            _set_map(mtl, "DiffuseStrength", (0.5, 0.5, 0.5))
+    return out
+
+
+class _MtlError(Exception):
+    pass
+
+
+def _parse_obj(path):
+    """File.parse (wfobj/parser.go:103-263) and File.Init (wfobj.go:21-54): (the PolyMesh node named by
+    the file, the ShaderStd nodes of its mtllib files in order)."""
+    with open(path, "rb") as f:
+        lines = f.read().decode("latin-1").split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    mesh = _blank("PolyMesh", path)
+    V, N, UV = [], [], []
+    FI, NI, TI, PC, SI = [], [], [], [], []
+    shaders, seen, mtl = [], set(), 0
+    for line in lines:
+        if line.endswith("\r"):
+            line = line[:-1]
+        L = _ObjLine(line)
+        cmd = L.token()
+        if cmd == "mtllib":
+            mp = _go_join(_go_dir(str(path)), L.rest())
+            if mp in seen:
+                continue
+            try:
+                got = _parse_mtl(mp, _go_dir(str(path)))
+            except (OSError, _MtlError):
+                continue                       # logged by the reference; the file's shaders are dropped
+            shaders += got
+            if len(shaders) > 255:
+                raise Unsupported("wfobj: too many shaders")
+            seen.add(mp)
+        elif cmd == "usemtl":
+            name = L.rest()
+            for i, sh in enumerate(shaders):
+                if sh["p"]["Name"]["value"] == name:
+                    mtl = i
+        elif cmd == "f":                       # parseFace (parser.go:41-101)
+            nv = 0
+            while True:
+                fld = L.token()
+                if fld == "":
+                    break
+                parts = fld.split("/")
+                p = _go_atoi(parts[0])[1]
+                t = _go_atoi(parts[1])[1] if len(parts) > 1 and parts[1] else 0
+                n = _go_atoi(parts[2])[1] if len(parts) > 2 and parts[2] else 0
+                p = len(V) + p + 1 if p < 0 else (1 if p == 0 else p)
+                t = len(UV) + t + 1 if t < 0 else (1 if t == 0 else t)
+                n = len(N) + n + 1 if n < 0 else (1 if n == 0 else n)
+                FI.append(wrap32(p - 1))
+                NI.append(wrap32(n - 1))
+                TI.append(wrap32(t - 1))
+                nv += 1
+            PC.append(nv)
+            SI.append(mtl)
+        elif cmd in ("v", "vn", "vt"):
+            w = 2 if cmd == "vt" else 3
+            (V if cmd == "v" else N if cmd == "vn" else UV).append(
+                [float(_go_float32(L.token())[1]) for _ in range(w)])
+    p = mesh["p"]
+
+    def put(k, v):
+        p[k]["value"] = v
+        p[k]["present"] = True
+
+    put("Name", str(path))
+    put("IsVisible", True)
+    put("Verts", (1, len(V), [x for v in V for x in v]))
+    if N:
+        put("Normals", (1, len(N), [x for v in N for x in v]))
+    if UV:
+        put("UV", (1, len(UV), [x for v in UV for x in v]))
+    if FI:
+        put("FaceIdx", FI)
+        put("PolyCount", PC)
+        put("ShaderIdx", SI)
+        if N:
+            put("NormalIdx", NI)
+        if UV:
+            put("UVIdx", TI)
+    put("Shader", [sh["p"]["Name"]["value"] for sh in shaders])
+    return mesh, shaders
+
+
 def _light_mesh(L):
     """Tri.createMesh (triangle.go:537-567) / Disk.createMesh (disk.go:224-262) as a PolyMesh node."""
     from vermeer_amd import scenes
@@ -644,6 +907,17 @@ def build(path, cwd=None):
                 g = by_name.get(_val(n, "Geom"))
                 if g is None or g["type"] not in ("PolyMesh", "Sphere", "GeomInstance", "Proc"):
                     raise Unsupported(f"Instance: Unable to find geom {_val(n, 'Geom')}")
+                continue
+            if t == "Proc" and _val(n, "Handler") == "wfobj":
+                # File.Init (wfobj.go:21-54): the mesh belongs to the Proc, the .mtl shaders join the scene
+                try:
+                    mesh, shs = _parse_obj(_val(n, "Data"))
+                except OSError:
+                    raise Unsupported(f"wfobj: cannot open {_val(n, 'Data')}")
+                for x in shs:
+                    add(x)
+                    added.append(x)
+                proc_geoms[id(n)] = [mesh]
                 continue
             if t == "Proc":   # Proc.PreRender (builtin/geom/proc/proc.go:140-172) with the "vnf" handler
                 if _val(n, "Handler") != "vnf":

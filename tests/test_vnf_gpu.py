@@ -227,12 +227,84 @@ def test_vnf_instances_procs_transforms_bitexact(tmp_path):
     f.close()
 
 
+def _obj_scene(tmp_path):
+    (tmp_path / "mats.mtl").write_text(
+        "newmtl red\nKd 0.8 0.15 0.1\nillum 1\n"
+        "newmtl shiny\nKd 0.2 0.3 0.6\nKs 0.9 0.9 0.8\nNs 300\nillum 2\n"
+        "newmtl glow\nKe 2 1.8 1.5\nKd 0.5 0.5 0.5\nillum 1\n")
+    # a pyramid on a square base (quads and triangles, corner normals, relative indices), and a small
+    # emissive quad
+    (tmp_path / "thing.obj").write_text(
+        "mtllib mats.mtl\n"
+        "v -0.5 0 -0.5\nv 0.5 0 -0.5\nv 0.5 0 0.5\nv -0.5 0 0.5\nv 0 0.9 0\n"
+        "vn 0 -1 0\nvn 0 0.5 -1\nvn 1 0.5 0\nvn 0 0.5 1\nvn -1 0.5 0\n"
+        "usemtl shiny\nf 1//1 4//1 3//1 2//1\n"
+        "usemtl red\nf 1//2 2//2 5//2\nf 2//3 3//3 5//3\nf -3//4 -2//4 -1//4\nf 4//5 1//5 5//5\n"
+        "v 0.8 0.6 -0.2\nv 1.0 0.6 -0.2\nv 1.0 0.8 -0.2\nv 0.8 0.8 -0.2\n"
+        "usemtl glow\nf -4 -3 -2 -1\n")
+    text = f"""Globals {{ XRes {W} YRes {H} MaxIter {PASSES} }}
+{CAMERA}
+ShaderStd {{ Name "white" DiffuseColour rgb 0.8 0.8 0.8 DiffuseStrength float 1 }}
+ShaderStd {{ Name "light" EmissionColour rgb 1 0.9 0.8 EmissionStrength float 12 DiffuseStrength float 1 }}
+PolyMesh {{
+    Name "floor"
+    Verts 1 4 point -3 0 -3  3 0 -3  3 0 3  -3 0 3
+    PolyCount 1 int 4
+    FaceIdx 4 int 0 3 2 1
+    Shader 1 string "white"
+}}
+Proc {{
+    Name "obj"
+    Handler "wfobj"
+    Data "{tmp_path / 'thing.obj'}"
+    BMin 1 1 point -1.5 -0.5 -1.5
+    BMax 1 1 point 1.5 1.5 1.5
+    Transform 1 matrix 1 0 0 0.1  0 1 0 0  0 0 1 0.3  0 0 0 1
+}}
+TriLight {{ Name "tri" P0 -0.5 2.9 -0.5 P1 0.5 2.9 -0.5 P2 0 2.9 0.5 Shader "light" Samples 1 }}
+"""
+    p = tmp_path / "obj.vnf"
+    p.write_text(text)
+    return p
+
+
+def test_vnf_wfobj_proc_bitexact(tmp_path):
+    """A Proc with the wfobj handler: an .obj mesh (quads and triangles, corner normals, relative
+    indices) with an .mtl of three materials (Kd / Ks / Ns / illum / Ke), under a moved Proc
+    transform; rendered bit-exact against the oracle's reading of the same files."""
+    print("\n[case] vnf wfobj proc", flush=True)
+    path = _obj_scene(tmp_path)
+    f = VnfFile(path)
+    R = f.build(0)
+    oracle.set_normalize_ieee(True)
+    try:
+        built = oracle_vnf.build(str(path))
+        assert len(built["desc"]["materials"]) == 5   # white, light and the three .mtl materials
+        o = oracle_vnf.renderer(built)
+        scr = scenes.pixel_scrambles(W, H, seed=12)
+        for it in range(1, PASSES + 1):
+            print(f"[case] vnf wfobj pass {it}", flush=True)
+            ng, gs = R.render_pass(scr, it, want_sample=True)
+            no, os_ = o.render_pass(scr, it, want_sample=True)
+            assert ng == no, f"pass {it}: shadow rays {ng} vs {no}"
+            assert _nan_equal(gs, os_), f"pass {it}: samples differ"
+        assert _nan_equal(R.film, o.film)
+        rays = scenes.pinhole_rays(W, H, fov_deg=50.0, eye=(0.0, 1.2, 4.0), target=(0.0, 0.5, 0.0))
+        assert (o.scene.trace(rays)[0]["geom"] == 1).sum() > 100   # the Proc is in view
+    finally:
+        oracle.set_normalize_ieee(False)
+    R.close()
+    f.close()
+
+
 def test_vnf_build_errors(tmp_path):
     from vermeer_amd import _lib
     cases = {
         "nocam.vnf": 'ShaderStd { Name "s" }\n',
         "noshader.vnf": 'TriLight { Name "t" P0 0 0 0 P1 1 0 0 P2 0 1 0 Shader "none" Samples 1 }\n',
-        "wfobj.vnf": ('ShaderStd { Name "s" }\nProc { Name "p" Handler "wfobj" Data "x.obj" '
+        "handler.vnf": ('ShaderStd { Name "s" }\nProc { Name "p" Handler "ply" Data "x.ply" '
+                        'BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 }\n'),
+        "noobj.vnf": (CAMERA + 'Proc { Name "p" Handler "wfobj" Data "no/such.obj" '
                       'BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 }\n'),
         "forward.vnf": (CAMERA + 'ShaderStd { Name "s" }\nGeomInstance { Name "i" Geom "m" BMin 1 1 point 0 0 0 '
                         'BMax 1 1 point 1 1 1 Transform 1 matrix 1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1 }\n'

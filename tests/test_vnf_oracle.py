@@ -133,7 +133,7 @@ def test_instances_and_procs(tmp_path):
     assert len(m) == 5 and b["desc"]["lights"] == []                       # the sub-file's TriLight is dropped
     assert len(b["desc"]["materials"]) == 2                                # "a" and the sub-file's "c"
     with pytest.raises(oracle_vnf.Unsupported, match="handler"):
-        _build(tmp_path, CAM + SH + 'Proc { Name "p" Handler "wfobj" Data "x.obj" BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 }\n')
+        _build(tmp_path, CAM + SH + 'Proc { Name "p" Handler "ply" Data "x.ply" BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 }\n')
     with pytest.raises(oracle_vnf.Unsupported, match="Unable to find geom"):
         _build(tmp_path, CAM + SH + 'GeomInstance { Name "i" Geom "none" BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 '
                                     'Transform 1 matrix 1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1 }\n')
@@ -141,3 +141,69 @@ def test_instances_and_procs(tmp_path):
         _build(tmp_path, CAM + SH + 'GeomInstance { Name "i" Geom "later" BMin 1 1 point 0 0 0 BMax 1 1 point 1 1 1 '
                                     'Transform 1 matrix 1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1 }\n'
                                     'PolyMesh { Name "later" Verts 1 3 point 0 0 0 1 0 0 0 1 0 Shader 1 string "a" }\n')
+
+
+def test_wfobj_handler_parses_as_the_reference(tmp_path):
+    """builtin/geom/proc/wfobj/parser.go and mtlparser.go: face fields p, p/t, p//n, p/t/n with
+    negative (relative) and zero indices, a '#' ending a line, usemtl by the rest of the line (last
+    match), a repeated mtllib read once, a missing one skipped, an .mtl whose bad line drops all its
+    shaders, Kd / Ks / Ns / illum / Ke / d 0, and the material strengths illum sets."""
+    (tmp_path / "a.mtl").write_text(
+        "newmtl red\nKd 0.8 0.1 0.1\nillum 1\n"
+        "newmtl shiny\nKs 0.9 0.9 0.9\nNs 250\nillum 5\n"
+        "newmtl lamp\nKe 3 4 0\nd 0\n")
+    (tmp_path / "bad.mtl").write_text("newmtl gone\nKd 1 1 x\n")
+    (tmp_path / "m.obj").write_text(
+        "# comment\nmtllib a.mtl\nmtllib a.mtl\nmtllib missing.mtl\nmtllib bad.mtl\n"
+        "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0.5 0.5 1e0\n"
+        "vn 0 0 1\nvn 0 0 -1\nvt 0 0\nvt 1 0\n"
+        "usemtl shiny\nf 1/1/1 2/2/1 3//1 4 # a quad\n"
+        "usemtl nothing\nf -1 -4 -3\n"
+        "usemtl red\nf 0 2//2 5/-1/-2\n"
+        "f 1 2\n")
+    b = _build(tmp_path, CAM + f'Proc {{ Name "p" Handler "wfobj" Data "{tmp_path / "m.obj"}" '
+                                'BMin 1 1 point -1 -1 -1 BMax 1 1 point 2 2 2 }\n')
+    mesh = b["desc"]["meshes"][0]
+    assert mesh["hidden"] and b["desc"]["meshes"][1]["proc"]["geoms"] == [0]
+    # the quad fans to (0,1,2) (0,2,3); "-1 -4 -3" is (5,2,3) counted from the end -> 0-based 4,1,2;
+    # "0" reads as the first vertex; "f 1 2" is a polygon of two vertices: no triangle
+    assert mesh["idx"].tolist() == [[0, 1, 2], [0, 2, 3], [4, 1, 2], [0, 1, 4]]
+    # usemtl "nothing" keeps the previous material (shiny = 1); "red" selects 0
+    assert mesh["shaderidx"].tolist() == [1, 1, 1, 0]
+    # normals per corner: vn 1 for the quad ("4" has n = 0 -> 1), the second face has none -> 1,
+    # "2//2" -> 2, "5/-1/-2" -> the second-last normal = 1
+    assert mesh["normal_idx"].tolist() == [0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0]
+    mats = b["desc"]["materials"]
+    # materials in the mesh's Shader order, red, shiny, lamp (bad.mtl's shaders are dropped)
+    assert len(mats) == 3
+    red, shiny, lamp = mats[0], mats[1], mats[2]
+    assert lamp["has_emission_colour"] and lamp["emission_strength"] == np.float32(5)
+    assert red["has_diffuse_colour"] and np.allclose(red["diffuse"], [0.8, 0.1, 0.1])
+    assert red["has_diffuse_strength"] and red["diffuse_strength"] == np.float32(0.5) and not red["has_spec1_strength"]
+    assert shiny["spec1_roughness"] == np.float32(1) - np.float32(250) / np.float32(1000)
+    assert shiny["diffuse_strength"] == np.float32(0.2) and shiny["spec1_strength"] == np.float32(0.5)
+    assert shiny["fresnel_model"] == 2                                   # illum 5: "Metal"
+
+
+def test_wfobj_mtl_values_and_errors(tmp_path):
+    from oracle_vnf import _parse_mtl, _parse_obj, _MtlError, _go_float32, _go_atoi
+    (tmp_path / "c.mtl").write_text("newmtl lamp\nKe 3 4 0\nd 0\nNs 4096\nillum 12\nnewmtl tex\nmap_Kd  a b.png\n")
+    lamp, tex = _parse_mtl(str(tmp_path / "c.mtl"), str(tmp_path))
+    p = lamp["p"]
+    assert p["EmissionColour"]["value"] == ("c", [0.6000000238418579, 0.800000011920929, 0.0])
+    assert p["EmissionStrength"]["value"][1][0] == 5.0
+    assert p["DiffuseColour"]["value"][1] == [0.5, 0.5, 0.5]            # d 0: 'dissolve'
+    assert p["Spec1Roughness"]["value"][1][0] == 0.0                      # Max(0, 1 - 4096/2048)
+    assert p["DiffuseStrength"]["value"][1][0] == 0.5                     # from d 0; illum 12 sets nothing
+    # map_Kd: the rest of the line after one separator, joined to the directory
+    assert tex["p"]["DiffuseColour"]["value"] == ("tex", str(tmp_path / " a b.png"))
+    (tmp_path / "e.mtl").write_text("Kd 1 1 1\n")
+    with pytest.raises(oracle_vnf.Unsupported):                          # a nil dereference in Go
+        _parse_mtl(str(tmp_path / "e.mtl"), str(tmp_path))
+    (tmp_path / "f.mtl").write_text("newmtl a\nKd x 1 1\nillum q\n")
+    with pytest.raises(_MtlError):                                         # Kd's first error is ignored,
+        _parse_mtl(str(tmp_path / "f.mtl"), str(tmp_path))                # illum's ends the file
+    assert _go_float32("1e") == (False, 0) and _go_float32("0x1p-2") == (True, 0.25)
+    assert _go_float32("1e39")[0] is False and np.isinf(_go_float32("1e39")[1])
+    assert _go_float32("+nan")[0] is False and _go_float32("-Inf")[1] == -np.inf
+    assert _go_atoi("+7") == (True, 7) and _go_atoi("1.5") == (False, 0)

@@ -14,10 +14,11 @@
 //     the next '}', which is the end of the following node (the reference does this too);
 //   - more than 10 errors stop the reference (os.Exit); here the load fails.
 // Every node type of the reference is recognised by name.  What this library does not render fails
-// the build with an error: DebugShader, QuadLight, textures, Proc handlers other than "vnf" (wfobj),
-// instances of instances or procs, an instance placed before its geom, instances and procs inside a
-// Proc's file.
+// the build with an error: DebugShader, QuadLight, textures, instances of instances or procs, an
+// instance placed before its geom, instances and procs inside a Proc's file.
+#include <cctype>
 #include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -599,6 +600,305 @@ int parse_file(vgpu_vnf* v, const std::string& path, std::vector<std::unique_ptr
     return VGPU_OK;
 }
 
+// ------------------------------------------------------------------ Wavefront .obj / .mtl (the wfobj Proc handler)
+//
+// builtin/geom/proc/wfobj/parser.go and mtlparser.go: one PolyMesh with a polygon per "f" line and a
+// ShaderStd per "newmtl".  Lines are split as lineScanner does: a token is a run of bytes other than
+// space and tab, '#' before a token ends the line, Rest() is what follows the command's first
+// separator.  Bytes are read as bytes (the reference decodes UTF-8 and drops invalid bytes).
+
+struct ObjLine {
+    std::string s;
+    size_t pos = 0;
+    std::string token() {
+        while (pos < s.size() && (s[pos] == ' ' || s[pos] == '\t')) pos++;
+        if (pos >= s.size() || s[pos] == '#' || s[pos] == '\0') { pos = s.size(); return std::string(); }
+        const size_t b = pos;
+        while (pos < s.size() && s[pos] != ' ' && s[pos] != '\t' && s[pos] != '\0') pos++;
+        std::string t = s.substr(b, pos - b);
+        if (pos < s.size()) pos++;   // the separator is consumed with the token
+        return t;
+    }
+    std::string rest() const { return pos < s.size() ? s.substr(pos) : std::string(); }
+};
+
+// strconv.Atoi: [+-]?digits; 0 (with an error) otherwise; out of range: the int64 limit.
+bool go_atoi(const std::string& t, int64_t& out) {
+    out = 0;
+    size_t i = 0;
+    bool neg = false;
+    if (i < t.size() && (t[i] == '+' || t[i] == '-')) neg = t[i++] == '-';
+    if (i >= t.size()) return false;
+    uint64_t v = 0;
+    bool over = false;
+    for (; i < t.size(); i++) {
+        if (t[i] < '0' || t[i] > '9') return false;
+        const uint64_t d = (uint64_t)(t[i] - '0');
+        if (v > (UINT64_MAX - d) / 10) over = true;
+        else v = v * 10 + d;
+    }
+    const uint64_t lim = neg ? (uint64_t)INT64_MAX + 1 : (uint64_t)INT64_MAX;
+    if (over || v > lim) { out = neg ? INT64_MIN : INT64_MAX; return false; }
+    out = neg ? (int64_t)(0 - v) : (int64_t)v;
+    return true;
+}
+
+// strconv.ParseFloat(t, 32): decimal, hexadecimal with a p exponent, inf / infinity / nan; the
+// float32 nearest (correctly rounded) as a double; a syntax error gives 0, overflow +-Inf.
+bool go_parse_float32(const std::string& t, double& out) {
+    out = 0.0;
+    size_t i = 0;
+    if (i < t.size() && (t[i] == '+' || t[i] == '-')) i++;
+    std::string body = t.substr(i);
+    std::string low = body;
+    for (char& c : low) c = (char)std::tolower((unsigned char)c);
+    bool ok = false;
+    if (low == "inf" || low == "infinity") ok = true;
+    else if (low == "nan") ok = i == 0;
+    else if (low.size() > 2 && low[0] == '0' && low[1] == 'x') {
+        size_t k = 2, nd = 0;
+        while (k < low.size() && std::isxdigit((unsigned char)low[k])) { k++; nd++; }
+        if (k < low.size() && low[k] == '.') { k++; while (k < low.size() && std::isxdigit((unsigned char)low[k])) { k++; nd++; } }
+        if (nd > 0 && k < low.size() && low[k] == 'p') {
+            k++;
+            if (k < low.size() && (low[k] == '+' || low[k] == '-')) k++;
+            size_t ne = 0;
+            while (k < low.size() && std::isdigit((unsigned char)low[k])) { k++; ne++; }
+            ok = ne > 0 && k == low.size();
+        }
+    } else {
+        size_t k = 0, nd = 0;
+        while (k < low.size() && std::isdigit((unsigned char)low[k])) { k++; nd++; }
+        if (k < low.size() && low[k] == '.') { k++; while (k < low.size() && std::isdigit((unsigned char)low[k])) { k++; nd++; } }
+        if (nd > 0 && k < low.size() && low[k] == 'e') {
+            size_t k0 = k++;
+            if (k < low.size() && (low[k] == '+' || low[k] == '-')) k++;
+            size_t ne = 0;
+            while (k < low.size() && std::isdigit((unsigned char)low[k])) { k++; ne++; }
+            if (ne == 0) k = k0;
+        }
+        ok = nd > 0 && k == low.size();
+    }
+    if (!ok) return false;
+    out = (double)std::strtof(t.c_str(), nullptr);
+    return !std::isinf(out) || low.rfind("inf", 0) == 0;
+}
+
+// path/filepath.Clean, Dir and Join for '/'-separated paths.
+std::string go_clean(const std::string& path) {
+    if (path.empty()) return ".";
+    const bool rooted = path[0] == '/';
+    std::vector<std::string> parts;
+    size_t i = 0;
+    while (i <= path.size()) {
+        size_t j = path.find('/', i);
+        if (j == std::string::npos) j = path.size();
+        const std::string e = path.substr(i, j - i);
+        if (e.empty() || e == ".") {
+        } else if (e == "..") {
+            if (!parts.empty() && parts.back() != "..") parts.pop_back();
+            else if (!rooted) parts.push_back("..");
+        } else {
+            parts.push_back(e);
+        }
+        i = j + 1;
+    }
+    std::string out = rooted ? "/" : "";
+    for (size_t k = 0; k < parts.size(); k++) out += (k ? "/" : "") + parts[k];
+    return out.empty() ? "." : out;
+}
+std::string go_dir(const std::string& path) {
+    const size_t k = path.rfind('/');
+    return go_clean(k == std::string::npos ? std::string() : path.substr(0, k + 1));
+}
+std::string go_join(const std::string& a, const std::string& b) {
+    if (a.empty()) return b.empty() ? std::string() : go_clean(b);
+    return go_clean(b.empty() ? a : a + "/" + b);
+}
+
+std::unique_ptr<Node> new_node(const char* type, const std::string& file) {
+    auto n = std::make_unique<Node>();
+    n->type = type;
+    for (const FieldDef& fd : find_type(type)->fields) n->p[fd.name].kind = fd.kind;
+    n->file = file;
+    return n;
+}
+void set_map(Node& n, const char* k, float r, float g, float b) {
+    Value& v = n.p[k];
+    v.present = v.map_set = true;
+    v.map_tex = false;
+    v.c[0] = r; v.c[1] = g; v.c[2] = b;
+}
+
+// parseMTL (mtlparser.go:93-296): the ShaderStd nodes of one .mtl file; an error ends the file and
+// the reference then drops every shader of it.
+int parse_mtl(const std::string& path, const std::string& dir, std::vector<std::unique_ptr<Node>>& out, std::string& err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { err = "cannot open " + path; return -1; }
+    std::string raw;
+    Node* mtl = nullptr;
+    int lineno = 0;
+    while (std::getline(f, raw)) {
+        lineno++;
+        if (!raw.empty() && raw.back() == '\r') raw.pop_back();
+        ObjLine L{raw};
+        const std::string cmd = L.token();
+        auto need = [&]() -> bool {
+            if (mtl) return true;
+            err = path + ":" + std::to_string(lineno) + ": " + cmd + " before newmtl (the reference dereferences nil)";
+            return false;
+        };
+        auto three = [&](double* c) -> bool {   // only the last value's error is tested (err is reassigned)
+            double x;
+            go_parse_float32(L.token(), x); c[0] = x;
+            go_parse_float32(L.token(), x); c[1] = x;
+            const bool ok = go_parse_float32(L.token(), x);
+            c[2] = x;
+            if (!ok) err = path + ":" + std::to_string(lineno) + ": bad " + cmd;
+            return ok;
+        };
+        if (cmd == "newmtl") {
+            auto n = new_node("ShaderStd", path);
+            n->line = lineno;
+            n->p["Name"].s = L.token();
+            n->p["Name"].present = true;
+            mtl = n.get();
+            out.push_back(std::move(n));
+        } else if (cmd == "d") {
+            double d;
+            if (!go_parse_float32(L.token(), d)) { err = path + ": bad d"; return -1; }
+            if (d == 0.0) {   // 'dissolve'
+                if (!need()) return -1;
+                set_map(*mtl, "DiffuseColour", 0.5f, 0.5f, 0.5f);
+                set_map(*mtl, "DiffuseStrength", 0.5f, 0, 0);
+            }
+        } else if (cmd == "Ke") {
+            double c[3];
+            if (!three(c)) return -1;
+            if (c[0] == 0.0 && c[1] == 0.0 && c[2] == 0.0) continue;
+            if (!need()) return -1;
+            const double st = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            set_map(*mtl, "EmissionColour", (float)(c[0] / st), (float)(c[1] / st), (float)(c[2] / st));
+            set_map(*mtl, "EmissionStrength", (float)st, 0, 0);
+        } else if (cmd == "Kd" || cmd == "Ks") {
+            double c[3];
+            if (!three(c)) return -1;
+            if (!need()) return -1;
+            set_map(*mtl, cmd == "Kd" ? "DiffuseColour" : "Spec1Colour", (float)c[0], (float)c[1], (float)c[2]);
+        } else if (cmd == "Ns") {
+            double ns;
+            if (!go_parse_float32(L.token(), ns)) { err = path + ": bad Ns"; return -1; }
+            if (!need()) return -1;
+            const float x = (float)ns;   // NsToSpecRoughness (mtlparser.go:25-31), m.Max as MAXSS
+            const float y = 1.0f - x / 2048.0f;
+            const float r = x <= 1000.0f ? 1.0f - x / 1000.0f : ((0.0f > y) ? 0.0f : y);
+            set_map(*mtl, "Spec1Roughness", r, 0, 0);
+        } else if (cmd == "illum") {
+            int64_t m;
+            if (!go_atoi(L.token(), m)) { err = path + ": bad illum"; return -1; }
+            if (m >= 0 && m <= 10 && !need()) return -1;
+            // the strengths per model (mtlparser.go:204-254)
+            if (m == 0 || m == 1) set_map(*mtl, "DiffuseStrength", 0.5f, 0, 0);
+            else if (m == 4 || m == 5) { set_map(*mtl, "DiffuseStrength", 0.2f, 0, 0); set_map(*mtl, "Spec1Strength", 0.5f, 0, 0); }
+            else if (m >= 2 && m <= 10) { set_map(*mtl, "DiffuseStrength", 0.5f, 0, 0); set_map(*mtl, "Spec1Strength", 0.5f, 0, 0); }
+            if (m == 5) { mtl->p["Spec1FresnelModel"].s = "Metal"; mtl->p["Spec1FresnelModel"].present = true; }
+        } else if (cmd == "map_Kd") {
+            if (!need()) return -1;
+            Value& v = mtl->p["DiffuseColour"];
+            v.present = v.map_set = v.map_tex = true;
+            v.s = go_join(dir, L.rest());
+            set_map(*mtl, "DiffuseStrength", 0.5f, 0, 0);
+        }
+    }
+    return 0;
+}
+
+// File.parse (parser.go:103-263) and the Init that follows it (wfobj.go:21-54): the mesh (named by the
+// file) and the shaders of its mtllib files, in order.
+int parse_obj(vgpu_vnf* v, const std::string& filename, std::unique_ptr<Node>& mesh,
+              std::vector<std::unique_ptr<Node>>& shaders) {
+    std::ifstream f(filename, std::ios::binary);
+    if (!f) return fail(VGPU_ERR_ARG, "wfobj: cannot open %s", filename.c_str());
+    mesh = new_node("PolyMesh", filename);
+    Node& m = *mesh;
+    m.line = 1;
+    m.p["Name"].s = filename;
+    m.p["IsVisible"].b = true;
+    Value& V = m.p["Verts"];
+    Value& N = m.p["Normals"];
+    Value& UV = m.p["UV"];
+    Value &FI = m.p["FaceIdx"], &NI = m.p["NormalIdx"], &TI = m.p["UVIdx"], &PC = m.p["PolyCount"], &SI = m.p["ShaderIdx"];
+    V.motion_keys = N.motion_keys = UV.motion_keys = 1;
+    std::map<std::string, bool> mtllibs;
+    uint8_t mtl = 0;
+    std::string raw;
+    while (std::getline(f, raw)) {
+        if (!raw.empty() && raw.back() == '\r') raw.pop_back();
+        ObjLine L{raw};
+        const std::string cmd = L.token();
+        if (cmd == "mtllib") {
+            const std::string path = go_join(go_dir(filename), L.rest());
+            if (mtllibs.count(path)) continue;
+            std::vector<std::unique_ptr<Node>> got;
+            std::string err;
+            if (parse_mtl(path, go_dir(filename), got, err)) {
+                v->log += "wfobj.parse: " + err + "\n";
+                continue;
+            }
+            for (auto& s : got) shaders.push_back(std::move(s));
+            if (shaders.size() > 255) return fail(VGPU_ERR_ARG, "wfobj.parse: too many shaders (%s)", filename.c_str());
+            mtllibs[path] = true;
+        } else if (cmd == "usemtl") {
+            const std::string name = L.rest();
+            for (size_t i = 0; i < shaders.size(); i++)
+                if (shaders[i]->s("Name") == name) mtl = (uint8_t)i;
+        } else if (cmd == "f") {   // parseFace (parser.go:41-101)
+            int32_t nv = 0;
+            for (std::string fld = L.token(); !fld.empty(); fld = L.token()) {
+                std::vector<std::string> parts;
+                size_t b = 0;
+                for (;;) {
+                    const size_t e = fld.find('/', b);
+                    parts.push_back(fld.substr(b, e == std::string::npos ? std::string::npos : e - b));
+                    if (e == std::string::npos) break;
+                    b = e + 1;
+                }
+                int64_t p = 0, t = 0, n = 0;
+                go_atoi(parts[0], p);
+                if (parts.size() > 1 && !parts[1].empty()) go_atoi(parts[1], t);
+                if (parts.size() > 2 && !parts[2].empty()) go_atoi(parts[2], n);
+                if (p < 0) p = (int64_t)(V.elems.size() / 3) + p + 1; else if (p == 0) p = 1;
+                if (t < 0) t = (int64_t)(UV.elems.size() / 2) + t + 1; else if (t == 0) t = 1;
+                if (n < 0) n = (int64_t)(N.elems.size() / 3) + n + 1; else if (n == 0) n = 1;
+                FI.ints.push_back((int32_t)(p - 1));
+                NI.ints.push_back((int32_t)(n - 1));
+                TI.ints.push_back((int32_t)(t - 1));
+                nv++;
+            }
+            PC.ints.push_back(nv);
+            SI.ints.push_back((int32_t)mtl);
+            FI.slice_set = NI.slice_set = TI.slice_set = PC.slice_set = SI.slice_set = true;
+        } else if (cmd == "v" || cmd == "vn" || cmd == "vt") {
+            Value& dst = cmd == "v" ? V : (cmd == "vn" ? N : UV);
+            const int w = cmd == "vt" ? 2 : 3;
+            for (int k = 0; k < w; k++) {
+                double x;
+                go_parse_float32(L.token(), x);   // an error is logged and the value (0, or +-Inf) kept
+                dst.elems.push_back((float)x);
+            }
+            dst.per_key++;
+            dst.present = true;
+        }
+    }
+    // no normals (or UVs): their indices are dropped (parser.go:252-260)
+    if (N.elems.empty()) { NI.ints.clear(); NI.slice_set = false; }
+    if (UV.elems.empty()) { TI.ints.clear(); TI.slice_set = false; }
+    Value& sh = m.p["Shader"];
+    sh.slice_set = sh.present = true;
+    for (auto& s : shaders) sh.strs.push_back(s->s("Name"));
+    return VGPU_OK;
+}
+
 bool is_geom(const Node& n) {
     return n.type == "PolyMesh" || n.type == "Sphere" || n.type == "GeomInstance" || n.type == "Proc";
 }
@@ -707,10 +1007,27 @@ struct Build {
                             n->s("Name").c_str(), n->s("Geom").c_str());
             return VGPU_OK;
         }
+        if (t == "Proc" && n->s("Handler") == "wfobj") {
+            // File.Init (builtin/geom/proc/wfobj/wfobj.go:21-54): the .obj file's mesh belongs to the Proc;
+            // the shaders of its .mtl files join the scene's nodes
+            std::unique_ptr<Node> mesh;
+            std::vector<std::unique_ptr<Node>> shs;
+            int rc = parse_obj(v, n->s("Data"), mesh, shs);
+            if (rc) return rc;
+            for (auto& x : shs) {
+                Node* xp = x.get();
+                made.push_back(std::move(x));
+                add_node(xp);
+                added.push_back(xp);
+            }
+            proc_geoms[n] = {mesh.get()};
+            made.push_back(std::move(mesh));
+            return VGPU_OK;
+        }
         if (t == "Proc") {   // Proc.PreRender (builtin/geom/proc/proc.go:140-172)
             if (n->s("Handler") != "vnf")
-                return fail(VGPU_ERR_ARG, "%s:%d: Proc %s: handler \"%s\" is not supported by this library (\"vnf\" is)",
-                            n->file.c_str(), n->line, n->s("Name").c_str(), n->s("Handler").c_str());
+                return fail(VGPU_ERR_ARG, "%s:%d: Proc %s: Unable to find handler %s", n->file.c_str(), n->line,
+                            n->s("Name").c_str(), n->s("Handler").c_str());
             // VNFFile.Init (builtin/geom/proc/vnf/vnf.go:19-56): the file's shaders join the scene's nodes;
             // its geoms belong to the Proc alone; its other nodes are dropped
             std::vector<std::unique_ptr<Node>> sub;
