@@ -14,15 +14,17 @@
 //   output   8 lobes of (u[3], kappa_raw, logit):
 //            mu = u / |u|, kappa = exp(clamp(kappa_raw, -5, 7)), w = softmax(logit)
 //   density  q(d) = sum_k w_k kappa_k / (2 pi (1 - exp(-2 kappa_k))) exp(kappa_k (mu_k . d - 1))
-//   loss     L = (1/B) sum_i -(c_i / cbar) log q(d_i), cbar = (1/B) sum_i c_i
+//   loss     L = (1/B) sum_i -(c_i / cbar) log q(d_i), cbar = (1/B) sum_i c_i,
+//            i.e. sum_i c_i (-log q(d_i)) / sum_i c_i
 //   update   Adam (Kingma & Ba 2015, Algorithm 1)
 // Parameters are packed W1 (64x144, row = output), b1, W2, b2, W3, b3, W4, b4,
 // W5 (40x64), b5: 24,360 floats.
 //
 // emulate = 1 rounds (to bf16, nearest even) what the GPU kernels round: the
 // features, the weights and the hidden activations before each product in the
-// forward pass, and the back-propagated gradients before each product in the
-// backward pass.  All sums are in double.
+// forward pass, and the back-propagated gradients (sample i's scaled by c_i,
+// not by c_i / cbar) before each product in the backward pass.  All sums are
+// in double.
 
 #include <cmath>
 #include <cstdint>
@@ -248,7 +250,9 @@ void ogd_nll(const double* raw, int64_t n, const float* dirs, double* out) {
     }
 }
 
-// Batch loss and gradient (mean over the batch).  Returns 0, or 1 when sum c = 0 (no step).
+// Batch loss sum_i c_i (-log q_i) / sum_i c_i and its gradient.  Each sample's output gradient is
+// weighted by its own c_i (the GPU rounds it to bf16 at that scale) and the sums are divided by
+// sum c_i at the end.  Returns 0, or 1 when sum c = 0 (no step).
 int ogd_loss_grad(const float* params, int64_t n, const float* pos, const float* nrm, const float* dirs,
                   const float* c, const float* bounds, int emulate, double* loss_out, double* grad_out) {
     double csum = 0.0;
@@ -256,7 +260,6 @@ int ogd_loss_grad(const float* params, int64_t n, const float* pos, const float*
     for (int p = 0; p < og::NPARAM; p++) grad_out[p] = 0.0;
     *loss_out = 0.0;
     if (!(csum > 0.0)) return 1;
-    const double cbar = csum / (double)n;
     const bool em = emulate != 0;
     int nt = 1;
 #ifdef _OPENMP
@@ -276,7 +279,7 @@ int ogd_loss_grad(const float* params, int64_t n, const float* pos, const float*
         og::Acts a;
         og::forward(params, pn, em, a);
         const double d[3] = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
-        const double s = (double)c[i] / cbar;
+        const double s = (double)c[i];
         og::Lobes L;
         og::decode(a.raw, L);
         lt[t] += -s * std::log(og::mix_pdf(L, d));
@@ -288,8 +291,8 @@ int ogd_loss_grad(const float* params, int64_t n, const float* pos, const float*
         *loss_out += lt[t];
         for (int p = 0; p < og::NPARAM; p++) grad_out[p] += Gt[t][p];
     }
-    *loss_out /= (double)n;
-    for (int p = 0; p < og::NPARAM; p++) grad_out[p] /= (double)n;
+    *loss_out /= csum;
+    for (int p = 0; p < og::NPARAM; p++) grad_out[p] /= csum;
     return 0;
 }
 

@@ -51,7 +51,7 @@ int upload_params(vgpu_guide* g, const float* P) {
     HIP_TRY(hipMemcpy(g->d_params, P, sizeof(float) * G::NPARAM, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(g->d_m, 0, sizeof(float) * G::NPARAM));
     HIP_TRY(hipMemset(g->d_v, 0, sizeof(float) * G::NPARAM));
-    HIP_TRY(hipMemset(g->tb.step, 0, sizeof(int32_t)));
+    HIP_TRY(hipMemset(g->tb.step, 0, 2 * sizeof(int32_t)));   // step count and its copy
     HIP_TRY(G::launch_pack(g->d_params, g->d_frag_fwd, g->d_frag_bwd, nullptr));
     HIP_TRY(hipDeviceSynchronize());
     return VGPU_OK;
@@ -90,7 +90,7 @@ int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_
     hipError_t e = hipGetDeviceProperties(&prop, device);
     g->num_cus = prop.multiProcessorCount;
     const size_t P4 = sizeof(float) * G::NPARAM;
-    const uint32_t nsplit = max_batch / G::KSPLIT;
+    const uint32_t nwg = max_batch / G::TRAIN_WG;
     if (e == hipSuccess) e = hipMalloc(&g->d_params, P4);
     if (e == hipSuccess) e = hipMalloc(&g->d_m, P4);
     if (e == hipSuccess) e = hipMalloc(&g->d_v, P4);
@@ -98,7 +98,7 @@ int vgpu_guide_create(vgpu_guide** out, int device, const float* bounds, uint64_
     if (e == hipSuccess) e = hipMalloc(&g->d_frag_bwd, (size_t)G::NF_BWD * 64 * 16);
     if (e == hipSuccess) e = hipMalloc(&g->tb.partial, (size_t)(max_batch / G::TRAIN_WG) * P4);
     if (e == hipSuccess) e = hipMalloc(&g->tb.loss, sizeof(float) * max_batch);
-    if (e == hipSuccess) e = hipMalloc(&g->tb.csum, sizeof(float) * (1 + nsplit) + 256);
+    if (e == hipSuccess) e = hipMalloc(&g->tb.csum, sizeof(float) * (1 + nwg));
     if (e == hipSuccess) e = hipMalloc(&g->tb.step, 256);
     if (e == hipSuccess) e = hipMalloc(&g->tb.grad, P4);
     if (e == hipSuccess) e = hipMalloc(&g->tb.slot_fwd, sizeof(int32_t) * G::NPARAM);
@@ -187,12 +187,15 @@ int vgpu_guide_train_step(vgpu_guide* g, uint32_t flags, const float* samples, u
                             (flags & VGPU_GUIDE_NO_UPDATE) == 0, g->tb, st));
     if (host) {
         if (loss_out) {
+            // per-sample c_i (-log q_i); the batch loss divides their sum by sum c_i
             std::vector<float> L(n);
+            float cs = 0.0f;
             HIP_TRY(hipMemcpyAsync(L.data(), g->tb.loss, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(&cs, g->tb.csum, sizeof(float), hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
             double s = 0.0;
             for (float v : L) s += v;
-            *loss_out = (float)(s / n);
+            *loss_out = cs > 0.0f ? (float)(s / cs) : 0.0f;
         }
         HIP_TRY(hipStreamSynchronize(st));
     }

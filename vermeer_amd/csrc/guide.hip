@@ -12,12 +12,13 @@
 // Weight fragments are packed once per parameter update (k_pack) and staged
 // into LDS by every workgroup (50 KB forward, +32 KB transposed for backward).
 //
-// Training (one step over B samples): k_csum_part sums the sample weights c per split of
-// KSPLIT samples and k_train adds the split sums; k_train runs forward, the loss gradient and
-// the backward pass in registers, and after each backward layer stages that layer's input X and
-// output gradient dY for its 256 samples in LDS and forms dW = dY X^T on the matrix cores (samples
-// as the k dimension) into one fp32 partial per workgroup; k_adam reduces the partials in a fixed
-// order, applies Adam and writes each updated weight's bf16 copy straight into its fragment slots.
+// Training (one step over B samples, two launches): k_train runs forward, the loss gradient and
+// the backward pass in registers, with each sample's loss weighted by its own c; after each
+// backward layer it stages that layer's input X and output gradient dY for its 256 samples in LDS
+// and forms dW = dY X^T on the matrix cores (samples as the k dimension) into one fp32 partial per
+// workgroup, beside the workgroup's sum of c.  k_adam reduces the partials and the c sums in a
+// fixed order, divides by the batch's c sum (the loss is sum c_i (-log q_i) / sum c_i), applies
+// Adam and writes each updated weight's bf16 copy straight into its fragment slots.
 
 #include <atomic>
 
@@ -275,6 +276,29 @@ __device__ __forceinline__ float quad_sum(float v) {
     return v + __shfl_xor(v, 32);
 }
 
+// decode_lobe, log_vmf_norm and dlog_norm for the training kernel's loss gradient, on the hardware
+// exp2 / log2 / reciprocal / square-root approximations (about 1 ulp; the gradient is compared
+// with the FP64 oracle at a relative tolerance far above that).
+__device__ __forceinline__ void decode_lobe_fast(const float* v, LobeD& L) {
+    const float len = __builtin_amdgcn_sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    L.ulen = len;
+    if (len > 0.0f) {
+        const float inv = __builtin_amdgcn_rcpf(len);
+        L.mu[0] = v[0] * inv; L.mu[1] = v[1] * inv; L.mu[2] = v[2] * inv;
+    } else {
+        L.mu[0] = 0.0f; L.mu[1] = 0.0f; L.mu[2] = 1.0f;
+    }
+    L.kin = v[3] > KAPPA_RAW_MIN && v[3] < KAPPA_RAW_MAX;
+    L.kappa = __expf(fminf(fmaxf(v[3], KAPPA_RAW_MIN), KAPPA_RAW_MAX));
+    L.logit = v[4];
+}
+__device__ __forceinline__ float log_vmf_norm_fast(float k) {
+    return __logf(k) - 1.8378770664093453f - __logf(-expm1f(-2.0f * k));
+}
+__device__ __forceinline__ float dlog_norm_fast(float k) {
+    if (k < 0.05f) return 1.0f - k * (1.0f / 3.0f) + k * k * k * (1.0f / 45.0f);
+    return __builtin_amdgcn_rcpf(k) - 2.0f * __builtin_amdgcn_rcpf(expm1f(2.0f * k));
+}
 // log of the vMF normalisation kappa / (2 pi (1 - exp(-2 kappa)))
 __device__ __forceinline__ float log_vmf_norm(float k) {
     return logf(k) - 1.8378770664093453f - logf(-expm1f(-2.0f * k));
@@ -355,8 +379,6 @@ __global__ __launch_bounds__(256) void k_infer(Net net, const float4* __restrict
 
 // ---------------------------------------------------------------- training
 
-// Sum of the sample weights in two deterministic stages: one partial per KSPLIT samples, then the
-// partials in order (the partial area is the tail of the weight-gradient partial buffer's first row).
 __device__ __forceinline__ uint32_t lowbias32(uint32_t x) {   // C. Wellons' integer hash
     x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
     return x;
@@ -372,25 +394,11 @@ __device__ __forceinline__ TrainSample fetch_sample(const SampleSource& src, uin
     return t;
 }
 
-__global__ __launch_bounds__(256) void k_csum_part(SampleSource src, float* __restrict__ part) {
-    __shared__ float red[256];
-    const uint32_t base = blockIdx.x * KSPLIT;
+// Sum of v[0 .. n): lane l adds v[l], v[l + 64], ... and the 64 lane sums are added pairwise across
+// the wavefront, the same order in every wavefront that calls it.
+__device__ __forceinline__ float wave_sum_fixed(const float* v, uint32_t n, int lane) {
     float a = 0.0f;
-    for (uint32_t i = threadIdx.x; i < (uint32_t)KSPLIT; i += 256) a += fetch_sample(src, base + i).c;
-    red[threadIdx.x] = a;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-// Sum of the split sums part[0 .. nsplit): lane l adds part[l], part[l + 64], ... and the 64 lane sums
-// are added pairwise across the wavefront, the same order in every workgroup that calls it.
-__device__ __forceinline__ float batch_csum(const float* __restrict__ part, uint32_t nsplit, int lane) {
-    float a = 0.0f;
-    for (uint32_t i = (uint32_t)lane; i < nsplit; i += 64) a += part[i];
+    for (uint32_t i = (uint32_t)lane; i < n; i += 64) a += v[i];
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
     return a;
@@ -443,7 +451,8 @@ static_assert(TRAIN_CHUNKS * TRAIN_CHUNK == TRAIN_WG, "chunks tile the workgroup
 constexpr int STG = TRAIN_CHUNK + 8;           // staged row stride, bf16 (272 bytes)
 constexpr size_t FWD_BYTES = (size_t)NF_FWD * 64 * 16, BWD_BYTES = (size_t)NF_BWD * 64 * 16;
 constexpr size_t STAGE_BYTES = (size_t)(NENC + NH) * STG * 2;   // X1 (144 rows) and dY1 (64 rows): the largest
-constexpr size_t TRAIN_LDS = FWD_BYTES + BWD_BYTES + STAGE_BYTES;
+constexpr size_t SAMPLE_BYTES = (size_t)TRAIN_WG * sizeof(TrainSample);   // the workgroup's batch slots
+constexpr size_t TRAIN_LDS = FWD_BYTES + BWD_BYTES + STAGE_BYTES + SAMPLE_BYTES;
 static_assert(STAGE_BYTES % 16 == 0 && TRAIN_LDS <= 160 * 1024, "LDS layout");
 
 // Tiles of one layer's dW held by a wave: tiles wave, wave + 8, ... of NOR x NIT.
@@ -519,30 +528,50 @@ __device__ __forceinline__ void dw_zero(DwTiles<NOR, NIT>& T) {
 }
 
 // One training step's forward, loss gradient, backward and weight-gradient partials for TRAIN_WG
-// samples per workgroup.
-// update: a parameter update follows (k_adam); workgroup 0 then advances the Adam step counter
-// when the batch has a positive weight sum, before k_adam reads it.
-__global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32_t B, TrainBuffers tb, int update) {
+// samples per workgroup; the workgroup's sum of c goes to tb.csum[1 + workgroup], and workgroup 0
+// copies the step count to tb.step[1] for k_adam.
+__global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, TrainBuffers tb) {
     constexpr int NT = 1;
     extern __shared__ u32x4 lds[];
     u32x4* sF = lds;
     u32x4* sB = lds + FWD_BYTES / 16;
     uint16_t* Xs = reinterpret_cast<uint16_t*>(lds + (FWD_BYTES + BWD_BYTES) / 16);
     uint16_t* Ys = Xs + NENC * STG;
-    for (int k = threadIdx.x; k < NF_FWD * 64; k += 512) sF[k] = reinterpret_cast<const u32x4*>(net.frag_fwd)[k];
-    for (int k = threadIdx.x; k < NF_BWD * 64; k += 512) sB[k] = reinterpret_cast<const u32x4*>(net.frag_bwd)[k];
+    TrainSample* sS = reinterpret_cast<TrainSample*>(lds + (FWD_BYTES + BWD_BYTES + STAGE_BYTES) / 16);
+    {
+        // the batch slots and both fragment sets (adjacent in LDS), every load issued before the
+        // first store so that their latencies overlap
+        constexpr int NFRAG = (NF_FWD + NF_BWD) * 64, NI = (NFRAG + 511) / 512;
+        TrainSample ts{};
+        if (threadIdx.x < TRAIN_WG) ts = fetch_sample(src, blockIdx.x * TRAIN_WG + threadIdx.x);
+        u32x4 fr[NI];
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int k = (int)threadIdx.x + 512 * i;
+            fr[i] = u32x4{0u, 0u, 0u, 0u};
+            if (k < NF_FWD * 64) fr[i] = reinterpret_cast<const u32x4*>(net.frag_fwd)[k];
+            else if (k < NFRAG) fr[i] = reinterpret_cast<const u32x4*>(net.frag_bwd)[k - NF_FWD * 64];
+        }
+        if (threadIdx.x < TRAIN_WG) sS[threadIdx.x] = ts;
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const int k = (int)threadIdx.x + 512 * i;
+            if (k < NFRAG) lds[k] = fr[i];
+        }
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
     const int col = wave * 16 + c;                          // this lane's staged column
     const float* P = net.params;
-    // every wavefront adds the split sums itself (no separate launch); workgroup 0 keeps the
-    // total for k_adam and advances the step counter
-    const float csum = batch_csum(tb.csum + 1, B / KSPLIT, lane);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        tb.csum[0] = csum;
-        if (update && csum > 0.0f) tb.step[0] += 1;
+    if (wave == 0) {
+        float a = 0.0f;
+#pragma unroll
+        for (int i = 0; i < TRAIN_WG / 64; i++) a += sS[lane + 64 * i].c;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+        if (lane == 0) tb.csum[1 + blockIdx.x] = a;
+        if (blockIdx.x == 0 && lane == 0) tb.step[1] = tb.step[0];   // the count k_adam reads
     }
-    const float cscale = csum > 0.0f ? (float)B / csum : 0.0f;
 
     DwTiles<4, 9> T1;
     DwTiles<4, 4> T2, T3, T4;
@@ -556,7 +585,7 @@ __global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32
         const uint64_t s0 = (uint64_t)blockIdx.x * TRAIN_WG + (uint64_t)chunk * TRAIN_CHUNK + wave * 16 + c;
         float x[NT][6], dir[3], cw;
         {
-            const TrainSample ts = fetch_sample(src, (uint32_t)s0);
+            const TrainSample ts = sS[chunk * TRAIN_CHUNK + wave * 16 + c];
             normalise(make_float4(ts.px, ts.py, ts.pz, 0.0f), make_float4(ts.nx, ts.ny, ts.nz, 0.0f), net.bounds, x[0]);
             dir[0] = ts.dx; dir[1] = ts.dy; dir[2] = ts.dz;
             cw = ts.c;
@@ -583,32 +612,34 @@ __global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[4 * T + r] = o5[T][0][r];
             LobeD L[2];
-            decode_lobe(v, L[0]);
-            decode_lobe(v + 5, L[1]);
+            decode_lobe_fast(v, L[0]);
+            decode_lobe_fast(v + 5, L[1]);
             const float mlog = quad_max(fmaxf(L[0].logit, L[1].logit));
-            const float e0 = expf(L[0].logit - mlog), e1 = expf(L[1].logit - mlog);
+            const float e0 = __expf(L[0].logit - mlog), e1 = __expf(L[1].logit - mlog);
             const float z = quad_sum(e0 + e1);
-            const float w[2] = {e0 / z, e1 / z};
+            const float iz = __builtin_amdgcn_rcpf(z);
+            const float w[2] = {e0 * iz, e1 * iz};
             float a[2], cosv[2];
             for (int k = 0; k < 2; k++) {
                 cosv[k] = L[k].mu[0] * dir[0] + L[k].mu[1] * dir[1] + L[k].mu[2] * dir[2];
-                a[k] = logf(w[k]) + log_vmf_norm(L[k].kappa) + L[k].kappa * (cosv[k] - 1.0f);
+                a[k] = __logf(w[k]) + log_vmf_norm_fast(L[k].kappa) + L[k].kappa * (cosv[k] - 1.0f);
             }
             const float am = quad_max(fmaxf(a[0], a[1]));
-            const float ex0 = expf(a[0] - am), ex1 = expf(a[1] - am);
+            const float ex0 = __expf(a[0] - am), ex1 = __expf(a[1] - am);
             const float se = quad_sum(ex0 + ex1);
-            const float logq = am + logf(se);
-            const float rr[2] = {ex0 / se, ex1 / se};
-            const float sc = cw * cscale;
+            const float logq = am + __logf(se);
+            const float ise = __builtin_amdgcn_rcpf(se);
+            const float rr[2] = {ex0 * ise, ex1 * ise};
+            const float sc = cw;
             float gr[12];
             for (int k = 0; k < 2; k++) {
                 const float kap = L[k].kappa;
                 gr[5 * k + 4] = sc * (w[k] - rr[k]);
-                const float dl = dlog_norm(kap) + (cosv[k] - 1.0f);
+                const float dl = dlog_norm_fast(kap) + (cosv[k] - 1.0f);
                 gr[5 * k + 3] = L[k].kin ? -sc * rr[k] * dl * kap : 0.0f;
+                const float iul = L[k].ulen > 0.0f ? __builtin_amdgcn_rcpf(L[k].ulen) : 0.0f;
                 for (int jj = 0; jj < 3; jj++)
-                    gr[5 * k + jj] =
-                        L[k].ulen > 0.0f ? -sc * rr[k] * kap * (dir[jj] - cosv[k] * L[k].mu[jj]) / L[k].ulen : 0.0f;
+                    gr[5 * k + jj] = -sc * rr[k] * kap * (dir[jj] - cosv[k] * L[k].mu[jj]) * iul;
             }
             gr[10] = 0.0f; gr[11] = 0.0f;
             if (g == 0) tb.loss[s0] = -sc * logq;
@@ -672,57 +703,84 @@ __global__ __launch_bounds__(512) void k_train(Net net, SampleSource src, uint32
     dw_write(T5, bs5, Pp, OW5, OB5, NH, NOUT);
 }
 
-// k_adam: ADAM_PARAMS parameters per workgroup, each summed by ADAM_GROUPS threads over contiguous
-// runs of partials; the run sums are then added in run order, so the reduction order is fixed.
-constexpr int ADAM_PARAMS = 32, ADAM_GROUPS = 8;
+// k_adam: 64 parameters per workgroup as ADAM_COLS float4 columns.  Each of ADAM_RUNS groups of
+// ADAM_COLS threads adds a contiguous run of the workgroup partials (16-byte loads, a wavefront
+// reading four 256-byte rows), and the run sums are then added in run order, so the reduction order
+// is fixed.
+constexpr int ADAM_COLS = 16, ADAM_RUNS = 16, ADAM_THREADS = ADAM_COLS * ADAM_RUNS;
+constexpr int ADAM_PARAMS = 4 * ADAM_COLS;
+static_assert(NPARAM % 4 == 0, "partial rows are whole float4 columns");
 
-// Adds the weight-gradient partials (workgroup order within each run, then run order), applies Adam
-// and writes the updated parameter's bf16 copy into its forward and backward fragment slots (so no
-// separate repacking pass).
-__global__ __launch_bounds__(ADAM_PARAMS * ADAM_GROUPS) void k_adam(
+// Adds the weight-gradient partials and the workgroup c sums (nwg of each), divides by the batch's
+// c sum, applies Adam and writes the updated parameter's bf16 copy into its forward and backward
+// fragment slots (so no separate repacking pass).  Every workgroup reads the step count from
+// step[1], which k_train copied from step[0] before this launch, so workgroup 0 can advance step[0]
+// without a race (and without device-wide atomics or fences); it also stores the c sum in csum[0].
+__global__ __launch_bounds__(ADAM_THREADS) void k_adam(
         float* __restrict__ P, float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
-        const float* __restrict__ partial, uint32_t nsplit, float invB, float lr, const float* __restrict__ csum,
-        const int32_t* __restrict__ step, int update, const int32_t* __restrict__ slot_fwd,
+        const float* __restrict__ partial, uint32_t nwg, float lr, float* __restrict__ csum,
+        int32_t* __restrict__ step, int update, const int32_t* __restrict__ slot_fwd,
         const int32_t* __restrict__ slot_bwd, uint16_t* __restrict__ ffwd, uint16_t* __restrict__ fbwd) {
-    __shared__ float run_sum[ADAM_GROUPS][ADAM_PARAMS];
-    const int q = threadIdx.x % ADAM_PARAMS, grp = threadIdx.x / ADAM_PARAMS;
-    const int p = blockIdx.x * ADAM_PARAMS + q;
-    const uint32_t per = (nsplit + ADAM_GROUPS - 1) / ADAM_GROUPS;
-    const uint32_t s0 = min(nsplit, (uint32_t)grp * per), s1 = min(nsplit, s0 + per);
-    float gs = 0.0f;
-    if (p < NPARAM) {
+    __shared__ float4 run_sum[ADAM_RUNS][ADAM_COLS];
+    __shared__ float s_csum;
+    const int colw = threadIdx.x % ADAM_COLS, run = threadIdx.x / ADAM_COLS;
+    const int q4 = blockIdx.x * ADAM_COLS + colw;
+    const uint32_t per = (nwg + ADAM_RUNS - 1) / ADAM_RUNS;
+    const uint32_t s0 = min(nwg, (uint32_t)run * per), s1 = min(nwg, s0 + per);
+    float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (q4 < NPARAM / 4) {
+        const float4* p4 = reinterpret_cast<const float4*>(partial) + q4;
         uint32_t s = s0;
-        for (; s + 8 <= s1; s += 8) {   // 8 loads in flight, then the adds in order
-            float t[8];
+        for (; s + 16 <= s1; s += 16) {   // 16 loads in flight, then the adds in order
+            float4 t[16];
 #pragma unroll
-            for (int j = 0; j < 8; j++) t[j] = partial[(uint64_t)(s + j) * NPARAM + p];
+            for (int j = 0; j < 16; j++) t[j] = p4[(uint64_t)(s + j) * (NPARAM / 4)];
 #pragma unroll
-            for (int j = 0; j < 8; j++) gs += t[j];
+            for (int j = 0; j < 16; j++) { gs.x += t[j].x; gs.y += t[j].y; gs.z += t[j].z; gs.w += t[j].w; }
         }
-        for (; s < s1; s++) gs += partial[(uint64_t)s * NPARAM + p];
+        for (; s < s1; s++) {
+            const float4 t = p4[(uint64_t)s * (NPARAM / 4)];
+            gs.x += t.x; gs.y += t.y; gs.z += t.z; gs.w += t.w;
+        }
     }
-    run_sum[grp][q] = gs;
+    run_sum[run][colw] = gs;
+    const int t = step[1] + 1;
+    if (threadIdx.x < 64) {
+        const float a = wave_sum_fixed(csum + 1, nwg, (int)threadIdx.x);
+        if (threadIdx.x == 0) s_csum = a;
+    }
     __syncthreads();
-    if (grp != 0 || p >= NPARAM) return;
-    gs = run_sum[0][q];
+    const float cs = s_csum;
+    const bool upd = update && cs > 0.0f;
+    if (threadIdx.x < ADAM_PARAMS) {
+        const int p = blockIdx.x * ADAM_PARAMS + threadIdx.x;
+        const float* rs = reinterpret_cast<const float*>(&run_sum[0][0]) + threadIdx.x;
+        float gsum = rs[0];
 #pragma unroll
-    for (int g = 1; g < ADAM_GROUPS; g++) gs += run_sum[g][q];
-    const float gr = gs * invB;
-    grad[p] = gr;
-    if (!update || !(csum[0] > 0.0f)) return;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const int t = step[0];   // advanced by k_train for this update
-    const float mm = b1 * m[p] + (1.0f - b1) * gr;
-    const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
-    m[p] = mm;
-    v[p] = vv;
-    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
-    const float np = P[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
-    P[p] = np;
-    const uint16_t h = bf16_bits((__bf16)np);
-    const int sf = slot_fwd[p], sb = slot_bwd[p];
-    if (sf >= 0) ffwd[sf] = h;
-    if (sb >= 0) fbwd[sb] = h;
+        for (int r = 1; r < ADAM_RUNS; r++) gsum += rs[r * ADAM_PARAMS];
+        if (p < NPARAM) {
+            const float gr = cs > 0.0f ? gsum / cs : 0.0f;
+            grad[p] = gr;
+            if (upd) {
+                const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+                const float mm = b1 * m[p] + (1.0f - b1) * gr;
+                const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
+                m[p] = mm;
+                v[p] = vv;
+                const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+                const float np = P[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+                P[p] = np;
+                const uint16_t h = bf16_bits((__bf16)np);
+                const int sf = slot_fwd[p], sb = slot_bwd[p];
+                if (sf >= 0) ffwd[sf] = h;
+                if (sb >= 0) fbwd[sb] = h;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (upd) step[0] = t;
+        csum[0] = cs;
+    }
 }
 
 
@@ -771,16 +829,12 @@ hipError_t launch_train(const Net& net, float* params, float* m, float* v, const
         if (e != hipSuccess) return e;
         attr_set[dev].store(true, std::memory_order_release);
     }
-    const uint32_t nsplit = B / KSPLIT;
     if (!src.S) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_csum_part, dim3(nsplit), dim3(256), 0, st, src, tb.csum + 1);
     const uint32_t nwg = B / TRAIN_WG;
-    hipLaunchKernelGGL(k_train, dim3(nwg), dim3(512), TRAIN_LDS, st, net, src, B, tb, update ? 1 : 0);
-    hipLaunchKernelGGL(k_adam, dim3((NPARAM + ADAM_PARAMS - 1) / ADAM_PARAMS), dim3(ADAM_PARAMS * ADAM_GROUPS), 0, st,
-                       params, m, v,
-                       tb.grad, tb.partial, nwg, 1.0f / (float)B, lr, tb.csum, tb.step, update ? 1 : 0,
-                       (const int32_t*)tb.slot_fwd, (const int32_t*)tb.slot_bwd, (uint16_t*)net.frag_fwd,
-                       (uint16_t*)net.frag_bwd);
+    hipLaunchKernelGGL(k_train, dim3(nwg), dim3(512), TRAIN_LDS, st, net, src, tb);
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + ADAM_PARAMS - 1) / ADAM_PARAMS), dim3(ADAM_THREADS), 0, st, params, m, v,
+                       tb.grad, tb.partial, nwg, lr, tb.csum, tb.step, update ? 1 : 0, (const int32_t*)tb.slot_fwd,
+                       (const int32_t*)tb.slot_bwd, (uint16_t*)net.frag_fwd, (uint16_t*)net.frag_bwd);
     return hipGetLastError();
 }
 

@@ -45,9 +45,9 @@ struct Net {
 
 struct TrainBuffers {
     float* partial;     // [batch / TRAIN_WG][NPARAM]: each training workgroup's gradient sums
-    float* loss;        // per sample
-    float* csum;        // [0]: sum of c over the batch; [1 .. 1 + max_batch/KSPLIT): per-split partial sums
-    int32_t* step;      // Adam step count (device)
+    float* loss;        // per sample: c_i (-log q_i)
+    float* csum;        // [0]: sum of c over the last batch; [1 .. 1 + max_batch/TRAIN_WG): per-workgroup sums
+    int32_t* step;      // [0]: Adam step count; [1]: its copy made by k_train, which k_adam reads
     float* grad;        // last mean gradient (NPARAM)
     int32_t* slot_fwd;  // NPARAM: bf16 element of the forward fragments holding each parameter, -1: none
     int32_t* slot_bwd;  // NPARAM: the same for the backward fragments
