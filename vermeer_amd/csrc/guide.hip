@@ -727,6 +727,18 @@ __global__ __launch_bounds__(ADAM_THREADS) void k_adam(
     const int q4 = blockIdx.x * ADAM_COLS + colw;
     const uint32_t per = (nwg + ADAM_RUNS - 1) / ADAM_RUNS;
     const uint32_t s0 = min(nwg, (uint32_t)run * per), s1 = min(nwg, s0 + per);
+    // the Adam operands, the step count and the c sum are loaded first, so that their latency
+    // overlaps the partial reads
+    const int p = blockIdx.x * ADAM_PARAMS + (int)threadIdx.x;
+    const bool owner = threadIdx.x < ADAM_PARAMS && p < NPARAM;
+    float m0 = 0.0f, v0 = 0.0f, p0 = 0.0f;
+    int sf = -1, sb = -1;
+    if (owner) { m0 = m[p]; v0 = v[p]; p0 = P[p]; sf = slot_fwd[p]; sb = slot_bwd[p]; }
+    const int t = step[1] + 1;
+    if (threadIdx.x < 64) {
+        const float a = wave_sum_fixed(csum + 1, nwg, (int)threadIdx.x);
+        if (threadIdx.x == 0) s_csum = a;
+    }
     float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (q4 < NPARAM / 4) {
         const float4* p4 = reinterpret_cast<const float4*>(partial) + q4;
@@ -744,37 +756,28 @@ __global__ __launch_bounds__(ADAM_THREADS) void k_adam(
         }
     }
     run_sum[run][colw] = gs;
-    const int t = step[1] + 1;
-    if (threadIdx.x < 64) {
-        const float a = wave_sum_fixed(csum + 1, nwg, (int)threadIdx.x);
-        if (threadIdx.x == 0) s_csum = a;
-    }
     __syncthreads();
     const float cs = s_csum;
     const bool upd = update && cs > 0.0f;
-    if (threadIdx.x < ADAM_PARAMS) {
-        const int p = blockIdx.x * ADAM_PARAMS + threadIdx.x;
+    if (owner) {
         const float* rs = reinterpret_cast<const float*>(&run_sum[0][0]) + threadIdx.x;
         float gsum = rs[0];
 #pragma unroll
         for (int r = 1; r < ADAM_RUNS; r++) gsum += rs[r * ADAM_PARAMS];
-        if (p < NPARAM) {
-            const float gr = cs > 0.0f ? gsum / cs : 0.0f;
-            grad[p] = gr;
-            if (upd) {
-                const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-                const float mm = b1 * m[p] + (1.0f - b1) * gr;
-                const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
-                m[p] = mm;
-                v[p] = vv;
-                const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
-                const float np = P[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
-                P[p] = np;
-                const uint16_t h = bf16_bits((__bf16)np);
-                const int sf = slot_fwd[p], sb = slot_bwd[p];
-                if (sf >= 0) ffwd[sf] = h;
-                if (sb >= 0) fbwd[sb] = h;
-            }
+        const float gr = cs > 0.0f ? gsum / cs : 0.0f;
+        grad[p] = gr;
+        if (upd) {
+            const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+            const float mm = b1 * m0 + (1.0f - b1) * gr;
+            const float vv = b2 * v0 + (1.0f - b2) * gr * gr;
+            m[p] = mm;
+            v[p] = vv;
+            const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+            const float np = p0 - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+            P[p] = np;
+            const uint16_t h = bf16_bits((__bf16)np);
+            if (sf >= 0) ffwd[sf] = h;
+            if (sb >= 0) fbwd[sb] = h;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
