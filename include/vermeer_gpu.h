@@ -180,7 +180,9 @@ int vgpu_qbvh_build(float* boxes, float* centroids, int32_t* idx, int32_t n, int
  * fail with VGPU_ERR_BUILD as on the host; so does a NaN box coordinate,
  * which vgpu_qbvh_build accepts.  A tree more than 192 levels deep below
  * its 2048-element subtrees fails with VGPU_ERR_STACK (vgpu_qbvh_build has no
- * such limit). */
+ * such limit).  Transfers of 4 MiB or more go through 128 MiB of pinned host
+ * memory per device, allocated by the first such call and kept until the
+ * process ends (calls on one device take turns on it). */
 int vgpu_qbvh_build_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leaf_max,
                         vgpu_qbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out,
                         float* bounds_out, void* stream);

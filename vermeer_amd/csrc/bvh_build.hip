@@ -34,6 +34,8 @@
 #include <algorithm>
 #include <cstring>
 #include <limits>
+#include <mutex>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -706,6 +708,159 @@ __global__ void k_place(const vgpu_qbvh_node* scratch, const int* src, const int
 
 // ---------------------------------------------------------------- host side
 
+// ---------------------------------------------------------------- pinned staging of the host arrays
+//
+// A pageable hipMemcpy moves the builder's host arrays (40 bytes per element each way) at about
+// 21 GB/s to the device and 14 GB/s back.  Here they go through kStageSlots pinned slots of
+// kStageSlot bytes: the host side of each chunk is copied by kCopyThreads threads while the DMA of
+// the neighbouring chunk runs.  The slots (128 MiB of pinned host memory per device) are allocated
+// by the first build of at least kStageMin bytes on that device and kept until the process ends;
+// builds that run at once take turns on them.  Smaller transfers use plain hipMemcpyAsync.
+constexpr size_t kStageSlot = (size_t)32 << 20, kStageMin = (size_t)4 << 20;
+constexpr int kStageSlots = 4, kCopyThreads = 8, kStageDevices = 64;
+
+struct HostSeg {
+    char* host;
+    char* dev;
+    size_t bytes;
+};
+
+struct PinnedStage {
+    std::mutex mu;
+    char* buf = nullptr;
+    hipEvent_t ev[kStageSlots] = {};
+    bool failed = false;   // the pinned allocation failed once: plain copies from then on
+
+    bool ready() {   // under mu, with the stage's device current
+        if (buf) return true;
+        if (failed) return false;
+        if (hipHostMalloc((void**)&buf, kStageSlot * kStageSlots, hipHostMallocDefault) != hipSuccess) {
+            buf = nullptr;
+            failed = true;
+            return false;
+        }
+        for (int i = 0; i < kStageSlots; i++)
+            if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) {
+                failed = true;
+                return false;   // buf stays allocated and unused
+            }
+        return true;
+    }
+};
+
+PinnedStage* stage_for_current_device() {
+    static PinnedStage stages[kStageDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kStageDevices) return nullptr;
+    return &stages[dev];
+}
+
+// memcpy split over up to kCopyThreads threads (64-byte aligned pieces).
+void par_copy(char* dst, const char* src, size_t bytes) {
+    const int T = bytes >= ((size_t)1 << 22) ? kCopyThreads : 1;
+    const size_t per = ((bytes + T - 1) / T + 63) & ~(size_t)63;
+    std::thread th[kCopyThreads];
+    for (int t = 1; t < T; t++) {
+        const size_t a = std::min(bytes, per * t), b = std::min(bytes, a + per);
+        if (b > a) th[t] = std::thread([=]() { std::memcpy(dst + a, src + a, b - a); });
+    }
+    std::memcpy(dst, src, std::min(bytes, per));
+    for (int t = 1; t < T; t++)
+        if (th[t].joinable()) th[t].join();
+}
+
+std::vector<HostSeg> chunks_of(const HostSeg* segs, int nseg, size_t& total) {
+    std::vector<HostSeg> c;
+    total = 0;
+    for (int i = 0; i < nseg; i++) {
+        total += segs[i].bytes;
+        for (size_t off = 0; off < segs[i].bytes; off += kStageSlot)
+            c.push_back(HostSeg{segs[i].host + off, segs[i].dev + off, std::min(kStageSlot, segs[i].bytes - off)});
+    }
+    return c;
+}
+
+// Host -> device; the copies are complete on the host side when it returns, and ordered before later
+// work on st.
+hipError_t stage_h2d(const HostSeg* segs, int nseg, hipStream_t st) {
+    size_t total = 0;
+    const std::vector<HostSeg> c = chunks_of(segs, nseg, total);
+    PinnedStage* S = total >= kStageMin ? stage_for_current_device() : nullptr;
+    std::unique_lock<std::mutex> lk;
+    if (S) {
+        lk = std::unique_lock<std::mutex>(S->mu);
+        if (!S->ready()) S = nullptr;
+    }
+    if (!S) {
+        for (int i = 0; i < nseg; i++) {
+            const hipError_t e = hipMemcpyAsync(segs[i].dev, segs[i].host, segs[i].bytes, hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    for (size_t k = 0; k < c.size(); k++) {
+        const int slot = (int)(k % kStageSlots);
+        char* sb = S->buf + (size_t)slot * kStageSlot;
+        // the slot's previous DMA (this call's, or an earlier call's that returned early) must be done
+        // before it is refilled; an event never recorded completes at once
+        const hipError_t w = hipEventSynchronize(S->ev[slot]);
+        if (w != hipSuccess) return w;
+        par_copy(sb, c[k].host, c[k].bytes);
+        hipError_t e = hipMemcpyAsync(c[k].dev, sb, c[k].bytes, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipEventRecord(S->ev[slot], st);
+        if (e != hipSuccess) return e;
+    }
+    // the slots are free for the next user only once their DMAs are done
+    for (size_t k = c.size() > (size_t)kStageSlots ? c.size() - kStageSlots : 0; k < c.size(); k++) {
+        const hipError_t e = hipEventSynchronize(S->ev[k % kStageSlots]);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Device -> host after the work already on st; the host arrays are complete when it returns.
+hipError_t stage_d2h(const HostSeg* segs, int nseg, hipStream_t st) {
+    size_t total = 0;
+    const std::vector<HostSeg> c = chunks_of(segs, nseg, total);
+    PinnedStage* S = total >= kStageMin ? stage_for_current_device() : nullptr;
+    std::unique_lock<std::mutex> lk;
+    if (S) {
+        lk = std::unique_lock<std::mutex>(S->mu);
+        if (!S->ready()) S = nullptr;
+    }
+    if (!S) {
+        for (int i = 0; i < nseg; i++) {
+            const hipError_t e = hipMemcpyAsync(segs[i].host, segs[i].dev, segs[i].bytes, hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipStreamSynchronize(st);
+    }
+    // chunk k's DMA is issued before chunk k - 1 is copied out on the host; a slot is refilled only
+    // after its previous chunk was copied out (kStageSlots >= 2)
+    auto drain = [&](size_t k) -> hipError_t {
+        const int slot = (int)(k % kStageSlots);
+        const hipError_t e = hipEventSynchronize(S->ev[slot]);
+        if (e != hipSuccess) return e;
+        par_copy(c[k].host, S->buf + (size_t)slot * kStageSlot, c[k].bytes);
+        return hipSuccess;
+    };
+    for (size_t k = 0; k < c.size(); k++) {
+        const int slot = (int)(k % kStageSlots);
+        if (k < (size_t)kStageSlots) {   // a DMA into the slot left by an earlier call that returned early
+            const hipError_t w = hipEventSynchronize(S->ev[slot]);
+            if (w != hipSuccess) return w;
+        }
+        char* sb = S->buf + (size_t)slot * kStageSlot;
+        hipError_t e = hipMemcpyAsync(sb, c[k].dev, c[k].bytes, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(S->ev[slot], st);
+        if (e == hipSuccess && k >= 1) e = drain(k - 1);
+        if (e != hipSuccess) return e;
+    }
+    if (!c.empty()) return drain(c.size() - 1);
+    return hipSuccess;
+}
+
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -817,9 +972,10 @@ int build_qbvh_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int3
     BUILD_TRY(d_scratch.alloc(N));
     BUILD_TRY(hipMemsetAsync(d_err.p, 0, sizeof(int), st));
     if (n > 0) {
-        BUILD_TRY(hipMemcpyAsync(d_boxes.p, boxes, sizeof(float) * 6 * n, hipMemcpyHostToDevice, st));
-        BUILD_TRY(hipMemcpyAsync(d_cent.p, centroids, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
-        BUILD_TRY(hipMemcpyAsync(d_idx.p, idx, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+        const HostSeg in[3] = {{(char*)boxes, (char*)d_boxes.p, sizeof(float) * 6 * n},
+                               {(char*)centroids, (char*)d_cent.p, sizeof(float) * 3 * n},
+                               {(char*)idx, (char*)d_idx.p, sizeof(int32_t) * n}};
+        BUILD_TRY(stage_h2d(in, 3, st));
         hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, st, d_boxes.p, d_cent.p, d_idx.p, n, d_E.p,
                            d_err.p);
         BUILD_TRY(hipGetLastError());
@@ -966,14 +1122,15 @@ int build_qbvh_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int3
                            d_nodes.p);
         BUILD_TRY(hipGetLastError());
     }
-    BUILD_TRY(hipMemcpyAsync(nodes_out, d_nodes.p, (size_t)total * sizeof(vgpu_qbvh_node), hipMemcpyDeviceToHost, st));
     if (n > 0) {
         hipLaunchKernelGGL(k_unpack, dim3((n + 255) / 256), dim3(256), 0, st, d_E.p, n, d_boxes.p, d_cent.p, d_idx.p);
         BUILD_TRY(hipGetLastError());
-        BUILD_TRY(hipMemcpyAsync(boxes, d_boxes.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, st));
-        BUILD_TRY(hipMemcpyAsync(centroids, d_cent.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
-        BUILD_TRY(hipMemcpyAsync(idx, d_idx.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     }
+    const HostSeg out[4] = {{(char*)nodes_out, (char*)d_nodes.p, (size_t)total * sizeof(vgpu_qbvh_node)},
+                            {(char*)boxes, (char*)d_boxes.p, n > 0 ? sizeof(float) * 6 * n : 0},
+                            {(char*)centroids, (char*)d_cent.p, n > 0 ? sizeof(float) * 3 * n : 0},
+                            {(char*)idx, (char*)d_idx.p, n > 0 ? sizeof(int32_t) * n : 0}};
+    BUILD_TRY(stage_d2h(out, 4, st));
     BUILD_TRY(hipStreamSynchronize(st));
     // the top nodes, children before parents (reverse creation order), with MINSS / MAXSS unions
     const float kInf = std::numeric_limits<float>::infinity();
