@@ -218,3 +218,39 @@ def test_gpu_builder_full_icosphere_field():
     print(f"host {1e3 * (t1 - t0):.0f} ms, gpu {1e3 * (t2 - t1):.0f} ms (first call)", flush=True)
     assert n1.tobytes() == n2.tobytes() and np.array_equal(i1, i2)
     assert b1.tobytes() == b2.tobytes() and c1.tobytes() == c2.tobytes() and bd1.tobytes() == bd2.tobytes()
+
+
+@pytest.mark.gpu
+def test_gpu_builder_concurrent_calls_share_the_pinned_staging():
+    """Builds large enough to go through the pinned staging slots (several 32 MiB chunks each way),
+    run from two threads at once (ctypes releases the GIL): each equals the host builder's."""
+    import threading
+    cases = [_random_boxes(900_000, 21 + k, spread=100.0) for k in range(2)]
+    want = []
+    for boxes, cent, idx in cases:
+        b, c, i = boxes.copy(), cent.copy(), idx.copy()
+        nodes, bd = va.build_accel(b, c, i, 4)
+        want.append((nodes.tobytes(), b.tobytes(), c.tobytes(), i.copy(), bd.tobytes()))
+    got = [None, None]
+    errors = []
+
+    def work(k):
+        try:
+            boxes, cent, idx = cases[k]
+            for _ in range(2):
+                b, c, i = boxes.copy(), cent.copy(), idx.copy()
+                nodes, bd = va.build_accel_gpu(b, c, i, 4)
+                got[k] = (nodes.tobytes(), b.tobytes(), c.tobytes(), i, bd.tobytes())
+        except Exception as e:   # reported by the main thread
+            errors.append(e)
+
+    print("\n[case] two threads, 900000 boxes each, leafMax=4", flush=True)
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for k in range(2):
+        assert got[k][0] == want[k][0] and got[k][1] == want[k][1] and got[k][2] == want[k][2]
+        assert np.array_equal(got[k][3], want[k][3]) and got[k][4] == want[k][4]
