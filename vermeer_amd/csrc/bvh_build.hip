@@ -793,6 +793,7 @@ hipError_t stage_h2d(const HostSeg* segs, int nseg, hipStream_t st) {
     }
     if (!S) {
         for (int i = 0; i < nseg; i++) {
+            if (segs[i].bytes == 0) continue;
             const hipError_t e = hipMemcpyAsync(segs[i].dev, segs[i].host, segs[i].bytes, hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return e;
         }
@@ -830,6 +831,7 @@ hipError_t stage_d2h(const HostSeg* segs, int nseg, hipStream_t st) {
     }
     if (!S) {
         for (int i = 0; i < nseg; i++) {
+            if (segs[i].bytes == 0) continue;
             const hipError_t e = hipMemcpyAsync(segs[i].host, segs[i].dev, segs[i].bytes, hipMemcpyDeviceToHost, st);
             if (e != hipSuccess) return e;
         }
