@@ -11,8 +11,36 @@
 // builder must round exactly as the reference's.
 
 #include <new>
+#include <thread>
 
 #include "scene_internal.h"
+
+namespace {
+
+// Runs f(begin, end) over [0, n), split over up to 8 threads when n is large (the flatten's
+// per-corner gathers of millions of triangles).  A thread that cannot be started runs its range on
+// the calling thread.
+template <class F>
+void parallel_ranges(size_t n, const F& f) {
+    constexpr int kMax = 8;
+    const int T = n >= ((size_t)1 << 18) ? kMax : 1;
+    const size_t per = (n + T - 1) / T;
+    std::thread th[kMax];
+    for (int t = 1; t < T; t++) {
+        const size_t a = std::min(n, per * t), b = std::min(n, a + per);
+        if (a >= b) continue;
+        try {
+            th[t] = std::thread([&f, a, b]() { f(a, b); });
+        } catch (...) {
+            f(a, b);
+        }
+    }
+    f(0, std::min(n, per));
+    for (int t = 1; t < T; t++)
+        if (th[t].joinable()) th[t].join();
+}
+
+}  // namespace
 
 namespace vgpu {
 
@@ -762,6 +790,14 @@ int vgpu_scene_commit(vgpu_scene* s) {
     std::vector<vgpu::DevNode> dnodes;
     std::vector<float4> dtris, dmnodes, dmtris, dspheres;
     std::vector<vgpu::DevGeom> dgeoms(nall);
+    {
+        size_t corners = 0;
+        for (int slot = 0; slot < nall; slot++) {
+            const MeshHost& m = s->meshes[s->order[slot]];
+            if (m.kind < vgpu::kXfInstance && !m.sphere && m.nkeys == 1) corners += (size_t)m.ntris * 3;
+        }
+        dtris.reserve(corners);
+    }
     if (!s->top_motion) {
         for (auto& nd : s->top_nodes) { vgpu::DevNode d; to_dev(nd, d); dnodes.push_back(d); }
     } else {
@@ -786,11 +822,15 @@ int vgpu_scene_commit(vgpu_scene* s) {
             g.node_base = (int32_t)dnodes.size();
             for (auto& nd : m.nodes) { vgpu::DevNode d; to_dev(nd, d); dnodes.push_back(d); }
             g.tri_base = (int32_t)(dtris.size() / 3);
-            for (uint32_t i = 0; i < m.ntris; i++)
-                for (int j = 0; j < 3; j++) {
-                    const float* v = m.vert(0, m.idxp_sorted[(size_t)i * 3 + j]);
-                    dtris.push_back(make_float4(v[0], v[1], v[2], 0.0f));
+            const size_t base = dtris.size();
+            dtris.resize(base + (size_t)m.ntris * 3);
+            float4* out = dtris.data() + base;
+            parallel_ranges((size_t)m.ntris * 3, [&](size_t a, size_t b) {
+                for (size_t k = a; k < b; k++) {
+                    const float* v = m.vert(0, m.idxp_sorted[k]);
+                    out[k] = make_float4(v[0], v[1], v[2], 0.0f);
                 }
+            });
         } else {
             g.node_base = (int32_t)dmnodes.size();
             append_motion_nodes(dmnodes, m.mnodes, m.mboxes, (int)m.nkeys);
@@ -830,11 +870,15 @@ int vgpu_scene_commit(vgpu_scene* s) {
             gsd.sidx_base = -1;
             if (!m.normals.empty() && m.nkeys == 1) {
                 gsd.nrm_base = (int32_t)dn.size();
-                for (uint32_t i = 0; i < m.ntris; i++)
-                    for (int j = 0; j < 3; j++) {
-                        const float* nv = &m.normals[(size_t)m.idxp_sorted[(size_t)i * 3 + j] * 3];
-                        dn.push_back(make_float4(nv[0], nv[1], nv[2], 0.0f));
+                const size_t base = dn.size();
+                dn.resize(base + (size_t)m.ntris * 3);
+                float4* out = dn.data() + base;
+                parallel_ranges((size_t)m.ntris * 3, [&](size_t a, size_t b) {
+                    for (size_t k = a; k < b; k++) {
+                        const float* nv = &m.normals[(size_t)m.idxp_sorted[k] * 3];
+                        out[k] = make_float4(nv[0], nv[1], nv[2], 0.0f);
                     }
+                });
             }
             if (m.nkeys > 1 && !m.sphere) {
                 // Moving mesh: leaf slot i shades face f = accel.idx[i], or f = i with the reference's
