@@ -98,23 +98,27 @@ int build_static_mesh(MeshHost& m, int leaf_max, int device) {
     const uint32_t n = m.ntris;
     std::vector<float> boxes((size_t)n * 6), cent((size_t)n * 3);
     std::vector<int32_t> ids(n);
-    for (uint32_t i = 0; i < n; i++) {
-        const float* V0 = m.vert(0, m.idx[i * 3 + 0]);
-        const float* V1 = m.vert(0, m.idx[i * 3 + 1]);
-        const float* V2 = m.vert(0, m.idx[i * 3 + 2]);
-        Box6 b; b.reset(); b.grow_point(V0); b.grow_point(V1); b.grow_point(V2);
-        std::memcpy(&boxes[(size_t)i * 6], b.v, sizeof(b.v));
-        for (int k = 0; k < 3; k++) cent[(size_t)i * 3 + k] = (V0[k] + V1[k] + V2[k]) / 3.0f;
-        ids[i] = (int32_t)i;
-    }
+    parallel_ranges(n, [&](size_t a, size_t e) {
+        for (size_t i = a; i < e; i++) {
+            const float* V0 = m.vert(0, m.idx[i * 3 + 0]);
+            const float* V1 = m.vert(0, m.idx[i * 3 + 1]);
+            const float* V2 = m.vert(0, m.idx[i * 3 + 2]);
+            Box6 b; b.reset(); b.grow_point(V0); b.grow_point(V1); b.grow_point(V2);
+            std::memcpy(&boxes[i * 6], b.v, sizeof(b.v));
+            for (int k = 0; k < 3; k++) cent[i * 3 + k] = (V0[k] + V1[k] + V2[k]) / 3.0f;
+            ids[i] = (int32_t)i;
+        }
+    });
     std::vector<vgpu_qbvh_node> nodes;
     const int rc = build_nodes(boxes.data(), cent.data(), ids.data(), n, leaf_max, nodes, m.bounds, device);
     if (rc) return rc;
     m.nodes.swap(nodes);
     m.perm = ids;
     m.idxp_sorted.resize((size_t)n * 3);
-    for (uint32_t i = 0; i < n; i++)
-        for (int c = 0; c < 3; c++) m.idxp_sorted[(size_t)i * 3 + c] = m.idx[(size_t)ids[i] * 3 + c];
+    parallel_ranges(n, [&](size_t a, size_t e) {
+        for (size_t i = a; i < e; i++)
+            for (int c = 0; c < 3; c++) m.idxp_sorted[i * 3 + c] = m.idx[(size_t)ids[i] * 3 + c];
+    });
     if (!m.shaderidx.empty()) {   // shaderidx follows the leaf order (buildqbvh.go:98-104)
         m.shaderidx_sorted.resize(n);
         for (uint32_t i = 0; i < n; i++) m.shaderidx_sorted[i] = m.shaderidx[ids[i]];
