@@ -506,7 +506,7 @@ int vgpu_guide_destroy(vgpu_guide* guide);
 /* Replaces the parameters (24,360 floats) and resets the Adam state. */
 int vgpu_guide_set_params(vgpu_guide* guide, const float* params);
 int vgpu_guide_get_params(vgpu_guide* guide, float* params);
-/* The mean gradient of the last training step (24,360 floats). */
+/* The gradient of the last training step's loss (24,360 floats). */
 int vgpu_guide_get_grad(vgpu_guide* guide, float* grad);
 /* Evaluates n queries (8 floats each: position xyz, pad, normal xyz, pad).
  * out: n x 40 floats, either 8 decoded lobes of (mu xyz, kappa, weight) or,
@@ -515,8 +515,10 @@ int vgpu_guide_get_grad(vgpu_guide* guide, float* grad);
 int vgpu_guide_infer(vgpu_guide* guide, uint32_t flags, const float* queries, uint64_t n, float* out, void* stream);
 /* One training step on n samples (12 floats each: position xyz, c, normal
  * xyz, pad, unit direction xyz, pad); n a multiple of 1024 up to max_batch.
- * Samples with c = 0 contribute no gradient; a batch whose c sum is 0 makes
- * no update.  *loss_out (nullable, host mode only): the batch loss. */
+ * The loss is sum_i c_i (-log q(d_i)) / sum_i c_i.  Samples with c = 0
+ * contribute no gradient; a batch whose c sum is not positive makes no update
+ * and leaves a zero gradient.  *loss_out (nullable, host mode only): the
+ * batch loss. */
 int vgpu_guide_train_step(vgpu_guide* guide, uint32_t flags, const float* samples, uint32_t n, float lr,
                           void* stream, float* loss_out);
 
