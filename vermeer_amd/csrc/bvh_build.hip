@@ -951,6 +951,11 @@ struct SegRunner {
 
 }  // namespace
 
+hipError_t stage_upload(void* dev, const void* host, size_t bytes, hipStream_t st) {
+    const HostSeg seg{(char*)host, (char*)dev, bytes};
+    return stage_h2d(&seg, 1, st);
+}
+
 int build_qbvh_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leafMax,
                    vgpu_qbvh_node* nodes_out, int32_t max_nodes, int32_t* nnodes_out, float* bounds_out,
                    hipStream_t st) {

@@ -20,4 +20,8 @@ int build_qbvh_motion(float* boxes, float* centroids, int32_t* idx, int32_t n, i
 int build_qbvh_gpu(float* boxes, float* centroids, int32_t* idx, int32_t n, int32_t leafMax, vgpu_qbvh_node* nodes_out,
                    int32_t max_nodes, int32_t* nnodes_out, float* bounds_out, hipStream_t st);
 
+// Host -> device copy through the GPU builder's pinned staging slots (a plain hipMemcpyAsync below
+// 4 MiB), on the current device and ordered on `st`; staged copies are complete on return.
+hipError_t stage_upload(void* dev, const void* host, size_t bytes, hipStream_t st);
+
 }  // namespace vgpu

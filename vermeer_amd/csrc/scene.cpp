@@ -239,7 +239,8 @@ template <class T>
 int upload(T** dptr, const std::vector<T>& v) {
     if (v.empty()) return VGPU_OK;
     HIP_TRY(hipMalloc((void**)dptr, v.size() * sizeof(T)));
-    HIP_TRY(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(vgpu::stage_upload(*dptr, v.data(), v.size() * sizeof(T), nullptr));   // pinned slots when large
+    HIP_TRY(hipStreamSynchronize(nullptr));
     return VGPU_OK;
 }
 
